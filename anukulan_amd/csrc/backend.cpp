@@ -1,0 +1,306 @@
+#include "backend.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace anuk {
+
+namespace {
+enum : int { kSlotRedA = 0, kSlotRedB = 1 };
+constexpr int kResultDoubles = 64;
+}  // namespace
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string("hip: ") + what + ": " + hipGetErrorString(e));
+}
+
+HipBackend::HipBackend(int device) : device_(device) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    throw std::runtime_error("hip: no device available");
+  if (device < 0 || device >= count) throw std::runtime_error("hip: device index out of range");
+  hip_check(hipSetDevice(device), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipMalloc(&result_dev_, kResultDoubles * sizeof(double)), "hipMalloc(result)");
+  hip_check(hipMemset(result_dev_, 0, kResultDoubles * sizeof(double)), "hipMemset(result)");
+  hip_check(hipHostMalloc(&result_host_, kResultDoubles * sizeof(double), hipHostMallocDefault),
+            "hipHostMalloc(result)");
+  hip_check(hipEventCreate(&stop_event_), "hipEventCreate");
+}
+
+HipBackend::~HipBackend() {
+  hipSetDevice(device_);
+  if (stream_) hipStreamSynchronize(stream_);
+  for (auto& kv : matrices_) free_matrix(&kv.second);
+  for (auto& kv : scratch_) hipFree(kv.second.first);
+  if (result_dev_) hipFree(result_dev_);
+  if (result_host_) hipHostFree(result_host_);
+  if (stop_event_) hipEventDestroy(stop_event_);
+  if (stream_) hipStreamDestroy(stream_);
+}
+
+void HipBackend::synchronize() { hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+HipBackend::Timed::Timed(HipBackend& b, const char* name) : b_(b), name_(name) {
+  if (!b_.profiling_) return;
+  hip_check(hipEventCreate(&start_), "hipEventCreate");
+  hip_check(hipEventRecord(start_, b_.stream_), "hipEventRecord");
+}
+
+HipBackend::Timed::~Timed() {
+  if (start_ == nullptr) return;
+  hipEventRecord(b_.stop_event_, b_.stream_);
+  hipEventSynchronize(b_.stop_event_);
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, start_, b_.stop_event_);
+  hipEventDestroy(start_);
+  auto& e = b_.profile_[name_];
+  e.ms += static_cast<double>(ms);
+  e.calls += 1;
+}
+
+// ---- memory ----------------------------------------------------------------
+
+double* HipBackend::allocate(Int n) {
+  if (n < 0) throw std::invalid_argument("allocate: negative size");
+  if (n == 0) return nullptr;
+  double* p = nullptr;
+  hip_check(hipMalloc(&p, static_cast<size_t>(n) * sizeof(double)), "hipMalloc");
+  // hipMalloc does not zero; the solver accumulates into fresh vectors.
+  hip_check(hipMemsetAsync(p, 0, static_cast<size_t>(n) * sizeof(double), stream_), "hipMemsetAsync");
+  return p;
+}
+
+void HipBackend::deallocate(double* p) {
+  if (p == nullptr) return;
+  synchronize();
+  hip_check(hipFree(p), "hipFree");
+}
+
+void HipBackend::upload(const double* host, double* target, Int n) {
+  if (n <= 0) return;
+  hip_check(hipMemcpyAsync(target, host, static_cast<size_t>(n) * sizeof(double), hipMemcpyHostToDevice,
+                           stream_),
+            "upload");
+  synchronize();
+}
+
+void HipBackend::download(const double* source, double* host, Int n) {
+  if (n <= 0) return;
+  hip_check(hipMemcpyAsync(host, source, static_cast<size_t>(n) * sizeof(double), hipMemcpyDeviceToHost,
+                           stream_),
+            "download");
+  synchronize();
+}
+
+void HipBackend::fill(double* target, Int n, double value) {
+  if (n <= 0) return;
+  Timed t(*this, "fill");
+  launch_fill(stream_, target, n, value);
+}
+
+void HipBackend::copy(const double* source, double* target, Int n) {
+  if (n <= 0) return;
+  Timed t(*this, "copy");
+  hip_check(hipMemcpyAsync(target, source, static_cast<size_t>(n) * sizeof(double),
+                           hipMemcpyDeviceToDevice, stream_),
+            "copy");
+}
+
+double* HipBackend::scratch(int slot, std::size_t doubles) {
+  auto it = scratch_.find(slot);
+  if (it != scratch_.end() && it->second.second >= doubles) return it->second.first;
+  if (it != scratch_.end()) {
+    synchronize();
+    hipFree(it->second.first);
+    scratch_.erase(it);
+  }
+  double* p = nullptr;
+  const std::size_t size = std::max<std::size_t>(doubles, 1024);
+  hip_check(hipMalloc(&p, size * sizeof(double)), "hipMalloc(scratch)");
+  scratch_[slot] = {p, size};
+  return p;
+}
+
+const double* HipBackend::fetch_result(int n) {
+  hip_check(hipMemcpyAsync(result_host_, result_dev_, static_cast<size_t>(n) * sizeof(double),
+                           hipMemcpyDeviceToHost, stream_),
+            "fetch_result");
+  synchronize();
+  return result_host_;
+}
+
+// ---- matrices ----------------------------------------------------------------
+
+void HipBackend::free_matrix(DeviceMatrix* dm) {
+  for (void* p : dm->owned) hipFree(p);
+  dm->owned.clear();
+}
+
+const DevCsr& HipBackend::prepare(const CsrView& a) {
+  auto it = matrices_.find(a.id);
+  if (it != matrices_.end()) return it->second.view;
+  if (a.rows < 0 || a.cols < 0 || a.start == nullptr)
+    throw std::invalid_argument("prepare: malformed matrix");
+  DeviceMatrix dm;
+  const Int nnz = a.nnz();
+  auto put = [&](const void* src, size_t bytes) -> void* {
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, std::max<size_t>(bytes, 8)), "hipMalloc(matrix)");
+    dm.owned.push_back(p);
+    if (bytes > 0) hip_check(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, stream_), "upload matrix");
+    return p;
+  };
+  Int* block_rows = nullptr;
+  Int blocks = 0;
+  plan_blocks(a.rows, a.start, kTileNnz, kThreads, &block_rows, &blocks);
+  dm.view.rows = a.rows;
+  dm.view.cols = a.cols;
+  dm.view.nnz = nnz;
+  dm.view.blocks = blocks;
+  dm.view.start = static_cast<const Int*>(put(a.start, (static_cast<size_t>(a.rows) + 1) * sizeof(Int)));
+  dm.view.index = static_cast<const Int*>(put(a.index, static_cast<size_t>(nnz) * sizeof(Int)));
+  dm.view.value = static_cast<const double*>(put(a.value, static_cast<size_t>(nnz) * sizeof(double)));
+  dm.view.block_row =
+      static_cast<const Int*>(put(block_rows, (static_cast<size_t>(blocks) + 1) * sizeof(Int)));
+  synchronize();
+  std::free(block_rows);
+  auto ins = matrices_.emplace(a.id, std::move(dm));
+  return ins.first->second.view;
+}
+
+void HipBackend::release() {
+  synchronize();
+  for (auto& kv : matrices_) free_matrix(&kv.second);
+  matrices_.clear();
+}
+
+void HipBackend::release_one(std::uint64_t id) {
+  auto it = matrices_.find(id);
+  if (it == matrices_.end()) return;
+  synchronize();
+  free_matrix(&it->second);
+  matrices_.erase(it);
+}
+
+void HipBackend::multiply(const CsrView& a, const double* x, double* y) {
+  const DevCsr& d = prepare(a);
+  Timed t(*this, "spmv");
+  launch_spmv(stream_, d, x, y);
+}
+
+// ---- reductions ---------------------------------------------------------------
+
+double HipBackend::dot(const double* a, const double* b, Int n) {
+  if (n <= 0) return 0.0;
+  Timed t(*this, "dot");
+  launch_dot(stream_, a, b, n, scratch(kSlotRedA, 1024 * kMaxRed), result_dev_);
+  return fetch_result(1)[0];
+}
+
+double HipBackend::two_norm(const double* a, Int n) { return std::sqrt(dot(a, a, n)); }
+
+double HipBackend::inf_norm(const double* a, Int n) {
+  if (n <= 0) return 0.0;
+  Timed t(*this, "inf_norm");
+  launch_absmax(stream_, a, n, scratch(kSlotRedA, 1024 * kMaxRed), result_dev_);
+  return fetch_result(1)[0];
+}
+
+void HipBackend::primal_update(Int n, double tau, double gamma, double halpern, const double* x,
+                               const double* c, const double* kt_y, const double* lower,
+                               const double* upper, const double* anchor, double* x_out,
+                               double* dx, double* x_bar) {
+  Timed t(*this, "primal_update");
+  launch_primal_update(stream_, n, tau, gamma, halpern, x, c, kt_y, lower, upper, anchor, x_out, dx,
+                       x_bar);
+}
+
+void HipBackend::dual_update(Int m, Int neq, double sigma, double gamma, double halpern,
+                             const double* y, const double* q, const double* k_x_bar,
+                             const double* k_x, const double* anchor_y, const double* anchor_kx,
+                             double* y_out, double* dy, double* k_out, double* k_dx) {
+  Timed t(*this, "dual_update");
+  launch_dual_update(stream_, m, neq, sigma, gamma, halpern, y, q, k_x_bar, k_x, anchor_y,
+                     anchor_y != nullptr ? anchor_kx : nullptr, y_out, dy, k_out, k_dx);
+}
+
+void HipBackend::accumulate(Int n, double weight, const double* v, double* sum) {
+  Timed t(*this, "accumulate");
+  launch_accumulate(stream_, n, weight, v, sum);
+}
+
+void HipBackend::scale_into(Int n, double weight, const double* sum, double* out) {
+  Timed t(*this, "scale_into");
+  launch_scale_into(stream_, n, weight, sum, out);
+}
+
+double HipBackend::weighted_norm_squared(Int n, Int m, const double* dx, const double* dy,
+                                         double omega) {
+  Timed t(*this, "weighted_norm");
+  launch_sumsq2(stream_, dx, n, dy, m, scratch(kSlotRedA, 1024 * kMaxRed),
+                scratch(kSlotRedB, 1024 * kMaxRed), result_dev_);
+  const double* r = fetch_result(2);
+  return omega * r[0] + r[1] / omega;
+}
+
+void HipBackend::step_size_terms(Int n, Int m, const double* dx, const double* dy,
+                                 const double* k_dx, double omega, double* interaction,
+                                 double* movement) {
+  Timed t(*this, "step_terms");
+  launch_dot_and_sumsq(stream_, dy, k_dx, m, dx, n, scratch(kSlotRedA, 1024 * kMaxRed),
+                       scratch(kSlotRedB, 1024 * kMaxRed), result_dev_);
+  const double* r = fetch_result(3);
+  // The absolute value belongs to the backend (backend.cpp:135-139).
+  *interaction = std::fabs(r[0]);
+  *movement = omega * r[2] + r[1] / omega;
+}
+
+Terms HipBackend::convergence_terms(Int n, Int m, const DevProblem& p, const double* x,
+                                    const double* k_x, const double* y, const double* kt_y) {
+  Timed t(*this, "convergence_terms");
+  launch_convergence_terms(stream_, n, m, p, x, k_x, y, kt_y, scratch(kSlotRedA, 1024 * kMaxRed),
+                           scratch(kSlotRedB, 1024 * kMaxRed), result_dev_);
+  const double* r = fetch_result(7);
+  Terms out;
+  out.primal_sq = r[0];
+  out.primal_inf = r[1];
+  out.q_dot_y = r[2];
+  out.dual_sq = r[3];
+  out.dual_inf = r[4];
+  out.bound_term = r[5];
+  out.c_dot_x = r[6];
+  return out;
+}
+
+// ---- profiling -------------------------------------------------------------
+
+void HipBackend::set_profiling(bool on) {
+  profiling_ = on;
+  if (on) profile_.clear();
+}
+
+std::string HipBackend::profile_report() const {
+  if (profile_.empty()) return {};
+  std::vector<std::pair<std::string, ProfileEntry>> rows(profile_.begin(), profile_.end());
+  std::sort(rows.begin(), rows.end(),
+            [](const auto& a, const auto& b) { return a.second.ms > b.second.ms; });
+  double total = 0.0;
+  for (const auto& r : rows) total += r.second.ms;
+  std::string out = "  kernel                  calls      total ms    mean us   share\n";
+  char line[160];
+  for (const auto& r : rows) {
+    std::snprintf(line, sizeof(line), "  %-22s %7ld %12.3f %10.2f %6.1f%%\n", r.first.c_str(),
+                  r.second.calls, r.second.ms, 1e3 * r.second.ms / static_cast<double>(r.second.calls),
+                  total > 0 ? 100.0 * r.second.ms / total : 0.0);
+    out += line;
+  }
+  return out;
+}
+
+}  // namespace anuk

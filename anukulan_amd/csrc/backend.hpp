@@ -1,0 +1,137 @@
+// HipBackend: the LinAlgBackend operations (src/anukulan/backend.hpp:24-215)
+// on one MI355X, one HIP stream, device-resident vectors.
+//
+// It owns every device allocation it hands out, caches uploaded matrices by
+// their 64-bit identity (never by host address, test_backend.cpp:194-225),
+// and keeps the small scratch the reductions fold through. Reductions return
+// host doubles and therefore synchronise, as the reference interface
+// requires; the fused solve in pdhg.cpp avoids them on the hot path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "kernels.hpp"
+#include "lp.hpp"
+
+namespace anuk {
+
+void hip_check(hipError_t e, const char* what);
+
+// A host CSR as the C ABI passes it: borrowed pointers plus an identity.
+struct CsrView {
+  std::uint64_t id = 0;
+  Int rows = 0;
+  Int cols = 0;
+  const Int* start = nullptr;
+  const Int* index = nullptr;
+  const double* value = nullptr;
+  Int nnz() const { return start ? start[rows] : 0; }
+};
+
+inline CsrView view_of(const Csr& a) {
+  return CsrView{a.id, a.rows, a.cols, a.start.data(), a.index.data(), a.value.data()};
+}
+
+class HipBackend {
+ public:
+  explicit HipBackend(int device);
+  ~HipBackend();
+  HipBackend(const HipBackend&) = delete;
+  HipBackend& operator=(const HipBackend&) = delete;
+
+  std::string name() const { return "hip"; }
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  void synchronize();
+
+  // memory (backend.hpp:53-58)
+  double* allocate(Int n);
+  void deallocate(double* p);
+  void upload(const double* host, double* target, Int n);
+  void download(const double* source, double* host, Int n);
+  void fill(double* target, Int n, double value);
+  void copy(const double* source, double* target, Int n);
+
+  // matrices (backend.hpp:35-36, 61-64)
+  const DevCsr& prepare(const CsrView& a);
+  void release();
+  void release_one(std::uint64_t id);
+  void multiply(const CsrView& a, const double* x, double* y);
+
+  // reductions and steps (backend.hpp:66-201)
+  double dot(const double* a, const double* b, Int n);
+  double two_norm(const double* a, Int n);
+  double inf_norm(const double* a, Int n);
+  void primal_update(Int n, double tau, double gamma, double halpern, const double* x,
+                     const double* c, const double* kt_y, const double* lower,
+                     const double* upper, const double* anchor, double* x_out, double* dx,
+                     double* x_bar);
+  void dual_update(Int m, Int neq, double sigma, double gamma, double halpern, const double* y,
+                   const double* q, const double* k_x_bar, const double* k_x,
+                   const double* anchor_y, const double* anchor_kx, double* y_out, double* dy,
+                   double* k_out, double* k_dx);
+  void accumulate(Int n, double weight, const double* v, double* sum);
+  void scale_into(Int n, double weight, const double* sum, double* out);
+  double weighted_norm_squared(Int n, Int m, const double* dx, const double* dy, double omega);
+  void step_size_terms(Int n, Int m, const double* dx, const double* dy, const double* k_dx,
+                       double omega, double* interaction, double* movement);
+  Terms convergence_terms(Int n, Int m, const DevProblem& p, const double* x,
+                          const double* k_x, const double* y, const double* kt_y);
+
+  // profiling (backend.hpp:203-214)
+  void set_profiling(bool on);
+  bool profiling() const { return profiling_; }
+  std::string profile_report() const;
+
+  // Scratch for the solver: a device buffer of at least `doubles`, one per
+  // slot, grown on demand and kept for the context's life.
+  double* scratch(int slot, std::size_t doubles);
+  // A small device result vector and its pinned host mirror.
+  double* result_device() { return result_dev_; }
+  double* result_host() { return result_host_; }
+  // Copies n doubles of the result vector to the pinned mirror and waits.
+  const double* fetch_result(int n);
+
+  // Brackets a launch sequence for the profiler; no-ops when profiling is off.
+  class Timed {
+   public:
+    Timed(HipBackend& b, const char* name);
+    ~Timed();
+    Timed(const Timed&) = delete;
+    Timed& operator=(const Timed&) = delete;
+
+   private:
+    HipBackend& b_;
+    const char* name_;
+    hipEvent_t start_ = nullptr;
+  };
+
+ private:
+  struct DeviceMatrix {
+    DevCsr view;
+    std::vector<void*> owned;
+  };
+  void free_matrix(DeviceMatrix* dm);
+
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  std::unordered_map<std::uint64_t, DeviceMatrix> matrices_;
+  std::map<int, std::pair<double*, std::size_t>> scratch_;
+  double* result_dev_ = nullptr;
+  double* result_host_ = nullptr;
+  bool profiling_ = false;
+  struct ProfileEntry {
+    double ms = 0.0;
+    long calls = 0;
+  };
+  std::map<std::string, ProfileEntry> profile_;
+  hipEvent_t stop_event_ = nullptr;
+};
+
+}  // namespace anuk

@@ -1,0 +1,501 @@
+// extern "C" entry points of libanukulan_hip.so (declared in
+// include/anukulan_hip.h). Each one validates its arguments, forwards to the
+// C++ backend or solver, and turns any exception into a status code plus a
+// message retrievable with anuk_last_error.
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/anukulan_hip.h"
+#include "backend.hpp"
+#include "lp.hpp"
+#include "pdhg.hpp"
+
+struct anuk_ctx {
+  std::unique_ptr<anuk::HipBackend> be;
+  std::string error;
+};
+
+struct anuk_lp {
+  anuk::StandardLp lp;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int status_of(anuk_ctx* ctx, int code, const char* what) {
+  if (ctx != nullptr) {
+    ctx->error = what;
+  } else {
+    g_create_error = what;
+  }
+  return code;
+}
+
+template <class F>
+int guarded(anuk_ctx* ctx, F&& f) {
+  try {
+    f();
+    if (ctx != nullptr) ctx->error.clear();
+    return ANUK_OK;
+  } catch (const std::bad_alloc&) {
+    return status_of(ctx, ANUK_ERR_NOMEM, "out of memory");
+  } catch (const std::invalid_argument& e) {
+    return status_of(ctx, ANUK_ERR_INVALID, e.what());
+  } catch (const std::runtime_error& e) {
+    return status_of(ctx, ANUK_ERR_DEVICE, e.what());
+  } catch (const std::exception& e) {
+    return status_of(ctx, ANUK_ERR_INTERNAL, e.what());
+  }
+}
+
+void need(bool cond, const char* msg) {
+  if (!cond) throw std::invalid_argument(msg);
+}
+
+anuk::CsrView view(const anuk_csr* a) {
+  need(a != nullptr && a->start != nullptr, "matrix is null");
+  need(a->rows >= 0 && a->cols >= 0, "matrix has negative dimensions");
+  const anuk::Int nnz = a->start[a->rows];
+  need(nnz == 0 || (a->index != nullptr && a->value != nullptr), "matrix arrays are null");
+  return anuk::CsrView{a->id, a->rows, a->cols, a->start, a->index, a->value};
+}
+
+anuk::DevProblem problem_of(const anuk_conv_problem* p, anuk::Int n, anuk::Int m) {
+  need(p != nullptr, "convergence problem is null");
+  anuk::DevProblem d;
+  d.n = n;
+  d.m = m;
+  d.num_equalities = p->num_equalities;
+  d.orig_c = p->c;
+  d.orig_lower = p->lower;
+  d.orig_upper = p->upper;
+  d.orig_q = p->q;
+  d.row_scale = p->row_scale;
+  d.col_scale = p->col_scale;
+  return d;
+}
+
+void fill_residual(const anuk::Residual& r, anuk_pdhg_residual* o) {
+  o->primal_residual = r.primal_residual;
+  o->dual_residual = r.dual_residual;
+  o->primal_residual_inf = r.primal_residual_inf;
+  o->dual_residual_inf = r.dual_residual_inf;
+  o->primal_objective = r.primal_objective;
+  o->dual_objective = r.dual_objective;
+  o->absolute_gap = r.absolute_gap;
+  o->relative_primal = r.relative_primal;
+  o->relative_dual = r.relative_dual;
+  o->relative_gap = r.relative_gap;
+  o->relative_primal_inf = r.relative_primal_inf;
+  o->relative_dual_inf = r.relative_dual_inf;
+}
+
+}  // namespace
+
+extern "C" {
+
+int anuk_create(int device, anuk_ctx** out) {
+  if (out == nullptr) return ANUK_ERR_INVALID;
+  *out = nullptr;
+  try {
+    auto ctx = std::make_unique<anuk_ctx>();
+    ctx->be = std::make_unique<anuk::HipBackend>(device);
+    *out = ctx.release();
+    g_create_error.clear();
+    return ANUK_OK;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return ANUK_ERR_DEVICE;
+  }
+}
+
+void anuk_destroy(anuk_ctx* ctx) { delete ctx; }
+
+const char* anuk_last_error(const anuk_ctx* ctx) {
+  return ctx != nullptr ? ctx->error.c_str() : g_create_error.c_str();
+}
+
+const char* anuk_name(const anuk_ctx* ctx) {
+  (void)ctx;
+  return "hip";
+}
+
+int anuk_synchronize(anuk_ctx* ctx) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] { ctx->be->synchronize(); });
+}
+
+int anuk_allocate(anuk_ctx* ctx, int32_t n, double** out) {
+  if (ctx == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0, "negative size");
+    *out = ctx->be->allocate(n);
+  });
+}
+
+int anuk_deallocate(anuk_ctx* ctx, double* p) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] { ctx->be->deallocate(p); });
+}
+
+int anuk_upload(anuk_ctx* ctx, const double* host, double* target, int32_t n) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && (n == 0 || (host && target)), "bad upload arguments");
+    ctx->be->upload(host, target, n);
+  });
+}
+
+int anuk_download(anuk_ctx* ctx, const double* source, double* host, int32_t n) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && (n == 0 || (host && source)), "bad download arguments");
+    ctx->be->download(source, host, n);
+  });
+}
+
+int anuk_fill(anuk_ctx* ctx, double* target, int32_t n, double value) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && (n == 0 || target), "bad fill arguments");
+    ctx->be->fill(target, n, value);
+  });
+}
+
+int anuk_copy(anuk_ctx* ctx, const double* source, double* target, int32_t n) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && (n == 0 || (source && target)), "bad copy arguments");
+    ctx->be->copy(source, target, n);
+  });
+}
+
+int anuk_prepare(anuk_ctx* ctx, const anuk_csr* a) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] { ctx->be->prepare(view(a)); });
+}
+
+int anuk_release(anuk_ctx* ctx) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] { ctx->be->release(); });
+}
+
+int anuk_multiply(anuk_ctx* ctx, const anuk_csr* a, const double* x, double* y) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    const anuk::CsrView v = view(a);
+    need(v.rows == 0 || y != nullptr, "output vector is null");
+    need(v.cols == 0 || x != nullptr || v.nnz() == 0, "input vector is null");
+    ctx->be->multiply(v, x, y);
+  });
+}
+
+int anuk_multiply_transpose(anuk_ctx* ctx, const anuk_csr* at, const double* y, double* x) {
+  return anuk_multiply(ctx, at, y, x);
+}
+
+int anuk_dot(anuk_ctx* ctx, const double* a, const double* b, int32_t n, double* out) {
+  if (ctx == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && (n == 0 || (a && b)), "bad dot arguments");
+    *out = ctx->be->dot(a, b, n);
+  });
+}
+
+int anuk_two_norm(anuk_ctx* ctx, const double* a, int32_t n, double* out) {
+  if (ctx == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && (n == 0 || a), "bad two_norm arguments");
+    *out = ctx->be->two_norm(a, n);
+  });
+}
+
+int anuk_inf_norm(anuk_ctx* ctx, const double* a, int32_t n, double* out) {
+  if (ctx == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && (n == 0 || a), "bad inf_norm arguments");
+    *out = ctx->be->inf_norm(a, n);
+  });
+}
+
+int anuk_primal_update(anuk_ctx* ctx, int32_t n, double tau, double gamma, double halpern,
+                       const double* x, const double* c, const double* kt_y, const double* lower,
+                       const double* upper, const double* anchor, double* x_out, double* dx,
+                       double* x_bar) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0, "negative size");
+    need(n == 0 || (x && c && kt_y && lower && upper && x_out && dx && x_bar), "null vector");
+    ctx->be->primal_update(n, tau, gamma, halpern, x, c, kt_y, lower, upper, anchor, x_out, dx, x_bar);
+  });
+}
+
+int anuk_dual_update(anuk_ctx* ctx, int32_t m, int32_t num_equalities, double sigma, double gamma,
+                     double halpern, const double* y, const double* q, const double* k_x_bar,
+                     const double* k_x, const double* anchor_y, const double* anchor_kx,
+                     double* y_out, double* dy, double* k_out, double* k_dx) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(m >= 0, "negative size");
+    need(m == 0 || (y && q && k_x_bar && k_x && y_out && dy && k_out && k_dx), "null vector");
+    need(anchor_y == nullptr || anchor_kx != nullptr, "anchor_y given without anchor_kx");
+    ctx->be->dual_update(m, num_equalities, sigma, gamma, halpern, y, q, k_x_bar, k_x, anchor_y,
+                         anchor_kx, y_out, dy, k_out, k_dx);
+  });
+}
+
+int anuk_accumulate(anuk_ctx* ctx, int32_t n, double weight, const double* v, double* sum) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && (n == 0 || (v && sum)), "bad accumulate arguments");
+    ctx->be->accumulate(n, weight, v, sum);
+  });
+}
+
+int anuk_scale_into(anuk_ctx* ctx, int32_t n, double weight, const double* sum, double* out) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && (n == 0 || (sum && out)), "bad scale_into arguments");
+    ctx->be->scale_into(n, weight, sum, out);
+  });
+}
+
+int anuk_weighted_norm_squared(anuk_ctx* ctx, int32_t n, int32_t m, const double* dx,
+                               const double* dy, double omega, double* out) {
+  if (ctx == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && m >= 0 && (n == 0 || dx) && (m == 0 || dy), "bad weighted norm arguments");
+    *out = ctx->be->weighted_norm_squared(n, m, dx, dy, omega);
+  });
+}
+
+int anuk_step_size_terms(anuk_ctx* ctx, int32_t n, int32_t m, const double* dx, const double* dy,
+                         const double* k_dx, double omega, double* interaction, double* movement) {
+  if (ctx == nullptr || interaction == nullptr || movement == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && m >= 0 && (n == 0 || dx) && (m == 0 || (dy && k_dx)), "bad step term arguments");
+    ctx->be->step_size_terms(n, m, dx, dy, k_dx, omega, interaction, movement);
+  });
+}
+
+int anuk_convergence_terms(anuk_ctx* ctx, int32_t n, int32_t m, const anuk_conv_problem* lp,
+                           const double* x, const double* k_x, const double* y, const double* kt_y,
+                           anuk_conv_terms* out) {
+  if (ctx == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && m >= 0, "negative size");
+    need(n == 0 || (x && kt_y && lp && lp->c && lp->lower && lp->upper && lp->col_scale),
+         "null column vector");
+    need(m == 0 || (k_x && y && lp && lp->q && lp->row_scale), "null row vector");
+    const anuk::Terms t = ctx->be->convergence_terms(n, m, problem_of(lp, n, m), x, k_x, y, kt_y);
+    out->primal_sq = t.primal_sq;
+    out->primal_inf = t.primal_inf;
+    out->q_dot_y = t.q_dot_y;
+    out->dual_sq = t.dual_sq;
+    out->dual_inf = t.dual_inf;
+    out->bound_term = t.bound_term;
+    out->c_dot_x = t.c_dot_x;
+  });
+}
+
+int anuk_set_profiling(anuk_ctx* ctx, int on) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] { ctx->be->set_profiling(on != 0); });
+}
+
+int anuk_profile_report(anuk_ctx* ctx, char* buf, size_t cap, size_t* needed) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    const std::string r = ctx->be->profile_report();
+    if (needed) *needed = r.size() + 1;
+    if (buf != nullptr && cap > 0) {
+      const size_t k = std::min(cap - 1, r.size());
+      std::memcpy(buf, r.data(), k);
+      buf[k] = '\0';
+    }
+  });
+}
+
+// ---- host layout ------------------------------------------------------------
+
+int anuk_lp_from_model(const anuk_model* model, anuk_lp** out) {
+  if (model == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  *out = nullptr;
+  return guarded(nullptr, [&] {
+    need(model->rows >= 0 && model->cols >= 0 && model->nnz >= 0, "negative model size");
+    need(model->objective && model->col_lower && model->col_upper, "null column data");
+    need(model->rows == 0 || (model->row_lower && model->row_upper), "null row data");
+    need(model->nnz == 0 || (model->row_index && model->col_index && model->value), "null triplets");
+    anuk::Model m;
+    m.rows = model->rows;
+    m.cols = model->cols;
+    m.entries.resize(static_cast<size_t>(model->nnz));
+    for (int64_t k = 0; k < model->nnz; ++k)
+      m.entries[static_cast<size_t>(k)] = {model->row_index[k], model->col_index[k], model->value[k]};
+    m.objective.assign(model->objective, model->objective + model->cols);
+    m.col_lower.assign(model->col_lower, model->col_lower + model->cols);
+    m.col_upper.assign(model->col_upper, model->col_upper + model->cols);
+    if (model->rows > 0) {
+      m.row_lower.assign(model->row_lower, model->row_lower + model->rows);
+      m.row_upper.assign(model->row_upper, model->row_upper + model->rows);
+    }
+    m.maximize = model->maximize != 0;
+    m.objective_offset = model->objective_offset;
+    auto h = std::make_unique<anuk_lp>();
+    h->lp = anuk::to_standard_form(m);
+    *out = h.release();
+  });
+}
+
+int anuk_lp_from_standard(int32_t rows, int32_t cols, int32_t num_equalities, const int32_t* start,
+                          const int32_t* index, const double* value, const double* c,
+                          const double* q, const double* lower, const double* upper,
+                          double objective_scale, double objective_offset, anuk_lp** out) {
+  if (out == nullptr) return ANUK_ERR_INVALID;
+  *out = nullptr;
+  return guarded(nullptr, [&] {
+    need(rows >= 0 && cols >= 0, "negative size");
+    need(num_equalities >= 0 && num_equalities <= rows, "num_equalities out of range");
+    need(start != nullptr && (cols == 0 || (c && lower && upper)) && (rows == 0 || q), "null array");
+    auto h = std::make_unique<anuk_lp>();
+    anuk::StandardLp& lp = h->lp;
+    lp.k = anuk::Csr::from_csr(rows, cols, start, index, value);
+    lp.kt = lp.k.transpose();
+    lp.c.assign(c, c + cols);
+    lp.q.assign(q, q + rows);
+    lp.lower.assign(lower, lower + cols);
+    lp.upper.assign(upper, upper + cols);
+    lp.num_equalities = num_equalities;
+    lp.objective_scale = objective_scale;
+    lp.objective_offset = objective_offset;
+    lp.row_model.resize(static_cast<size_t>(rows));
+    lp.row_sign.assign(static_cast<size_t>(rows), 1.0);
+    for (int32_t i = 0; i < rows; ++i) lp.row_model[static_cast<size_t>(i)] = i;
+    for (int32_t j = 0; j < cols; ++j)
+      need(!(lp.lower[static_cast<size_t>(j)] > lp.upper[static_cast<size_t>(j)]), "empty bound interval");
+    *out = h.release();
+  });
+}
+
+void anuk_lp_free(anuk_lp* lp) { delete lp; }
+
+int anuk_lp_info_get(const anuk_lp* lp, anuk_lp_info* out) {
+  if (lp == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  out->rows = lp->lp.rows();
+  out->cols = lp->lp.cols();
+  out->nnz = lp->lp.k.nnz();
+  out->num_equalities = lp->lp.num_equalities;
+  out->objective_scale = lp->lp.objective_scale;
+  out->objective_offset = lp->lp.objective_offset;
+  return ANUK_OK;
+}
+
+int anuk_lp_export(const anuk_lp* h, int32_t* start, int32_t* index, double* value, double* c,
+                   double* q, double* lower, double* upper, int32_t* row_model, double* row_sign) {
+  if (h == nullptr) return ANUK_ERR_INVALID;
+  const anuk::StandardLp& lp = h->lp;
+  auto put_i = [](int32_t* d, const std::vector<int32_t>& s) {
+    if (d && !s.empty()) std::memcpy(d, s.data(), s.size() * sizeof(int32_t));
+  };
+  auto put_d = [](double* d, const std::vector<double>& s) {
+    if (d && !s.empty()) std::memcpy(d, s.data(), s.size() * sizeof(double));
+  };
+  put_i(start, lp.k.start);
+  put_i(index, lp.k.index);
+  put_d(value, lp.k.value);
+  put_d(c, lp.c);
+  put_d(q, lp.q);
+  put_d(lower, lp.lower);
+  put_d(upper, lp.upper);
+  put_i(row_model, lp.row_model);
+  put_d(row_sign, lp.row_sign);
+  return ANUK_OK;
+}
+
+int anuk_lp_scale(const anuk_lp* lp, int32_t ruiz_iterations, int32_t pock_chambolle,
+                  anuk_lp** scaled, double* row_scale, double* col_scale) {
+  if (lp == nullptr || scaled == nullptr) return ANUK_ERR_INVALID;
+  *scaled = nullptr;
+  return guarded(nullptr, [&] {
+    need(ruiz_iterations >= 0, "negative sweep count");
+    auto h = std::make_unique<anuk_lp>();
+    h->lp = lp->lp;
+    const anuk::Scaling s = anuk::scale_lp(&h->lp, ruiz_iterations, pock_chambolle != 0);
+    if (row_scale && !s.row_scale.empty())
+      std::memcpy(row_scale, s.row_scale.data(), s.row_scale.size() * sizeof(double));
+    if (col_scale && !s.col_scale.empty())
+      std::memcpy(col_scale, s.col_scale.data(), s.col_scale.size() * sizeof(double));
+    *scaled = h.release();
+  });
+}
+
+// ---- the solve --------------------------------------------------------------
+
+void anuk_pdhg_default_options(anuk_pdhg_options* options) {
+  if (options != nullptr) anuk::default_options(options);
+}
+
+int anuk_solve_pdhg(anuk_ctx* ctx, const anuk_lp* lp, const anuk_pdhg_options* options,
+                    anuk_pdhg_result* result, double* x_out, double* y_out) {
+  if (ctx == nullptr || lp == nullptr || result == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    anuk_pdhg_options o;
+    if (options != nullptr) {
+      o = *options;
+    } else {
+      anuk::default_options(&o);
+    }
+    need(o.termination_check_frequency > 0, "termination_check_frequency must be positive");
+    need(o.max_iterations >= 0, "max_iterations must be non-negative");
+    need(o.polish_first_iteration > 0, "polish_first_iteration must be positive");
+    const anuk::SolveResult r = anuk::solve_pdhg(*ctx->be, lp->lp, o);
+    std::memset(result, 0, sizeof(*result));
+    result->status = r.status;
+    result->iterations = r.iterations;
+    result->restarts = r.restarts;
+    result->polish_attempts = r.polish_attempts;
+    result->polish_iterations = r.polish_iterations;
+    result->polished = r.polished ? 1 : 0;
+    result->objective = r.objective;
+    result->solve_seconds = r.solve_seconds;
+    result->matrix_norm_estimate = r.matrix_norm_estimate;
+    result->final_step_size = r.final_step_size;
+    result->final_primal_weight = r.final_primal_weight;
+    result->phase_scaling = r.phases.scaling;
+    result->phase_matrix_norm = r.phases.matrix_norm;
+    result->phase_setup = r.phases.setup;
+    result->phase_iterating = r.phases.iterating;
+    result->phase_checks = r.phases.checks;
+    result->phase_polishing = r.phases.polishing;
+    fill_residual(r.residual, &result->residual);
+    std::strncpy(result->message, r.message.c_str(), sizeof(result->message) - 1);
+    if (x_out && !r.x.empty()) std::memcpy(x_out, r.x.data(), r.x.size() * sizeof(double));
+    if (y_out && !r.y.empty()) std::memcpy(y_out, r.y.data(), r.y.size() * sizeof(double));
+  });
+}
+
+int anuk_estimate_matrix_norm(anuk_ctx* ctx, const anuk_lp* lp, int32_t iterations,
+                              double tolerance, double* out) {
+  if (ctx == nullptr || lp == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(iterations >= 0, "negative iteration count");
+    *out = anuk::estimate_matrix_norm(*ctx->be, lp->lp, iterations, tolerance);
+  });
+}
+
+int anuk_evaluate_residual(const anuk_lp* lp, const double* x, const double* y,
+                           anuk_pdhg_residual* out) {
+  if (lp == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  return guarded(nullptr, [&] {
+    need((lp->lp.cols() == 0 || x) && (lp->lp.rows() == 0 || y), "null vector");
+    const std::vector<double> vx(x, x + lp->lp.cols());
+    const std::vector<double> vy(y, y + lp->lp.rows());
+    fill_residual(anuk::evaluate_residual(lp->lp, vx, vy), out);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,703 @@
+// CDNA4 (gfx950) kernels for the PDHG backend. See kernels.hpp for the
+// numerical contract. Compiled with -ffp-contract=off: every multiply and add
+// below rounds on its own, as the reference's CPU loops do.
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+#include <vector>
+
+namespace anuk {
+
+namespace {
+
+constexpr int kWaves = kThreads / 64;
+
+// Workgroups are dispatched round-robin over the 8 XCDs; this maps hardware
+// workgroup i to a logical block so each XCD gets one contiguous range of
+// row blocks, and neighbouring rows share that XCD's L2 for their gathers.
+__device__ __forceinline__ int xcd_block(int i, int g) {
+  const int x = i & 7, local = i >> 3, per = g >> 3, rem = g & 7;
+  return x * per + (x < rem ? x : rem) + local;
+}
+
+__device__ __forceinline__ double wave_fold(double v, bool is_max) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = is_max ? fmax(v, o) : v + o;
+  }
+  return v;
+}
+
+// Folds R per-thread values over the workgroup in a fixed tree and has thread
+// 0 store them to out[0..R). Every thread of the workgroup must call it.
+template <int R>
+__device__ void block_fold_store(double (&v)[R], unsigned max_mask, double* out) {
+  __shared__ double lds[R][kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = wave_fold(v[r], (max_mask >> r) & 1u);
+  if (lane == 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[r][wave] = v[r];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double a = lds[r][0];
+      const bool mx = (max_mask >> r) & 1u;
+      for (int w = 1; w < kWaves; ++w) a = mx ? fmax(a, lds[r][w]) : a + lds[r][w];
+      out[r] = a;
+    }
+  }
+}
+
+__device__ double block_sum(double v) {
+  __shared__ double lds[kWaves];
+  v = wave_fold(v, false);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double a = lds[0];
+  for (int w = 1; w < kWaves; ++w) a += lds[w];
+  return a;  // every thread gets the same value
+}
+
+// ---------------------------------------------------------------------------
+// Row-blocked CSR product with an epilogue.
+//
+// A short-row block stages value*x[index] for all its nonzeros in LDS with
+// coalesced loads, then thread t sums row r0+t serially in storage order: the
+// CPU's order, so the row sum has the CPU's bits. A long-row block (one row,
+// more nonzeros than a tile) strides the row over the workgroup and folds in a
+// fixed tree. The epilogue receives (row, sum) from exactly one thread per
+// row and may add to up to kR reduction values, folded per block into
+// partials[block * kR ...].
+// ---------------------------------------------------------------------------
+template <class Epi>
+__global__ void __launch_bounds__(kThreads)
+    spmv_kernel(DevCsr a, const double* __restrict__ x, Epi epi, double* __restrict__ partials) {
+  __shared__ double tile[kTileNnz];
+  constexpr int R = Epi::kR > 0 ? Epi::kR : 1;
+  const Epi e = epi.resolve();
+  if (e.skip()) return;  // uniform across the grid
+  const int b = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  const Int r0 = a.block_row[b], r1 = a.block_row[b + 1];
+  const Int k0 = a.start[r0], k1 = a.start[r1];
+  const Int len = k1 - k0;
+  double red[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) red[r] = 0.0;
+
+  if (len <= kTileNnz) {
+#pragma unroll
+    for (int u = 0; u < kTileNnz / kThreads; ++u) {
+      const Int k = static_cast<Int>(threadIdx.x) + u * kThreads;
+      if (k < len) tile[k] = a.value[k0 + k] * x[a.index[k0 + k]];
+    }
+    __syncthreads();
+    const Int r = r0 + static_cast<Int>(threadIdx.x);
+    if (r < r1) {
+      const Int lo = a.start[r] - k0, hi = a.start[r + 1] - k0;
+      double s = 0.0;
+      for (Int k = lo; k < hi; ++k) s += tile[k];
+      e(r, s, red);
+    }
+  } else {
+    double s = 0.0;
+    for (Int k = k0 + static_cast<Int>(threadIdx.x); k < k1; k += kThreads)
+      s += a.value[k] * x[a.index[k]];
+    s = block_sum(s);
+    if (threadIdx.x == 0) e(r0, s, red);
+  }
+  if constexpr (Epi::kR > 0) {
+    block_fold_store<Epi::kR>(red, Epi::kMaxMask, partials + static_cast<size_t>(b) * Epi::kR);
+  }
+}
+
+struct PlainEpi {
+  static constexpr int kR = 0;
+  static constexpr unsigned kMaxMask = 0;
+  double* y;
+  __device__ PlainEpi resolve() const { return *this; }
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(Int r, double s, double*) const { y[r] = s; }
+};
+
+// Halpern schedule of one step of a period, from the period's starting state:
+// an epoch's first step takes no blend and its result becomes the anchor; the
+// k-th step after it blends with weight (k+1)/(k+2).
+struct StepScalars {
+  double tau, sigma, gamma, w;
+  bool blend, open_epoch;
+};
+
+__device__ __forceinline__ StepScalars step_scalars(const IterParams* p, int step) {
+  StepScalars s;
+  s.tau = p->tau;
+  s.sigma = p->sigma;
+  s.gamma = p->gamma;
+  if (!p->halpern) {
+    s.blend = false;
+    s.open_epoch = false;
+    s.w = 1.0;
+  } else if (p->reset_at_start && step == 0) {
+    s.blend = false;
+    s.open_epoch = true;
+    s.w = 1.0;
+  } else {
+    const Int k = p->reset_at_start ? step - 1 : p->k_at_start + step;
+    s.blend = true;
+    s.open_epoch = false;
+    s.w = static_cast<double>(k + 1) / static_cast<double>(k + 2);
+  }
+  return s;
+}
+
+// Primal side of one PDHG step on column j, given (K^T y)_j
+// (backend.cpp:60-79 order).
+template <bool kDiag>
+struct PrimalStepEpi {
+  static constexpr int kR = kDiag ? 1 : 0;
+  static constexpr unsigned kMaxMask = 0;
+  DevProblem p;
+  PrimalBufs b;
+  const IterParams* params;
+  int step;
+  StepScalars sc;
+  __device__ PrimalStepEpi resolve() const {
+    PrimalStepEpi e = *this;
+    e.sc = step_scalars(params, step);
+    return e;
+  }
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(Int j, double kty, double* red) const {
+    const double xj = b.x[j];
+    double v = xj - sc.tau * (p.c[j] - kty);
+    if (v < p.lower[j]) v = p.lower[j];
+    if (v > p.upper[j]) v = p.upper[j];
+    const double d = v - xj;
+    b.x_bar[j] = v + d;
+    double next = v;
+    if (sc.gamma != 0.0) next += sc.gamma * d;
+    if (sc.blend) next = sc.w * next + (1.0 - sc.w) * b.anchor[j];
+    b.x_out[j] = next;
+    if (sc.open_epoch) b.anchor_out[j] = next;
+    if constexpr (kDiag) {
+      b.dx[j] = d;
+      red[0] += d * d;
+    }
+  }
+};
+
+// Dual side on row i, given (K x_bar)_i (backend.cpp:81-111 order). On a
+// diagonal step it also produces the row half of the convergence terms for
+// the new point (backend.cpp:153-164, unscaled through D1) and the row half of
+// the Farkas test on dy (pdhg.cpp:193-198, 215-216).
+template <bool kDiag>
+struct DualStepEpi {
+  static constexpr int kR = kDiag ? 7 : 0;
+  static constexpr unsigned kMaxMask = (1u << 2) | (1u << 4) | (1u << 5);
+  DevProblem p;
+  DualBufs b;
+  const IterParams* params;
+  int step;
+  StepScalars sc;
+  __device__ DualStepEpi resolve() const {
+    DualStepEpi e = *this;
+    e.sc = step_scalars(params, step);
+    return e;
+  }
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(Int i, double kxb, double* red) const {
+    const double yi = b.y[i];
+    const double kx = b.k_x[i];
+    double v = yi + sc.sigma * (p.q[i] - kxb);
+    if (i >= p.num_equalities && v < 0.0) v = 0.0;
+    const double d = v - yi;
+    const double kd = 0.5 * (kxb - kx);
+    double ny = v;
+    double nk = 0.5 * (kxb + kx);
+    if (sc.gamma != 0.0) {
+      ny += sc.gamma * d;
+      nk += sc.gamma * kd;
+    }
+    if (sc.blend) {
+      ny = sc.w * ny + (1.0 - sc.w) * b.anchor_y[i];
+      nk = sc.w * nk + (1.0 - sc.w) * b.anchor_kx[i];
+    }
+    b.y_out[i] = ny;
+    b.k_out[i] = nk;
+    if (sc.open_epoch) {
+      b.anchor_y_out[i] = ny;
+      b.anchor_kx_out[i] = nk;
+    }
+    if constexpr (kDiag) {
+      b.dy[i] = d;
+      b.k_dx[i] = kd;
+      red[0] += d * d;
+      const double scale = p.row_scale[i];
+      const double row = scale != 0.0 ? nk / scale : nk;
+      const double slack = row - p.orig_q[i];
+      const double viol = (i < p.num_equalities) ? slack : fmin(slack, 0.0);
+      red[1] += viol * viol;
+      red[2] = fmax(red[2], fabs(viol));
+      red[3] += p.orig_q[i] * (scale * ny);
+      if (i >= p.num_equalities && d < 0.0) red[4] = 1.0;
+      red[5] = fmax(red[5], fabs(d));
+      red[6] += p.q[i] * d;
+    }
+  }
+};
+
+// Column half of the convergence terms from (K~^T y~)_j (backend.cpp:166-199).
+struct ConvProductEpi {
+  static constexpr int kR = 4;
+  static constexpr unsigned kMaxMask = 1u << 1;
+  DevProblem p;
+  const double* x;
+  __device__ ConvProductEpi resolve() const { return *this; }
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(Int j, double kty, double* red) const {
+    const double scale = p.col_scale[j];
+    const double product = scale != 0.0 ? kty / scale : kty;
+    const double lambda = p.orig_c[j] - product;
+    const bool has_lo = p.orig_lower[j] > -__builtin_inf();
+    const bool has_hi = p.orig_upper[j] < __builtin_inf();
+    double left;
+    if (has_lo && has_hi) {
+      left = 0.0;
+    } else if (has_lo) {
+      left = fmin(lambda, 0.0);
+    } else if (has_hi) {
+      left = fmax(lambda, 0.0);
+    } else {
+      left = lambda;
+    }
+    red[0] += left * left;
+    red[1] = fmax(red[1], fabs(left));
+    const double absorbed = lambda - left;
+    if (absorbed > 0.0) {
+      red[2] += absorbed * p.orig_lower[j];
+    } else if (absorbed < 0.0) {
+      red[2] += absorbed * p.orig_upper[j];
+    }
+    red[3] += p.orig_c[j] * (scale * x[j]);
+  }
+};
+
+// Column half of the Farkas test from lambda = (K~^T dy)_j (pdhg.cpp:203-213).
+// Skipped as a whole when the row half already rules a certificate out.
+struct FarkasProductEpi {
+  static constexpr int kR = 2;
+  static constexpr unsigned kMaxMask = 1u << 0;
+  DevProblem p;
+  const double* check;  // row slots already folded
+  bool skip_all;
+  __device__ FarkasProductEpi resolve() const {
+    FarkasProductEpi e = *this;
+    e.skip_all = check[kFarkasRowBad] != 0.0 || !(check[kFarkasScale] > 0.0);
+    return e;
+  }
+  __device__ bool skip() const { return skip_all; }
+  __device__ void operator()(Int j, double lambda, double* red) const {
+    if (lambda > 0.0) {
+      if (p.upper[j] >= __builtin_inf()) {
+        red[0] = 1.0;
+      } else {
+        red[1] += lambda * p.upper[j];
+      }
+    } else if (lambda < 0.0) {
+      if (p.lower[j] <= -__builtin_inf()) {
+        red[0] = 1.0;
+      } else {
+        red[1] += lambda * p.lower[j];
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Folding per-block partials into final values, one workgroup, fixed order.
+// ---------------------------------------------------------------------------
+struct FoldSeg {
+  const double* partials;
+  Int blocks;
+  int nvals;
+  int slot0;
+  unsigned max_mask;
+};
+
+struct FoldArgs {
+  FoldSeg seg[2];
+  int nseg;
+  double* out;
+};
+
+__global__ void __launch_bounds__(kThreads) fold_kernel(FoldArgs a) {
+  for (int s = 0; s < a.nseg; ++s) {
+    const FoldSeg g = a.seg[s];
+    for (int v = 0; v < g.nvals; ++v) {
+      const bool mx = (g.max_mask >> v) & 1u;
+      double acc = 0.0;
+      for (Int b = static_cast<Int>(threadIdx.x); b < g.blocks; b += kThreads) {
+        const double t = g.partials[static_cast<size_t>(b) * g.nvals + v];
+        acc = mx ? fmax(acc, t) : acc + t;
+      }
+      double r[1] = {acc};
+      block_fold_store<1>(r, mx ? 1u : 0u, a.out + g.slot0 + v);
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Grid-stride reductions over vectors (the standalone backend operations).
+// ---------------------------------------------------------------------------
+template <class Op>
+__global__ void __launch_bounds__(kThreads) reduce_kernel(Int len, Op op, double* partials) {
+  double red[Op::kR];
+#pragma unroll
+  for (int r = 0; r < Op::kR; ++r) red[r] = 0.0;
+  const Int stride = static_cast<Int>(gridDim.x) * kThreads;
+  for (Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x); i < len;
+       i += stride)
+    op(i, red);
+  block_fold_store<Op::kR>(red, Op::kMaxMask, partials + static_cast<size_t>(blockIdx.x) * Op::kR);
+}
+
+struct DotOp {
+  static constexpr int kR = 1;
+  static constexpr unsigned kMaxMask = 0;
+  const double* a;
+  const double* b;
+  __device__ void operator()(Int i, double* red) const { red[0] += a[i] * b[i]; }
+};
+
+struct AbsMaxOp {
+  static constexpr int kR = 1;
+  static constexpr unsigned kMaxMask = 1;
+  const double* a;
+  __device__ void operator()(Int i, double* red) const { red[0] = fmax(red[0], fabs(a[i])); }
+};
+
+struct DotSumSqOp {  // dy'k_dx and dy'dy over the same rows
+  static constexpr int kR = 2;
+  static constexpr unsigned kMaxMask = 0;
+  const double* dy;
+  const double* k_dx;
+  __device__ void operator()(Int i, double* red) const {
+    red[0] += dy[i] * k_dx[i];
+    red[1] += dy[i] * dy[i];
+  }
+};
+
+struct ConvRowOp {  // backend.cpp:153-164
+  static constexpr int kR = 3;
+  static constexpr unsigned kMaxMask = 1u << 1;
+  const double *q, *row_scale, *k_x, *y;
+  Int neq;
+  __device__ void operator()(Int i, double* red) const {
+    const double scale = row_scale[i];
+    const double row = scale != 0.0 ? k_x[i] / scale : k_x[i];
+    const double slack = row - q[i];
+    const double viol = (i < neq) ? slack : fmin(slack, 0.0);
+    red[0] += viol * viol;
+    red[1] = fmax(red[1], fabs(viol));
+    red[2] += q[i] * (scale * y[i]);
+  }
+};
+
+// backend.cpp:166-199 with a K^T y already in memory.
+struct ConvColOp {
+  static constexpr int kR = 4;
+  static constexpr unsigned kMaxMask = 1u << 1;
+  ConvProductEpi inner;
+  const double* kt_y;
+  __device__ void operator()(Int j, double* red) const { inner(j, kt_y[j], red); }
+};
+
+// ---------------------------------------------------------------------------
+// Elementwise kernels.
+// ---------------------------------------------------------------------------
+__global__ void fill_kernel(double* p, Int n, double v) {
+  const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (i < n) p[i] = v;
+}
+
+__global__ void accumulate_kernel(Int n, double w, const double* v, double* sum) {
+  const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (i < n) sum[i] += w * v[i];
+}
+
+__global__ void scale_into_kernel(Int n, double w, const double* sum, double* out) {
+  const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (i < n) out[i] = sum[i] / w;
+}
+
+// The standalone fused updates (LinAlgBackend::primal_update/dual_update).
+// Every input is read into registers before any output is written, so x_out
+// may alias x and y_out / k_out may alias y / k_x.
+__global__ void primal_update_kernel(Int n, double tau, double gamma, double h,
+                                     const double* x, const double* c, const double* kt_y,
+                                     const double* lower, const double* upper,
+                                     const double* anchor, double* x_out, double* dx,
+                                     double* x_bar) {
+  const Int j = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (j >= n) return;
+  const double xj = x[j];
+  const double a = anchor != nullptr ? anchor[j] : 0.0;
+  double v = xj - tau * (c[j] - kt_y[j]);
+  if (v < lower[j]) v = lower[j];
+  if (v > upper[j]) v = upper[j];
+  const double d = v - xj;
+  double next = v;
+  if (gamma != 0.0) next += gamma * d;
+  if (anchor != nullptr) next = h * next + (1.0 - h) * a;
+  dx[j] = d;
+  x_bar[j] = v + d;
+  x_out[j] = next;
+}
+
+__global__ void dual_update_kernel(Int m, Int neq, double sigma, double gamma, double h,
+                                   const double* y, const double* q, const double* k_x_bar,
+                                   const double* k_x, const double* anchor_y,
+                                   const double* anchor_kx, double* y_out, double* dy,
+                                   double* k_out, double* k_dx) {
+  const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (i >= m) return;
+  const double yi = y[i];
+  const double bar = k_x_bar[i];
+  const double kx = k_x[i];
+  const bool blend = anchor_y != nullptr;
+  const double ay = blend ? anchor_y[i] : 0.0;
+  const double ak = blend ? anchor_kx[i] : 0.0;
+  double v = yi + sigma * (q[i] - bar);
+  if (i >= neq && v < 0.0) v = 0.0;
+  const double d = v - yi;
+  const double kd = 0.5 * (bar - kx);
+  double ny = v;
+  double nk = 0.5 * (bar + kx);
+  if (gamma != 0.0) {
+    ny += gamma * d;
+    nk += gamma * kd;
+  }
+  if (blend) {
+    ny = h * ny + (1.0 - h) * ay;
+    nk = h * nk + (1.0 - h) * ak;
+  }
+  dy[i] = d;
+  k_dx[i] = kd;
+  y_out[i] = ny;
+  k_out[i] = nk;
+}
+
+inline unsigned grid_for(Int n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+
+void check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("launch ") + what + ": " + hipGetErrorString(e));
+}
+
+void fold(hipStream_t s, std::initializer_list<FoldSeg> segs, double* out) {
+  FoldArgs a{};
+  a.nseg = 0;
+  for (const FoldSeg& g : segs) a.seg[a.nseg++] = g;
+  a.out = out;
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(kThreads), 0, s, a);
+  check_launch("fold");
+}
+
+template <class Epi>
+void spmv_with(hipStream_t s, const DevCsr& a, const double* x, const Epi& e, double* partials,
+               const char* what) {
+  if (a.blocks <= 0) return;
+  hipLaunchKernelGGL(spmv_kernel<Epi>, dim3(static_cast<unsigned>(a.blocks)), dim3(kThreads), 0,
+                     s, a, x, e, partials);
+  check_launch(what);
+}
+
+template <class Op>
+void reduce_with(hipStream_t s, Int len, const Op& op, double* partials) {
+  hipLaunchKernelGGL(reduce_kernel<Op>, dim3(static_cast<unsigned>(reduce_grid(len))),
+                     dim3(kThreads), 0, s, len, op, partials);
+  check_launch("reduce");
+}
+
+}  // namespace
+
+// Greedy row blocks over the CSR: a block closes when the next row would push
+// its nonzeros past a tile or its rows past one workgroup; a row longer than a
+// tile is a block of its own.
+void plan_blocks(Int rows, const Int* start, Int tile_nnz, int max_rows, std::int32_t** out_rows,
+                 Int* out_blocks) {
+  std::vector<Int> br;
+  br.reserve(static_cast<size_t>(rows) / 16 + 2);
+  br.push_back(0);
+  Int r = 0;
+  while (r < rows) {
+    if (start[r + 1] - start[r] > tile_nnz) {
+      ++r;
+      br.push_back(r);
+      continue;
+    }
+    Int nnz = 0;
+    int count = 0;
+    while (r < rows && count < max_rows) {
+      const Int len = start[r + 1] - start[r];
+      if (len > tile_nnz || nnz + len > tile_nnz) break;
+      nnz += len;
+      ++count;
+      ++r;
+    }
+    br.push_back(r);
+  }
+  *out_blocks = static_cast<Int>(br.size() - 1);
+  auto* mem = static_cast<std::int32_t*>(std::malloc(br.size() * sizeof(std::int32_t)));
+  if (mem == nullptr) throw std::bad_alloc();
+  std::copy(br.begin(), br.end(), mem);
+  *out_rows = mem;
+}
+
+int reduce_grid(Int len) {
+  const Int per = kThreads * 4;
+  Int g = (len + per - 1) / per;
+  if (g < 1) g = 1;
+  if (g > 1024) g = 1024;
+  return static_cast<int>(g);
+}
+
+void launch_spmv(hipStream_t s, const DevCsr& a, const double* x, double* y) {
+  spmv_with(s, a, x, PlainEpi{y}, nullptr, "spmv");
+}
+
+void launch_fill(hipStream_t s, double* p, Int n, double v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
+  check_launch("fill");
+}
+
+void launch_accumulate(hipStream_t s, Int n, double w, const double* v, double* sum) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(accumulate_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, n, w, v, sum);
+  check_launch("accumulate");
+}
+
+void launch_scale_into(hipStream_t s, Int n, double w, const double* sum, double* out) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(scale_into_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, n, w, sum, out);
+  check_launch("scale_into");
+}
+
+void launch_primal_update(hipStream_t s, Int n, double tau, double gamma, double h,
+                          const double* x, const double* c, const double* kt_y,
+                          const double* lower, const double* upper, const double* anchor,
+                          double* x_out, double* dx, double* x_bar) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(primal_update_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, n, tau, gamma,
+                     h, x, c, kt_y, lower, upper, anchor, x_out, dx, x_bar);
+  check_launch("primal_update");
+}
+
+void launch_dual_update(hipStream_t s, Int m, Int neq, double sigma, double gamma, double h,
+                        const double* y, const double* q, const double* k_x_bar,
+                        const double* k_x, const double* anchor_y, const double* anchor_kx,
+                        double* y_out, double* dy, double* k_out, double* k_dx) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(dual_update_kernel, dim3(grid_for(m)), dim3(kThreads), 0, s, m, neq, sigma,
+                     gamma, h, y, q, k_x_bar, k_x, anchor_y, anchor_kx, y_out, dy, k_out, k_dx);
+  check_launch("dual_update");
+}
+
+void launch_dot(hipStream_t s, const double* a, const double* b, Int n, double* partials,
+                double* out) {
+  reduce_with(s, n, DotOp{a, b}, partials);
+  fold(s, {FoldSeg{partials, reduce_grid(n), 1, 0, 0u}}, out);
+}
+
+void launch_sumsq2(hipStream_t s, const double* a, Int n, const double* b, Int m,
+                   double* partials_a, double* partials_b, double* out2) {
+  reduce_with(s, n, DotOp{a, a}, partials_a);
+  reduce_with(s, m, DotOp{b, b}, partials_b);
+  fold(s, {FoldSeg{partials_a, reduce_grid(n), 1, 0, 0u}, FoldSeg{partials_b, reduce_grid(m), 1, 1, 0u}},
+       out2);
+}
+
+void launch_absmax(hipStream_t s, const double* a, Int n, double* partials, double* out) {
+  reduce_with(s, n, AbsMaxOp{a}, partials);
+  fold(s, {FoldSeg{partials, reduce_grid(n), 1, 0, 1u}}, out);
+}
+
+void launch_dot_and_sumsq(hipStream_t s, const double* dy, const double* k_dx, Int m,
+                          const double* dx, Int n, double* partials_a, double* partials_b,
+                          double* out3) {
+  reduce_with(s, m, DotSumSqOp{dy, k_dx}, partials_a);
+  reduce_with(s, n, DotOp{dx, dx}, partials_b);
+  fold(s, {FoldSeg{partials_a, reduce_grid(m), 2, 0, 0u}, FoldSeg{partials_b, reduce_grid(n), 1, 2, 0u}},
+       out3);
+}
+
+void launch_convergence_terms(hipStream_t s, Int n, Int m, const DevProblem& p,
+                              const double* x, const double* k_x, const double* y,
+                              const double* kt_y, double* partials_a, double* partials_b,
+                              double* out7) {
+  ConvRowOp row{p.orig_q, p.row_scale, k_x, y, p.num_equalities};
+  ConvColOp col{ConvProductEpi{p, x}, kt_y};
+  reduce_with(s, m, row, partials_a);
+  reduce_with(s, n, col, partials_b);
+  fold(s, {FoldSeg{partials_a, reduce_grid(m), 3, 0, ConvRowOp::kMaxMask},
+           FoldSeg{partials_b, reduce_grid(n), 4, 3, ConvColOp::kMaxMask}},
+       out7);
+}
+
+void launch_primal_step(hipStream_t s, const DevCsr& kt, const double* y, const DevProblem& p,
+                        const IterParams* params, int step, const PrimalBufs& b, unsigned flags,
+                        double* partials) {
+  if (flags & kStepDiag) {
+    spmv_with(s, kt, y, PrimalStepEpi<true>{p, b, params, step, {}}, partials, "primal_step");
+  } else {
+    spmv_with(s, kt, y, PrimalStepEpi<false>{p, b, params, step, {}}, partials, "primal_step");
+  }
+}
+
+void launch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar, const DevProblem& p,
+                      const IterParams* params, int step, const DualBufs& b, unsigned flags,
+                      double* partials) {
+  if (flags & kStepDiag) {
+    spmv_with(s, k, x_bar, DualStepEpi<true>{p, b, params, step, {}}, partials, "dual_step");
+  } else {
+    spmv_with(s, k, x_bar, DualStepEpi<false>{p, b, params, step, {}}, partials, "dual_step");
+  }
+}
+
+void launch_conv_product(hipStream_t s, const DevCsr& kt, const double* y, const double* x,
+                         const DevProblem& p, double* partials) {
+  spmv_with(s, kt, y, ConvProductEpi{p, x}, partials, "conv_product");
+}
+
+void launch_farkas_product(hipStream_t s, const DevCsr& kt, const double* dy,
+                           const DevProblem& p, const double* check, double* partials) {
+  spmv_with(s, kt, dy, FarkasProductEpi{p, check, false}, partials, "farkas_product");
+}
+
+void launch_check_fold_rows(hipStream_t s, Int primal_blocks, const double* pa, Int dual_blocks,
+                            const double* pb, double* out) {
+  fold(s, {FoldSeg{pa, primal_blocks, 1, kSumDx2, 0u},
+           FoldSeg{pb, dual_blocks, 7, kSumDy2, DualStepEpi<true>::kMaxMask}},
+       out);
+}
+
+void launch_check_fold_cols(hipStream_t s, Int conv_blocks, const double* pc, Int farkas_blocks,
+                            const double* pd, double* out) {
+  if (pd != nullptr) {
+    fold(s, {FoldSeg{pc, conv_blocks, 4, kDualSq, ConvProductEpi::kMaxMask},
+             FoldSeg{pd, farkas_blocks, 2, kFarkasColBad, FarkasProductEpi::kMaxMask}},
+         out);
+  } else {
+    fold(s, {FoldSeg{pc, conv_blocks, 4, kDualSq, ConvProductEpi::kMaxMask}}, out);
+  }
+}
+
+}  // namespace anuk

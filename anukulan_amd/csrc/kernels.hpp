@@ -1,0 +1,184 @@
+// Device kernels of the MI355X backend and the host launchers for them.
+//
+// Everything is fp64. Elementwise arithmetic is written in the reference's
+// operation order and compiled without FMA contraction, so a fused step gives
+// the same bits as the CPU backend (backend.cpp:60-111). Row sums of the
+// sparse products are taken serially in storage order for every row that fits
+// one LDS tile, which makes them the same bits as SparseMatrix::multiply
+// (sparse.cpp:72-80) too. Reductions are deterministic (fixed grid, fixed tree)
+// but not serial, so they agree with the CPU's serial sums to rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace anuk {
+
+using Int = std::int32_t;
+
+inline constexpr int kThreads = 256;      // one workgroup = 4 wave64s
+inline constexpr int kTileNnz = 1024;     // products staged in LDS per row block
+inline constexpr int kMaxRed = 8;         // most reduction values one kernel emits
+
+// A CSR matrix on the device plus its row-block plan: block b covers rows
+// [block_row[b], block_row[b+1]). A block either holds up to kThreads rows
+// whose nonzeros together fit one LDS tile (summed serially per row), or a
+// single longer row (summed by the whole workgroup).
+struct DevCsr {
+  Int rows = 0;
+  Int cols = 0;
+  Int nnz = 0;
+  Int blocks = 0;
+  const Int* start = nullptr;
+  const Int* index = nullptr;
+  const double* value = nullptr;
+  const Int* block_row = nullptr;
+};
+
+// Per-iteration scalars the fused kernels read from device memory, so a
+// captured period of iterations can be replayed with new values.
+struct IterParams {
+  double tau;
+  double sigma;
+  double gamma;              // reflection
+  Int halpern;               // 1 when the Halpern scheme is on
+  Int reset_at_start;        // the period's first step begins a new epoch
+  Int k_at_start;            // Halpern counter before the period's first step
+  Int num_equalities;
+};
+
+// Which of the per-iteration extras a fused step produces.
+enum StepFlags : unsigned {
+  kStepDiag = 1u,   // write dx / dy / k_dx and the check reductions
+};
+
+// Problem data the fused kernels read. Scaled data drive the iteration and
+// the Farkas test; the original data and the two diagonals give the
+// convergence measures of the unscaled problem (backend.hpp:150-165).
+struct DevProblem {
+  Int n = 0, m = 0, num_equalities = 0;
+  const double* c = nullptr;       // scaled
+  const double* lower = nullptr;
+  const double* upper = nullptr;
+  const double* q = nullptr;
+  const double* orig_c = nullptr;  // unscaled
+  const double* orig_lower = nullptr;
+  const double* orig_upper = nullptr;
+  const double* orig_q = nullptr;
+  const double* row_scale = nullptr;
+  const double* col_scale = nullptr;
+};
+
+// The iterate buffers of one side of a step.
+struct PrimalBufs {
+  const double* x;      // current
+  const double* anchor; // Halpern anchor
+  double* x_out;
+  double* x_bar;
+  double* dx;           // written on diagnostic steps
+  double* anchor_out;   // written when the step opens an epoch
+};
+
+struct DualBufs {
+  const double* y;
+  const double* k_x;
+  const double* anchor_y;
+  const double* anchor_kx;
+  double* y_out;
+  double* k_out;
+  double* dy;
+  double* k_dx;
+  double* anchor_y_out;
+  double* anchor_kx_out;
+};
+
+// Slots of the check result vector (all fp64, reduced on the device).
+enum CheckSlot : int {
+  kSumDx2 = 0,
+  kSumDy2,
+  kPrimalSq,
+  kPrimalInf,
+  kQDotY,
+  kFarkasRowBad,    // any inequality row with dy < 0 (max of 0/1)
+  kFarkasScale,     // max |dy|
+  kFarkasRhs,       // q' dy
+  kDualSq,
+  kDualInf,
+  kBoundTerm,
+  kCDotX,
+  kFarkasColBad,    // lambda pointing at an infinite bound (max of 0/1)
+  kFarkasSupport,
+  kCheckSlots
+};
+
+// Workspace for multi-block reductions: partials[blocks * kMaxRed] per source.
+struct ReduceScratch {
+  double* partials_a = nullptr;  // primal step / generic
+  double* partials_b = nullptr;  // dual step
+  double* partials_c = nullptr;  // convergence product
+  double* partials_d = nullptr;  // Farkas product
+  Int capacity = 0;              // blocks each array holds
+};
+
+// ---- launchers (stream-ordered, asynchronous) --------------------------
+
+// Builds the row-block plan of a CSR on the host.
+void plan_blocks(Int rows, const Int* start, Int tile_nnz, int max_rows,
+                 std::int32_t** out_rows, Int* out_blocks);
+
+void launch_spmv(hipStream_t s, const DevCsr& a, const double* x, double* y);
+
+void launch_fill(hipStream_t s, double* p, Int n, double v);
+void launch_accumulate(hipStream_t s, Int n, double w, const double* v, double* sum);
+void launch_scale_into(hipStream_t s, Int n, double w, const double* sum, double* out);
+void launch_primal_update(hipStream_t s, Int n, double tau, double gamma, double h,
+                          const double* x, const double* c, const double* kt_y,
+                          const double* lower, const double* upper, const double* anchor,
+                          double* x_out, double* dx, double* x_bar);
+void launch_dual_update(hipStream_t s, Int m, Int neq, double sigma, double gamma, double h,
+                        const double* y, const double* q, const double* k_x_bar,
+                        const double* k_x, const double* anchor_y, const double* anchor_kx,
+                        double* y_out, double* dy, double* k_out, double* k_dx);
+
+// Reductions. Each writes its results to `out` (device) after a fixed-order
+// finish; `partials` must hold reduce_grid(len) * kMaxRed doubles.
+int reduce_grid(Int len);
+void launch_dot(hipStream_t s, const double* a, const double* b, Int n, double* partials,
+                double* out);
+void launch_sumsq2(hipStream_t s, const double* a, Int n, const double* b, Int m,
+                   double* partials_a, double* partials_b, double* out2);
+void launch_absmax(hipStream_t s, const double* a, Int n, double* partials, double* out);
+void launch_dot_and_sumsq(hipStream_t s, const double* dy, const double* k_dx, Int m,
+                          const double* dx, Int n, double* partials_a, double* partials_b,
+                          double* out3);
+void launch_convergence_terms(hipStream_t s, Int n, Int m, const DevProblem& p,
+                              const double* x, const double* k_x, const double* y,
+                              const double* kt_y, double* partials_a, double* partials_b,
+                              double* out7);
+
+// The fused PDHG step. `flags` selects the diagnostic variant; `step` is the
+// position of this step in its period (for the Halpern weight).
+void launch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
+                        const DevProblem& p, const IterParams* params, int step,
+                        const PrimalBufs& b, unsigned flags, double* partials);
+void launch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
+                      const DevProblem& p, const IterParams* params, int step,
+                      const DualBufs& b, unsigned flags, double* partials);
+// Check products: dual-side convergence terms from K^T y, and the Farkas
+// support from K^T dy. The Farkas product reads the row slots of `check`
+// (folded first by launch_check_fold_rows) and does nothing when the row half
+// of the test has already ruled a certificate out.
+void launch_conv_product(hipStream_t s, const DevCsr& kt, const double* y,
+                         const double* x, const DevProblem& p, double* partials);
+void launch_farkas_product(hipStream_t s, const DevCsr& kt, const double* dy,
+                           const DevProblem& p, const double* check, double* partials);
+// Folds the partials of a check into check[kCheckSlots]: first the two step
+// kernels' (slots kSumDx2..kFarkasRhs), then the two products' (kDualSq..).
+// pd may be null when infeasibility detection is off.
+void launch_check_fold_rows(hipStream_t s, Int primal_blocks, const double* pa,
+                            Int dual_blocks, const double* pb, double* check);
+void launch_check_fold_cols(hipStream_t s, Int conv_blocks, const double* pc,
+                            Int farkas_blocks, const double* pd, double* check);
+
+}  // namespace anuk

@@ -1,0 +1,373 @@
+/*
+ * anukulan_hip.h - C ABI of the MI355X (gfx950) backend for Anukulan's
+ * first-order LP method (restarted reflected-Halpern PDHG).
+ *
+ * Two layers sit behind this header, both in libanukulan_hip.so:
+ *
+ *  1. The backend operations. One entry point per virtual of the reference's
+ *     LinAlgBackend (src/anukulan/backend.hpp:24-215), with the same argument
+ *     lists and the same semantics: device-resident double vectors owned by
+ *     the context, CSR matrices cached by a caller-supplied 64-bit identity,
+ *     aliasing rules as documented there. A C++ LinAlgBackend subclass that
+ *     forwards every virtual here is a complete drop-in for the reference's
+ *     CudaBackend (src/anukulan/cuda_backend.cu).
+ *
+ *  2. Host-side LP layout and a whole device-resident PDHG solve, replacing
+ *     solve_pdhg (src/anukulan/pdhg.cpp:377-1285) for callers that want the
+ *     fused, launch-light loop rather than one backend call per operation.
+ *
+ * Conventions: every function returns ANUK_OK (0) or a nonzero status; the
+ * message for the last failure on a context is anuk_last_error(ctx). No C++
+ * exception crosses this boundary. Sizes are int32 (the reference's Int,
+ * src/anukulan/sparse.hpp:11). A context is bound to one device and one HIP
+ * stream and is not thread-safe; use one context per host thread.
+ */
+#ifndef ANUKULAN_HIP_H
+#define ANUKULAN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum anuk_status {
+  ANUK_OK = 0,
+  ANUK_ERR_INVALID = 1,     /* bad argument (null, negative size, shape mismatch) */
+  ANUK_ERR_DEVICE = 2,      /* HIP runtime error, or no device */
+  ANUK_ERR_NOMEM = 3,       /* device or host allocation failed */
+  ANUK_ERR_UNSUPPORTED = 4, /* option combination this backend does not run */
+  ANUK_ERR_INTERNAL = 5
+};
+
+typedef struct anuk_ctx anuk_ctx;
+typedef struct anuk_lp anuk_lp;
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Creates a context on HIP device `device`. Replaces the process-wide
+ * cuda_backend() singleton (backend.hpp:233-236, cuda_backend.cu:894-897);
+ * fails with ANUK_ERR_DEVICE where the reference constructor throws
+ * (cuda_backend.cu:441). */
+int anuk_create(int device, anuk_ctx** out);
+void anuk_destroy(anuk_ctx* ctx);
+/* Message of the last failed call on ctx ("" if none). With ctx == NULL, the
+ * last failure on this thread of a call that takes no context (anuk_create,
+ * the anuk_lp_* functions, anuk_evaluate_residual). */
+const char* anuk_last_error(const anuk_ctx* ctx);
+/* LinAlgBackend::name (backend.hpp:28). */
+const char* anuk_name(const anuk_ctx* ctx);
+/* Blocks until every launch on the context's stream has finished. */
+int anuk_synchronize(anuk_ctx* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Memory: LinAlgBackend::allocate..copy (backend.hpp:53-58)                 */
+/* ------------------------------------------------------------------------ */
+
+/* Device vector of n doubles, zero filled (test_backend.cpp:85-98 relies on
+ * this). n == 0 returns NULL and ANUK_OK, as the CPU backend does
+ * (backend.cpp:22-24). */
+int anuk_allocate(anuk_ctx* ctx, int32_t n, double** out);
+int anuk_deallocate(anuk_ctx* ctx, double* p);
+int anuk_upload(anuk_ctx* ctx, const double* host, double* target, int32_t n);
+int anuk_download(anuk_ctx* ctx, const double* source, double* host, int32_t n);
+int anuk_fill(anuk_ctx* ctx, double* target, int32_t n, double value);
+int anuk_copy(anuk_ctx* ctx, const double* source, double* target, int32_t n);
+
+/* ------------------------------------------------------------------------ */
+/* Matrices                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* A host CSR matrix as the reference stores it (sparse.hpp:37-101): rows+1
+ * int32 row starts, int32 column indices sorted and unique per row, fp64
+ * values. `id` is SparseMatrix::id() (sparse.hpp:51): the device copy is
+ * cached under it, never under the host address. */
+typedef struct anuk_csr {
+  uint64_t id;
+  int32_t rows;
+  int32_t cols;
+  const int32_t* start;
+  const int32_t* index;
+  const double* value;
+} anuk_csr;
+
+/* LinAlgBackend::prepare (backend.hpp:35). Idempotent per id. */
+int anuk_prepare(anuk_ctx* ctx, const anuk_csr* a);
+/* LinAlgBackend::release (backend.hpp:36): frees every cached matrix. */
+int anuk_release(anuk_ctx* ctx);
+
+/* y = A x (backend.hpp:61). Uploads A on first use if it was not prepared. */
+int anuk_multiply(anuk_ctx* ctx, const anuk_csr* a, const double* x, double* y);
+/* x = A^T y, given A^T as its own CSR (backend.hpp:62-64). */
+int anuk_multiply_transpose(anuk_ctx* ctx, const anuk_csr* at, const double* y,
+                            double* x);
+
+/* ------------------------------------------------------------------------ */
+/* Reductions and fused steps (backend.hpp:66-201)                           */
+/* ------------------------------------------------------------------------ */
+
+int anuk_dot(anuk_ctx* ctx, const double* a, const double* b, int32_t n, double* out);
+int anuk_two_norm(anuk_ctx* ctx, const double* a, int32_t n, double* out);
+int anuk_inf_norm(anuk_ctx* ctx, const double* a, int32_t n, double* out);
+
+/* LinAlgBackend::primal_update (backend.hpp:70-93). anchor == NULL means no
+ * Halpern blend. x_out may alias x. */
+int anuk_primal_update(anuk_ctx* ctx, int32_t n, double tau, double gamma,
+                       double halpern, const double* x, const double* c,
+                       const double* kt_y, const double* lower, const double* upper,
+                       const double* anchor, double* x_out, double* dx,
+                       double* x_bar);
+
+/* LinAlgBackend::dual_update (backend.hpp:95-119). anchor_y == NULL means no
+ * blend (anchor_kx is then ignored). y_out may alias y, k_out may alias k_x. */
+int anuk_dual_update(anuk_ctx* ctx, int32_t m, int32_t num_equalities, double sigma,
+                     double gamma, double halpern, const double* y, const double* q,
+                     const double* k_x_bar, const double* k_x, const double* anchor_y,
+                     const double* anchor_kx, double* y_out, double* dy,
+                     double* k_out, double* k_dx);
+
+/* sum += weight * v (backend.hpp:121-123). */
+int anuk_accumulate(anuk_ctx* ctx, int32_t n, double weight, const double* v,
+                    double* sum);
+/* out = sum / weight, true division (backend.hpp:124-126, backend.cpp:118-121). */
+int anuk_scale_into(anuk_ctx* ctx, int32_t n, double weight, const double* sum,
+                    double* out);
+
+/* omega ||dx||^2 + ||dy||^2 / omega (backend.hpp:128-130). */
+int anuk_weighted_norm_squared(anuk_ctx* ctx, int32_t n, int32_t m, const double* dx,
+                               const double* dy, double omega, double* out);
+
+/* interaction = |dy' k_dx|, movement = weighted norm (backend.hpp:146-148,
+ * backend.cpp:131-141), both from one pass. */
+int anuk_step_size_terms(anuk_ctx* ctx, int32_t n, int32_t m, const double* dx,
+                         const double* dy, const double* k_dx, double omega,
+                         double* interaction, double* movement);
+
+/* LinAlgBackend::ConvergenceProblem (backend.hpp:157-165): device pointers. */
+typedef struct anuk_conv_problem {
+  const double* c;
+  const double* lower;
+  const double* upper;
+  const double* q;
+  const double* row_scale;
+  const double* col_scale;
+  int32_t num_equalities;
+} anuk_conv_problem;
+
+/* LinAlgBackend::ConvergenceTerms (backend.hpp:168-176), same field order. */
+typedef struct anuk_conv_terms {
+  double primal_sq;
+  double primal_inf;
+  double q_dot_y;
+  double dual_sq;
+  double dual_inf;
+  double bound_term;
+  double c_dot_x;
+} anuk_conv_terms;
+
+/* LinAlgBackend::convergence_terms (backend.hpp:198-201, backend.cpp:143-202). */
+int anuk_convergence_terms(anuk_ctx* ctx, int32_t n, int32_t m,
+                           const anuk_conv_problem* lp, const double* x,
+                           const double* k_x, const double* y, const double* kt_y,
+                           anuk_conv_terms* out);
+
+/* LinAlgBackend::set_profiling / profile_report (backend.hpp:213-214). The
+ * report is written NUL-terminated into buf (truncated to cap); *needed gets
+ * the full length including the NUL. */
+int anuk_set_profiling(anuk_ctx* ctx, int on);
+int anuk_profile_report(anuk_ctx* ctx, char* buf, size_t cap, size_t* needed);
+
+/* ------------------------------------------------------------------------ */
+/* Host LP layout (standard_form.hpp:23-84, sparse.cpp:24-118)               */
+/* ------------------------------------------------------------------------ */
+
+/* A model in its natural form (model.hpp:23-49, LP part only):
+ *   optimise  c'x + offset   s.t.  row_lower <= A x <= row_upper,
+ *                                  col_lower <= x <= col_upper
+ * A is given as triplets; duplicates are summed and exact zeros dropped, as
+ * SparseMatrix::from_triplets does (sparse.cpp:24-70). +-INFINITY (or any
+ * magnitude >= 1e30) is an absent bound. */
+typedef struct anuk_model {
+  int32_t rows;
+  int32_t cols;
+  int64_t nnz;
+  const int32_t* row_index;
+  const int32_t* col_index;
+  const double* value;
+  const double* objective;
+  const double* row_lower;
+  const double* row_upper;
+  const double* col_lower;
+  const double* col_upper;
+  int32_t maximize;
+  double objective_offset;
+} anuk_model;
+
+/* to_standard_form (standard_form.cpp:72-165): equalities first, <= rows
+ * negated, ranged rows split in two, free rows dropped, K^T cached. */
+int anuk_lp_from_model(const anuk_model* model, anuk_lp** out);
+
+/* A standard-form LP given directly: K as CSR (rows = num_equalities
+ * equality rows first, then >= rows), c, q, bounds, and the map back to the
+ * model objective (standard_form.hpp:36-48). The CSR must be valid
+ * (sparse.hpp:87-88); it is copied. */
+int anuk_lp_from_standard(int32_t rows, int32_t cols, int32_t num_equalities,
+                          const int32_t* start, const int32_t* index,
+                          const double* value, const double* c, const double* q,
+                          const double* lower, const double* upper,
+                          double objective_scale, double objective_offset,
+                          anuk_lp** out);
+void anuk_lp_free(anuk_lp* lp);
+
+typedef struct anuk_lp_info {
+  int32_t rows;
+  int32_t cols;
+  int32_t nnz;
+  int32_t num_equalities;
+  double objective_scale;
+  double objective_offset;
+} anuk_lp_info;
+int anuk_lp_info_get(const anuk_lp* lp, anuk_lp_info* out);
+
+/* Copies the standard form out. Any pointer may be NULL to skip it. start has
+ * rows+1 entries, index/value nnz, c/lower/upper cols, q rows. row_model and
+ * row_sign (rows entries) are StandardLp::row_origin (standard_form.hpp:50-55). */
+int anuk_lp_export(const anuk_lp* lp, int32_t* start, int32_t* index, double* value,
+                   double* c, double* q, double* lower, double* upper,
+                   int32_t* row_model, double* row_sign);
+
+/* scale_lp (scaling.cpp:75-100) on a copy of lp: Ruiz infinity-norm sweeps,
+ * then one Pock-Chambolle 1-norm sweep. Writes the scaled LP to *scaled and
+ * the diagonals D1 (rows) and D2 (cols). */
+int anuk_lp_scale(const anuk_lp* lp, int32_t ruiz_iterations, int32_t pock_chambolle,
+                  anuk_lp** scaled, double* row_scale, double* col_scale);
+
+/* ------------------------------------------------------------------------ */
+/* Whole solve (pdhg.hpp:19-543)                                             */
+/* ------------------------------------------------------------------------ */
+
+enum anuk_termination {
+  ANUK_TERM_FULL = 0,
+  ANUK_TERM_PRIMAL_FEASIBILITY = 1,
+  ANUK_TERM_DUAL_FEASIBILITY = 2
+};
+
+/* PdhgOptions (pdhg.hpp:19-415), every field the plain solve reads. Fill with
+ * anuk_pdhg_default_options, which gives the reference defaults. */
+typedef struct anuk_pdhg_options {
+  double tolerance;                    /* 1e-6 */
+  double gap_tolerance;                /* 0: same as tolerance */
+  int32_t max_iterations;              /* 200000 */
+  double time_limit_seconds;           /* 300 */
+  int32_t termination_check_frequency; /* 40 */
+  double absolute_tolerance;           /* 0 */
+  int32_t require_inf_norm_termination;/* 1 */
+  int32_t adaptive_step_size;          /* 1 (inactive while constant_step_size) */
+  int32_t primal_weight_updates;       /* 1 */
+  int32_t restarts;                    /* 1 */
+  int32_t halpern;                     /* 1 */
+  int32_t halpern_restart;             /* 0 */
+  int32_t halpern_minimum_epoch;       /* 16 */
+  int32_t polish_feasibility;          /* 1 */
+  int32_t polish_first_iteration;      /* 100 */
+  double polish_iteration_factor;      /* 0.125 */
+  int32_t polish_minimum_iterations;   /* 40 */
+  int32_t polish_maximum_iterations;   /* 20000 */
+  int32_t polish_on_exit;              /* 1 */
+  double reflection;                   /* 1.0 */
+  int32_t constant_step_size;          /* 1 */
+  double constant_step_scale;          /* 0.90 */
+  int32_t restart_on_fixed_point;      /* 1 */
+  int32_t pid_primal_weight;           /* 1 */
+  double primal_weight_kp;             /* 0.5 */
+  double primal_weight_ki;             /* 0.0 */
+  double primal_weight_kd;             /* 0.3 */
+  int32_t ruiz_iterations;             /* 10 */
+  int32_t pock_chambolle;              /* 1 */
+  int32_t reuse_products;              /* 1 */
+  int32_t detect_infeasibility;        /* 1 */
+  double infeasibility_tolerance;      /* 1e-8 */
+  int32_t termination;                 /* ANUK_TERM_FULL */
+  const double* warm_x;                /* NULL, or cols entries (see pdhg.hpp:386-391) */
+  const double* warm_y;                /* NULL, or rows entries */
+  double known_matrix_norm;            /* 0: estimate */
+  double initial_step_size;            /* 0: derive */
+  double initial_primal_weight;        /* 0: derive */
+  int32_t verbose;                     /* 0 */
+  int32_t log_frequency;               /* 5000 */
+  const volatile int32_t* stop;        /* NULL; nonzero stops at the next check */
+  int32_t use_graphs;                  /* 1: replay each check period as a HIP graph */
+} anuk_pdhg_options;
+
+void anuk_pdhg_default_options(anuk_pdhg_options* options);
+
+enum anuk_pdhg_status {
+  ANUK_PDHG_OPTIMAL = 0,
+  ANUK_PDHG_PRIMAL_INFEASIBLE = 1,
+  ANUK_PDHG_ITERATION_LIMIT = 2,
+  ANUK_PDHG_TIME_LIMIT = 3,
+  ANUK_PDHG_NUMERICAL_ERROR = 4
+};
+
+/* PdhgResidual (pdhg.hpp:429-496), the numbers only. */
+typedef struct anuk_pdhg_residual {
+  double primal_residual;
+  double dual_residual;
+  double primal_residual_inf;
+  double dual_residual_inf;
+  double primal_objective;
+  double dual_objective;
+  double absolute_gap;
+  double relative_primal;
+  double relative_dual;
+  double relative_gap;
+  double relative_primal_inf;
+  double relative_dual_inf;
+} anuk_pdhg_residual;
+
+/* PdhgResult (pdhg.hpp:498-541) without the vectors, which go to x_out/y_out. */
+typedef struct anuk_pdhg_result {
+  int32_t status;
+  int32_t iterations;
+  int32_t restarts;
+  int32_t polish_attempts;
+  int32_t polish_iterations;
+  int32_t polished;
+  double objective;
+  double solve_seconds;
+  double matrix_norm_estimate;
+  double final_step_size;
+  double final_primal_weight;
+  double phase_scaling;
+  double phase_matrix_norm;
+  double phase_setup;
+  double phase_iterating;
+  double phase_checks;
+  double phase_polishing;
+  anuk_pdhg_residual residual;
+  char message[128];
+} anuk_pdhg_result;
+
+/* solve_pdhg (pdhg.hpp:543, pdhg.cpp:377-1285) on the device of ctx.
+ * x_out (cols doubles) and y_out (rows doubles) receive the unscaled
+ * primal-dual pair; either may be NULL. */
+int anuk_solve_pdhg(anuk_ctx* ctx, const anuk_lp* lp, const anuk_pdhg_options* options,
+                    anuk_pdhg_result* result, double* x_out, double* y_out);
+
+/* estimate_matrix_norm (pdhg.hpp:580-581, pdhg.cpp:223-282) of lp's K as
+ * given (no scaling), by power iteration on the device. */
+int anuk_estimate_matrix_norm(anuk_ctx* ctx, const anuk_lp* lp, int32_t iterations,
+                              double tolerance, double* out);
+
+/* evaluate_residual (pdhg.hpp:548-549, pdhg.cpp:107-174) on the host. */
+int anuk_evaluate_residual(const anuk_lp* lp, const double* x, const double* y,
+                           anuk_pdhg_residual* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANUKULAN_HIP_H */

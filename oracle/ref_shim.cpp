@@ -1,0 +1,482 @@
+// Test infrastructure only: a C entry layer over the reference's own CPU path,
+// compiled from the reference sources where they lie (see oracle/Makefile) into
+// oracle/_ref/libanukulan_ref.so. Only tests/, __graft_entry__.smoke() and
+// bench.py's CPU-baseline leg load it, and only as the checker.
+//
+// Every function here is a thin marshalling wrapper around one reference
+// function; the arithmetic is the reference's.
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "anukulan/backend.hpp"
+#include "anukulan/mps_reader.hpp"
+#include "anukulan/pdhg.hpp"
+#include "anukulan/scaling.hpp"
+#include "anukulan/simplex.hpp"
+#include "anukulan/standard_form.hpp"
+#include "anukulan/threading.hpp"
+
+#include "anukulan_hip.h"
+
+using namespace anukulan;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct RefLp {
+  StandardLp lp;
+};
+
+SparseMatrix csr_to_matrix(Int rows, Int cols, const Int* start, const Int* index,
+                           const double* value) {
+  std::vector<Triplet> t;
+  t.reserve(static_cast<std::size_t>(start[rows]));
+  for (Int r = 0; r < rows; ++r)
+    for (Int k = start[r]; k < start[r + 1]; ++k) t.push_back(Triplet{r, index[k], value[k]});
+  return SparseMatrix::from_triplets(rows, cols, std::move(t));
+}
+
+PdhgOptions to_options(const anuk_pdhg_options* o) {
+  PdhgOptions p;
+  if (o == nullptr) return p;
+  p.tolerance = o->tolerance;
+  p.gap_tolerance = o->gap_tolerance;
+  p.max_iterations = o->max_iterations;
+  p.time_limit_seconds = o->time_limit_seconds;
+  p.termination_check_frequency = o->termination_check_frequency;
+  p.absolute_tolerance = o->absolute_tolerance;
+  p.require_inf_norm_termination = o->require_inf_norm_termination != 0;
+  p.adaptive_step_size = o->adaptive_step_size != 0;
+  p.primal_weight_updates = o->primal_weight_updates != 0;
+  p.restarts = o->restarts != 0;
+  p.halpern = o->halpern != 0;
+  p.halpern_restart = o->halpern_restart != 0;
+  p.halpern_minimum_epoch = o->halpern_minimum_epoch;
+  p.polish_feasibility = o->polish_feasibility != 0;
+  p.polish_first_iteration = o->polish_first_iteration;
+  p.polish_iteration_factor = o->polish_iteration_factor;
+  p.polish_minimum_iterations = o->polish_minimum_iterations;
+  p.polish_maximum_iterations = o->polish_maximum_iterations;
+  p.polish_on_exit = o->polish_on_exit != 0;
+  p.reflection = o->reflection;
+  p.constant_step_size = o->constant_step_size != 0;
+  p.constant_step_scale = o->constant_step_scale;
+  p.restart_on_fixed_point = o->restart_on_fixed_point != 0;
+  p.pid_primal_weight = o->pid_primal_weight != 0;
+  p.primal_weight_kp = o->primal_weight_kp;
+  p.primal_weight_ki = o->primal_weight_ki;
+  p.primal_weight_kd = o->primal_weight_kd;
+  p.scaling.ruiz_iterations = o->ruiz_iterations;
+  p.scaling.pock_chambolle = o->pock_chambolle != 0;
+  p.reuse_products = o->reuse_products != 0;
+  p.detect_infeasibility = o->detect_infeasibility != 0;
+  p.infeasibility_tolerance = o->infeasibility_tolerance;
+  switch (o->termination) {
+    case ANUK_TERM_PRIMAL_FEASIBILITY:
+      p.termination = PdhgOptions::Termination::kPrimalFeasibility;
+      break;
+    case ANUK_TERM_DUAL_FEASIBILITY:
+      p.termination = PdhgOptions::Termination::kDualFeasibility;
+      break;
+    default:
+      p.termination = PdhgOptions::Termination::kFull;
+      break;
+  }
+  p.known_matrix_norm = o->known_matrix_norm;
+  p.initial_step_size = o->initial_step_size;
+  p.initial_primal_weight = o->initial_primal_weight;
+  p.verbose = o->verbose != 0;
+  p.log_frequency = o->log_frequency;
+  return p;
+}
+
+void fill_residual(const PdhgResidual& r, anuk_pdhg_residual* out) {
+  out->primal_residual = r.primal_residual;
+  out->dual_residual = r.dual_residual;
+  out->primal_residual_inf = r.primal_residual_inf;
+  out->dual_residual_inf = r.dual_residual_inf;
+  out->primal_objective = r.primal_objective;
+  out->dual_objective = r.dual_objective;
+  out->absolute_gap = r.absolute_gap;
+  out->relative_primal = r.relative_primal;
+  out->relative_dual = r.relative_dual;
+  out->relative_gap = r.relative_gap;
+  out->relative_primal_inf = r.relative_primal_inf;
+  out->relative_dual_inf = r.relative_dual_inf;
+}
+
+int fail(const std::exception& e) {
+  g_error = e.what();
+  return ANUK_ERR_INTERNAL;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_error.c_str(); }
+
+int ref_default_thread_count() { return default_thread_count(); }
+
+void ref_pdhg_default_options(anuk_pdhg_options* o) {
+  const PdhgOptions p;
+  std::memset(o, 0, sizeof(*o));
+  o->tolerance = p.tolerance;
+  o->gap_tolerance = p.gap_tolerance;
+  o->max_iterations = p.max_iterations;
+  o->time_limit_seconds = p.time_limit_seconds;
+  o->termination_check_frequency = p.termination_check_frequency;
+  o->absolute_tolerance = p.absolute_tolerance;
+  o->require_inf_norm_termination = p.require_inf_norm_termination;
+  o->adaptive_step_size = p.adaptive_step_size;
+  o->primal_weight_updates = p.primal_weight_updates;
+  o->restarts = p.restarts;
+  o->halpern = p.halpern;
+  o->halpern_restart = p.halpern_restart;
+  o->halpern_minimum_epoch = p.halpern_minimum_epoch;
+  o->polish_feasibility = p.polish_feasibility;
+  o->polish_first_iteration = p.polish_first_iteration;
+  o->polish_iteration_factor = p.polish_iteration_factor;
+  o->polish_minimum_iterations = p.polish_minimum_iterations;
+  o->polish_maximum_iterations = p.polish_maximum_iterations;
+  o->polish_on_exit = p.polish_on_exit;
+  o->reflection = p.reflection;
+  o->constant_step_size = p.constant_step_size;
+  o->constant_step_scale = p.constant_step_scale;
+  o->restart_on_fixed_point = p.restart_on_fixed_point;
+  o->pid_primal_weight = p.pid_primal_weight;
+  o->primal_weight_kp = p.primal_weight_kp;
+  o->primal_weight_ki = p.primal_weight_ki;
+  o->primal_weight_kd = p.primal_weight_kd;
+  o->ruiz_iterations = p.scaling.ruiz_iterations;
+  o->pock_chambolle = p.scaling.pock_chambolle;
+  o->reuse_products = p.reuse_products;
+  o->detect_infeasibility = p.detect_infeasibility;
+  o->infeasibility_tolerance = p.infeasibility_tolerance;
+  o->termination = ANUK_TERM_FULL;
+  o->known_matrix_norm = p.known_matrix_norm;
+  o->initial_step_size = p.initial_step_size;
+  o->initial_primal_weight = p.initial_primal_weight;
+  o->verbose = p.verbose;
+  o->log_frequency = p.log_frequency;
+  o->use_graphs = 1;
+}
+
+// ---- LP construction (standard_form.cpp:72-165, mps_reader.cpp) ----------
+
+void* ref_lp_from_model(const anuk_model* m) {
+  try {
+    Model model;
+    model.sense = m->maximize ? ObjSense::kMaximize : ObjSense::kMinimize;
+    model.objective_offset = m->objective_offset;
+    model.objective.assign(m->objective, m->objective + m->cols);
+    std::vector<Triplet> t(static_cast<std::size_t>(m->nnz));
+    for (int64_t k = 0; k < m->nnz; ++k)
+      t[static_cast<std::size_t>(k)] = Triplet{m->row_index[k], m->col_index[k], m->value[k]};
+    model.constraints = SparseMatrix::from_triplets(m->rows, m->cols, std::move(t));
+    auto inf_clip = [](double v) {
+      if (v >= kFileInf) return kInf;
+      if (v <= -kFileInf) return -kInf;
+      return v;
+    };
+    model.row_lower.resize(static_cast<std::size_t>(m->rows));
+    model.row_upper.resize(static_cast<std::size_t>(m->rows));
+    for (Int i = 0; i < m->rows; ++i) {
+      model.row_lower[sz(i)] = inf_clip(m->row_lower[i]);
+      model.row_upper[sz(i)] = inf_clip(m->row_upper[i]);
+    }
+    model.col_lower.resize(static_cast<std::size_t>(m->cols));
+    model.col_upper.resize(static_cast<std::size_t>(m->cols));
+    for (Int j = 0; j < m->cols; ++j) {
+      model.col_lower[sz(j)] = inf_clip(m->col_lower[j]);
+      model.col_upper[sz(j)] = inf_clip(m->col_upper[j]);
+    }
+    model.col_type.assign(static_cast<std::size_t>(m->cols), VarType::kContinuous);
+    StandardFormResult sf = to_standard_form(model);
+    if (!sf.ok) {
+      g_error = sf.error;
+      return nullptr;
+    }
+    auto* out = new RefLp;
+    out->lp = std::move(sf.lp);
+    return out;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+
+void* ref_lp_from_mps(const char* text) {
+  try {
+    std::istringstream in(text);
+    const MpsReadResult r = read_mps_stream(in, "<text>");
+    if (!r.ok) {
+      g_error = r.error;
+      return nullptr;
+    }
+    StandardFormResult sf = to_standard_form(r.model);
+    if (!sf.ok) {
+      g_error = sf.error;
+      return nullptr;
+    }
+    auto* out = new RefLp;
+    out->lp = std::move(sf.lp);
+    return out;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+
+void* ref_lp_from_standard(int32_t rows, int32_t cols, int32_t neq, const int32_t* start,
+                           const int32_t* index, const double* value, const double* c,
+                           const double* q, const double* lower, const double* upper,
+                           double objective_scale, double objective_offset) {
+  try {
+    auto* out = new RefLp;
+    StandardLp& lp = out->lp;
+    lp.k = csr_to_matrix(rows, cols, start, index, value);
+    lp.kt = lp.k.transpose();
+    lp.c.assign(c, c + cols);
+    lp.q.assign(q, q + rows);
+    lp.lower.assign(lower, lower + cols);
+    lp.upper.assign(upper, upper + cols);
+    lp.num_equalities = neq;
+    lp.objective_scale = objective_scale;
+    lp.objective_offset = objective_offset;
+    lp.row_origin.resize(static_cast<std::size_t>(rows));
+    for (Int i = 0; i < rows; ++i) lp.row_origin[sz(i)] = StandardLp::RowOrigin{i, 1.0};
+    return out;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+
+void ref_lp_free(void* h) { delete static_cast<RefLp*>(h); }
+
+int ref_lp_info(const void* h, anuk_lp_info* out) {
+  const StandardLp& lp = static_cast<const RefLp*>(h)->lp;
+  out->rows = lp.num_rows();
+  out->cols = lp.num_cols();
+  out->nnz = lp.k.nnz();
+  out->num_equalities = lp.num_equalities;
+  out->objective_scale = lp.objective_scale;
+  out->objective_offset = lp.objective_offset;
+  return 0;
+}
+
+int ref_lp_export(const void* h, int32_t* start, int32_t* index, double* value, double* c,
+                  double* q, double* lower, double* upper, int32_t* row_model,
+                  double* row_sign) {
+  const StandardLp& lp = static_cast<const RefLp*>(h)->lp;
+  auto put_i = [](int32_t* dst, const std::vector<Int>& src) {
+    if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(Int));
+  };
+  auto put_d = [](double* dst, const std::vector<double>& src) {
+    if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(double));
+  };
+  put_i(start, lp.k.start());
+  put_i(index, lp.k.index());
+  put_d(value, lp.k.value());
+  put_d(c, lp.c);
+  put_d(q, lp.q);
+  put_d(lower, lp.lower);
+  put_d(upper, lp.upper);
+  for (std::size_t i = 0; i < lp.row_origin.size(); ++i) {
+    if (row_model) row_model[i] = lp.row_origin[i].model_row;
+    if (row_sign) row_sign[i] = lp.row_origin[i].sign;
+  }
+  return 0;
+}
+
+// scale_lp (scaling.cpp:75-100) on a copy.
+void* ref_lp_scale(const void* h, int32_t ruiz, int32_t pc, double* row_scale,
+                   double* col_scale) {
+  try {
+    auto* out = new RefLp;
+    out->lp = static_cast<const RefLp*>(h)->lp;
+    ScalingOptions so;
+    so.ruiz_iterations = ruiz;
+    so.pock_chambolle = pc != 0;
+    const ScalingReport rep = scale_lp(&out->lp, so);
+    std::memcpy(row_scale, rep.scaling.row_scale.data(),
+                rep.scaling.row_scale.size() * sizeof(double));
+    std::memcpy(col_scale, rep.scaling.col_scale.data(),
+                rep.scaling.col_scale.size() * sizeof(double));
+    return out;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+
+// ---- The solve (pdhg.cpp:377-1285) ----------------------------------------
+
+int ref_solve_pdhg(const void* h, const anuk_pdhg_options* o, int32_t threads,
+                   anuk_pdhg_result* res, double* x, double* y) {
+  try {
+    const StandardLp& lp = static_cast<const RefLp*>(h)->lp;
+    PdhgOptions p = to_options(o);
+    std::vector<double> wx, wy;
+    if (o != nullptr && o->warm_x != nullptr && o->warm_y != nullptr) {
+      wx.assign(o->warm_x, o->warm_x + lp.num_cols());
+      wy.assign(o->warm_y, o->warm_y + lp.num_rows());
+      p.warm_x = &wx;
+      p.warm_y = &wy;
+    }
+    p.backend = threads > 1 ? &threaded_cpu_backend(threads) : &cpu_backend();
+    const PdhgResult r = solve_pdhg(lp, p);
+    std::memset(res, 0, sizeof(*res));
+    res->status = static_cast<int32_t>(r.status);
+    res->iterations = r.iterations;
+    res->restarts = r.restarts;
+    res->polish_attempts = r.polish_attempts;
+    res->polish_iterations = r.polish_iterations;
+    res->polished = r.polished ? 1 : 0;
+    res->objective = r.objective;
+    res->solve_seconds = r.solve_seconds;
+    res->matrix_norm_estimate = r.matrix_norm_estimate;
+    res->final_step_size = r.final_step_size;
+    res->final_primal_weight = r.final_primal_weight;
+    res->phase_scaling = r.phases.scaling;
+    res->phase_matrix_norm = r.phases.matrix_norm;
+    res->phase_setup = r.phases.setup;
+    res->phase_iterating = r.phases.iterating;
+    res->phase_checks = r.phases.checks;
+    res->phase_polishing = r.phases.polishing;
+    fill_residual(r.residual, &res->residual);
+    std::strncpy(res->message, r.message.c_str(), sizeof(res->message) - 1);
+    if (x) std::memcpy(x, r.x.data(), r.x.size() * sizeof(double));
+    if (y) std::memcpy(y, r.y.data(), r.y.size() * sizeof(double));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+double ref_estimate_matrix_norm(const void* h, int32_t iterations, double tolerance,
+                                int32_t use_backend) {
+  const StandardLp& lp = static_cast<const RefLp*>(h)->lp;
+  return estimate_matrix_norm(lp, iterations, tolerance,
+                              use_backend ? &cpu_backend() : nullptr);
+}
+
+int ref_evaluate_residual(const void* h, const double* x, const double* y,
+                          anuk_pdhg_residual* out) {
+  const StandardLp& lp = static_cast<const RefLp*>(h)->lp;
+  const std::vector<double> vx(x, x + lp.num_cols());
+  const std::vector<double> vy(y, y + lp.num_rows());
+  fill_residual(evaluate_residual(lp, vx, vy), out);
+  return 0;
+}
+
+// The reference simplex (simplex.hpp:441-457): the FP64 optimum and the final
+// basis. basic (rows entries) receives final_basic when non-null.
+int ref_simplex(const void* h, double* x, double* y, double* objective, int32_t* status,
+                int32_t* basic, int32_t* basic_count) {
+  try {
+    const StandardLp& lp = static_cast<const RefLp*>(h)->lp;
+    SimplexOptions so;
+    const SimplexResult r = solve_lp(lp, so);
+    *status = static_cast<int32_t>(r.status);
+    *objective = r.objective;
+    if (x) std::memcpy(x, r.x.data(), r.x.size() * sizeof(double));
+    if (y) std::memcpy(y, r.y.data(), r.y.size() * sizeof(double));
+    if (basic_count) *basic_count = static_cast<int32_t>(r.final_basic.size());
+    if (basic) std::memcpy(basic, r.final_basic.data(), r.final_basic.size() * sizeof(Int));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// ---- The CPU reference backend (backend.cpp:9-202), host pointers ---------
+
+static const LinAlgBackend& pick(int32_t threads) {
+  return threads > 1 ? threaded_cpu_backend(threads) : cpu_backend();
+}
+
+void ref_op_multiply(int32_t threads, int32_t rows, int32_t cols, const int32_t* start,
+                     const int32_t* index, const double* value, const double* x,
+                     double* y) {
+  const SparseMatrix a = csr_to_matrix(rows, cols, start, index, value);
+  pick(threads).multiply(a, x, y);
+}
+
+void ref_op_transpose(int32_t rows, int32_t cols, const int32_t* start,
+                      const int32_t* index, const double* value, int32_t* t_start,
+                      int32_t* t_index, double* t_value) {
+  const SparseMatrix a = csr_to_matrix(rows, cols, start, index, value);
+  const SparseMatrix t = a.transpose();
+  std::memcpy(t_start, t.start().data(), t.start().size() * sizeof(Int));
+  std::memcpy(t_index, t.index().data(), t.index().size() * sizeof(Int));
+  std::memcpy(t_value, t.value().data(), t.value().size() * sizeof(double));
+}
+
+double ref_op_dot(const double* a, const double* b, int32_t n) {
+  return cpu_backend().dot(a, b, n);
+}
+double ref_op_two_norm(const double* a, int32_t n) { return cpu_backend().two_norm(a, n); }
+double ref_op_inf_norm(const double* a, int32_t n) { return cpu_backend().inf_norm(a, n); }
+
+void ref_op_primal_update(int32_t n, double tau, double gamma, double halpern,
+                          const double* x, const double* c, const double* kt_y,
+                          const double* lower, const double* upper, const double* anchor,
+                          double* x_out, double* dx, double* x_bar) {
+  cpu_backend().primal_update(n, tau, gamma, halpern, x, c, kt_y, lower, upper, anchor,
+                              x_out, dx, x_bar);
+}
+
+void ref_op_dual_update(int32_t m, int32_t neq, double sigma, double gamma, double halpern,
+                        const double* y, const double* q, const double* k_x_bar,
+                        const double* k_x, const double* anchor_y,
+                        const double* anchor_kx, double* y_out, double* dy, double* k_out,
+                        double* k_dx) {
+  cpu_backend().dual_update(m, neq, sigma, gamma, halpern, y, q, k_x_bar, k_x, anchor_y,
+                            anchor_kx, y_out, dy, k_out, k_dx);
+}
+
+void ref_op_accumulate(int32_t n, double w, const double* v, double* sum) {
+  cpu_backend().accumulate(n, w, v, sum);
+}
+void ref_op_scale_into(int32_t n, double w, const double* sum, double* out) {
+  cpu_backend().scale_into(n, w, sum, out);
+}
+double ref_op_weighted_norm_squared(int32_t n, int32_t m, const double* dx,
+                                    const double* dy, double omega) {
+  return cpu_backend().weighted_norm_squared(n, m, dx, dy, omega);
+}
+void ref_op_step_size_terms(int32_t n, int32_t m, const double* dx, const double* dy,
+                            const double* k_dx, double omega, double* interaction,
+                            double* movement) {
+  cpu_backend().step_size_terms(n, m, dx, dy, k_dx, omega, interaction, movement);
+}
+void ref_op_convergence_terms(int32_t n, int32_t m, const anuk_conv_problem* p,
+                              const double* x, const double* k_x, const double* y,
+                              const double* kt_y, anuk_conv_terms* out) {
+  LinAlgBackend::ConvergenceProblem lp;
+  lp.c = p->c;
+  lp.lower = p->lower;
+  lp.upper = p->upper;
+  lp.q = p->q;
+  lp.row_scale = p->row_scale;
+  lp.col_scale = p->col_scale;
+  lp.num_equalities = p->num_equalities;
+  LinAlgBackend::ConvergenceTerms t;
+  cpu_backend().convergence_terms(n, m, lp, x, k_x, y, kt_y, &t);
+  out->primal_sq = t.primal_sq;
+  out->primal_inf = t.primal_inf;
+  out->q_dot_y = t.q_dot_y;
+  out->dual_sq = t.dual_sq;
+  out->dual_inf = t.dual_inf;
+  out->bound_term = t.bound_term;
+  out->c_dot_x = t.c_dot_x;
+}
+
+}  // extern "C"

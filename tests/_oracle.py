@@ -1,0 +1,381 @@
+"""Test-side bindings of the checkers under oracle/ (test infrastructure).
+
+liboracle.so       the plain-C restatement of the reference arithmetic
+libanukulan_ref.so the reference's own CPU path built from its sources
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
+these, and only to check or time against; the product never loads them.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, byref, c_char_p, c_double, c_int, c_int32, c_int64, c_void_p
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libanukulan_ref.so")
+
+DP = POINTER(c_double)
+IP = POINTER(c_int32)
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(IP) if a is not None else None
+
+
+def f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+class ConvProblemC(ctypes.Structure):
+    _fields_ = [("c", DP), ("lower", DP), ("upper", DP), ("q", DP), ("row_scale", DP),
+                ("col_scale", DP), ("num_equalities", c_int32)]
+
+
+class ConvTermsC(ctypes.Structure):
+    _fields_ = [(k, c_double) for k in ("primal_sq", "primal_inf", "q_dot_y", "dual_sq",
+                                        "dual_inf", "bound_term", "c_dot_x")]
+
+
+TERM_NAMES = [k for k, _ in ConvTermsC._fields_]
+
+
+def _load_oracle():
+    if not os.path.exists(ORACLE_PATH):
+        return None
+    L = ctypes.CDLL(ORACLE_PATH)
+    L.orc_from_triplets.restype = c_int64
+    L.orc_from_triplets.argtypes = [c_int32, c_int32, c_int64, IP, IP, DP, IP, IP, DP]
+    L.orc_transpose.argtypes = [c_int32, c_int32, IP, IP, DP, IP, IP, DP]
+    L.orc_spmv.argtypes = [c_int32, IP, IP, DP, DP, DP]
+    for f in ("orc_dot",):
+        getattr(L, f).restype = c_double
+        getattr(L, f).argtypes = [DP, DP, c_int32]
+    for f in ("orc_two_norm", "orc_inf_norm"):
+        getattr(L, f).restype = c_double
+        getattr(L, f).argtypes = [DP, c_int32]
+    L.orc_primal_update.argtypes = [c_int32, c_double, c_double, c_double] + [DP] * 9
+    L.orc_dual_update.argtypes = [c_int32, c_int32, c_double, c_double, c_double] + [DP] * 10
+    L.orc_accumulate.argtypes = [c_int32, c_double, DP, DP]
+    L.orc_scale_into.argtypes = [c_int32, c_double, DP, DP]
+    L.orc_weighted_norm_squared.restype = c_double
+    L.orc_weighted_norm_squared.argtypes = [c_int32, c_int32, DP, DP, c_double]
+    L.orc_step_size_terms.argtypes = [c_int32, c_int32, DP, DP, DP, c_double, DP, DP]
+    L.orc_convergence_terms.argtypes = [c_int32, c_int32, POINTER(ConvProblemC), DP, DP, DP, DP,
+                                        POINTER(ConvTermsC)]
+    L.orc_residual_from_terms.argtypes = [POINTER(ConvTermsC), c_double, c_double, c_double,
+                                          c_double, DP]
+    L.orc_estimate_matrix_norm.restype = c_double
+    L.orc_estimate_matrix_norm.argtypes = [c_int32, c_int32, IP, IP, DP, IP, IP, DP, c_int,
+                                           c_double]
+    L.orc_is_infeasibility_certificate.restype = c_int
+    L.orc_is_infeasibility_certificate.argtypes = [c_int32, c_int32, c_int32, IP, IP, DP, DP, DP,
+                                                   DP, DP, c_double]
+    L.orc_scale_lp.argtypes = [c_int32, c_int32, IP, IP, DP, DP, DP, DP, DP, DP, DP, c_int, c_int]
+    L.orc_to_standard_rows.argtypes = [c_int32, IP, IP, DP, DP, DP, IP, IP, POINTER(c_int64), DP,
+                                       IP, DP, IP, IP, DP]
+    return L
+
+
+oracle = _load_oracle()
+
+
+# ---- plain-C restatement, numpy in / numpy out ---------------------------------
+
+def csr_from_triplets(rows, cols, ri, ci, v):
+    ri, ci, v = i32(ri), i32(ci), f64(v)
+    start = np.zeros(rows + 1, np.int32)
+    index = np.zeros(max(v.size, 1), np.int32)
+    value = np.zeros(max(v.size, 1))
+    nnz = oracle.orc_from_triplets(rows, cols, v.size, _ip(ri), _ip(ci), _dp(v), _ip(start),
+                                   _ip(index), _dp(value))
+    return start, index[:nnz].copy(), value[:nnz].copy()
+
+
+def transpose(rows, cols, start, index, value):
+    nnz = int(start[-1])
+    ts = np.zeros(cols + 1, np.int32)
+    ti = np.zeros(max(nnz, 1), np.int32)
+    tv = np.zeros(max(nnz, 1))
+    oracle.orc_transpose(rows, cols, _ip(i32(start)), _ip(i32(index)), _dp(f64(value)), _ip(ts),
+                         _ip(ti), _dp(tv))
+    return ts, ti[:nnz].copy(), tv[:nnz].copy()
+
+
+def spmv(start, index, value, x):
+    rows = len(start) - 1
+    y = np.zeros(rows)
+    oracle.orc_spmv(rows, _ip(i32(start)), _ip(i32(index)), _dp(f64(value)), _dp(f64(x)), _dp(y))
+    return y
+
+
+def dot(a, b):
+    a, b = f64(a), f64(b)
+    return oracle.orc_dot(_dp(a), _dp(b), a.size)
+
+
+def two_norm(a):
+    a = f64(a)
+    return oracle.orc_two_norm(_dp(a), a.size)
+
+
+def inf_norm(a):
+    a = f64(a)
+    return oracle.orc_inf_norm(_dp(a), a.size)
+
+
+def primal_update(tau, gamma, h, x, c, kty, lo, hi, anchor):
+    x, c, kty, lo, hi = (f64(a) for a in (x, c, kty, lo, hi))
+    n = x.size
+    xo, dx, xb = np.zeros(n), np.zeros(n), np.zeros(n)
+    oracle.orc_primal_update(n, tau, gamma, h, _dp(x), _dp(c), _dp(kty), _dp(lo), _dp(hi),
+                             _dp(f64(anchor)) if anchor is not None else None, _dp(xo), _dp(dx),
+                             _dp(xb))
+    return xo, dx, xb
+
+
+def dual_update(neq, sigma, gamma, h, y, q, kxb, kx, ay, ak):
+    y, q, kxb, kx = (f64(a) for a in (y, q, kxb, kx))
+    m = y.size
+    yo, dy, ko, kd = np.zeros(m), np.zeros(m), np.zeros(m), np.zeros(m)
+    oracle.orc_dual_update(m, neq, sigma, gamma, h, _dp(y), _dp(q), _dp(kxb), _dp(kx),
+                           _dp(f64(ay)) if ay is not None else None,
+                           _dp(f64(ak)) if ak is not None else None, _dp(yo), _dp(dy), _dp(ko),
+                           _dp(kd))
+    return yo, dy, ko, kd
+
+
+def weighted_norm_squared(dx, dy, omega):
+    dx, dy = f64(dx), f64(dy)
+    return oracle.orc_weighted_norm_squared(dx.size, dy.size, _dp(dx), _dp(dy), omega)
+
+
+def step_size_terms(dx, dy, kdx, omega):
+    dx, dy, kdx = f64(dx), f64(dy), f64(kdx)
+    i, mv = c_double(), c_double()
+    oracle.orc_step_size_terms(dx.size, dy.size, _dp(dx), _dp(dy), _dp(kdx), omega, byref(i),
+                               byref(mv))
+    return i.value, mv.value
+
+
+def convergence_terms(c, lo, hi, q, rs, cs, neq, x, kx, y, kty):
+    arrs = [f64(a) for a in (c, lo, hi, q, rs, cs, x, kx, y, kty)]
+    p = ConvProblemC(_dp(arrs[0]), _dp(arrs[1]), _dp(arrs[2]), _dp(arrs[3]), _dp(arrs[4]),
+                     _dp(arrs[5]), neq)
+    t = ConvTermsC()
+    oracle.orc_convergence_terms(arrs[0].size, arrs[3].size, byref(p), _dp(arrs[6]),
+                                 _dp(arrs[7]), _dp(arrs[8]), _dp(arrs[9]), byref(t))
+    return {k: getattr(t, k) for k in TERM_NAMES}
+
+
+def residual_from_terms(terms: dict, q_two, c_two, q_inf, c_inf):
+    t = ConvTermsC(*[terms[k] for k in TERM_NAMES])
+    out = np.zeros(12)
+    oracle.orc_residual_from_terms(byref(t), q_two, c_two, q_inf, c_inf, _dp(out))
+    return out
+
+
+def estimate_matrix_norm(m, n, ks, ki, kv, ts, ti, tv, iterations=200, tol=1e-8):
+    return oracle.orc_estimate_matrix_norm(m, n, _ip(i32(ks)), _ip(i32(ki)), _dp(f64(kv)),
+                                           _ip(i32(ts)), _ip(i32(ti)), _dp(f64(tv)), iterations,
+                                           tol)
+
+
+def is_infeasibility_certificate(neq, ts, ti, tv, q, lo, hi, dy, tol=1e-8):
+    q, lo, hi, dy = f64(q), f64(lo), f64(hi), f64(dy)
+    return bool(oracle.orc_is_infeasibility_certificate(
+        q.size, lo.size, neq, _ip(i32(ts)), _ip(i32(ti)), _dp(f64(tv)), _dp(q), _dp(lo), _dp(hi),
+        _dp(dy), tol))
+
+
+def scale_lp(m, n, start, index, value, q, c, lo, hi, ruiz=10, pc=True):
+    value, q, c, lo, hi = (f64(a).copy() for a in (value, q, c, lo, hi))
+    rs, cs = np.zeros(m), np.zeros(n)
+    oracle.orc_scale_lp(m, n, _ip(i32(start)), _ip(i32(index)), _dp(value), _dp(q), _dp(c),
+                        _dp(lo), _dp(hi), _dp(rs), _dp(cs), ruiz, 1 if pc else 0)
+    return value, q, c, lo, hi, rs, cs
+
+
+def to_standard_form(rows, cols, ri, ci, v, objective, row_lower, row_upper, col_lower,
+                     col_upper, maximize=False):
+    """to_standard_form of a natural model, from the restated pieces."""
+    a_start, a_index, a_value = csr_from_triplets(rows, cols, ri, ci, v)
+    rl, ru = f64(row_lower), f64(row_upper)
+    rl = np.where(rl <= -1e30, -np.inf, rl)
+    ru = np.where(ru >= 1e30, np.inf, ru)
+    srows, neq, count = c_int32(), c_int32(), c_int64()
+    oracle.orc_to_standard_rows(rows, _ip(a_start), _ip(a_index), _dp(a_value), _dp(rl), _dp(ru),
+                                byref(srows), byref(neq), byref(count), None, None, None, None,
+                                None, None)
+    S = srows.value
+    q, rm, rsg = np.zeros(S), np.zeros(S, np.int32), np.zeros(S)
+    tri, tci, tv = (np.zeros(max(count.value, 1), np.int32), np.zeros(max(count.value, 1), np.int32),
+                    np.zeros(max(count.value, 1)))
+    oracle.orc_to_standard_rows(rows, _ip(a_start), _ip(a_index), _dp(a_value), _dp(rl), _dp(ru),
+                                byref(srows), byref(neq), byref(count), _dp(q), _ip(rm), _dp(rsg),
+                                _ip(tri), _ip(tci), _dp(tv))
+    k = count.value
+    start, index, value = csr_from_triplets(S, cols, tri[:k], tci[:k], tv[:k])
+    sc = -1.0 if maximize else 1.0
+    cl, cu = f64(col_lower), f64(col_upper)
+    return {"start": start, "index": index, "value": value, "q": q, "row_model": rm,
+            "row_sign": rsg, "num_equalities": neq.value, "c": sc * f64(objective),
+            "lower": np.where(cl <= -1e30, -np.inf, cl), "upper": np.where(cu >= 1e30, np.inf, cu),
+            "objective_scale": sc}
+
+
+# ---- the reference build ---------------------------------------------------------
+
+def _load_ref():
+    if not os.path.exists(REF_PATH):
+        return None
+    from anukulan_amd import _lib as P  # struct layouts only: the shared header's types
+    L = ctypes.CDLL(REF_PATH)
+    L.ref_last_error.restype = c_char_p
+    L.ref_default_thread_count.restype = c_int
+    L.ref_pdhg_default_options.argtypes = [POINTER(P.PdhgOptions)]
+    L.ref_lp_from_model.restype = c_void_p
+    L.ref_lp_from_model.argtypes = [POINTER(P.Model)]
+    L.ref_lp_from_mps.restype = c_void_p
+    L.ref_lp_from_mps.argtypes = [c_char_p]
+    L.ref_lp_from_standard.restype = c_void_p
+    L.ref_lp_from_standard.argtypes = [c_int32, c_int32, c_int32, IP, IP, DP, DP, DP, DP, DP,
+                                       c_double, c_double]
+    L.ref_lp_free.argtypes = [c_void_p]
+    L.ref_lp_info.argtypes = [c_void_p, POINTER(P.LpInfo)]
+    L.ref_lp_export.argtypes = [c_void_p, IP, IP, DP, DP, DP, DP, DP, IP, DP]
+    L.ref_lp_scale.restype = c_void_p
+    L.ref_lp_scale.argtypes = [c_void_p, c_int32, c_int32, DP, DP]
+    L.ref_solve_pdhg.argtypes = [c_void_p, POINTER(P.PdhgOptions), c_int32, POINTER(P.PdhgResult),
+                                 DP, DP]
+    L.ref_estimate_matrix_norm.restype = c_double
+    L.ref_estimate_matrix_norm.argtypes = [c_void_p, c_int32, c_double, c_int32]
+    L.ref_evaluate_residual.argtypes = [c_void_p, DP, DP, POINTER(P.PdhgResidual)]
+    L.ref_simplex.argtypes = [c_void_p, DP, DP, DP, IP, IP, IP]
+    L.ref_op_multiply.argtypes = [c_int32, c_int32, c_int32, IP, IP, DP, DP, DP]
+    L.ref_op_transpose.argtypes = [c_int32, c_int32, IP, IP, DP, IP, IP, DP]
+    for f in ("ref_op_dot",):
+        getattr(L, f).restype = c_double
+        getattr(L, f).argtypes = [DP, DP, c_int32]
+    for f in ("ref_op_two_norm", "ref_op_inf_norm"):
+        getattr(L, f).restype = c_double
+        getattr(L, f).argtypes = [DP, c_int32]
+    L.ref_op_primal_update.argtypes = [c_int32, c_double, c_double, c_double] + [DP] * 9
+    L.ref_op_dual_update.argtypes = [c_int32, c_int32, c_double, c_double, c_double] + [DP] * 10
+    L.ref_op_accumulate.argtypes = [c_int32, c_double, DP, DP]
+    L.ref_op_scale_into.argtypes = [c_int32, c_double, DP, DP]
+    L.ref_op_weighted_norm_squared.restype = c_double
+    L.ref_op_weighted_norm_squared.argtypes = [c_int32, c_int32, DP, DP, c_double]
+    L.ref_op_step_size_terms.argtypes = [c_int32, c_int32, DP, DP, DP, c_double, DP, DP]
+    L.ref_op_convergence_terms.argtypes = [c_int32, c_int32, POINTER(ConvProblemC), DP, DP, DP,
+                                           DP, POINTER(ConvTermsC)]
+    return L
+
+
+ref = _load_ref()
+
+
+class RefLp:
+    """A StandardLp held by the reference build."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise RuntimeError("reference build: " + ref.ref_last_error().decode())
+        from anukulan_amd import _lib as P
+        self.h = handle
+        info = P.LpInfo()
+        ref.ref_lp_info(self.h, byref(info))
+        self.rows, self.cols, self.nnz = info.rows, info.cols, info.nnz
+        self.num_equalities = info.num_equalities
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref.ref_lp_free(self.h)
+            self.h = None
+
+    @staticmethod
+    def from_natural(nm):
+        from anukulan_amd import _lib as P
+        ri, ci, v = i32(nm.row_index), i32(nm.col_index), f64(nm.value)
+        arrs = [f64(a) for a in (nm.objective, nm.row_lower, nm.row_upper, nm.col_lower,
+                                 nm.col_upper)]
+        m = P.Model(nm.rows, nm.cols, v.size, _ip(ri), _ip(ci), _dp(v), *[_dp(a) for a in arrs],
+                    1 if nm.maximize else 0, nm.objective_offset)
+        return RefLp(ref.ref_lp_from_model(byref(m)))
+
+    @staticmethod
+    def from_mps(text: str):
+        return RefLp(ref.ref_lp_from_mps(text.encode()))
+
+    @staticmethod
+    def from_export(e: dict):
+        rows, cols = len(e["q"]), len(e["c"])
+        return RefLp(ref.ref_lp_from_standard(
+            rows, cols, e["num_equalities"], _ip(i32(e["start"])), _ip(i32(e["index"])),
+            _dp(f64(e["value"])), _dp(f64(e["c"])), _dp(f64(e["q"])), _dp(f64(e["lower"])),
+            _dp(f64(e["upper"])), e.get("objective_scale", 1.0), e.get("objective_offset", 0.0)))
+
+    def export(self):
+        o = {"start": np.zeros(self.rows + 1, np.int32), "index": np.zeros(max(self.nnz, 1), np.int32),
+             "value": np.zeros(max(self.nnz, 1)), "c": np.zeros(self.cols), "q": np.zeros(self.rows),
+             "lower": np.zeros(self.cols), "upper": np.zeros(self.cols),
+             "row_model": np.zeros(self.rows, np.int32), "row_sign": np.zeros(self.rows)}
+        ref.ref_lp_export(self.h, _ip(o["start"]), _ip(o["index"]), _dp(o["value"]), _dp(o["c"]),
+                          _dp(o["q"]), _dp(o["lower"]), _dp(o["upper"]), _ip(o["row_model"]),
+                          _dp(o["row_sign"]))
+        o["index"], o["value"] = o["index"][:self.nnz], o["value"][:self.nnz]
+        o["num_equalities"] = self.num_equalities
+        return o
+
+    def scaled(self, ruiz=10, pc=True):
+        rs, cs = np.zeros(self.rows), np.zeros(self.cols)
+        h = ref.ref_lp_scale(self.h, ruiz, 1 if pc else 0, _dp(rs), _dp(cs))
+        return RefLp(h), rs, cs
+
+    def solve(self, threads=1, warm_x=None, warm_y=None, **overrides):
+        from anukulan_amd import _lib as P
+        from anukulan_amd.api import TERMINATION, result_from
+        o = P.PdhgOptions()
+        ref.ref_pdhg_default_options(byref(o))
+        for k, v in overrides.items():
+            setattr(o, k, TERMINATION[v] if k == "termination" and isinstance(v, str) else v)
+        keep = []
+        if warm_x is not None:
+            wx, wy = f64(warm_x), f64(warm_y)
+            keep += [wx, wy]
+            o.warm_x, o.warm_y = _dp(wx), _dp(wy)
+        x, y = np.zeros(self.cols), np.zeros(self.rows)
+        r = P.PdhgResult()
+        rc = ref.ref_solve_pdhg(self.h, byref(o), threads, byref(r), _dp(x), _dp(y))
+        if rc != 0:
+            raise RuntimeError(ref.ref_last_error().decode())
+        return result_from(r, x, y)
+
+    def matrix_norm(self, iterations=200, tol=1e-8, use_backend=True):
+        return ref.ref_estimate_matrix_norm(self.h, iterations, tol, 1 if use_backend else 0)
+
+    def evaluate_residual(self, x, y):
+        from anukulan_amd import _lib as P
+        r = P.PdhgResidual()
+        ref.ref_evaluate_residual(self.h, _dp(f64(x)), _dp(f64(y)), byref(r))
+        return {k: getattr(r, k) for k, _ in P.PdhgResidual._fields_}
+
+    def simplex(self):
+        x, y = np.zeros(self.cols), np.zeros(self.rows)
+        obj, st = c_double(), c_int32()
+        basic = np.zeros(self.rows + self.cols, np.int32)
+        cnt = c_int32()
+        ref.ref_simplex(self.h, _dp(x), _dp(y), byref(obj), byref(st), _ip(basic), byref(cnt))
+        return {"status": st.value, "objective": obj.value, "x": x, "y": y,
+                "basic": basic[:cnt.value].copy()}

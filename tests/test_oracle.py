@@ -1,0 +1,293 @@
+"""Pins the oracle (oracle/oracle.c) before anything is judged against it.
+
+Each restated function is checked three ways where the domain allows:
+bit-exact against the reference's own build (oracle/_ref, compiled from the
+reference sources), against a closed form or brute force, and against a
+known answer from the reference's tests. A mistake of the plausible kinds
+(dropped term, wrong sign or index, transposed operand) fails at least one.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import _oracle as O
+
+INF = np.inf
+
+
+def random_csr(rows, cols, density, rng, dup=False):
+    nnz = int(rows * cols * density)
+    ri = rng.integers(0, rows, nnz)
+    ci = rng.integers(0, cols, nnz)
+    v = rng.uniform(-3, 3, nnz)
+    if dup:  # a few exact duplicates and an exact cancellation
+        ri = np.concatenate([ri, ri[:5], [0, 0]])
+        ci = np.concatenate([ci, ci[:5], [1, 1]])
+        v = np.concatenate([v, v[:5], [2.5, -2.5]])
+    return ri, ci, v
+
+
+def dense_of(rows, cols, ri, ci, v):
+    d = np.zeros((rows, cols))
+    np.add.at(d, (ri, ci), v)
+    return d
+
+
+def ref_multiply(s, i, v, cols, x):
+    rows = len(s) - 1
+    y = np.zeros(rows)
+    O.ref.ref_op_multiply(1, rows, cols, O._ip(O.i32(s)), O._ip(O.i32(i)), O._dp(O.f64(v)),
+                          O._dp(O.f64(x)), O._dp(y))
+    return y
+
+
+def test_csr_build_transpose_and_product(orc, ref):
+    rng = np.random.default_rng(1)
+    for rows, cols in ((1, 1), (7, 3), (40, 55), (300, 200)):
+        ri, ci, v = random_csr(rows, cols, 0.1, rng, dup=rows > 1)
+        s, i, val = O.csr_from_triplets(rows, cols, ri, ci, v)
+        d = dense_of(rows, cols, ri, ci, v)
+        # structure: sorted unique columns, exact zeros dropped
+        for r in range(rows):
+            cc = i[s[r]:s[r + 1]]
+            assert np.all(np.diff(cc) > 0)
+        assert np.all(val != 0.0)
+        assert np.count_nonzero(d) == len(val)
+        # reference build: identical arrays
+        rl = O.RefLp.from_standard if False else None  # noqa: F841
+        ts, ti, tv = O.transpose(rows, cols, s, i, val)
+        rts, rti, rtv = np.zeros(cols + 1, np.int32), np.zeros(len(i), np.int32), np.zeros(len(i))
+        O.ref.ref_op_transpose(rows, cols, O._ip(O.i32(s)), O._ip(O.i32(i)), O._dp(val),
+                               O._ip(rts), O._ip(rti), O._dp(rtv))
+        assert np.array_equal(ts, rts) and np.array_equal(ti, rti) and np.array_equal(tv, rtv)
+        x = rng.uniform(-2, 2, cols)
+        y = O.spmv(s, i, val, x)
+        assert np.array_equal(y, ref_multiply(s, i, val, cols, x))
+        assert np.allclose(y, d @ x, rtol=0, atol=1e-12)
+        z = rng.uniform(-2, 2, rows)
+        assert np.allclose(O.spmv(ts, ti, tv, z), d.T @ z, rtol=0, atol=1e-12)
+
+
+def test_reductions(orc, ref):
+    rng = np.random.default_rng(2)
+    for n in (0, 1, 5, 1000, 200003):
+        a, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+        assert O.dot(a, b) == O.ref.ref_op_dot(O._dp(a), O._dp(b), n)
+        assert O.two_norm(a) == O.ref.ref_op_two_norm(O._dp(a), n)
+        assert O.inf_norm(a) == O.ref.ref_op_inf_norm(O._dp(a), n)
+        exact = math.fsum(a * b)
+        assert abs(O.dot(a, b) - exact) <= n * 2.3e-16 * float(np.abs(a * b).sum()) + 1e-300
+        assert O.inf_norm(a) == (float(np.max(np.abs(a))) if n else 0.0)
+
+
+def test_fused_updates_are_the_reference(orc, ref):
+    rng = np.random.default_rng(3)
+    n, m, neq = 513, 377, 101
+    x, c, kty = rng.uniform(-2, 2, n), rng.uniform(-2, 2, n), rng.uniform(-2, 2, n)
+    lo = np.where(rng.uniform(size=n) < 0.3, -INF, rng.uniform(-2, 0, n))
+    hi = np.where(rng.uniform(size=n) < 0.3, INF, rng.uniform(0, 2, n))
+    anc = rng.uniform(-1, 1, n)
+    for gamma, h, blend in [(0.0, 1.0, False), (1.0, 1.0, False), (1.0, 0.75, True), (0.5, 0.5, True)]:
+        got = O.primal_update(0.37, gamma, h, x, c, kty, lo, hi, anc if blend else None)
+        want = [np.zeros(n) for _ in range(3)]
+        O.ref.ref_op_primal_update(n, 0.37, gamma, h, *[O._dp(a) for a in (x, c, kty, lo, hi)],
+                                   O._dp(anc) if blend else None, *[O._dp(w) for w in want])
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+        # closed form: projection, then reflection, then blend
+        v = np.minimum(np.maximum(x - 0.37 * (c - kty), lo), hi)
+        cf = v + gamma * (v - x)
+        if blend:
+            cf = h * cf + (1 - h) * anc
+        assert np.allclose(got[0], cf, rtol=0, atol=1e-14)
+        assert np.array_equal(got[1], v - x)
+    y, q, kxb, kx = (rng.uniform(-2, 2, m) for _ in range(4))
+    ay, ak = rng.uniform(-1, 1, m), rng.uniform(-1, 1, m)
+    for blend in (False, True):
+        got = O.dual_update(neq, 0.61, 1.0, 0.8, y, q, kxb, kx, ay if blend else None,
+                            ak if blend else None)
+        want = [np.zeros(m) for _ in range(4)]
+        O.ref.ref_op_dual_update(m, neq, 0.61, 1.0, 0.8, *[O._dp(a) for a in (y, q, kxb, kx)],
+                                 O._dp(ay) if blend else None, O._dp(ak) if blend else None,
+                                 *[O._dp(w) for w in want])
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+        v = y + 0.61 * (q - kxb)
+        v[neq:] = np.maximum(v[neq:], 0.0)
+        assert np.array_equal(got[1], v - y)       # projection split at the equality block
+        assert np.array_equal(got[3], 0.5 * (kxb - kx))
+
+
+def test_kx_identities_hold_for_the_oracle(orc):
+    # test_backend.cpp:408-448 against a real matrix: k_dx = K dx, k_out = K x_pdhg.
+    rng = np.random.default_rng(4)
+    ri, ci, v = random_csr(30, 20, 0.2, rng)
+    s, i, val = O.csr_from_triplets(30, 20, ri, ci, v)
+    x, xp = rng.uniform(-1, 1, 20), rng.uniform(-1, 1, 20)
+    kx, kxb = O.spmv(s, i, val, x), O.spmv(s, i, val, 2 * xp - x)
+    z = np.zeros(30)
+    _, _, ko, kd = O.dual_update(30, 0.0, 0.0, 1.0, z, z, kxb, kx, None, None)
+    assert np.allclose(kd, O.spmv(s, i, val, xp - x), atol=1e-13)
+    assert np.allclose(ko, O.spmv(s, i, val, xp), atol=1e-13)
+
+
+def test_norms_and_step_terms(orc, ref):
+    rng = np.random.default_rng(5)
+    for n, m in ((7, 3), (1400, 517), (517, 1400)):
+        dx, dy, kdx = rng.uniform(-2, 2, n), rng.uniform(-2, 2, m), rng.uniform(-2, 2, m)
+        w = O.weighted_norm_squared(dx, dy, 2.5)
+        assert w == O.ref.ref_op_weighted_norm_squared(n, m, O._dp(dx), O._dp(dy), 2.5)
+        assert math.isclose(w, 2.5 * math.fsum(dx * dx) + math.fsum(dy * dy) / 2.5, rel_tol=1e-12)
+        i1, m1 = O.step_size_terms(dx, dy, kdx, 2.5)
+        ri, rm = O.ctypes_pair() if hasattr(O, "ctypes_pair") else (None, None)
+        from ctypes import byref, c_double
+        a, b = c_double(), c_double()
+        O.ref.ref_op_step_size_terms(n, m, O._dp(dx), O._dp(dy), O._dp(kdx), 2.5, byref(a), byref(b))
+        assert (i1, m1) == (a.value, b.value)
+        assert i1 == abs(O.dot(dy, kdx))
+    s = np.zeros(4)
+    O.oracle.orc_accumulate(4, 2.0, O._dp(np.array([1.0, 2, 3, 4])), O._dp(s))
+    assert np.array_equal(s, [2.0, 4, 6, 8])
+    out = np.zeros(4)
+    O.oracle.orc_scale_into(4, 3.0, O._dp(s), O._dp(out))
+    assert np.array_equal(out, s / 3.0)
+
+
+def conv_case(rng, n=1301, m=907):
+    x, kty, c = rng.uniform(-2, 2, n), rng.uniform(-2, 2, n), rng.uniform(-2, 2, n)
+    cs = 0.25 + 0.5 * (np.arange(n) % 5)
+    cs[7] = 0.0
+    cls = np.arange(n) % 4
+    lo = np.select([cls == 0, cls == 1], [-1.0, -0.5], -INF)
+    hi = np.select([cls == 0, cls == 2], [2.0, 1.5], INF)
+    y, kx, q = rng.uniform(-2, 2, m), rng.uniform(-2, 2, m), rng.uniform(-2, 2, m)
+    rs = 0.5 + 0.25 * (np.arange(m) % 3)
+    rs[11] = 0.0
+    return c, lo, hi, q, rs, cs, x, kx, y, kty
+
+
+def test_convergence_terms(orc, ref):
+    rng = np.random.default_rng(90210)
+    c, lo, hi, q, rs, cs, x, kx, y, kty = conv_case(rng)
+    got = O.convergence_terms(c, lo, hi, q, rs, cs, 401, x, kx, y, kty)
+    p = O.ConvProblemC(*[O._dp(a) for a in (c, lo, hi, q, rs, cs)], 401)
+    t = O.ConvTermsC()
+    from ctypes import byref
+    O.ref.ref_op_convergence_terms(len(c), len(q), byref(p), O._dp(x), O._dp(kx), O._dp(y),
+                                   O._dp(kty), byref(t))
+    for k in O.TERM_NAMES:
+        assert got[k] == getattr(t, k), k
+    # vectorised restatement: every bound class and the zero-scale fall-through
+    row = np.where(rs != 0, kx / np.where(rs != 0, rs, 1), kx)
+    viol = np.concatenate([row[:401] - q[:401], np.minimum(row[401:] - q[401:], 0)])
+    assert math.isclose(got["primal_sq"], math.fsum(viol ** 2), rel_tol=1e-12)
+    assert got["primal_inf"] == np.max(np.abs(viol))
+    prod = np.where(cs != 0, kty / np.where(cs != 0, cs, 1), kty)
+    lam = c - prod
+    hl, hh = lo > -INF, hi < INF
+    left = np.select([hl & hh, hl, hh], [0.0, np.minimum(lam, 0), np.maximum(lam, 0)], lam)
+    assert got["dual_inf"] == np.max(np.abs(left))
+    assert math.isclose(got["dual_sq"], math.fsum(left ** 2), rel_tol=1e-12)
+    assert math.isfinite(got["bound_term"])
+
+
+def test_residual_from_terms_equals_evaluate_residual(orc, ref):
+    # With unit scales the device path and the host residual are the same
+    # arithmetic (backend.hpp:195-197): compare against the reference build's
+    # evaluate_residual on a real LP.
+    from anukulan_amd import generators as G
+    nm = G.refinery_model(2)
+    rlp = O.RefLp.from_natural(nm)
+    e = rlp.export()
+    rng = np.random.default_rng(6)
+    x, y = rng.uniform(0, 5, rlp.cols), rng.uniform(-1, 1, rlp.rows)
+    ts, ti, tv = O.transpose(rlp.rows, rlp.cols, e["start"], e["index"], e["value"])
+    kx = O.spmv(e["start"], e["index"], e["value"], x)
+    kty = O.spmv(ts, ti, tv, y)
+    terms = O.convergence_terms(e["c"], e["lower"], e["upper"], e["q"], np.ones(rlp.rows),
+                                np.ones(rlp.cols), e["num_equalities"], x, kx, y, kty)
+    r = O.residual_from_terms(terms, O.two_norm(e["q"]), O.two_norm(e["c"]), O.inf_norm(e["q"]),
+                              O.inf_norm(e["c"]))
+    want = rlp.evaluate_residual(x, y)
+    names = ["primal_residual", "dual_residual", "primal_residual_inf", "dual_residual_inf",
+             "primal_objective", "dual_objective", "absolute_gap", "relative_primal",
+             "relative_dual", "relative_gap", "relative_primal_inf", "relative_dual_inf"]
+    for k, v in zip(names, r):
+        assert v == want[k], k
+
+
+def test_matrix_norm(orc, ref):
+    from anukulan_amd import generators as G
+    from golden.known_lps import natural
+    # known answer: diag(3, 4) -> 4 (test_pdhg.cpp:206-224)
+    d = natural("norm_diag").to_lp().export()
+    ts, ti, tv = O.transpose(2, 2, d["start"], d["index"], d["value"])
+    assert abs(O.estimate_matrix_norm(2, 2, d["start"], d["index"], d["value"], ts, ti, tv, 500,
+                                      1e-12) - 4.0) <= 1e-6
+    for nm in (G.refinery_model(2), natural("wyndor")):
+        rlp = O.RefLp.from_natural(nm)
+        e = rlp.export()
+        ts, ti, tv = O.transpose(rlp.rows, rlp.cols, e["start"], e["index"], e["value"])
+        mine = O.estimate_matrix_norm(rlp.rows, rlp.cols, e["start"], e["index"], e["value"], ts,
+                                      ti, tv)
+        assert mine == rlp.matrix_norm(200, 1e-8, use_backend=True)
+        assert mine == rlp.matrix_norm(200, 1e-8, use_backend=False)
+        dense = np.zeros((rlp.rows, rlp.cols))
+        for r in range(rlp.rows):
+            dense[r, e["index"][e["start"][r]:e["start"][r + 1]]] = e["value"][e["start"][r]:e["start"][r + 1]]
+        sv = np.linalg.svd(dense, compute_uv=False)[0]
+        assert abs(mine - sv) <= 1e-4 * sv
+
+
+def test_farkas_certificate(orc):
+    # x + y >= 3 with x, y in [0, 1] is infeasible: y = 1 is a ray with
+    # q'y = 3 > max over the box of y'Kx = 2.
+    s, i, v = O.csr_from_triplets(1, 2, [0, 0], [0, 1], [1.0, 1.0])
+    ts, ti, tv = O.transpose(1, 2, s, i, v)
+    q, lo, hi = np.array([3.0]), np.zeros(2), np.ones(2)
+    assert O.is_infeasibility_certificate(0, ts, ti, tv, q, lo, hi, np.array([1.0]))
+    assert not O.is_infeasibility_certificate(0, ts, ti, tv, q, lo, hi, np.array([-1.0]))
+    assert O.is_infeasibility_certificate(1, ts, ti, tv, q, lo, hi, np.array([1.0]))  # equality row
+    assert not O.is_infeasibility_certificate(0, ts, ti, tv, q, lo, hi, np.array([0.0]))
+    # the same ray against a box that reaches 3 is no certificate
+    assert not O.is_infeasibility_certificate(0, ts, ti, tv, q, lo, np.array([2.0, 2.0]),
+                                              np.array([1.0]))
+    # an infinite bound in the ray's direction voids it
+    assert not O.is_infeasibility_certificate(0, ts, ti, tv, q, lo, np.array([1.0, INF]),
+                                              np.array([1.0]))
+
+
+def test_scaling(orc, ref):
+    from anukulan_amd import generators as G
+    nm = G.refinery_model(3)
+    rlp = O.RefLp.from_natural(nm)
+    e = rlp.export()
+    for ruiz, pc in ((10, True), (3, False), (0, True)):
+        got = O.scale_lp(rlp.rows, rlp.cols, e["start"], e["index"], e["value"], e["q"], e["c"],
+                         e["lower"], e["upper"], ruiz, pc)
+        srl, rs, cs = rlp.scaled(ruiz, pc)
+        se = srl.export()
+        assert np.array_equal(got[0], se["value"])
+        assert np.array_equal(got[1], se["q"]) and np.array_equal(got[2], se["c"])
+        assert np.array_equal(got[3], se["lower"]) and np.array_equal(got[4], se["upper"])
+        assert np.array_equal(got[5], rs) and np.array_equal(got[6], cs)
+        # invariant (test_scaling.cpp:112-138): K~ = D1 K D2, q~ = D1 q, c~ = D2 c
+        rows_of = np.repeat(np.arange(rlp.rows), np.diff(e["start"]))
+        assert np.allclose(got[0], rs[rows_of] * e["value"] * cs[e["index"]], rtol=1e-13, atol=0)
+        assert np.allclose(got[1], rs * e["q"], rtol=1e-13, atol=0)
+        assert np.allclose(got[2], cs * e["c"], rtol=1e-13, atol=0)
+
+
+def test_standard_form(orc, ref):
+    from golden.known_lps import KNOWN, natural
+    from anukulan_amd import generators as G
+    cases = [natural(k) for k in KNOWN] + [G.refinery_model(2)]
+    for nm in cases:
+        got = O.to_standard_form(nm.rows, nm.cols, nm.row_index, nm.col_index, nm.value,
+                                 nm.objective, nm.row_lower, nm.row_upper, nm.col_lower,
+                                 nm.col_upper, nm.maximize)
+        want = O.RefLp.from_natural(nm).export()
+        for k in ("start", "index", "value", "q", "row_model", "row_sign", "c", "lower", "upper"):
+            assert np.array_equal(got[k], want[k]), (nm.name, k)
+        assert got["num_equalities"] == want["num_equalities"]

@@ -216,8 +216,8 @@ class Context:
 
     def convergence_terms(self, n, m, c, lower, upper, q, row_scale, col_scale, neq, x, k_x, y,
                           kt_y) -> dict:
-        p = _lib.ConvProblem(c.ptr, lower.ptr, upper.ptr, q.ptr, row_scale.ptr, col_scale.ptr,
-                             int(neq))
+        dev = [ctypes.cast(v.ptr, DP) for v in (c, lower, upper, q, row_scale, col_scale)]
+        p = _lib.ConvProblem(*dev, int(neq))
         t = _lib.ConvTerms()
         self._call(lib.anuk_convergence_terms, int(n), int(m), byref(p), x.ptr, k_x.ptr, y.ptr,
                    kt_y.ptr, byref(t))

@@ -34,34 +34,58 @@ HipBackend::HipBackend(int device) : device_(device) {
 }
 
 HipBackend::~HipBackend() {
-  hipSetDevice(device_);
-  if (stream_) hipStreamSynchronize(stream_);
+  // Teardown: failures here have nowhere to go, so their statuses are dropped.
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  clear_profile();
+  for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
   for (auto& kv : matrices_) free_matrix(&kv.second);
-  for (auto& kv : scratch_) hipFree(kv.second.first);
-  if (result_dev_) hipFree(result_dev_);
-  if (result_host_) hipHostFree(result_host_);
-  if (stop_event_) hipEventDestroy(stop_event_);
-  if (stream_) hipStreamDestroy(stream_);
+  for (auto& kv : scratch_) (void)hipFree(kv.second.first);
+  if (result_dev_) (void)hipFree(result_dev_);
+  if (result_host_) (void)hipHostFree(result_host_);
+  if (stop_event_) (void)hipEventDestroy(stop_event_);
+  if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 void HipBackend::synchronize() { hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
 
+hipEvent_t HipBackend::take_event() {
+  if (!event_pool_.empty()) {
+    hipEvent_t e = event_pool_.back();
+    event_pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hip_check(hipEventCreate(&e), "hipEventCreate");
+  return e;
+}
+
+void HipBackend::clear_profile() {
+  (void)hipStreamSynchronize(stream_);
+  for (auto& kv : profile_) {
+    for (auto& p : kv.second.pairs) {
+      event_pool_.push_back(p.first);
+      event_pool_.push_back(p.second);
+    }
+  }
+  profile_.clear();
+}
+
 HipBackend::Timed::Timed(HipBackend& b, const char* name) : b_(b), name_(name) {
   if (!b_.profiling_) return;
-  hip_check(hipEventCreate(&start_), "hipEventCreate");
+  auto& e = b_.profile_[name_];
+  ++e.calls;
+  if (static_cast<int>(e.pairs.size()) >= kMaxTimed) return;
+  start_ = b_.take_event();
   hip_check(hipEventRecord(start_, b_.stream_), "hipEventRecord");
+  active_ = true;
 }
 
 HipBackend::Timed::~Timed() {
-  if (start_ == nullptr) return;
-  hipEventRecord(b_.stop_event_, b_.stream_);
-  hipEventSynchronize(b_.stop_event_);
-  float ms = 0.0f;
-  hipEventElapsedTime(&ms, start_, b_.stop_event_);
-  hipEventDestroy(start_);
-  auto& e = b_.profile_[name_];
-  e.ms += static_cast<double>(ms);
-  e.calls += 1;
+  if (!active_) return;
+  hipEvent_t stop = b_.take_event();
+  (void)hipEventRecord(stop, b_.stream_);
+  b_.profile_[name_].pairs.emplace_back(start_, stop);
 }
 
 // ---- memory ----------------------------------------------------------------
@@ -117,7 +141,7 @@ double* HipBackend::scratch(int slot, std::size_t doubles) {
   if (it != scratch_.end() && it->second.second >= doubles) return it->second.first;
   if (it != scratch_.end()) {
     synchronize();
-    hipFree(it->second.first);
+    (void)hipFree(it->second.first);
     scratch_.erase(it);
   }
   double* p = nullptr;
@@ -138,7 +162,7 @@ const double* HipBackend::fetch_result(int n) {
 // ---- matrices ----------------------------------------------------------------
 
 void HipBackend::free_matrix(DeviceMatrix* dm) {
-  for (void* p : dm->owned) hipFree(p);
+  for (void* p : dm->owned) (void)hipFree(p);
   dm->owned.clear();
 }
 
@@ -281,23 +305,42 @@ Terms HipBackend::convergence_terms(Int n, Int m, const DevProblem& p, const dou
 // ---- profiling -------------------------------------------------------------
 
 void HipBackend::set_profiling(bool on) {
+  if (on) clear_profile();
   profiling_ = on;
-  if (on) profile_.clear();
 }
 
+// One line per launch group: calls, timed calls, mean microseconds of the
+// timed ones, and the estimated total (mean x calls) with its share.
 std::string HipBackend::profile_report() const {
   if (profile_.empty()) return {};
-  std::vector<std::pair<std::string, ProfileEntry>> rows(profile_.begin(), profile_.end());
-  std::sort(rows.begin(), rows.end(),
-            [](const auto& a, const auto& b) { return a.second.ms > b.second.ms; });
+  (void)hipStreamSynchronize(stream_);
+  struct Row {
+    std::string name;
+    long calls;
+    long timed;
+    double mean_us;
+    double total_ms;
+  };
+  std::vector<Row> rows;
   double total = 0.0;
-  for (const auto& r : rows) total += r.second.ms;
-  std::string out = "  kernel                  calls      total ms    mean us   share\n";
-  char line[160];
+  for (const auto& kv : profile_) {
+    double ms = 0.0;
+    for (const auto& p : kv.second.pairs) {
+      float t = 0.0f;
+      if (hipEventElapsedTime(&t, p.first, p.second) == hipSuccess) ms += static_cast<double>(t);
+    }
+    const long timed = static_cast<long>(kv.second.pairs.size());
+    const double mean_us = timed > 0 ? 1e3 * ms / static_cast<double>(timed) : 0.0;
+    const double est = 1e-3 * mean_us * static_cast<double>(kv.second.calls);
+    rows.push_back({kv.first, kv.second.calls, timed, mean_us, est});
+    total += est;
+  }
+  std::sort(rows.begin(), rows.end(), [](const Row& a, const Row& b) { return a.total_ms > b.total_ms; });
+  std::string out = "  kernel                    calls    timed      mean us     total ms   share\n";
+  char line[200];
   for (const auto& r : rows) {
-    std::snprintf(line, sizeof(line), "  %-22s %7ld %12.3f %10.2f %6.1f%%\n", r.first.c_str(),
-                  r.second.calls, r.second.ms, 1e3 * r.second.ms / static_cast<double>(r.second.calls),
-                  total > 0 ? 100.0 * r.second.ms / total : 0.0);
+    std::snprintf(line, sizeof(line), "  %-22s %8ld %8ld %12.3f %12.3f %6.1f%%\n", r.name.c_str(),
+                  r.calls, r.timed, r.mean_us, r.total_ms, total > 0 ? 100.0 * r.total_ms / total : 0.0);
     out += line;
   }
   return out;

@@ -110,6 +110,7 @@ class HipBackend {
     HipBackend& b_;
     const char* name_;
     hipEvent_t start_ = nullptr;
+    bool active_ = false;
   };
 
  private:
@@ -126,11 +127,18 @@ class HipBackend {
   double* result_dev_ = nullptr;
   double* result_host_ = nullptr;
   bool profiling_ = false;
+  // Event pairs recorded around launches without waiting; their times are
+  // read when the report is asked for. At most kMaxTimed pairs per name are
+  // kept, which bounds the event count on long solves.
+  static constexpr int kMaxTimed = 4096;
   struct ProfileEntry {
-    double ms = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
     long calls = 0;
   };
-  std::map<std::string, ProfileEntry> profile_;
+  mutable std::map<std::string, ProfileEntry> profile_;
+  std::vector<hipEvent_t> event_pool_;
+  hipEvent_t take_event();
+  void clear_profile();
   hipEvent_t stop_event_ = nullptr;
 };
 

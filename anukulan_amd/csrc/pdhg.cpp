@@ -172,15 +172,16 @@ class Solver {
 };
 
 Solver::~Solver() {
+  // Teardown statuses are dropped: there is no caller left to report to.
   for (auto& kv : graphs_) {
-    hipGraphExecDestroy(kv.second.second);
-    hipGraphDestroy(kv.second.first);
+    (void)hipGraphExecDestroy(kv.second.second);
+    (void)hipGraphDestroy(kv.second.first);
   }
   if (slab_ != nullptr) {
-    hipStreamSynchronize(be_.stream());
-    hipFree(slab_);
+    (void)hipStreamSynchronize(be_.stream());
+    (void)hipFree(slab_);
   }
-  if (h_pinned_ != nullptr) hipHostFree(h_pinned_);
+  if (h_pinned_ != nullptr) (void)hipHostFree(h_pinned_);
   delete scales_;
   --g_depth;
   if (outermost_) {
@@ -407,12 +408,19 @@ void Solver::enqueue_steps(int steps, bool has_check, int parity) {
   hipStream_t s = be_.stream();
   int p = parity;
   for (int step = 0; step < steps; ++step) {
-    const unsigned flags = (has_check && step == steps - 1) ? kStepDiag : 0u;
+    const bool diag = has_check && step == steps - 1;
+    const unsigned flags = diag ? kStepDiag : 0u;
     PrimalBufs pb{xb_[p], anchor_x_, xb_[1 - p], x_bar_, dx_, anchor_x_};
-    launch_primal_step(s, dkt_, yb_[p], prob_, d_params_, step, pb, flags, part_a_);
+    {
+      HipBackend::Timed t(be_, diag ? "primal_step_check" : "primal_step");
+      launch_primal_step(s, dkt_, yb_[p], prob_, d_params_, step, pb, flags, part_a_);
+    }
     DualBufs db{yb_[p], kxb_[p], anchor_y_, anchor_kx_, yb_[1 - p], kxb_[1 - p],
                 dy_, k_dx_, anchor_y_, anchor_kx_};
-    launch_dual_step(s, dk_, x_bar_, prob_, d_params_, step, db, flags, part_b_);
+    {
+      HipBackend::Timed t(be_, diag ? "dual_step_check" : "dual_step");
+      launch_dual_step(s, dk_, x_bar_, prob_, d_params_, step, db, flags, part_b_);
+    }
     p = 1 - p;
   }
   if (has_check) enqueue_check(p);
@@ -420,6 +428,7 @@ void Solver::enqueue_steps(int steps, bool has_check, int parity) {
 
 void Solver::enqueue_check(int p) {
   hipStream_t s = be_.stream();
+  HipBackend::Timed t(be_, "check_reductions");
   launch_check_fold_rows(s, dkt_.blocks, part_a_, dk_.blocks, part_b_, check_);
   launch_conv_product(s, dkt_, yb_[p], xb_[p], prob_, part_c_);
   const bool farkas = opt_.detect_infeasibility != 0;
@@ -440,8 +449,8 @@ hipGraphExec_t Solver::graph_for(int parity) {
   try {
     enqueue_steps(opt_.termination_check_frequency, true, parity);
   } catch (...) {
-    hipStreamEndCapture(s, &g);
-    if (g) hipGraphDestroy(g);
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
     throw;
   }
   hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
@@ -468,7 +477,6 @@ void Solver::run_period(int steps, bool has_check) {
     hip_check(hipGraphLaunch(graph_for(cur_), be_.stream()), "hipGraphLaunch");
     ++res_.graph_launches;
   } else {
-    HipBackend::Timed t(be_, "fused period");
     enqueue_steps(steps, has_check, cur_);
     ++res_.direct_periods;
   }

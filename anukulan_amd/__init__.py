@@ -5,8 +5,8 @@ host side behind the C ABI in include/anukulan_hip.h); this package loads it
 and exposes it to Python. Importing raises if the library was not built.
 """
 from .api import (LP, AnukError, Context, DeviceVector, Matrix, SolveResult, default_options,
-                  estimate_matrix_norm, solve_pdhg)
+                  estimate_matrix_norm, solve_pdhg, solve_pdhg_batch)
 from ._lib import LIB_PATH
 
 __all__ = ["LP", "AnukError", "Context", "DeviceVector", "Matrix", "SolveResult",
-           "default_options", "estimate_matrix_norm", "solve_pdhg", "LIB_PATH"]
+           "default_options", "estimate_matrix_norm", "solve_pdhg", "solve_pdhg_batch", "LIB_PATH"]

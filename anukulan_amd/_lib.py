@@ -129,6 +129,8 @@ SIGNATURES = {
     "anuk_lp_scale": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_void_p), DP, DP]),
     "anuk_pdhg_default_options": (None, [POINTER(PdhgOptions)]),
     "anuk_solve_pdhg": (c_int, [c_void_p, c_void_p, POINTER(PdhgOptions), POINTER(PdhgResult), DP, DP]),
+    "anuk_solve_pdhg_batch": (c_int, [c_void_p, POINTER(c_void_p), c_int32, POINTER(PdhgOptions),
+                                      POINTER(PdhgResult), DP, DP]),
     "anuk_estimate_matrix_norm": (c_int, [c_void_p, c_void_p, c_int32, c_double, DP]),
     "anuk_evaluate_residual": (c_int, [c_void_p, DP, DP, POINTER(PdhgResidual)]),
 }

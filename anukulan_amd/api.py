@@ -374,6 +374,26 @@ def solve_pdhg(ctx: Context, lp: LP, options: Optional[_lib.PdhgOptions] = None,
     return result_from(r, x, y)
 
 
+def solve_pdhg_batch(ctx: Context, lps, options: Optional[_lib.PdhgOptions] = None,
+                     **overrides) -> list:
+    """Solves LPs that share one constraint matrix together; result i equals
+    solve_pdhg(ctx, lps[i], ...) bit for bit."""
+    lps = list(lps)
+    if not lps:
+        return []
+    o = options if options is not None else default_options(**overrides)
+    if options is not None and overrides:
+        for k, v in overrides.items():
+            setattr(o, k, TERMINATION[v] if k == "termination" and isinstance(v, str) else v)
+    count = len(lps)
+    handles = (c_void_p * count)(*[lp.h for lp in lps])
+    results = (_lib.PdhgResult * count)()
+    n, m = lps[0].cols, lps[0].rows
+    xs, ys = np.empty((count, n)), np.empty((count, m))
+    ctx._call(lib.anuk_solve_pdhg_batch, handles, count, byref(o), results, _dp(xs), _dp(ys))
+    return [result_from(results[i], xs[i].copy(), ys[i].copy()) for i in range(count)]
+
+
 def estimate_matrix_norm(ctx: Context, lp: LP, iterations=200, tolerance=1e-8) -> float:
     out = c_double()
     ctx._call(lib.anuk_estimate_matrix_norm, lp.h, int(iterations), float(tolerance), byref(out))

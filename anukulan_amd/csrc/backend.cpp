@@ -71,6 +71,19 @@ void HipBackend::clear_profile() {
   profile_.clear();
 }
 
+LaunchEvents HipBackend::launch_events(const char* name, double bytes) {
+  LaunchEvents ev;
+  if (!profiling_) return ev;
+  auto& e = profile_[name];
+  ++e.calls;
+  if (static_cast<int>(e.pairs.size()) >= kMaxTimed) return ev;
+  ev.start = take_event();
+  ev.stop = take_event();
+  e.pairs.emplace_back(ev.start, ev.stop);
+  e.bytes += bytes;
+  return ev;
+}
+
 HipBackend::Timed::Timed(HipBackend& b, const char* name) : b_(b), name_(name) {
   if (!b_.profiling_) return;
   auto& e = b_.profile_[name_];
@@ -320,6 +333,7 @@ std::string HipBackend::profile_report() const {
     long timed;
     double mean_us;
     double total_ms;
+    double gbs;  // algorithmic bytes of the timed launches over their time
   };
   std::vector<Row> rows;
   double total = 0.0;
@@ -332,15 +346,18 @@ std::string HipBackend::profile_report() const {
     const long timed = static_cast<long>(kv.second.pairs.size());
     const double mean_us = timed > 0 ? 1e3 * ms / static_cast<double>(timed) : 0.0;
     const double est = 1e-3 * mean_us * static_cast<double>(kv.second.calls);
-    rows.push_back({kv.first, kv.second.calls, timed, mean_us, est});
+    const double gbs = ms > 0.0 ? kv.second.bytes / (1e-3 * ms) / 1e9 : 0.0;
+    rows.push_back({kv.first, kv.second.calls, timed, mean_us, est, gbs});
     total += est;
   }
   std::sort(rows.begin(), rows.end(), [](const Row& a, const Row& b) { return a.total_ms > b.total_ms; });
-  std::string out = "  kernel                    calls    timed      mean us     total ms   share\n";
-  char line[200];
+  std::string out =
+      "  kernel                    calls    timed      mean us     total ms   share      GB/s\n";
+  char line[220];
   for (const auto& r : rows) {
-    std::snprintf(line, sizeof(line), "  %-22s %8ld %8ld %12.3f %12.3f %6.1f%%\n", r.name.c_str(),
-                  r.calls, r.timed, r.mean_us, r.total_ms, total > 0 ? 100.0 * r.total_ms / total : 0.0);
+    std::snprintf(line, sizeof(line), "  %-22s %8ld %8ld %12.3f %12.3f %6.1f%% %9.1f\n",
+                  r.name.c_str(), r.calls, r.timed, r.mean_us, r.total_ms,
+                  total > 0 ? 100.0 * r.total_ms / total : 0.0, r.gbs);
     out += line;
   }
   return out;

@@ -98,6 +98,11 @@ class HipBackend {
   // Copies n doubles of the result vector to the pinned mirror and waits.
   const double* fetch_result(int n);
 
+  // Events for one kernel launch, recorded by the dispatch itself and filed
+  // under `name` in the profile, with the launch's algorithmic HBM bytes
+  // (its minimum traffic); null events when profiling is off.
+  LaunchEvents launch_events(const char* name, double bytes = 0.0);
+
   // Brackets a launch sequence for the profiler; no-ops when profiling is off.
   class Timed {
    public:
@@ -134,6 +139,7 @@ class HipBackend {
   struct ProfileEntry {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
     long calls = 0;
+    double bytes = 0.0;  // algorithmic bytes of the timed launches
   };
   mutable std::map<std::string, ProfileEntry> profile_;
   std::vector<hipEvent_t> event_pool_;

@@ -1,0 +1,814 @@
+#include "batch.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+
+#include "batch_kernels.hpp"
+#include "control.hpp"
+
+namespace anuk {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+using Mask = unsigned long long;
+
+double seconds_since(Clock::time_point t) {
+  return std::chrono::duration<double>(Clock::now() - t).count();
+}
+
+// One LP of a batch in the solver's working space.
+struct Lane {
+  std::vector<double> c, q, lower, upper;  // iteration data
+  std::vector<double> oc, oq, olo, ohi;    // data the convergence test measures
+  std::vector<double> warm_x, warm_y;      // empty: cold start
+  double initial_omega = 0.0;              // 0: derive from ||c|| / ||q||
+  double q_two = 0, c_two = 0, q_inf = 0, c_inf = 0;  // ResidualScales of (oq, oc)
+  void set_scales() {
+    q_two = host_two_norm(oq);
+    c_two = host_two_norm(oc);
+    q_inf = host_inf_norm(oq);
+    c_inf = host_inf_norm(oc);
+  }
+};
+
+struct LaneOutcome {
+  int status = ANUK_PDHG_ITERATION_LIMIT;
+  Int iterations = 0;
+  Int restarts = 0;
+  double eta = 0.0, omega = 0.0;
+  double seconds = 0.0;
+  std::string message;
+  std::vector<double> x, y;  // working-space iterate, unless polished
+  bool polished = false;     // x, y are the unscaled polished pair
+  Residual polished_r;
+  Int polish_attempts = 0, polish_iterations = 0;
+};
+
+// What a polishing attempt needs from the enclosing batch; absent in
+// sub-batches, which never polish.
+struct PolishContext {
+  const std::vector<const StandardLp*>* originals;  // per lane of this engine
+  const Scaling* scaling;
+};
+
+class Engine {
+ public:
+  Engine(HipBackend& be, const Csr& k, const Csr& kt, Int neq, std::vector<double> row_scale,
+         std::vector<double> col_scale, double matrix_norm, const anuk_pdhg_options& opt,
+         std::vector<Lane>& lanes, Clock::time_point start, const PolishContext* polish)
+      : be_(be), k_(k), kt_(kt), neq_(neq), row_scale_(std::move(row_scale)),
+        col_scale_(std::move(col_scale)), norm_(matrix_norm), opt_(opt), lanes_(lanes),
+        start_(start), polish_(polish) {}
+  ~Engine();
+  std::vector<LaneOutcome> run();
+
+ private:
+  void allocate();
+  void upload_lanes();
+  void upload_params();
+  void enqueue_period(int steps, bool has_check, int parity);
+  hipGraphExec_t graph_for(int parity);
+  void fetch(const double* dev, double* host, size_t doubles);
+  void retire_extract(Mask mask, std::vector<LaneOutcome>& out);
+  void run_restarts(Mask mask);
+  void run_polish(const std::vector<int>& who, Int budget, std::vector<LaneOutcome>& out,
+                  std::vector<bool>* stopped, Int iteration);
+
+  HipBackend& be_;
+  const Csr& k_;
+  const Csr& kt_;
+  Int neq_;
+  std::vector<double> row_scale_, col_scale_;
+  double norm_;
+  const anuk_pdhg_options& opt_;
+  std::vector<Lane>& lanes_;
+  Clock::time_point start_;
+  const PolishContext* polish_;
+
+  Int n_ = 0, m_ = 0;
+  int count_ = 0;
+  DevCsr dk_, dkt_;
+  std::vector<Control> ctl_;
+  Mask active_ = 0;
+  int cur_ = 0;
+
+  double* slab_ = nullptr;
+  double *X_[2], *Y_[2], *KX_[2];
+  double *XBAR_, *DX_, *AX_, *KTY_, *LAM_, *RX_, *PRX_;
+  double *DY_, *KDX_, *AY_, *AKX_, *RY_, *PRY_;
+  double *EXT_, *CHECK_, *RESTART_, *CHECKP_;
+  LaneParams* d_params_ = nullptr;
+  BatchData data_;
+  double* pinned_ = nullptr;  // params + check + restart staging
+  std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
+};
+
+Engine::~Engine() {
+  for (auto& g : graphs_) {
+    (void)hipGraphExecDestroy(g.second.second);
+    (void)hipGraphDestroy(g.second.first);
+  }
+  if (slab_) {
+    (void)hipStreamSynchronize(be_.stream());
+    (void)hipFree(slab_);
+  }
+  if (pinned_) (void)hipHostFree(pinned_);
+}
+
+constexpr size_t kPinnedDoubles = 64 * 8 + 64 * kLaneCheckStride + 128;
+
+void Engine::allocate() {
+  const size_t N = static_cast<size_t>(n_) * kLanes, M = static_cast<size_t>(m_) * kLanes;
+  auto round = [](size_t d) { return (std::max<size_t>(d, 1) + 31) / 32 * 32; };
+  const size_t ext = static_cast<size_t>(std::max(n_, m_)) * kLanes;
+  const size_t sizes[] = {
+      N, N, N, N, N, N, N, N, N, N,            // X0 X1 XBAR DX AX KTY LAM RX PRX c
+      N, N, N, N, N,                           // lower upper oc olo ohi
+      M, M, M, M, M, M, M, M, M, M, M, M,      // Y0 Y1 KX0 KX1 DY KDX AY AKX RY PRY q oq
+      static_cast<size_t>(n_), static_cast<size_t>(m_),  // col_scale row_scale
+      ext, 64 * kLaneCheckStride, 128, 64 * 8,           // EXT CHECK RESTART params
+      batch_check_scratch_doubles()};                    // check partials
+  size_t total = 0;
+  for (size_t s : sizes) total += round(s);
+  hip_check(hipMalloc(&slab_, total * sizeof(double)), "hipMalloc(batch state)");
+  hip_check(hipMemsetAsync(slab_, 0, total * sizeof(double), be_.stream()), "hipMemsetAsync(batch)");
+  double* p = slab_;
+  size_t i = 0;
+  auto take = [&]() {
+    double* q = p;
+    p += round(sizes[i++]);
+    return q;
+  };
+  X_[0] = take(); X_[1] = take(); XBAR_ = take(); DX_ = take(); AX_ = take(); KTY_ = take();
+  LAM_ = take(); RX_ = take(); PRX_ = take();
+  double* c = take(); double* lo = take(); double* hi = take();
+  double* oc = take(); double* olo = take(); double* ohi = take();
+  Y_[0] = take(); Y_[1] = take(); KX_[0] = take(); KX_[1] = take(); DY_ = take(); KDX_ = take();
+  AY_ = take(); AKX_ = take(); RY_ = take(); PRY_ = take();
+  double* q = take(); double* oq = take();
+  double* cs = take(); double* rs = take();
+  EXT_ = take(); CHECK_ = take(); RESTART_ = take();
+  d_params_ = reinterpret_cast<LaneParams*>(take());
+  CHECKP_ = take();
+  data_.n = n_;
+  data_.m = m_;
+  data_.num_equalities = neq_;
+  data_.c = c; data_.lower = lo; data_.upper = hi; data_.q = q;
+  data_.orig_c = oc; data_.orig_lower = olo; data_.orig_upper = ohi; data_.orig_q = oq;
+  data_.row_scale = rs; data_.col_scale = cs;
+  hip_check(hipHostMalloc(&pinned_, kPinnedDoubles * sizeof(double), hipHostMallocDefault),
+            "hipHostMalloc(batch)");
+}
+
+// Row-interleaves one vector per lane (rows entries each) and uploads it.
+void upload_interleaved(HipBackend& be, double* dst, Int rows, int count,
+                        const std::vector<const std::vector<double>*>& cols) {
+  std::vector<double> buf(static_cast<size_t>(rows) * kLanes, 0.0);
+  for (int l = 0; l < count; ++l) {
+    const std::vector<double>& v = *cols[static_cast<size_t>(l)];
+    for (Int r = 0; r < rows; ++r) buf[static_cast<size_t>(r) * kLanes + l] = v[uz(r)];
+  }
+  be.upload(buf.data(), dst, rows * kLanes);
+}
+
+void Engine::upload_lanes() {
+  auto gather = [&](std::vector<double> Lane::*f) {
+    std::vector<const std::vector<double>*> v;
+    for (Lane& l : lanes_) v.push_back(&(l.*f));
+    return v;
+  };
+  upload_interleaved(be_, const_cast<double*>(data_.c), n_, count_, gather(&Lane::c));
+  upload_interleaved(be_, const_cast<double*>(data_.lower), n_, count_, gather(&Lane::lower));
+  upload_interleaved(be_, const_cast<double*>(data_.upper), n_, count_, gather(&Lane::upper));
+  upload_interleaved(be_, const_cast<double*>(data_.orig_c), n_, count_, gather(&Lane::oc));
+  upload_interleaved(be_, const_cast<double*>(data_.orig_lower), n_, count_, gather(&Lane::olo));
+  upload_interleaved(be_, const_cast<double*>(data_.orig_upper), n_, count_, gather(&Lane::ohi));
+  upload_interleaved(be_, const_cast<double*>(data_.q), m_, count_, gather(&Lane::q));
+  upload_interleaved(be_, const_cast<double*>(data_.orig_q), m_, count_, gather(&Lane::oq));
+  be_.upload(col_scale_.data(), const_cast<double*>(data_.col_scale), n_);
+  be_.upload(row_scale_.data(), const_cast<double*>(data_.row_scale), m_);
+
+  // Starting points: a warm start is put in the dual cone and the primal box,
+  // a cold one is the projection of zero (pdhg.cpp:479-503).
+  std::vector<std::vector<double>> xs(static_cast<size_t>(count_)), ys(static_cast<size_t>(count_));
+  std::vector<const std::vector<double>*> px, py;
+  for (int l = 0; l < count_; ++l) {
+    Lane& ln = lanes_[static_cast<size_t>(l)];
+    const bool warm = !ln.warm_x.empty() && !ln.warm_y.empty();
+    std::vector<double>& x = xs[static_cast<size_t>(l)];
+    std::vector<double>& y = ys[static_cast<size_t>(l)];
+    x.assign(uz(n_), 0.0);
+    y.assign(uz(m_), 0.0);
+    if (warm) {
+      x = ln.warm_x;
+      y = ln.warm_y;
+      for (Int i = neq_; i < m_; ++i)
+        if (!(y[uz(i)] > 0.0)) y[uz(i)] = 0.0;
+    }
+    for (Int j = 0; j < n_; ++j) {
+      const double lo = ln.lower[uz(j)], hi = ln.upper[uz(j)];
+      double v = warm ? x[uz(j)] : 0.0;
+      if (!std::isfinite(v)) v = 0.0;
+      if (v < lo) v = std::isfinite(lo) ? lo : 0.0;
+      if (v > hi) v = std::isfinite(hi) ? hi : 0.0;
+      x[uz(j)] = v;
+    }
+    px.push_back(&x);
+    py.push_back(&y);
+  }
+  upload_interleaved(be_, X_[0], n_, count_, px);
+  upload_interleaved(be_, Y_[0], m_, count_, py);
+  // The first restart point (and the one before it) is the starting point.
+  be_.copy(X_[0], RX_, n_ * kLanes);
+  be_.copy(X_[0], PRX_, n_ * kLanes);
+  be_.copy(Y_[0], RY_, m_ * kLanes);
+  be_.copy(Y_[0], PRY_, m_ * kLanes);
+}
+
+void Engine::upload_params() {
+  auto* P = reinterpret_cast<LaneParams*>(pinned_);
+  for (int l = 0; l < kLanes; ++l) {
+    LaneParams lp{};
+    if (l < count_ && ((active_ >> l) & 1ull)) {
+      const IterParams ip = ctl_[static_cast<size_t>(l)].period_params(neq_);
+      lp.tau = ip.tau;
+      lp.sigma = ip.sigma;
+      lp.gamma = ip.gamma;
+      lp.halpern = ip.halpern;
+      lp.reset_at_start = ip.reset_at_start;
+      lp.k_at_start = ip.k_at_start;
+      lp.active = 1;
+    }
+    P[l] = lp;
+  }
+  hip_check(hipMemcpyAsync(d_params_, P, kLanes * sizeof(LaneParams), hipMemcpyHostToDevice,
+                           be_.stream()),
+            "upload lane params");
+}
+
+void Engine::enqueue_period(int steps, bool has_check, int parity) {
+  hipStream_t s = be_.stream();
+  int p = parity;
+  // Algorithmic bytes per launch: the matrix once (8 B value + 4 B index per
+  // nonzero, 4 B per row start), the gathered vector once per active LP, and
+  // the seven per-LP vectors the epilogue reads and writes (nine or ten on the
+  // check step), 8 B each.
+  const double lanes = static_cast<double>(__builtin_popcountll(active_));
+  const double nnz = static_cast<double>(k_.nnz()), n = n_, m = m_;
+  for (int step = 0; step < steps; ++step) {
+    const bool diag = has_check && step == steps - 1;
+    PrimalBufs pb{X_[p], AX_, X_[1 - p], XBAR_, DX_, AX_};
+    const LaunchEvents pe =
+        be_.launch_events(diag ? "batch_primal_step_check" : "batch_primal_step",
+                          12.0 * nnz + 4.0 * (n + 1) + 8.0 * lanes * (m + (diag ? 8.0 : 7.0) * n));
+    launch_batch_primal_step(s, dkt_, Y_[p], data_, d_params_, step, pb, diag, &pe);
+    DualBufs db{Y_[p], KX_[p], AY_, AKX_, Y_[1 - p], KX_[1 - p], DY_, KDX_, AY_, AKX_};
+    const LaunchEvents de =
+        be_.launch_events(diag ? "batch_dual_step_check" : "batch_dual_step",
+                          12.0 * nnz + 4.0 * (m + 1) + 8.0 * lanes * (n + (diag ? 9.0 : 7.0) * m));
+    launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, step, db, diag, &de);
+    p = 1 - p;
+  }
+  if (has_check) {
+    const bool farkas = opt_.detect_infeasibility != 0;
+    HipBackend::Timed t(be_, "batch_check");
+    launch_batch_spmm(s, dkt_, Y_[p], KTY_, farkas ? DY_ : nullptr, farkas ? LAM_ : nullptr,
+                      d_params_);
+    launch_batch_check(s, data_, X_[p], Y_[p], KX_[p], DX_, DY_, KTY_, LAM_, farkas, d_params_,
+                       CHECKP_, CHECK_);
+  }
+}
+
+hipGraphExec_t Engine::graph_for(int parity) {
+  auto it = graphs_.find(parity);
+  if (it != graphs_.end()) return it->second.second;
+  hipStream_t s = be_.stream();
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  try {
+    enqueue_period(opt_.termination_check_frequency, true, parity);
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+  hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  graphs_[parity] = {g, e};
+  return e;
+}
+
+void Engine::fetch(const double* dev, double* host, size_t doubles) {
+  hip_check(hipMemcpyAsync(host, dev, doubles * sizeof(double), hipMemcpyDeviceToHost, be_.stream()),
+            "batch fetch");
+  be_.synchronize();
+}
+
+// Copies the current iterate of the masked lanes out to their outcomes.
+void Engine::retire_extract(Mask mask, std::vector<LaneOutcome>& out) {
+  if (mask == 0) return;
+  const int cnt = __builtin_popcountll(mask);
+  std::vector<double> xs(static_cast<size_t>(cnt) * uz(n_)), ys(static_cast<size_t>(cnt) * uz(m_));
+  launch_batch_extract(be_.stream(), n_, X_[cur_], mask, EXT_);
+  be_.download(EXT_, xs.data(), cnt * n_);
+  launch_batch_extract(be_.stream(), m_, Y_[cur_], mask, EXT_);
+  be_.download(EXT_, ys.data(), cnt * m_);
+  int slot = 0;
+  for (int l = 0; l < count_; ++l) {
+    if (!((mask >> l) & 1ull)) continue;
+    LaneOutcome& o = out[static_cast<size_t>(l)];
+    if (!o.polished) {
+      o.x.assign(xs.begin() + static_cast<long>(slot) * n_, xs.begin() + static_cast<long>(slot + 1) * n_);
+      o.y.assign(ys.begin() + static_cast<long>(slot) * m_, ys.begin() + static_cast<long>(slot + 1) * m_);
+    }
+    ++slot;
+  }
+}
+
+// A restart for the masked lanes (pdhg.cpp:1124-1202): K x recomputed, the
+// distances moved since the last restart (for the primal weight) measured
+// lane-serially, the restart points advanced.
+void Engine::run_restarts(Mask mask) {
+  if (mask == 0) return;
+  launch_batch_spmm(be_.stream(), dk_, X_[cur_], KX_[cur_], nullptr, nullptr, nullptr, mask);
+  const bool previous = !opt_.pid_primal_weight;
+  launch_batch_restart(be_.stream(), n_, m_, X_[cur_], Y_[cur_], RX_, RY_, PRX_, PRY_, previous,
+                       true, mask, RESTART_);
+  double* h = pinned_ + 64 * 8 + 64 * kLaneCheckStride;
+  fetch(RESTART_, h, 128);
+  for (int l = 0; l < count_; ++l) {
+    if (!((mask >> l) & 1ull)) continue;
+    ctl_[static_cast<size_t>(l)].restart(std::sqrt(h[2 * l]), std::sqrt(h[2 * l + 1]));
+  }
+}
+
+std::vector<LaneOutcome> run_engine(HipBackend& be, const Csr& k, const Csr& kt, Int neq,
+                                    std::vector<double> row_scale, std::vector<double> col_scale,
+                                    double norm, const anuk_pdhg_options& opt,
+                                    std::vector<Lane>& lanes, Clock::time_point start,
+                                    const PolishContext* polish) {
+  Engine e(be, k, kt, neq, std::move(row_scale), std::move(col_scale), norm, opt, lanes, start,
+           polish);
+  return e.run();
+}
+
+// One polished pair per seed, unscaled, with the iterations it cost.
+struct Polished {
+  std::vector<double> x, y;
+  Int spent = 0;
+};
+
+// The two feasibility sub-problems of each seed, as one sub-batch each
+// (pdhg.cpp:680-743): primal with c = 0 started from (x, 0), dual with q = 0
+// and the finite bounds zeroed started from (0, y), both with the caller's
+// step size and each LP's primal weight, no scaling (the seeds are already in
+// the scaled space), then unscaled with the stack's diagonals.
+std::vector<Polished> polish_lanes(HipBackend& be, const Csr& k, const Csr& kt, Int neq,
+                                   double norm, const anuk_pdhg_options& opt,
+                                   const std::vector<const Lane*>& src,
+                                   const std::vector<std::vector<double>>& seed_x,
+                                   const std::vector<std::vector<double>>& seed_y,
+                                   const std::vector<double>& omega, double eta, Int budget,
+                                   const Scaling& scaling, Clock::time_point start) {
+  const Int n = k.cols, m = k.rows;
+  anuk_pdhg_options sub = opt;
+  sub.polish_feasibility = 0;
+  sub.polish_on_exit = 0;
+  sub.detect_infeasibility = 0;
+  sub.verbose = 0;
+  sub.known_matrix_norm = norm;
+  sub.ruiz_iterations = 0;
+  sub.pock_chambolle = 0;
+  sub.max_iterations = budget;
+  sub.termination_check_frequency =
+      std::max<Int>(10, std::min<Int>(opt.termination_check_frequency, budget / 4));
+  sub.time_limit_seconds = std::fmax(0.0, opt.time_limit_seconds - seconds_since(start));
+  sub.initial_step_size = eta;
+  sub.warm_x = nullptr;
+  sub.warm_y = nullptr;
+  const size_t cnt = src.size();
+  std::vector<Lane> primal(cnt), dual(cnt);
+  for (size_t s = 0; s < cnt; ++s) {
+    Lane& p = primal[s];
+    p.c.assign(uz(n), 0.0);
+    p.q = src[s]->q;
+    p.lower = src[s]->lower;
+    p.upper = src[s]->upper;
+    p.warm_x = seed_x[s];
+    p.warm_y.assign(uz(m), 0.0);
+    p.initial_omega = omega[s];
+    Lane& d = dual[s];
+    d.c = src[s]->c;
+    d.q.assign(uz(m), 0.0);
+    d.lower = src[s]->lower;
+    d.upper = src[s]->upper;
+    for (Int j = 0; j < n; ++j) {
+      if (d.lower[uz(j)] > -kInf) d.lower[uz(j)] = 0.0;
+      if (d.upper[uz(j)] < kInf) d.upper[uz(j)] = 0.0;
+    }
+    d.warm_x.assign(uz(n), 0.0);
+    d.warm_y = seed_y[s];
+    d.initial_omega = omega[s];
+    for (Lane* t : {&p, &d}) {
+      t->oc = t->c;
+      t->oq = t->q;
+      t->olo = t->lower;
+      t->ohi = t->upper;
+    }
+  }
+  const std::vector<double> ones_m(uz(m), 1.0), ones_n(uz(n), 1.0);
+  sub.termination = ANUK_TERM_PRIMAL_FEASIBILITY;
+  const std::vector<LaneOutcome> pr =
+      run_engine(be, k, kt, neq, ones_m, ones_n, norm, sub, primal, start, nullptr);
+  sub.termination = ANUK_TERM_DUAL_FEASIBILITY;
+  const std::vector<LaneOutcome> dr =
+      run_engine(be, k, kt, neq, ones_m, ones_n, norm, sub, dual, start, nullptr);
+  std::vector<Polished> out(cnt);
+  for (size_t s = 0; s < cnt; ++s) {
+    scaling.unscale_primal(pr[s].x, &out[s].x);
+    scaling.unscale_dual(dr[s].y, &out[s].y);
+    out[s].spent = pr[s].iterations + dr[s].iterations;
+  }
+  return out;
+}
+
+// A polish attempt for the lanes `who` at a check: adopted, and the lane
+// stopped as optimal, when the polished pair passes the full test on the
+// lane's original LP.
+void Engine::run_polish(const std::vector<int>& who, Int budget, std::vector<LaneOutcome>& out,
+                        std::vector<bool>* stopped, Int iteration) {
+  (void)iteration;
+  Mask mask = 0;
+  for (int l : who) mask |= 1ull << l;
+  const int cnt = static_cast<int>(who.size());
+  std::vector<double> xs(static_cast<size_t>(cnt) * uz(n_)), ys(static_cast<size_t>(cnt) * uz(m_));
+  launch_batch_extract(be_.stream(), n_, X_[cur_], mask, EXT_);
+  be_.download(EXT_, xs.data(), cnt * n_);
+  launch_batch_extract(be_.stream(), m_, Y_[cur_], mask, EXT_);
+  be_.download(EXT_, ys.data(), cnt * m_);
+  std::vector<const Lane*> src;
+  std::vector<std::vector<double>> sx, sy;
+  std::vector<double> omega;
+  for (int s = 0; s < cnt; ++s) {
+    const int l = who[static_cast<size_t>(s)];
+    src.push_back(&lanes_[static_cast<size_t>(l)]);
+    sx.emplace_back(xs.begin() + static_cast<long>(s) * n_, xs.begin() + static_cast<long>(s + 1) * n_);
+    sy.emplace_back(ys.begin() + static_cast<long>(s) * m_, ys.begin() + static_cast<long>(s + 1) * m_);
+    omega.push_back(ctl_[static_cast<size_t>(l)].omega);
+  }
+  std::vector<Polished> pol = polish_lanes(be_, k_, kt_, neq_, norm_, opt_, src, sx, sy, omega,
+                                           ctl_[static_cast<size_t>(who[0])].eta, budget,
+                                           *polish_->scaling, start_);
+  for (int s = 0; s < cnt; ++s) {
+    const int l = who[static_cast<size_t>(s)];
+    LaneOutcome& o = out[static_cast<size_t>(l)];
+    std::vector<double>& tx = pol[static_cast<size_t>(s)].x;
+    std::vector<double>& ty = pol[static_cast<size_t>(s)].y;
+    const Residual tr = evaluate_residual(*(*polish_->originals)[static_cast<size_t>(l)], tx, ty);
+    ++o.polish_attempts;
+    o.polish_iterations += pol[static_cast<size_t>(s)].spent;
+    Control& c = ctl_[static_cast<size_t>(l)];
+    if (tr.converged(c.tol, opt_.absolute_tolerance, opt_.require_inf_norm_termination != 0,
+                     c.gap_tol)) {
+      o.x = std::move(tx);
+      o.y = std::move(ty);
+      o.polished = true;
+      o.polished_r = tr;
+      c.status = ANUK_PDHG_OPTIMAL;
+      c.message = "converged after feasibility polishing";
+      (*stopped)[static_cast<size_t>(l)] = true;
+    }
+  }
+}
+
+std::vector<LaneOutcome> Engine::run() {
+  count_ = static_cast<int>(lanes_.size());
+  if (count_ <= 0 || count_ > kLanes) throw std::invalid_argument("a batch holds 1 to 64 LPs");
+  n_ = k_.cols;
+  m_ = k_.rows;
+  dk_ = be_.prepare(view_of(k_));
+  dkt_ = be_.prepare(view_of(kt_));
+  allocate();
+  for (Lane& l : lanes_) l.set_scales();
+  ctl_.resize(static_cast<size_t>(count_));
+  for (int l = 0; l < count_; ++l) {
+    const Lane& ln = lanes_[static_cast<size_t>(l)];
+    anuk_pdhg_options o = opt_;
+    if (ln.initial_omega > 0.0) o.initial_primal_weight = ln.initial_omega;
+    Control& c = ctl_[static_cast<size_t>(l)];
+    c.init(opt_, norm_, host_two_norm(ln.c), host_two_norm(ln.q));
+    if (ln.initial_omega > 0.0) c.omega = ln.initial_omega;
+  }
+  upload_lanes();
+  active_ = (count_ == 64) ? ~0ull : ((1ull << count_) - 1ull);
+  launch_batch_spmm(be_.stream(), dk_, X_[0], KX_[0], nullptr, nullptr, nullptr, active_);
+  cur_ = 0;
+
+  std::vector<LaneOutcome> out(static_cast<size_t>(count_));
+  const Int freq = opt_.termination_check_frequency;
+  Int iteration = 0;
+  double* h_check = pinned_ + 64 * 8;
+  while (iteration < opt_.max_iterations && active_ != 0) {
+    const Int period_end = std::min<Int>((iteration / freq + 1) * freq, opt_.max_iterations);
+    const int steps = period_end - iteration;
+    const bool has_check = (period_end % freq == 0);
+    upload_params();
+    const bool graph = opt_.use_graphs != 0 && has_check && steps == freq && !be_.profiling();
+    if (graph) {
+      hip_check(hipGraphLaunch(graph_for(cur_), be_.stream()), "hipGraphLaunch(batch)");
+    } else {
+      enqueue_period(steps, has_check, cur_);
+    }
+    for (int l = 0; l < count_; ++l)
+      if ((active_ >> l) & 1ull) ctl_[static_cast<size_t>(l)].advance(steps);
+    if (steps % 2 == 1) cur_ = 1 - cur_;
+    iteration = period_end;
+    if (!has_check) {
+      be_.synchronize();
+      continue;
+    }
+    fetch(CHECK_, h_check, static_cast<size_t>(64) * kLaneCheckStride);
+    const Int i = period_end - 1;
+    const double elapsed = seconds_since(start_);
+
+    std::vector<bool> stopped(static_cast<size_t>(count_), false);
+    std::vector<Residual> res(static_cast<size_t>(count_));
+    std::map<Int, std::vector<int>> polish_by_budget;
+    for (int l = 0; l < count_; ++l) {
+      if (!((active_ >> l) & 1ull)) continue;
+      Control& c = ctl_[static_cast<size_t>(l)];
+      const Lane& ln = lanes_[static_cast<size_t>(l)];
+      const double* h = h_check + static_cast<size_t>(l) * kLaneCheckStride;
+      c.movement = c.omega * h[kSumDx2] + h[kSumDy2] / c.omega;
+      bool farkas = false;
+      if (opt_.detect_infeasibility) {
+        farkas = h[kFarkasRowBad] == 0.0 && h[kFarkasScale] > 0.0 && h[kFarkasColBad] == 0.0 &&
+                 (h[kFarkasRhs] - h[kFarkasSupport]) / h[kFarkasScale] > opt_.infeasibility_tolerance;
+      }
+      Terms t;
+      t.primal_sq = h[kPrimalSq];
+      t.primal_inf = h[kPrimalInf];
+      t.q_dot_y = h[kQDotY];
+      t.dual_sq = h[kDualSq];
+      t.dual_inf = h[kDualInf];
+      t.bound_term = h[kBoundTerm];
+      t.c_dot_x = h[kCDotX];
+      ResidualScales rs;
+      rs.q_two = ln.q_two;
+      rs.c_two = ln.c_two;
+      rs.q_inf = ln.q_inf;
+      rs.c_inf = ln.c_inf;
+      res[static_cast<size_t>(l)] = residual_from_terms(t, rs);
+      Int budget = 0;
+      const Control::Verdict v = c.check_first(i, res[static_cast<size_t>(l)], farkas, &budget);
+      if (v == Control::Verdict::kStop) {
+        stopped[static_cast<size_t>(l)] = true;
+      } else if (v == Control::Verdict::kPolish) {
+        polish_by_budget[budget].push_back(l);
+      }
+    }
+    for (const auto& kv : polish_by_budget) {
+      if (polish_ == nullptr) continue;  // sub-batches never polish
+      run_polish(kv.second, kv.first, out, &stopped, i);
+    }
+    Mask restart = 0;
+    for (int l = 0; l < count_; ++l) {
+      if (!((active_ >> l) & 1ull) || stopped[static_cast<size_t>(l)]) continue;
+      const Control::Verdict v =
+          ctl_[static_cast<size_t>(l)].check_second(i, res[static_cast<size_t>(l)], 0.0, elapsed);
+      if (v == Control::Verdict::kStop) {
+        stopped[static_cast<size_t>(l)] = true;
+      } else if (v == Control::Verdict::kRestart) {
+        restart |= 1ull << l;
+      }
+    }
+    Mask retiring = 0;
+    for (int l = 0; l < count_; ++l) {
+      if (!stopped[static_cast<size_t>(l)]) continue;
+      retiring |= 1ull << l;
+      LaneOutcome& o = out[static_cast<size_t>(l)];
+      o.iterations = i + 1;
+      o.seconds = seconds_since(start_);
+    }
+    retire_extract(retiring, out);
+    active_ &= ~retiring;
+    run_restarts(restart & active_);
+  }
+  // Lanes still running stopped at the iteration limit.
+  Mask rest = active_;
+  for (int l = 0; l < count_; ++l) {
+    if (!((rest >> l) & 1ull)) continue;
+    out[static_cast<size_t>(l)].iterations = iteration;
+    out[static_cast<size_t>(l)].seconds = seconds_since(start_);
+  }
+  retire_extract(rest, out);
+  for (int l = 0; l < count_; ++l) {
+    LaneOutcome& o = out[static_cast<size_t>(l)];
+    const Control& c = ctl_[static_cast<size_t>(l)];
+    o.status = c.status;
+    o.message = c.message;
+    o.restarts = c.restarts;
+    o.eta = c.eta;
+    o.omega = c.omega;
+  }
+  return out;
+}
+
+}  // namespace
+
+bool batch_supports(const anuk_pdhg_options& o, const char** why) {
+  auto no = [&](const char* w) {
+    if (why) *why = w;
+    return false;
+  };
+  if (o.adaptive_step_size && !o.constant_step_size) return no("adaptive step size");
+  if (o.halpern_restart) return no("Halpern restart test");
+  if (o.restarts && !o.halpern) return no("averaged restarts");
+  if (o.restarts && !o.restart_on_fixed_point) return no("KKT restart test");
+  if (!o.reuse_products) return no("host-side residual products");
+  if (o.verbose) return no("verbose logging");
+  if (o.warm_x != nullptr || o.warm_y != nullptr) return no("a shared warm start");
+  if (o.termination_check_frequency <= 0) return no("a non-positive check frequency");
+  return true;
+}
+
+std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<const StandardLp*>& lps,
+                                          const anuk_pdhg_options& opt) {
+  const Clock::time_point start = Clock::now();
+  const char* why = nullptr;
+  if (!batch_supports(opt, &why)) throw std::invalid_argument(std::string("batch: unsupported option: ") + why);
+  if (lps.empty()) return {};
+  const StandardLp& first = *lps[0];
+  for (const StandardLp* lp : lps) {
+    if (lp->rows() != first.rows() || lp->cols() != first.cols() ||
+        lp->num_equalities != first.num_equalities || lp->k.start != first.k.start ||
+        lp->k.index != first.k.index || lp->k.value != first.k.value)
+      throw std::invalid_argument("batch: every LP must share the same constraint matrix");
+  }
+  const Int n = first.cols(), m = first.rows();
+
+  // Scaling depends on K alone: equilibrate it once and replay the sweeps on
+  // each LP's vectors.
+  Csr k = first.k;
+  Csr kt;
+  std::vector<Sweep> sweeps;
+  const bool scales = opt.ruiz_iterations > 0 || opt.pock_chambolle != 0;
+  if (scales) {
+    sweeps = scale_matrix(&k, opt.ruiz_iterations, opt.pock_chambolle != 0);
+    kt = k.transpose();
+  } else {
+    kt = first.kt;
+  }
+  Scaling scaling;
+  std::vector<Lane> lanes(lps.size());
+  for (size_t i = 0; i < lps.size(); ++i) {
+    Lane& ln = lanes[i];
+    ln.c = lps[i]->c;
+    ln.q = lps[i]->q;
+    ln.lower = lps[i]->lower;
+    ln.upper = lps[i]->upper;
+    Scaling s;
+    apply_sweeps(sweeps, &ln.q, &ln.c, &ln.lower, &ln.upper, &s);
+    if (i == 0) scaling = s;
+    ln.oc = lps[i]->c;
+    ln.oq = lps[i]->q;
+    ln.olo = lps[i]->lower;
+    ln.ohi = lps[i]->upper;
+  }
+  StandardLp shape;
+  shape.k = k;
+  shape.kt = kt;
+  be.prepare(view_of(k));
+  be.prepare(view_of(kt));
+  struct Release {
+    HipBackend& b;
+    std::uint64_t a, c;
+    ~Release() {
+      b.release_one(a);
+      b.release_one(c);
+    }
+  } release{be, k.id, kt.id};
+  const double norm = opt.known_matrix_norm > 0.0 ? opt.known_matrix_norm
+                                                 : estimate_matrix_norm(be, shape, 200, 1e-8);
+
+  std::vector<SolveResult> results(lps.size());
+  if (norm <= 0.0) {
+    // No coupling: every LP gets its plain solve's componentwise answer.
+    for (size_t i = 0; i < lps.size(); ++i) results[i] = solve_pdhg(be, *lps[i], opt);
+    return results;
+  }
+
+  std::vector<LaneOutcome> outcomes;
+  outcomes.reserve(lps.size());
+  for (size_t g0 = 0; g0 < lps.size(); g0 += kLanes) {
+    const size_t g1 = std::min(lps.size(), g0 + kLanes);
+    std::vector<Lane> group(lanes.begin() + static_cast<long>(g0), lanes.begin() + static_cast<long>(g1));
+    std::vector<const StandardLp*> originals(lps.begin() + static_cast<long>(g0), lps.begin() + static_cast<long>(g1));
+    PolishContext pc{&originals, &scaling};
+    std::vector<LaneOutcome> part = run_engine(be, k, kt, first.num_equalities, scaling.row_scale,
+                                               scaling.col_scale, norm, opt, group, start,
+                                               opt.polish_feasibility ? &pc : nullptr);
+    for (auto& o : part) outcomes.push_back(std::move(o));
+  }
+
+  // Final answers, and one more polish for the LPs that ran out of
+  // iterations (pdhg.cpp:1205-1269), batched by budget.
+  std::map<Int, std::vector<size_t>> exit_polish;
+  for (size_t i = 0; i < lps.size(); ++i) {
+    LaneOutcome& o = outcomes[i];
+    SolveResult& r = results[i];
+    r.status = o.status;
+    r.message = o.message;
+    r.iterations = o.iterations;
+    r.restarts = o.restarts;
+    r.polish_attempts = o.polish_attempts;
+    r.polish_iterations = o.polish_iterations;
+    r.matrix_norm_estimate = norm;
+    r.final_step_size = o.eta;
+    r.final_primal_weight = o.omega;
+    r.solve_seconds = o.seconds;
+    if (o.polished) {
+      r.x = std::move(o.x);
+      r.y = std::move(o.y);
+      r.residual = o.polished_r;
+      r.polished = true;
+    } else {
+      scaling.unscale_primal(o.x, &r.x);
+      scaling.unscale_dual(o.y, &r.y);
+      r.residual = evaluate_residual(*lps[i], r.x, r.y);
+      const bool worth = opt.polish_feasibility && opt.polish_on_exit &&
+                         opt.termination == ANUK_TERM_FULL && r.status != ANUK_PDHG_OPTIMAL &&
+                         r.status != ANUK_PDHG_PRIMAL_INFEASIBLE &&
+                         r.status != ANUK_PDHG_NUMERICAL_ERROR &&
+                         std::isfinite(r.residual.primal_residual) &&
+                         std::isfinite(r.residual.dual_residual) &&
+                         seconds_since(start) < opt.time_limit_seconds;
+      if (worth) {
+        Control c;
+        c.opt = &opt;
+        const Int budget = c.polish_budget(r.iterations);
+        if (budget > 0) exit_polish[budget].push_back(i);
+      }
+    }
+  }
+  for (const auto& kv : exit_polish) {
+    // Seeds are the working-space iterates the loop stopped at; adoption
+    // follows pdhg.cpp:1243-1268 (neither feasibility measure worse, one
+    // better, the gap within what the caller accepts or already had).
+    for (size_t g0 = 0; g0 < kv.second.size(); g0 += kLanes) {
+      const size_t g1 = std::min(kv.second.size(), g0 + kLanes);
+      std::vector<size_t> idx(kv.second.begin() + static_cast<long>(g0),
+                              kv.second.begin() + static_cast<long>(g1));
+      // LPs that share a budget reached it at the same iteration and so with
+      // the same step size; the primal weight is each LP's own.
+      std::vector<const Lane*> src;
+      std::vector<std::vector<double>> sx, sy;
+      std::vector<double> omega;
+      for (size_t i : idx) {
+        src.push_back(&lanes[i]);
+        sx.push_back(outcomes[i].x);
+        sy.push_back(outcomes[i].y);
+        omega.push_back(outcomes[i].omega);
+      }
+      std::vector<Polished> pol = polish_lanes(be, k, kt, first.num_equalities, norm, opt, src, sx,
+                                               sy, omega, outcomes[idx[0]].eta, kv.first, scaling,
+                                               start);
+      const double gap_tol = opt.gap_tolerance > 0.0 ? opt.gap_tolerance : opt.tolerance;
+      for (size_t s = 0; s < idx.size(); ++s) {
+        SolveResult& r = results[idx[s]];
+        Polished& p = pol[s];
+        const Residual tr = evaluate_residual(*lps[idx[s]], p.x, p.y);
+        ++r.polish_attempts;
+        r.polish_iterations += p.spent;
+        const double ceiling = std::fmax(gap_tol, r.residual.relative_gap);
+        const bool better = tr.relative_primal <= r.residual.relative_primal &&
+                            tr.relative_dual <= r.residual.relative_dual &&
+                            (tr.relative_primal < r.residual.relative_primal ||
+                             tr.relative_dual < r.residual.relative_dual);
+        if (better && tr.relative_gap <= ceiling) {
+          r.x = std::move(p.x);
+          r.y = std::move(p.y);
+          r.residual = tr;
+          r.polished = true;
+          if (tr.converged(opt.tolerance, opt.absolute_tolerance,
+                           opt.require_inf_norm_termination != 0, gap_tol)) {
+            r.status = ANUK_PDHG_OPTIMAL;
+            r.message = "converged after feasibility polishing";
+          } else {
+            r.message = r.message.empty() ? "feasibility polished" : r.message + ", feasibility polished";
+          }
+        }
+      }
+    }
+  }
+  for (size_t i = 0; i < lps.size(); ++i) results[i].objective = lps[i]->model_objective(results[i].x);
+  return results;
+}
+
+}  // namespace anuk

@@ -1,0 +1,456 @@
+// gfx950 kernels of the batched solver (see batch_kernels.hpp). Compiled with
+// -ffp-contract=off like the plain kernels: each lane's arithmetic is the
+// reference's, operation for operation.
+#include "batch_kernels.hpp"
+
+#include <hip/hip_ext.h>
+
+#include <stdexcept>
+#include <string>
+
+namespace anuk {
+
+namespace {
+
+constexpr int kWavesPerBlock = 4;  // 256 threads, one row per wave
+
+__device__ __forceinline__ int xcd_remap(int i, int g) {
+  const int x = i & 7, local = i >> 3, per = g >> 3, rem = g & 7;
+  return x * per + (x < rem ? x : rem) + local;
+}
+
+// The row this wave owns, made wave-uniform so the matrix loads are scalar.
+__device__ __forceinline__ Int wave_row() {
+  const int blk = xcd_remap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  return __builtin_amdgcn_readfirstlane(blk * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6));
+}
+
+__device__ __forceinline__ size_t at(Int row, int lane) {
+  return static_cast<size_t>(row) * kLanes + static_cast<size_t>(lane);
+}
+
+struct LaneScalars {
+  double tau, sigma, gamma, w;
+  bool blend, open_epoch;
+};
+
+// The Halpern schedule of one step, per lane (as step_scalars in kernels.hip).
+__device__ __forceinline__ LaneScalars lane_scalars(const LaneParams& p, int step) {
+  LaneScalars s;
+  s.tau = p.tau;
+  s.sigma = p.sigma;
+  s.gamma = p.gamma;
+  if (!p.halpern) {
+    s.blend = false;
+    s.open_epoch = false;
+    s.w = 1.0;
+  } else if (p.reset_at_start && step == 0) {
+    s.blend = false;
+    s.open_epoch = true;
+    s.w = 1.0;
+  } else {
+    const Int k = p.reset_at_start ? step - 1 : p.k_at_start + step;
+    s.blend = true;
+    s.open_epoch = false;
+    s.w = static_cast<double>(k + 1) / static_cast<double>(k + 2);
+  }
+  return s;
+}
+
+// Lane-serial row sum of A[row, :] v[:, lane] in storage order. The gathers
+// of four nonzeros are issued together so a long row is not a chain of
+// memory latencies; the adds still happen one by one in storage order.
+__device__ __forceinline__ double row_sum(const DevCsr& a, Int row, const double* __restrict__ v,
+                                          int lane) {
+  const Int lo = a.start[row], hi = a.start[row + 1];
+  double s = 0.0;
+  Int k = lo;
+  for (; k + 4 <= hi; k += 4) {
+    const double a0 = a.value[k], a1 = a.value[k + 1], a2 = a.value[k + 2], a3 = a.value[k + 3];
+    const double x0 = v[at(a.index[k], lane)];
+    const double x1 = v[at(a.index[k + 1], lane)];
+    const double x2 = v[at(a.index[k + 2], lane)];
+    const double x3 = v[at(a.index[k + 3], lane)];
+    s += a0 * x0;
+    s += a1 * x1;
+    s += a2 * x2;
+    s += a3 * x3;
+  }
+  for (; k < hi; ++k) s += a.value[k] * v[at(a.index[k], lane)];
+  return s;
+}
+
+template <bool kDiag>
+__global__ void __launch_bounds__(256)
+    bprimal_kernel(DevCsr kt, const double* __restrict__ y, BatchData d, PrimalBufs b,
+                   const LaneParams* __restrict__ params, int step) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const Int j = wave_row();
+  if (j >= kt.rows) return;
+  const LaneParams P = params[lane];
+  if (!P.active) return;
+  const LaneScalars sc = lane_scalars(P, step);
+  const double kty = row_sum(kt, j, y, lane);
+  const size_t o = at(j, lane);
+  const double xj = b.x[o];
+  double v = xj - sc.tau * (d.c[o] - kty);
+  if (v < d.lower[o]) v = d.lower[o];
+  if (v > d.upper[o]) v = d.upper[o];
+  const double dd = v - xj;
+  b.x_bar[o] = v + dd;
+  double next = v;
+  if (sc.gamma != 0.0) next += sc.gamma * dd;
+  if (sc.blend) next = sc.w * next + (1.0 - sc.w) * b.anchor[o];
+  b.x_out[o] = next;
+  if (sc.open_epoch) b.anchor_out[o] = next;
+  if constexpr (kDiag) b.dx[o] = dd;
+}
+
+template <bool kDiag>
+__global__ void __launch_bounds__(256)
+    bdual_kernel(DevCsr k, const double* __restrict__ x_bar, BatchData d, DualBufs b,
+                 const LaneParams* __restrict__ params, int step) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const Int i = wave_row();
+  if (i >= k.rows) return;
+  const LaneParams P = params[lane];
+  if (!P.active) return;
+  const LaneScalars sc = lane_scalars(P, step);
+  const double kxb = row_sum(k, i, x_bar, lane);
+  const size_t o = at(i, lane);
+  const double yi = b.y[o];
+  const double kx = b.k_x[o];
+  double v = yi + sc.sigma * (d.q[o] - kxb);
+  if (i >= d.num_equalities && v < 0.0) v = 0.0;
+  const double dd = v - yi;
+  const double kd = 0.5 * (kxb - kx);
+  double ny = v;
+  double nk = 0.5 * (kxb + kx);
+  if (sc.gamma != 0.0) {
+    ny += sc.gamma * dd;
+    nk += sc.gamma * kd;
+  }
+  if (sc.blend) {
+    ny = sc.w * ny + (1.0 - sc.w) * b.anchor_y[o];
+    nk = sc.w * nk + (1.0 - sc.w) * b.anchor_kx[o];
+  }
+  b.y_out[o] = ny;
+  b.k_out[o] = nk;
+  if (sc.open_epoch) {
+    b.anchor_y_out[o] = ny;
+    b.anchor_kx_out[o] = nk;
+  }
+  if constexpr (kDiag) {
+    b.dy[o] = dd;
+    b.k_dx[o] = kd;
+  }
+}
+
+__device__ __forceinline__ bool lane_on(const LaneParams* params, unsigned long long mask,
+                                        int lane) {
+  return params != nullptr ? params[lane].active != 0 : ((mask >> lane) & 1ull) != 0;
+}
+
+__global__ void __launch_bounds__(256)
+    bspmm_kernel(DevCsr a, const double* __restrict__ v1, double* __restrict__ out1,
+                 const double* __restrict__ v2, double* __restrict__ out2,
+                 const LaneParams* __restrict__ params, unsigned long long mask) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const Int r = wave_row();
+  if (r >= a.rows || !lane_on(params, mask, lane)) return;
+  out1[at(r, lane)] = row_sum(a, r, v1, lane);
+  if (v2 != nullptr) out2[at(r, lane)] = row_sum(a, r, v2, lane);
+}
+
+// Check reductions, per lane, in a fixed tree: each wave folds a contiguous
+// run of rows in index order, the four waves of a block fold in wave order
+// into per-block partials [block][value][lane], and bfold_kernel folds the
+// blocks in block order. The per-row terms are those of backend.cpp:143-202
+// (convergence terms of the unscaled problem), backend.cpp:123-126 (the
+// movement) and pdhg.cpp:191-221 (the Farkas test).
+constexpr int kSideVals = 7;
+// Row side: sum dy^2, primal_sq, primal_inf (max), q'y, Farkas row flag (max),
+// max |dy|, q~'dy.
+constexpr unsigned kRowMax = (1u << 2) | (1u << 4) | (1u << 5);
+constexpr int kRowSlot[kSideVals] = {kSumDy2, kPrimalSq, kPrimalInf, kQDotY,
+                                     kFarkasRowBad, kFarkasScale, kFarkasRhs};
+// Column side: sum dx^2, dual_sq, dual_inf (max), bound term, c'x, Farkas
+// column flag (max), support.
+constexpr unsigned kColMax = (1u << 2) | (1u << 5);
+constexpr int kColSlot[kSideVals] = {kSumDx2, kDualSq, kDualInf, kBoundTerm,
+                                     kCDotX, kFarkasColBad, kFarkasSupport};
+
+__device__ __forceinline__ void block_store_side(double (&v)[kSideVals], unsigned max_mask,
+                                                 double* __restrict__ partials) {
+  __shared__ double lds[4][kSideVals][kLanes];
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int wave = static_cast<int>(threadIdx.x >> 6);
+#pragma unroll
+  for (int k = 0; k < kSideVals; ++k) lds[wave][k][lane] = v[k];
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int k = 0; k < kSideVals; ++k) {
+      const bool mx = (max_mask >> k) & 1u;
+      double a = lds[0][k][lane];
+      for (int w = 1; w < 4; ++w) a = mx ? fmax(a, lds[w][k][lane]) : a + lds[w][k][lane];
+      partials[(static_cast<size_t>(blockIdx.x) * kSideVals + k) * kLanes + lane] = a;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    bcheck_rows_kernel(BatchData d, const double* __restrict__ y, const double* __restrict__ k_x,
+                       const double* __restrict__ dy, int farkas, Int rows_per_wave,
+                       const LaneParams* __restrict__ params, double* __restrict__ partials) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const Int r0 = (static_cast<Int>(blockIdx.x) * 4 + static_cast<Int>(threadIdx.x >> 6)) * rows_per_wave;
+  const Int r1 = min(r0 + rows_per_wave, d.m);
+  double v[kSideVals] = {0, 0, 0, 0, 0, 0, 0};
+  if (params[lane].active != 0) {
+    for (Int i = r0; i < r1; ++i) {
+      const size_t p = at(i, lane);
+      const double e = dy[p];
+      v[0] += e * e;
+      const double scale = d.row_scale[i];
+      const double row = scale != 0.0 ? k_x[p] / scale : k_x[p];
+      const double slack = row - d.orig_q[p];
+      const double viol = (i < d.num_equalities) ? slack : fmin(slack, 0.0);
+      v[1] += viol * viol;
+      v[2] = fmax(v[2], fabs(viol));
+      v[3] += d.orig_q[p] * (scale * y[p]);
+      if (farkas) {
+        if (i >= d.num_equalities && e < 0.0) v[4] = 1.0;
+        v[5] = fmax(v[5], fabs(e));
+        v[6] += d.q[p] * e;
+      }
+    }
+  }
+  block_store_side(v, kRowMax, partials);
+}
+
+__global__ void __launch_bounds__(256)
+    bcheck_cols_kernel(BatchData d, const double* __restrict__ x, const double* __restrict__ dx,
+                       const double* __restrict__ kt_y, const double* __restrict__ lambda,
+                       int farkas, Int rows_per_wave, const LaneParams* __restrict__ params,
+                       double* __restrict__ partials) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const Int j0 = (static_cast<Int>(blockIdx.x) * 4 + static_cast<Int>(threadIdx.x >> 6)) * rows_per_wave;
+  const Int j1 = min(j0 + rows_per_wave, d.n);
+  double v[kSideVals] = {0, 0, 0, 0, 0, 0, 0};
+  if (params[lane].active != 0) {
+    for (Int j = j0; j < j1; ++j) {
+      const size_t p = at(j, lane);
+      const double e = dx[p];
+      v[0] += e * e;
+      const double scale = d.col_scale[j];
+      const double product = scale != 0.0 ? kt_y[p] / scale : kt_y[p];
+      const double c = d.orig_c[p];
+      const double lam = c - product;
+      const double lo = d.orig_lower[p], hi = d.orig_upper[p];
+      const bool has_lo = lo > -__builtin_inf(), has_hi = hi < __builtin_inf();
+      double left;
+      if (has_lo && has_hi) {
+        left = 0.0;
+      } else if (has_lo) {
+        left = fmin(lam, 0.0);
+      } else if (has_hi) {
+        left = fmax(lam, 0.0);
+      } else {
+        left = lam;
+      }
+      v[1] += left * left;
+      v[2] = fmax(v[2], fabs(left));
+      const double absorbed = lam - left;
+      if (absorbed > 0.0) {
+        v[3] += absorbed * lo;
+      } else if (absorbed < 0.0) {
+        v[3] += absorbed * hi;
+      }
+      v[4] += c * (scale * x[p]);
+      if (farkas) {
+        const double l = lambda[p];
+        if (l > 0.0) {
+          if (d.upper[p] >= __builtin_inf()) {
+            v[5] = 1.0;
+          } else {
+            v[6] += l * d.upper[p];
+          }
+        } else if (l < 0.0) {
+          if (d.lower[p] <= -__builtin_inf()) {
+            v[5] = 1.0;
+          } else {
+            v[6] += l * d.lower[p];
+          }
+        }
+      }
+    }
+  }
+  block_store_side(v, kColMax, partials);
+}
+
+// One thread per (side value, lane): folds that value's per-block partials in
+// block order into out[lane * kLaneCheckStride + slot].
+__global__ void __launch_bounds__(1024)
+    bfold_kernel(const double* __restrict__ row_partials, Int row_blocks,
+                 const double* __restrict__ col_partials, Int col_blocks,
+                 const LaneParams* __restrict__ params, double* __restrict__ out) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int k = static_cast<int>(threadIdx.x >> 6);  // 0..15
+  if (k >= 2 * kSideVals || params[lane].active == 0) return;
+  const bool rows = k < kSideVals;
+  const int v = rows ? k : k - kSideVals;
+  const double* part = rows ? row_partials : col_partials;
+  const Int blocks = rows ? row_blocks : col_blocks;
+  const bool mx = ((rows ? kRowMax : kColMax) >> v) & 1u;
+  double a = 0.0;
+  for (Int b = 0; b < blocks; ++b) {
+    const double t = part[(static_cast<size_t>(b) * kSideVals + v) * kLanes + lane];
+    a = mx ? fmax(a, t) : a + t;
+  }
+  out[static_cast<size_t>(lane) * kLaneCheckStride + (rows ? kRowSlot[v] : kColSlot[v])] = a;
+}
+
+__global__ void __launch_bounds__(128)
+    brestart_kernel(Int n, Int m, const double* __restrict__ x, const double* __restrict__ y,
+                    double* rx, double* ry, double* prx, double* pry, int use_previous,
+                    int advance, unsigned long long mask, double* __restrict__ out) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int wave = static_cast<int>(threadIdx.x >> 6);
+  if (!((mask >> lane) & 1ull)) return;
+  const Int len = wave == 0 ? n : m;
+  const double* v = wave == 0 ? x : y;
+  double* r = wave == 0 ? rx : ry;
+  double* pr = wave == 0 ? prx : pry;
+  double acc = 0.0;
+  for (Int i = 0; i < len; ++i) {
+    const size_t p = at(i, lane);
+    const double cur = v[p];
+    const double base = use_previous ? pr[p] : r[p];
+    const double e = cur - base;
+    acc += e * e;
+    if (advance) {
+      if (pr != nullptr) pr[p] = r[p];
+      r[p] = cur;
+    }
+  }
+  out[2 * lane + wave] = acc;
+}
+
+__global__ void bextract_kernel(Int rows, const double* __restrict__ v, unsigned long long mask,
+                                double* __restrict__ out) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  if (!((mask >> lane) & 1ull)) return;
+  const int slot = __builtin_popcountll(mask & ((1ull << lane) - 1ull));
+  const Int stride = static_cast<Int>(gridDim.x * (blockDim.x >> 6));
+  for (Int r = static_cast<Int>(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); r < rows;
+       r += stride)
+    out[static_cast<size_t>(slot) * rows + r] = v[at(r, lane)];
+}
+
+void check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("launch ") + what + ": " + hipGetErrorString(e));
+}
+
+inline dim3 row_grid(Int rows) {
+  return dim3(static_cast<unsigned>((rows + kWavesPerBlock - 1) / kWavesPerBlock));
+}
+
+template <class K, class... Args>
+void launch_rows(K kernel, Int rows, hipStream_t s, const LaunchEvents* ev, const char* what,
+                 Args... args) {
+  if (rows <= 0) return;
+  if (ev != nullptr && ev->start != nullptr) {
+    hipExtLaunchKernelGGL(kernel, row_grid(rows), dim3(256), 0, s, ev->start, ev->stop, 0, args...);
+  } else {
+    hipLaunchKernelGGL(kernel, row_grid(rows), dim3(256), 0, s, args...);
+  }
+  check_launch(what);
+}
+
+}  // namespace
+
+void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
+                              const BatchData& d, const LaneParams* params, int step,
+                              const PrimalBufs& b, bool diag, const LaunchEvents* ev) {
+  if (diag) {
+    launch_rows(bprimal_kernel<true>, kt.rows, s, ev, "batch_primal_step", kt, y, d, b, params, step);
+  } else {
+    launch_rows(bprimal_kernel<false>, kt.rows, s, ev, "batch_primal_step", kt, y, d, b, params, step);
+  }
+}
+
+void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
+                            const BatchData& d, const LaneParams* params, int step,
+                            const DualBufs& b, bool diag, const LaunchEvents* ev) {
+  if (diag) {
+    launch_rows(bdual_kernel<true>, k.rows, s, ev, "batch_dual_step", k, x_bar, d, b, params, step);
+  } else {
+    launch_rows(bdual_kernel<false>, k.rows, s, ev, "batch_dual_step", k, x_bar, d, b, params, step);
+  }
+}
+
+void launch_batch_spmm(hipStream_t s, const DevCsr& a, const double* v1, double* out1,
+                       const double* v2, double* out2, const LaneParams* params,
+                       unsigned long long mask) {
+  launch_rows(bspmm_kernel, a.rows, s, nullptr, "batch_spmm", a, v1, out1, v2, out2, params, mask);
+}
+
+namespace {
+// Rows per wave so a side needs at most kCheckBlocks blocks of four waves.
+Int check_rows_per_wave(Int rows) {
+  const Int waves = kCheckBlocks * 4;
+  return rows <= waves ? 1 : (rows + waves - 1) / waves;
+}
+Int check_blocks(Int rows) {
+  const Int per_block = 4 * check_rows_per_wave(rows);
+  return rows <= 0 ? 0 : (rows + per_block - 1) / per_block;
+}
+}  // namespace
+
+size_t batch_check_scratch_doubles() {
+  return static_cast<size_t>(2) * kCheckBlocks * kSideVals * kLanes;
+}
+
+void launch_batch_check(hipStream_t s, const BatchData& d, const double* x, const double* y,
+                        const double* k_x, const double* dx, const double* dy,
+                        const double* kt_y, const double* lambda, bool farkas,
+                        const LaneParams* params, double* scratch, double* out) {
+  double* row_part = scratch;
+  double* col_part = scratch + static_cast<size_t>(kCheckBlocks) * kSideVals * kLanes;
+  const Int rb = check_blocks(d.m), cb = check_blocks(d.n);
+  if (rb > 0) {
+    hipLaunchKernelGGL(bcheck_rows_kernel, dim3(static_cast<unsigned>(rb)), dim3(256), 0, s, d, y,
+                       k_x, dy, farkas ? 1 : 0, check_rows_per_wave(d.m), params, row_part);
+    check_launch("batch_check_rows");
+  }
+  if (cb > 0) {
+    hipLaunchKernelGGL(bcheck_cols_kernel, dim3(static_cast<unsigned>(cb)), dim3(256), 0, s, d, x,
+                       dx, kt_y, lambda, farkas ? 1 : 0, check_rows_per_wave(d.n), params,
+                       col_part);
+    check_launch("batch_check_cols");
+  }
+  hipLaunchKernelGGL(bfold_kernel, dim3(1), dim3(1024), 0, s, row_part, rb, col_part, cb, params,
+                     out);
+  check_launch("batch_check_fold");
+}
+
+void launch_batch_restart(hipStream_t s, Int n, Int m, const double* x, const double* y,
+                          double* rx, double* ry, double* prx, double* pry, bool use_previous,
+                          bool advance, unsigned long long mask, double* out) {
+  hipLaunchKernelGGL(brestart_kernel, dim3(1), dim3(128), 0, s, n, m, x, y, rx, ry, prx, pry,
+                     use_previous ? 1 : 0, advance ? 1 : 0, mask, out);
+  check_launch("batch_restart");
+}
+
+void launch_batch_extract(hipStream_t s, Int rows, const double* v, unsigned long long mask,
+                          double* out) {
+  if (rows <= 0 || mask == 0) return;
+  const unsigned blocks = static_cast<unsigned>(std::min<Int>((rows + 3) / 4, 1024));
+  hipLaunchKernelGGL(bextract_kernel, dim3(blocks), dim3(256), 0, s, rows, v, mask, out);
+  check_launch("batch_extract");
+}
+
+}  // namespace anuk

@@ -1,0 +1,88 @@
+// Kernels of the batched solver: up to 64 LPs that share one constraint
+// matrix K, one wave64 lane per LP.
+//
+// Every per-LP vector is stored row-interleaved, v[row * 64 + lane], so the
+// wave that owns a row of K touches one contiguous 512-byte line per vector
+// and the matrix itself (value, index, row start) is read once for all the
+// LPs with wave-uniform scalar loads. Each lane sums its own row serially in
+// storage order and runs the reference's elementwise steps in the
+// reference's order, so lane i computes exactly the bits a plain solve of LP
+// i computes (and the bits the reference CPU backend computes). The check
+// reductions are serial per lane in index order for the same reason.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace anuk {
+
+inline constexpr int kLanes = 64;
+
+// Per-LP step scalars; the device holds kLanes of them.
+struct LaneParams {
+  double tau;
+  double sigma;
+  double gamma;
+  Int halpern;
+  Int reset_at_start;
+  Int k_at_start;
+  Int active;  // 0: retired or padding, the lane does nothing
+};
+
+// Problem data, per LP row-interleaved except the shared scaling diagonals.
+struct BatchData {
+  Int n = 0, m = 0, num_equalities = 0;
+  const double* c = nullptr;      // [n][64] iteration (scaled) data
+  const double* lower = nullptr;
+  const double* upper = nullptr;
+  const double* q = nullptr;      // [m][64]
+  const double* orig_c = nullptr; // [n][64] convergence (unscaled) data
+  const double* orig_lower = nullptr;
+  const double* orig_upper = nullptr;
+  const double* orig_q = nullptr; // [m][64]
+  const double* row_scale = nullptr;  // [m] shared
+  const double* col_scale = nullptr;  // [n] shared
+};
+
+// Slots of the per-lane check vector; the same quantities as CheckSlot.
+inline constexpr int kLaneCheckStride = 16;
+
+void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
+                              const BatchData& d, const LaneParams* params, int step,
+                              const PrimalBufs& b, bool diag, const LaunchEvents* ev = nullptr);
+void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
+                            const BatchData& d, const LaneParams* params, int step,
+                            const DualBufs& b, bool diag, const LaunchEvents* ev = nullptr);
+// out1 = A v1 (and out2 = A v2 when v2 is non-null) for the active lanes:
+// those whose LaneParams say so when `params` is given (read at run time, so
+// a captured graph follows retirements), else those set in `mask`. Other
+// lanes are left alone.
+void launch_batch_spmm(hipStream_t s, const DevCsr& a, const double* v1, double* out1,
+                       const double* v2, double* out2, const LaneParams* params,
+                       unsigned long long mask = ~0ull);
+// Per-lane check reductions into out[lane * kLaneCheckStride + slot] (slots
+// as CheckSlot) for the active lanes: movement sums, convergence terms of the
+// unscaled problem, and both halves of the Farkas test. Deterministic: a
+// fixed tree whose shape depends on the row counts only. `scratch` holds
+// batch_check_scratch_doubles() doubles.
+inline constexpr int kCheckBlocks = 64;
+size_t batch_check_scratch_doubles();
+void launch_batch_check(hipStream_t s, const BatchData& d, const double* x, const double* y,
+                        const double* k_x, const double* dx, const double* dy,
+                        const double* kt_y, const double* lambda, bool farkas,
+                        const LaneParams* params, double* scratch, double* out);
+// For masked lanes: out[2 lane] = sum_j (x - base_x)^2 and out[2 lane + 1] =
+// sum_i (y - base_y)^2, serially, where the base is the last restart point
+// (rx, ry) or, with use_previous, the one before it (prx, pry). Then, when
+// `advance`, prx <- rx and rx <- x, pry <- ry and ry <- y (prx/pry may be
+// null when use_previous is false).
+void launch_batch_restart(hipStream_t s, Int n, Int m, const double* x, const double* y,
+                          double* rx, double* ry, double* prx, double* pry, bool use_previous,
+                          bool advance, unsigned long long mask, double* out);
+// Copies the columns of the masked lanes, in lane order, into dense
+// [count][rows] arrays.
+void launch_batch_extract(hipStream_t s, Int rows, const double* v, unsigned long long mask,
+                          double* out);
+
+}  // namespace anuk

@@ -11,6 +11,7 @@
 #include "../../include/anukulan_hip.h"
 #include "backend.hpp"
 #include "lp.hpp"
+#include "batch.hpp"
 #include "pdhg.hpp"
 
 struct anuk_ctx {
@@ -439,40 +440,84 @@ void anuk_pdhg_default_options(anuk_pdhg_options* options) {
   if (options != nullptr) anuk::default_options(options);
 }
 
+namespace {
+
+anuk_pdhg_options checked_options(const anuk_pdhg_options* options) {
+  anuk_pdhg_options o;
+  if (options != nullptr) {
+    o = *options;
+  } else {
+    anuk::default_options(&o);
+  }
+  need(o.termination_check_frequency > 0, "termination_check_frequency must be positive");
+  need(o.max_iterations >= 0, "max_iterations must be non-negative");
+  need(o.polish_first_iteration > 0, "polish_first_iteration must be positive");
+  return o;
+}
+
+void fill_result(const anuk::SolveResult& r, anuk_pdhg_result* result) {
+  std::memset(result, 0, sizeof(*result));
+  result->status = r.status;
+  result->iterations = r.iterations;
+  result->restarts = r.restarts;
+  result->polish_attempts = r.polish_attempts;
+  result->polish_iterations = r.polish_iterations;
+  result->polished = r.polished ? 1 : 0;
+  result->objective = r.objective;
+  result->solve_seconds = r.solve_seconds;
+  result->matrix_norm_estimate = r.matrix_norm_estimate;
+  result->final_step_size = r.final_step_size;
+  result->final_primal_weight = r.final_primal_weight;
+  result->phase_scaling = r.phases.scaling;
+  result->phase_matrix_norm = r.phases.matrix_norm;
+  result->phase_setup = r.phases.setup;
+  result->phase_iterating = r.phases.iterating;
+  result->phase_checks = r.phases.checks;
+  result->phase_polishing = r.phases.polishing;
+  fill_residual(r.residual, &result->residual);
+  std::strncpy(result->message, r.message.c_str(), sizeof(result->message) - 1);
+}
+
+}  // namespace
+
+int anuk_solve_pdhg_batch(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,
+                          const anuk_pdhg_options* options, anuk_pdhg_result* results,
+                          double* x_out, double* y_out) {
+  if (ctx == nullptr || lps == nullptr || results == nullptr || count < 0) return ANUK_ERR_INVALID;
+  const int rc = [&] {
+    anuk_pdhg_options o;
+    const int g = guarded(ctx, [&] { o = checked_options(options); });
+    if (g != ANUK_OK) return g;
+    const char* why = nullptr;
+    if (!anuk::batch_supports(o, &why)) {
+      ctx->error = std::string("batched solve does not run: ") + why;
+      return static_cast<int>(ANUK_ERR_UNSUPPORTED);
+    }
+    return guarded(ctx, [&] {
+      std::vector<const anuk::StandardLp*> v;
+      for (int32_t i = 0; i < count; ++i) {
+        need(lps[i] != nullptr, "null LP in the batch");
+        v.push_back(&lps[i]->lp);
+      }
+      const std::vector<anuk::SolveResult> rs = anuk::solve_pdhg_batch(*ctx->be, v, o);
+      for (size_t i = 0; i < rs.size(); ++i) {
+        fill_result(rs[i], &results[i]);
+        const size_t cols = static_cast<size_t>(v[i]->cols()), rows = static_cast<size_t>(v[i]->rows());
+        if (x_out && !rs[i].x.empty()) std::memcpy(x_out + i * cols, rs[i].x.data(), cols * sizeof(double));
+        if (y_out && !rs[i].y.empty()) std::memcpy(y_out + i * rows, rs[i].y.data(), rows * sizeof(double));
+      }
+    });
+  }();
+  return rc;
+}
+
 int anuk_solve_pdhg(anuk_ctx* ctx, const anuk_lp* lp, const anuk_pdhg_options* options,
                     anuk_pdhg_result* result, double* x_out, double* y_out) {
   if (ctx == nullptr || lp == nullptr || result == nullptr) return ANUK_ERR_INVALID;
   return guarded(ctx, [&] {
-    anuk_pdhg_options o;
-    if (options != nullptr) {
-      o = *options;
-    } else {
-      anuk::default_options(&o);
-    }
-    need(o.termination_check_frequency > 0, "termination_check_frequency must be positive");
-    need(o.max_iterations >= 0, "max_iterations must be non-negative");
-    need(o.polish_first_iteration > 0, "polish_first_iteration must be positive");
+    const anuk_pdhg_options o = checked_options(options);
     const anuk::SolveResult r = anuk::solve_pdhg(*ctx->be, lp->lp, o);
-    std::memset(result, 0, sizeof(*result));
-    result->status = r.status;
-    result->iterations = r.iterations;
-    result->restarts = r.restarts;
-    result->polish_attempts = r.polish_attempts;
-    result->polish_iterations = r.polish_iterations;
-    result->polished = r.polished ? 1 : 0;
-    result->objective = r.objective;
-    result->solve_seconds = r.solve_seconds;
-    result->matrix_norm_estimate = r.matrix_norm_estimate;
-    result->final_step_size = r.final_step_size;
-    result->final_primal_weight = r.final_primal_weight;
-    result->phase_scaling = r.phases.scaling;
-    result->phase_matrix_norm = r.phases.matrix_norm;
-    result->phase_setup = r.phases.setup;
-    result->phase_iterating = r.phases.iterating;
-    result->phase_checks = r.phases.checks;
-    result->phase_polishing = r.phases.polishing;
-    fill_residual(r.residual, &result->residual);
-    std::strncpy(result->message, r.message.c_str(), sizeof(result->message) - 1);
+    fill_result(r, result);
     if (x_out && !r.x.empty()) std::memcpy(x_out, r.x.data(), r.x.size() * sizeof(double));
     if (y_out && !r.y.empty()) std::memcpy(y_out, r.y.data(), r.y.size() * sizeof(double));
   });

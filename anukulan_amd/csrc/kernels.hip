@@ -3,6 +3,8 @@
 // below rounds on its own, as the reference's CPU loops do.
 #include "kernels.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <stdexcept>
 #include <vector>
@@ -512,10 +514,15 @@ void fold(hipStream_t s, std::initializer_list<FoldSeg> segs, double* out) {
 
 template <class Epi>
 void spmv_with(hipStream_t s, const DevCsr& a, const double* x, const Epi& e, double* partials,
-               const char* what) {
+               const char* what, const LaunchEvents* ev = nullptr) {
   if (a.blocks <= 0) return;
-  hipLaunchKernelGGL(spmv_kernel<Epi>, dim3(static_cast<unsigned>(a.blocks)), dim3(kThreads), 0,
-                     s, a, x, e, partials);
+  const dim3 grid(static_cast<unsigned>(a.blocks));
+  if (ev != nullptr && ev->start != nullptr) {
+    hipExtLaunchKernelGGL(spmv_kernel<Epi>, grid, dim3(kThreads), 0, s, ev->start, ev->stop, 0, a,
+                          x, e, partials);
+  } else {
+    hipLaunchKernelGGL(spmv_kernel<Epi>, grid, dim3(kThreads), 0, s, a, x, e, partials);
+  }
   check_launch(what);
 }
 
@@ -569,8 +576,9 @@ int reduce_grid(Int len) {
   return static_cast<int>(g);
 }
 
-void launch_spmv(hipStream_t s, const DevCsr& a, const double* x, double* y) {
-  spmv_with(s, a, x, PlainEpi{y}, nullptr, "spmv");
+void launch_spmv(hipStream_t s, const DevCsr& a, const double* x, double* y,
+                 const LaunchEvents* ev) {
+  spmv_with(s, a, x, PlainEpi{y}, nullptr, "spmv", ev);
 }
 
 void launch_fill(hipStream_t s, double* p, Int n, double v) {
@@ -654,21 +662,21 @@ void launch_convergence_terms(hipStream_t s, Int n, Int m, const DevProblem& p,
 
 void launch_primal_step(hipStream_t s, const DevCsr& kt, const double* y, const DevProblem& p,
                         const IterParams* params, int step, const PrimalBufs& b, unsigned flags,
-                        double* partials) {
+                        double* partials, const LaunchEvents* ev) {
   if (flags & kStepDiag) {
-    spmv_with(s, kt, y, PrimalStepEpi<true>{p, b, params, step, {}}, partials, "primal_step");
+    spmv_with(s, kt, y, PrimalStepEpi<true>{p, b, params, step, {}}, partials, "primal_step", ev);
   } else {
-    spmv_with(s, kt, y, PrimalStepEpi<false>{p, b, params, step, {}}, partials, "primal_step");
+    spmv_with(s, kt, y, PrimalStepEpi<false>{p, b, params, step, {}}, partials, "primal_step", ev);
   }
 }
 
 void launch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar, const DevProblem& p,
                       const IterParams* params, int step, const DualBufs& b, unsigned flags,
-                      double* partials) {
+                      double* partials, const LaunchEvents* ev) {
   if (flags & kStepDiag) {
-    spmv_with(s, k, x_bar, DualStepEpi<true>{p, b, params, step, {}}, partials, "dual_step");
+    spmv_with(s, k, x_bar, DualStepEpi<true>{p, b, params, step, {}}, partials, "dual_step", ev);
   } else {
-    spmv_with(s, k, x_bar, DualStepEpi<false>{p, b, params, step, {}}, partials, "dual_step");
+    spmv_with(s, k, x_bar, DualStepEpi<false>{p, b, params, step, {}}, partials, "dual_step", ev);
   }
 }
 

@@ -112,13 +112,12 @@ enum CheckSlot : int {
   kCheckSlots
 };
 
-// Workspace for multi-block reductions: partials[blocks * kMaxRed] per source.
-struct ReduceScratch {
-  double* partials_a = nullptr;  // primal step / generic
-  double* partials_b = nullptr;  // dual step
-  double* partials_c = nullptr;  // convergence product
-  double* partials_d = nullptr;  // Farkas product
-  Int capacity = 0;              // blocks each array holds
+// An event pair the dispatch itself records at kernel start and end
+// (hipExtLaunchKernel), the same interval a kernel-trace profiler reports.
+// Null events mean a plain launch.
+struct LaunchEvents {
+  hipEvent_t start = nullptr;
+  hipEvent_t stop = nullptr;
 };
 
 // ---- launchers (stream-ordered, asynchronous) --------------------------
@@ -127,7 +126,8 @@ struct ReduceScratch {
 void plan_blocks(Int rows, const Int* start, Int tile_nnz, int max_rows,
                  std::int32_t** out_rows, Int* out_blocks);
 
-void launch_spmv(hipStream_t s, const DevCsr& a, const double* x, double* y);
+void launch_spmv(hipStream_t s, const DevCsr& a, const double* x, double* y,
+                 const LaunchEvents* ev = nullptr);
 
 void launch_fill(hipStream_t s, double* p, Int n, double v);
 void launch_accumulate(hipStream_t s, Int n, double w, const double* v, double* sum);
@@ -161,10 +161,12 @@ void launch_convergence_terms(hipStream_t s, Int n, Int m, const DevProblem& p,
 // position of this step in its period (for the Halpern weight).
 void launch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                         const DevProblem& p, const IterParams* params, int step,
-                        const PrimalBufs& b, unsigned flags, double* partials);
+                        const PrimalBufs& b, unsigned flags, double* partials,
+                        const LaunchEvents* ev = nullptr);
 void launch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                       const DevProblem& p, const IterParams* params, int step,
-                      const DualBufs& b, unsigned flags, double* partials);
+                      const DualBufs& b, unsigned flags, double* partials,
+                      const LaunchEvents* ev = nullptr);
 // Check products: dual-side convergence terms from K^T y, and the Farkas
 // support from K^T dy. The Farkas product reads the row slots of `check`
 // (folded first by launch_check_fold_rows) and does nothing when the row half

@@ -217,11 +217,10 @@ void Scaling::unscale_dual(const std::vector<double>& scaled, std::vector<double
 
 namespace {
 
-// One equilibration sweep. Row and column norms are the infinity norm
+// One equilibration sweep of K. Row and column norms are the infinity norm
 // (one_norm == false) or the 1-norm, accumulated in storage order; each row
 // and column is divided by the square root of its norm, empty ones left alone.
-void sweep(StandardLp* lp, bool one_norm, Scaling* acc) {
-  Csr& k = lp->k;
+Sweep sweep_matrix(Csr& k, bool one_norm) {
   std::vector<double> rn(uz(k.rows), 0.0), cn(uz(k.cols), 0.0);
   auto fold = [one_norm](double a, double v) { return one_norm ? a + v : std::fmax(a, v); };
   for (Int r = 0; r < k.rows; ++r) {
@@ -239,27 +238,43 @@ void sweep(StandardLp* lp, bool one_norm, Scaling* acc) {
   for (Int r = 0; r < k.rows; ++r)
     for (Int p = k.start[uz(r)]; p < k.start[uz(r) + 1]; ++p)
       k.value[uz(p)] = k.value[uz(p)] * rn[uz(r)] * cn[uz(k.index[uz(p)])];
-  for (std::size_t i = 0; i < rn.size(); ++i) {
-    lp->q[i] *= rn[i];
-    acc->row_scale[i] *= rn[i];
-  }
-  for (std::size_t j = 0; j < cn.size(); ++j) {
-    lp->c[j] *= cn[j];
-    lp->lower[j] /= cn[j];
-    lp->upper[j] /= cn[j];
-    acc->col_scale[j] *= cn[j];
-  }
+  return Sweep{std::move(rn), std::move(cn)};
 }
 
 }  // namespace
 
+std::vector<Sweep> scale_matrix(Csr* k, int ruiz_iterations, bool pock_chambolle) {
+  std::vector<Sweep> sweeps;
+  for (int i = 0; i < ruiz_iterations; ++i) sweeps.push_back(sweep_matrix(*k, false));
+  if (pock_chambolle) sweeps.push_back(sweep_matrix(*k, true));
+  k->renew_id();
+  return sweeps;
+}
+
+void apply_sweeps(const std::vector<Sweep>& sweeps, std::vector<double>* q, std::vector<double>* c,
+                  std::vector<double>* lower, std::vector<double>* upper, Scaling* acc) {
+  acc->row_scale.assign(q->size(), 1.0);
+  acc->col_scale.assign(c->size(), 1.0);
+  for (const Sweep& s : sweeps) {
+    for (std::size_t i = 0; i < s.row.size(); ++i) {
+      (*q)[i] *= s.row[i];
+      acc->row_scale[i] *= s.row[i];
+    }
+    // x = D2 x~: a smaller d2 makes the scaled variable larger, so the bounds
+    // divide; infinite bounds stay infinite.
+    for (std::size_t j = 0; j < s.col.size(); ++j) {
+      (*c)[j] *= s.col[j];
+      (*lower)[j] /= s.col[j];
+      (*upper)[j] /= s.col[j];
+      acc->col_scale[j] *= s.col[j];
+    }
+  }
+}
+
 Scaling scale_lp(StandardLp* lp, int ruiz_iterations, bool pock_chambolle) {
   Scaling s;
-  s.row_scale.assign(uz(lp->rows()), 1.0);
-  s.col_scale.assign(uz(lp->cols()), 1.0);
-  for (int i = 0; i < ruiz_iterations; ++i) sweep(lp, false, &s);
-  if (pock_chambolle) sweep(lp, true, &s);
-  lp->k.renew_id();
+  const std::vector<Sweep> sweeps = scale_matrix(&lp->k, ruiz_iterations, pock_chambolle);
+  apply_sweeps(sweeps, &lp->q, &lp->c, &lp->lower, &lp->upper, &s);
   lp->kt = lp->k.transpose();
   return s;
 }

@@ -100,6 +100,20 @@ struct Scaling {
 // rebuilds K^T. Returns the accumulated diagonals.
 Scaling scale_lp(StandardLp* lp, int ruiz_iterations, bool pock_chambolle);
 
+// The same in two halves, for LPs that share K: scale_matrix equilibrates K
+// in place and returns each sweep's row and column factors; apply_sweeps
+// then applies them, sweep by sweep, to one LP's q, c and bounds. Together
+// they give exactly scale_lp's bits (the vector updates never feed back into
+// the matrix norms).
+struct Sweep {
+  std::vector<double> row;  // d1 of the sweep
+  std::vector<double> col;  // d2 of the sweep
+};
+std::vector<Sweep> scale_matrix(Csr* k, int ruiz_iterations, bool pock_chambolle);
+void apply_sweeps(const std::vector<Sweep>& sweeps, std::vector<double>* q,
+                  std::vector<double>* c, std::vector<double>* lower, std::vector<double>* upper,
+                  Scaling* acc);
+
 struct Residual {
   double primal_residual = 0.0;
   double dual_residual = 0.0;
@@ -134,6 +148,7 @@ struct Terms {
 // The four data norms every relative measure divides by (pdhg.cpp:55-66).
 struct ResidualScales {
   double q_two = 0.0, c_two = 0.0, q_inf = 0.0, c_inf = 0.0;
+  ResidualScales() = default;
   explicit ResidualScales(const StandardLp& lp);
 };
 

@@ -407,20 +407,24 @@ void Solver::bookkeep_step() {
 void Solver::enqueue_steps(int steps, bool has_check, int parity) {
   hipStream_t s = be_.stream();
   int p = parity;
+  // Algorithmic bytes per launch: the matrix once (8 B value + 4 B index per
+  // nonzero, 4 B per row start), the gathered vector once, and the seven
+  // vectors the epilogue reads and writes (eight or nine on the check step).
+  const double nnz = static_cast<double>(lp_.k.nnz()), n = n_, m = m_;
   for (int step = 0; step < steps; ++step) {
     const bool diag = has_check && step == steps - 1;
     const unsigned flags = diag ? kStepDiag : 0u;
     PrimalBufs pb{xb_[p], anchor_x_, xb_[1 - p], x_bar_, dx_, anchor_x_};
-    {
-      HipBackend::Timed t(be_, diag ? "primal_step_check" : "primal_step");
-      launch_primal_step(s, dkt_, yb_[p], prob_, d_params_, step, pb, flags, part_a_);
-    }
+    const LaunchEvents pe = be_.launch_events(diag ? "primal_step_check" : "primal_step",
+                                              12.0 * nnz + 4.0 * (n + 1) + 8.0 * m +
+                                                  (diag ? 64.0 : 56.0) * n);
+    launch_primal_step(s, dkt_, yb_[p], prob_, d_params_, step, pb, flags, part_a_, &pe);
     DualBufs db{yb_[p], kxb_[p], anchor_y_, anchor_kx_, yb_[1 - p], kxb_[1 - p],
                 dy_, k_dx_, anchor_y_, anchor_kx_};
-    {
-      HipBackend::Timed t(be_, diag ? "dual_step_check" : "dual_step");
-      launch_dual_step(s, dk_, x_bar_, prob_, d_params_, step, db, flags, part_b_);
-    }
+    const LaunchEvents de = be_.launch_events(diag ? "dual_step_check" : "dual_step",
+                                              12.0 * nnz + 4.0 * (m + 1) + 8.0 * n +
+                                                  (diag ? 72.0 : 56.0) * m);
+    launch_dual_step(s, dk_, x_bar_, prob_, d_params_, step, db, flags, part_b_, &de);
     p = 1 - p;
   }
   if (has_check) enqueue_check(p);

@@ -357,6 +357,19 @@ typedef struct anuk_pdhg_result {
 int anuk_solve_pdhg(anuk_ctx* ctx, const anuk_lp* lp, const anuk_pdhg_options* options,
                     anuk_pdhg_result* result, double* x_out, double* y_out);
 
+/* Batched solve of `count` LPs that share one constraint matrix: the same
+ * rows, columns, equality count and CSR arrays (a base plan and its what-if
+ * cases), differing in c, q and bounds. No reference counterpart: the
+ * reference solves such a stack one LP at a time (solve_pdhg per case). Each
+ * LP gets exactly the result its own anuk_solve_pdhg gives: results[i], its
+ * primal in x_out + i * cols and its dual in y_out + i * rows (either may be
+ * NULL). Runs the options the fused loop runs (constant step size, Halpern
+ * or no restarts, fixed-point restart test); others fail with
+ * ANUK_ERR_UNSUPPORTED. options->warm_x / warm_y must be NULL. */
+int anuk_solve_pdhg_batch(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,
+                          const anuk_pdhg_options* options, anuk_pdhg_result* results,
+                          double* x_out, double* y_out);
+
 /* estimate_matrix_norm (pdhg.hpp:580-581, pdhg.cpp:223-282) of lp's K as
  * given (no scaling), by power iteration on the device. */
 int anuk_estimate_matrix_norm(anuk_ctx* ctx, const anuk_lp* lp, int32_t iterations,

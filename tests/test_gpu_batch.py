@@ -1,0 +1,116 @@
+"""Batched case stacks on the GPU.
+
+The batch has no counterpart in the reference (its case stack is solved one
+LP at a time), so it is pinned by invariants: LP i of a batch must reach what
+the reference's own solve_pdhg reaches on LP i, and what the plain GPU solve
+of LP i reaches. Each lane's row sums and elementwise steps are the
+reference's operations in the reference's order (bit-identical); only the
+check reductions (a fixed tree instead of a serial loop) and ||K|| differ in
+the last bits, and they only steer decisions. The bar is the north star's:
+same status, objective within 1e-8 relative, iterations within 2%.
+"""
+import numpy as np
+import pytest
+
+from anukulan_amd import AnukError, generators as G, solve_pdhg, solve_pdhg_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return abs(a - b) / max(1.0, abs(b))
+
+
+def stack(periods, count=None):
+    base, cases = G.case_stack(periods)
+    if count is not None:
+        cases = cases[:count]
+    return base, cases, [G.case_model(base, c) for c in cases]
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+def test_batch_matches_the_reference(ctx, ref, tol):
+    base, cases, models = stack(4, 24)
+    lps = [m.to_lp() for m in models]
+    got = solve_pdhg_batch(ctx, lps, tolerance=tol)
+    same = 0
+    for m, g in zip(models, got):
+        w = ref.RefLp.from_natural(m).solve(tolerance=tol)
+        assert g.status == w.status, m.name
+        assert relerr(g.objective, w.objective) <= 1e-8, (m.name, g.objective, w.objective)
+        assert abs(g.iterations - w.iterations) <= 0.02 * w.iterations, (m.name, g.iterations,
+                                                                         w.iterations)
+        assert abs(g.matrix_norm_estimate - w.matrix_norm_estimate) <= 1e-12 * w.matrix_norm_estimate
+        same += g.iterations == w.iterations
+    print(f"\n{same} of {len(models)} cases took exactly the reference's iteration count")
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+def test_batch_matches_plain_gpu_solves(ctx, tol):
+    base, cases, models = stack(4, 12)
+    lps = [m.to_lp() for m in models]
+    got = solve_pdhg_batch(ctx, lps, tolerance=tol)
+    for lp, g in zip(lps, got):
+        p = solve_pdhg(ctx, lp, tolerance=tol)
+        assert g.status == p.status == "optimal"
+        assert relerr(g.objective, p.objective) <= 1e-8
+        assert abs(g.iterations - p.iterations) <= 0.02 * p.iterations
+
+
+def test_full_24_period_stack_against_the_fixture(ctx):
+    # The 24-period base plan is case 0 of the stack: its batched solve must
+    # reproduce the reference's fixture exactly at 1e-4.
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "pdhg_reference.json")))["refinery"]["24"]["1e-4"]
+    base, cases, models = stack(24)
+    got = solve_pdhg_batch(ctx, [m.to_lp() for m in models], tolerance=1e-4)
+    assert all(g.status == "optimal" for g in got)
+    assert abs(got[0].iterations - fx["iterations"]) <= 0.02 * fx["iterations"]
+    assert relerr(got[0].objective, fx["objective"]) <= 1e-8
+
+
+def test_more_than_one_wave_of_cases(ctx):
+    # 70 LPs run as two groups of lanes; duplicates must give identical answers.
+    base, cases, models = stack(4, 7)
+    lps = [m.to_lp() for m in models] * 10
+    got = solve_pdhg_batch(ctx, lps, tolerance=1e-4)
+    for i, g in enumerate(got):
+        h = got[i % 7]
+        assert g.iterations == h.iterations and np.array_equal(g.x, h.x)
+
+
+def test_batch_iteration_limit_and_exit_polish(ctx, ref):
+    base, cases, models = stack(4, 6)
+    lps = [m.to_lp() for m in models]
+    got = solve_pdhg_batch(ctx, lps, tolerance=1e-9, max_iterations=200)
+    for m, g in zip(models, got):
+        w = ref.RefLp.from_natural(m).solve(tolerance=1e-9, max_iterations=200)
+        assert g.iterations == w.iterations == 200
+        assert (g.status, g.polished, g.polish_attempts) == (w.status, w.polished, w.polish_attempts)
+        assert relerr(g.objective, w.objective) <= 1e-8
+
+
+def test_batch_rejects_options_it_does_not_run(ctx):
+    base, cases, models = stack(4, 2)
+    lps = [m.to_lp() for m in models]
+    with pytest.raises(AnukError) as e:
+        solve_pdhg_batch(ctx, lps, constant_step_size=0)
+    assert e.value.code == 4
+    other = G.refinery_model(5).to_lp()
+    with pytest.raises(AnukError) as e:
+        solve_pdhg_batch(ctx, [lps[0], other])
+    assert e.value.code == 1
+
+
+def test_batch_of_infeasible_and_feasible(ctx):
+    # A Farkas certificate in one lane must not disturb the others.
+    from golden.known_lps import natural
+    inf = natural("infeasible")
+    feas = G.NaturalModel(**{**inf.__dict__, "row_lower": np.array([1.5])})
+    got = solve_pdhg_batch(ctx, [inf.to_lp(), feas.to_lp(), inf.to_lp()], tolerance=1e-8)
+    assert [g.status for g in got] == ["primal infeasible", "optimal", "primal infeasible"]
+    plain = solve_pdhg(ctx, feas.to_lp(), tolerance=1e-8)
+    assert got[1].iterations == plain.iterations
+    assert abs(got[1].objective - 1.5) <= 1e-6

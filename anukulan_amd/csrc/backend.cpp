@@ -43,6 +43,7 @@ HipBackend::~HipBackend() {
   for (auto& kv : scratch_) (void)hipFree(kv.second.first);
   if (result_dev_) (void)hipFree(result_dev_);
   if (result_host_) (void)hipHostFree(result_host_);
+  if (pinned_) (void)hipHostFree(pinned_);
   if (stop_event_) (void)hipEventDestroy(stop_event_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -162,6 +163,34 @@ double* HipBackend::scratch(int slot, std::size_t doubles) {
   hip_check(hipMalloc(&p, size * sizeof(double)), "hipMalloc(scratch)");
   scratch_[slot] = {p, size};
   return p;
+}
+
+double* HipBackend::pinned_scratch(std::size_t doubles) {
+  if (pinned_ != nullptr && pinned_size_ >= doubles) return pinned_;
+  if (pinned_ != nullptr) {
+    synchronize();
+    (void)hipHostFree(pinned_);
+    pinned_ = nullptr;
+  }
+  pinned_size_ = std::max<std::size_t>(doubles, std::size_t{1} << 16);
+  hip_check(hipHostMalloc(&pinned_, pinned_size_ * sizeof(double), hipHostMallocDefault),
+            "hipHostMalloc(staging)");
+  return pinned_;
+}
+
+void HipBackend::upload_staged(const double* host, double* target, std::size_t n) {
+  if (n == 0) return;
+  // Large enough chunks to run at DMA speed, small enough to keep the
+  // staging buffer modest; each chunk's copy completes before it is reused.
+  const std::size_t chunk = std::size_t{1} << 20;
+  double* stage = pinned_scratch(std::min(n, chunk));
+  for (std::size_t off = 0; off < n; off += chunk) {
+    const std::size_t len = std::min(chunk, n - off);
+    std::memcpy(stage, host + off, len * sizeof(double));
+    hip_check(hipMemcpyAsync(target + off, stage, len * sizeof(double), hipMemcpyHostToDevice, stream_),
+              "upload_staged");
+    synchronize();
+  }
 }
 
 const double* HipBackend::fetch_result(int n) {

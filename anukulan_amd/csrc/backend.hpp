@@ -92,6 +92,12 @@ class HipBackend {
   // Scratch for the solver: a device buffer of at least `doubles`, one per
   // slot, grown on demand and kept for the context's life.
   double* scratch(int slot, std::size_t doubles);
+  // A pinned host staging buffer of at least `doubles`, grown on demand.
+  double* pinned_scratch(std::size_t doubles);
+  // Copies n doubles host -> device through the pinned staging buffer and
+  // waits (fast DMA for large uploads from pageable memory).
+  void upload_staged(const double* host, double* target, std::size_t n);
+
   // A small device result vector and its pinned host mirror.
   double* result_device() { return result_dev_; }
   double* result_host() { return result_host_; }
@@ -129,6 +135,8 @@ class HipBackend {
   hipStream_t stream_ = nullptr;
   std::unordered_map<std::uint64_t, DeviceMatrix> matrices_;
   std::map<int, std::pair<double*, std::size_t>> scratch_;
+  double* pinned_ = nullptr;
+  std::size_t pinned_size_ = 0;
   double* result_dev_ = nullptr;
   double* result_host_ = nullptr;
   bool profiling_ = false;

@@ -17,6 +17,10 @@ namespace {
 using Clock = std::chrono::steady_clock;
 using Mask = unsigned long long;
 
+// Backend scratch slots holding the state slabs of batch engines, one per
+// nesting depth (the stack, and the polishing sub-batches it runs).
+constexpr int kBatchScratchSlot = 100;
+
 double seconds_since(Clock::time_point t) {
   return std::chrono::duration<double>(Clock::now() - t).count();
 }
@@ -60,10 +64,11 @@ class Engine {
  public:
   Engine(HipBackend& be, const Csr& k, const Csr& kt, Int neq, std::vector<double> row_scale,
          std::vector<double> col_scale, double matrix_norm, const anuk_pdhg_options& opt,
-         std::vector<Lane>& lanes, Clock::time_point start, const PolishContext* polish)
+         std::vector<Lane>& lanes, Clock::time_point start, const PolishContext* polish,
+         Phases* phases, int depth)
       : be_(be), k_(k), kt_(kt), neq_(neq), row_scale_(std::move(row_scale)),
         col_scale_(std::move(col_scale)), norm_(matrix_norm), opt_(opt), lanes_(lanes),
-        start_(start), polish_(polish) {}
+        start_(start), polish_(polish), phases_(phases), depth_(depth) {}
   ~Engine();
   std::vector<LaneOutcome> run();
 
@@ -89,6 +94,8 @@ class Engine {
   std::vector<Lane>& lanes_;
   Clock::time_point start_;
   const PolishContext* polish_;
+  Phases* phases_;  // where this engine's time goes (null: not recorded)
+  int depth_;       // 0: the stack itself, 1: a polishing sub-batch
 
   Int n_ = 0, m_ = 0;
   int count_ = 0;
@@ -113,10 +120,9 @@ Engine::~Engine() {
     (void)hipGraphExecDestroy(g.second.second);
     (void)hipGraphDestroy(g.second.first);
   }
-  if (slab_) {
-    (void)hipStreamSynchronize(be_.stream());
-    (void)hipFree(slab_);
-  }
+  // The slab is the backend's scratch for this nesting depth, kept for the
+  // next batch of the same depth.
+  if (slab_) (void)hipStreamSynchronize(be_.stream());
   if (pinned_) (void)hipHostFree(pinned_);
 }
 
@@ -135,7 +141,7 @@ void Engine::allocate() {
       batch_check_scratch_doubles()};                    // check partials
   size_t total = 0;
   for (size_t s : sizes) total += round(s);
-  hip_check(hipMalloc(&slab_, total * sizeof(double)), "hipMalloc(batch state)");
+  slab_ = be_.scratch(kBatchScratchSlot + depth_, total);
   hip_check(hipMemsetAsync(slab_, 0, total * sizeof(double), be_.stream()), "hipMemsetAsync(batch)");
   double* p = slab_;
   size_t i = 0;
@@ -173,7 +179,7 @@ void upload_interleaved(HipBackend& be, double* dst, Int rows, int count,
     const std::vector<double>& v = *cols[static_cast<size_t>(l)];
     for (Int r = 0; r < rows; ++r) buf[static_cast<size_t>(r) * kLanes + l] = v[uz(r)];
   }
-  be.upload(buf.data(), dst, rows * kLanes);
+  be.upload_staged(buf.data(), dst, buf.size());
 }
 
 void Engine::upload_lanes() {
@@ -339,7 +345,7 @@ void Engine::run_restarts(Mask mask) {
   launch_batch_spmm(be_.stream(), dk_, X_[cur_], KX_[cur_], nullptr, nullptr, nullptr, mask);
   const bool previous = !opt_.pid_primal_weight;
   launch_batch_restart(be_.stream(), n_, m_, X_[cur_], Y_[cur_], RX_, RY_, PRX_, PRY_, previous,
-                       true, mask, RESTART_);
+                       true, mask, CHECKP_, RESTART_);
   double* h = pinned_ + 64 * 8 + 64 * kLaneCheckStride;
   fetch(RESTART_, h, 128);
   for (int l = 0; l < count_; ++l) {
@@ -352,9 +358,10 @@ std::vector<LaneOutcome> run_engine(HipBackend& be, const Csr& k, const Csr& kt,
                                     std::vector<double> row_scale, std::vector<double> col_scale,
                                     double norm, const anuk_pdhg_options& opt,
                                     std::vector<Lane>& lanes, Clock::time_point start,
-                                    const PolishContext* polish) {
+                                    const PolishContext* polish, Phases* phases = nullptr,
+                                    int depth = 1) {
   Engine e(be, k, kt, neq, std::move(row_scale), std::move(col_scale), norm, opt, lanes, start,
-           polish);
+           polish, phases, depth);
   return e.run();
 }
 
@@ -488,6 +495,7 @@ void Engine::run_polish(const std::vector<int>& who, Int budget, std::vector<Lan
 }
 
 std::vector<LaneOutcome> Engine::run() {
+  Clock::time_point mark = Clock::now();
   count_ = static_cast<int>(lanes_.size());
   if (count_ <= 0 || count_ > kLanes) throw std::invalid_argument("a batch holds 1 to 64 LPs");
   n_ = k_.cols;
@@ -509,6 +517,13 @@ std::vector<LaneOutcome> Engine::run() {
   active_ = (count_ == 64) ? ~0ull : ((1ull << count_) - 1ull);
   launch_batch_spmm(be_.stream(), dk_, X_[0], KX_[0], nullptr, nullptr, nullptr, active_);
   cur_ = 0;
+  auto lap = [&](double Phases::*field) {
+    const Clock::time_point now = Clock::now();
+    if (phases_ != nullptr) phases_->*field += std::chrono::duration<double>(now - mark).count();
+    mark = now;
+  };
+  be_.synchronize();
+  lap(&Phases::setup);
 
   std::vector<LaneOutcome> out(static_cast<size_t>(count_));
   const Int freq = opt_.termination_check_frequency;
@@ -531,9 +546,11 @@ std::vector<LaneOutcome> Engine::run() {
     iteration = period_end;
     if (!has_check) {
       be_.synchronize();
+      lap(&Phases::iterating);
       continue;
     }
     fetch(CHECK_, h_check, static_cast<size_t>(64) * kLaneCheckStride);
+    lap(&Phases::iterating);
     const Int i = period_end - 1;
     const double elapsed = seconds_since(start_);
 
@@ -573,10 +590,12 @@ std::vector<LaneOutcome> Engine::run() {
         polish_by_budget[budget].push_back(l);
       }
     }
+    lap(&Phases::checks);
     for (const auto& kv : polish_by_budget) {
       if (polish_ == nullptr) continue;  // sub-batches never polish
       run_polish(kv.second, kv.first, out, &stopped, i);
     }
+    lap(&Phases::polishing);
     Mask restart = 0;
     for (int l = 0; l < count_; ++l) {
       if (!((active_ >> l) & 1ull) || stopped[static_cast<size_t>(l)]) continue;
@@ -599,6 +618,7 @@ std::vector<LaneOutcome> Engine::run() {
     retire_extract(retiring, out);
     active_ &= ~retiring;
     run_restarts(restart & active_);
+    lap(&Phases::checks);
   }
   // Lanes still running stopped at the iteration limit.
   Mask rest = active_;
@@ -653,6 +673,14 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
   }
   const Int n = first.cols(), m = first.rows();
 
+  Phases phases;
+  Clock::time_point mark = start;
+  auto lap = [&](double Phases::*field) {
+    const Clock::time_point now = Clock::now();
+    phases.*field += std::chrono::duration<double>(now - mark).count();
+    mark = now;
+  };
+
   // Scaling depends on K alone: equilibrate it once and replay the sweeps on
   // each LP's vectors.
   Csr k = first.k;
@@ -681,6 +709,7 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
     ln.olo = lps[i]->lower;
     ln.ohi = lps[i]->upper;
   }
+  lap(&Phases::scaling);
   StandardLp shape;
   shape.k = k;
   shape.kt = kt;
@@ -696,6 +725,7 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
   } release{be, k.id, kt.id};
   const double norm = opt.known_matrix_norm > 0.0 ? opt.known_matrix_norm
                                                  : estimate_matrix_norm(be, shape, 200, 1e-8);
+  lap(&Phases::matrix_norm);
 
   std::vector<SolveResult> results(lps.size());
   if (norm <= 0.0) {
@@ -713,7 +743,8 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
     PolishContext pc{&originals, &scaling};
     std::vector<LaneOutcome> part = run_engine(be, k, kt, first.num_equalities, scaling.row_scale,
                                                scaling.col_scale, norm, opt, group, start,
-                                               opt.polish_feasibility ? &pc : nullptr);
+                                               opt.polish_feasibility ? &pc : nullptr, &phases,
+                                               0);
     for (auto& o : part) outcomes.push_back(std::move(o));
   }
 
@@ -757,6 +788,7 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
       }
     }
   }
+  mark = Clock::now();
   for (const auto& kv : exit_polish) {
     // Seeds are the working-space iterates the loop stopped at; adoption
     // follows pdhg.cpp:1243-1268 (neither feasibility measure worse, one
@@ -807,7 +839,11 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
       }
     }
   }
-  for (size_t i = 0; i < lps.size(); ++i) results[i].objective = lps[i]->model_objective(results[i].x);
+  lap(&Phases::polishing);
+  for (size_t i = 0; i < lps.size(); ++i) {
+    results[i].objective = lps[i]->model_objective(results[i].x);
+    results[i].phases = phases;
+  }
   return results;
 }
 

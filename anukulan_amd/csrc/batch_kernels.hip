@@ -58,28 +58,39 @@ __device__ __forceinline__ LaneScalars lane_scalars(const LaneParams& p, int ste
 }
 
 // Lane-serial row sum of A[row, :] v[:, lane] in storage order. The gathers
-// of four nonzeros are issued together so a long row is not a chain of
+// of up to eight nonzeros are issued together so a long row is not a chain of
 // memory latencies; the adds still happen one by one in storage order.
+template <int W>
+__device__ __forceinline__ void chunk_sum(const DevCsr& a, Int k, const double* __restrict__ v,
+                                          int lane, double& s) {
+  double w[W], x[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    w[u] = a.value[k + u];
+    x[u] = v[at(a.index[k + u], lane)];
+  }
+#pragma unroll
+  for (int u = 0; u < W; ++u) s += w[u] * x[u];
+}
+
 __device__ __forceinline__ double row_sum(const DevCsr& a, Int row, const double* __restrict__ v,
                                           int lane) {
   const Int lo = a.start[row], hi = a.start[row + 1];
   double s = 0.0;
   Int k = lo;
-  for (; k + 4 <= hi; k += 4) {
-    const double a0 = a.value[k], a1 = a.value[k + 1], a2 = a.value[k + 2], a3 = a.value[k + 3];
-    const double x0 = v[at(a.index[k], lane)];
-    const double x1 = v[at(a.index[k + 1], lane)];
-    const double x2 = v[at(a.index[k + 2], lane)];
-    const double x3 = v[at(a.index[k + 3], lane)];
-    s += a0 * x0;
-    s += a1 * x1;
-    s += a2 * x2;
-    s += a3 * x3;
+  for (; k + 8 <= hi; k += 8) chunk_sum<8>(a, k, v, lane, s);
+  if (k + 4 <= hi) {
+    chunk_sum<4>(a, k, v, lane, s);
+    k += 4;
   }
   for (; k < hi; ++k) s += a.value[k] * v[at(a.index[k], lane)];
   return s;
 }
 
+// The step kernels issue every load that does not depend on the row sum
+// (the lane's scalars, the epilogue operands) before the gathers, so the
+// wave waits for one round of memory latency rather than three. A retired
+// lane loads and computes with the rest of its wave and stores nothing.
 template <bool kDiag>
 __global__ void __launch_bounds__(256)
     bprimal_kernel(DevCsr kt, const double* __restrict__ y, BatchData d, PrimalBufs b,
@@ -87,20 +98,20 @@ __global__ void __launch_bounds__(256)
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int j = wave_row();
   if (j >= kt.rows) return;
+  const size_t o = at(j, lane);
   const LaneParams P = params[lane];
+  const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
+  const double kty = row_sum(kt, j, y, lane);
   if (!P.active) return;
   const LaneScalars sc = lane_scalars(P, step);
-  const double kty = row_sum(kt, j, y, lane);
-  const size_t o = at(j, lane);
-  const double xj = b.x[o];
-  double v = xj - sc.tau * (d.c[o] - kty);
-  if (v < d.lower[o]) v = d.lower[o];
-  if (v > d.upper[o]) v = d.upper[o];
+  double v = xj - sc.tau * (cj - kty);
+  if (v < lo) v = lo;
+  if (v > hi) v = hi;
   const double dd = v - xj;
   b.x_bar[o] = v + dd;
   double next = v;
   if (sc.gamma != 0.0) next += sc.gamma * dd;
-  if (sc.blend) next = sc.w * next + (1.0 - sc.w) * b.anchor[o];
+  if (sc.blend) next = sc.w * next + (1.0 - sc.w) * anc;
   b.x_out[o] = next;
   if (sc.open_epoch) b.anchor_out[o] = next;
   if constexpr (kDiag) b.dx[o] = dd;
@@ -113,14 +124,13 @@ __global__ void __launch_bounds__(256)
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int i = wave_row();
   if (i >= k.rows) return;
+  const size_t o = at(i, lane);
   const LaneParams P = params[lane];
+  const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
+  const double kxb = row_sum(k, i, x_bar, lane);
   if (!P.active) return;
   const LaneScalars sc = lane_scalars(P, step);
-  const double kxb = row_sum(k, i, x_bar, lane);
-  const size_t o = at(i, lane);
-  const double yi = b.y[o];
-  const double kx = b.k_x[o];
-  double v = yi + sc.sigma * (d.q[o] - kxb);
+  double v = yi + sc.sigma * (qi - kxb);
   if (i >= d.num_equalities && v < 0.0) v = 0.0;
   const double dd = v - yi;
   const double kd = 0.5 * (kxb - kx);
@@ -131,8 +141,8 @@ __global__ void __launch_bounds__(256)
     nk += sc.gamma * kd;
   }
   if (sc.blend) {
-    ny = sc.w * ny + (1.0 - sc.w) * b.anchor_y[o];
-    nk = sc.w * nk + (1.0 - sc.w) * b.anchor_kx[o];
+    ny = sc.w * ny + (1.0 - sc.w) * ay;
+    nk = sc.w * nk + (1.0 - sc.w) * ak;
   }
   b.y_out[o] = ny;
   b.k_out[o] = nk;
@@ -311,30 +321,51 @@ __global__ void __launch_bounds__(1024)
   out[static_cast<size_t>(lane) * kLaneCheckStride + (rows ? kRowSlot[v] : kColSlot[v])] = a;
 }
 
-__global__ void __launch_bounds__(128)
-    brestart_kernel(Int n, Int m, const double* __restrict__ x, const double* __restrict__ y,
-                    double* rx, double* ry, double* prx, double* pry, int use_previous,
-                    int advance, unsigned long long mask, double* __restrict__ out) {
+// Distances moved since a restart point, per lane, in the same fixed tree as
+// the check: per-wave runs of rows, waves in order, then blocks in order.
+// The restart points advance in the same pass when `advance` is set.
+__global__ void __launch_bounds__(256)
+    brestart_part_kernel(Int rows, Int rows_per_wave, const double* __restrict__ v, double* r,
+                         double* pr, int use_previous, int advance, unsigned long long mask,
+                         double* __restrict__ partials) {
+  __shared__ double lds[4][kLanes];
   const int lane = static_cast<int>(threadIdx.x & 63);
   const int wave = static_cast<int>(threadIdx.x >> 6);
-  if (!((mask >> lane) & 1ull)) return;
-  const Int len = wave == 0 ? n : m;
-  const double* v = wave == 0 ? x : y;
-  double* r = wave == 0 ? rx : ry;
-  double* pr = wave == 0 ? prx : pry;
+  const Int r0 = (static_cast<Int>(blockIdx.x) * 4 + wave) * rows_per_wave;
+  const Int r1 = min(r0 + rows_per_wave, rows);
   double acc = 0.0;
-  for (Int i = 0; i < len; ++i) {
-    const size_t p = at(i, lane);
-    const double cur = v[p];
-    const double base = use_previous ? pr[p] : r[p];
-    const double e = cur - base;
-    acc += e * e;
-    if (advance) {
-      if (pr != nullptr) pr[p] = r[p];
-      r[p] = cur;
+  if ((mask >> lane) & 1ull) {
+    for (Int i = r0; i < r1; ++i) {
+      const size_t p = at(i, lane);
+      const double cur = v[p];
+      const double e = cur - (use_previous ? pr[p] : r[p]);
+      acc += e * e;
+      if (advance) {
+        if (pr != nullptr) pr[p] = r[p];
+        r[p] = cur;
+      }
     }
   }
-  out[2 * lane + wave] = acc;
+  lds[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0) {
+    double a = lds[0][lane];
+    for (int w = 1; w < 4; ++w) a += lds[w][lane];
+    partials[static_cast<size_t>(blockIdx.x) * kLanes + lane] = a;
+  }
+}
+
+__global__ void __launch_bounds__(128)
+    brestart_fold_kernel(const double* __restrict__ px, Int bx, const double* __restrict__ py,
+                         Int by, unsigned long long mask, double* __restrict__ out) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int side = static_cast<int>(threadIdx.x >> 6);
+  if (!((mask >> lane) & 1ull)) return;
+  const double* part = side == 0 ? px : py;
+  const Int blocks = side == 0 ? bx : by;
+  double a = 0.0;
+  for (Int b = 0; b < blocks; ++b) a += part[static_cast<size_t>(b) * kLanes + lane];
+  out[2 * lane + side] = a;
 }
 
 __global__ void bextract_kernel(Int rows, const double* __restrict__ v, unsigned long long mask,
@@ -439,10 +470,24 @@ void launch_batch_check(hipStream_t s, const BatchData& d, const double* x, cons
 
 void launch_batch_restart(hipStream_t s, Int n, Int m, const double* x, const double* y,
                           double* rx, double* ry, double* prx, double* pry, bool use_previous,
-                          bool advance, unsigned long long mask, double* out) {
-  hipLaunchKernelGGL(brestart_kernel, dim3(1), dim3(128), 0, s, n, m, x, y, rx, ry, prx, pry,
-                     use_previous ? 1 : 0, advance ? 1 : 0, mask, out);
-  check_launch("batch_restart");
+                          bool advance, unsigned long long mask, double* scratch, double* out) {
+  double* px = scratch;
+  double* py = scratch + static_cast<size_t>(kCheckBlocks) * kLanes;
+  const Int bx = check_blocks(n), by = check_blocks(m);
+  if (bx > 0) {
+    hipLaunchKernelGGL(brestart_part_kernel, dim3(static_cast<unsigned>(bx)), dim3(256), 0, s, n,
+                       check_rows_per_wave(n), x, rx, prx, use_previous ? 1 : 0, advance ? 1 : 0,
+                       mask, px);
+    check_launch("batch_restart_x");
+  }
+  if (by > 0) {
+    hipLaunchKernelGGL(brestart_part_kernel, dim3(static_cast<unsigned>(by)), dim3(256), 0, s, m,
+                       check_rows_per_wave(m), y, ry, pry, use_previous ? 1 : 0, advance ? 1 : 0,
+                       mask, py);
+    check_launch("batch_restart_y");
+  }
+  hipLaunchKernelGGL(brestart_fold_kernel, dim3(1), dim3(128), 0, s, px, bx, py, by, mask, out);
+  check_launch("batch_restart_fold");
 }
 
 void launch_batch_extract(hipStream_t s, Int rows, const double* v, unsigned long long mask,

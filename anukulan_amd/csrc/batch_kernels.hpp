@@ -6,9 +6,11 @@
 // and the matrix itself (value, index, row start) is read once for all the
 // LPs with wave-uniform scalar loads. Each lane sums its own row serially in
 // storage order and runs the reference's elementwise steps in the
-// reference's order, so lane i computes exactly the bits a plain solve of LP
-// i computes (and the bits the reference CPU backend computes). The check
-// reductions are serial per lane in index order for the same reason.
+// reference's order, so every product and update of lane i has exactly the
+// bits the reference CPU backend computes for LP i. The reductions (check
+// terms, restart distances) fold each lane's rows in a fixed tree, which
+// agrees with the reference's serial sums to rounding and is the same on
+// every run.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -73,13 +75,14 @@ void launch_batch_check(hipStream_t s, const BatchData& d, const double* x, cons
                         const double* kt_y, const double* lambda, bool farkas,
                         const LaneParams* params, double* scratch, double* out);
 // For masked lanes: out[2 lane] = sum_j (x - base_x)^2 and out[2 lane + 1] =
-// sum_i (y - base_y)^2, serially, where the base is the last restart point
-// (rx, ry) or, with use_previous, the one before it (prx, pry). Then, when
-// `advance`, prx <- rx and rx <- x, pry <- ry and ry <- y (prx/pry may be
-// null when use_previous is false).
+// sum_i (y - base_y)^2 in a fixed tree, where the base is the last restart
+// point (rx, ry) or, with use_previous, the one before it (prx, pry). Then,
+// when `advance`, prx <- rx and rx <- x, pry <- ry and ry <- y (prx/pry may
+// be null when use_previous is false). `scratch` holds
+// batch_check_scratch_doubles() doubles.
 void launch_batch_restart(hipStream_t s, Int n, Int m, const double* x, const double* y,
                           double* rx, double* ry, double* prx, double* pry, bool use_previous,
-                          bool advance, unsigned long long mask, double* out);
+                          bool advance, unsigned long long mask, double* scratch, double* out);
 // Copies the columns of the masked lanes, in lane order, into dense
 // [count][rows] arrays.
 void launch_batch_extract(hipStream_t s, Int rows, const double* v, unsigned long long mask,

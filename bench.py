@@ -161,6 +161,7 @@ def main():
                    "cases_optimal": solved, "lane_iterations_per_step": lane_iters,
                    "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "plain": plain,
+        "batch_phases_s": {k: round(v, 4) for k, v in results[0].phases.items()},
         "kernels": {k: {"mean_us": v["mean_us"], "calls": v["calls"], "GB/s": v["gbs"]}
                     for k, v in kernels.items()},
     }

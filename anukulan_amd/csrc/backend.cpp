@@ -61,6 +61,20 @@ hipEvent_t HipBackend::take_event() {
   return e;
 }
 
+// Folds the elapsed times of an entry's recorded pairs into its totals and
+// returns the events to the pool. Waits for the stream.
+void HipBackend::harvest(ProfileEntry& e) const {
+  (void)hipStreamSynchronize(stream_);
+  for (auto& p : e.pairs) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, p.first, p.second) == hipSuccess) e.ms += static_cast<double>(t);
+    event_pool_.push_back(p.first);
+    event_pool_.push_back(p.second);
+  }
+  e.timed += static_cast<long>(e.pairs.size());
+  e.pairs.clear();
+}
+
 void HipBackend::clear_profile() {
   (void)hipStreamSynchronize(stream_);
   for (auto& kv : profile_) {
@@ -77,7 +91,7 @@ LaunchEvents HipBackend::launch_events(const char* name, double bytes) {
   if (!profiling_) return ev;
   auto& e = profile_[name];
   ++e.calls;
-  if (static_cast<int>(e.pairs.size()) >= kMaxTimed) return ev;
+  if (static_cast<int>(e.pairs.size()) >= kMaxTimed) harvest(e);
   ev.start = take_event();
   ev.stop = take_event();
   e.pairs.emplace_back(ev.start, ev.stop);
@@ -85,11 +99,12 @@ LaunchEvents HipBackend::launch_events(const char* name, double bytes) {
   return ev;
 }
 
-HipBackend::Timed::Timed(HipBackend& b, const char* name) : b_(b), name_(name) {
+HipBackend::Timed::Timed(HipBackend& b, const char* name, double bytes) : b_(b), name_(name) {
   if (!b_.profiling_) return;
   auto& e = b_.profile_[name_];
   ++e.calls;
-  if (static_cast<int>(e.pairs.size()) >= kMaxTimed) return;
+  e.bytes += bytes;
+  if (static_cast<int>(e.pairs.size()) >= kMaxTimed) b_.harvest(e);
   start_ = b_.take_event();
   hip_check(hipEventRecord(start_, b_.stream_), "hipEventRecord");
   active_ = true;
@@ -144,7 +159,7 @@ void HipBackend::fill(double* target, Int n, double value) {
 
 void HipBackend::copy(const double* source, double* target, Int n) {
   if (n <= 0) return;
-  Timed t(*this, "copy");
+  Timed t(*this, "copy", 16.0 * n);
   hip_check(hipMemcpyAsync(target, source, static_cast<size_t>(n) * sizeof(double),
                            hipMemcpyDeviceToDevice, stream_),
             "copy");
@@ -256,8 +271,11 @@ void HipBackend::release_one(std::uint64_t id) {
 
 void HipBackend::multiply(const CsrView& a, const double* x, double* y) {
   const DevCsr& d = prepare(a);
-  Timed t(*this, "spmv");
-  launch_spmv(stream_, d, x, y);
+  // Algorithmic bytes: value and index once, row starts once, x gathered
+  // once per column, y written once.
+  const double bytes = 12.0 * d.nnz + 4.0 * (d.rows + 1.0) + 8.0 * d.cols + 8.0 * d.rows;
+  const LaunchEvents ev = launch_events("spmv", bytes);
+  launch_spmv(stream_, d, x, y, profiling_ ? &ev : nullptr);
 }
 
 // ---- reductions ---------------------------------------------------------------
@@ -351,8 +369,8 @@ void HipBackend::set_profiling(bool on) {
   profiling_ = on;
 }
 
-// One line per launch group: calls, timed calls, mean microseconds of the
-// timed ones, and the estimated total (mean x calls) with its share.
+// One line per launch group: calls, timed calls (all of them), mean
+// microseconds, and the total (mean x calls) with its share.
 std::string HipBackend::profile_report() const {
   if (profile_.empty()) return {};
   (void)hipStreamSynchronize(stream_);
@@ -366,13 +384,10 @@ std::string HipBackend::profile_report() const {
   };
   std::vector<Row> rows;
   double total = 0.0;
-  for (const auto& kv : profile_) {
-    double ms = 0.0;
-    for (const auto& p : kv.second.pairs) {
-      float t = 0.0f;
-      if (hipEventElapsedTime(&t, p.first, p.second) == hipSuccess) ms += static_cast<double>(t);
-    }
-    const long timed = static_cast<long>(kv.second.pairs.size());
+  for (auto& kv : profile_) {
+    harvest(kv.second);
+    const double ms = kv.second.ms;
+    const long timed = kv.second.timed;
     const double mean_us = timed > 0 ? 1e3 * ms / static_cast<double>(timed) : 0.0;
     const double est = 1e-3 * mean_us * static_cast<double>(kv.second.calls);
     const double gbs = ms > 0.0 ? kv.second.bytes / (1e-3 * ms) / 1e9 : 0.0;

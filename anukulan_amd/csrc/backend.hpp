@@ -109,10 +109,11 @@ class HipBackend {
   // (its minimum traffic); null events when profiling is off.
   LaunchEvents launch_events(const char* name, double bytes = 0.0);
 
-  // Brackets a launch sequence for the profiler; no-ops when profiling is off.
+  // Brackets a launch sequence for the profiler, filed under `name` with the
+  // sequence's algorithmic bytes; no-ops when profiling is off.
   class Timed {
    public:
-    Timed(HipBackend& b, const char* name);
+    Timed(HipBackend& b, const char* name, double bytes = 0.0);
     ~Timed();
     Timed(const Timed&) = delete;
     Timed& operator=(const Timed&) = delete;
@@ -140,17 +141,20 @@ class HipBackend {
   double* result_dev_ = nullptr;
   double* result_host_ = nullptr;
   bool profiling_ = false;
-  // Event pairs recorded around launches without waiting; their times are
-  // read when the report is asked for. At most kMaxTimed pairs per name are
-  // kept, which bounds the event count on long solves.
+  // Every launch is timed by an event pair recorded without waiting. Once a
+  // name holds kMaxTimed outstanding pairs their times are read (one stream
+  // wait) and the events reused, which bounds the event count on long solves.
   static constexpr int kMaxTimed = 4096;
   struct ProfileEntry {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
     long calls = 0;
-    double bytes = 0.0;  // algorithmic bytes of the timed launches
+    long timed = 0;      // launches whose time is in `ms`
+    double ms = 0.0;
+    double bytes = 0.0;  // algorithmic bytes of all timed launches
   };
+  void harvest(ProfileEntry& e) const;
   mutable std::map<std::string, ProfileEntry> profile_;
-  std::vector<hipEvent_t> event_pool_;
+  mutable std::vector<hipEvent_t> event_pool_;
   hipEvent_t take_event();
   void clear_profile();
   hipEvent_t stop_event_ = nullptr;

@@ -671,7 +671,6 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
         lp->k.index != first.k.index || lp->k.value != first.k.value)
       throw std::invalid_argument("batch: every LP must share the same constraint matrix");
   }
-  const Int n = first.cols(), m = first.rows();
 
   Phases phases;
   Clock::time_point mark = start;

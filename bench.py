@@ -8,7 +8,8 @@ batched solver (all cases in one set of launches, sharing K) over the timed
 steps. Beside it: the same stack solved one case at a time by the plain GPU
 solver (`plain`), a profiled pass that times every kernel launch with HIP
 events the dispatch records (hipExtLaunchKernel) and divides each launch's
-algorithmic bytes by its time (`roofline`, dominant kernel), and the
+algorithmic bytes by its time (`roofline`, dominant kernel), the
+1M-column random LP's SpMV bandwidth and plain solve (`c1`), and the
 reference's own solve_pdhg (oracle/_ref, one thread) on a bounded sample of
 the cases (`cpu_baseline`).
 
@@ -72,6 +73,62 @@ def cpu_baseline(base, cases, tol, budget_s):
                       f"(CPU backend, one thread): {iters} iterations in {spent:.1f} s"}
 
 
+def c1_section(ctx, args):
+    """The 1M-column random LP (m=400k, n=1M, ~8M nnz): SpMV bandwidth of
+    K x and K'y on the device, then a plain solve to rel KKT 1e-4 and a
+    profiled run of its fused steps. Bytes per launch are algorithmic."""
+    import numpy as np
+    from anukulan_amd import Matrix, generators as G, solve_pdhg
+
+    t0 = time.perf_counter()
+    lp = G.random_lp(m=args.c1_rows, n=args.c1_cols).to_lp()
+    e = lp.export()
+    k = Matrix(lp.rows, lp.cols, e["start"], e["index"], e["value"])
+    kt = k.transpose()
+    build_s = time.perf_counter() - t0
+    rng = np.random.default_rng(5)
+    x, y = ctx.vector(rng.uniform(-1, 1, lp.cols)), ctx.allocate(lp.rows)
+    yv, xv = ctx.vector(rng.uniform(-1, 1, lp.rows)), ctx.allocate(lp.cols)
+    ctx.prepare(k)
+    ctx.prepare(kt)
+    ctx.multiply(k, x, y)
+    ctx.multiply_transpose(kt, yv, xv)
+    ctx.synchronize()
+    spmv = {}
+    for name, fn in (("K x", lambda: ctx.multiply(k, x, y)),
+                     ("K'y", lambda: ctx.multiply_transpose(kt, yv, xv))):
+        ctx.set_profiling(True)
+        for _ in range(args.c1_reps):
+            fn()
+        row = parse_profile(ctx.profile_report())["spmv"]
+        ctx.set_profiling(False)
+        spmv[name] = {"mean_us": row["mean_us"], "achieved_GBs": round(row["gbs"], 1),
+                      "frac": round(row["gbs"] / HBM_PEAK_GBS, 4), "launches": row["timed"]}
+    for v in (x, y, yv, xv):
+        v.free()
+    ctx.release()
+
+    t0 = time.perf_counter()
+    r = solve_pdhg(ctx, lp, tolerance=1e-4, time_limit_seconds=args.c1_seconds)
+    solve_s = time.perf_counter() - t0
+    ctx.set_profiling(True)
+    solve_pdhg(ctx, lp, tolerance=1e-4, max_iterations=min(max(r.iterations, 80), 2000),
+               use_graphs=0)
+    kern = parse_profile(ctx.profile_report())
+    ctx.set_profiling(False)
+    steps = {name: {"mean_us": kern[name]["mean_us"], "achieved_GBs": round(kern[name]["gbs"], 1),
+                    "frac": round(kern[name]["gbs"] / HBM_PEAK_GBS, 4)}
+             for name in ("primal_step", "dual_step") if name in kern}
+    it_s = r.phases.get("iterating", 0.0) + r.phases.get("checks", 0.0)
+    return {"workload": f"random LP m={lp.rows} n={lp.cols} nnz={lp.nnz} (Zipf column counts, "
+                        f"seed 20261204), plain PDHG to rel KKT 1e-4",
+            "build_s": round(build_s, 2), "spmv": spmv, "fused_steps": steps,
+            "solve": {"status": r.status, "iterations": r.iterations,
+                      "objective": r.objective, "seconds": round(solve_s, 3),
+                      "iterations_per_s": round(r.iterations / it_s, 1) if it_s > 0 else None,
+                      "phases_s": {k2: round(v, 4) for k2, v in r.phases.items()}}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +144,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly (for external profilers)")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 1M-column random LP")
+    ap.add_argument("--c1-rows", type=int, default=400_000)
+    ap.add_argument("--c1-cols", type=int, default=1_000_000)
+    ap.add_argument("--c1-reps", type=int, default=50)
+    ap.add_argument("--c1-seconds", type=float, default=60.0,
+                    help="time limit of the random LP's plain solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -144,6 +207,8 @@ def main():
                         "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                         "mean_launch_us": k["mean_us"], "launches_timed": k["timed"]}
 
+    c1 = None if args.no_c1 else c1_section(ctx, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(base, cases, args.tol, args.cpu_seconds)
@@ -160,7 +225,7 @@ def main():
                    "rows": m, "cols": n, "nnz": nnz, "cases": len(lps),
                    "cases_optimal": solved, "lane_iterations_per_step": lane_iters,
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "plain": plain,
+        "roofline": roofline, "cpu_baseline": cpu, "plain": plain, "c1": c1,
         "batch_phases_s": {k: round(v, 4) for k, v in results[0].phases.items()},
         "kernels": {k: {"mean_us": v["mean_us"], "calls": v["calls"], "GB/s": v["gbs"]}
                     for k, v in kernels.items()},

@@ -298,6 +298,7 @@ class RefLp:
         ref.ref_lp_info(self.h, byref(info))
         self.rows, self.cols, self.nnz = info.rows, info.cols, info.nnz
         self.num_equalities = info.num_equalities
+        self.objective_scale, self.objective_offset = info.objective_scale, info.objective_offset
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -336,6 +337,8 @@ class RefLp:
                           _dp(o["row_sign"]))
         o["index"], o["value"] = o["index"][:self.nnz], o["value"][:self.nnz]
         o["num_equalities"] = self.num_equalities
+        o["objective_scale"] = self.objective_scale
+        o["objective_offset"] = self.objective_offset
         return o
 
     def scaled(self, ruiz=10, pc=True):

@@ -6,7 +6,7 @@ the seeded refinery plans, and records what parity is judged against:
 status, iterations, restarts, polish counts, objective, ||K|| and the
 simplex optimum. Run in the container that has the reference sources:
 
-    make -C oracle && python tests/golden/make_fixtures.py
+    make -C oracle && python tests/golden/make_fixtures.py [--only random]
 """
 import json
 import os
@@ -30,10 +30,35 @@ def record(rlp, tol, **opts):
             "final_primal_weight": s.final_primal_weight}
 
 
+# (rows, cols, tolerances, CPU threads). The threaded CPU backend gives the
+# serial backend's bits at any thread count (threaded_backend.cpp), so the
+# full-size LP is solved on all cores.
+RANDOM = ((2000, 5000, ("1e-4", "1e-8"), 1), (10000, 25000, ("1e-4",), 1),
+          (400000, 1000000, ("1e-4",), 8))
+
+
+def random_section():
+    """Seeded random LPs (generators.random_lp), up to the 1M-column LP."""
+    out = {}
+    for m, n, tols, threads in RANDOM:
+        rlp = _oracle.RefLp.from_natural(G.random_lp(m=m, n=n))
+        out[f"{m}x{n}"] = {"seed": 20261204, "rows": rlp.rows, "cols": rlp.cols, "nnz": rlp.nnz,
+                           **{t: record(rlp, float(t), threads=threads) for t in tols}}
+    return out
+
+
 def main():
     if _oracle.ref is None:
         sys.exit("oracle/_ref/libanukulan_ref.so is not built")
-    out = {"known": {}, "refinery": {}}
+    path = os.path.join(HERE, "pdhg_reference.json")
+    if sys.argv[1:] == ["--only", "random"]:
+        out = json.load(open(path))
+        out["random"] = random_section()
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+        print("updated", path)
+        return
+    out = {"known": {}, "refinery": {}, "random": random_section()}
     for name in KNOWN:
         rlp = _oracle.RefLp.from_natural(natural(name))
         out["known"][name] = {"1e-8": record(rlp, 1e-8), "1e-4": record(rlp, 1e-4),
@@ -45,7 +70,6 @@ def main():
             "seed": 20261203, "rows": rlp.rows, "cols": rlp.cols, "nnz": rlp.nnz,
             "1e-4": record(rlp, 1e-4), "1e-8": record(rlp, 1e-8),
             "simplex_status": sx["status"], "simplex_objective": sx["objective"]}
-    path = os.path.join(HERE, "pdhg_reference.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print("wrote", path)

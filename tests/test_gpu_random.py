@@ -1,0 +1,99 @@
+"""The random-LP workload on the GPU: seeded LPs from generators.random_lp.
+
+At sizes the CPU solves in seconds the plain GPU solve is held to the
+reference's fixture (tests/golden/pdhg_reference.json, written by the
+reference's own solve_pdhg): same status, objective within 1e-8 relative,
+iterations within 2%. At the full 400,000 x 1,000,000 size the products are
+checked against the oracle's serial CSR loop (bit-exact on every row of up
+to 1024 nonzeros, which the LDS path sums in storage order; rounding-level on
+the longer rows, which a workgroup folds in a fixed tree), and the solve is
+checked against the reference's fixture for that size (solved on the
+threaded CPU backend, which has the serial backend's bits) and by its own
+residual.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from anukulan_amd import Matrix, generators as G, solve_pdhg
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "pdhg_reference.json")))["random"]
+
+
+def relerr(a, b):
+    return abs(a - b) / max(1.0, abs(b))
+
+
+CASES = [(k, t) for k, v in sorted(FIXTURE.items()) for t in ("1e-4", "1e-8")
+         if t in v and v["cols"] < 1_000_000]
+
+
+@pytest.mark.parametrize("key,tol", CASES)
+def test_random_lp_matches_the_reference(ctx, key, tol):
+    fx = FIXTURE[key]
+    m, n = (int(v) for v in key.split("x"))
+    lp = G.random_lp(m=m, n=n).to_lp()
+    assert (lp.rows, lp.cols, lp.nnz) == (fx["rows"], fx["cols"], fx["nnz"])
+    want = fx[tol]
+    got = solve_pdhg(ctx, lp, tolerance=float(tol))
+    print(f"\n{key} {tol}: {got.iterations} iterations (reference {want['iterations']}), "
+          f"objective {got.objective!r} (reference {want['objective']!r})")
+    assert got.status == want["status"]
+    assert relerr(got.objective, want["objective"]) <= 1e-8
+    assert abs(got.iterations - want["iterations"]) <= 0.02 * want["iterations"]
+    assert abs(got.matrix_norm_estimate - want["matrix_norm"]) <= 1e-12 * want["matrix_norm"]
+
+
+@pytest.fixture(scope="module")
+def big():
+    lp = G.random_lp().to_lp()
+    e = lp.export()
+    k = Matrix(lp.rows, lp.cols, e["start"], e["index"], e["value"])
+    return lp, k, k.transpose()
+
+
+def check_product(ctx, orc, a: Matrix, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, a.cols)
+    dx, dy = ctx.vector(x), ctx.allocate(a.rows)
+    ctx.multiply(a, dx, dy)
+    got = dy.download()
+    dx.free()
+    dy.free()
+    want = orc.spmv(a.start, a.index, a.value, x)
+    lens = np.diff(a.start)
+    short = lens <= 1024
+    assert np.array_equal(got[short], want[short])
+    if (~short).any():
+        scale = np.maximum(1.0, np.abs(want[~short]))
+        assert np.max(np.abs(got[~short] - want[~short]) / scale) <= 1e-12
+    return int((~short).sum())
+
+
+def test_products_on_the_full_size_lp(ctx, orc, big):
+    lp, k, kt = big
+    assert (lp.rows, lp.cols) == (400_000, 1_000_000)
+    assert 7_000_000 <= lp.nnz <= 9_000_000
+    long_k = check_product(ctx, orc, k, 1)
+    long_kt = check_product(ctx, orc, kt, 2)
+    print(f"\nrows of K over 1024 nnz: {long_k}; rows of K' over 1024 nnz: {long_kt}")
+    ctx.release()
+
+
+def test_full_size_lp_solves_to_tolerance(ctx, big):
+    lp, _, _ = big
+    r = solve_pdhg(ctx, lp, tolerance=1e-4, time_limit_seconds=90.0)
+    print(f"\n{r.iterations} iterations, {r.solve_seconds:.2f} s, objective {r.objective!r}")
+    assert r.status == "optimal"
+    res = lp.evaluate_residual(r.x, r.y)
+    assert res["relative_primal"] <= 1e-4 and res["relative_dual"] <= 1e-4
+    assert res["relative_gap"] <= 1e-4
+    fx = FIXTURE.get("400000x1000000", {}).get("1e-4")
+    if fx:
+        assert relerr(r.objective, fx["objective"]) <= 1e-8
+        assert abs(r.iterations - fx["iterations"]) <= 0.02 * fx["iterations"]

@@ -55,7 +55,6 @@ def test_csr_build_transpose_and_product(orc, ref):
         assert np.all(val != 0.0)
         assert np.count_nonzero(d) == len(val)
         # reference build: identical arrays
-        rl = O.RefLp.from_standard if False else None  # noqa: F841
         ts, ti, tv = O.transpose(rows, cols, s, i, val)
         rts, rti, rtv = np.zeros(cols + 1, np.int32), np.zeros(len(i), np.int32), np.zeros(len(i))
         O.ref.ref_op_transpose(rows, cols, O._ip(O.i32(s)), O._ip(O.i32(i)), O._dp(val),
@@ -140,7 +139,6 @@ def test_norms_and_step_terms(orc, ref):
         assert w == O.ref.ref_op_weighted_norm_squared(n, m, O._dp(dx), O._dp(dy), 2.5)
         assert math.isclose(w, 2.5 * math.fsum(dx * dx) + math.fsum(dy * dy) / 2.5, rel_tol=1e-12)
         i1, m1 = O.step_size_terms(dx, dy, kdx, 2.5)
-        ri, rm = O.ctypes_pair() if hasattr(O, "ctypes_pair") else (None, None)
         from ctypes import byref, c_double
         a, b = c_double(), c_double()
         O.ref.ref_op_step_size_terms(n, m, O._dp(dx), O._dp(dy), O._dp(kdx), 2.5, byref(a), byref(b))

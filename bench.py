@@ -73,6 +73,19 @@ def cpu_baseline(base, cases, tol, budget_s):
                       f"(CPU backend, one thread): {iters} iterations in {spent:.1f} s"}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/pmc_traffic.txt), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 2 and not line.startswith("#") and parts[0] == kernel:
+            return float(parts[1])
+    return None
+
+
 def c1_section(ctx, args):
     """The 1M-column random LP (m=400k, n=1M, ~8M nnz): SpMV bandwidth of
     K x and K'y on the device, then a plain solve to rel KKT 1e-4 and a
@@ -109,7 +122,8 @@ def c1_section(ctx, args):
     ctx.release()
 
     t0 = time.perf_counter()
-    r = solve_pdhg(ctx, lp, tolerance=1e-4, time_limit_seconds=args.c1_seconds)
+    r = solve_pdhg(ctx, lp, tolerance=1e-4, time_limit_seconds=args.c1_seconds,
+                   use_graphs=0 if args.no_graphs else 1)
     solve_s = time.perf_counter() - t0
     ctx.set_profiling(True)
     solve_pdhg(ctx, lp, tolerance=1e-4, max_iterations=min(max(r.iterations, 80), 2000),
@@ -145,6 +159,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly (for external profilers)")
     ap.add_argument("--no-c1", action="store_true", help="skip the 1M-column random LP")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB and exit: the "
+                         "known-size reference for HBM byte counters")
     ap.add_argument("--c1-rows", type=int, default=400_000)
     ap.add_argument("--c1-cols", type=int, default=1_000_000)
     ap.add_argument("--c1-reps", type=int, default=50)
@@ -158,6 +175,20 @@ def main():
         sys.exit(2)
 
     from anukulan_amd import Context, solve_pdhg, solve_pdhg_batch
+
+    if args.copy_probe_mb:
+        ctx = Context(0)
+        n = args.copy_probe_mb * (1 << 20) // 8
+        a, b = ctx.allocate(n), ctx.allocate(n)
+        ctx.set_profiling(True)
+        ctx.copy(a, b)
+        row = parse_profile(ctx.profile_report())["copy"]
+        print(json.dumps({"copy_probe_bytes": 8 * n, "mean_us": row["mean_us"],
+                          "GB/s": row["gbs"]}))
+        a.free()
+        b.free()
+        ctx.close()
+        return
 
     base, cases, lps = build_stack(args.periods, args.cases)
     n, m, nnz = lps[0].cols, lps[0].rows, lps[0].nnz
@@ -204,7 +235,9 @@ def main():
             name, k = ranked[0]
             roofline = {"bound": "hbm", "kernel": name, "achieved": round(k["gbs"], 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                        "frac": round(k["gbs"] / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic(name),
+                        "traffic_source": "profiles/pmc_traffic.txt (rocprofv3 --pmc, calibrated)",
                         "mean_launch_us": k["mean_us"], "launches_timed": k["timed"]}
 
     c1 = None if args.no_c1 else c1_section(ctx, args)

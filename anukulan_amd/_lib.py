@@ -89,6 +89,17 @@ class PdhgResult(ctypes.Structure):
     ]
 
 
+class AdjointOptions(ctypes.Structure):
+    _fields_ = [("method", c_int32), ("max_iterations", c_int32), ("tolerance", c_double),
+                ("active_tolerance", c_double), ("primal_weight", c_double)]
+
+
+class AdjointInfo(ctypes.Structure):
+    _fields_ = [("iterations", c_int32), ("converged", c_int32), ("residual_primal", c_double),
+                ("residual_dual", c_double), ("free_columns", c_int32), ("binding_rows", c_int32),
+                ("lower_columns", c_int32), ("upper_columns", c_int32)]
+
+
 # name -> (restype, argtypes); every function the header declares.
 SIGNATURES = {
     "anuk_create": (c_int, [c_int, POINTER(c_void_p)]),
@@ -131,6 +142,12 @@ SIGNATURES = {
     "anuk_solve_pdhg": (c_int, [c_void_p, c_void_p, POINTER(PdhgOptions), POINTER(PdhgResult), DP, DP]),
     "anuk_solve_pdhg_batch": (c_int, [c_void_p, POINTER(c_void_p), c_int32, POINTER(PdhgOptions),
                                       POINTER(PdhgResult), DP, DP]),
+    "anuk_solve_pdhg_multi": (c_int, [c_void_p, POINTER(c_void_p), c_int32, POINTER(PdhgOptions),
+                                      POINTER(PdhgResult), DP, DP]),
+    "anuk_adjoint_default_options": (None, [POINTER(AdjointOptions)]),
+    "anuk_pdhg_adjoint": (c_int, [c_void_p, POINTER(c_void_p), c_int32, POINTER(PdhgOptions),
+                                  POINTER(AdjointOptions), DP, DP, DP, DP, DP, DP, DP, DP,
+                                  POINTER(AdjointInfo)]),
     "anuk_estimate_matrix_norm": (c_int, [c_void_p, c_void_p, c_int32, c_double, DP]),
     "anuk_evaluate_residual": (c_int, [c_void_p, DP, DP, POINTER(PdhgResidual)]),
 }

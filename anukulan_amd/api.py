@@ -394,7 +394,121 @@ def solve_pdhg_batch(ctx: Context, lps, options: Optional[_lib.PdhgOptions] = No
     return [result_from(results[i], xs[i].copy(), ys[i].copy()) for i in range(count)]
 
 
+def solve_pdhg_multi(ctx: Context, lps, options: Optional[_lib.PdhgOptions] = None,
+                     **overrides) -> list:
+    """Solves independent LPs of any size (each with its own K) in one set of
+    launches over their block-diagonal stack; result i is what
+    solve_pdhg(ctx, lps[i], ...) reaches."""
+    lps = list(lps)
+    if not lps:
+        return []
+    o = options if options is not None else default_options(**overrides)
+    if options is not None and overrides:
+        for k, v in overrides.items():
+            setattr(o, k, TERMINATION[v] if k == "termination" and isinstance(v, str) else v)
+    count = len(lps)
+    handles = (c_void_p * count)(*[lp.h for lp in lps])
+    results = (_lib.PdhgResult * count)()
+    xs = np.empty(sum(lp.cols for lp in lps))
+    ys = np.empty(sum(lp.rows for lp in lps))
+    ctx._call(lib.anuk_solve_pdhg_multi, handles, count, byref(o), results, _dp(xs), _dp(ys))
+    out, xo, yo = [], 0, 0
+    for i, lp in enumerate(lps):
+        out.append(result_from(results[i], xs[xo:xo + lp.cols].copy(), ys[yo:yo + lp.rows].copy()))
+        xo += lp.cols
+        yo += lp.rows
+    return out
+
+
 def estimate_matrix_norm(ctx: Context, lp: LP, iterations=200, tolerance=1e-8) -> float:
     out = c_double()
     ctx._call(lib.anuk_estimate_matrix_norm, lp.h, int(iterations), float(tolerance), byref(out))
     return out.value
+
+
+# ---- sensitivities -----------------------------------------------------------
+
+ADJOINT_METHODS = {"cg": 0, "fixed_point": 1}
+
+
+@dataclass
+class AdjointResult:
+    """Gradients of L = w_x'x + w_y'y with respect to one LP's standard-form
+    data (c, q, lower, upper), and how the adjoint solve went."""
+    grad_c: np.ndarray
+    grad_q: np.ndarray
+    grad_lower: np.ndarray
+    grad_upper: np.ndarray
+    iterations: int
+    converged: bool
+    residual_primal: float
+    residual_dual: float
+    free_columns: int
+    binding_rows: int
+    lower_columns: int
+    upper_columns: int
+
+    def model_gradients(self, lp: LP, model_rows: int) -> dict:
+        """The same gradients with respect to the natural model's data:
+        objective, row_lower, row_upper, col_lower, col_upper. A standard row
+        is sign * (model row) with q = sign * (the bound it came from); for an
+        equality row both row_lower and row_upper report dL/db for its value b,
+        which the two bounds share."""
+        e = lp.export()
+        lo, up = np.zeros(model_rows), np.zeros(model_rows)
+        for r in range(lp.rows):
+            i, sign, g = int(e["row_model"][r]), float(e["row_sign"][r]), float(self.grad_q[r])
+            if r < lp.num_equalities:
+                lo[i] += sign * g
+                up[i] += sign * g
+            elif sign > 0:
+                lo[i] += g
+            else:
+                up[i] -= g
+        return {"objective": lp.objective_scale * self.grad_c, "row_lower": lo, "row_upper": up,
+                "col_lower": self.grad_lower.copy(), "col_upper": self.grad_upper.copy()}
+
+
+def adjoint_options(**overrides) -> _lib.AdjointOptions:
+    o = _lib.AdjointOptions()
+    lib.anuk_adjoint_default_options(byref(o))
+    for k, v in overrides.items():
+        if k == "method" and isinstance(v, str):
+            v = ADJOINT_METHODS[v]
+        if not hasattr(o, k):
+            raise KeyError(f"unknown adjoint option {k}")
+        setattr(o, k, v)
+    return o
+
+
+def pdhg_adjoint(ctx: Context, lps, xs, ys, w_x=None, w_y=None,
+                 scaling: Optional[_lib.PdhgOptions] = None, **overrides) -> list:
+    """Gradients of L = w_x'x + w_y'y at PDHG solutions (xs[i], ys[i]) of LPs
+    that share K, one wave64 lane per LP. A single LP may be passed bare."""
+    single = isinstance(lps, LP)
+    if single:
+        lps, xs, ys = [lps], [xs], [ys]
+        w_x = None if w_x is None else [w_x]
+        w_y = None if w_y is None else [w_y]
+    lps = list(lps)
+    count = len(lps)
+    if count == 0:
+        return []
+    n, m = lps[0].cols, lps[0].rows
+    X = np.ascontiguousarray(np.asarray(xs, dtype=np.float64).reshape(count, n))
+    Y = np.ascontiguousarray(np.asarray(ys, dtype=np.float64).reshape(count, m))
+    WX = None if w_x is None else np.ascontiguousarray(np.asarray(w_x, dtype=np.float64).reshape(count, n))
+    WY = None if w_y is None else np.ascontiguousarray(np.asarray(w_y, dtype=np.float64).reshape(count, m))
+    gc, gl, gu = np.zeros((count, n)), np.zeros((count, n)), np.zeros((count, n))
+    gq = np.zeros((count, m))
+    info = (_lib.AdjointInfo * count)()
+    sc = scaling if scaling is not None else default_options()
+    o = adjoint_options(**overrides)
+    handles = (c_void_p * count)(*[lp.h for lp in lps])
+    ctx._call(lib.anuk_pdhg_adjoint, handles, count, byref(sc), byref(o), _dp(X), _dp(Y),
+              _dp(WX), _dp(WY), _dp(gc), _dp(gq), _dp(gl), _dp(gu), info)
+    out = [AdjointResult(gc[i].copy(), gq[i].copy(), gl[i].copy(), gu[i].copy(),
+                         info[i].iterations, bool(info[i].converged), info[i].residual_primal,
+                         info[i].residual_dual, info[i].free_columns, info[i].binding_rows,
+                         info[i].lower_columns, info[i].upper_columns) for i in range(count)]
+    return out[0] if single else out

@@ -2,6 +2,7 @@
 // -ffp-contract=off like the plain kernels: each lane's arithmetic is the
 // reference's, operation for operation.
 #include "batch_kernels.hpp"
+#include "lanes.hpp"
 
 #include <hip/hip_ext.h>
 
@@ -12,22 +13,10 @@ namespace anuk {
 
 namespace {
 
-constexpr int kWavesPerBlock = 4;  // 256 threads, one row per wave
-
-__device__ __forceinline__ int xcd_remap(int i, int g) {
-  const int x = i & 7, local = i >> 3, per = g >> 3, rem = g & 7;
-  return x * per + (x < rem ? x : rem) + local;
-}
-
-// The row this wave owns, made wave-uniform so the matrix loads are scalar.
-__device__ __forceinline__ Int wave_row() {
-  const int blk = xcd_remap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
-  return __builtin_amdgcn_readfirstlane(blk * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6));
-}
-
-__device__ __forceinline__ size_t at(Int row, int lane) {
-  return static_cast<size_t>(row) * kLanes + static_cast<size_t>(lane);
-}
+using lanes::at;
+using lanes::kWavesPerBlock;
+using lanes::row_sum;
+using lanes::wave_row;
 
 struct LaneScalars {
   double tau, sigma, gamma, w;
@@ -54,36 +43,6 @@ __device__ __forceinline__ LaneScalars lane_scalars(const LaneParams& p, int ste
     s.open_epoch = false;
     s.w = static_cast<double>(k + 1) / static_cast<double>(k + 2);
   }
-  return s;
-}
-
-// Lane-serial row sum of A[row, :] v[:, lane] in storage order. The gathers
-// of up to eight nonzeros are issued together so a long row is not a chain of
-// memory latencies; the adds still happen one by one in storage order.
-template <int W>
-__device__ __forceinline__ void chunk_sum(const DevCsr& a, Int k, const double* __restrict__ v,
-                                          int lane, double& s) {
-  double w[W], x[W];
-#pragma unroll
-  for (int u = 0; u < W; ++u) {
-    w[u] = a.value[k + u];
-    x[u] = v[at(a.index[k + u], lane)];
-  }
-#pragma unroll
-  for (int u = 0; u < W; ++u) s += w[u] * x[u];
-}
-
-__device__ __forceinline__ double row_sum(const DevCsr& a, Int row, const double* __restrict__ v,
-                                          int lane) {
-  const Int lo = a.start[row], hi = a.start[row + 1];
-  double s = 0.0;
-  Int k = lo;
-  for (; k + 8 <= hi; k += 8) chunk_sum<8>(a, k, v, lane, s);
-  if (k + 4 <= hi) {
-    chunk_sum<4>(a, k, v, lane, s);
-    k += 4;
-  }
-  for (; k < hi; ++k) s += a.value[k] * v[at(a.index[k], lane)];
   return s;
 }
 

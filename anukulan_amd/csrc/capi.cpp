@@ -11,7 +11,9 @@
 #include "../../include/anukulan_hip.h"
 #include "backend.hpp"
 #include "lp.hpp"
+#include "adjoint.hpp"
 #include "batch.hpp"
+#include "multi.hpp"
 #include "pdhg.hpp"
 
 struct anuk_ctx {
@@ -509,6 +511,93 @@ int anuk_solve_pdhg_batch(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t coun
     });
   }();
   return rc;
+}
+
+int anuk_solve_pdhg_multi(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,
+                          const anuk_pdhg_options* options, anuk_pdhg_result* results,
+                          double* x_out, double* y_out) {
+  if (ctx == nullptr || lps == nullptr || results == nullptr || count < 0) return ANUK_ERR_INVALID;
+  anuk_pdhg_options o;
+  const int g = guarded(ctx, [&] { o = checked_options(options); });
+  if (g != ANUK_OK) return g;
+  const char* why = nullptr;
+  if (!anuk::batch_supports(o, &why)) {
+    ctx->error = std::string("multi-LP solve does not run: ") + why;
+    return static_cast<int>(ANUK_ERR_UNSUPPORTED);
+  }
+  return guarded(ctx, [&] {
+    std::vector<const anuk::StandardLp*> v;
+    for (int32_t i = 0; i < count; ++i) {
+      need(lps[i] != nullptr, "null LP in the stack");
+      v.push_back(&lps[i]->lp);
+    }
+    const std::vector<anuk::SolveResult> rs = anuk::solve_pdhg_multi(*ctx->be, v, o);
+    size_t xo = 0, yo = 0;
+    for (size_t i = 0; i < rs.size(); ++i) {
+      fill_result(rs[i], &results[i]);
+      const size_t cols = static_cast<size_t>(v[i]->cols()), rows = static_cast<size_t>(v[i]->rows());
+      if (x_out && rs[i].x.size() == cols) std::memcpy(x_out + xo, rs[i].x.data(), cols * sizeof(double));
+      if (y_out && rs[i].y.size() == rows) std::memcpy(y_out + yo, rs[i].y.data(), rows * sizeof(double));
+      xo += cols;
+      yo += rows;
+    }
+  });
+}
+
+void anuk_adjoint_default_options(anuk_adjoint_options* options) {
+  if (options != nullptr) anuk::default_adjoint_options(options);
+}
+
+int anuk_pdhg_adjoint(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,
+                      const anuk_pdhg_options* scaling, const anuk_adjoint_options* options,
+                      const double* x, const double* y, const double* w_x, const double* w_y,
+                      double* grad_c, double* grad_q, double* grad_lower, double* grad_upper,
+                      anuk_adjoint_info* info) {
+  if (ctx == nullptr || lps == nullptr || count < 0 || x == nullptr || y == nullptr)
+    return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    const anuk_pdhg_options sc = checked_options(scaling);
+    anuk_adjoint_options o;
+    anuk::default_adjoint_options(&o);
+    if (options != nullptr) o = *options;
+    std::vector<const anuk::StandardLp*> v;
+    std::vector<const double*> xs, ys, wx, wy;
+    size_t col_off = 0, row_off = 0;
+    for (int32_t i = 0; i < count; ++i) {
+      need(lps[i] != nullptr, "null LP in the adjoint batch");
+      const anuk::StandardLp& lp = lps[i]->lp;
+      v.push_back(&lp);
+      xs.push_back(x + col_off);
+      ys.push_back(y + row_off);
+      wx.push_back(w_x ? w_x + col_off : nullptr);
+      wy.push_back(w_y ? w_y + row_off : nullptr);
+      col_off += static_cast<size_t>(lp.cols());
+      row_off += static_cast<size_t>(lp.rows());
+    }
+    const std::vector<anuk::AdjointOutcome> out = anuk::pdhg_adjoint(*ctx->be, v, sc, o, xs, ys, wx, wy);
+    col_off = row_off = 0;
+    for (size_t i = 0; i < out.size(); ++i) {
+      const anuk::AdjointOutcome& a = out[i];
+      const size_t n = a.grad_c.size(), m = a.grad_q.size();
+      if (grad_c) std::memcpy(grad_c + col_off, a.grad_c.data(), n * sizeof(double));
+      if (grad_q) std::memcpy(grad_q + row_off, a.grad_q.data(), m * sizeof(double));
+      if (grad_lower) std::memcpy(grad_lower + col_off, a.grad_lower.data(), n * sizeof(double));
+      if (grad_upper) std::memcpy(grad_upper + col_off, a.grad_upper.data(), n * sizeof(double));
+      if (info) {
+        anuk_adjoint_info& f = info[i];
+        f.iterations = a.iterations;
+        f.converged = a.converged ? 1 : 0;
+        f.residual_primal = a.residual_primal;
+        f.residual_dual = a.residual_dual;
+        f.free_columns = a.free_columns;
+        f.binding_rows = a.binding_rows;
+        f.lower_columns = a.lower_columns;
+        f.upper_columns = a.upper_columns;
+      }
+      col_off += n;
+      row_off += m;
+    }
+  });
 }
 
 int anuk_solve_pdhg(anuk_ctx* ctx, const anuk_lp* lp, const anuk_pdhg_options* options,

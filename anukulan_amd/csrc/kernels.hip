@@ -82,9 +82,9 @@ __global__ void __launch_bounds__(kThreads)
     spmv_kernel(DevCsr a, const double* __restrict__ x, Epi epi, double* __restrict__ partials) {
   __shared__ double tile[kTileNnz];
   constexpr int R = Epi::kR > 0 ? Epi::kR : 1;
-  const Epi e = epi.resolve();
-  if (e.skip()) return;  // uniform across the grid
   const int b = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  const Epi e = epi.resolve(b);
+  if (e.skip()) return;  // uniform across the block
   const Int r0 = a.block_row[b], r1 = a.block_row[b + 1];
   const Int k0 = a.start[r0], k1 = a.start[r1];
   const Int len = k1 - k0;
@@ -122,9 +122,25 @@ struct PlainEpi {
   static constexpr int kR = 0;
   static constexpr unsigned kMaxMask = 0;
   double* y;
-  __device__ PlainEpi resolve() const { return *this; }
+  __device__ PlainEpi resolve(int) const { return *this; }
   __device__ bool skip() const { return false; }
   __device__ void operator()(Int r, double s, double*) const { y[r] = s; }
+};
+
+// Where a fused epilogue finds its step scalars: one IterParams for the
+// whole matrix (one LP), or, for a block-diagonal stack of LPs whose row
+// blocks never straddle two LPs, one per LP picked by the block's LP.
+struct OneLp {
+  const IterParams* params;
+  __device__ int lp(int) const { return 0; }
+  __device__ const IterParams* get(int) const { return params; }
+};
+
+struct ManyLps {
+  const IterParams* params;
+  const Int* block_lp;
+  __device__ int lp(int b) const { return block_lp[b]; }
+  __device__ const IterParams* get(int b) const { return params + block_lp[b]; }
 };
 
 // Halpern schedule of one step of a period, from the period's starting state:
@@ -133,6 +149,7 @@ struct PlainEpi {
 struct StepScalars {
   double tau, sigma, gamma, w;
   bool blend, open_epoch;
+  Int eq_end;
 };
 
 __device__ __forceinline__ StepScalars step_scalars(const IterParams* p, int step) {
@@ -140,6 +157,7 @@ __device__ __forceinline__ StepScalars step_scalars(const IterParams* p, int ste
   s.tau = p->tau;
   s.sigma = p->sigma;
   s.gamma = p->gamma;
+  s.eq_end = p->num_equalities;
   if (!p->halpern) {
     s.blend = false;
     s.open_epoch = false;
@@ -159,21 +177,24 @@ __device__ __forceinline__ StepScalars step_scalars(const IterParams* p, int ste
 
 // Primal side of one PDHG step on column j, given (K^T y)_j
 // (backend.cpp:60-79 order).
-template <bool kDiag>
+template <bool kDiag, class Src = OneLp>
 struct PrimalStepEpi {
   static constexpr int kR = kDiag ? 1 : 0;
   static constexpr unsigned kMaxMask = 0;
   DevProblem p;
   PrimalBufs b;
-  const IterParams* params;
+  Src src;
   int step;
   StepScalars sc;
-  __device__ PrimalStepEpi resolve() const {
+  bool off;
+  __device__ PrimalStepEpi resolve(int blk) const {
     PrimalStepEpi e = *this;
-    e.sc = step_scalars(params, step);
+    const IterParams* P = src.get(blk);
+    e.off = P->active == 0;
+    e.sc = step_scalars(P, step);
     return e;
   }
-  __device__ bool skip() const { return false; }
+  __device__ bool skip() const { return off; }
   __device__ void operator()(Int j, double kty, double* red) const {
     const double xj = b.x[j];
     double v = xj - sc.tau * (p.c[j] - kty);
@@ -197,26 +218,29 @@ struct PrimalStepEpi {
 // diagonal step it also produces the row half of the convergence terms for
 // the new point (backend.cpp:153-164, unscaled through D1) and the row half of
 // the Farkas test on dy (pdhg.cpp:193-198, 215-216).
-template <bool kDiag>
+template <bool kDiag, class Src = OneLp>
 struct DualStepEpi {
   static constexpr int kR = kDiag ? 7 : 0;
   static constexpr unsigned kMaxMask = (1u << 2) | (1u << 4) | (1u << 5);
   DevProblem p;
   DualBufs b;
-  const IterParams* params;
+  Src src;
   int step;
   StepScalars sc;
-  __device__ DualStepEpi resolve() const {
+  bool off;
+  __device__ DualStepEpi resolve(int blk) const {
     DualStepEpi e = *this;
-    e.sc = step_scalars(params, step);
+    const IterParams* P = src.get(blk);
+    e.off = P->active == 0;
+    e.sc = step_scalars(P, step);
     return e;
   }
-  __device__ bool skip() const { return false; }
+  __device__ bool skip() const { return off; }
   __device__ void operator()(Int i, double kxb, double* red) const {
     const double yi = b.y[i];
     const double kx = b.k_x[i];
     double v = yi + sc.sigma * (p.q[i] - kxb);
-    if (i >= p.num_equalities && v < 0.0) v = 0.0;
+    if (i >= sc.eq_end && v < 0.0) v = 0.0;
     const double d = v - yi;
     const double kd = 0.5 * (kxb - kx);
     double ny = v;
@@ -242,11 +266,11 @@ struct DualStepEpi {
       const double scale = p.row_scale[i];
       const double row = scale != 0.0 ? nk / scale : nk;
       const double slack = row - p.orig_q[i];
-      const double viol = (i < p.num_equalities) ? slack : fmin(slack, 0.0);
+      const double viol = (i < sc.eq_end) ? slack : fmin(slack, 0.0);
       red[1] += viol * viol;
       red[2] = fmax(red[2], fabs(viol));
       red[3] += p.orig_q[i] * (scale * ny);
-      if (i >= p.num_equalities && d < 0.0) red[4] = 1.0;
+      if (i >= sc.eq_end && d < 0.0) red[4] = 1.0;
       red[5] = fmax(red[5], fabs(d));
       red[6] += p.q[i] * d;
     }
@@ -254,13 +278,20 @@ struct DualStepEpi {
 };
 
 // Column half of the convergence terms from (K~^T y~)_j (backend.cpp:166-199).
-struct ConvProductEpi {
+template <class Src = OneLp>
+struct ConvProductEpiT {
   static constexpr int kR = 4;
   static constexpr unsigned kMaxMask = 1u << 1;
   DevProblem p;
   const double* x;
-  __device__ ConvProductEpi resolve() const { return *this; }
-  __device__ bool skip() const { return false; }
+  Src src;
+  bool off;
+  __device__ ConvProductEpiT resolve(int blk) const {
+    ConvProductEpiT e = *this;
+    e.off = src.params != nullptr && src.get(blk)->active == 0;
+    return e;
+  }
+  __device__ bool skip() const { return off; }
   __device__ void operator()(Int j, double kty, double* red) const {
     const double scale = p.col_scale[j];
     const double product = scale != 0.0 ? kty / scale : kty;
@@ -289,17 +320,24 @@ struct ConvProductEpi {
   }
 };
 
+using ConvProductEpi = ConvProductEpiT<OneLp>;
+
 // Column half of the Farkas test from lambda = (K~^T dy)_j (pdhg.cpp:203-213).
-// Skipped as a whole when the row half already rules a certificate out.
-struct FarkasProductEpi {
+// Skipped as a whole (per LP) when the row half already rules a certificate
+// out.
+template <class Src = OneLp>
+struct FarkasProductEpiT {
   static constexpr int kR = 2;
   static constexpr unsigned kMaxMask = 1u << 0;
   DevProblem p;
-  const double* check;  // row slots already folded
+  const double* check;  // row slots already folded, kCheckSlots per LP
+  Src src;
   bool skip_all;
-  __device__ FarkasProductEpi resolve() const {
-    FarkasProductEpi e = *this;
-    e.skip_all = check[kFarkasRowBad] != 0.0 || !(check[kFarkasScale] > 0.0);
+  __device__ FarkasProductEpiT resolve(int blk) const {
+    FarkasProductEpiT e = *this;
+    const double* c = check + static_cast<size_t>(src.lp(blk)) * kCheckSlots;
+    e.skip_all = (src.params != nullptr && src.get(blk)->active == 0) || c[kFarkasRowBad] != 0.0 ||
+                 !(c[kFarkasScale] > 0.0);
     return e;
   }
   __device__ bool skip() const { return skip_all; }
@@ -319,6 +357,8 @@ struct FarkasProductEpi {
     }
   }
 };
+
+using FarkasProductEpi = FarkasProductEpiT<OneLp>;
 
 // ---------------------------------------------------------------------------
 // Folding per-block partials into final values, one workgroup, fixed order.
@@ -423,6 +463,86 @@ struct ConvColOp {
 // ---------------------------------------------------------------------------
 // Elementwise kernels.
 // ---------------------------------------------------------------------------
+// y = A x on the blocks of active LPs.
+struct MultiPlainEpi {
+  static constexpr int kR = 0;
+  static constexpr unsigned kMaxMask = 0;
+  double* y;
+  ManyLps src;
+  bool off;
+  __device__ MultiPlainEpi resolve(int blk) const {
+    MultiPlainEpi e = *this;
+    e.off = src.get(blk)->active == 0;
+    return e;
+  }
+  __device__ bool skip() const { return off; }
+  __device__ void operator()(Int r, double s, double*) const { y[r] = s; }
+};
+
+// One thread per (LP, value): folds the LP's block partials in block order.
+__global__ void __launch_bounds__(kThreads)
+    multi_fold_kernel(const double* __restrict__ partials, int nvals, int slot0, unsigned max_mask,
+                      const Int* __restrict__ lp_blocks, Int lps,
+                      const IterParams* __restrict__ params, double* __restrict__ out) {
+  const Int t = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  const Int l = t / nvals;
+  const int v = static_cast<int>(t % nvals);
+  if (l >= lps || params[l].active == 0) return;
+  const bool mx = (max_mask >> v) & 1u;
+  double acc = 0.0;
+  for (Int b = lp_blocks[l]; b < lp_blocks[l + 1]; ++b) {
+    const double x = partials[static_cast<size_t>(b) * nvals + v];
+    acc = mx ? fmax(acc, x) : acc + x;
+  }
+  out[static_cast<size_t>(l) * kCheckSlots + slot0 + v] = acc;
+}
+
+// One workgroup per chunk (at most kTileNnz elements of one LP).
+__global__ void __launch_bounds__(kThreads)
+    multi_sumsq_kernel(const Int* __restrict__ chunk_start, const Int* __restrict__ chunk_lp,
+                       const double* __restrict__ v, const double* from, double* base,
+                       double* prev, int advance, const IterParams* __restrict__ params,
+                       double* __restrict__ partials) {
+  const Int c = static_cast<Int>(blockIdx.x);
+  if (params[chunk_lp[c]].active == 0) return;  // uniform across the block
+  const Int i0 = chunk_start[c], i1 = chunk_start[c + 1];
+  double acc = 0.0;
+  for (Int i = i0 + static_cast<Int>(threadIdx.x); i < i1; i += kThreads) {
+    const double cur = v[i];
+    const double e = from != nullptr ? cur - from[i] : cur;
+    acc += e * e;
+    if (advance) {
+      if (prev != nullptr) prev[i] = base[i];
+      base[i] = cur;
+    }
+  }
+  double r[1] = {acc};
+  block_fold_store<1>(r, 0u, partials + c);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    multi_chunk_fold_kernel(const double* __restrict__ partials, const Int* __restrict__ lp_chunks,
+                            Int lps, const IterParams* __restrict__ params, double* __restrict__ out,
+                            int stride, int slot) {
+  const Int l = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (l >= lps || params[l].active == 0) return;
+  double acc = 0.0;
+  for (Int c = lp_chunks[l]; c < lp_chunks[l + 1]; ++c) acc += partials[c];
+  out[static_cast<size_t>(l) * stride + slot] = acc;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    multi_divide_kernel(const Int* __restrict__ chunk_start, const Int* __restrict__ chunk_lp,
+                        const double* __restrict__ v, const double* __restrict__ divisor,
+                        const IterParams* __restrict__ params, double* __restrict__ out) {
+  const Int c = static_cast<Int>(blockIdx.x);
+  const Int l = chunk_lp[c];
+  if (params[l].active == 0) return;
+  const double d = divisor[l];
+  for (Int i = chunk_start[c] + static_cast<Int>(threadIdx.x); i < chunk_start[c + 1]; i += kThreads)
+    out[i] = v[i] / d;
+}
+
 __global__ void fill_kernel(double* p, Int n, double v) {
   const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
   if (i < n) p[i] = v;
@@ -652,7 +772,7 @@ void launch_convergence_terms(hipStream_t s, Int n, Int m, const DevProblem& p,
                               const double* kt_y, double* partials_a, double* partials_b,
                               double* out7) {
   ConvRowOp row{p.orig_q, p.row_scale, k_x, y, p.num_equalities};
-  ConvColOp col{ConvProductEpi{p, x}, kt_y};
+  ConvColOp col{ConvProductEpi{p, x, OneLp{nullptr}, false}, kt_y};
   reduce_with(s, m, row, partials_a);
   reduce_with(s, n, col, partials_b);
   fold(s, {FoldSeg{partials_a, reduce_grid(m), 3, 0, ConvRowOp::kMaxMask},
@@ -664,9 +784,11 @@ void launch_primal_step(hipStream_t s, const DevCsr& kt, const double* y, const 
                         const IterParams* params, int step, const PrimalBufs& b, unsigned flags,
                         double* partials, const LaunchEvents* ev) {
   if (flags & kStepDiag) {
-    spmv_with(s, kt, y, PrimalStepEpi<true>{p, b, params, step, {}}, partials, "primal_step", ev);
+    spmv_with(s, kt, y, PrimalStepEpi<true>{p, b, OneLp{params}, step, {}, false}, partials,
+              "primal_step", ev);
   } else {
-    spmv_with(s, kt, y, PrimalStepEpi<false>{p, b, params, step, {}}, partials, "primal_step", ev);
+    spmv_with(s, kt, y, PrimalStepEpi<false>{p, b, OneLp{params}, step, {}, false}, partials,
+              "primal_step", ev);
   }
 }
 
@@ -674,20 +796,102 @@ void launch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar, const
                       const IterParams* params, int step, const DualBufs& b, unsigned flags,
                       double* partials, const LaunchEvents* ev) {
   if (flags & kStepDiag) {
-    spmv_with(s, k, x_bar, DualStepEpi<true>{p, b, params, step, {}}, partials, "dual_step", ev);
+    spmv_with(s, k, x_bar, DualStepEpi<true>{p, b, OneLp{params}, step, {}, false}, partials,
+              "dual_step", ev);
   } else {
-    spmv_with(s, k, x_bar, DualStepEpi<false>{p, b, params, step, {}}, partials, "dual_step", ev);
+    spmv_with(s, k, x_bar, DualStepEpi<false>{p, b, OneLp{params}, step, {}, false}, partials,
+              "dual_step", ev);
   }
 }
 
 void launch_conv_product(hipStream_t s, const DevCsr& kt, const double* y, const double* x,
                          const DevProblem& p, double* partials) {
-  spmv_with(s, kt, y, ConvProductEpi{p, x}, partials, "conv_product");
+  spmv_with(s, kt, y, ConvProductEpi{p, x, OneLp{nullptr}, false}, partials, "conv_product");
 }
 
 void launch_farkas_product(hipStream_t s, const DevCsr& kt, const double* dy,
                            const DevProblem& p, const double* check, double* partials) {
-  spmv_with(s, kt, dy, FarkasProductEpi{p, check, false}, partials, "farkas_product");
+  spmv_with(s, kt, dy, FarkasProductEpi{p, check, OneLp{nullptr}, false}, partials,
+            "farkas_product");
+}
+
+void launch_multi_primal_step(hipStream_t s, const DevCsr& kt, const Int* block_lp,
+                              const double* y, const DevProblem& p, const IterParams* params,
+                              int step, const PrimalBufs& b, bool diag, double* partials,
+                              const LaunchEvents* ev) {
+  const ManyLps src{params, block_lp};
+  if (diag) {
+    spmv_with(s, kt, y, PrimalStepEpi<true, ManyLps>{p, b, src, step, {}, false}, partials,
+              "multi primal_step", ev);
+  } else {
+    spmv_with(s, kt, y, PrimalStepEpi<false, ManyLps>{p, b, src, step, {}, false}, partials,
+              "multi primal_step", ev);
+  }
+}
+
+void launch_multi_dual_step(hipStream_t s, const DevCsr& k, const Int* block_lp,
+                            const double* x_bar, const DevProblem& p, const IterParams* params,
+                            int step, const DualBufs& b, bool diag, double* partials,
+                            const LaunchEvents* ev) {
+  const ManyLps src{params, block_lp};
+  if (diag) {
+    spmv_with(s, k, x_bar, DualStepEpi<true, ManyLps>{p, b, src, step, {}, false}, partials,
+              "multi dual_step", ev);
+  } else {
+    spmv_with(s, k, x_bar, DualStepEpi<false, ManyLps>{p, b, src, step, {}, false}, partials,
+              "multi dual_step", ev);
+  }
+}
+
+void launch_multi_conv_product(hipStream_t s, const DevCsr& kt, const Int* block_lp,
+                               const double* y, const double* x, const DevProblem& p,
+                               const IterParams* params, double* partials) {
+  spmv_with(s, kt, y, ConvProductEpiT<ManyLps>{p, x, ManyLps{params, block_lp}, false}, partials,
+            "multi conv_product");
+}
+
+void launch_multi_farkas_product(hipStream_t s, const DevCsr& kt, const Int* block_lp,
+                                 const double* dy, const DevProblem& p, const IterParams* params,
+                                 const double* check, double* partials) {
+  spmv_with(s, kt, dy, FarkasProductEpiT<ManyLps>{p, check, ManyLps{params, block_lp}, false},
+            partials, "multi farkas_product");
+}
+
+void launch_multi_product(hipStream_t s, const DevCsr& a, const Int* block_lp, const double* x,
+                          double* y, const IterParams* params) {
+  spmv_with(s, a, x, MultiPlainEpi{y, ManyLps{params, block_lp}, false}, nullptr, "multi product");
+}
+
+void launch_multi_fold(hipStream_t s, const double* partials, int nvals, int slot0,
+                       unsigned max_mask, const Int* lp_blocks, Int lps, const IterParams* params,
+                       double* out) {
+  if (lps <= 0) return;
+  const Int threads = lps * nvals;
+  hipLaunchKernelGGL(multi_fold_kernel, dim3(static_cast<unsigned>((threads + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, s, partials, nvals, slot0, max_mask, lp_blocks, lps, params,
+                     out);
+  check_launch("multi fold");
+}
+
+void launch_multi_sumsq(hipStream_t s, const ChunkPlan& plan, const double* v, const double* from,
+                        double* base, double* prev, bool advance, const IterParams* params,
+                        double* partials, const Int* lp_chunks, Int lps, double* out, int stride,
+                        int slot) {
+  if (plan.chunks <= 0 || lps <= 0) return;
+  hipLaunchKernelGGL(multi_sumsq_kernel, dim3(static_cast<unsigned>(plan.chunks)), dim3(kThreads), 0,
+                     s, plan.start, plan.lp, v, from, base, prev, advance ? 1 : 0, params, partials);
+  check_launch("multi sumsq");
+  hipLaunchKernelGGL(multi_chunk_fold_kernel, dim3(static_cast<unsigned>((lps + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, s, partials, lp_chunks, lps, params, out, stride, slot);
+  check_launch("multi chunk fold");
+}
+
+void launch_multi_divide(hipStream_t s, const ChunkPlan& plan, const double* v,
+                         const double* divisor, const IterParams* params, double* out) {
+  if (plan.chunks <= 0) return;
+  hipLaunchKernelGGL(multi_divide_kernel, dim3(static_cast<unsigned>(plan.chunks)), dim3(kThreads), 0,
+                     s, plan.start, plan.lp, v, divisor, params, out);
+  check_launch("multi divide");
 }
 
 void launch_check_fold_rows(hipStream_t s, Int primal_blocks, const double* pa, Int dual_blocks,

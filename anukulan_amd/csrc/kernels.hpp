@@ -45,7 +45,8 @@ struct IterParams {
   Int halpern;               // 1 when the Halpern scheme is on
   Int reset_at_start;        // the period's first step begins a new epoch
   Int k_at_start;            // Halpern counter before the period's first step
-  Int num_equalities;
+  Int num_equalities;        // end of the equality rows (a global row index)
+  Int active = 1;            // 0: the LP does nothing this period
 };
 
 // Which of the per-iteration extras a fused step produces.
@@ -182,5 +183,55 @@ void launch_check_fold_rows(hipStream_t s, Int primal_blocks, const double* pa,
                             Int dual_blocks, const double* pb, double* check);
 void launch_check_fold_cols(hipStream_t s, Int conv_blocks, const double* pc,
                             Int farkas_blocks, const double* pd, double* check);
+
+// ---- block-diagonal stacks of LPs (each with its own K) -----------------
+//
+// The LPs' matrices are concatenated block-diagonally and their vectors end
+// to end. Row blocks are planned per LP, so no block straddles two LPs, and
+// `block_lp[b]` names the LP of block b; each LP has its own IterParams
+// (num_equalities holds the global end row of its equalities; active = 0
+// makes its blocks return at once). Vector segments are cut into chunks of
+// at most kTileNnz elements within one LP for the per-LP reductions.
+struct ChunkPlan {
+  Int chunks = 0;
+  const Int* start = nullptr;  // chunks + 1 element offsets
+  const Int* lp = nullptr;     // LP of each chunk
+};
+
+void launch_multi_primal_step(hipStream_t s, const DevCsr& kt, const Int* block_lp,
+                              const double* y, const DevProblem& p, const IterParams* params,
+                              int step, const PrimalBufs& b, bool diag, double* partials,
+                              const LaunchEvents* ev = nullptr);
+void launch_multi_dual_step(hipStream_t s, const DevCsr& k, const Int* block_lp,
+                            const double* x_bar, const DevProblem& p, const IterParams* params,
+                            int step, const DualBufs& b, bool diag, double* partials,
+                            const LaunchEvents* ev = nullptr);
+void launch_multi_conv_product(hipStream_t s, const DevCsr& kt, const Int* block_lp,
+                               const double* y, const double* x, const DevProblem& p,
+                               const IterParams* params, double* partials);
+// `check` holds kCheckSlots doubles per LP, row slots already folded.
+void launch_multi_farkas_product(hipStream_t s, const DevCsr& kt, const Int* block_lp,
+                                 const double* dy, const DevProblem& p, const IterParams* params,
+                                 const double* check, double* partials);
+// y = A x on the blocks of the LPs whose params are active (others untouched).
+void launch_multi_product(hipStream_t s, const DevCsr& a, const Int* block_lp, const double* x,
+                          double* y, const IterParams* params);
+// Folds nvals per-block partials of each active LP over its blocks
+// [lp_blocks[l], lp_blocks[l+1]) in block order into
+// out[l * kCheckSlots + slot0 + v].
+void launch_multi_fold(hipStream_t s, const double* partials, int nvals, int slot0,
+                       unsigned max_mask, const Int* lp_blocks, Int lps, const IterParams* params,
+                       double* out);
+// Per chunk of each active LP: sum (v - from)^2 (from null: sum v^2) into
+// partials[chunk]; with `advance`, prev <- base (when prev is set) and
+// base <- v afterwards. Then folded per LP in chunk order into
+// out[l * stride + slot].
+void launch_multi_sumsq(hipStream_t s, const ChunkPlan& plan, const double* v, const double* from,
+                        double* base, double* prev, bool advance, const IterParams* params,
+                        double* partials, const Int* lp_chunks, Int lps, double* out, int stride,
+                        int slot);
+// out = v / divisor[lp] element-wise on the chunks of active LPs.
+void launch_multi_divide(hipStream_t s, const ChunkPlan& plan, const double* v,
+                         const double* divisor, const IterParams* params, double* out);
 
 }  // namespace anuk

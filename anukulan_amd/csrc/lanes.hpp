@@ -1,0 +1,70 @@
+// Device helpers shared by the kernels that run one LP per wave64 lane over
+// a shared CSR matrix: vectors are row-interleaved, v[row * 64 + lane], and a
+// wave owns one row, so the matrix loads are wave-uniform (scalar) and each
+// vector access is one contiguous 512-byte line.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace anuk {
+namespace lanes {
+
+inline constexpr int kLaneCount = 64;
+inline constexpr int kWavesPerBlock = 4;  // 256 threads, one row per wave
+
+// Workgroups are dispatched round-robin over the 8 XCDs; this gives each XCD
+// one contiguous range of logical blocks, so neighbouring rows share its L2.
+__device__ __forceinline__ int xcd_remap(int i, int g) {
+  const int x = i & 7, local = i >> 3, per = g >> 3, rem = g & 7;
+  return x * per + (x < rem ? x : rem) + local;
+}
+
+// The row this wave owns, made wave-uniform so the matrix loads are scalar.
+__device__ __forceinline__ Int wave_row() {
+  const int blk = xcd_remap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  return __builtin_amdgcn_readfirstlane(blk * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6));
+}
+
+__device__ __forceinline__ size_t at(Int row, int lane) {
+  return static_cast<size_t>(row) * kLaneCount + static_cast<size_t>(lane);
+}
+
+// Lane-serial row sum of A[row, :] v[:, lane] in storage order. The gathers
+// of up to eight nonzeros are issued together so a long row is not a chain of
+// memory latencies; the adds still happen one by one in storage order.
+template <int W>
+__device__ __forceinline__ void chunk_sum(const DevCsr& a, Int k, const double* __restrict__ v,
+                                          int lane, double& s) {
+  double w[W], x[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    w[u] = a.value[k + u];
+    x[u] = v[at(a.index[k + u], lane)];
+  }
+#pragma unroll
+  for (int u = 0; u < W; ++u) s += w[u] * x[u];
+}
+
+__device__ __forceinline__ double row_sum(const DevCsr& a, Int row, const double* __restrict__ v,
+                                          int lane) {
+  const Int lo = a.start[row], hi = a.start[row + 1];
+  double s = 0.0;
+  Int k = lo;
+  for (; k + 8 <= hi; k += 8) chunk_sum<8>(a, k, v, lane, s);
+  if (k + 4 <= hi) {
+    chunk_sum<4>(a, k, v, lane, s);
+    k += 4;
+  }
+  for (; k < hi; ++k) s += a.value[k] * v[at(a.index[k], lane)];
+  return s;
+}
+
+// Blocks needed for one wave per row.
+inline unsigned row_blocks(Int rows) {
+  return static_cast<unsigned>((rows + kWavesPerBlock - 1) / kWavesPerBlock);
+}
+
+}  // namespace lanes
+}  // namespace anuk

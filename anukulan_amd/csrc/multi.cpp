@@ -1,0 +1,1056 @@
+#include "multi.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <thread>
+
+#include "batch.hpp"
+#include "control.hpp"
+#include "kernels.hpp"
+
+namespace anuk {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// Backend scratch slots of the engines' state slabs, one per nesting depth.
+constexpr int kMultiScratchSlot = 500;
+constexpr size_t kParamDoubles = (sizeof(IterParams) + 7) / 8;
+
+double seconds_since(Clock::time_point t) {
+  return std::chrono::duration<double>(Clock::now() - t).count();
+}
+
+// ---- the stack on the host ------------------------------------------------
+
+struct Stack {
+  Int lps = 0;
+  Int n = 0, m = 0;
+  std::vector<Int> col_off, row_off;  // lps + 1
+  Csr k, kt;                          // block diagonal, scaled
+  std::vector<Int> eq_end;            // global end row of each LP's equalities
+  std::vector<double> row_scale, col_scale;
+  std::vector<double> c, q, lower, upper;  // scaled
+  std::vector<double> oc, oq, olo, ohi;    // as given
+  std::vector<Scaling> scaling;            // per LP
+  Int cols(Int l) const { return col_off[uz(l) + 1] - col_off[uz(l)]; }
+  Int rows(Int l) const { return row_off[uz(l) + 1] - row_off[uz(l)]; }
+};
+
+// Runs f(i) for i in [0, count) on up to `threads` host threads.
+template <class F>
+void parallel_for(size_t count, F f) {
+  const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+  const size_t threads = std::min<size_t>(std::min<size_t>(hw, 16), count);
+  if (threads <= 1) {
+    for (size_t i = 0; i < count; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  std::vector<std::exception_ptr> errors(threads);
+  for (size_t t = 0; t < threads; ++t) {
+    pool.emplace_back([&, t] {
+      try {
+        for (size_t i = t; i < count; i += threads) f(i);
+      } catch (...) {
+        errors[t] = std::current_exception();
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  for (auto& e : errors)
+    if (e) std::rethrow_exception(e);
+}
+
+// Scales every LP (scale_lp, scaling.cpp:32-100, on a copy) and concatenates
+// the scaled matrices block-diagonally and the vectors end to end.
+Stack build_stack(const std::vector<const StandardLp*>& lps, const anuk_pdhg_options& opt) {
+  Stack st;
+  st.lps = static_cast<Int>(lps.size());
+  std::vector<StandardLp> scaled(lps.size());
+  st.scaling.resize(lps.size());
+  const bool scales = opt.ruiz_iterations > 0 || opt.pock_chambolle != 0;
+  parallel_for(lps.size(), [&](size_t i) {
+    scaled[i] = *lps[i];
+    if (scales) {
+      st.scaling[i] = scale_lp(&scaled[i], opt.ruiz_iterations, opt.pock_chambolle != 0);
+    } else {
+      st.scaling[i].row_scale.assign(uz(scaled[i].rows()), 1.0);
+      st.scaling[i].col_scale.assign(uz(scaled[i].cols()), 1.0);
+    }
+  });
+  st.col_off.assign(lps.size() + 1, 0);
+  st.row_off.assign(lps.size() + 1, 0);
+  size_t nnz = 0;
+  for (size_t i = 0; i < lps.size(); ++i) {
+    const long cols = static_cast<long>(st.col_off[i]) + lps[i]->cols();
+    const long rows = static_cast<long>(st.row_off[i]) + lps[i]->rows();
+    if (cols > (1L << 30) || rows > (1L << 30)) throw std::invalid_argument("multi: stack too large");
+    st.col_off[i + 1] = static_cast<Int>(cols);
+    st.row_off[i + 1] = static_cast<Int>(rows);
+    nnz += static_cast<size_t>(lps[i]->k.nnz());
+  }
+  if (nnz > static_cast<size_t>(1) << 30) throw std::invalid_argument("multi: stack too large");
+  st.n = st.col_off.back();
+  st.m = st.row_off.back();
+  auto concat = [&](Csr* out, bool transpose) {
+    out->rows = transpose ? st.n : st.m;
+    out->cols = transpose ? st.m : st.n;
+    out->start.assign(uz(out->rows) + 1, 0);
+    out->index.resize(nnz);
+    out->value.resize(nnz);
+    size_t at = 0;
+    for (size_t i = 0; i < lps.size(); ++i) {
+      const Csr& a = transpose ? scaled[i].kt : scaled[i].k;
+      const Int r0 = transpose ? st.col_off[i] : st.row_off[i];
+      const Int c0 = transpose ? st.row_off[i] : st.col_off[i];
+      for (Int r = 0; r < a.rows; ++r) out->start[uz(r0 + r) + 1] = static_cast<Int>(at) + a.start[uz(r) + 1];
+      for (Int e = 0; e < a.nnz(); ++e) {
+        out->index[at + uz(e)] = a.index[uz(e)] + c0;
+        out->value[at + uz(e)] = a.value[uz(e)];
+      }
+      at += uz(a.nnz());
+    }
+    out->renew_id();
+  };
+  concat(&st.k, false);
+  concat(&st.kt, true);
+  auto cat = [&](std::vector<double> StandardLp::*f, bool from_scaled) {
+    std::vector<double> v;
+    for (size_t i = 0; i < lps.size(); ++i) {
+      const std::vector<double>& src = from_scaled ? scaled[i].*f : lps[i]->*f;
+      v.insert(v.end(), src.begin(), src.end());
+    }
+    return v;
+  };
+  st.c = cat(&StandardLp::c, true);
+  st.q = cat(&StandardLp::q, true);
+  st.lower = cat(&StandardLp::lower, true);
+  st.upper = cat(&StandardLp::upper, true);
+  st.oc = cat(&StandardLp::c, false);
+  st.oq = cat(&StandardLp::q, false);
+  st.olo = cat(&StandardLp::lower, false);
+  st.ohi = cat(&StandardLp::upper, false);
+  for (size_t i = 0; i < lps.size(); ++i) {
+    st.eq_end.push_back(st.row_off[i] + lps[i]->num_equalities);
+    st.row_scale.insert(st.row_scale.end(), st.scaling[i].row_scale.begin(), st.scaling[i].row_scale.end());
+    st.col_scale.insert(st.col_scale.end(), st.scaling[i].col_scale.begin(), st.scaling[i].col_scale.end());
+  }
+  return st;
+}
+
+// ---- the stack on the device ----------------------------------------------
+
+struct DeviceStack {
+  HipBackend& be;
+  DevCsr k, kt;
+  const Int* k_block_lp = nullptr;
+  const Int* kt_block_lp = nullptr;
+  const Int* k_lp_blocks = nullptr;
+  const Int* kt_lp_blocks = nullptr;
+  ChunkPlan cols, rows;
+  const Int* col_lp_chunks = nullptr;
+  const Int* row_lp_chunks = nullptr;
+  Int col_chunks = 0, row_chunks = 0;
+  std::vector<void*> owned;
+
+  explicit DeviceStack(HipBackend& b) : be(b) {}
+  ~DeviceStack() {
+    (void)hipStreamSynchronize(be.stream());
+    for (void* p : owned) (void)hipFree(p);
+  }
+  DeviceStack(const DeviceStack&) = delete;
+  DeviceStack& operator=(const DeviceStack&) = delete;
+
+  template <class T>
+  const T* put(const std::vector<T>& h) {
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, std::max<size_t>(h.size(), 1) * sizeof(T)), "hipMalloc(stack)");
+    owned.push_back(p);
+    if (!h.empty())
+      hip_check(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, be.stream()),
+                "upload stack");
+    return static_cast<const T*>(p);
+  }
+};
+
+// Row blocks planned per LP (so none straddles two), with each block's LP
+// and each LP's block range.
+struct Plan {
+  std::vector<Int> block_row, block_lp, lp_blocks;
+};
+
+Plan plan_stack(const Csr& a, const std::vector<Int>& off) {
+  Plan p;
+  p.block_row.push_back(0);
+  p.lp_blocks.push_back(0);
+  for (size_t l = 0; l + 1 < off.size(); ++l) {
+    const Int rows = off[l + 1] - off[l];
+    if (rows > 0) {
+      std::int32_t* br = nullptr;
+      Int blocks = 0;
+      plan_blocks(rows, a.start.data() + off[l], kTileNnz, kThreads, &br, &blocks);
+      for (Int b = 1; b <= blocks; ++b) {
+        p.block_row.push_back(off[l] + br[b]);
+        p.block_lp.push_back(static_cast<Int>(l));
+      }
+      std::free(br);
+    }
+    p.lp_blocks.push_back(static_cast<Int>(p.block_lp.size()));
+  }
+  return p;
+}
+
+struct Chunks {
+  std::vector<Int> start, lp, lp_chunks;
+};
+
+Chunks chunk_stack(const std::vector<Int>& off) {
+  Chunks c;
+  c.start.push_back(0);
+  c.lp_chunks.push_back(0);
+  for (size_t l = 0; l + 1 < off.size(); ++l) {
+    for (Int i = off[l]; i < off[l + 1]; i += kTileNnz) {
+      c.start.push_back(std::min(off[l + 1], i + kTileNnz));
+      c.lp.push_back(static_cast<Int>(l));
+    }
+    c.lp_chunks.push_back(static_cast<Int>(c.lp.size()));
+  }
+  return c;
+}
+
+void upload_stack(const Stack& st, DeviceStack* ds) {
+  auto dev_csr = [&](const Csr& a, const Plan& p) {
+    DevCsr d;
+    d.rows = a.rows;
+    d.cols = a.cols;
+    d.nnz = a.nnz();
+    d.blocks = static_cast<Int>(p.block_lp.size());
+    d.start = ds->put(a.start);
+    d.index = ds->put(a.index);
+    d.value = ds->put(a.value);
+    d.block_row = ds->put(p.block_row);
+    return d;
+  };
+  const Plan pk = plan_stack(st.k, st.row_off);
+  const Plan pkt = plan_stack(st.kt, st.col_off);
+  ds->k = dev_csr(st.k, pk);
+  ds->kt = dev_csr(st.kt, pkt);
+  ds->k_block_lp = ds->put(pk.block_lp);
+  ds->kt_block_lp = ds->put(pkt.block_lp);
+  ds->k_lp_blocks = ds->put(pk.lp_blocks);
+  ds->kt_lp_blocks = ds->put(pkt.lp_blocks);
+  const Chunks cc = chunk_stack(st.col_off), rc = chunk_stack(st.row_off);
+  ds->col_chunks = static_cast<Int>(cc.lp.size());
+  ds->row_chunks = static_cast<Int>(rc.lp.size());
+  ds->cols = ChunkPlan{ds->col_chunks, ds->put(cc.start), ds->put(cc.lp)};
+  ds->rows = ChunkPlan{ds->row_chunks, ds->put(rc.start), ds->put(rc.lp)};
+  ds->col_lp_chunks = ds->put(cc.lp_chunks);
+  ds->row_lp_chunks = ds->put(rc.lp_chunks);
+  ds->be.synchronize();
+}
+
+// Device-resident per-LP flags as IterParams (only `active` is read).
+std::vector<IterParams> flags(const std::vector<char>& on) {
+  std::vector<IterParams> p(on.size());
+  for (size_t i = 0; i < on.size(); ++i) {
+    p[i] = IterParams{};
+    p[i].active = on[i] ? 1 : 0;
+  }
+  return p;
+}
+
+// ||K_l|| for every LP by the power iteration on K_l^T K_l (pdhg.cpp:223-263),
+// all LPs in the same launches, each stopping on its own test.
+std::vector<double> stack_norms(HipBackend& be, const Stack& st, const DeviceStack& ds,
+                                int iterations, double tolerance) {
+  const Int L = st.lps;
+  std::vector<double> norm(uz(L), 0.0), sigma(uz(L), 0.0);
+  std::vector<char> on(uz(L), 0);
+  std::vector<double> v(uz(st.n), 0.0);
+  for (Int l = 0; l < L; ++l) {
+    const Int n = st.cols(l), m = st.rows(l);
+    const Int nnz = st.k.start[uz(st.row_off[uz(l) + 1])] - st.k.start[uz(st.row_off[uz(l)])];
+    if (n == 0 || m == 0 || nnz == 0) continue;
+    std::vector<double> s(uz(n));
+    for (Int j = 0; j < n; ++j) s[uz(j)] = 1.0 + 0.1 * static_cast<double>(j % 7);
+    const double z = host_two_norm(s);
+    for (Int j = 0; j < n; ++j) v[uz(st.col_off[uz(l)] + j)] = s[uz(j)] / z;
+    on[uz(l)] = 1;
+  }
+  const size_t N = uz(st.n), M = uz(st.m);
+  double* d_v = be.scratch(kMultiScratchSlot + 90, N);
+  double* d_kv = be.scratch(kMultiScratchSlot + 91, M);
+  double* d_u = be.scratch(kMultiScratchSlot + 92, N);
+  double* d_part = be.scratch(kMultiScratchSlot + 93, uz(ds.col_chunks) + 1);
+  double* d_out = be.scratch(kMultiScratchSlot + 94, uz(L) + 1);
+  double* d_div = be.scratch(kMultiScratchSlot + 95, uz(L) + 1);
+  auto* d_params = reinterpret_cast<IterParams*>(be.scratch(kMultiScratchSlot + 96, uz(L) * kParamDoubles + 1));
+  be.upload_staged(v.data(), d_v, N);
+  auto put_flags = [&] {
+    const std::vector<IterParams> p = flags(on);
+    hip_check(hipMemcpyAsync(d_params, p.data(), p.size() * sizeof(IterParams), hipMemcpyHostToDevice,
+                             be.stream()),
+              "norm flags");
+  };
+  put_flags();
+  std::vector<double> out(uz(L)), div(uz(L), 1.0);
+  hipStream_t s = be.stream();
+  for (int it = 0; it < iterations && std::any_of(on.begin(), on.end(), [](char c) { return c; }); ++it) {
+    launch_multi_product(s, ds.k, ds.k_block_lp, d_v, d_kv, d_params);
+    launch_multi_product(s, ds.kt, ds.kt_block_lp, d_kv, d_u, d_params);
+    launch_multi_sumsq(s, ds.cols, d_u, nullptr, nullptr, nullptr, false, d_params, d_part,
+                       ds.col_lp_chunks, L, d_out, 1, 0);
+    hip_check(hipMemcpyAsync(out.data(), d_out, uz(L) * sizeof(double), hipMemcpyDeviceToHost, s), "norms");
+    be.synchronize();
+    bool changed = false;
+    for (Int l = 0; l < L; ++l) {
+      if (!on[uz(l)]) continue;
+      const double next = std::sqrt(out[uz(l)]);
+      if (next == 0.0) {
+        norm[uz(l)] = 0.0;
+        on[uz(l)] = 0;
+        changed = true;
+        continue;
+      }
+      div[uz(l)] = next;
+      const double estimate = std::sqrt(next);
+      if (it > 0 && std::fabs(estimate - sigma[uz(l)]) <= tolerance * estimate) {
+        norm[uz(l)] = estimate;
+        on[uz(l)] = 0;
+        changed = true;
+        continue;
+      }
+      sigma[uz(l)] = estimate;
+      norm[uz(l)] = estimate;
+    }
+    hip_check(hipMemcpyAsync(d_div, div.data(), uz(L) * sizeof(double), hipMemcpyHostToDevice, s), "divisors");
+    if (changed) put_flags();
+    launch_multi_divide(s, ds.cols, d_u, d_div, d_params, d_v);
+  }
+  be.synchronize();
+  return norm;
+}
+
+// ---- one engine: a set of LPs of the stack iterated together --------------
+
+struct LpRun {
+  bool participates = true;
+  double initial_omega = 0.0;  // 0: from ||c|| / ||q||
+  double initial_eta = 0.0;    // 0: 0.9 / ||K||
+  std::vector<double> warm_x, warm_y;
+  double q_two = 0, c_two = 0, q_inf = 0, c_inf = 0;
+};
+
+struct Outcome {
+  int status = ANUK_PDHG_ITERATION_LIMIT;
+  Int iterations = 0, restarts = 0;
+  double eta = 0.0, omega = 0.0, seconds = 0.0;
+  std::string message;
+  std::vector<double> x, y;  // working space, unless polished
+  bool polished = false;
+  Residual polished_r;
+  Int polish_attempts = 0, polish_iterations = 0;
+};
+
+// Iteration and convergence data of an engine, concatenated over the stack.
+struct EngineData {
+  const std::vector<double>* c;
+  const std::vector<double>* q;
+  const std::vector<double>* lower;
+  const std::vector<double>* upper;
+  const std::vector<double>* oc;
+  const std::vector<double>* oq;
+  const std::vector<double>* olo;
+  const std::vector<double>* ohi;
+  const std::vector<double>* row_scale;
+  const std::vector<double>* col_scale;
+};
+
+struct PolishContext {
+  const std::vector<const StandardLp*>* originals;
+};
+
+std::vector<Outcome> run_engine(HipBackend& be, const Stack& st, const DeviceStack& ds,
+                                const std::vector<double>& norms, const anuk_pdhg_options& opt,
+                                const EngineData& data, std::vector<LpRun>& runs,
+                                Clock::time_point start, const PolishContext* polish,
+                                Phases* phases, int depth);
+
+class Engine {
+ public:
+  Engine(HipBackend& be, const Stack& st, const DeviceStack& ds, const std::vector<double>& norms,
+         const anuk_pdhg_options& opt, const EngineData& data, std::vector<LpRun>& runs,
+         Clock::time_point start, const PolishContext* polish, Phases* phases, int depth)
+      : be_(be), st_(st), ds_(ds), norms_(norms), opt_(opt), data_(data), runs_(runs),
+        start_(start), polish_(polish), phases_(phases), depth_(depth) {}
+  ~Engine() {
+    for (auto& g : graphs_) {
+      (void)hipGraphExecDestroy(g.second.second);
+      (void)hipGraphDestroy(g.second.first);
+    }
+    if (pinned_) {
+      (void)hipStreamSynchronize(be_.stream());
+      (void)hipHostFree(pinned_);
+    }
+  }
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+  std::vector<Outcome> run();
+
+ private:
+  void allocate();
+  void upload();
+  void upload_params();
+  void enqueue_period(int steps, bool has_check, int parity);
+  hipGraphExec_t graph_for(int parity);
+  void extract(Int l, Outcome* o);
+  void run_restarts(const std::vector<Int>& who);
+  void run_polish(const std::vector<Int>& who, Int budget, std::vector<Outcome>& out,
+                  std::vector<char>* stopped);
+
+  HipBackend& be_;
+  const Stack& st_;
+  const DeviceStack& ds_;
+  const std::vector<double>& norms_;
+  const anuk_pdhg_options& opt_;
+  EngineData data_;
+  std::vector<LpRun>& runs_;
+  Clock::time_point start_;
+  const PolishContext* polish_;
+  Phases* phases_;
+  int depth_;
+
+  Int L_ = 0;
+  std::vector<Control> ctl_;
+  std::vector<char> active_;
+  int cur_ = 0;
+  double *X_[2], *Y_[2], *KX_[2];
+  double *XBAR_, *DX_, *AX_, *LAM_, *RX_, *PRX_;
+  double *DY_, *KDX_, *AY_, *AKX_, *RY_, *PRY_;
+  double *CHECK_, *RESTART_, *PA_, *PB_, *PC_, *PD_, *PCH_;
+  IterParams* d_params_ = nullptr;
+  IterParams* d_rparams_ = nullptr;
+  DevProblem prob_;
+  double* pinned_ = nullptr;  // params staging, then check values
+  size_t pinned_doubles_ = 0;
+  std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
+};
+
+void Engine::allocate() {
+  const size_t N = uz(st_.n), M = uz(st_.m), L = uz(L_);
+  auto round = [](size_t d) { return (std::max<size_t>(d, 1) + 31) / 32 * 32; };
+  const size_t pa = uz(ds_.kt.blocks), pb = uz(ds_.k.blocks) * 7, pc = uz(ds_.kt.blocks) * 4,
+               pd = uz(ds_.kt.blocks) * 2, pch = uz(std::max(ds_.col_chunks, ds_.row_chunks));
+  const size_t sizes[] = {
+      N, N, N, N, N, N, N, N,               // X0 X1 XBAR DX AX LAM RX PRX
+      N, N, N, N, N, N, N,                  // c lower upper oc olo ohi col_scale
+      M, M, M, M, M, M, M, M, M, M,         // Y0 Y1 KX0 KX1 DY KDX AY AKX RY PRY
+      M, M, M,                              // q oq row_scale
+      L * kCheckSlots, L * 2,               // CHECK RESTART
+      L * kParamDoubles, L * kParamDoubles, // params, restart params
+      pa, pb, pc, pd, pch};
+  size_t total = 0;
+  for (size_t s : sizes) total += round(s);
+  double* slab = be_.scratch(kMultiScratchSlot + depth_, total);
+  hip_check(hipMemsetAsync(slab, 0, total * sizeof(double), be_.stream()), "hipMemsetAsync(multi)");
+  double* p = slab;
+  size_t i = 0;
+  auto take = [&]() {
+    double* q = p;
+    p += round(sizes[i++]);
+    return q;
+  };
+  X_[0] = take(); X_[1] = take(); XBAR_ = take(); DX_ = take(); AX_ = take(); LAM_ = take();
+  RX_ = take(); PRX_ = take();
+  double* c = take(); double* lo = take(); double* hi = take();
+  double* oc = take(); double* olo = take(); double* ohi = take(); double* cs = take();
+  Y_[0] = take(); Y_[1] = take(); KX_[0] = take(); KX_[1] = take(); DY_ = take(); KDX_ = take();
+  AY_ = take(); AKX_ = take(); RY_ = take(); PRY_ = take();
+  double* q = take(); double* oq = take(); double* rs = take();
+  CHECK_ = take(); RESTART_ = take();
+  d_params_ = reinterpret_cast<IterParams*>(take());
+  d_rparams_ = reinterpret_cast<IterParams*>(take());
+  PA_ = take(); PB_ = take(); PC_ = take(); PD_ = take(); PCH_ = take();
+  prob_.n = st_.n;
+  prob_.m = st_.m;
+  prob_.num_equalities = 0;  // per LP, in IterParams
+  prob_.c = c; prob_.lower = lo; prob_.upper = hi; prob_.q = q;
+  prob_.orig_c = oc; prob_.orig_lower = olo; prob_.orig_upper = ohi; prob_.orig_q = oq;
+  prob_.row_scale = rs; prob_.col_scale = cs;
+  pinned_doubles_ = std::max(L * kParamDoubles, L * kCheckSlots) + L * 2 + 64;
+  hip_check(hipHostMalloc(&pinned_, pinned_doubles_ * sizeof(double), hipHostMallocDefault),
+            "hipHostMalloc(multi)");
+}
+
+void Engine::upload() {
+  // A sub-engine (polishing a few LPs) uploads only its LPs' ranges.
+  Int taking = 0;
+  for (const LpRun& r : runs_) taking += r.participates ? 1 : 0;
+  const bool whole = taking == L_;
+  auto put_range = [&](const std::vector<double>& h, const double* d, const std::vector<Int>& off) {
+    if (whole) {
+      be_.upload_staged(h.data(), const_cast<double*>(d), h.size());
+      return;
+    }
+    for (Int l = 0; l < L_; ++l) {
+      if (!runs_[uz(l)].participates) continue;
+      const Int a = off[uz(l)], b = off[uz(l) + 1];
+      if (b > a) be_.upload_staged(h.data() + a, const_cast<double*>(d) + a, uz(b - a));
+    }
+  };
+  put_range(*data_.c, prob_.c, st_.col_off);
+  put_range(*data_.lower, prob_.lower, st_.col_off);
+  put_range(*data_.upper, prob_.upper, st_.col_off);
+  put_range(*data_.oc, prob_.orig_c, st_.col_off);
+  put_range(*data_.olo, prob_.orig_lower, st_.col_off);
+  put_range(*data_.ohi, prob_.orig_upper, st_.col_off);
+  put_range(*data_.col_scale, prob_.col_scale, st_.col_off);
+  put_range(*data_.q, prob_.q, st_.row_off);
+  put_range(*data_.oq, prob_.orig_q, st_.row_off);
+  put_range(*data_.row_scale, prob_.row_scale, st_.row_off);
+  // Starting points (pdhg.cpp:479-503): a warm start put in the dual cone and
+  // the primal box, a cold one the projection of zero.
+  std::vector<double> x(uz(st_.n), 0.0), y(uz(st_.m), 0.0);
+  for (Int l = 0; l < L_; ++l) {
+    const LpRun& r = runs_[uz(l)];
+    if (!r.participates) continue;
+    const Int c0 = st_.col_off[uz(l)], r0 = st_.row_off[uz(l)];
+    const bool warm = !r.warm_x.empty() && !r.warm_y.empty();
+    if (warm) {
+      for (Int i = 0; i < st_.rows(l); ++i) {
+        double v = r.warm_y[uz(i)];
+        if (r0 + i >= st_.eq_end[uz(l)] && !(v > 0.0)) v = 0.0;
+        y[uz(r0 + i)] = v;
+      }
+    }
+    for (Int j = 0; j < st_.cols(l); ++j) {
+      const double lo = (*data_.lower)[uz(c0 + j)], hi = (*data_.upper)[uz(c0 + j)];
+      double v = warm ? r.warm_x[uz(j)] : 0.0;
+      if (!std::isfinite(v)) v = 0.0;
+      if (v < lo) v = std::isfinite(lo) ? lo : 0.0;
+      if (v > hi) v = std::isfinite(hi) ? hi : 0.0;
+      x[uz(c0 + j)] = v;
+    }
+  }
+  put_range(x, X_[0], st_.col_off);
+  put_range(y, Y_[0], st_.row_off);
+  be_.copy(X_[0], RX_, st_.n);
+  be_.copy(X_[0], PRX_, st_.n);
+  be_.copy(Y_[0], RY_, st_.m);
+  be_.copy(Y_[0], PRY_, st_.m);
+}
+
+void Engine::upload_params() {
+  auto* P = reinterpret_cast<IterParams*>(pinned_);
+  for (Int l = 0; l < L_; ++l) {
+    IterParams ip{};
+    if (active_[uz(l)]) {
+      ip = ctl_[uz(l)].period_params(st_.eq_end[uz(l)]);
+      ip.active = 1;
+    } else {
+      ip.active = 0;
+      ip.num_equalities = st_.eq_end[uz(l)];
+    }
+    P[l] = ip;
+  }
+  hip_check(hipMemcpyAsync(d_params_, P, uz(L_) * sizeof(IterParams), hipMemcpyHostToDevice,
+                           be_.stream()),
+            "upload multi params");
+}
+
+void Engine::enqueue_period(int steps, bool has_check, int parity) {
+  hipStream_t s = be_.stream();
+  int p = parity;
+  for (int step = 0; step < steps; ++step) {
+    const bool diag = has_check && step == steps - 1;
+    PrimalBufs pb{X_[p], AX_, X_[1 - p], XBAR_, DX_, AX_};
+    const LaunchEvents pe = be_.launch_events(diag ? "multi_primal_step_check" : "multi_primal_step");
+    launch_multi_primal_step(s, ds_.kt, ds_.kt_block_lp, Y_[p], prob_, d_params_, step, pb, diag,
+                             PA_, &pe);
+    DualBufs db{Y_[p], KX_[p], AY_, AKX_, Y_[1 - p], KX_[1 - p], DY_, KDX_, AY_, AKX_};
+    const LaunchEvents de = be_.launch_events(diag ? "multi_dual_step_check" : "multi_dual_step");
+    launch_multi_dual_step(s, ds_.k, ds_.k_block_lp, XBAR_, prob_, d_params_, step, db, diag, PB_,
+                           &de);
+    p = 1 - p;
+  }
+  if (has_check) {
+    HipBackend::Timed t(be_, "multi_check");
+    launch_multi_fold(s, PA_, 1, kSumDx2, 0u, ds_.kt_lp_blocks, L_, d_params_, CHECK_);
+    launch_multi_fold(s, PB_, 7, kSumDy2, (1u << 2) | (1u << 4) | (1u << 5), ds_.k_lp_blocks, L_,
+                      d_params_, CHECK_);
+    launch_multi_conv_product(s, ds_.kt, ds_.kt_block_lp, Y_[p], X_[p], prob_, d_params_, PC_);
+    launch_multi_fold(s, PC_, 4, kDualSq, 1u << 1, ds_.kt_lp_blocks, L_, d_params_, CHECK_);
+    if (opt_.detect_infeasibility) {
+      launch_multi_farkas_product(s, ds_.kt, ds_.kt_block_lp, DY_, prob_, d_params_, CHECK_, PD_);
+      launch_multi_fold(s, PD_, 2, kFarkasColBad, 1u << 0, ds_.kt_lp_blocks, L_, d_params_, CHECK_);
+    }
+  }
+}
+
+hipGraphExec_t Engine::graph_for(int parity) {
+  auto it = graphs_.find(parity);
+  if (it != graphs_.end()) return it->second.second;
+  hipStream_t s = be_.stream();
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  try {
+    enqueue_period(opt_.termination_check_frequency, true, parity);
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+  hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  graphs_[parity] = {g, e};
+  return e;
+}
+
+void Engine::extract(Int l, Outcome* o) {
+  const Int n = st_.cols(l), m = st_.rows(l);
+  o->x.resize(uz(n));
+  o->y.resize(uz(m));
+  if (n > 0)
+    hip_check(hipMemcpyAsync(o->x.data(), X_[cur_] + st_.col_off[uz(l)], uz(n) * sizeof(double),
+                             hipMemcpyDeviceToHost, be_.stream()),
+              "multi extract");
+  if (m > 0)
+    hip_check(hipMemcpyAsync(o->y.data(), Y_[cur_] + st_.row_off[uz(l)], uz(m) * sizeof(double),
+                             hipMemcpyDeviceToHost, be_.stream()),
+              "multi extract");
+}
+
+// A restart for the listed LPs (pdhg.cpp:1124-1202): K x recomputed, the
+// distances moved since the last restart measured, the restart points
+// advanced.
+void Engine::run_restarts(const std::vector<Int>& who) {
+  if (who.empty()) return;
+  std::vector<char> on(uz(L_), 0);
+  for (Int l : who) on[uz(l)] = 1;
+  const std::vector<IterParams> f = flags(on);
+  hipStream_t s = be_.stream();
+  hip_check(hipMemcpyAsync(d_rparams_, f.data(), f.size() * sizeof(IterParams), hipMemcpyHostToDevice, s),
+            "restart flags");
+  launch_multi_product(s, ds_.k, ds_.k_block_lp, X_[cur_], KX_[cur_], d_rparams_);
+  const bool previous = !opt_.pid_primal_weight;
+  launch_multi_sumsq(s, ds_.cols, X_[cur_], previous ? PRX_ : RX_, RX_, PRX_, true, d_rparams_, PCH_,
+                     ds_.col_lp_chunks, L_, RESTART_, 2, 0);
+  launch_multi_sumsq(s, ds_.rows, Y_[cur_], previous ? PRY_ : RY_, RY_, PRY_, true, d_rparams_, PCH_,
+                     ds_.row_lp_chunks, L_, RESTART_, 2, 1);
+  double* h = pinned_ + std::max(uz(L_) * kParamDoubles, uz(L_) * kCheckSlots);
+  hip_check(hipMemcpyAsync(h, RESTART_, uz(L_) * 2 * sizeof(double), hipMemcpyDeviceToHost, s),
+            "restart distances");
+  be_.synchronize();
+  for (Int l : who) ctl_[uz(l)].restart(std::sqrt(h[2 * l]), std::sqrt(h[2 * l + 1]));
+}
+
+// Polished pairs for the LPs `who` (one budget): the primal feasibility
+// problem (c = 0) from (x, 0) and the dual one (q = 0, finite bounds zeroed)
+// from (0, y), each LP with its step size and primal weight, no scaling (the
+// seeds are in the scaled space), as sub-engines over the same stack
+// (pdhg.cpp:680-743); then unscaled with each LP's diagonals.
+struct Polished {
+  std::vector<double> x, y;
+  Int spent = 0;
+};
+
+std::vector<Polished> polish_stack(HipBackend& be, const Stack& st, const DeviceStack& ds,
+                                   const std::vector<double>& norms, const anuk_pdhg_options& opt,
+                                   const std::vector<Int>& who,
+                                   const std::vector<std::vector<double>>& seed_x,
+                                   const std::vector<std::vector<double>>& seed_y,
+                                   const std::vector<double>& omega, const std::vector<double>& eta,
+                                   Int budget, Clock::time_point start) {
+  anuk_pdhg_options sub = opt;
+  sub.polish_feasibility = 0;
+  sub.polish_on_exit = 0;
+  sub.detect_infeasibility = 0;
+  sub.verbose = 0;
+  sub.ruiz_iterations = 0;
+  sub.pock_chambolle = 0;
+  sub.max_iterations = budget;
+  sub.termination_check_frequency =
+      std::max<Int>(10, std::min<Int>(opt.termination_check_frequency, budget / 4));
+  sub.time_limit_seconds = std::fmax(0.0, opt.time_limit_seconds - seconds_since(start));
+  sub.warm_x = nullptr;
+  sub.warm_y = nullptr;
+  const std::vector<double> ones_n(uz(st.n), 1.0), ones_m(uz(st.m), 1.0);
+  const std::vector<double> zero_n(uz(st.n), 0.0), zero_m(uz(st.m), 0.0);
+  std::vector<double> dlo = st.lower, dhi = st.upper;
+  for (size_t j = 0; j < dlo.size(); ++j) {
+    if (dlo[j] > -kInf) dlo[j] = 0.0;
+    if (dhi[j] < kInf) dhi[j] = 0.0;
+  }
+  auto runs_for = [&](bool primal) {
+    std::vector<LpRun> runs(uz(st.lps));
+    for (LpRun& r : runs) r.participates = false;
+    for (size_t s = 0; s < who.size(); ++s) {
+      const Int l = who[s];
+      LpRun& r = runs[uz(l)];
+      r.participates = true;
+      r.initial_omega = omega[s];
+      r.initial_eta = eta[s];
+      if (primal) {
+        r.warm_x = seed_x[s];
+        r.warm_y.assign(uz(st.rows(l)), 0.0);
+      } else {
+        r.warm_x.assign(uz(st.cols(l)), 0.0);
+        r.warm_y = seed_y[s];
+      }
+    }
+    return runs;
+  };
+  std::vector<LpRun> pr = runs_for(true), dr = runs_for(false);
+  sub.termination = ANUK_TERM_PRIMAL_FEASIBILITY;
+  const EngineData pdata{&zero_n, &st.q, &st.lower, &st.upper, &zero_n, &st.q, &st.lower, &st.upper,
+                         &ones_m, &ones_n};
+  const std::vector<Outcome> po = run_engine(be, st, ds, norms, sub, pdata, pr, start, nullptr, nullptr, 1);
+  sub.termination = ANUK_TERM_DUAL_FEASIBILITY;
+  const EngineData ddata{&st.c, &zero_m, &dlo, &dhi, &st.c, &zero_m, &dlo, &dhi, &ones_m, &ones_n};
+  const std::vector<Outcome> dq = run_engine(be, st, ds, norms, sub, ddata, dr, start, nullptr, nullptr, 1);
+  std::vector<Polished> out(who.size());
+  for (size_t s = 0; s < who.size(); ++s) {
+    const Int l = who[s];
+    st.scaling[uz(l)].unscale_primal(po[uz(l)].x, &out[s].x);
+    st.scaling[uz(l)].unscale_dual(dq[uz(l)].y, &out[s].y);
+    out[s].spent = po[uz(l)].iterations + dq[uz(l)].iterations;
+  }
+  return out;
+}
+
+void Engine::run_polish(const std::vector<Int>& who, Int budget, std::vector<Outcome>& out,
+                        std::vector<char>* stopped) {
+  std::vector<std::vector<double>> sx, sy;
+  std::vector<double> omega, eta;
+  for (Int l : who) {
+    Outcome tmp;
+    extract(l, &tmp);
+    sx.push_back(std::move(tmp.x));
+    sy.push_back(std::move(tmp.y));
+    omega.push_back(ctl_[uz(l)].omega);
+    eta.push_back(ctl_[uz(l)].eta);
+  }
+  be_.synchronize();
+  std::vector<Polished> pol = polish_stack(be_, st_, ds_, norms_, opt_, who, sx, sy, omega, eta,
+                                           budget, start_);
+  for (size_t s = 0; s < who.size(); ++s) {
+    const Int l = who[s];
+    Outcome& o = out[uz(l)];
+    const Residual tr = evaluate_residual(*(*polish_->originals)[uz(l)], pol[s].x, pol[s].y);
+    ++o.polish_attempts;
+    o.polish_iterations += pol[s].spent;
+    Control& c = ctl_[uz(l)];
+    if (tr.converged(c.tol, opt_.absolute_tolerance, opt_.require_inf_norm_termination != 0, c.gap_tol)) {
+      o.x = std::move(pol[s].x);
+      o.y = std::move(pol[s].y);
+      o.polished = true;
+      o.polished_r = tr;
+      c.status = ANUK_PDHG_OPTIMAL;
+      c.message = "converged after feasibility polishing";
+      (*stopped)[uz(l)] = 1;
+    }
+  }
+}
+
+double range_two_norm(const std::vector<double>& v, Int a, Int b) {
+  double s = 0.0;
+  for (Int i = a; i < b; ++i) s += v[uz(i)] * v[uz(i)];
+  return std::sqrt(s);
+}
+
+double range_inf_norm(const std::vector<double>& v, Int a, Int b) {
+  double s = 0.0;
+  for (Int i = a; i < b; ++i) s = std::fmax(s, std::fabs(v[uz(i)]));
+  return s;
+}
+
+std::vector<Outcome> Engine::run() {
+  Clock::time_point mark = Clock::now();
+  L_ = st_.lps;
+  allocate();
+  ctl_.resize(uz(L_));
+  active_.assign(uz(L_), 0);
+  for (Int l = 0; l < L_; ++l) {
+    LpRun& r = runs_[uz(l)];
+    if (!r.participates) continue;
+    const Int c0 = st_.col_off[uz(l)], c1 = st_.col_off[uz(l) + 1];
+    const Int r0 = st_.row_off[uz(l)], r1 = st_.row_off[uz(l) + 1];
+    r.q_two = range_two_norm(*data_.oq, r0, r1);
+    r.c_two = range_two_norm(*data_.oc, c0, c1);
+    r.q_inf = range_inf_norm(*data_.oq, r0, r1);
+    r.c_inf = range_inf_norm(*data_.oc, c0, c1);
+    Control& c = ctl_[uz(l)];
+    c.init(opt_, norms_[uz(l)], range_two_norm(*data_.c, c0, c1), range_two_norm(*data_.q, r0, r1));
+    if (r.initial_eta > 0.0) c.eta = r.initial_eta;
+    if (r.initial_omega > 0.0) c.omega = r.initial_omega;
+    active_[uz(l)] = norms_[uz(l)] > 0.0 ? 1 : 0;
+  }
+  upload();
+  upload_params();
+  launch_multi_product(be_.stream(), ds_.k, ds_.k_block_lp, X_[0], KX_[0], d_params_);
+  cur_ = 0;
+  auto lap = [&](double Phases::*field) {
+    const Clock::time_point now = Clock::now();
+    if (phases_ != nullptr) phases_->*field += std::chrono::duration<double>(now - mark).count();
+    mark = now;
+  };
+  be_.synchronize();
+  lap(&Phases::setup);
+
+  std::vector<Outcome> out(uz(L_));
+  auto any_active = [&] { return std::any_of(active_.begin(), active_.end(), [](char c) { return c; }); };
+  const Int freq = opt_.termination_check_frequency;
+  Int iteration = 0;
+  while (iteration < opt_.max_iterations && any_active()) {
+    const Int period_end = std::min<Int>((iteration / freq + 1) * freq, opt_.max_iterations);
+    const int steps = period_end - iteration;
+    const bool has_check = (period_end % freq == 0);
+    upload_params();
+    const bool graph = opt_.use_graphs != 0 && has_check && steps == freq && !be_.profiling();
+    if (graph) {
+      hip_check(hipGraphLaunch(graph_for(cur_), be_.stream()), "hipGraphLaunch(multi)");
+    } else {
+      enqueue_period(steps, has_check, cur_);
+    }
+    for (Int l = 0; l < L_; ++l)
+      if (active_[uz(l)]) ctl_[uz(l)].advance(steps);
+    if (steps % 2 == 1) cur_ = 1 - cur_;
+    iteration = period_end;
+    if (!has_check) {
+      be_.synchronize();
+      lap(&Phases::iterating);
+      continue;
+    }
+    double* h_check = pinned_;
+    hip_check(hipMemcpyAsync(h_check, CHECK_, uz(L_) * kCheckSlots * sizeof(double),
+                             hipMemcpyDeviceToHost, be_.stream()),
+              "multi check");
+    be_.synchronize();
+    lap(&Phases::iterating);
+    const Int i = period_end - 1;
+    const double elapsed = seconds_since(start_);
+    std::vector<char> stopped(uz(L_), 0);
+    std::vector<Residual> res(uz(L_));
+    std::map<Int, std::vector<Int>> polish_by_budget;
+    for (Int l = 0; l < L_; ++l) {
+      if (!active_[uz(l)]) continue;
+      Control& c = ctl_[uz(l)];
+      const LpRun& r = runs_[uz(l)];
+      const double* h = h_check + uz(l) * kCheckSlots;
+      c.movement = c.omega * h[kSumDx2] + h[kSumDy2] / c.omega;
+      bool farkas = false;
+      if (opt_.detect_infeasibility) {
+        farkas = h[kFarkasRowBad] == 0.0 && h[kFarkasScale] > 0.0 && h[kFarkasColBad] == 0.0 &&
+                 (h[kFarkasRhs] - h[kFarkasSupport]) / h[kFarkasScale] > opt_.infeasibility_tolerance;
+      }
+      Terms t;
+      t.primal_sq = h[kPrimalSq];
+      t.primal_inf = h[kPrimalInf];
+      t.q_dot_y = h[kQDotY];
+      t.dual_sq = h[kDualSq];
+      t.dual_inf = h[kDualInf];
+      t.bound_term = h[kBoundTerm];
+      t.c_dot_x = h[kCDotX];
+      ResidualScales rs;
+      rs.q_two = r.q_two;
+      rs.c_two = r.c_two;
+      rs.q_inf = r.q_inf;
+      rs.c_inf = r.c_inf;
+      res[uz(l)] = residual_from_terms(t, rs);
+      Int budget = 0;
+      const Control::Verdict v = c.check_first(i, res[uz(l)], farkas, &budget);
+      if (v == Control::Verdict::kStop) {
+        stopped[uz(l)] = 1;
+      } else if (v == Control::Verdict::kPolish) {
+        polish_by_budget[budget].push_back(l);
+      }
+    }
+    lap(&Phases::checks);
+    for (const auto& kv : polish_by_budget) {
+      if (polish_ == nullptr) continue;
+      run_polish(kv.second, kv.first, out, &stopped);
+    }
+    lap(&Phases::polishing);
+    std::vector<Int> restart;
+    for (Int l = 0; l < L_; ++l) {
+      if (!active_[uz(l)] || stopped[uz(l)]) continue;
+      const Control::Verdict v = ctl_[uz(l)].check_second(i, res[uz(l)], 0.0, elapsed);
+      if (v == Control::Verdict::kStop) {
+        stopped[uz(l)] = 1;
+      } else if (v == Control::Verdict::kRestart) {
+        restart.push_back(l);
+      }
+    }
+    for (Int l = 0; l < L_; ++l) {
+      if (!stopped[uz(l)]) continue;
+      Outcome& o = out[uz(l)];
+      o.iterations = i + 1;
+      o.seconds = seconds_since(start_);
+      if (!o.polished) extract(l, &o);
+      active_[uz(l)] = 0;
+    }
+    run_restarts(restart);
+    be_.synchronize();
+    lap(&Phases::checks);
+  }
+  for (Int l = 0; l < L_; ++l) {
+    if (!active_[uz(l)]) continue;
+    Outcome& o = out[uz(l)];
+    o.iterations = iteration;
+    o.seconds = seconds_since(start_);
+    extract(l, &o);
+  }
+  be_.synchronize();
+  for (Int l = 0; l < L_; ++l) {
+    Outcome& o = out[uz(l)];
+    const Control& c = ctl_[uz(l)];
+    if (!runs_[uz(l)].participates) continue;
+    if (norms_[uz(l)] <= 0.0) {
+      o.status = ANUK_PDHG_NUMERICAL_ERROR;
+      o.message = "the constraint matrix is empty";
+      continue;
+    }
+    o.status = c.status;
+    o.message = c.message;
+    o.restarts = c.restarts;
+    o.eta = c.eta;
+    o.omega = c.omega;
+  }
+  return out;
+}
+
+std::vector<Outcome> run_engine(HipBackend& be, const Stack& st, const DeviceStack& ds,
+                                const std::vector<double>& norms, const anuk_pdhg_options& opt,
+                                const EngineData& data, std::vector<LpRun>& runs,
+                                Clock::time_point start, const PolishContext* polish,
+                                Phases* phases, int depth) {
+  Engine e(be, st, ds, norms, opt, data, runs, start, polish, phases, depth);
+  return e.run();
+}
+
+}  // namespace
+
+std::vector<SolveResult> solve_pdhg_multi(HipBackend& be, const std::vector<const StandardLp*>& lps,
+                                          const anuk_pdhg_options& opt) {
+  const Clock::time_point start = Clock::now();
+  const char* why = nullptr;
+  if (!batch_supports(opt, &why)) throw std::invalid_argument(std::string("multi: unsupported option: ") + why);
+  if (lps.empty()) return {};
+  Phases phases;
+  Clock::time_point mark = start;
+  auto lap = [&](double Phases::*field) {
+    const Clock::time_point now = Clock::now();
+    phases.*field += std::chrono::duration<double>(now - mark).count();
+    mark = now;
+  };
+  const Stack st = build_stack(lps, opt);
+  lap(&Phases::scaling);
+  DeviceStack ds(be);
+  upload_stack(st, &ds);
+  std::vector<double> norms(uz(st.lps), opt.known_matrix_norm);
+  if (!(opt.known_matrix_norm > 0.0)) norms = stack_norms(be, st, ds, 200, 1e-8);
+  lap(&Phases::matrix_norm);
+
+  std::vector<LpRun> runs(uz(st.lps));
+  const EngineData data{&st.c, &st.q, &st.lower, &st.upper, &st.oc, &st.oq, &st.olo, &st.ohi,
+                        &st.row_scale, &st.col_scale};
+  PolishContext pc{&lps};
+  std::vector<Outcome> outcomes = run_engine(be, st, ds, norms, opt, data, runs, start,
+                                             opt.polish_feasibility ? &pc : nullptr, &phases, 0);
+
+  // Final answers, and one more polish for the LPs that ran out of
+  // iterations (pdhg.cpp:1205-1269), grouped by budget.
+  std::vector<SolveResult> results(lps.size());
+  std::map<Int, std::vector<Int>> exit_polish;
+  for (size_t i = 0; i < lps.size(); ++i) {
+    Outcome& o = outcomes[i];
+    SolveResult& r = results[i];
+    r.status = o.status;
+    r.message = o.message;
+    r.iterations = o.iterations;
+    r.restarts = o.restarts;
+    r.polish_attempts = o.polish_attempts;
+    r.polish_iterations = o.polish_iterations;
+    r.matrix_norm_estimate = norms[i];
+    r.final_step_size = o.eta;
+    r.final_primal_weight = o.omega;
+    r.solve_seconds = o.seconds;
+    if (o.polished) {
+      r.x = std::move(o.x);
+      r.y = std::move(o.y);
+      r.residual = o.polished_r;
+      r.polished = true;
+    } else {
+      st.scaling[i].unscale_primal(o.x, &r.x);
+      st.scaling[i].unscale_dual(o.y, &r.y);
+      r.residual = evaluate_residual(*lps[i], r.x, r.y);
+      const bool worth = opt.polish_feasibility && opt.polish_on_exit &&
+                         opt.termination == ANUK_TERM_FULL && r.status != ANUK_PDHG_OPTIMAL &&
+                         r.status != ANUK_PDHG_PRIMAL_INFEASIBLE &&
+                         r.status != ANUK_PDHG_NUMERICAL_ERROR &&
+                         std::isfinite(r.residual.primal_residual) &&
+                         std::isfinite(r.residual.dual_residual) &&
+                         seconds_since(start) < opt.time_limit_seconds;
+      if (worth) {
+        Control c;
+        c.opt = &opt;
+        const Int budget = c.polish_budget(r.iterations);
+        if (budget > 0) exit_polish[budget].push_back(static_cast<Int>(i));
+      }
+    }
+  }
+  mark = Clock::now();
+  const double gap_tol = opt.gap_tolerance > 0.0 ? opt.gap_tolerance : opt.tolerance;
+  for (const auto& kv : exit_polish) {
+    std::vector<std::vector<double>> sx, sy;
+    std::vector<double> omega, eta;
+    for (Int i : kv.second) {
+      sx.push_back(outcomes[uz(i)].x);
+      sy.push_back(outcomes[uz(i)].y);
+      omega.push_back(outcomes[uz(i)].omega);
+      eta.push_back(outcomes[uz(i)].eta);
+    }
+    std::vector<Polished> pol = polish_stack(be, st, ds, norms, opt, kv.second, sx, sy, omega, eta,
+                                             kv.first, start);
+    for (size_t s = 0; s < kv.second.size(); ++s) {
+      SolveResult& r = results[uz(kv.second[s])];
+      Polished& p = pol[s];
+      const Residual tr = evaluate_residual(*lps[uz(kv.second[s])], p.x, p.y);
+      ++r.polish_attempts;
+      r.polish_iterations += p.spent;
+      const double ceiling = std::fmax(gap_tol, r.residual.relative_gap);
+      const bool better = tr.relative_primal <= r.residual.relative_primal &&
+                          tr.relative_dual <= r.residual.relative_dual &&
+                          (tr.relative_primal < r.residual.relative_primal ||
+                           tr.relative_dual < r.residual.relative_dual);
+      if (better && tr.relative_gap <= ceiling) {
+        r.x = std::move(p.x);
+        r.y = std::move(p.y);
+        r.residual = tr;
+        r.polished = true;
+        if (tr.converged(opt.tolerance, opt.absolute_tolerance, opt.require_inf_norm_termination != 0,
+                         gap_tol)) {
+          r.status = ANUK_PDHG_OPTIMAL;
+          r.message = "converged after feasibility polishing";
+        } else {
+          r.message = r.message.empty() ? "feasibility polished" : r.message + ", feasibility polished";
+        }
+      }
+    }
+  }
+  lap(&Phases::polishing);
+  for (size_t i = 0; i < lps.size(); ++i) {
+    results[i].objective = lps[i]->model_objective(results[i].x);
+    results[i].phases = phases;
+  }
+  return results;
+}
+
+}  // namespace anuk

@@ -380,3 +380,21 @@ def random_lp(m: int = 400_000, n: int = 1_000_000, mean_col_nnz: float = 8.0,
                         row_lower=row_lower, row_upper=row_upper, col_lower=np.zeros(n),
                         col_upper=col_upper, name=f"random{m}x{n}",
                         tags={"zipf_exponent": a, "nnz": int(key.size)})
+
+
+def uneven_stack(count: int = 512, seed: int = 20261206, max_rows: int = 50_000) -> List[NaturalModel]:
+    """Independent LPs of uneven size, each with its own constraint matrix:
+    count // 2 refinery plans with period counts drawn uniformly from
+    {4, 12, 24, 52} (each with its own data seed), and the rest random LPs
+    (random_lp) with m ~ logU[1e3, max_rows], n = 2.5 m and a mean of 8
+    nonzeros per column, shuffled together."""
+    g = np.random.default_rng(np.random.PCG64([seed, count]))
+    models = []
+    for i in range(count // 2):
+        periods = int(g.choice([4, 12, 24, 52]))
+        models.append(refinery_model(periods, seed=seed + 1 + i))
+    for i in range(count - count // 2):
+        m = int(round(float(np.exp(g.uniform(np.log(1e3), np.log(max_rows))))))
+        models.append(random_lp(m=m, n=int(2.5 * m), seed=seed + 100_000 + i))
+    order = g.permutation(len(models))
+    return [models[i] for i in order]

@@ -9,7 +9,9 @@ steps. Beside it: the same stack solved one case at a time by the plain GPU
 solver (`plain`), a profiled pass that times every kernel launch with HIP
 events the dispatch records (hipExtLaunchKernel) and divides each launch's
 algorithmic bytes by its time (`roofline`, dominant kernel), the
-1M-column random LP's SpMV bandwidth and plain solve (`c1`), and the
+1M-column random LP's SpMV bandwidth and plain solve (`c1`), the uneven
+512-LP stack solved in one set of launches (`c3`: LPs/s, p50/p99
+time-to-solve), and the
 reference's own solve_pdhg (oracle/_ref, one thread) on a bounded sample of
 the cases (`cpu_baseline`).
 
@@ -143,6 +145,47 @@ def c1_section(ctx, args):
                       "phases_s": {k2: round(v, 4) for k2, v in r.phases.items()}}}
 
 
+def c3_section(ctx, args):
+    """The uneven stack: count independent LPs (half refinery plans of 4-52
+    periods, half random LPs with m ~ logU[1e3, 5e4], n = 2.5 m), each with its
+    own K, solved to rel KKT 1e-4 in one set of launches over their
+    block-diagonal stack; beside it, plain GPU solves of the same LPs one at a
+    time on a bounded sample."""
+    import numpy as np
+    from anukulan_amd import generators as G, solve_pdhg, solve_pdhg_multi
+
+    t0 = time.perf_counter()
+    lps = [m.to_lp() for m in G.uneven_stack(args.c3_count)]
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rs = solve_pdhg_multi(ctx, lps, tolerance=1e-4, use_graphs=0 if args.no_graphs else 1)
+    solve_s = time.perf_counter() - t0
+    tts = np.array([r.solve_seconds for r in rs])
+    plain_n, plain_s, worst = 0, 0.0, 0.0
+    while plain_n < len(lps) and plain_s < args.c3_plain_seconds:
+        t1 = time.perf_counter()
+        p = solve_pdhg(ctx, lps[plain_n], tolerance=1e-4, use_graphs=0 if args.no_graphs else 1)
+        plain_s += time.perf_counter() - t1
+        worst = max(worst, abs(p.objective - rs[plain_n].objective) / max(1.0, abs(p.objective)))
+        plain_n += 1
+    return {"workload": f"{len(lps)} independent LPs of uneven size (half refinery plans of 4-52 "
+                        f"periods, half random LPs m~logU[1e3,5e4], n=2.5m), each its own K, "
+                        f"rel KKT 1e-4",
+            "rows": int(sum(lp.rows for lp in lps)), "cols": int(sum(lp.cols for lp in lps)),
+            "nnz": int(sum(lp.nnz for lp in lps)), "build_s": round(build_s, 2),
+            "lps_per_s": round(len(lps) / solve_s, 2), "seconds": round(solve_s, 3),
+            "optimal": int(sum(r.status == "optimal" for r in rs)),
+            "time_to_solve_s": {"p50": round(float(np.percentile(tts, 50)), 4),
+                                "p99": round(float(np.percentile(tts, 99)), 4),
+                                "max": round(float(tts.max()), 4)},
+            "iterations": {"p50": int(np.percentile([r.iterations for r in rs], 50)),
+                           "max": int(max(r.iterations for r in rs))},
+            "phases_s": {k: round(v, 4) for k, v in rs[0].phases.items()},
+            "plain_sample": {"lps": plain_n, "seconds": round(plain_s, 3),
+                             "lps_per_s": round(plain_n / plain_s, 2) if plain_s > 0 else None,
+                             "max_rel_objective_diff": worst}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +202,10 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly (for external profilers)")
     ap.add_argument("--no-c1", action="store_true", help="skip the 1M-column random LP")
+    ap.add_argument("--no-c3", action="store_true", help="skip the uneven 512-LP stack")
+    ap.add_argument("--c3-count", type=int, default=512)
+    ap.add_argument("--c3-plain-seconds", type=float, default=10.0,
+                    help="time spent on plain one-at-a-time solves of the uneven stack's LPs")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB and exit: the "
                          "known-size reference for HBM byte counters")
@@ -241,6 +288,7 @@ def main():
                         "mean_launch_us": k["mean_us"], "launches_timed": k["timed"]}
 
     c1 = None if args.no_c1 else c1_section(ctx, args)
+    c3 = None if args.no_c3 else c3_section(ctx, args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -258,7 +306,7 @@ def main():
                    "rows": m, "cols": n, "nnz": nnz, "cases": len(lps),
                    "cases_optimal": solved, "lane_iterations_per_step": lane_iters,
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "plain": plain, "c1": c1,
+        "roofline": roofline, "cpu_baseline": cpu, "plain": plain, "c1": c1, "c3": c3,
         "batch_phases_s": {k: round(v, 4) for k, v in results[0].phases.items()},
         "kernels": {k: {"mean_us": v["mean_us"], "calls": v["calls"], "GB/s": v["gbs"]}
                     for k, v in kernels.items()},

@@ -370,6 +370,70 @@ int anuk_solve_pdhg_batch(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t coun
                           const anuk_pdhg_options* options, anuk_pdhg_result* results,
                           double* x_out, double* y_out);
 
+/* Solves `count` independent LPs, each with its own constraint matrix and
+ * of any size, in one set of launches: the scaled matrices are concatenated
+ * block-diagonally and every period of fused steps runs over the whole
+ * stack, each LP under its own control (as its anuk_solve_pdhg would), an
+ * LP that has stopped dropping out of the launches. x_out receives the
+ * primals end to end (sum of cols), y_out the duals (sum of rows); either may
+ * be NULL. Runs the options anuk_solve_pdhg_batch runs. No reference
+ * counterpart: the reference solves LPs one at a time. */
+int anuk_solve_pdhg_multi(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,
+                          const anuk_pdhg_options* options, anuk_pdhg_result* results,
+                          double* x_out, double* y_out);
+
+/* ------------------------------------------------------------------------ */
+/* Sensitivities of a solution (no reference counterpart)                    */
+/* ------------------------------------------------------------------------ */
+
+enum anuk_adjoint_method {
+  ANUK_ADJOINT_CG = 0,          /* CGLS on the fixed-point equations */
+  ANUK_ADJOINT_FIXED_POINT = 1  /* the transposed PDHG map iterated as is */
+};
+
+typedef struct anuk_adjoint_options {
+  int32_t method;           /* ANUK_ADJOINT_CG */
+  int32_t max_iterations;   /* 20000 */
+  double tolerance;         /* 1e-11: CG stops at this relative normal-equation
+                               residual; the fixed-point method reports
+                               converged when both residuals are below its
+                               square root */
+  double active_tolerance;  /* 1e-6: a column within this * (1 + |bound|) of a
+                               bound is at it; a row within this * (1 + |q|)
+                               of its rhs binds */
+  double primal_weight;     /* 1.0: omega of the fixed-point method's steps,
+                               tau = 0.9 / (||K~|| omega), sigma = 0.9 omega / ||K~|| */
+} anuk_adjoint_options;
+
+typedef struct anuk_adjoint_info {
+  int32_t iterations;
+  int32_t converged;
+  double residual_primal;   /* ||B(B'mu - g)|| / ||B g|| on the active set */
+  double residual_dual;     /* ||B'(B nu - h)|| / ||B' h|| */
+  int32_t free_columns;
+  int32_t binding_rows;
+  int32_t lower_columns;
+  int32_t upper_columns;
+} anuk_adjoint_info;
+
+void anuk_adjoint_default_options(anuk_adjoint_options* options);
+
+/* Gradients of L = w_x' x + w_y' y at the PDHG solutions (x, y) of `count`
+ * LPs that share K, with respect to each LP's c, q, lower and upper (all in
+ * standard form, unscaled), by implicit differentiation of the PDHG fixed
+ * point on the final active set: the transposed map's fixed point, reached
+ * by the method in options, one wave64 lane per LP. `scaling` gives the
+ * solve's ruiz_iterations and pock_chambolle (the space the map lives in).
+ * x is count * cols, y count * rows, in LP order; w_x / w_y likewise or NULL
+ * for a zero seed; any gradient pointer or info may be NULL. No counterpart
+ * in the reference, which has no sensitivities; for callers that want
+ * d(x*, y*)/d(c, q, bounds) without re-solving. */
+int anuk_pdhg_adjoint(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,
+                      const anuk_pdhg_options* scaling, const anuk_adjoint_options* options,
+                      const double* x, const double* y, const double* w_x, const double* w_y,
+                      double* grad_c, double* grad_q, double* grad_lower, double* grad_upper,
+                      anuk_adjoint_info* info);
+
 /* estimate_matrix_norm (pdhg.hpp:580-581, pdhg.cpp:223-282) of lp's K as
  * given (no scaling), by power iteration on the device. */
 int anuk_estimate_matrix_norm(anuk_ctx* ctx, const anuk_lp* lp, int32_t iterations,

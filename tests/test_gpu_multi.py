@@ -1,0 +1,105 @@
+"""Stacks of independent LPs of uneven size, each with its own K, solved in
+one set of launches (solve_pdhg_multi).
+
+No counterpart in the reference (it solves LPs one at a time), so LP i of a
+stack is held to what its own plain GPU solve reaches and to the reference's
+fixtures: same status, objective within 1e-8 relative, iterations within 2%.
+Row sums and elementwise steps are the plain kernels' (the same bits); the
+per-LP reductions and ||K|| follow a different fixed tree, which only moves
+decisions at rounding level.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from anukulan_amd import AnukError, generators as G, solve_pdhg, solve_pdhg_multi
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "pdhg_reference.json")))
+
+
+def relerr(a, b):
+    return abs(a - b) / max(1.0, abs(b))
+
+
+def mixed_models():
+    base, cases = G.case_stack(4)
+    return ([G.refinery_model(4, seed=7), G.refinery_model(12)] +
+            [G.case_model(base, c) for c in cases[:3]] +
+            [G.random_lp(m=2000, n=5000), G.random_lp(m=700, n=1750, seed=11),
+             G.random_lp(m=3000, n=7500, seed=12)])
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+def test_stack_matches_plain_solves(ctx, tol):
+    lps = [m.to_lp() for m in mixed_models()]
+    got = solve_pdhg_multi(ctx, lps, tolerance=tol)
+    same = 0
+    for lp, g in zip(lps, got):
+        p = solve_pdhg(ctx, lp, tolerance=tol)
+        assert g.status == p.status == "optimal"
+        assert relerr(g.objective, p.objective) <= 1e-8, (g.objective, p.objective)
+        assert abs(g.iterations - p.iterations) <= 0.02 * p.iterations, (g.iterations, p.iterations)
+        same += g.iterations == p.iterations
+    print(f"\n{same} of {len(lps)} LPs took exactly their plain solve's iteration count")
+
+
+def test_stack_reaches_the_reference_fixtures(ctx):
+    ref12 = FIXTURE["refinery"]["12"]["1e-4"]
+    refr = FIXTURE["random"]["2000x5000"]["1e-4"]
+    lps = [G.random_lp(m=700, n=1750, seed=11).to_lp(), G.refinery_model(12).to_lp(),
+           G.random_lp(m=2000, n=5000).to_lp(), G.refinery_model(4, seed=7).to_lp()]
+    got = solve_pdhg_multi(ctx, lps, tolerance=1e-4)
+    for g, want in ((got[1], ref12), (got[2], refr)):
+        assert g.status == want["status"]
+        assert relerr(g.objective, want["objective"]) <= 1e-8
+        assert abs(g.iterations - want["iterations"]) <= 0.02 * want["iterations"]
+
+
+def test_one_lp_stack_is_the_plain_solve(ctx):
+    lp = G.refinery_model(4).to_lp()
+    g = solve_pdhg_multi(ctx, [lp], tolerance=1e-8)[0]
+    p = solve_pdhg(ctx, lp, tolerance=1e-8)
+    assert g.status == p.status
+    assert relerr(g.objective, p.objective) <= 1e-10
+    assert abs(g.iterations - p.iterations) <= 0.02 * p.iterations
+
+
+def test_infeasible_member_does_not_disturb_the_rest(ctx):
+    from golden.known_lps import natural
+    inf = natural("infeasible").to_lp()
+    feas = G.refinery_model(4).to_lp()
+    got = solve_pdhg_multi(ctx, [feas, inf, feas], tolerance=1e-8)
+    assert [g.status for g in got] == ["optimal", "primal infeasible", "optimal"]
+    assert got[0].iterations == got[2].iterations
+    assert np.array_equal(got[0].x, got[2].x)
+
+
+def test_iteration_limit_and_exit_polish_match_plain(ctx):
+    lps = [G.refinery_model(4).to_lp(), G.random_lp(m=700, n=1750, seed=11).to_lp()]
+    got = solve_pdhg_multi(ctx, lps, tolerance=1e-9, max_iterations=200)
+    for lp, g in zip(lps, got):
+        p = solve_pdhg(ctx, lp, tolerance=1e-9, max_iterations=200)
+        assert g.iterations == p.iterations == 200
+        assert (g.status, g.polish_attempts) == (p.status, p.polish_attempts)
+        assert relerr(g.objective, p.objective) <= 1e-8
+
+
+def test_rejects_options_it_does_not_run(ctx):
+    with pytest.raises(AnukError) as e:
+        solve_pdhg_multi(ctx, [G.refinery_model(4).to_lp()], constant_step_size=0)
+    assert e.value.code == 4
+
+
+def test_small_uneven_stack_solves(ctx):
+    models = G.uneven_stack(24, max_rows=4000)
+    lps = [m.to_lp() for m in models]
+    got = solve_pdhg_multi(ctx, lps, tolerance=1e-4)
+    assert all(g.status == "optimal" for g in got)
+    for lp, g in zip(lps, got):
+        r = lp.evaluate_residual(g.x, g.y)
+        assert r["relative_primal"] <= 1e-4 and r["relative_dual"] <= 1e-4
+        assert r["relative_gap"] <= 1e-4

@@ -66,7 +66,7 @@ class PdhgOptions(ctypes.Structure):
         ("termination", c_int32), ("warm_x", DP), ("warm_y", DP),
         ("known_matrix_norm", c_double), ("initial_step_size", c_double),
         ("initial_primal_weight", c_double), ("verbose", c_int32), ("log_frequency", c_int32),
-        ("stop", IP), ("use_graphs", c_int32),
+        ("stop", IP), ("use_graphs", c_int32), ("product_order", c_int32),
     ]
 
 
@@ -138,6 +138,7 @@ SIGNATURES = {
     "anuk_lp_info_get": (c_int, [c_void_p, POINTER(LpInfo)]),
     "anuk_lp_export": (c_int, [c_void_p, IP, IP, DP, DP, DP, DP, DP, IP, DP]),
     "anuk_lp_scale": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_void_p), DP, DP]),
+    "anuk_lp_scale_device": (c_int, [c_void_p, c_void_p, c_int32, c_int32, POINTER(c_void_p), DP, DP]),
     "anuk_pdhg_default_options": (None, [POINTER(PdhgOptions)]),
     "anuk_solve_pdhg": (c_int, [c_void_p, c_void_p, POINTER(PdhgOptions), POINTER(PdhgResult), DP, DP]),
     "anuk_solve_pdhg_batch": (c_int, [c_void_p, POINTER(c_void_p), c_int32, POINTER(PdhgOptions),

@@ -302,6 +302,14 @@ class LP:
                                _dp(rs), _dp(cs))
         return LP._made(rc, h), rs, cs
 
+    def scaled_on_device(self, ctx: "Context", ruiz_iterations=10, pock_chambolle=True):
+        """scaled() computed on the device (the same bits)."""
+        rs, cs = np.empty(self.rows), np.empty(self.cols)
+        h = c_void_p()
+        ctx._call(lib.anuk_lp_scale_device, self.h, int(ruiz_iterations),
+                  1 if pock_chambolle else 0, byref(h), _dp(rs), _dp(cs))
+        return LP(h), rs, cs
+
     def evaluate_residual(self, x, y) -> dict:
         x, y = _f64(x), _f64(y)
         r = _lib.PdhgResidual()

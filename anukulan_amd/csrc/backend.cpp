@@ -239,7 +239,7 @@ const DevCsr& HipBackend::prepare(const CsrView& a) {
   };
   Int* block_rows = nullptr;
   Int blocks = 0;
-  plan_blocks(a.rows, a.start, kTileNnz, kThreads, &block_rows, &blocks);
+  plan_blocks(a.rows, a.start, kTileNnz, kTileRows, &block_rows, &blocks);
   dm.view.rows = a.rows;
   dm.view.cols = a.cols;
   dm.view.nnz = nnz;
@@ -255,6 +255,58 @@ const DevCsr& HipBackend::prepare(const CsrView& a) {
   return ins.first->second.view;
 }
 
+namespace {
+// Cache key of a matrix's panel stack: the matrix's own identity with the top
+// bit flipped (identities are drawn from a counter and never reach it).
+constexpr std::uint64_t kPanelKey = 1ull << 63;
+}  // namespace
+
+const DevCsr& HipBackend::prepare_panels(const CsrView& a) {
+  const std::uint64_t key = a.id ^ kPanelKey;
+  auto it = matrices_.find(key);
+  if (it != matrices_.end()) return it->second.view;
+  const Int nnz = a.nnz();
+  // Column boundaries that split the nonzeros into kPanels near-equal parts.
+  std::vector<long> col_nnz(uz(a.cols) + 1, 0);
+  for (Int k = 0; k < nnz; ++k) ++col_nnz[uz(a.index[k]) + 1];
+  for (Int j = 0; j < a.cols; ++j) col_nnz[uz(j) + 1] += col_nnz[uz(j)];
+  std::vector<Int> bound(kPanels + 1, a.cols);
+  bound[0] = 0;
+  for (int p = 1; p < kPanels; ++p) {
+    const long target = static_cast<long>(nnz) * p / kPanels;
+    bound[uz(p)] = static_cast<Int>(std::lower_bound(col_nnz.begin(), col_nnz.end(), target) - col_nnz.begin());
+    bound[uz(p)] = std::min(std::max(bound[uz(p)], bound[uz(p) - 1]), a.cols);
+  }
+  // Row r of panel p: the part of row r with columns in [bound[p], bound[p+1]).
+  const Int rows = a.rows;
+  std::vector<Int> start(uz(kPanels) * uz(rows) + 1, 0);
+  std::vector<Int> cut(uz(rows) * (kPanels + 1));
+  for (Int r = 0; r < rows; ++r) {
+    const Int* b = a.index + a.start[r];
+    const Int* e = a.index + a.start[r + 1];
+    for (int p = 0; p <= kPanels; ++p)
+      cut[uz(r) * (kPanels + 1) + uz(p)] =
+          static_cast<Int>(std::lower_bound(b, e, bound[uz(p)]) - a.index);
+    cut[uz(r) * (kPanels + 1) + kPanels] = a.start[r + 1];
+  }
+  size_t at = 0;
+  std::vector<Int> index(uz(nnz));
+  std::vector<double> value(uz(nnz));
+  for (int p = 0; p < kPanels; ++p) {
+    for (Int r = 0; r < rows; ++r) {
+      const Int k0 = cut[uz(r) * (kPanels + 1) + uz(p)], k1 = cut[uz(r) * (kPanels + 1) + uz(p) + 1];
+      for (Int k = k0; k < k1; ++k) {
+        index[at] = a.index[k];
+        value[at] = a.value[k];
+        ++at;
+      }
+      start[uz(p) * uz(rows) + uz(r) + 1] = static_cast<Int>(at);
+    }
+  }
+  const CsrView stacked{key, kPanels * rows, a.cols, start.data(), index.data(), value.data()};
+  return prepare(stacked);
+}
+
 void HipBackend::release() {
   synchronize();
   for (auto& kv : matrices_) free_matrix(&kv.second);
@@ -262,11 +314,13 @@ void HipBackend::release() {
 }
 
 void HipBackend::release_one(std::uint64_t id) {
-  auto it = matrices_.find(id);
-  if (it == matrices_.end()) return;
-  synchronize();
-  free_matrix(&it->second);
-  matrices_.erase(it);
+  for (const std::uint64_t key : {id, id ^ kPanelKey}) {
+    auto it = matrices_.find(key);
+    if (it == matrices_.end()) continue;
+    synchronize();
+    free_matrix(&it->second);
+    matrices_.erase(it);
+  }
 }
 
 void HipBackend::multiply(const CsrView& a, const double* x, double* y) {

@@ -60,6 +60,9 @@ class HipBackend {
 
   // matrices (backend.hpp:35-36, 61-64)
   const DevCsr& prepare(const CsrView& a);
+  // The column-panel stack of `a` (kPanels panels balanced by nonzeros, see
+  // kernels.hpp), built on the host and cached with `a`.
+  const DevCsr& prepare_panels(const CsrView& a);
   void release();
   void release_one(std::uint64_t id);
   void multiply(const CsrView& a, const double* x, double* y);

@@ -14,6 +14,7 @@
 #include "adjoint.hpp"
 #include "batch.hpp"
 #include "multi.hpp"
+#include "scaling.hpp"
 #include "pdhg.hpp"
 
 struct anuk_ctx {
@@ -436,6 +437,25 @@ int anuk_lp_scale(const anuk_lp* lp, int32_t ruiz_iterations, int32_t pock_chamb
   });
 }
 
+int anuk_lp_scale_device(anuk_ctx* ctx, const anuk_lp* lp, int32_t ruiz_iterations,
+                         int32_t pock_chambolle, anuk_lp** scaled, double* row_scale,
+                         double* col_scale) {
+  if (ctx == nullptr || lp == nullptr || scaled == nullptr) return ANUK_ERR_INVALID;
+  *scaled = nullptr;
+  return guarded(ctx, [&] {
+    need(ruiz_iterations >= 0, "negative sweep count");
+    auto h = std::make_unique<anuk_lp>();
+    h->lp = lp->lp;
+    const anuk::Scaling s =
+        anuk::scale_lp_device(*ctx->be, &h->lp, ruiz_iterations, pock_chambolle != 0);
+    if (row_scale && !s.row_scale.empty())
+      std::memcpy(row_scale, s.row_scale.data(), s.row_scale.size() * sizeof(double));
+    if (col_scale && !s.col_scale.empty())
+      std::memcpy(col_scale, s.col_scale.data(), s.col_scale.size() * sizeof(double));
+    *scaled = h.release();
+  });
+}
+
 // ---- the solve --------------------------------------------------------------
 
 void anuk_pdhg_default_options(anuk_pdhg_options* options) {
@@ -454,6 +474,7 @@ anuk_pdhg_options checked_options(const anuk_pdhg_options* options) {
   need(o.termination_check_frequency > 0, "termination_check_frequency must be positive");
   need(o.max_iterations >= 0, "max_iterations must be non-negative");
   need(o.polish_first_iteration > 0, "polish_first_iteration must be positive");
+  need(o.product_order >= 0 && o.product_order <= 2, "product_order must be 0, 1 or 2");
   return o;
 }
 

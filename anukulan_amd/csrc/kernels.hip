@@ -79,7 +79,8 @@ __device__ double block_sum(double v) {
 // ---------------------------------------------------------------------------
 template <class Epi>
 __global__ void __launch_bounds__(kThreads)
-    spmv_kernel(DevCsr a, const double* __restrict__ x, Epi epi, double* __restrict__ partials) {
+    spmv_kernel(DevCsr a, const double* __restrict__ x, Epi epi, double* __restrict__ partials,
+                int fast) {
   __shared__ double tile[kTileNnz];
   constexpr int R = Epi::kR > 0 ? Epi::kR : 1;
   const int b = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
@@ -99,12 +100,36 @@ __global__ void __launch_bounds__(kThreads)
       if (k < len) tile[k] = a.value[k0 + k] * x[a.index[k0 + k]];
     }
     __syncthreads();
-    const Int r = r0 + static_cast<Int>(threadIdx.x);
-    if (r < r1) {
-      const Int lo = a.start[r] - k0, hi = a.start[r + 1] - k0;
-      double s = 0.0;
-      for (Int k = lo; k < hi; ++k) s += tile[k];
-      e(r, s, red);
+    // Rows in rounds of kThreads, one row per thread per round.
+    for (Int round = r0; round < r1; round += kThreads) {
+      const Int r = round + static_cast<Int>(threadIdx.x);
+      bool wide = false;
+      if (r < r1) {
+        const Int lo = a.start[r] - k0, hi = a.start[r + 1] - k0;
+        wide = fast && hi - lo > kSerialRowMax;
+        if (!wide) {
+          double s = 0.0;
+          for (Int k = lo; k < hi; ++k) s += tile[k];
+          e(r, s, red);
+        }
+      }
+      if (fast) {
+        // The wave's long rows, one at a time: lanes stride the row's staged
+        // products, then a butterfly adds them (a fixed tree).
+        unsigned long long mask = __ballot(wide);
+        const int lane = static_cast<int>(threadIdx.x & 63);
+        const Int base = round + static_cast<Int>(threadIdx.x & ~63u);
+        while (mask != 0ull) {
+          const int bit = __ffsll(static_cast<long long>(mask)) - 1;
+          mask &= mask - 1ull;
+          const Int rr = base + bit;
+          const Int lo = a.start[rr] - k0, hi = a.start[rr + 1] - k0;
+          double s = 0.0;
+          for (Int k = lo + lane; k < hi; k += 64) s += tile[k];
+          s = wave_fold(s, false);
+          if (lane == 0) e(rr, s, red);
+        }
+      }
     }
   } else {
     double s = 0.0;
@@ -543,6 +568,65 @@ __global__ void __launch_bounds__(kThreads)
     out[i] = v[i] / d;
 }
 
+__global__ void __launch_bounds__(kThreads)
+    row_norms_kernel(const Int* __restrict__ start, Int rows, const double* __restrict__ value,
+                     int one_norm, double* __restrict__ out) {
+  const Int r = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (r >= rows) return;
+  double acc = 0.0;
+  for (Int k = start[r]; k < start[r + 1]; ++k) {
+    const double v = fabs(value[k]);
+    acc = one_norm ? acc + v : fmax(acc, v);
+  }
+  out[r] = acc;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    inv_sqrt_kernel(const double* __restrict__ norm, Int n, double* __restrict__ d) {
+  const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (i >= n) return;
+  const double v = norm[i];
+  d[i] = v > 0.0 ? 1.0 / sqrt(v) : 1.0;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    scale_values_kernel(const Int* __restrict__ start, const Int* __restrict__ index, Int rows,
+                        double* __restrict__ value, const double* __restrict__ first,
+                        const double* __restrict__ second, int first_by_row) {
+  const Int r = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (r >= rows) return;
+  for (Int k = start[r]; k < start[r + 1]; ++k) {
+    double v = value[k];
+    if (first_by_row) {
+      v = v * first[r];
+      v = v * second[index[k]];
+    } else {
+      v = v * first[index[k]];
+      v = v * second[r];
+    }
+    value[k] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    scale_vectors_kernel(Int m, const double* __restrict__ d1, double* __restrict__ q,
+                         double* __restrict__ rs, Int n, const double* __restrict__ d2,
+                         double* __restrict__ c, double* __restrict__ lo, double* __restrict__ hi,
+                         double* __restrict__ cs) {
+  const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (i < m) {
+    q[i] *= d1[i];
+    rs[i] *= d1[i];
+  }
+  if (i < n) {
+    const double d = d2[i];
+    c[i] *= d;
+    lo[i] /= d;
+    hi[i] /= d;
+    cs[i] *= d;
+  }
+}
+
 __global__ void fill_kernel(double* p, Int n, double v) {
   const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
   if (i < n) p[i] = v;
@@ -634,16 +718,56 @@ void fold(hipStream_t s, std::initializer_list<FoldSeg> segs, double* out) {
 
 template <class Epi>
 void spmv_with(hipStream_t s, const DevCsr& a, const double* x, const Epi& e, double* partials,
-               const char* what, const LaunchEvents* ev = nullptr) {
+               const char* what, const LaunchEvents* ev = nullptr, bool fast = false) {
   if (a.blocks <= 0) return;
   const dim3 grid(static_cast<unsigned>(a.blocks));
-  if (ev != nullptr && ev->start != nullptr) {
+  const int f = fast ? 1 : 0;
+  if (ev != nullptr && (ev->start != nullptr || ev->stop != nullptr)) {
     hipExtLaunchKernelGGL(spmv_kernel<Epi>, grid, dim3(kThreads), 0, s, ev->start, ev->stop, 0, a,
-                          x, e, partials);
+                          x, e, partials, f);
   } else {
-    hipLaunchKernelGGL(spmv_kernel<Epi>, grid, dim3(kThreads), 0, s, a, x, e, partials);
+    hipLaunchKernelGGL(spmv_kernel<Epi>, grid, dim3(kThreads), 0, s, a, x, e, partials, f);
   }
   check_launch(what);
+}
+
+// Adds each row's kPanels partial sums in panel order and runs the epilogue.
+template <class Epi>
+__global__ void __launch_bounds__(kThreads)
+    panel_combine_kernel(Int rows, const double* __restrict__ sums, Epi epi,
+                         double* __restrict__ partials) {
+  constexpr int R = Epi::kR > 0 ? Epi::kR : 1;
+  const Epi e = epi.resolve(0);
+  if (e.skip()) return;
+  double red[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) red[i] = 0.0;
+  const Int r = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (r < rows) {
+    double v = sums[r];
+#pragma unroll
+    for (int p = 1; p < kPanels; ++p) v += sums[static_cast<size_t>(p) * rows + r];
+    e(r, v, red);
+  }
+  if constexpr (Epi::kR > 0) {
+    block_fold_store<Epi::kR>(red, Epi::kMaxMask, partials + static_cast<size_t>(blockIdx.x) * Epi::kR);
+  }
+}
+
+template <class Epi>
+void panel_step_with(hipStream_t s, const DevCsr& panels, double* sums, Int rows,
+                     const double* x, const Epi& e, double* partials, const LaunchEvents* ev) {
+  const LaunchEvents first{ev != nullptr ? ev->start : nullptr, nullptr};
+  spmv_with(s, panels, x, PlainEpi{sums}, nullptr, "panel products", &first, true);
+  if (rows <= 0) return;
+  const dim3 grid(static_cast<unsigned>((rows + kThreads - 1) / kThreads));
+  if (ev != nullptr && ev->stop != nullptr) {
+    hipExtLaunchKernelGGL(panel_combine_kernel<Epi>, grid, dim3(kThreads), 0, s, nullptr, ev->stop, 0,
+                          rows, sums, e, partials);
+  } else {
+    hipLaunchKernelGGL(panel_combine_kernel<Epi>, grid, dim3(kThreads), 0, s, rows, sums, e, partials);
+  }
+  check_launch("panel combine");
 }
 
 template <class Op>
@@ -782,37 +906,55 @@ void launch_convergence_terms(hipStream_t s, Int n, Int m, const DevProblem& p,
 
 void launch_primal_step(hipStream_t s, const DevCsr& kt, const double* y, const DevProblem& p,
                         const IterParams* params, int step, const PrimalBufs& b, unsigned flags,
-                        double* partials, const LaunchEvents* ev) {
+                        double* partials, const LaunchEvents* ev, bool fast) {
   if (flags & kStepDiag) {
     spmv_with(s, kt, y, PrimalStepEpi<true>{p, b, OneLp{params}, step, {}, false}, partials,
-              "primal_step", ev);
+              "primal_step", ev, fast);
   } else {
     spmv_with(s, kt, y, PrimalStepEpi<false>{p, b, OneLp{params}, step, {}, false}, partials,
-              "primal_step", ev);
+              "primal_step", ev, fast);
   }
 }
 
 void launch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar, const DevProblem& p,
                       const IterParams* params, int step, const DualBufs& b, unsigned flags,
-                      double* partials, const LaunchEvents* ev) {
+                      double* partials, const LaunchEvents* ev, bool fast) {
   if (flags & kStepDiag) {
     spmv_with(s, k, x_bar, DualStepEpi<true>{p, b, OneLp{params}, step, {}, false}, partials,
-              "dual_step", ev);
+              "dual_step", ev, fast);
   } else {
     spmv_with(s, k, x_bar, DualStepEpi<false>{p, b, OneLp{params}, step, {}, false}, partials,
-              "dual_step", ev);
+              "dual_step", ev, fast);
+  }
+}
+
+Int panel_combine_blocks(Int rows) { return (rows + kThreads - 1) / kThreads; }
+
+void launch_dual_step_panels(hipStream_t s, const DevCsr& panels, double* panel_sums,
+                             const double* x_bar, const DevProblem& p, const IterParams* params,
+                             int step, const DualBufs& b, unsigned flags, double* partials,
+                             const LaunchEvents* ev) {
+  const Int m = p.m;
+  if (panels.rows != kPanels * m) throw std::invalid_argument("dual step: panel stack does not fit K");
+  if (flags & kStepDiag) {
+    panel_step_with(s, panels, panel_sums, m, x_bar,
+                    DualStepEpi<true>{p, b, OneLp{params}, step, {}, false}, partials, ev);
+  } else {
+    panel_step_with(s, panels, panel_sums, m, x_bar,
+                    DualStepEpi<false>{p, b, OneLp{params}, step, {}, false}, partials, ev);
   }
 }
 
 void launch_conv_product(hipStream_t s, const DevCsr& kt, const double* y, const double* x,
-                         const DevProblem& p, double* partials) {
-  spmv_with(s, kt, y, ConvProductEpi{p, x, OneLp{nullptr}, false}, partials, "conv_product");
+                         const DevProblem& p, double* partials, bool fast) {
+  spmv_with(s, kt, y, ConvProductEpi{p, x, OneLp{nullptr}, false}, partials, "conv_product",
+            nullptr, fast);
 }
 
 void launch_farkas_product(hipStream_t s, const DevCsr& kt, const double* dy,
-                           const DevProblem& p, const double* check, double* partials) {
+                           const DevProblem& p, const double* check, double* partials, bool fast) {
   spmv_with(s, kt, dy, FarkasProductEpi{p, check, OneLp{nullptr}, false}, partials,
-            "farkas_product");
+            "farkas_product", nullptr, fast);
 }
 
 void launch_multi_primal_step(hipStream_t s, const DevCsr& kt, const Int* block_lp,
@@ -892,6 +1034,39 @@ void launch_multi_divide(hipStream_t s, const ChunkPlan& plan, const double* v,
   hipLaunchKernelGGL(multi_divide_kernel, dim3(static_cast<unsigned>(plan.chunks)), dim3(kThreads), 0,
                      s, plan.start, plan.lp, v, divisor, params, out);
   check_launch("multi divide");
+}
+
+void launch_row_norms(hipStream_t s, const Int* start, Int rows, const double* value, bool one_norm,
+                      double* out) {
+  if (rows <= 0) return;
+  hipLaunchKernelGGL(row_norms_kernel, dim3(grid_for(rows)), dim3(kThreads), 0, s, start, rows, value,
+                     one_norm ? 1 : 0, out);
+  check_launch("row norms");
+}
+
+void launch_inv_sqrt(hipStream_t s, const double* norm, Int n, double* d) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(inv_sqrt_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, norm, n, d);
+  check_launch("inverse square roots");
+}
+
+void launch_scale_values(hipStream_t s, const Int* start, const Int* index, Int rows,
+                         double* value, const double* first, const double* second,
+                         bool first_by_row) {
+  if (rows <= 0) return;
+  hipLaunchKernelGGL(scale_values_kernel, dim3(grid_for(rows)), dim3(kThreads), 0, s, start, index,
+                     rows, value, first, second, first_by_row ? 1 : 0);
+  check_launch("scale values");
+}
+
+void launch_scale_vectors(hipStream_t s, Int m, const double* d1, double* q, double* row_scale,
+                          Int n, const double* d2, double* c, double* lower, double* upper,
+                          double* col_scale) {
+  const Int len = m > n ? m : n;
+  if (len <= 0) return;
+  hipLaunchKernelGGL(scale_vectors_kernel, dim3(grid_for(len)), dim3(kThreads), 0, s, m, d1, q,
+                     row_scale, n, d2, c, lower, upper, col_scale);
+  check_launch("scale vectors");
 }
 
 void launch_check_fold_rows(hipStream_t s, Int primal_blocks, const double* pa, Int dual_blocks,

@@ -19,6 +19,7 @@ using Int = std::int32_t;
 
 inline constexpr int kThreads = 256;      // one workgroup = 4 wave64s
 inline constexpr int kTileNnz = 1024;     // products staged in LDS per row block
+inline constexpr int kTileRows = 256;     // most rows per row block
 inline constexpr int kMaxRed = 8;         // most reduction values one kernel emits
 
 // A CSR matrix on the device plus its row-block plan: block b covers rows
@@ -121,6 +122,17 @@ struct LaunchEvents {
   hipEvent_t stop = nullptr;
 };
 
+// Rows longer than this are summed by a whole wave in the fast product
+// order; at most this many are summed serially by one thread.
+inline constexpr Int kSerialRowMax = 32;
+// Column panels of a K whose gathered vector outgrows one XCD's L2: panel p
+// holds the columns of the p-th eighth of K's nonzeros, so the XCD that runs
+// it keeps its slice of x in L2. The panels are stacked as one CSR of
+// kPanels * rows rows; row p * rows + r holds row r's entries in panel p.
+inline constexpr int kPanels = 8;
+// A gathered vector larger than this (half an XCD's 4 MB L2) gets panels.
+inline constexpr double kPanelVectorBytes = 2.0 * 1024 * 1024;
+
 // ---- launchers (stream-ordered, asynchronous) --------------------------
 
 // Builds the row-block plan of a CSR on the host.
@@ -160,22 +172,34 @@ void launch_convergence_terms(hipStream_t s, Int n, Int m, const DevProblem& p,
 
 // The fused PDHG step. `flags` selects the diagnostic variant; `step` is the
 // position of this step in its period (for the Halpern weight).
+// `fast` selects the fast product order (see anuk_pdhg_options).
 void launch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                         const DevProblem& p, const IterParams* params, int step,
                         const PrimalBufs& b, unsigned flags, double* partials,
-                        const LaunchEvents* ev = nullptr);
+                        const LaunchEvents* ev = nullptr, bool fast = false);
 void launch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                       const DevProblem& p, const IterParams* params, int step,
                       const DualBufs& b, unsigned flags, double* partials,
-                      const LaunchEvents* ev = nullptr);
+                      const LaunchEvents* ev = nullptr, bool fast = false);
+// The dual step over K's column panels: the panels' partial row sums into
+// `panel_sums` (kPanels * m), then one thread per row adds them in panel
+// order and runs the dual epilogue. Its check partials come from
+// panel_combine_blocks(m) blocks.
+void launch_dual_step_panels(hipStream_t s, const DevCsr& panels, double* panel_sums,
+                             const double* x_bar, const DevProblem& p, const IterParams* params,
+                             int step, const DualBufs& b, unsigned flags, double* partials,
+                             const LaunchEvents* ev = nullptr);
+Int panel_combine_blocks(Int rows);
 // Check products: dual-side convergence terms from K^T y, and the Farkas
 // support from K^T dy. The Farkas product reads the row slots of `check`
 // (folded first by launch_check_fold_rows) and does nothing when the row half
 // of the test has already ruled a certificate out.
 void launch_conv_product(hipStream_t s, const DevCsr& kt, const double* y,
-                         const double* x, const DevProblem& p, double* partials);
+                         const double* x, const DevProblem& p, double* partials,
+                         bool fast = false);
 void launch_farkas_product(hipStream_t s, const DevCsr& kt, const double* dy,
-                           const DevProblem& p, const double* check, double* partials);
+                           const DevProblem& p, const double* check, double* partials,
+                           bool fast = false);
 // Folds the partials of a check into check[kCheckSlots]: first the two step
 // kernels' (slots kSumDx2..kFarkasRhs), then the two products' (kDualSq..).
 // pd may be null when infeasibility detection is off.
@@ -183,6 +207,25 @@ void launch_check_fold_rows(hipStream_t s, Int primal_blocks, const double* pa,
                             Int dual_blocks, const double* pb, double* check);
 void launch_check_fold_cols(hipStream_t s, Int conv_blocks, const double* pc,
                             Int farkas_blocks, const double* pd, double* check);
+
+// ---- scaling sweeps (scaling.cpp:24-61) ----------------------------------
+// out[r] = infinity norm (max |a|) or one norm (serial sum of |a| in storage
+// order) of row r, one thread per row.
+void launch_row_norms(hipStream_t s, const Int* start, Int rows, const double* value, bool one_norm,
+                      double* out);
+// d = 1 / sqrt(norm) where norm > 0, else 1.
+void launch_inv_sqrt(hipStream_t s, const double* norm, Int n, double* d);
+// Every entry a[r][k] becomes (a * first) * second, rounding after each
+// product; `first_by_row` says whether `first` is indexed by the row (and
+// `second` by the column) or the other way round.
+void launch_scale_values(hipStream_t s, const Int* start, const Int* index, Int rows,
+                         double* value, const double* first, const double* second,
+                         bool first_by_row);
+// q *= d1, row_scale *= d1 (m entries); c *= d2, lower /= d2, upper /= d2,
+// col_scale *= d2 (n entries).
+void launch_scale_vectors(hipStream_t s, Int m, const double* d1, double* q, double* row_scale,
+                          Int n, const double* d2, double* c, double* lower, double* upper,
+                          double* col_scale);
 
 // ---- block-diagonal stacks of LPs (each with its own K) -----------------
 //

@@ -32,7 +32,7 @@ struct Stack {
   Int lps = 0;
   Int n = 0, m = 0;
   std::vector<Int> col_off, row_off;  // lps + 1
-  Csr k, kt;                          // block diagonal, scaled
+  Csr k, kt;                          // block diagonal, as given (scaled on the device only)
   std::vector<Int> eq_end;            // global end row of each LP's equalities
   std::vector<double> row_scale, col_scale;
   std::vector<double> c, q, lower, upper;  // scaled
@@ -67,80 +67,61 @@ void parallel_for(size_t count, F f) {
     if (e) std::rethrow_exception(e);
 }
 
-// Scales every LP (scale_lp, scaling.cpp:32-100, on a copy) and concatenates
-// the scaled matrices block-diagonally and the vectors end to end.
-Stack build_stack(const std::vector<const StandardLp*>& lps, const anuk_pdhg_options& opt) {
+// Concatenates the LPs: K and K^T block-diagonally, the vectors end to end
+// (each LP's slice written by its own host thread). The matrices stay
+// unscaled here; the scaling runs on the device (scale_stack).
+Stack concat_stack(const std::vector<const StandardLp*>& lps) {
   Stack st;
   st.lps = static_cast<Int>(lps.size());
-  std::vector<StandardLp> scaled(lps.size());
   st.scaling.resize(lps.size());
-  const bool scales = opt.ruiz_iterations > 0 || opt.pock_chambolle != 0;
-  parallel_for(lps.size(), [&](size_t i) {
-    scaled[i] = *lps[i];
-    if (scales) {
-      st.scaling[i] = scale_lp(&scaled[i], opt.ruiz_iterations, opt.pock_chambolle != 0);
-    } else {
-      st.scaling[i].row_scale.assign(uz(scaled[i].rows()), 1.0);
-      st.scaling[i].col_scale.assign(uz(scaled[i].cols()), 1.0);
-    }
-  });
   st.col_off.assign(lps.size() + 1, 0);
   st.row_off.assign(lps.size() + 1, 0);
-  size_t nnz = 0;
+  std::vector<size_t> nz_off(lps.size() + 1, 0);
   for (size_t i = 0; i < lps.size(); ++i) {
     const long cols = static_cast<long>(st.col_off[i]) + lps[i]->cols();
     const long rows = static_cast<long>(st.row_off[i]) + lps[i]->rows();
     if (cols > (1L << 30) || rows > (1L << 30)) throw std::invalid_argument("multi: stack too large");
     st.col_off[i + 1] = static_cast<Int>(cols);
     st.row_off[i + 1] = static_cast<Int>(rows);
-    nnz += static_cast<size_t>(lps[i]->k.nnz());
+    nz_off[i + 1] = nz_off[i] + static_cast<size_t>(lps[i]->k.nnz());
   }
+  const size_t nnz = nz_off.back();
   if (nnz > static_cast<size_t>(1) << 30) throw std::invalid_argument("multi: stack too large");
   st.n = st.col_off.back();
   st.m = st.row_off.back();
-  auto concat = [&](Csr* out, bool transpose) {
+  for (Csr* out : {&st.k, &st.kt}) {
+    const bool transpose = out == &st.kt;
     out->rows = transpose ? st.n : st.m;
     out->cols = transpose ? st.m : st.n;
     out->start.assign(uz(out->rows) + 1, 0);
     out->index.resize(nnz);
     out->value.resize(nnz);
-    size_t at = 0;
-    for (size_t i = 0; i < lps.size(); ++i) {
-      const Csr& a = transpose ? scaled[i].kt : scaled[i].k;
+  }
+  for (std::vector<double>* v : {&st.oc, &st.olo, &st.ohi}) v->resize(uz(st.n));
+  st.oq.resize(uz(st.m));
+  parallel_for(lps.size(), [&](size_t i) {
+    for (int side = 0; side < 2; ++side) {
+      const bool transpose = side == 1;
+      const Csr& a = transpose ? lps[i]->kt : lps[i]->k;
+      Csr& out = transpose ? st.kt : st.k;
       const Int r0 = transpose ? st.col_off[i] : st.row_off[i];
       const Int c0 = transpose ? st.row_off[i] : st.col_off[i];
-      for (Int r = 0; r < a.rows; ++r) out->start[uz(r0 + r) + 1] = static_cast<Int>(at) + a.start[uz(r) + 1];
+      const size_t at = nz_off[i];
+      for (Int r = 0; r < a.rows; ++r)
+        out.start[uz(r0 + r) + 1] = static_cast<Int>(at) + a.start[uz(r) + 1];
       for (Int e = 0; e < a.nnz(); ++e) {
-        out->index[at + uz(e)] = a.index[uz(e)] + c0;
-        out->value[at + uz(e)] = a.value[uz(e)];
+        out.index[at + uz(e)] = a.index[uz(e)] + c0;
+        out.value[at + uz(e)] = a.value[uz(e)];
       }
-      at += uz(a.nnz());
     }
-    out->renew_id();
-  };
-  concat(&st.k, false);
-  concat(&st.kt, true);
-  auto cat = [&](std::vector<double> StandardLp::*f, bool from_scaled) {
-    std::vector<double> v;
-    for (size_t i = 0; i < lps.size(); ++i) {
-      const std::vector<double>& src = from_scaled ? scaled[i].*f : lps[i]->*f;
-      v.insert(v.end(), src.begin(), src.end());
-    }
-    return v;
-  };
-  st.c = cat(&StandardLp::c, true);
-  st.q = cat(&StandardLp::q, true);
-  st.lower = cat(&StandardLp::lower, true);
-  st.upper = cat(&StandardLp::upper, true);
-  st.oc = cat(&StandardLp::c, false);
-  st.oq = cat(&StandardLp::q, false);
-  st.olo = cat(&StandardLp::lower, false);
-  st.ohi = cat(&StandardLp::upper, false);
-  for (size_t i = 0; i < lps.size(); ++i) {
-    st.eq_end.push_back(st.row_off[i] + lps[i]->num_equalities);
-    st.row_scale.insert(st.row_scale.end(), st.scaling[i].row_scale.begin(), st.scaling[i].row_scale.end());
-    st.col_scale.insert(st.col_scale.end(), st.scaling[i].col_scale.begin(), st.scaling[i].col_scale.end());
-  }
+    std::copy(lps[i]->c.begin(), lps[i]->c.end(), st.oc.begin() + st.col_off[i]);
+    std::copy(lps[i]->lower.begin(), lps[i]->lower.end(), st.olo.begin() + st.col_off[i]);
+    std::copy(lps[i]->upper.begin(), lps[i]->upper.end(), st.ohi.begin() + st.col_off[i]);
+    std::copy(lps[i]->q.begin(), lps[i]->q.end(), st.oq.begin() + st.row_off[i]);
+  });
+  st.k.renew_id();
+  st.kt.renew_id();
+  for (size_t i = 0; i < lps.size(); ++i) st.eq_end.push_back(st.row_off[i] + lps[i]->num_equalities);
   return st;
 }
 
@@ -149,6 +130,8 @@ Stack build_stack(const std::vector<const StandardLp*>& lps, const anuk_pdhg_opt
 struct DeviceStack {
   HipBackend& be;
   DevCsr k, kt;
+  double* k_value = nullptr;   // k.value and kt.value, writable for the scaling
+  double* kt_value = nullptr;
   const Int* k_block_lp = nullptr;
   const Int* kt_block_lp = nullptr;
   const Int* k_lp_blocks = nullptr;
@@ -194,7 +177,7 @@ Plan plan_stack(const Csr& a, const std::vector<Int>& off) {
     if (rows > 0) {
       std::int32_t* br = nullptr;
       Int blocks = 0;
-      plan_blocks(rows, a.start.data() + off[l], kTileNnz, kThreads, &br, &blocks);
+      plan_blocks(rows, a.start.data() + off[l], kTileNnz, kTileRows, &br, &blocks);
       for (Int b = 1; b <= blocks; ++b) {
         p.block_row.push_back(off[l] + br[b]);
         p.block_lp.push_back(static_cast<Int>(l));
@@ -241,6 +224,8 @@ void upload_stack(const Stack& st, DeviceStack* ds) {
   const Plan pkt = plan_stack(st.kt, st.col_off);
   ds->k = dev_csr(st.k, pk);
   ds->kt = dev_csr(st.kt, pkt);
+  ds->k_value = const_cast<double*>(ds->k.value);
+  ds->kt_value = const_cast<double*>(ds->kt.value);
   ds->k_block_lp = ds->put(pk.block_lp);
   ds->kt_block_lp = ds->put(pkt.block_lp);
   ds->k_lp_blocks = ds->put(pk.lp_blocks);
@@ -253,6 +238,69 @@ void upload_stack(const Stack& st, DeviceStack* ds) {
   ds->col_lp_chunks = ds->put(cc.lp_chunks);
   ds->row_lp_chunks = ds->put(rc.lp_chunks);
   ds->be.synchronize();
+}
+
+// scale_lp (scaling.cpp:32-100) of every LP at once, in place on the device
+// stack. A row or column of the block-diagonal stack belongs to one LP, so
+// the stack's sweeps are each LP's sweeps, with the same operations in the
+// same order: every LP gets exactly its own scale_lp's bits. The scaled
+// vectors and the diagonals come back to the host.
+void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, const DeviceStack& ds) {
+  const Int m = st->m, n = st->n;
+  st->c = st->oc;
+  st->q = st->oq;
+  st->lower = st->olo;
+  st->upper = st->ohi;
+  st->row_scale.assign(uz(m), 1.0);
+  st->col_scale.assign(uz(n), 1.0);
+  const bool scales = opt.ruiz_iterations > 0 || opt.pock_chambolle != 0;
+  if (scales) {
+    hipStream_t s = be.stream();
+    int slot = kMultiScratchSlot + 80;
+    auto put = [&](const std::vector<double>& h) {
+      double* d = be.scratch(slot++, std::max<size_t>(h.size(), 1));
+      be.upload_staged(h.data(), d, h.size());
+      return d;
+    };
+    double* q = put(st->q);
+    double* c = put(st->c);
+    double* lo = put(st->lower);
+    double* hi = put(st->upper);
+    double* rs = put(st->row_scale);
+    double* cs = put(st->col_scale);
+    double* rn = be.scratch(slot++, uz(m) + 1);
+    double* cn = be.scratch(slot++, uz(n) + 1);
+    double* d1 = be.scratch(slot++, uz(m) + 1);
+    double* d2 = be.scratch(slot++, uz(n) + 1);
+    auto sweep = [&](bool one_norm) {
+      launch_row_norms(s, ds.k.start, m, ds.k_value, one_norm, rn);
+      launch_row_norms(s, ds.kt.start, n, ds.kt_value, one_norm, cn);
+      launch_inv_sqrt(s, rn, m, d1);
+      launch_inv_sqrt(s, cn, n, d2);
+      launch_scale_values(s, ds.k.start, ds.k.index, m, ds.k_value, d1, d2, true);
+      launch_scale_values(s, ds.kt.start, ds.kt.index, n, ds.kt_value, d1, d2, false);
+      launch_scale_vectors(s, m, d1, q, rs, n, d2, c, lo, hi, cs);
+    };
+    for (int i = 0; i < opt.ruiz_iterations; ++i) sweep(false);
+    if (opt.pock_chambolle) sweep(true);
+    auto get = [&](const double* d, std::vector<double>* h) {
+      if (!h->empty())
+        hip_check(hipMemcpyAsync(h->data(), d, h->size() * sizeof(double), hipMemcpyDeviceToHost, s),
+                  "stack scaling");
+    };
+    get(q, &st->q);
+    get(c, &st->c);
+    get(lo, &st->lower);
+    get(hi, &st->upper);
+    get(rs, &st->row_scale);
+    get(cs, &st->col_scale);
+    be.synchronize();
+  }
+  for (Int l = 0; l < st->lps; ++l) {
+    Scaling& sc = st->scaling[uz(l)];
+    sc.row_scale.assign(st->row_scale.begin() + st->row_off[uz(l)], st->row_scale.begin() + st->row_off[uz(l) + 1]);
+    sc.col_scale.assign(st->col_scale.begin() + st->col_off[uz(l)], st->col_scale.begin() + st->col_off[uz(l) + 1]);
+  }
 }
 
 // Device-resident per-LP flags as IterParams (only `active` is read).
@@ -950,10 +998,11 @@ std::vector<SolveResult> solve_pdhg_multi(HipBackend& be, const std::vector<cons
     phases.*field += std::chrono::duration<double>(now - mark).count();
     mark = now;
   };
-  const Stack st = build_stack(lps, opt);
-  lap(&Phases::scaling);
+  Stack st = concat_stack(lps);
   DeviceStack ds(be);
   upload_stack(st, &ds);
+  scale_stack(be, opt, &st, ds);
+  lap(&Phases::scaling);
   std::vector<double> norms(uz(st.lps), opt.known_matrix_norm);
   if (!(opt.known_matrix_norm > 0.0)) norms = stack_norms(be, st, ds, 200, 1e-8);
   lap(&Phases::matrix_norm);

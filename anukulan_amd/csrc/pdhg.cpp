@@ -1,5 +1,7 @@
 #include "pdhg.hpp"
 
+#include "scaling.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -141,6 +143,13 @@ class Solver {
   IterParams* d_params_ = nullptr;
   double* h_pinned_ = nullptr;  // kCheckSlots doubles
   DevCsr dk_, dkt_;
+  // Fast product order: long rows summed by waves, and K x_bar over K's
+  // column panels when x_bar outgrows an XCD's L2 (the panel sums live in
+  // panel_sums_, kPanels * m doubles).
+  bool fast_ = false;
+  bool panels_ = false;
+  DevCsr dpk_;
+  double* panel_sums_ = nullptr;
   DevProblem prob_;
   std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
 
@@ -211,7 +220,9 @@ Int Solver::polish_budget(Int at) const {
 
 void Solver::allocate_device() {
   const size_t N = static_cast<size_t>(n_), M = static_cast<size_t>(m_);
-  const size_t kb = static_cast<size_t>(std::max(dk_.blocks, dkt_.blocks)) * kMaxRed;
+  const size_t kb =
+      static_cast<size_t>(std::max({dk_.blocks, dkt_.blocks, panel_combine_blocks(m_)})) * kMaxRed;
+  const size_t P = panels_ ? static_cast<size_t>(kPanels) * M : 0;
   // Every vector starts 256-byte aligned.
   auto round = [](size_t d) { return (d + 31) / 32 * 32; };
   const size_t layout[] = {
@@ -221,7 +232,8 @@ void Solver::allocate_device() {
       M, M, M, M, M, M, M, M, M,   // yb0 yb1 kx0 kx1 dy k_dx k_x_bar anchor_y anchor_kx
       M, M, M, M, M,               // q orig_q row_scale sum_y avg_y
       kb, kb, kb, kb,              // partials
-      32, 32                       // check, params
+      32, 32,                      // check, params
+      P                            // panel sums
   };
   size_t total = 0;
   for (size_t d : layout) total += round(std::max<size_t>(d, 1));
@@ -245,6 +257,7 @@ void Solver::allocate_device() {
   part_a_ = take(); part_b_ = take(); part_c_ = take(); part_d_ = take();
   check_ = take();
   d_params_ = reinterpret_cast<IterParams*>(take());
+  panel_sums_ = take();
   prob_.n = n_;
   prob_.m = m_;
   prob_.num_equalities = lp_.num_equalities;
@@ -291,7 +304,9 @@ SolveResult Solver::run() {
   lp_ = orig_;
   const bool scales = opt_.ruiz_iterations > 0 || opt_.pock_chambolle != 0;
   if (scales) {
-    scaling_ = scale_lp(&lp_, opt_.ruiz_iterations, opt_.pock_chambolle != 0);
+    scaling_ = lp_.k.nnz() >= kDeviceScalingNnz
+                   ? scale_lp_device(be_, &lp_, opt_.ruiz_iterations, opt_.pock_chambolle != 0)
+                   : scale_lp(&lp_, opt_.ruiz_iterations, opt_.pock_chambolle != 0);
   } else {
     // No sweep changes nothing: keep K, K^T and their identities, so a
     // polishing sub-solve reuses the enclosing solve's device copies.
@@ -302,6 +317,10 @@ SolveResult Solver::run() {
 
   dk_ = be_.prepare(view_of(lp_.k));
   dkt_ = be_.prepare(view_of(lp_.kt));
+  // The fast product order by default only where it pays: 4M nonzeros and up.
+  fast_ = opt_.product_order == 2 || (opt_.product_order == 0 && lp_.k.nnz() >= (Int{1} << 22));
+  panels_ = fast_ && static_cast<double>(n_) * sizeof(double) > kPanelVectorBytes && m_ > 0;
+  if (panels_) dpk_ = be_.prepare_panels(view_of(lp_.k));
 
   matrix_norm_ = opt_.known_matrix_norm > 0.0 ? opt_.known_matrix_norm
                                              : estimate_matrix_norm(be_, lp_, 200, 1e-8);
@@ -418,13 +437,18 @@ void Solver::enqueue_steps(int steps, bool has_check, int parity) {
     const LaunchEvents pe = be_.launch_events(diag ? "primal_step_check" : "primal_step",
                                               12.0 * nnz + 4.0 * (n + 1) + 8.0 * m +
                                                   (diag ? 64.0 : 56.0) * n);
-    launch_primal_step(s, dkt_, yb_[p], prob_, d_params_, step, pb, flags, part_a_, &pe);
+    launch_primal_step(s, dkt_, yb_[p], prob_, d_params_, step, pb, flags, part_a_, &pe, fast_);
     DualBufs db{yb_[p], kxb_[p], anchor_y_, anchor_kx_, yb_[1 - p], kxb_[1 - p],
                 dy_, k_dx_, anchor_y_, anchor_kx_};
     const LaunchEvents de = be_.launch_events(diag ? "dual_step_check" : "dual_step",
                                               12.0 * nnz + 4.0 * (m + 1) + 8.0 * n +
                                                   (diag ? 72.0 : 56.0) * m);
-    launch_dual_step(s, dk_, x_bar_, prob_, d_params_, step, db, flags, part_b_, &de);
+    if (panels_) {
+      launch_dual_step_panels(s, dpk_, panel_sums_, x_bar_, prob_, d_params_, step, db, flags,
+                              part_b_, &de);
+    } else {
+      launch_dual_step(s, dk_, x_bar_, prob_, d_params_, step, db, flags, part_b_, &de, fast_);
+    }
     p = 1 - p;
   }
   if (has_check) enqueue_check(p);
@@ -433,10 +457,11 @@ void Solver::enqueue_steps(int steps, bool has_check, int parity) {
 void Solver::enqueue_check(int p) {
   hipStream_t s = be_.stream();
   HipBackend::Timed t(be_, "check_reductions");
-  launch_check_fold_rows(s, dkt_.blocks, part_a_, dk_.blocks, part_b_, check_);
-  launch_conv_product(s, dkt_, yb_[p], xb_[p], prob_, part_c_);
+  launch_check_fold_rows(s, dkt_.blocks, part_a_, panels_ ? panel_combine_blocks(m_) : dk_.blocks,
+                         part_b_, check_);
+  launch_conv_product(s, dkt_, yb_[p], xb_[p], prob_, part_c_, fast_);
   const bool farkas = opt_.detect_infeasibility != 0;
-  if (farkas) launch_farkas_product(s, dkt_, dy_, prob_, check_, part_d_);
+  if (farkas) launch_farkas_product(s, dkt_, dy_, prob_, check_, part_d_, fast_);
   launch_check_fold_cols(s, dkt_.blocks, part_c_, dkt_.blocks, farkas ? part_d_ : nullptr, check_);
 }
 

@@ -245,6 +245,13 @@ int anuk_lp_export(const anuk_lp* lp, int32_t* start, int32_t* index, double* va
 int anuk_lp_scale(const anuk_lp* lp, int32_t ruiz_iterations, int32_t pock_chambolle,
                   anuk_lp** scaled, double* row_scale, double* col_scale);
 
+/* The same scaling computed on the device of ctx: the sweeps run there with
+ * the host's operations in the host's order, so the result is bit for bit
+ * anuk_lp_scale's. The solvers use it for matrices of 1M nonzeros and up. */
+int anuk_lp_scale_device(anuk_ctx* ctx, const anuk_lp* lp, int32_t ruiz_iterations,
+                         int32_t pock_chambolle, anuk_lp** scaled, double* row_scale,
+                         double* col_scale);
+
 /* ------------------------------------------------------------------------ */
 /* Whole solve (pdhg.hpp:19-543)                                             */
 /* ------------------------------------------------------------------------ */
@@ -300,6 +307,12 @@ typedef struct anuk_pdhg_options {
   int32_t log_frequency;               /* 5000 */
   const volatile int32_t* stop;        /* NULL; nonzero stops at the next check */
   int32_t use_graphs;                  /* 1: replay each check period as a HIP graph */
+  int32_t product_order;               /* 0: automatic (storage order below 4M nonzeros,
+                                          fastest above), 1: every row sum serially in
+                                          CSR storage order (the CPU's bits), 2: fastest
+                                          (long rows summed by a wave in a fixed tree,
+                                          large K x split into per-XCD column panels);
+                                          deterministic either way */
 } anuk_pdhg_options;
 
 void anuk_pdhg_default_options(anuk_pdhg_options* options);

@@ -9,7 +9,10 @@ to 1024 nonzeros, which the LDS path sums in storage order; rounding-level on
 the longer rows, which a workgroup folds in a fixed tree), and the solve is
 checked against the reference's fixture for that size (solved on the
 threaded CPU backend, which has the serial backend's bits) and by its own
-residual.
+residual, in both product orders: the storage order (every row sum serially,
+the CPU's bits) and the fast order the solver picks at this size (long rows
+summed by a wave, K x split into per-XCD column panels), which moves the
+iterates at rounding level only.
 """
 import json
 import os
@@ -97,3 +100,27 @@ def test_full_size_lp_solves_to_tolerance(ctx, big):
     if fx:
         assert relerr(r.objective, fx["objective"]) <= 1e-8
         assert abs(r.iterations - fx["iterations"]) <= 0.02 * fx["iterations"]
+
+
+def test_full_size_lp_in_storage_order(ctx, big):
+    lp, _, _ = big
+    fx = FIXTURE["400000x1000000"]["1e-4"]
+    r = solve_pdhg(ctx, lp, tolerance=1e-4, product_order=1)
+    print(f"\nstorage order: {r.iterations} iterations (reference {fx['iterations']}), "
+          f"objective {r.objective!r}")
+    assert r.status == "optimal"
+    assert relerr(r.objective, fx["objective"]) <= 1e-8
+    assert abs(r.iterations - fx["iterations"]) <= 0.02 * fx["iterations"]
+
+
+@pytest.mark.parametrize("m", [150_000])
+def test_fast_order_with_panels_matches_storage_order(ctx, m):
+    # n = 2.5 m = 375k columns: x is 3 MB, past the panel threshold.
+    lp = G.random_lp(m=m, n=int(2.5 * m), seed=31).to_lp()
+    exact = solve_pdhg(ctx, lp, tolerance=1e-4, product_order=1)
+    fast = solve_pdhg(ctx, lp, tolerance=1e-4, product_order=2)
+    print(f"\nstorage {exact.iterations} / fast {fast.iterations} iterations; objectives "
+          f"{exact.objective!r} / {fast.objective!r}")
+    assert exact.status == fast.status == "optimal"
+    assert relerr(fast.objective, exact.objective) <= 1e-8
+    assert abs(fast.iterations - exact.iterations) <= 0.02 * exact.iterations

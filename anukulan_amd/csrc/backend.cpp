@@ -256,10 +256,42 @@ const DevCsr& HipBackend::prepare(const CsrView& a) {
 }
 
 namespace {
-// Cache key of a matrix's panel stack: the matrix's own identity with the top
-// bit flipped (identities are drawn from a counter and never reach it).
+// Cache keys of a matrix's fast plan and of its panel stack: the matrix's own
+// identity with a top bit flipped (identities are drawn from a counter and
+// never reach them).
 constexpr std::uint64_t kPanelKey = 1ull << 63;
+constexpr std::uint64_t kFastKey = 1ull << 62;
 }  // namespace
+
+void HipBackend::replan_fast(const CsrView& a, DeviceMatrix* dm) {
+  Int* block_rows = nullptr;
+  Int blocks = 0;
+  plan_blocks(a.rows, a.start, kFastTileNnz, kFastTileRows, &block_rows, &blocks);
+  void* p = nullptr;
+  const size_t bytes = (static_cast<size_t>(blocks) + 1) * sizeof(Int);
+  const hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    std::free(block_rows);
+    hip_check(e, "hipMalloc(fast plan)");
+  }
+  dm->owned.push_back(p);
+  hip_check(hipMemcpyAsync(p, block_rows, bytes, hipMemcpyHostToDevice, stream_), "upload fast plan");
+  synchronize();
+  std::free(block_rows);
+  dm->view.blocks = blocks;
+  dm->view.block_row = static_cast<const Int*>(p);
+}
+
+const DevCsr& HipBackend::prepare_fast(const CsrView& a) {
+  const std::uint64_t key = a.id ^ kFastKey;
+  auto it = matrices_.find(key);
+  if (it != matrices_.end()) return it->second.view;
+  DeviceMatrix dm;
+  dm.view = prepare(a);  // the arrays stay owned by the matrix's own entry
+  replan_fast(a, &dm);
+  auto ins = matrices_.emplace(key, std::move(dm));
+  return ins.first->second.view;
+}
 
 const DevCsr& HipBackend::prepare_panels(const CsrView& a) {
   const std::uint64_t key = a.id ^ kPanelKey;
@@ -304,7 +336,11 @@ const DevCsr& HipBackend::prepare_panels(const CsrView& a) {
     }
   }
   const CsrView stacked{key, kPanels * rows, a.cols, start.data(), index.data(), value.data()};
-  return prepare(stacked);
+  prepare(stacked);
+  // The panel stack only ever runs in the fast order: give it that plan.
+  DeviceMatrix& dm = matrices_.at(key);
+  replan_fast(stacked, &dm);
+  return dm.view;
 }
 
 void HipBackend::release() {
@@ -314,7 +350,8 @@ void HipBackend::release() {
 }
 
 void HipBackend::release_one(std::uint64_t id) {
-  for (const std::uint64_t key : {id, id ^ kPanelKey}) {
+  // The fast plan borrows the matrix's arrays; it goes first.
+  for (const std::uint64_t key : {id ^ kFastKey, id ^ kPanelKey, id}) {
     auto it = matrices_.find(key);
     if (it == matrices_.end()) continue;
     synchronize();

@@ -60,8 +60,12 @@ class HipBackend {
 
   // matrices (backend.hpp:35-36, 61-64)
   const DevCsr& prepare(const CsrView& a);
+  // `a` with the fast order's row-block plan (large tiles), sharing the
+  // arrays of prepare(a); cached with `a`.
+  const DevCsr& prepare_fast(const CsrView& a);
   // The column-panel stack of `a` (kPanels panels balanced by nonzeros, see
-  // kernels.hpp), built on the host and cached with `a`.
+  // kernels.hpp), built on the host, planned for the fast order and cached
+  // with `a`.
   const DevCsr& prepare_panels(const CsrView& a);
   void release();
   void release_one(std::uint64_t id);
@@ -134,6 +138,8 @@ class HipBackend {
     std::vector<void*> owned;
   };
   void free_matrix(DeviceMatrix* dm);
+  // Plans `a` for the fast order and stores the plan in `dm`.
+  void replan_fast(const CsrView& a, DeviceMatrix* dm);
 
   int device_ = 0;
   hipStream_t stream_ = nullptr;

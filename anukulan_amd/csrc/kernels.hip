@@ -77,11 +77,11 @@ __device__ double block_sum(double v) {
 // row and may add to up to kR reduction values, folded per block into
 // partials[block * kR ...].
 // ---------------------------------------------------------------------------
-template <class Epi>
+template <class Epi, int kTile>
 __global__ void __launch_bounds__(kThreads)
     spmv_kernel(DevCsr a, const double* __restrict__ x, Epi epi, double* __restrict__ partials,
                 int fast) {
-  __shared__ double tile[kTileNnz];
+  __shared__ double tile[kTile];
   constexpr int R = Epi::kR > 0 ? Epi::kR : 1;
   const int b = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
   const Epi e = epi.resolve(b);
@@ -93,9 +93,9 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
   for (int r = 0; r < R; ++r) red[r] = 0.0;
 
-  if (len <= kTileNnz) {
+  if (len <= kTile) {
 #pragma unroll
-    for (int u = 0; u < kTileNnz / kThreads; ++u) {
+    for (int u = 0; u < kTile / kThreads; ++u) {
       const Int k = static_cast<Int>(threadIdx.x) + u * kThreads;
       if (k < len) tile[k] = a.value[k0 + k] * x[a.index[k0 + k]];
     }
@@ -103,19 +103,18 @@ __global__ void __launch_bounds__(kThreads)
     // Rows in rounds of kThreads, one row per thread per round.
     for (Int round = r0; round < r1; round += kThreads) {
       const Int r = round + static_cast<Int>(threadIdx.x);
+      double sum = 0.0;
       bool wide = false;
       if (r < r1) {
         const Int lo = a.start[r] - k0, hi = a.start[r + 1] - k0;
         wide = fast && hi - lo > kSerialRowMax;
-        if (!wide) {
-          double s = 0.0;
-          for (Int k = lo; k < hi; ++k) s += tile[k];
-          e(r, s, red);
-        }
+        if (!wide)
+          for (Int k = lo; k < hi; ++k) sum += tile[k];
       }
       if (fast) {
         // The wave's long rows, one at a time: lanes stride the row's staged
-        // products, then a butterfly adds them (a fixed tree).
+        // products, a butterfly adds them (a fixed tree), and the row's own
+        // thread keeps the total for its epilogue.
         unsigned long long mask = __ballot(wide);
         const int lane = static_cast<int>(threadIdx.x & 63);
         const Int base = round + static_cast<Int>(threadIdx.x & ~63u);
@@ -127,9 +126,10 @@ __global__ void __launch_bounds__(kThreads)
           double s = 0.0;
           for (Int k = lo + lane; k < hi; k += 64) s += tile[k];
           s = wave_fold(s, false);
-          if (lane == 0) e(rr, s, red);
+          if (lane == bit) sum = s;
         }
       }
+      if (r < r1) e(r, sum, red);
     }
   } else {
     double s = 0.0;
@@ -721,12 +721,20 @@ void spmv_with(hipStream_t s, const DevCsr& a, const double* x, const Epi& e, do
                const char* what, const LaunchEvents* ev = nullptr, bool fast = false) {
   if (a.blocks <= 0) return;
   const dim3 grid(static_cast<unsigned>(a.blocks));
-  const int f = fast ? 1 : 0;
-  if (ev != nullptr && (ev->start != nullptr || ev->stop != nullptr)) {
-    hipExtLaunchKernelGGL(spmv_kernel<Epi>, grid, dim3(kThreads), 0, s, ev->start, ev->stop, 0, a,
-                          x, e, partials, f);
+  // The fast order runs on plans with the large tile (prepare_fast), the
+  // storage order on plans with the small one.
+  auto launch = [&](auto kernel) {
+    if (ev != nullptr && (ev->start != nullptr || ev->stop != nullptr)) {
+      hipExtLaunchKernelGGL(kernel, grid, dim3(kThreads), 0, s, ev->start, ev->stop, 0, a, x, e,
+                            partials, fast ? 1 : 0);
+    } else {
+      hipLaunchKernelGGL(kernel, grid, dim3(kThreads), 0, s, a, x, e, partials, fast ? 1 : 0);
+    }
+  };
+  if (fast) {
+    launch(spmv_kernel<Epi, kFastTileNnz>);
   } else {
-    hipLaunchKernelGGL(spmv_kernel<Epi>, grid, dim3(kThreads), 0, s, a, x, e, partials, f);
+    launch(spmv_kernel<Epi, kTileNnz>);
   }
   check_launch(what);
 }
@@ -757,9 +765,9 @@ __global__ void __launch_bounds__(kThreads)
 template <class Epi>
 void panel_step_with(hipStream_t s, const DevCsr& panels, double* sums, Int rows,
                      const double* x, const Epi& e, double* partials, const LaunchEvents* ev) {
+  if (rows <= 0) return;
   const LaunchEvents first{ev != nullptr ? ev->start : nullptr, nullptr};
   spmv_with(s, panels, x, PlainEpi{sums}, nullptr, "panel products", &first, true);
-  if (rows <= 0) return;
   const dim3 grid(static_cast<unsigned>((rows + kThreads - 1) / kThreads));
   if (ev != nullptr && ev->stop != nullptr) {
     hipExtLaunchKernelGGL(panel_combine_kernel<Epi>, grid, dim3(kThreads), 0, s, nullptr, ev->stop, 0,

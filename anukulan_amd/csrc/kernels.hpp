@@ -20,6 +20,11 @@ using Int = std::int32_t;
 inline constexpr int kThreads = 256;      // one workgroup = 4 wave64s
 inline constexpr int kTileNnz = 1024;     // products staged in LDS per row block
 inline constexpr int kTileRows = 256;     // most rows per row block
+// Plans for the fast product order: larger tiles, rows taken in rounds of
+// kThreads, so each block issues its whole tile's loads in two memory trips
+// and a few thousand blocks cover a large matrix.
+inline constexpr int kFastTileNnz = 4096;
+inline constexpr int kFastTileRows = 2048;
 inline constexpr int kMaxRed = 8;         // most reduction values one kernel emits
 
 // A CSR matrix on the device plus its row-block plan: block b covers rows

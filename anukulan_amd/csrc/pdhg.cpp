@@ -148,7 +148,7 @@ class Solver {
   // panel_sums_, kPanels * m doubles).
   bool fast_ = false;
   bool panels_ = false;
-  DevCsr dpk_;
+  DevCsr dpk_, dk_fast_, dkt_fast_;
   double* panel_sums_ = nullptr;
   DevProblem prob_;
   std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
@@ -321,6 +321,8 @@ SolveResult Solver::run() {
   fast_ = opt_.product_order == 2 || (opt_.product_order == 0 && lp_.k.nnz() >= (Int{1} << 22));
   panels_ = fast_ && static_cast<double>(n_) * sizeof(double) > kPanelVectorBytes && m_ > 0;
   if (panels_) dpk_ = be_.prepare_panels(view_of(lp_.k));
+  if (fast_) dkt_fast_ = be_.prepare_fast(view_of(lp_.kt));
+  if (fast_ && !panels_) dk_fast_ = be_.prepare_fast(view_of(lp_.k));
 
   matrix_norm_ = opt_.known_matrix_norm > 0.0 ? opt_.known_matrix_norm
                                              : estimate_matrix_norm(be_, lp_, 200, 1e-8);
@@ -437,7 +439,8 @@ void Solver::enqueue_steps(int steps, bool has_check, int parity) {
     const LaunchEvents pe = be_.launch_events(diag ? "primal_step_check" : "primal_step",
                                               12.0 * nnz + 4.0 * (n + 1) + 8.0 * m +
                                                   (diag ? 64.0 : 56.0) * n);
-    launch_primal_step(s, dkt_, yb_[p], prob_, d_params_, step, pb, flags, part_a_, &pe, fast_);
+    launch_primal_step(s, fast_ ? dkt_fast_ : dkt_, yb_[p], prob_, d_params_, step, pb, flags,
+                       part_a_, &pe, fast_);
     DualBufs db{yb_[p], kxb_[p], anchor_y_, anchor_kx_, yb_[1 - p], kxb_[1 - p],
                 dy_, k_dx_, anchor_y_, anchor_kx_};
     const LaunchEvents de = be_.launch_events(diag ? "dual_step_check" : "dual_step",
@@ -447,7 +450,8 @@ void Solver::enqueue_steps(int steps, bool has_check, int parity) {
       launch_dual_step_panels(s, dpk_, panel_sums_, x_bar_, prob_, d_params_, step, db, flags,
                               part_b_, &de);
     } else {
-      launch_dual_step(s, dk_, x_bar_, prob_, d_params_, step, db, flags, part_b_, &de, fast_);
+      launch_dual_step(s, fast_ ? dk_fast_ : dk_, x_bar_, prob_, d_params_, step, db, flags, part_b_,
+                       &de, fast_);
     }
     p = 1 - p;
   }
@@ -457,12 +461,13 @@ void Solver::enqueue_steps(int steps, bool has_check, int parity) {
 void Solver::enqueue_check(int p) {
   hipStream_t s = be_.stream();
   HipBackend::Timed t(be_, "check_reductions");
-  launch_check_fold_rows(s, dkt_.blocks, part_a_, panels_ ? panel_combine_blocks(m_) : dk_.blocks,
-                         part_b_, check_);
-  launch_conv_product(s, dkt_, yb_[p], xb_[p], prob_, part_c_, fast_);
+  const DevCsr& kt = fast_ ? dkt_fast_ : dkt_;
+  const Int dual_blocks = panels_ ? panel_combine_blocks(m_) : (fast_ ? dk_fast_ : dk_).blocks;
+  launch_check_fold_rows(s, kt.blocks, part_a_, dual_blocks, part_b_, check_);
+  launch_conv_product(s, kt, yb_[p], xb_[p], prob_, part_c_, fast_);
   const bool farkas = opt_.detect_infeasibility != 0;
-  if (farkas) launch_farkas_product(s, dkt_, dy_, prob_, check_, part_d_, fast_);
-  launch_check_fold_cols(s, dkt_.blocks, part_c_, dkt_.blocks, farkas ? part_d_ : nullptr, check_);
+  if (farkas) launch_farkas_product(s, kt, dy_, prob_, check_, part_d_, fast_);
+  launch_check_fold_cols(s, kt.blocks, part_c_, kt.blocks, farkas ? part_d_ : nullptr, check_);
 }
 
 hipGraphExec_t Solver::graph_for(int parity) {

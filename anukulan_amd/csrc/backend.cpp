@@ -280,6 +280,7 @@ void HipBackend::replan_fast(const CsrView& a, DeviceMatrix* dm) {
   std::free(block_rows);
   dm->view.blocks = blocks;
   dm->view.block_row = static_cast<const Int*>(p);
+  dm->view.max_rows = kFastTileRows;
 }
 
 const DevCsr& HipBackend::prepare_fast(const CsrView& a) {

@@ -94,6 +94,20 @@ __global__ void __launch_bounds__(kThreads)
   for (int r = 0; r < R; ++r) red[r] = 0.0;
 
   if (len <= kTile) {
+    // Every round's row bounds are loaded up front, with the staging loads,
+    // so a block waits for memory twice however many rounds it has.
+    constexpr int kRounds = (kTile == kFastTileNnz ? kFastTileRows : kTileRows) / kThreads;
+    Int lo_r[kRounds], hi_r[kRounds];
+#pragma unroll
+    for (int u = 0; u < kRounds; ++u) {
+      const Int r = r0 + static_cast<Int>(threadIdx.x) + u * kThreads;
+      lo_r[u] = 0;
+      hi_r[u] = 0;
+      if (r < r1) {
+        lo_r[u] = a.start[r] - k0;
+        hi_r[u] = a.start[r + 1] - k0;
+      }
+    }
 #pragma unroll
     for (int u = 0; u < kTile / kThreads; ++u) {
       const Int k = static_cast<Int>(threadIdx.x) + u * kThreads;
@@ -101,15 +115,17 @@ __global__ void __launch_bounds__(kThreads)
     }
     __syncthreads();
     // Rows in rounds of kThreads, one row per thread per round.
-    for (Int round = r0; round < r1; round += kThreads) {
+#pragma unroll
+    for (int u = 0; u < kRounds; ++u) {
+      const Int round = r0 + u * kThreads;
+      if (round >= r1) break;  // uniform across the block
       const Int r = round + static_cast<Int>(threadIdx.x);
       double sum = 0.0;
       bool wide = false;
       if (r < r1) {
-        const Int lo = a.start[r] - k0, hi = a.start[r + 1] - k0;
-        wide = fast && hi - lo > kSerialRowMax;
+        wide = fast && hi_r[u] - lo_r[u] > kSerialRowMax;
         if (!wide)
-          for (Int k = lo; k < hi; ++k) sum += tile[k];
+          for (Int k = lo_r[u]; k < hi_r[u]; ++k) sum += tile[k];
       }
       if (fast) {
         // The wave's long rows, one at a time: lanes stride the row's staged
@@ -117,12 +133,10 @@ __global__ void __launch_bounds__(kThreads)
         // thread keeps the total for its epilogue.
         unsigned long long mask = __ballot(wide);
         const int lane = static_cast<int>(threadIdx.x & 63);
-        const Int base = round + static_cast<Int>(threadIdx.x & ~63u);
         while (mask != 0ull) {
           const int bit = __ffsll(static_cast<long long>(mask)) - 1;
           mask &= mask - 1ull;
-          const Int rr = base + bit;
-          const Int lo = a.start[rr] - k0, hi = a.start[rr + 1] - k0;
+          const Int lo = __shfl(lo_r[u], bit, 64), hi = __shfl(hi_r[u], bit, 64);
           double s = 0.0;
           for (Int k = lo + lane; k < hi; k += 64) s += tile[k];
           s = wave_fold(s, false);
@@ -720,6 +734,8 @@ template <class Epi>
 void spmv_with(hipStream_t s, const DevCsr& a, const double* x, const Epi& e, double* partials,
                const char* what, const LaunchEvents* ev = nullptr, bool fast = false) {
   if (a.blocks <= 0) return;
+  if (!fast && a.max_rows > kTileRows)
+    throw std::invalid_argument(std::string(what) + ": a fast-order plan in the storage order");
   const dim3 grid(static_cast<unsigned>(a.blocks));
   // The fast order runs on plans with the large tile (prepare_fast), the
   // storage order on plans with the small one.

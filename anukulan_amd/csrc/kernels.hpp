@@ -20,11 +20,13 @@ using Int = std::int32_t;
 inline constexpr int kThreads = 256;      // one workgroup = 4 wave64s
 inline constexpr int kTileNnz = 1024;     // products staged in LDS per row block
 inline constexpr int kTileRows = 256;     // most rows per row block
-// Plans for the fast product order: larger tiles, rows taken in rounds of
-// kThreads, so each block issues its whole tile's loads in two memory trips
-// and a few thousand blocks cover a large matrix.
-inline constexpr int kFastTileNnz = 4096;
-inline constexpr int kFastTileRows = 2048;
+// Plans for the fast product order. Measured on the 1M-column random LP,
+// larger tiles (4096 nonzeros, rows in rounds of kThreads) were slower than
+// the storage order's tile: the products are bound by the L2 request rate of
+// their random gathers, not by per-block latency. The machinery stays
+// (kernels are templated on the tile; rounds prefetch their row bounds).
+inline constexpr int kFastTileNnz = 1024;
+inline constexpr int kFastTileRows = 256;
 inline constexpr int kMaxRed = 8;         // most reduction values one kernel emits
 
 // A CSR matrix on the device plus its row-block plan: block b covers rows
@@ -40,6 +42,7 @@ struct DevCsr {
   const Int* index = nullptr;
   const double* value = nullptr;
   const Int* block_row = nullptr;
+  Int max_rows = kTileRows;  // most rows in one block of this plan
 };
 
 // Per-iteration scalars the fused kernels read from device memory, so a

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <iterator>
 #include <map>
 #include <stdexcept>
 
@@ -31,6 +32,7 @@ struct Lane {
   std::vector<double> oc, oq, olo, ohi;    // data the convergence test measures
   std::vector<double> warm_x, warm_y;      // empty: cold start
   double initial_omega = 0.0;              // 0: derive from ||c|| / ||q||
+  int termination = -1;                    // -1: the engine's; else this LP's own test
   double q_two = 0, c_two = 0, q_inf = 0, c_inf = 0;  // ResidualScales of (oq, oc)
   void set_scales() {
     q_two = host_two_norm(oq);
@@ -101,6 +103,7 @@ class Engine {
   int count_ = 0;
   DevCsr dk_, dkt_;
   std::vector<Control> ctl_;
+  std::vector<anuk_pdhg_options> lane_opt_;  // each lane's options (its termination test)
   Mask active_ = 0;
   int cur_ = 0;
 
@@ -430,12 +433,29 @@ std::vector<Polished> polish_lanes(HipBackend& be, const Csr& k, const Csr& kt, 
     }
   }
   const std::vector<double> ones_m(uz(m), 1.0), ones_n(uz(n), 1.0);
-  sub.termination = ANUK_TERM_PRIMAL_FEASIBILITY;
-  const std::vector<LaneOutcome> pr =
-      run_engine(be, k, kt, neq, ones_m, ones_n, norm, sub, primal, start, nullptr);
-  sub.termination = ANUK_TERM_DUAL_FEASIBILITY;
-  const std::vector<LaneOutcome> dr =
-      run_engine(be, k, kt, neq, ones_m, ones_n, norm, sub, dual, start, nullptr);
+  std::vector<LaneOutcome> pr, dr;
+  if (2 * cnt <= static_cast<size_t>(kLanes)) {
+    // Both feasibility problems of every seed share K: one sub-batch, each
+    // lane with its own termination test.
+    std::vector<Lane> both;
+    for (Lane& p : primal) {
+      p.termination = ANUK_TERM_PRIMAL_FEASIBILITY;
+      both.push_back(std::move(p));
+    }
+    for (Lane& d : dual) {
+      d.termination = ANUK_TERM_DUAL_FEASIBILITY;
+      both.push_back(std::move(d));
+    }
+    std::vector<LaneOutcome> out =
+        run_engine(be, k, kt, neq, ones_m, ones_n, norm, sub, both, start, nullptr);
+    pr.assign(std::make_move_iterator(out.begin()), std::make_move_iterator(out.begin() + static_cast<long>(cnt)));
+    dr.assign(std::make_move_iterator(out.begin() + static_cast<long>(cnt)), std::make_move_iterator(out.end()));
+  } else {
+    sub.termination = ANUK_TERM_PRIMAL_FEASIBILITY;
+    pr = run_engine(be, k, kt, neq, ones_m, ones_n, norm, sub, primal, start, nullptr);
+    sub.termination = ANUK_TERM_DUAL_FEASIBILITY;
+    dr = run_engine(be, k, kt, neq, ones_m, ones_n, norm, sub, dual, start, nullptr);
+  }
   std::vector<Polished> out(cnt);
   for (size_t s = 0; s < cnt; ++s) {
     scaling.unscale_primal(pr[s].x, &out[s].x);
@@ -505,12 +525,13 @@ std::vector<LaneOutcome> Engine::run() {
   allocate();
   for (Lane& l : lanes_) l.set_scales();
   ctl_.resize(static_cast<size_t>(count_));
+  lane_opt_.assign(static_cast<size_t>(count_), opt_);
   for (int l = 0; l < count_; ++l) {
     const Lane& ln = lanes_[static_cast<size_t>(l)];
-    anuk_pdhg_options o = opt_;
-    if (ln.initial_omega > 0.0) o.initial_primal_weight = ln.initial_omega;
+    anuk_pdhg_options& o = lane_opt_[static_cast<size_t>(l)];
+    if (ln.termination >= 0) o.termination = ln.termination;
     Control& c = ctl_[static_cast<size_t>(l)];
-    c.init(opt_, norm_, host_two_norm(ln.c), host_two_norm(ln.q));
+    c.init(o, norm_, host_two_norm(ln.c), host_two_norm(ln.q));
     if (ln.initial_omega > 0.0) c.omega = ln.initial_omega;
   }
   upload_lanes();
@@ -534,7 +555,10 @@ std::vector<LaneOutcome> Engine::run() {
     const int steps = period_end - iteration;
     const bool has_check = (period_end % freq == 0);
     upload_params();
-    const bool graph = opt_.use_graphs != 0 && has_check && steps == freq && !be_.profiling();
+    // Polishing sub-batches run a few periods each; capturing graphs for
+    // them costs more than it saves.
+    const bool graph =
+        opt_.use_graphs != 0 && depth_ == 0 && has_check && steps == freq && !be_.profiling();
     if (graph) {
       hip_check(hipGraphLaunch(graph_for(cur_), be_.stream()), "hipGraphLaunch(batch)");
     } else {

@@ -126,9 +126,15 @@ __global__ void __launch_bounds__(256)
                  const LaneParams* __restrict__ params, unsigned long long mask) {
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int r = wave_row();
-  if (r >= a.rows || !lane_on(params, mask, lane)) return;
-  out1[at(r, lane)] = row_sum(a, r, v1, lane);
-  if (v2 != nullptr) out2[at(r, lane)] = row_sum(a, r, v2, lane);
+  if (r >= a.rows) return;  // uniform across the wave
+  // Every lane takes part in the row sums; only the active lanes store.
+  const bool on = lane_on(params, mask, lane);
+  const double s1 = row_sum(a, r, v1, lane);
+  if (on) out1[at(r, lane)] = s1;
+  if (v2 != nullptr) {
+    const double s2 = row_sum(a, r, v2, lane);
+    if (on) out2[at(r, lane)] = s2;
+  }
 }
 
 // Check reductions, per lane, in a fixed tree: each wave folds a contiguous

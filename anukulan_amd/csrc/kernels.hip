@@ -83,7 +83,8 @@ __global__ void __launch_bounds__(kThreads)
                 int fast) {
   __shared__ double tile[kTile];
   constexpr int R = Epi::kR > 0 ? Epi::kR : 1;
-  const int b = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  int b = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  if (a.block_map != nullptr) b = a.block_map[b];
   const Epi e = epi.resolve(b);
   if (e.skip()) return;  // uniform across the block
   const Int r0 = a.block_row[b], r1 = a.block_row[b + 1];

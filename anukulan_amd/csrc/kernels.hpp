@@ -43,6 +43,9 @@ struct DevCsr {
   const double* value = nullptr;
   const Int* block_row = nullptr;
   Int max_rows = kTileRows;  // most rows in one block of this plan
+  // Optional compaction: launched block i runs planned block block_map[i]
+  // (`blocks` then counts the launched ones). Null: every planned block.
+  const Int* block_map = nullptr;
 };
 
 // Per-iteration scalars the fused kernels read from device memory, so a

@@ -38,6 +38,10 @@ struct Stack {
   std::vector<double> c, q, lower, upper;  // scaled
   std::vector<double> oc, oq, olo, ohi;    // as given
   std::vector<Scaling> scaling;            // per LP
+  // The polishing sub-problems' data, stack-wide (built once per solve):
+  // zeros and ones of both sizes, and the dual problem's bounds (the finite
+  // ones zeroed).
+  std::vector<double> zero_n, zero_m, ones_n, ones_m, dual_lower, dual_upper;
   Int cols(Int l) const { return col_off[uz(l) + 1] - col_off[uz(l)]; }
   Int rows(Int l) const { return row_off[uz(l) + 1] - row_off[uz(l)]; }
 };
@@ -136,6 +140,7 @@ struct DeviceStack {
   const Int* kt_block_lp = nullptr;
   const Int* k_lp_blocks = nullptr;
   const Int* kt_lp_blocks = nullptr;
+  std::vector<Int> k_lp_blocks_host, kt_lp_blocks_host;  // the same, on the host
   ChunkPlan cols, rows;
   const Int* col_lp_chunks = nullptr;
   const Int* row_lp_chunks = nullptr;
@@ -230,6 +235,8 @@ void upload_stack(const Stack& st, DeviceStack* ds) {
   ds->kt_block_lp = ds->put(pkt.block_lp);
   ds->k_lp_blocks = ds->put(pk.lp_blocks);
   ds->kt_lp_blocks = ds->put(pkt.lp_blocks);
+  ds->k_lp_blocks_host = pk.lp_blocks;
+  ds->kt_lp_blocks_host = pkt.lp_blocks;
   const Chunks cc = chunk_stack(st.col_off), rc = chunk_stack(st.row_off);
   ds->col_chunks = static_cast<Int>(cc.lp.size());
   ds->row_chunks = static_cast<Int>(rc.lp.size());
@@ -295,6 +302,16 @@ void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, const 
     get(rs, &st->row_scale);
     get(cs, &st->col_scale);
     be.synchronize();
+  }
+  st->zero_n.assign(uz(n), 0.0);
+  st->zero_m.assign(uz(m), 0.0);
+  st->ones_n.assign(uz(n), 1.0);
+  st->ones_m.assign(uz(m), 1.0);
+  st->dual_lower = st->lower;
+  st->dual_upper = st->upper;
+  for (Int j = 0; j < n; ++j) {
+    if (st->dual_lower[uz(j)] > -kInf) st->dual_lower[uz(j)] = 0.0;
+    if (st->dual_upper[uz(j)] < kInf) st->dual_upper[uz(j)] = 0.0;
   }
   for (Int l = 0; l < st->lps; ++l) {
     Scaling& sc = st->scaling[uz(l)];
@@ -458,6 +475,7 @@ class Engine {
   void enqueue_period(int steps, bool has_check, int parity);
   hipGraphExec_t graph_for(int parity);
   void extract(Int l, Outcome* o);
+  void compact();
   void run_restarts(const std::vector<Int>& who);
   void run_polish(const std::vector<Int>& who, Int budget, std::vector<Outcome>& out,
                   std::vector<char>* stopped);
@@ -475,6 +493,10 @@ class Engine {
   int depth_;
 
   Int L_ = 0;
+  // The stack's matrices as launched: only the active LPs' row blocks
+  // (block_map), rebuilt when a quarter of them have retired.
+  DevCsr k_run_, kt_run_;
+  Int compacted_blocks_ = 0;
   std::vector<Control> ctl_;
   std::vector<char> active_;
   int cur_ = 0;
@@ -489,6 +511,37 @@ class Engine {
   size_t pinned_doubles_ = 0;
   std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
 };
+
+void Engine::compact() {
+  std::vector<Int> km, ktm;
+  for (Int l = 0; l < L_; ++l) {
+    if (!active_[uz(l)]) continue;
+    for (Int b = ds_.k_lp_blocks_host[uz(l)]; b < ds_.k_lp_blocks_host[uz(l) + 1]; ++b) km.push_back(b);
+    for (Int b = ds_.kt_lp_blocks_host[uz(l)]; b < ds_.kt_lp_blocks_host[uz(l) + 1]; ++b) ktm.push_back(b);
+  }
+  auto put = [&](int slot, const std::vector<Int>& h) {
+    Int* d = reinterpret_cast<Int*>(be_.scratch(slot, h.size() / 2 + 1));
+    if (!h.empty())
+      hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(Int), hipMemcpyHostToDevice, be_.stream()),
+                "upload block map");
+    return d;
+  };
+  // The maps feed the captured graphs: wait for them before rewriting.
+  be_.synchronize();
+  for (auto& g : graphs_) {
+    (void)hipGraphExecDestroy(g.second.second);
+    (void)hipGraphDestroy(g.second.first);
+  }
+  graphs_.clear();
+  k_run_ = ds_.k;
+  kt_run_ = ds_.kt;
+  k_run_.block_map = put(kMultiScratchSlot + 10 + 2 * depth_, km);
+  kt_run_.block_map = put(kMultiScratchSlot + 11 + 2 * depth_, ktm);
+  k_run_.blocks = static_cast<Int>(km.size());
+  kt_run_.blocks = static_cast<Int>(ktm.size());
+  compacted_blocks_ = k_run_.blocks + kt_run_.blocks;
+  be_.synchronize();  // the host lists go out of scope
+}
 
 void Engine::allocate() {
   const size_t N = uz(st_.n), M = uz(st_.m), L = uz(L_);
@@ -619,11 +672,11 @@ void Engine::enqueue_period(int steps, bool has_check, int parity) {
     const bool diag = has_check && step == steps - 1;
     PrimalBufs pb{X_[p], AX_, X_[1 - p], XBAR_, DX_, AX_};
     const LaunchEvents pe = be_.launch_events(diag ? "multi_primal_step_check" : "multi_primal_step");
-    launch_multi_primal_step(s, ds_.kt, ds_.kt_block_lp, Y_[p], prob_, d_params_, step, pb, diag,
+    launch_multi_primal_step(s, kt_run_, ds_.kt_block_lp, Y_[p], prob_, d_params_, step, pb, diag,
                              PA_, &pe);
     DualBufs db{Y_[p], KX_[p], AY_, AKX_, Y_[1 - p], KX_[1 - p], DY_, KDX_, AY_, AKX_};
     const LaunchEvents de = be_.launch_events(diag ? "multi_dual_step_check" : "multi_dual_step");
-    launch_multi_dual_step(s, ds_.k, ds_.k_block_lp, XBAR_, prob_, d_params_, step, db, diag, PB_,
+    launch_multi_dual_step(s, k_run_, ds_.k_block_lp, XBAR_, prob_, d_params_, step, db, diag, PB_,
                            &de);
     p = 1 - p;
   }
@@ -632,10 +685,10 @@ void Engine::enqueue_period(int steps, bool has_check, int parity) {
     launch_multi_fold(s, PA_, 1, kSumDx2, 0u, ds_.kt_lp_blocks, L_, d_params_, CHECK_);
     launch_multi_fold(s, PB_, 7, kSumDy2, (1u << 2) | (1u << 4) | (1u << 5), ds_.k_lp_blocks, L_,
                       d_params_, CHECK_);
-    launch_multi_conv_product(s, ds_.kt, ds_.kt_block_lp, Y_[p], X_[p], prob_, d_params_, PC_);
+    launch_multi_conv_product(s, kt_run_, ds_.kt_block_lp, Y_[p], X_[p], prob_, d_params_, PC_);
     launch_multi_fold(s, PC_, 4, kDualSq, 1u << 1, ds_.kt_lp_blocks, L_, d_params_, CHECK_);
     if (opt_.detect_infeasibility) {
-      launch_multi_farkas_product(s, ds_.kt, ds_.kt_block_lp, DY_, prob_, d_params_, CHECK_, PD_);
+      launch_multi_farkas_product(s, kt_run_, ds_.kt_block_lp, DY_, prob_, d_params_, CHECK_, PD_);
       launch_multi_fold(s, PD_, 2, kFarkasColBad, 1u << 0, ds_.kt_lp_blocks, L_, d_params_, CHECK_);
     }
   }
@@ -686,7 +739,7 @@ void Engine::run_restarts(const std::vector<Int>& who) {
   hipStream_t s = be_.stream();
   hip_check(hipMemcpyAsync(d_rparams_, f.data(), f.size() * sizeof(IterParams), hipMemcpyHostToDevice, s),
             "restart flags");
-  launch_multi_product(s, ds_.k, ds_.k_block_lp, X_[cur_], KX_[cur_], d_rparams_);
+  launch_multi_product(s, k_run_, ds_.k_block_lp, X_[cur_], KX_[cur_], d_rparams_);
   const bool previous = !opt_.pid_primal_weight;
   launch_multi_sumsq(s, ds_.cols, X_[cur_], previous ? PRX_ : RX_, RX_, PRX_, true, d_rparams_, PCH_,
                      ds_.col_lp_chunks, L_, RESTART_, 2, 0);
@@ -729,13 +782,12 @@ std::vector<Polished> polish_stack(HipBackend& be, const Stack& st, const Device
   sub.time_limit_seconds = std::fmax(0.0, opt.time_limit_seconds - seconds_since(start));
   sub.warm_x = nullptr;
   sub.warm_y = nullptr;
-  const std::vector<double> ones_n(uz(st.n), 1.0), ones_m(uz(st.m), 1.0);
-  const std::vector<double> zero_n(uz(st.n), 0.0), zero_m(uz(st.m), 0.0);
-  std::vector<double> dlo = st.lower, dhi = st.upper;
-  for (size_t j = 0; j < dlo.size(); ++j) {
-    if (dlo[j] > -kInf) dlo[j] = 0.0;
-    if (dhi[j] < kInf) dhi[j] = 0.0;
-  }
+  const std::vector<double>& ones_n = st.ones_n;
+  const std::vector<double>& ones_m = st.ones_m;
+  const std::vector<double>& zero_n = st.zero_n;
+  const std::vector<double>& zero_m = st.zero_m;
+  const std::vector<double>& dlo = st.dual_lower;
+  const std::vector<double>& dhi = st.dual_upper;
   auto runs_for = [&](bool primal) {
     std::vector<LpRun> runs(uz(st.lps));
     for (LpRun& r : runs) r.participates = false;
@@ -842,7 +894,8 @@ std::vector<Outcome> Engine::run() {
   }
   upload();
   upload_params();
-  launch_multi_product(be_.stream(), ds_.k, ds_.k_block_lp, X_[0], KX_[0], d_params_);
+  compact();
+  launch_multi_product(be_.stream(), k_run_, ds_.k_block_lp, X_[0], KX_[0], d_params_);
   cur_ = 0;
   auto lap = [&](double Phases::*field) {
     const Clock::time_point now = Clock::now();
@@ -946,6 +999,13 @@ std::vector<Outcome> Engine::run() {
     }
     run_restarts(restart);
     be_.synchronize();
+    Int live = 0;
+    for (Int l = 0; l < L_; ++l) {
+      if (!active_[uz(l)]) continue;
+      live += ds_.k_lp_blocks_host[uz(l) + 1] - ds_.k_lp_blocks_host[uz(l)];
+      live += ds_.kt_lp_blocks_host[uz(l) + 1] - ds_.kt_lp_blocks_host[uz(l)];
+    }
+    if (4 * live < 3 * compacted_blocks_) compact();
     lap(&Phases::checks);
   }
   for (Int l = 0; l < L_; ++l) {

@@ -41,6 +41,18 @@ class Model(ctypes.Structure):
                 ("objective_offset", c_double)]
 
 
+class MpsOptions(ctypes.Structure):
+    _fields_ = [("format", c_int32), ("negative_up_bound", c_int32),
+                ("infinity_threshold", c_double)]
+
+
+class MpsInfo(ctypes.Structure):
+    _fields_ = [("rows", c_int32), ("cols", c_int32), ("nnz", c_int64), ("hessian_nnz", c_int64),
+                ("maximize", c_int32), ("fixed_format", c_int32), ("objective_offset", c_double),
+                ("free_rows_dropped", c_int32), ("warnings", c_int32),
+                ("objective_nonzeros", c_int64)]
+
+
 class LpInfo(ctypes.Structure):
     _fields_ = [("rows", c_int32), ("cols", c_int32), ("nnz", c_int32),
                 ("num_equalities", c_int32), ("objective_scale", c_double),
@@ -151,6 +163,22 @@ SIGNATURES = {
                                   POINTER(AdjointInfo)]),
     "anuk_estimate_matrix_norm": (c_int, [c_void_p, c_void_p, c_int32, c_double, DP]),
     "anuk_evaluate_residual": (c_int, [c_void_p, DP, DP, POINTER(PdhgResidual)]),
+    "anuk_mps_default_options": (None, [POINTER(MpsOptions)]),
+    "anuk_mps_read": (c_int, [c_char_p, POINTER(MpsOptions), POINTER(c_void_p)]),
+    "anuk_mps_read_text": (c_int, [c_char_p, c_int64, c_char_p, POINTER(MpsOptions),
+                                   POINTER(c_void_p)]),
+    "anuk_mps_free": (None, [c_void_p]),
+    "anuk_mps_info_get": (c_int, [c_void_p, POINTER(MpsInfo)]),
+    "anuk_mps_model": (c_int, [c_void_p, POINTER(Model)]),
+    "anuk_mps_matrix": (c_int, [c_void_p, POINTER(IP), POINTER(IP), POINTER(DP)]),
+    "anuk_mps_hessian": (c_int, [c_void_p, POINTER(IP), POINTER(IP), POINTER(DP)]),
+    "anuk_mps_col_types": (IP, [c_void_p]),
+    "anuk_mps_name": (c_char_p, [c_void_p]),
+    "anuk_mps_row_name": (c_char_p, [c_void_p, c_int32]),
+    "anuk_mps_col_name": (c_char_p, [c_void_p, c_int32]),
+    "anuk_mps_warning": (c_char_p, [c_void_p, c_int32]),
+    "anuk_mps_write": (c_int, [c_char_p, POINTER(Model), c_char_p, POINTER(c_char_p),
+                               POINTER(c_char_p), IP]),
 }
 
 

@@ -29,6 +29,7 @@ class AnukError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"anukulan_hip error {code}: {message}")
         self.code = code
+        self.message = message
 
 
 def _dp(a: np.ndarray):

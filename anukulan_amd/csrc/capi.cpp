@@ -2,6 +2,7 @@
 // include/anukulan_hip.h). Each one validates its arguments, forwards to the
 // C++ backend or solver, and turns any exception into a status code plus a
 // message retrievable with anuk_last_error.
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -11,6 +12,7 @@
 #include "../../include/anukulan_hip.h"
 #include "backend.hpp"
 #include "lp.hpp"
+#include "mps.hpp"
 #include "adjoint.hpp"
 #include "batch.hpp"
 #include "multi.hpp"
@@ -24,6 +26,10 @@ struct anuk_ctx {
 
 struct anuk_lp {
   anuk::StandardLp lp;
+};
+
+struct anuk_mps {
+  anuk::MpsModel m;
 };
 
 namespace {
@@ -49,6 +55,10 @@ int guarded(anuk_ctx* ctx, F&& f) {
     return status_of(ctx, ANUK_ERR_NOMEM, "out of memory");
   } catch (const std::invalid_argument& e) {
     return status_of(ctx, ANUK_ERR_INVALID, e.what());
+  } catch (const anuk::MpsError& e) {
+    return status_of(ctx, ANUK_ERR_PARSE, e.what());
+  } catch (const anuk::FileError& e) {
+    return status_of(ctx, ANUK_ERR_IO, e.what());
   } catch (const std::runtime_error& e) {
     return status_of(ctx, ANUK_ERR_DEVICE, e.what());
   } catch (const std::exception& e) {
@@ -96,6 +106,25 @@ void fill_residual(const anuk::Residual& r, anuk_pdhg_residual* o) {
   o->relative_gap = r.relative_gap;
   o->relative_primal_inf = r.relative_primal_inf;
   o->relative_dual_inf = r.relative_dual_inf;
+}
+
+anuk::MpsOptions mps_options(const anuk_mps_options* o) {
+  anuk::MpsOptions opt;
+  if (o == nullptr) return opt;
+  need(o->format >= 0 && o->format <= 2, "format must be 0 (auto), 1 (free) or 2 (fixed)");
+  need(o->negative_up_bound == 0 || o->negative_up_bound == 1, "negative_up_bound must be 0 or 1");
+  need(o->infinity_threshold > 0.0, "infinity_threshold must be positive");
+  opt.format = o->format;
+  opt.negative_up_minus_infinity = o->negative_up_bound == 1;
+  opt.infinity_threshold = o->infinity_threshold;
+  return opt;
+}
+
+int csr_out(const anuk::Csr& a, const int32_t** start, const int32_t** index, const double** value) {
+  if (start) *start = a.start.data();
+  if (index) *index = a.index.data();
+  if (value) *value = a.value.data();
+  return ANUK_OK;
 }
 
 }  // namespace
@@ -331,7 +360,7 @@ int anuk_lp_from_model(const anuk_model* model, anuk_lp** out) {
   *out = nullptr;
   return guarded(nullptr, [&] {
     need(model->rows >= 0 && model->cols >= 0 && model->nnz >= 0, "negative model size");
-    need(model->objective && model->col_lower && model->col_upper, "null column data");
+    need(model->cols == 0 || (model->objective && model->col_lower && model->col_upper), "null column data");
     need(model->rows == 0 || (model->row_lower && model->row_upper), "null row data");
     need(model->nnz == 0 || (model->row_index && model->col_index && model->value), "null triplets");
     anuk::Model m;
@@ -650,6 +679,149 @@ int anuk_evaluate_residual(const anuk_lp* lp, const double* x, const double* y,
     const std::vector<double> vx(x, x + lp->lp.cols());
     const std::vector<double> vy(y, y + lp->lp.rows());
     fill_residual(anuk::evaluate_residual(lp->lp, vx, vy), out);
+  });
+}
+
+// ---- MPS files --------------------------------------------------------------
+
+void anuk_mps_default_options(anuk_mps_options* out) {
+  if (out == nullptr) return;
+  const anuk::MpsOptions d;
+  out->format = d.format;
+  out->negative_up_bound = d.negative_up_minus_infinity ? 1 : 0;
+  out->infinity_threshold = d.infinity_threshold;
+}
+
+int anuk_mps_read(const char* path, const anuk_mps_options* options, anuk_mps** out) {
+  if (path == nullptr || out == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded(nullptr, [&] {
+    auto h = std::make_unique<anuk_mps>();
+    h->m = anuk::read_mps_file(path, mps_options(options));
+    *out = h.release();
+  });
+}
+
+int anuk_mps_read_text(const char* text, int64_t length, const char* source,
+                       const anuk_mps_options* options, anuk_mps** out) {
+  if (out == nullptr || length < 0 || (text == nullptr && length > 0))
+    return status_of(nullptr, ANUK_ERR_INVALID, "null or negative argument");
+  *out = nullptr;
+  return guarded(nullptr, [&] {
+    auto h = std::make_unique<anuk_mps>();
+    h->m = anuk::read_mps_text(std::string_view(text ? text : "", static_cast<size_t>(length)),
+                               source ? source : "<text>", mps_options(options));
+    *out = h.release();
+  });
+}
+
+void anuk_mps_free(anuk_mps* mps) { delete mps; }
+
+int anuk_mps_info_get(const anuk_mps* mps, anuk_mps_info* out) {
+  if (mps == nullptr || out == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  const anuk::MpsModel& m = mps->m;
+  out->rows = m.a.rows;
+  out->cols = m.a.cols;
+  out->nnz = m.a.nnz();
+  out->hessian_nnz = m.hessian.nnz();
+  out->maximize = m.maximize ? 1 : 0;
+  out->fixed_format = m.fixed_format ? 1 : 0;
+  out->objective_offset = m.objective_offset;
+  out->free_rows_dropped = m.free_rows_dropped;
+  out->warnings = static_cast<int32_t>(m.warnings.size());
+  out->objective_nonzeros = m.objective_nonzeros;
+  return ANUK_OK;
+}
+
+int anuk_mps_model(const anuk_mps* mps, anuk_model* out) {
+  if (mps == nullptr || out == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  const anuk::MpsModel& m = mps->m;
+  out->rows = m.a.rows;
+  out->cols = m.a.cols;
+  out->nnz = m.a.nnz();
+  out->row_index = m.coo_row.data();
+  out->col_index = m.coo_col.data();
+  out->value = m.a.value.data();
+  out->objective = m.objective.data();
+  out->row_lower = m.row_lower.data();
+  out->row_upper = m.row_upper.data();
+  out->col_lower = m.col_lower.data();
+  out->col_upper = m.col_upper.data();
+  out->maximize = m.maximize ? 1 : 0;
+  out->objective_offset = m.objective_offset;
+  return ANUK_OK;
+}
+
+int anuk_mps_matrix(const anuk_mps* mps, const int32_t** start, const int32_t** index,
+                    const double** value) {
+  if (mps == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  return csr_out(mps->m.a, start, index, value);
+}
+
+int anuk_mps_hessian(const anuk_mps* mps, const int32_t** start, const int32_t** index,
+                     const double** value) {
+  if (mps == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  return csr_out(mps->m.hessian, start, index, value);
+}
+
+const int32_t* anuk_mps_col_types(const anuk_mps* mps) {
+  return mps == nullptr ? nullptr : mps->m.col_type.data();
+}
+
+const char* anuk_mps_name(const anuk_mps* mps) { return mps == nullptr ? nullptr : mps->m.name.c_str(); }
+
+const char* anuk_mps_row_name(const anuk_mps* mps, int32_t row) {
+  if (mps == nullptr || row < 0 || static_cast<size_t>(row) >= mps->m.row_names.size()) return nullptr;
+  return mps->m.row_names[static_cast<size_t>(row)].c_str();
+}
+
+const char* anuk_mps_col_name(const anuk_mps* mps, int32_t col) {
+  if (mps == nullptr || col < 0 || static_cast<size_t>(col) >= mps->m.col_names.size()) return nullptr;
+  return mps->m.col_names[static_cast<size_t>(col)].c_str();
+}
+
+const char* anuk_mps_warning(const anuk_mps* mps, int32_t k) {
+  if (mps == nullptr || k < 0 || static_cast<size_t>(k) >= mps->m.warnings.size()) return nullptr;
+  return mps->m.warnings[static_cast<size_t>(k)].c_str();
+}
+
+int anuk_mps_write(const char* path, const anuk_model* model, const char* name,
+                   const char* const* row_names, const char* const* col_names,
+                   const int32_t* col_types) {
+  if (path == nullptr || model == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  return guarded(nullptr, [&] {
+    need(model->rows >= 0 && model->cols >= 0 && model->nnz >= 0, "negative model size");
+    need(model->cols == 0 || (model->objective && model->col_lower && model->col_upper), "null column data");
+    need(model->rows == 0 || (model->row_lower && model->row_upper), "null row data");
+    need(model->nnz == 0 || (model->row_index && model->col_index && model->value), "null triplets");
+    std::vector<anuk::Entry> e(static_cast<size_t>(model->nnz));
+    for (int64_t k = 0; k < model->nnz; ++k) e[static_cast<size_t>(k)] = {model->row_index[k], model->col_index[k], model->value[k]};
+    const anuk::Csr a = anuk::Csr::from_triplets(model->rows, model->cols, std::move(e));
+    anuk::MpsWriteModel w;
+    w.name = name ? name : "";
+    w.rows = model->rows;
+    w.cols = model->cols;
+    w.start = a.start.data();
+    w.index = a.index.data();
+    w.value = a.value.data();
+    w.objective = model->objective;
+    w.row_lower = model->row_lower;
+    w.row_upper = model->row_upper;
+    w.col_lower = model->col_lower;
+    w.col_upper = model->col_upper;
+    w.maximize = model->maximize != 0;
+    w.objective_offset = model->objective_offset;
+    w.row_names = row_names;
+    w.col_names = col_names;
+    w.col_type = col_types;
+    if (col_types)
+      for (int32_t j = 0; j < model->cols; ++j) need(col_types[j] >= 0 && col_types[j] <= 3, "column type out of range");
+    const std::string text = anuk::write_mps_text(w);
+    std::FILE* f = std::fopen(path, "wb");
+    if (f == nullptr) throw anuk::FileError(std::string("cannot open \"") + path + "\" for writing");
+    const size_t wrote = std::fwrite(text.data(), 1, text.size(), f);
+    const int closed = std::fclose(f);
+    if (wrote != text.size() || closed != 0) throw anuk::FileError(std::string("cannot write \"") + path + "\"");
   });
 }
 

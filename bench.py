@@ -13,7 +13,8 @@ algorithmic bytes by its time (`roofline`, dominant kernel), the
 512-LP stack solved in one set of launches (`c3`: LPs/s, p50/p99
 time-to-solve), and the
 reference's own solve_pdhg (oracle/_ref, one thread) on a bounded sample of
-the cases (`cpu_baseline`).
+the cases (`cpu_baseline`), and the same on every case, one case per thread
+over the host's cores (`cpu_baseline_all_cores`).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -73,6 +74,34 @@ def cpu_baseline(base, cases, tol, budget_s):
     return {"value": round(done / spent, 4), "unit": "cases/s", "cores": 1, "kind": "reference",
             "sample": f"first {done} of {len(cases)} cases solved by the reference's solve_pdhg "
                       f"(CPU backend, one thread): {iters} iterations in {spent:.1f} s"}
+
+
+def cpu_baseline_parallel(base, cases, tol, threads):
+    """Every case solved by the reference build's solve_pdhg, one case per
+    thread over `threads` threads (the reference's CPU backend keeps no shared
+    mutable state, and the ctypes calls release the GIL). The LPs are built
+    before the clock starts; the time is the wall clock of the whole pool."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        import _oracle
+    except Exception as e:
+        return {"value": None, "unit": "cases/s", "cores": threads, "kind": "reference",
+                "sample": f"unavailable: {e}"}
+    if _oracle.ref is None:
+        return {"value": None, "unit": "cases/s", "cores": threads, "kind": "reference",
+                "sample": "unavailable: oracle/_ref not built"}
+    from anukulan_amd import generators as G
+    lps = [_oracle.RefLp.from_natural(G.case_model(base, c)) for c in cases]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        rs = list(pool.map(lambda lp: lp.solve(threads=1, tolerance=tol), lps))
+    wall = time.perf_counter() - t0
+    iters = sum(r.iterations for r in rs)
+    return {"value": round(len(lps) / wall, 4), "unit": "cases/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"all {len(lps)} cases solved by the reference's solve_pdhg, one case per "
+                      f"thread on {threads} threads: {iters} iterations in {wall:.1f} s wall"}
 
 
 def measured_traffic(kernel):
@@ -197,6 +226,9 @@ def main():
                     help="solve only the first N cases of the stack (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the all-cores reference baseline (default: the usable "
+                         "cores, at most 16)")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
@@ -290,9 +322,11 @@ def main():
     c1 = None if args.no_c1 else c1_section(ctx, args)
     c3 = None if args.no_c3 else c3_section(ctx, args)
 
-    cpu = None
+    cpu = cpu_par = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(base, cases, args.tol, args.cpu_seconds)
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        cpu_par = cpu_baseline_parallel(base, cases, args.tol, threads)
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "cases/s", "n_gpus": 1,
@@ -306,7 +340,8 @@ def main():
                    "rows": m, "cols": n, "nnz": nnz, "cases": len(lps),
                    "cases_optimal": solved, "lane_iterations_per_step": lane_iters,
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "plain": plain, "c1": c1, "c3": c3,
+        "roofline": roofline, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_par,
+        "plain": plain, "c1": c1, "c3": c3,
         "batch_phases_s": {k: round(v, 4) for k, v in results[0].phases.items()},
         "kernels": {k: {"mean_us": v["mean_us"], "calls": v["calls"], "GB/s": v["gbs"]}
                     for k, v in kernels.items()},
@@ -315,6 +350,8 @@ def main():
         out["batched_vs_plain"] = round(value / plain["value"], 2)
     if cpu and cpu.get("value"):
         out["batched_vs_cpu_baseline"] = round(value / cpu["value"], 2)
+    if cpu_par and cpu_par.get("value"):
+        out["batched_vs_cpu_all_cores"] = round(value / cpu_par["value"], 2)
     ctx.close()
     print(json.dumps(out))
 

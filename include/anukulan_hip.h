@@ -38,7 +38,9 @@ enum anuk_status {
   ANUK_ERR_DEVICE = 2,      /* HIP runtime error, or no device */
   ANUK_ERR_NOMEM = 3,       /* device or host allocation failed */
   ANUK_ERR_UNSUPPORTED = 4, /* option combination this backend does not run */
-  ANUK_ERR_INTERNAL = 5
+  ANUK_ERR_INTERNAL = 5,
+  ANUK_ERR_PARSE = 6,       /* an MPS file the reader rejects; the message is "source:line: what" */
+  ANUK_ERR_IO = 7           /* a file that cannot be opened, read or written */
 };
 
 typedef struct anuk_ctx anuk_ctx;
@@ -455,6 +457,82 @@ int anuk_estimate_matrix_norm(anuk_ctx* ctx, const anuk_lp* lp, int32_t iteratio
 /* evaluate_residual (pdhg.hpp:548-549, pdhg.cpp:107-174) on the host. */
 int anuk_evaluate_residual(const anuk_lp* lp, const double* x, const double* y,
                            anuk_pdhg_residual* out);
+
+/* ------------------------------------------------------------------------ */
+/* MPS files: read_mps / read_mps_stream (mps_reader.hpp:13-50,              */
+/* mps_reader.cpp:711-728) and a writer the reference does not have         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct anuk_mps anuk_mps;
+
+/* MpsOptions (mps_reader.hpp:12-30). format: 0 decides from the ROWS
+ * section, 1 free, 2 fixed. negative_up_bound: 0 keeps the lower bound at 0
+ * (HiGHS), 1 sets it to -infinity on a continuous column (CPLEX). BOUNDS
+ * values at or beyond infinity_threshold in magnitude are infinite. */
+typedef struct anuk_mps_options {
+  int32_t format;
+  int32_t negative_up_bound;
+  double infinity_threshold;
+} anuk_mps_options;
+void anuk_mps_default_options(anuk_mps_options* out);
+
+/* Reads an MPS (or QPS) file into a model owned by *out. A file the reader
+ * rejects returns ANUK_ERR_PARSE and the message of MpsReadResult::error
+ * ("source:line: what", mps_reader.cpp:164-168), read with
+ * anuk_last_error(NULL); a file that cannot be opened returns ANUK_ERR_IO.
+ * options may be NULL for the defaults. */
+int anuk_mps_read(const char* path, const anuk_mps_options* options, anuk_mps** out);
+/* The same from `length` bytes of text in memory (read_mps_stream); `source`
+ * names it in messages. */
+int anuk_mps_read_text(const char* text, int64_t length, const char* source,
+                       const anuk_mps_options* options, anuk_mps** out);
+void anuk_mps_free(anuk_mps* mps);
+
+/* Sizes and the reader's counters (MpsReadResult, mps_reader.hpp:32-42). */
+typedef struct anuk_mps_info {
+  int32_t rows;
+  int32_t cols;
+  int64_t nnz;
+  int64_t hessian_nnz;
+  int32_t maximize;
+  int32_t fixed_format;
+  double objective_offset;
+  int32_t free_rows_dropped;
+  int32_t warnings;
+  int64_t objective_nonzeros;
+} anuk_mps_info;
+int anuk_mps_info_get(const anuk_mps* mps, anuk_mps_info* out);
+
+/* The model as an anuk_model whose arrays point into `mps` (valid until
+ * anuk_mps_free): triplets in CSR order, ready for anuk_lp_from_model.
+ * Column types and the Hessian are not part of anuk_model; the LP built from
+ * it is the continuous relaxation. */
+int anuk_mps_model(const anuk_mps* mps, anuk_model* out);
+/* A (rows x cols) and the Hessian Q (cols x cols, full symmetric, empty for
+ * an LP) as CSR arrays inside `mps`; any pointer may be NULL. */
+int anuk_mps_matrix(const anuk_mps* mps, const int32_t** start, const int32_t** index,
+                    const double** value);
+int anuk_mps_hessian(const anuk_mps* mps, const int32_t** start, const int32_t** index,
+                     const double** value);
+/* Column types (VarType order, model.hpp:12: 0 continuous, 1 integer,
+ * 2 semi-continuous, 3 semi-integer), cols entries inside `mps`. */
+const int32_t* anuk_mps_col_types(const anuk_mps* mps);
+/* Names inside `mps`; NULL when out of range. */
+const char* anuk_mps_name(const anuk_mps* mps);
+const char* anuk_mps_row_name(const anuk_mps* mps, int32_t row);
+const char* anuk_mps_col_name(const anuk_mps* mps, int32_t col);
+const char* anuk_mps_warning(const anuk_mps* mps, int32_t k);
+
+/* Writes `model` as free-format MPS that anuk_mps_read turns back into the
+ * same model, bit for bit (values in their shortest exact form). row_names,
+ * col_names and col_types may be NULL (names R<i> / C<j>, all continuous);
+ * names may not hold white space. MPS cannot state a free row (written as an
+ * N row, which the reader drops), a finite bound at or beyond 1e30, or a
+ * ranged row whose bounds no range reproduces exactly. No counterpart in the
+ * reference, which has no writer; for callers that store generated stacks. */
+int anuk_mps_write(const char* path, const anuk_model* model, const char* name,
+                   const char* const* row_names, const char* const* col_names,
+                   const int32_t* col_types);
 
 #ifdef __cplusplus
 }

@@ -233,6 +233,95 @@ void* ref_lp_from_mps(const char* text) {
   }
 }
 
+// ---- The MPS reader (mps_reader.cpp:711-728), model exported as read --------
+
+struct RefMps {
+  MpsReadResult r;
+  std::vector<int32_t> col_type;
+};
+
+// options: format 0 auto / 1 free / 2 fixed; negative_up 0 HiGHS / 1 CPLEX.
+// Returns a handle even when the file is rejected (ref_mps_ok says so).
+void* ref_mps_read_text(const char* text, int64_t length, int32_t format, int32_t negative_up,
+                        double infinity_threshold) {
+  try {
+    MpsOptions o;
+    o.format = format == 1 ? MpsOptions::Format::kFree
+               : format == 2 ? MpsOptions::Format::kFixed
+                             : MpsOptions::Format::kAuto;
+    o.negative_up_bound = negative_up == 1 ? MpsOptions::NegativeUpBound::kMinusInfinity
+                                           : MpsOptions::NegativeUpBound::kKeepLower;
+    o.infinity_threshold = infinity_threshold;
+    std::istringstream in(std::string(text, static_cast<std::size_t>(length)));
+    auto* h = new RefMps;
+    h->r = read_mps_stream(in, "<text>", o);
+    for (VarType t : h->r.model.col_type) h->col_type.push_back(static_cast<int32_t>(t));
+    return h;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+
+void ref_mps_free(void* h) { delete static_cast<RefMps*>(h); }
+
+int ref_mps_ok(const void* h) { return static_cast<const RefMps*>(h)->r.ok ? 1 : 0; }
+const char* ref_mps_error(const void* h) { return static_cast<const RefMps*>(h)->r.error.c_str(); }
+
+// sizes[0..7]: rows, cols, nnz, hessian nnz, maximize, free rows dropped,
+// objective nonzeros, fixed format; sizes[8]: warning count.
+void ref_mps_sizes(const void* h, int64_t* sizes) {
+  const MpsReadResult& r = static_cast<const RefMps*>(h)->r;
+  sizes[0] = r.model.num_rows();
+  sizes[1] = r.model.num_cols();
+  sizes[2] = r.model.constraints.nnz();
+  sizes[3] = r.model.hessian.nnz();
+  sizes[4] = r.model.sense == ObjSense::kMaximize ? 1 : 0;
+  sizes[5] = r.free_rows_dropped;
+  sizes[6] = r.objective_nonzeros;
+  sizes[7] = r.fixed_format ? 1 : 0;
+  sizes[8] = static_cast<int64_t>(r.warnings.size());
+}
+
+double ref_mps_offset(const void* h) { return static_cast<const RefMps*>(h)->r.model.objective_offset; }
+const char* ref_mps_name(const void* h) { return static_cast<const RefMps*>(h)->r.model.name.c_str(); }
+
+// Copies A (and Q when non-empty) as CSR and the vectors; NULL skips one.
+void ref_mps_arrays(const void* h, int32_t* start, int32_t* index, double* value,
+                    int32_t* q_start, int32_t* q_index, double* q_value, double* objective,
+                    double* row_lower, double* row_upper, double* col_lower, double* col_upper,
+                    int32_t* col_type) {
+  const RefMps* p = static_cast<const RefMps*>(h);
+  const Model& m = p->r.model;
+  auto put = [](auto* dst, const auto& src) {
+    if (dst != nullptr && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+  };
+  put(start, m.constraints.start());
+  put(index, m.constraints.index());
+  put(value, m.constraints.value());
+  if (m.hessian.nnz() > 0) {
+    put(q_start, m.hessian.start());
+    put(q_index, m.hessian.index());
+    put(q_value, m.hessian.value());
+  }
+  put(objective, m.objective);
+  put(row_lower, m.row_lower);
+  put(row_upper, m.row_upper);
+  put(col_lower, m.col_lower);
+  put(col_upper, m.col_upper);
+  put(col_type, p->col_type);
+}
+
+const char* ref_mps_row_name(const void* h, int32_t i) {
+  return static_cast<const RefMps*>(h)->r.model.row_names[static_cast<std::size_t>(i)].c_str();
+}
+const char* ref_mps_col_name(const void* h, int32_t j) {
+  return static_cast<const RefMps*>(h)->r.model.col_names[static_cast<std::size_t>(j)].c_str();
+}
+const char* ref_mps_warning(const void* h, int32_t k) {
+  return static_cast<const RefMps*>(h)->r.warnings[static_cast<std::size_t>(k)].c_str();
+}
+
 void* ref_lp_from_standard(int32_t rows, int32_t cols, int32_t neq, const int32_t* start,
                            const int32_t* index, const double* value, const double* c,
                            const double* q, const double* lower, const double* upper,

@@ -280,10 +280,64 @@ def _load_ref():
     L.ref_op_step_size_terms.argtypes = [c_int32, c_int32, DP, DP, DP, c_double, DP, DP]
     L.ref_op_convergence_terms.argtypes = [c_int32, c_int32, POINTER(ConvProblemC), DP, DP, DP,
                                            DP, POINTER(ConvTermsC)]
+    L.ref_mps_read_text.restype = c_void_p
+    L.ref_mps_read_text.argtypes = [c_char_p, ctypes.c_int64, c_int32, c_int32, c_double]
+    L.ref_mps_free.argtypes = [c_void_p]
+    L.ref_mps_ok.argtypes = [c_void_p]
+    L.ref_mps_error.restype = c_char_p
+    L.ref_mps_error.argtypes = [c_void_p]
+    L.ref_mps_sizes.argtypes = [c_void_p, POINTER(ctypes.c_int64)]
+    L.ref_mps_offset.restype = c_double
+    L.ref_mps_offset.argtypes = [c_void_p]
+    L.ref_mps_name.restype = c_char_p
+    L.ref_mps_name.argtypes = [c_void_p]
+    L.ref_mps_arrays.argtypes = [c_void_p, IP, IP, DP, IP, IP, DP, DP, DP, DP, DP, DP, IP]
+    for f in ("ref_mps_row_name", "ref_mps_col_name", "ref_mps_warning"):
+        getattr(L, f).restype = c_char_p
+        getattr(L, f).argtypes = [c_void_p, c_int32]
     return L
 
 
 ref = _load_ref()
+
+
+def ref_read_mps(text, format=0, negative_up=0, infinity_threshold=1e30) -> dict:
+    """The reference reader's result on `text` (read_mps_stream, source
+    "<text>"): {"ok", "error"} and, when ok, every field of the model."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    h = ref.ref_mps_read_text(data, len(data), format, negative_up, infinity_threshold)
+    if not h:
+        raise RuntimeError(ref.ref_last_error().decode())
+    try:
+        out = {"ok": bool(ref.ref_mps_ok(h)), "error": ref.ref_mps_error(h).decode()}
+        sz = (ctypes.c_int64 * 9)()
+        ref.ref_mps_sizes(h, sz)
+        rows, cols, nnz, qnnz = sz[0], sz[1], sz[2], sz[3]
+        out["warnings"] = [ref.ref_mps_warning(h, k).decode() for k in range(sz[8])]
+        if not out["ok"]:
+            return out
+        a = {"start": np.zeros(rows + 1, np.int32), "index": np.zeros(nnz, np.int32),
+             "value": np.zeros(nnz), "objective": np.zeros(cols), "row_lower": np.zeros(rows),
+             "row_upper": np.zeros(rows), "col_lower": np.zeros(cols), "col_upper": np.zeros(cols),
+             "col_type": np.zeros(cols, np.int32)}
+        q = (np.zeros(cols + 1, np.int32), np.zeros(qnnz, np.int32), np.zeros(qnnz)) if qnnz else None
+
+        def p(x, t):
+            return x.ctypes.data_as(t) if x is not None and x.size else None
+        ref.ref_mps_arrays(h, p(a["start"], IP), p(a["index"], IP), p(a["value"], DP),
+                           p(q[0], IP) if q else None, p(q[1], IP) if q else None,
+                           p(q[2], DP) if q else None, p(a["objective"], DP),
+                           p(a["row_lower"], DP), p(a["row_upper"], DP), p(a["col_lower"], DP),
+                           p(a["col_upper"], DP), p(a["col_type"], IP))
+        out.update(a)
+        out.update(rows=rows, cols=cols, maximize=bool(sz[4]), free_rows_dropped=sz[5],
+                   objective_nonzeros=sz[6], fixed_format=bool(sz[7]), hessian=q,
+                   objective_offset=ref.ref_mps_offset(h), name=ref.ref_mps_name(h).decode(),
+                   row_names=[ref.ref_mps_row_name(h, i).decode() for i in range(rows)],
+                   col_names=[ref.ref_mps_col_name(h, j).decode() for j in range(cols)])
+        return out
+    finally:
+        ref.ref_mps_free(h)
 
 
 class RefLp:

@@ -5,10 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
-#include <fstream>
-#include <iterator>
-#include <sstream>
-#include <unordered_map>
+#include <cstdio>
 #include <unordered_set>
 
 namespace anuk {
@@ -101,6 +98,58 @@ Part part_of(std::string_view word) {
 
 bool is_max(std::string_view w) { return same_upper(w, "MAX") || same_upper(w, "MAXIMIZE"); }
 
+// Names to indices: open addressing with linear probing at load <= 1/2,
+// keys are views into the file buffer stored with their 64-bit FNV-1a hash.
+// One probe usually touches one slot and the name itself, where a node-based
+// map walks a bucket and a node first.
+class NameTable {
+ public:
+  const Int* find(std::string_view k) const {
+    if (slots_.empty()) return nullptr;
+    const std::uint64_t h = hash(k);
+    for (std::size_t i = h & mask_;; i = (i + 1) & mask_) {
+      const Slot& s = slots_[i];
+      if (s.p == nullptr) return nullptr;
+      if (s.h == h && s.len == k.size() && std::memcmp(s.p, k.data(), k.size()) == 0) return &s.v;
+    }
+  }
+  bool contains(std::string_view k) const { return find(k) != nullptr; }
+  void insert(std::string_view k, Int v) {
+    if (2 * (used_ + 1) > slots_.size()) grow();
+    place(hash(k), k, v);
+    ++used_;
+  }
+
+ private:
+  struct Slot {
+    std::uint64_t h = 0;
+    const char* p = nullptr;  // null: empty slot
+    std::size_t len = 0;
+    Int v = 0;
+  };
+  static std::uint64_t hash(std::string_view k) {
+    std::uint64_t h = 1469598103934665603ull;
+    for (char c : k) h = (h ^ static_cast<unsigned char>(c)) * 1099511628211ull;
+    return h ^ (h >> 29);
+  }
+  void place(std::uint64_t h, std::string_view k, Int v) {
+    std::size_t i = h & mask_;
+    while (slots_[i].p != nullptr) i = (i + 1) & mask_;
+    // An empty name still needs a non-null pointer to mark the slot used.
+    slots_[i] = Slot{h, k.data() != nullptr ? k.data() : "", k.size(), v};
+  }
+  void grow() {
+    std::vector<Slot> old = std::move(slots_);
+    slots_.assign(old.empty() ? 64 : old.size() * 2, Slot{});
+    mask_ = slots_.size() - 1;
+    for (const Slot& s : old)
+      if (s.p != nullptr) place(s.h, std::string_view(s.p, s.len), s.v);
+  }
+  std::vector<Slot> slots_;
+  std::size_t mask_ = 0;
+  std::size_t used_ = 0;
+};
+
 constexpr Int kObjectiveRow = -1;
 constexpr Int kDroppedRow = -2;
 constexpr std::size_t kWarningCap = 64;
@@ -126,15 +175,20 @@ class Parser {
   }
 
   Int row_of(std::string_view name, bool* known) const {
-    const auto it = rows_.find(name);
-    *known = it != rows_.end();
-    return *known ? it->second : 0;
+    const Int* v = rows_.find(name);
+    *known = v != nullptr;
+    return v != nullptr ? *v : 0;
   }
+  // Consecutive COLUMNS lines almost always name the same column, so the
+  // last one looked up is checked before the map.
   Int column(std::string_view name) {
-    const auto it = cols_.find(name);
-    if (it != cols_.end()) return it->second;
+    if (last_col_ >= 0 && name == last_name_) return last_col_;
+    const Int* v = cols_.find(name);
+    last_name_ = name;
+    if (v != nullptr) return last_col_ = *v;
     const Int j = static_cast<Int>(m_.col_names.size());
-    cols_.emplace(name, j);
+    last_col_ = j;
+    cols_.insert(name, j);
     m_.col_names.emplace_back(name);
     m_.objective.push_back(0.0);
     m_.col_lower.push_back(0.0);
@@ -167,8 +221,10 @@ class Parser {
   std::vector<std::string_view> held_rows_;
   std::vector<long long> held_lines_;
 
-  std::unordered_map<std::string_view, Int> rows_;
-  std::unordered_map<std::string_view, Int> cols_;
+  NameTable rows_;
+  NameTable cols_;
+  std::string_view last_name_;
+  Int last_col_ = -1;
 
   std::vector<char> sense_;  // 'E', 'L', 'G' per kept row
   std::vector<double> rhs_, range_;
@@ -214,7 +270,7 @@ void Parser::on_row(const Tokens& t) {
   if (t.size() < 2) fail("ROWS entry needs a type and a name");
   const std::string type = upper_copy(t[0]);
   const std::string_view name = t[1];
-  if (rows_.count(name) != 0) {
+  if (rows_.contains(name)) {
     warn("duplicate row name \"" + std::string(name) + "\": later definition ignored");
     return;
   }
@@ -222,15 +278,15 @@ void Parser::on_row(const Tokens& t) {
     // The first N row is the objective; later ones are free rows, dropped.
     if (!objective_seen_) {
       objective_seen_ = true;
-      rows_.emplace(name, kObjectiveRow);
+      rows_.insert(name, kObjectiveRow);
     } else {
-      rows_.emplace(name, kDroppedRow);
+      rows_.insert(name, kDroppedRow);
       ++m_.free_rows_dropped;
     }
     return;
   }
   if (type != "L" && type != "G" && type != "E") fail("unknown row type \"" + std::string(t[0]) + "\"");
-  rows_.emplace(name, static_cast<Int>(sense_.size()));
+  rows_.insert(name, static_cast<Int>(sense_.size()));
   sense_.push_back(type[0]);
   rhs_.push_back(0.0);
   has_rhs_.push_back(0);
@@ -340,7 +396,7 @@ void Parser::on_bound(const Tokens& t) {
   } else if (t.size() == 2) {
     at = 1;
   } else if (t.size() == 3) {
-    at = cols_.count(t[1]) != 0 ? 1 : 2;
+    at = cols_.contains(t[1]) ? 1 : 2;
   } else if (t.size() == 4) {
     at = 2;
   } else {
@@ -590,9 +646,14 @@ MpsModel read_mps_text(std::string_view text, const std::string& source, const M
 }
 
 MpsModel read_mps_file(const std::string& path, const MpsOptions& opt) {
-  std::ifstream in(path, std::ios::binary);
-  if (!in) throw FileError("cannot open \"" + path + "\"");
-  std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+  std::FILE* f = std::fopen(path.c_str(), "rb");
+  if (f == nullptr) throw FileError("cannot open \"" + path + "\"");
+  std::string text;
+  char buf[1 << 16];
+  for (std::size_t got; (got = std::fread(buf, 1, sizeof(buf), f)) > 0;) text.append(buf, got);
+  const bool failed = std::ferror(f) != 0;
+  std::fclose(f);
+  if (failed) throw FileError("cannot read \"" + path + "\"");
   return read_mps_text(text, path, opt);
 }
 

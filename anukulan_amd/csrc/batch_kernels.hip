@@ -137,10 +137,11 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Check reductions, per lane, in a fixed tree: each wave folds a contiguous
-// run of rows in index order, the four waves of a block fold in wave order
-// into per-block partials [block][value][lane], and bfold_kernel folds the
-// blocks in block order. The per-row terms are those of backend.cpp:143-202
+// Check reductions, per lane, in a fixed tree whose shape depends on the row
+// counts only: each wave folds a contiguous run of rows in index order, the
+// 16 waves of a block fold in wave order into per-block partials
+// [block][value][lane], and bfold_kernel folds every 16th block per wave, then
+// the waves in order. The per-row terms are those of backend.cpp:143-202
 // (convergence terms of the unscaled problem), backend.cpp:123-126 (the
 // movement) and pdhg.cpp:191-221 (the Farkas test).
 constexpr int kSideVals = 7;
@@ -155,135 +156,158 @@ constexpr unsigned kColMax = (1u << 2) | (1u << 5);
 constexpr int kColSlot[kSideVals] = {kSumDx2, kDualSq, kDualInf, kBoundTerm,
                                      kCDotX, kFarkasColBad, kFarkasSupport};
 
-__device__ __forceinline__ void block_store_side(double (&v)[kSideVals], unsigned max_mask,
-                                                 double* __restrict__ partials) {
-  __shared__ double lds[4][kSideVals][kLanes];
+// The per-row terms of the check: movement, the primal residual of the
+// unscaled problem, q'y and the row half of the Farkas test.
+__device__ __forceinline__ void row_terms(const BatchData& d, Int i, int lane,
+                                          const double* __restrict__ y,
+                                          const double* __restrict__ k_x,
+                                          const double* __restrict__ dy, int farkas,
+                                          double (&v)[kSideVals]) {
+  const size_t p = at(i, lane);
+  const double e = dy[p];
+  v[0] += e * e;
+  const double scale = d.row_scale[i];
+  const double row = scale != 0.0 ? k_x[p] / scale : k_x[p];
+  const double slack = row - d.orig_q[p];
+  const double viol = (i < d.num_equalities) ? slack : fmin(slack, 0.0);
+  v[1] += viol * viol;
+  v[2] = fmax(v[2], fabs(viol));
+  v[3] += d.orig_q[p] * (scale * y[p]);
+  if (farkas) {
+    if (i >= d.num_equalities && e < 0.0) v[4] = 1.0;
+    v[5] = fmax(v[5], fabs(e));
+    v[6] += d.q[p] * e;
+  }
+}
+
+// The per-column terms: movement, the dual residual, the bound term, c'x and
+// the column half of the Farkas test.
+__device__ __forceinline__ void col_terms(const BatchData& d, Int j, int lane,
+                                          const double* __restrict__ x,
+                                          const double* __restrict__ dx,
+                                          const double* __restrict__ kt_y,
+                                          const double* __restrict__ lambda, int farkas,
+                                          double (&v)[kSideVals]) {
+  const size_t p = at(j, lane);
+  const double e = dx[p];
+  v[0] += e * e;
+  const double scale = d.col_scale[j];
+  const double product = scale != 0.0 ? kt_y[p] / scale : kt_y[p];
+  const double c = d.orig_c[p];
+  const double lam = c - product;
+  const double lo = d.orig_lower[p], hi = d.orig_upper[p];
+  const bool has_lo = lo > -__builtin_inf(), has_hi = hi < __builtin_inf();
+  double left;
+  if (has_lo && has_hi) {
+    left = 0.0;
+  } else if (has_lo) {
+    left = fmin(lam, 0.0);
+  } else if (has_hi) {
+    left = fmax(lam, 0.0);
+  } else {
+    left = lam;
+  }
+  v[1] += left * left;
+  v[2] = fmax(v[2], fabs(left));
+  const double absorbed = lam - left;
+  if (absorbed > 0.0) {
+    v[3] += absorbed * lo;
+  } else if (absorbed < 0.0) {
+    v[3] += absorbed * hi;
+  }
+  v[4] += c * (scale * x[p]);
+  if (farkas) {
+    const double l = lambda[p];
+    if (l > 0.0) {
+      if (d.upper[p] >= __builtin_inf()) {
+        v[5] = 1.0;
+      } else {
+        v[6] += l * d.upper[p];
+      }
+    } else if (l < 0.0) {
+      if (d.lower[p] <= -__builtin_inf()) {
+        v[5] = 1.0;
+      } else {
+        v[6] += l * d.lower[p];
+      }
+    }
+  }
+}
+
+// One launch for both sides of the check: blocks [0, row_blocks) take rows of
+// K, the rest take columns. Each of a block's 16 waves folds a contiguous run
+// of `per_wave` rows in index order; the block folds its waves in wave order
+// into partials[block][value][lane].
+constexpr int kCheckWaves = 16;
+
+__global__ void __launch_bounds__(1024)
+    bcheck_kernel(BatchData d, const double* __restrict__ x, const double* __restrict__ y,
+                  const double* __restrict__ k_x, const double* __restrict__ dx,
+                  const double* __restrict__ dy, const double* __restrict__ kt_y,
+                  const double* __restrict__ lambda, int farkas, Int row_blocks, Int row_per_wave,
+                  Int col_per_wave, const LaneParams* __restrict__ params,
+                  double* __restrict__ row_partials, double* __restrict__ col_partials) {
+  __shared__ double lds[kCheckWaves][kSideVals][kLanes];
   const int lane = static_cast<int>(threadIdx.x & 63);
   const int wave = static_cast<int>(threadIdx.x >> 6);
+  const bool rows = static_cast<Int>(blockIdx.x) < row_blocks;
+  const Int block = rows ? static_cast<Int>(blockIdx.x) : static_cast<Int>(blockIdx.x) - row_blocks;
+  const Int per = rows ? row_per_wave : col_per_wave;
+  const Int total = rows ? d.m : d.n;
+  const Int r0 = (block * kCheckWaves + wave) * per;
+  const Int r1 = min(r0 + per, total);
+  double v[kSideVals] = {0, 0, 0, 0, 0, 0, 0};
+  if (params[lane].active != 0) {
+    if (rows) {
+      for (Int i = r0; i < r1; ++i) row_terms(d, i, lane, y, k_x, dy, farkas, v);
+    } else {
+      for (Int j = r0; j < r1; ++j) col_terms(d, j, lane, x, dx, kt_y, lambda, farkas, v);
+    }
+  }
 #pragma unroll
   for (int k = 0; k < kSideVals; ++k) lds[wave][k][lane] = v[k];
   __syncthreads();
-  if (wave == 0) {
-#pragma unroll
-    for (int k = 0; k < kSideVals; ++k) {
-      const bool mx = (max_mask >> k) & 1u;
-      double a = lds[0][k][lane];
-      for (int w = 1; w < 4; ++w) a = mx ? fmax(a, lds[w][k][lane]) : a + lds[w][k][lane];
-      partials[(static_cast<size_t>(blockIdx.x) * kSideVals + k) * kLanes + lane] = a;
-    }
+  if (wave < kSideVals) {
+    const int k = wave;  // wave k folds value k over the 16 waves
+    const bool mx = (((rows ? kRowMax : kColMax) >> k) & 1u) != 0;
+    double a = lds[0][k][lane];
+    for (int w = 1; w < kCheckWaves; ++w) a = mx ? fmax(a, lds[w][k][lane]) : a + lds[w][k][lane];
+    double* part = rows ? row_partials : col_partials;
+    part[(static_cast<size_t>(block) * kSideVals + k) * kLanes + lane] = a;
   }
 }
 
-__global__ void __launch_bounds__(256)
-    bcheck_rows_kernel(BatchData d, const double* __restrict__ y, const double* __restrict__ k_x,
-                       const double* __restrict__ dy, int farkas, Int rows_per_wave,
-                       const LaneParams* __restrict__ params, double* __restrict__ partials) {
-  const int lane = static_cast<int>(threadIdx.x & 63);
-  const Int r0 = (static_cast<Int>(blockIdx.x) * 4 + static_cast<Int>(threadIdx.x >> 6)) * rows_per_wave;
-  const Int r1 = min(r0 + rows_per_wave, d.m);
-  double v[kSideVals] = {0, 0, 0, 0, 0, 0, 0};
-  if (params[lane].active != 0) {
-    for (Int i = r0; i < r1; ++i) {
-      const size_t p = at(i, lane);
-      const double e = dy[p];
-      v[0] += e * e;
-      const double scale = d.row_scale[i];
-      const double row = scale != 0.0 ? k_x[p] / scale : k_x[p];
-      const double slack = row - d.orig_q[p];
-      const double viol = (i < d.num_equalities) ? slack : fmin(slack, 0.0);
-      v[1] += viol * viol;
-      v[2] = fmax(v[2], fabs(viol));
-      v[3] += d.orig_q[p] * (scale * y[p]);
-      if (farkas) {
-        if (i >= d.num_equalities && e < 0.0) v[4] = 1.0;
-        v[5] = fmax(v[5], fabs(e));
-        v[6] += d.q[p] * e;
-      }
-    }
-  }
-  block_store_side(v, kRowMax, partials);
-}
-
-__global__ void __launch_bounds__(256)
-    bcheck_cols_kernel(BatchData d, const double* __restrict__ x, const double* __restrict__ dx,
-                       const double* __restrict__ kt_y, const double* __restrict__ lambda,
-                       int farkas, Int rows_per_wave, const LaneParams* __restrict__ params,
-                       double* __restrict__ partials) {
-  const int lane = static_cast<int>(threadIdx.x & 63);
-  const Int j0 = (static_cast<Int>(blockIdx.x) * 4 + static_cast<Int>(threadIdx.x >> 6)) * rows_per_wave;
-  const Int j1 = min(j0 + rows_per_wave, d.n);
-  double v[kSideVals] = {0, 0, 0, 0, 0, 0, 0};
-  if (params[lane].active != 0) {
-    for (Int j = j0; j < j1; ++j) {
-      const size_t p = at(j, lane);
-      const double e = dx[p];
-      v[0] += e * e;
-      const double scale = d.col_scale[j];
-      const double product = scale != 0.0 ? kt_y[p] / scale : kt_y[p];
-      const double c = d.orig_c[p];
-      const double lam = c - product;
-      const double lo = d.orig_lower[p], hi = d.orig_upper[p];
-      const bool has_lo = lo > -__builtin_inf(), has_hi = hi < __builtin_inf();
-      double left;
-      if (has_lo && has_hi) {
-        left = 0.0;
-      } else if (has_lo) {
-        left = fmin(lam, 0.0);
-      } else if (has_hi) {
-        left = fmax(lam, 0.0);
-      } else {
-        left = lam;
-      }
-      v[1] += left * left;
-      v[2] = fmax(v[2], fabs(left));
-      const double absorbed = lam - left;
-      if (absorbed > 0.0) {
-        v[3] += absorbed * lo;
-      } else if (absorbed < 0.0) {
-        v[3] += absorbed * hi;
-      }
-      v[4] += c * (scale * x[p]);
-      if (farkas) {
-        const double l = lambda[p];
-        if (l > 0.0) {
-          if (d.upper[p] >= __builtin_inf()) {
-            v[5] = 1.0;
-          } else {
-            v[6] += l * d.upper[p];
-          }
-        } else if (l < 0.0) {
-          if (d.lower[p] <= -__builtin_inf()) {
-            v[5] = 1.0;
-          } else {
-            v[6] += l * d.lower[p];
-          }
-        }
-      }
-    }
-  }
-  block_store_side(v, kColMax, partials);
-}
-
-// One thread per (side value, lane): folds that value's per-block partials in
-// block order into out[lane * kLaneCheckStride + slot].
+// One block per (side, value): wave w folds blocks w, w + 16, ... in order,
+// then wave 0 folds the 16 waves in order into out[lane * kLaneCheckStride +
+// slot].
 __global__ void __launch_bounds__(1024)
     bfold_kernel(const double* __restrict__ row_partials, Int row_blocks,
                  const double* __restrict__ col_partials, Int col_blocks,
                  const LaneParams* __restrict__ params, double* __restrict__ out) {
+  __shared__ double lds[kCheckWaves][kLanes];
   const int lane = static_cast<int>(threadIdx.x & 63);
-  const int k = static_cast<int>(threadIdx.x >> 6);  // 0..15
-  if (k >= 2 * kSideVals || params[lane].active == 0) return;
-  const bool rows = k < kSideVals;
-  const int v = rows ? k : k - kSideVals;
+  const int wave = static_cast<int>(threadIdx.x >> 6);
+  const int side_value = static_cast<int>(blockIdx.x);  // 0..2*kSideVals-1
+  const bool rows = side_value < kSideVals;
+  const int v = rows ? side_value : side_value - kSideVals;
   const double* part = rows ? row_partials : col_partials;
   const Int blocks = rows ? row_blocks : col_blocks;
-  const bool mx = ((rows ? kRowMax : kColMax) >> v) & 1u;
+  const bool mx = (((rows ? kRowMax : kColMax) >> v) & 1u) != 0;
   double a = 0.0;
-  for (Int b = 0; b < blocks; ++b) {
+  bool first = true;
+#pragma unroll 8
+  for (Int b = wave; b < blocks; b += kCheckWaves) {
     const double t = part[(static_cast<size_t>(b) * kSideVals + v) * kLanes + lane];
-    a = mx ? fmax(a, t) : a + t;
+    a = first ? t : (mx ? fmax(a, t) : a + t);
+    first = false;
   }
-  out[static_cast<size_t>(lane) * kLaneCheckStride + (rows ? kRowSlot[v] : kColSlot[v])] = a;
+  lds[wave][lane] = a;
+  __syncthreads();
+  if (wave == 0 && params[lane].active != 0) {
+    double r = lds[0][lane];
+    for (int w = 1; w < kCheckWaves; ++w) r = mx ? fmax(r, lds[w][lane]) : r + lds[w][lane];
+    out[static_cast<size_t>(lane) * kLaneCheckStride + (rows ? kRowSlot[v] : kColSlot[v])] = r;
+  }
 }
 
 // Distances moved since a restart point, per lane, in the same fixed tree as
@@ -395,7 +419,8 @@ void launch_batch_spmm(hipStream_t s, const DevCsr& a, const double* v1, double*
 }
 
 namespace {
-// Rows per wave so a side needs at most kCheckBlocks blocks of four waves.
+// Rows per wave so a side needs at most kCheckBlocks blocks of four waves
+// (the restart distances).
 Int check_rows_per_wave(Int rows) {
   const Int waves = kCheckBlocks * 4;
   return rows <= waves ? 1 : (rows + waves - 1) / waves;
@@ -404,10 +429,20 @@ Int check_blocks(Int rows) {
   const Int per_block = 4 * check_rows_per_wave(rows);
   return rows <= 0 ? 0 : (rows + per_block - 1) / per_block;
 }
+// The check's shape: one row per wave while a side fits kCheckSideBlocks
+// blocks of 16 waves, more rows per wave beyond that.
+Int side_rows_per_wave(Int rows) {
+  const Int waves = static_cast<Int>(kCheckSideBlocks) * kCheckWaves;
+  return rows <= waves ? 1 : (rows + waves - 1) / waves;
+}
+Int side_blocks(Int rows) {
+  const Int per_block = kCheckWaves * side_rows_per_wave(rows);
+  return rows <= 0 ? 0 : (rows + per_block - 1) / per_block;
+}
 }  // namespace
 
 size_t batch_check_scratch_doubles() {
-  return static_cast<size_t>(2) * kCheckBlocks * kSideVals * kLanes;
+  return static_cast<size_t>(2) * kCheckSideBlocks * kSideVals * kLanes;
 }
 
 void launch_batch_check(hipStream_t s, const BatchData& d, const double* x, const double* y,
@@ -415,21 +450,16 @@ void launch_batch_check(hipStream_t s, const BatchData& d, const double* x, cons
                         const double* kt_y, const double* lambda, bool farkas,
                         const LaneParams* params, double* scratch, double* out) {
   double* row_part = scratch;
-  double* col_part = scratch + static_cast<size_t>(kCheckBlocks) * kSideVals * kLanes;
-  const Int rb = check_blocks(d.m), cb = check_blocks(d.n);
-  if (rb > 0) {
-    hipLaunchKernelGGL(bcheck_rows_kernel, dim3(static_cast<unsigned>(rb)), dim3(256), 0, s, d, y,
-                       k_x, dy, farkas ? 1 : 0, check_rows_per_wave(d.m), params, row_part);
-    check_launch("batch_check_rows");
+  double* col_part = scratch + static_cast<size_t>(kCheckSideBlocks) * kSideVals * kLanes;
+  const Int rb = side_blocks(d.m), cb = side_blocks(d.n);
+  if (rb + cb > 0) {
+    hipLaunchKernelGGL(bcheck_kernel, dim3(static_cast<unsigned>(rb + cb)), dim3(1024), 0, s, d, x,
+                       y, k_x, dx, dy, kt_y, lambda, farkas ? 1 : 0, rb, side_rows_per_wave(d.m),
+                       side_rows_per_wave(d.n), params, row_part, col_part);
+    check_launch("batch_check");
   }
-  if (cb > 0) {
-    hipLaunchKernelGGL(bcheck_cols_kernel, dim3(static_cast<unsigned>(cb)), dim3(256), 0, s, d, x,
-                       dx, kt_y, lambda, farkas ? 1 : 0, check_rows_per_wave(d.n), params,
-                       col_part);
-    check_launch("batch_check_cols");
-  }
-  hipLaunchKernelGGL(bfold_kernel, dim3(1), dim3(1024), 0, s, row_part, rb, col_part, cb, params,
-                     out);
+  hipLaunchKernelGGL(bfold_kernel, dim3(2 * kSideVals), dim3(1024), 0, s, row_part, rb, col_part,
+                     cb, params, out);
   check_launch("batch_check_fold");
 }
 

@@ -68,7 +68,8 @@ void launch_batch_spmm(hipStream_t s, const DevCsr& a, const double* v1, double*
 // unscaled problem, and both halves of the Farkas test. Deterministic: a
 // fixed tree whose shape depends on the row counts only. `scratch` holds
 // batch_check_scratch_doubles() doubles.
-inline constexpr int kCheckBlocks = 64;
+inline constexpr int kCheckBlocks = 64;       // restart distances: blocks per side
+inline constexpr int kCheckSideBlocks = 1024;  // check terms: at most this many blocks per side
 size_t batch_check_scratch_doubles();
 void launch_batch_check(hipStream_t s, const BatchData& d, const double* x, const double* y,
                         const double* k_x, const double* dx, const double* dy,

@@ -85,6 +85,14 @@ class Engine {
   void run_restarts(Mask mask);
   void run_polish(const std::vector<int>& who, Int budget, std::vector<LaneOutcome>& out,
                   std::vector<bool>* stopped, Int iteration);
+  void compact();
+  // The device slots of a set of lanes.
+  Mask slots(Mask lanes) const {
+    Mask m = 0;
+    for (int l = 0; l < count_; ++l)
+      if (((lanes >> l) & 1ull) && slot_of_[l] >= 0) m |= 1ull << slot_of_[l];
+    return m;
+  }
 
   HipBackend& be_;
   const Csr& k_;
@@ -104,8 +112,16 @@ class Engine {
   DevCsr dk_, dkt_;
   std::vector<Control> ctl_;
   std::vector<anuk_pdhg_options> lane_opt_;  // each lane's options (its termination test)
-  Mask active_ = 0;
+  Mask active_ = 0;  // by lane
   int cur_ = 0;
+  // Lane l's state sits in device slot slot_of_[l] (-1 once retired and
+  // moved out); lane_at_[s] is the lane in slot s (-1: none). Live lanes are
+  // kept in the lowest slots (compact).
+  int slot_of_[kLanes];
+  int lane_at_[kLanes];
+  // One past the highest live slot, rounded up to a whole 16-slot line: the
+  // step kernels' lane bound, so at most four graph variants per parity.
+  Int span_ = kLanes;
 
   double* slab_ = nullptr;
   double *X_[2], *Y_[2], *KX_[2];
@@ -130,6 +146,7 @@ Engine::~Engine() {
 }
 
 constexpr size_t kPinnedDoubles = 64 * 8 + 64 * kLaneCheckStride + 128;
+static_assert(sizeof(LaneTable) <= 64 * 8 * sizeof(double), "the lane table fits its slab entry");
 
 void Engine::allocate() {
   const size_t N = static_cast<size_t>(n_) * kLanes, M = static_cast<size_t>(m_) * kLanes;
@@ -240,10 +257,14 @@ void Engine::upload_lanes() {
 }
 
 void Engine::upload_params() {
-  auto* P = reinterpret_cast<LaneParams*>(pinned_);
-  for (int l = 0; l < kLanes; ++l) {
+  auto* T = reinterpret_cast<LaneTable*>(pinned_);
+  LaneParams* P = T->lane;
+  T->span = 0;
+  for (int slot = 0; slot < kLanes; ++slot) {
     LaneParams lp{};
-    if (l < count_ && ((active_ >> l) & 1ull)) {
+    const int l = lane_at_[slot];
+    if (l >= 0 && ((active_ >> l) & 1ull)) {
+      T->span = slot + 1;
       const IterParams ip = ctl_[static_cast<size_t>(l)].period_params(neq_);
       lp.tau = ip.tau;
       lp.sigma = ip.sigma;
@@ -253,10 +274,10 @@ void Engine::upload_params() {
       lp.k_at_start = ip.k_at_start;
       lp.active = 1;
     }
-    P[l] = lp;
+    P[slot] = lp;
   }
-  hip_check(hipMemcpyAsync(d_params_, P, kLanes * sizeof(LaneParams), hipMemcpyHostToDevice,
-                           be_.stream()),
+  span_ = std::max<Int>(16, (T->span + 15) / 16 * 16);
+  hip_check(hipMemcpyAsync(d_params_, T, sizeof(LaneTable), hipMemcpyHostToDevice, be_.stream()),
             "upload lane params");
 }
 
@@ -275,12 +296,12 @@ void Engine::enqueue_period(int steps, bool has_check, int parity) {
     const LaunchEvents pe =
         be_.launch_events(diag ? "batch_primal_step_check" : "batch_primal_step",
                           12.0 * nnz + 4.0 * (n + 1) + 8.0 * lanes * (m + (diag ? 8.0 : 7.0) * n));
-    launch_batch_primal_step(s, dkt_, Y_[p], data_, d_params_, step, pb, diag, &pe);
+    launch_batch_primal_step(s, dkt_, Y_[p], data_, d_params_, step, span_, pb, diag, &pe);
     DualBufs db{Y_[p], KX_[p], AY_, AKX_, Y_[1 - p], KX_[1 - p], DY_, KDX_, AY_, AKX_};
     const LaunchEvents de =
         be_.launch_events(diag ? "batch_dual_step_check" : "batch_dual_step",
                           12.0 * nnz + 4.0 * (m + 1) + 8.0 * lanes * (n + (diag ? 9.0 : 7.0) * m));
-    launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, step, db, diag, &de);
+    launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, step, span_, db, diag, &de);
     p = 1 - p;
   }
   if (has_check) {
@@ -294,7 +315,9 @@ void Engine::enqueue_period(int steps, bool has_check, int parity) {
 }
 
 hipGraphExec_t Engine::graph_for(int parity) {
-  auto it = graphs_.find(parity);
+  // One graph per parity and span (the span is a kernel argument).
+  const int key = parity * 8 + span_ / 16;
+  auto it = graphs_.find(key);
   if (it != graphs_.end()) return it->second.second;
   hipStream_t s = be_.stream();
   hipGraph_t g = nullptr;
@@ -309,7 +332,7 @@ hipGraphExec_t Engine::graph_for(int parity) {
   }
   hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
   hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "hipGraphInstantiate");
-  graphs_[parity] = {g, e};
+  graphs_[key] = {g, e};
   return e;
 }
 
@@ -323,20 +346,22 @@ void Engine::fetch(const double* dev, double* host, size_t doubles) {
 void Engine::retire_extract(Mask mask, std::vector<LaneOutcome>& out) {
   if (mask == 0) return;
   const int cnt = __builtin_popcountll(mask);
+  const Mask dev = slots(mask);
   std::vector<double> xs(static_cast<size_t>(cnt) * uz(n_)), ys(static_cast<size_t>(cnt) * uz(m_));
-  launch_batch_extract(be_.stream(), n_, X_[cur_], mask, EXT_);
+  launch_batch_extract(be_.stream(), n_, X_[cur_], dev, EXT_);
   be_.download(EXT_, xs.data(), cnt * n_);
-  launch_batch_extract(be_.stream(), m_, Y_[cur_], mask, EXT_);
+  launch_batch_extract(be_.stream(), m_, Y_[cur_], dev, EXT_);
   be_.download(EXT_, ys.data(), cnt * m_);
-  int slot = 0;
-  for (int l = 0; l < count_; ++l) {
-    if (!((mask >> l) & 1ull)) continue;
-    LaneOutcome& o = out[static_cast<size_t>(l)];
+  // The extract is in slot order.
+  int k = 0;
+  for (int slot = 0; slot < kLanes; ++slot) {
+    if (!((dev >> slot) & 1ull)) continue;
+    LaneOutcome& o = out[static_cast<size_t>(lane_at_[slot])];
     if (!o.polished) {
-      o.x.assign(xs.begin() + static_cast<long>(slot) * n_, xs.begin() + static_cast<long>(slot + 1) * n_);
-      o.y.assign(ys.begin() + static_cast<long>(slot) * m_, ys.begin() + static_cast<long>(slot + 1) * m_);
+      o.x.assign(xs.begin() + static_cast<long>(k) * n_, xs.begin() + static_cast<long>(k + 1) * n_);
+      o.y.assign(ys.begin() + static_cast<long>(k) * m_, ys.begin() + static_cast<long>(k + 1) * m_);
     }
-    ++slot;
+    ++k;
   }
 }
 
@@ -345,15 +370,17 @@ void Engine::retire_extract(Mask mask, std::vector<LaneOutcome>& out) {
 // lane-serially, the restart points advanced.
 void Engine::run_restarts(Mask mask) {
   if (mask == 0) return;
-  launch_batch_spmm(be_.stream(), dk_, X_[cur_], KX_[cur_], nullptr, nullptr, nullptr, mask);
+  const Mask dev = slots(mask);
+  launch_batch_spmm(be_.stream(), dk_, X_[cur_], KX_[cur_], nullptr, nullptr, nullptr, dev);
   const bool previous = !opt_.pid_primal_weight;
   launch_batch_restart(be_.stream(), n_, m_, X_[cur_], Y_[cur_], RX_, RY_, PRX_, PRY_, previous,
-                       true, mask, CHECKP_, RESTART_);
+                       true, dev, CHECKP_, RESTART_);
   double* h = pinned_ + 64 * 8 + 64 * kLaneCheckStride;
   fetch(RESTART_, h, 128);
   for (int l = 0; l < count_; ++l) {
     if (!((mask >> l) & 1ull)) continue;
-    ctl_[static_cast<size_t>(l)].restart(std::sqrt(h[2 * l]), std::sqrt(h[2 * l + 1]));
+    const int slot = slot_of_[l];
+    ctl_[static_cast<size_t>(l)].restart(std::sqrt(h[2 * slot]), std::sqrt(h[2 * slot + 1]));
   }
 }
 
@@ -471,19 +498,23 @@ std::vector<Polished> polish_lanes(HipBackend& be, const Csr& k, const Csr& kt, 
 void Engine::run_polish(const std::vector<int>& who, Int budget, std::vector<LaneOutcome>& out,
                         std::vector<bool>* stopped, Int iteration) {
   (void)iteration;
+  // The extract comes out in slot order, so the lanes are taken in that order.
+  std::vector<int> order(who);
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return slot_of_[a] < slot_of_[b]; });
   Mask mask = 0;
-  for (int l : who) mask |= 1ull << l;
-  const int cnt = static_cast<int>(who.size());
+  for (int l : order) mask |= 1ull << l;
+  const Mask dev = slots(mask);
+  const int cnt = static_cast<int>(order.size());
   std::vector<double> xs(static_cast<size_t>(cnt) * uz(n_)), ys(static_cast<size_t>(cnt) * uz(m_));
-  launch_batch_extract(be_.stream(), n_, X_[cur_], mask, EXT_);
+  launch_batch_extract(be_.stream(), n_, X_[cur_], dev, EXT_);
   be_.download(EXT_, xs.data(), cnt * n_);
-  launch_batch_extract(be_.stream(), m_, Y_[cur_], mask, EXT_);
+  launch_batch_extract(be_.stream(), m_, Y_[cur_], dev, EXT_);
   be_.download(EXT_, ys.data(), cnt * m_);
   std::vector<const Lane*> src;
   std::vector<std::vector<double>> sx, sy;
   std::vector<double> omega;
   for (int s = 0; s < cnt; ++s) {
-    const int l = who[static_cast<size_t>(s)];
+    const int l = order[static_cast<size_t>(s)];
     src.push_back(&lanes_[static_cast<size_t>(l)]);
     sx.emplace_back(xs.begin() + static_cast<long>(s) * n_, xs.begin() + static_cast<long>(s + 1) * n_);
     sy.emplace_back(ys.begin() + static_cast<long>(s) * m_, ys.begin() + static_cast<long>(s + 1) * m_);
@@ -493,7 +524,7 @@ void Engine::run_polish(const std::vector<int>& who, Int budget, std::vector<Lan
                                            ctl_[static_cast<size_t>(who[0])].eta, budget,
                                            *polish_->scaling, start_);
   for (int s = 0; s < cnt; ++s) {
-    const int l = who[static_cast<size_t>(s)];
+    const int l = order[static_cast<size_t>(s)];
     LaneOutcome& o = out[static_cast<size_t>(l)];
     std::vector<double>& tx = pol[static_cast<size_t>(s)].x;
     std::vector<double>& ty = pol[static_cast<size_t>(s)].y;
@@ -512,6 +543,45 @@ void Engine::run_polish(const std::vector<int>& who, Int budget, std::vector<Lan
       (*stopped)[static_cast<size_t>(l)] = true;
     }
   }
+}
+
+// Moves the live lanes into the lowest slots, in lane order, when that frees
+// a whole 16-slot (128-byte) line of every row: the step kernels then touch
+// ceil(live / 16) lines per row instead of the span of the scattered slots.
+// Only the state that outlives a period moves; a lane's arithmetic does not
+// depend on its slot, so the results are unchanged.
+void Engine::compact() {
+  const int live = __builtin_popcountll(active_);
+  if (live == 0) return;
+  int top = -1;
+  for (int l = 0; l < count_; ++l)
+    if ((active_ >> l) & 1ull) top = std::max(top, slot_of_[l]);
+  if ((live + 15) / 16 >= (top + 1 + 15) / 16) return;
+  SlotMap map{};
+  map.count = live;
+  int next = 0;
+  int lane_at[kLanes];
+  std::fill(lane_at, lane_at + kLanes, -1);
+  for (int l = 0; l < count_; ++l) {
+    if (!((active_ >> l) & 1ull)) continue;
+    map.from[next] = static_cast<std::int8_t>(slot_of_[l]);
+    lane_at[next] = l;
+    ++next;
+  }
+  hipStream_t st = be_.stream();
+  for (double* v : {X_[cur_], AX_, RX_, PRX_, const_cast<double*>(data_.c),
+                    const_cast<double*>(data_.lower), const_cast<double*>(data_.upper),
+                    const_cast<double*>(data_.orig_c), const_cast<double*>(data_.orig_lower),
+                    const_cast<double*>(data_.orig_upper)})
+    launch_batch_permute(st, n_, v, EXT_, map);
+  for (double* v : {Y_[cur_], KX_[cur_], AY_, AKX_, RY_, PRY_, const_cast<double*>(data_.q),
+                    const_cast<double*>(data_.orig_q)})
+    launch_batch_permute(st, m_, v, EXT_, map);
+  // Retired lanes keep no slot; their outcomes are already extracted.
+  for (int l = 0; l < count_; ++l) slot_of_[l] = -1;
+  std::copy(lane_at, lane_at + kLanes, lane_at_);
+  for (int slot = 0; slot < kLanes; ++slot)
+    if (lane_at_[slot] >= 0) slot_of_[lane_at_[slot]] = slot;
 }
 
 std::vector<LaneOutcome> Engine::run() {
@@ -533,6 +603,10 @@ std::vector<LaneOutcome> Engine::run() {
     Control& c = ctl_[static_cast<size_t>(l)];
     c.init(o, norm_, host_two_norm(ln.c), host_two_norm(ln.q));
     if (ln.initial_omega > 0.0) c.omega = ln.initial_omega;
+  }
+  for (int l = 0; l < kLanes; ++l) {
+    slot_of_[l] = l;
+    lane_at_[l] = l < count_ ? l : -1;
   }
   upload_lanes();
   active_ = (count_ == 64) ? ~0ull : ((1ull << count_) - 1ull);
@@ -585,7 +659,7 @@ std::vector<LaneOutcome> Engine::run() {
       if (!((active_ >> l) & 1ull)) continue;
       Control& c = ctl_[static_cast<size_t>(l)];
       const Lane& ln = lanes_[static_cast<size_t>(l)];
-      const double* h = h_check + static_cast<size_t>(l) * kLaneCheckStride;
+      const double* h = h_check + static_cast<size_t>(slot_of_[l]) * kLaneCheckStride;
       c.movement = c.omega * h[kSumDx2] + h[kSumDy2] / c.omega;
       bool farkas = false;
       if (opt_.detect_infeasibility) {
@@ -642,6 +716,7 @@ std::vector<LaneOutcome> Engine::run() {
     retire_extract(retiring, out);
     active_ &= ~retiring;
     run_restarts(restart & active_);
+    compact();
     lap(&Phases::checks);
   }
   // Lanes still running stopped at the iteration limit.

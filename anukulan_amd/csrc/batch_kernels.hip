@@ -18,6 +18,11 @@ using lanes::kWavesPerBlock;
 using lanes::row_sum;
 using lanes::wave_row;
 
+// The slot span stored after the 64 LaneParams (LaneTable::span).
+__device__ __forceinline__ Int span_of(const LaneParams* params) {
+  return reinterpret_cast<const LaneTable*>(params)->span;
+}
+
 struct LaneScalars {
   double tau, sigma, gamma, w;
   bool blend, open_epoch;
@@ -49,14 +54,15 @@ __device__ __forceinline__ LaneScalars lane_scalars(const LaneParams& p, int ste
 // The step kernels issue every load that does not depend on the row sum
 // (the lane's scalars, the epilogue operands) before the gathers, so the
 // wave waits for one round of memory latency rather than three. A retired
-// lane loads and computes with the rest of its wave and stores nothing.
+// lane issues no vector load and stores nothing: with the live lanes kept in
+// the lowest slots, a row touches only the cache lines that hold them.
 template <bool kDiag>
 __global__ void __launch_bounds__(256)
     bprimal_kernel(DevCsr kt, const double* __restrict__ y, BatchData d, PrimalBufs b,
-                   const LaneParams* __restrict__ params, int step) {
+                   const LaneParams* __restrict__ params, int step, Int span) {
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int j = wave_row();
-  if (j >= kt.rows) return;
+  if (j >= kt.rows || lane >= span) return;
   const size_t o = at(j, lane);
   const LaneParams P = params[lane];
   const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
@@ -79,10 +85,10 @@ __global__ void __launch_bounds__(256)
 template <bool kDiag>
 __global__ void __launch_bounds__(256)
     bdual_kernel(DevCsr k, const double* __restrict__ x_bar, BatchData d, DualBufs b,
-                 const LaneParams* __restrict__ params, int step) {
+                 const LaneParams* __restrict__ params, int step, Int span) {
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int i = wave_row();
-  if (i >= k.rows) return;
+  if (i >= k.rows || lane >= span) return;
   const size_t o = at(i, lane);
   const LaneParams P = params[lane];
   const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
@@ -127,14 +133,14 @@ __global__ void __launch_bounds__(256)
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int r = wave_row();
   if (r >= a.rows) return;  // uniform across the wave
-  // Every lane takes part in the row sums; only the active lanes store.
-  const bool on = lane_on(params, mask, lane);
+  // The row sums are lane-serial (no cross-lane operation), so lanes beyond
+  // the span sit out and the rest store only if live.
+  if (params != nullptr && lane >= span_of(params)) return;
   const double s1 = row_sum(a, r, v1, lane);
-  if (on) out1[at(r, lane)] = s1;
-  if (v2 != nullptr) {
-    const double s2 = row_sum(a, r, v2, lane);
-    if (on) out2[at(r, lane)] = s2;
-  }
+  const double s2 = v2 != nullptr ? row_sum(a, r, v2, lane) : 0.0;
+  if (!lane_on(params, mask, lane)) return;
+  out1[at(r, lane)] = s1;
+  if (v2 != nullptr) out2[at(r, lane)] = s2;
 }
 
 // Check reductions, per lane, in a fixed tree whose shape depends on the row
@@ -310,6 +316,18 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
+// dst[row][s] = src[row][perm[s]] for the first `count` slots; the other
+// slots of dst are left as they are.
+__global__ void __launch_bounds__(256)
+    bpermute_kernel(Int rows, const double* __restrict__ src, double* __restrict__ dst,
+                    SlotMap map) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t r = i / kLanes;
+  const int s = static_cast<int>(i % kLanes);
+  if (r >= static_cast<size_t>(rows) || s >= map.count) return;
+  dst[r * kLanes + static_cast<size_t>(s)] = src[r * kLanes + static_cast<size_t>(map.from[s])];
+}
+
 // Distances moved since a restart point, per lane, in the same fixed tree as
 // the check: per-wave runs of rows, waves in order, then blocks in order.
 // The restart points advance in the same pass when `advance` is set.
@@ -393,22 +411,26 @@ void launch_rows(K kernel, Int rows, hipStream_t s, const LaunchEvents* ev, cons
 }  // namespace
 
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
-                              const BatchData& d, const LaneParams* params, int step,
+                              const BatchData& d, const LaneParams* params, int step, Int span,
                               const PrimalBufs& b, bool diag, const LaunchEvents* ev) {
   if (diag) {
-    launch_rows(bprimal_kernel<true>, kt.rows, s, ev, "batch_primal_step", kt, y, d, b, params, step);
+    launch_rows(bprimal_kernel<true>, kt.rows, s, ev, "batch_primal_step", kt, y, d, b, params, step,
+                span);
   } else {
-    launch_rows(bprimal_kernel<false>, kt.rows, s, ev, "batch_primal_step", kt, y, d, b, params, step);
+    launch_rows(bprimal_kernel<false>, kt.rows, s, ev, "batch_primal_step", kt, y, d, b, params, step,
+                span);
   }
 }
 
 void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
-                            const BatchData& d, const LaneParams* params, int step,
+                            const BatchData& d, const LaneParams* params, int step, Int span,
                             const DualBufs& b, bool diag, const LaunchEvents* ev) {
   if (diag) {
-    launch_rows(bdual_kernel<true>, k.rows, s, ev, "batch_dual_step", k, x_bar, d, b, params, step);
+    launch_rows(bdual_kernel<true>, k.rows, s, ev, "batch_dual_step", k, x_bar, d, b, params, step,
+                span);
   } else {
-    launch_rows(bdual_kernel<false>, k.rows, s, ev, "batch_dual_step", k, x_bar, d, b, params, step);
+    launch_rows(bdual_kernel<false>, k.rows, s, ev, "batch_dual_step", k, x_bar, d, b, params, step,
+                span);
   }
 }
 
@@ -483,6 +505,16 @@ void launch_batch_restart(hipStream_t s, Int n, Int m, const double* x, const do
   }
   hipLaunchKernelGGL(brestart_fold_kernel, dim3(1), dim3(128), 0, s, px, bx, py, by, mask, out);
   check_launch("batch_restart_fold");
+}
+
+void launch_batch_permute(hipStream_t s, Int rows, double* v, double* tmp, const SlotMap& map) {
+  if (rows <= 0 || map.count <= 0) return;
+  const size_t n = static_cast<size_t>(rows) * kLanes;
+  const hipError_t e = hipMemcpyAsync(tmp, v, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) throw std::runtime_error(std::string("batch permute copy: ") + hipGetErrorString(e));
+  const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+  hipLaunchKernelGGL(bpermute_kernel, dim3(blocks), dim3(256), 0, s, rows, tmp, v, map);
+  check_launch("batch_permute");
 }
 
 void launch_batch_extract(hipStream_t s, Int rows, const double* v, unsigned long long mask,

@@ -32,6 +32,15 @@ struct LaneParams {
   Int active;  // 0: retired or padding, the lane does nothing
 };
 
+// The lane table the kernels read: one LaneParams per slot, then the slot
+// span (one past the highest live slot). Loads are issued by the lanes below
+// the span, whose bound is wave-uniform and arrives with the row's first
+// scalar loads; stores are left to the live lanes.
+struct LaneTable {
+  LaneParams lane[64];
+  Int span;
+};
+
 // Problem data, per LP row-interleaved except the shared scaling diagonals.
 struct BatchData {
   Int n = 0, m = 0, num_equalities = 0;
@@ -50,11 +59,13 @@ struct BatchData {
 // Slots of the per-lane check vector; the same quantities as CheckSlot.
 inline constexpr int kLaneCheckStride = 16;
 
+// `span`: the lanes below it load and compute (the live ones among them
+// store); a kernel argument, so the bound costs no dependent load.
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
-                              const BatchData& d, const LaneParams* params, int step,
+                              const BatchData& d, const LaneParams* params, int step, Int span,
                               const PrimalBufs& b, bool diag, const LaunchEvents* ev = nullptr);
 void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
-                            const BatchData& d, const LaneParams* params, int step,
+                            const BatchData& d, const LaneParams* params, int step, Int span,
                             const DualBufs& b, bool diag, const LaunchEvents* ev = nullptr);
 // out1 = A v1 (and out2 = A v2 when v2 is non-null) for the active lanes:
 // those whose LaneParams say so when `params` is given (read at run time, so
@@ -84,6 +95,15 @@ void launch_batch_check(hipStream_t s, const BatchData& d, const double* x, cons
 void launch_batch_restart(hipStream_t s, Int n, Int m, const double* x, const double* y,
                           double* rx, double* ry, double* prx, double* pry, bool use_previous,
                           bool advance, unsigned long long mask, double* scratch, double* out);
+// Moves lanes between slots: slot s of every row receives what slot
+// from[s] held, for s < count (passed by value to the kernel).
+struct SlotMap {
+  std::int8_t from[64];
+  int count;
+};
+// Permutes the slots of v (rows x 64, row-interleaved) by `map`, staging
+// through tmp (at least rows x 64 doubles).
+void launch_batch_permute(hipStream_t s, Int rows, double* v, double* tmp, const SlotMap& map);
 // Copies the columns of the masked lanes, in lane order, into dense
 // [count][rows] arrays.
 void launch_batch_extract(hipStream_t s, Int rows, const double* v, unsigned long long mask,

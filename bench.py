@@ -104,6 +104,29 @@ def cpu_baseline_parallel(base, cases, tol, threads):
                       f"thread on {threads} threads: {iters} iterations in {wall:.1f} s wall"}
 
 
+def simplex_optima(periods, count):
+    """The reference simplex's FP64 optimum of each case of the stack, as
+    committed in tests/golden/pdhg_reference.json (stack24), or None."""
+    path = os.path.join(ROOT, "tests", "golden", "pdhg_reference.json")
+    if not os.path.exists(path):
+        return None
+    import json as _json
+    st = _json.load(open(path)).get(f"stack{periods}")
+    if st is None or len(st["simplex_objective"]) < count:
+        return None
+    return st["simplex_objective"][:count]
+
+
+def error_vs_optimum(results, optima):
+    """Relative objective error of each result against its FP64 optimum:
+    |obj - opt| / max(1, |opt|); max and median over the cases."""
+    if optima is None:
+        return None
+    errs = sorted(abs(r.objective - o) / max(1.0, abs(o)) for r, o in zip(results, optima))
+    return {"max_rel": errs[-1], "median_rel": errs[len(errs) // 2],
+            "optimum": "reference simplex (tests/golden/pdhg_reference.json stack24)"}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (profiles/pmc_traffic.txt), or None."""
@@ -289,6 +312,7 @@ def main():
     value = len(lps) * args.steps / elapsed
     solved = sum(1 for r in results if r.status == "optimal")
     lane_iters = sum(r.iterations for r in results)
+    optima = simplex_optima(args.periods, len(lps))
 
     plain = None
     if not args.no_plain:
@@ -300,7 +324,8 @@ def main():
         plain = {"value": round(len(lps) / pt, 3), "unit": "cases/s",
                  "seconds": round(pt, 3), "iterations": sum(r.iterations for r in pres),
                  "cases_optimal": sum(1 for r in pres if r.status == "optimal"),
-                 "max_rel_objective_diff_vs_batched": worst}
+                 "max_rel_objective_diff_vs_batched": worst,
+                 "error_vs_simplex": error_vs_optimum(pres, optima)}
 
     roofline, kernels = None, {}
     if not args.no_profile:
@@ -340,6 +365,7 @@ def main():
                    "rows": m, "cols": n, "nnz": nnz, "cases": len(lps),
                    "cases_optimal": solved, "lane_iterations_per_step": lane_iters,
                    "parallelism": "single GPU"},
+        "error_vs_simplex": error_vs_optimum(results, optima),
         "roofline": roofline, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_par,
         "plain": plain, "c1": c1, "c3": c3,
         "batch_phases_s": {k: round(v, 4) for k, v in results[0].phases.items()},

@@ -6,7 +6,7 @@ the seeded refinery plans, and records what parity is judged against:
 status, iterations, restarts, polish counts, objective, ||K|| and the
 simplex optimum. Run in the container that has the reference sources:
 
-    make -C oracle && python tests/golden/make_fixtures.py [--only random]
+    make -C oracle && python tests/golden/make_fixtures.py [--only random|stack24]
 """
 import json
 import os
@@ -47,18 +47,34 @@ def random_section():
     return out
 
 
+def stack_section(periods=24, threads=8):
+    """The FP64 optimum of every case of the case stack by the reference's
+    simplex (the error every batched and plain result is stated against)."""
+    from concurrent.futures import ThreadPoolExecutor
+    base, cases = G.case_stack(periods)
+    lps = [_oracle.RefLp.from_natural(G.case_model(base, c)) for c in cases]
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        sx = list(pool.map(lambda lp: lp.simplex(), lps))
+    return {"periods": periods, "cases": len(cases),
+            "simplex_status": [int(r["status"]) for r in sx],
+            "simplex_objective": [r["objective"] for r in sx]}
+
+
+SECTIONS = {"random": random_section, "stack24": stack_section}
+
+
 def main():
     if _oracle.ref is None:
         sys.exit("oracle/_ref/libanukulan_ref.so is not built")
     path = os.path.join(HERE, "pdhg_reference.json")
-    if sys.argv[1:] == ["--only", "random"]:
+    if len(sys.argv) == 3 and sys.argv[1] == "--only" and sys.argv[2] in SECTIONS:
         out = json.load(open(path))
-        out["random"] = random_section()
+        out[sys.argv[2]] = SECTIONS[sys.argv[2]]()
         with open(path, "w") as f:
             json.dump(out, f, indent=1, sort_keys=True)
         print("updated", path)
         return
-    out = {"known": {}, "refinery": {}, "random": random_section()}
+    out = {"known": {}, "refinery": {}, "random": random_section(), "stack24": stack_section()}
     for name in KNOWN:
         rlp = _oracle.RefLp.from_natural(natural(name))
         out["known"][name] = {"1e-8": record(rlp, 1e-8), "1e-4": record(rlp, 1e-4),

@@ -71,6 +71,25 @@ def test_full_24_period_stack_against_the_fixture(ctx):
     assert relerr(got[0].objective, fx["objective"]) <= 1e-8
 
 
+def test_full_stack_against_the_simplex_optima(ctx):
+    # Every case of the 61-case stack, batched to rel KKT 1e-8, against the
+    # FP64 optimum of the reference's simplex on that case
+    # (tests/golden/pdhg_reference.json stack24, make_fixtures.py).
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "pdhg_reference.json")))["stack24"]
+    base, cases, models = stack(24)
+    assert len(models) == fx["cases"] and set(fx["simplex_status"]) == {0}
+    got = solve_pdhg_batch(ctx, [m.to_lp() for m in models], tolerance=1e-8,
+                           max_iterations=2_000_000, time_limit_seconds=1e9)
+    assert all(g.status == "optimal" for g in got)
+    errs = [relerr(g.objective, o) for g, o in zip(got, fx["simplex_objective"])]
+    print(f"\n61 cases at 1e-8 vs simplex: max rel error {max(errs):.2e}, "
+          f"median {sorted(errs)[30]:.2e}")
+    assert max(errs) <= 1e-6
+
+
 def test_more_than_one_wave_of_cases(ctx):
     # 70 LPs run as two groups of lanes; duplicates must give identical answers.
     base, cases, models = stack(4, 7)

@@ -78,7 +78,8 @@ class PdhgOptions(ctypes.Structure):
         ("termination", c_int32), ("warm_x", DP), ("warm_y", DP),
         ("known_matrix_norm", c_double), ("initial_step_size", c_double),
         ("initial_primal_weight", c_double), ("verbose", c_int32), ("log_frequency", c_int32),
-        ("stop", IP), ("use_graphs", c_int32), ("product_order", c_int32),
+        ("stop", IP), ("use_graphs", c_int32), ("product_order", c_int32), ("autotune", c_int32),
+        ("batch_waves", c_int32),
     ]
 
 
@@ -143,6 +144,7 @@ SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p, POINTER(ConvTerms)]),
     "anuk_set_profiling": (c_int, [c_void_p, c_int]),
     "anuk_profile_report": (c_int, [c_void_p, c_char_p, c_size_t, POINTER(c_size_t)]),
+    "anuk_tuning_report": (c_int, [c_void_p, c_char_p, c_size_t, POINTER(c_size_t)]),
     "anuk_lp_from_model": (c_int, [POINTER(Model), POINTER(c_void_p)]),
     "anuk_lp_from_standard": (c_int, [c_int32, c_int32, c_int32, IP, IP, DP, DP, DP, DP, DP,
                                       c_double, c_double, POINTER(c_void_p)]),

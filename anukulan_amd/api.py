@@ -234,6 +234,16 @@ class Context:
         self._call(lib.anuk_profile_report, buf, need.value, byref(need))
         return buf.value.decode()
 
+    def tuning_report(self) -> str:
+        """The batched solver's launch-shape tuning: one line per kernel and
+        matrix with the timing of every shape and the one chosen."""
+        need = c_size_t(0)
+        self._call(lib.anuk_tuning_report, None, 0, byref(need))
+        buf = ctypes.create_string_buffer(max(1, need.value))
+        self._call(lib.anuk_tuning_report, buf, need.value, None)
+        return buf.value.decode()
+
+
 
 class LP:
     """A standard-form LP held by the library (StandardLp)."""
@@ -385,8 +395,11 @@ def solve_pdhg(ctx: Context, lp: LP, options: Optional[_lib.PdhgOptions] = None,
 
 def solve_pdhg_batch(ctx: Context, lps, options: Optional[_lib.PdhgOptions] = None,
                      **overrides) -> list:
-    """Solves LPs that share one constraint matrix together; result i equals
-    solve_pdhg(ctx, lps[i], ...) bit for bit."""
+    """Solves LPs that share one constraint matrix together. Each lane's row
+    sums and steps are the plain solve's operations in the same order; the
+    check reductions fold in a different fixed tree, so result i reaches what
+    solve_pdhg(ctx, lps[i], ...) reaches (same status and decisions in the
+    tested stacks, objective within rounding), not necessarily its bits."""
     lps = list(lps)
     if not lps:
         return []

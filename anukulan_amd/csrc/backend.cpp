@@ -454,6 +454,26 @@ Terms HipBackend::convergence_terms(Int n, Int m, const DevProblem& p, const dou
   return out;
 }
 
+// ---- tuning ------------------------------------------------------------------
+
+std::string HipBackend::tuning_report() const {
+  std::string out;
+  char buf[96];
+  for (const auto& kv : tuned_) {
+    const Tuning& t = kv.second;
+    out += t.kernel;
+    std::snprintf(buf, sizeof(buf), " rows=%d nnz=%d", t.rows, t.nnz);
+    out += buf;
+    for (const auto& c : t.us) {
+      std::snprintf(buf, sizeof(buf), " us@%d=%.3f", c.first, c.second);
+      out += buf;
+    }
+    std::snprintf(buf, sizeof(buf), " chosen=%d\n", t.chosen);
+    out += buf;
+  }
+  return out;
+}
+
 // ---- profiling -------------------------------------------------------------
 
 void HipBackend::set_profiling(bool on) {

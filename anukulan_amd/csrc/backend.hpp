@@ -116,6 +116,23 @@ class HipBackend {
   // (its minimum traffic); null events when profiling is off.
   LaunchEvents launch_events(const char* name, double bytes = 0.0);
 
+  // Launch shapes chosen by a tuner, per kernel name and matrix identity,
+  // with the timings they were chosen from (for the report).
+  struct Tuning {
+    std::string kernel;
+    Int rows = 0, nnz = 0;
+    std::vector<std::pair<int, double>> us;  // (shape, mean microseconds)
+    int chosen = 0;
+  };
+  const Tuning* tuned(const std::string& kernel, std::uint64_t matrix) const {
+    auto it = tuned_.find({kernel, matrix});
+    return it == tuned_.end() ? nullptr : &it->second;
+  }
+  void record_tuning(std::uint64_t matrix, Tuning t) {
+    tuned_[{t.kernel, matrix}] = std::move(t);
+  }
+  std::string tuning_report() const;
+
   // Brackets a launch sequence for the profiler, filed under `name` with the
   // sequence's algorithmic bytes; no-ops when profiling is off.
   class Timed {
@@ -138,6 +155,7 @@ class HipBackend {
     std::vector<void*> owned;
   };
   void free_matrix(DeviceMatrix* dm);
+  std::map<std::pair<std::string, std::uint64_t>, Tuning> tuned_;
   // Plans `a` for the fast order and stores the plan in `dm`.
   void replan_fast(const CsrView& a, DeviceMatrix* dm);
 

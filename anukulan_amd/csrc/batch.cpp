@@ -86,6 +86,7 @@ class Engine {
   void run_polish(const std::vector<int>& who, Int budget, std::vector<LaneOutcome>& out,
                   std::vector<bool>* stopped, Int iteration);
   void compact();
+  void tune();
   // The device slots of a set of lanes.
   Mask slots(Mask lanes) const {
     Mask m = 0;
@@ -122,6 +123,8 @@ class Engine {
   // One past the highest live slot, rounded up to a whole 16-slot line: the
   // step kernels' lane bound, so at most four graph variants per parity.
   Int span_ = kLanes;
+  // Rows per workgroup of the primal (K^T) and dual (K) launches.
+  int pw_ = 4, dw_ = 4;
 
   double* slab_ = nullptr;
   double *X_[2], *Y_[2], *KX_[2];
@@ -296,19 +299,19 @@ void Engine::enqueue_period(int steps, bool has_check, int parity) {
     const LaunchEvents pe =
         be_.launch_events(diag ? "batch_primal_step_check" : "batch_primal_step",
                           12.0 * nnz + 4.0 * (n + 1) + 8.0 * lanes * (m + (diag ? 8.0 : 7.0) * n));
-    launch_batch_primal_step(s, dkt_, Y_[p], data_, d_params_, step, span_, pb, diag, &pe);
+    launch_batch_primal_step(s, dkt_, Y_[p], data_, d_params_, step, span_, pb, diag, pw_, &pe);
     DualBufs db{Y_[p], KX_[p], AY_, AKX_, Y_[1 - p], KX_[1 - p], DY_, KDX_, AY_, AKX_};
     const LaunchEvents de =
         be_.launch_events(diag ? "batch_dual_step_check" : "batch_dual_step",
                           12.0 * nnz + 4.0 * (m + 1) + 8.0 * lanes * (n + (diag ? 9.0 : 7.0) * m));
-    launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, step, span_, db, diag, &de);
+    launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, step, span_, db, diag, dw_, &de);
     p = 1 - p;
   }
   if (has_check) {
     const bool farkas = opt_.detect_infeasibility != 0;
     HipBackend::Timed t(be_, "batch_check");
     launch_batch_spmm(s, dkt_, Y_[p], KTY_, farkas ? DY_ : nullptr, farkas ? LAM_ : nullptr,
-                      d_params_);
+                      d_params_, ~0ull, pw_);
     launch_batch_check(s, data_, X_[p], Y_[p], KX_[p], DX_, DY_, KTY_, LAM_, farkas, d_params_,
                        CHECKP_, CHECK_);
   }
@@ -371,7 +374,7 @@ void Engine::retire_extract(Mask mask, std::vector<LaneOutcome>& out) {
 void Engine::run_restarts(Mask mask) {
   if (mask == 0) return;
   const Mask dev = slots(mask);
-  launch_batch_spmm(be_.stream(), dk_, X_[cur_], KX_[cur_], nullptr, nullptr, nullptr, dev);
+  launch_batch_spmm(be_.stream(), dk_, X_[cur_], KX_[cur_], nullptr, nullptr, nullptr, dev, dw_);
   const bool previous = !opt_.pid_primal_weight;
   launch_batch_restart(be_.stream(), n_, m_, X_[cur_], Y_[cur_], RX_, RY_, PRX_, PRY_, previous,
                        true, dev, CHECKP_, RESTART_);
@@ -545,6 +548,82 @@ void Engine::run_polish(const std::vector<int>& who, Int budget, std::vector<Lan
   }
 }
 
+// A key for a matrix's sparsity pattern (the launch shapes depend on nothing
+// else): its size and an FNV-1a hash of the row starts and column indices.
+std::uint64_t pattern_key(const Csr& a) {
+  std::uint64_t h = 1469598103934665603ull;
+  auto mix = [&](std::uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  mix(static_cast<std::uint64_t>(a.rows));
+  mix(static_cast<std::uint64_t>(a.cols));
+  for (Int v : a.start) mix(static_cast<std::uint64_t>(static_cast<std::uint32_t>(v)));
+  for (Int v : a.index) mix(static_cast<std::uint64_t>(static_cast<std::uint32_t>(v)));
+  return h;
+}
+
+// The launch-shape tuner: times the primal and dual step kernels with 1, 2,
+// 4 and 8 row-waves per workgroup on this stack's K^T and K (the mean of 20
+// back-to-back launches, best of three trials) and keeps the fastest, per
+// sparsity pattern for the context's life. Every shape computes the same
+// bits, so the choice changes only the time. The trial launches read the
+// current iterate and write only buffers the first real step overwrites (the
+// next parity, x-bar), with a step index that opens no Halpern epoch.
+void Engine::tune() {
+  if (opt_.batch_waves != 0) {
+    pw_ = dw_ = opt_.batch_waves;
+    return;
+  }
+  const std::uint64_t kp = pattern_key(k_), ktp = pattern_key(kt_);
+  const HipBackend::Tuning* tp = be_.tuned("batch_primal_step", ktp);
+  const HipBackend::Tuning* td = be_.tuned("batch_dual_step", kp);
+  if (tp != nullptr && td != nullptr) {
+    pw_ = tp->chosen;
+    dw_ = td->chosen;
+    return;
+  }
+  if (opt_.autotune == 0 || depth_ != 0 || be_.profiling()) return;
+  upload_params();
+  hipStream_t s = be_.stream();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hip_check(hipEventCreate(&e0), "hipEventCreate(tune)");
+  hip_check(hipEventCreate(&e1), "hipEventCreate(tune)");
+  constexpr int kReps = 20, kTrials = 3;
+  auto mean_us = [&](const auto& launch) {
+    double best = 1e300;
+    for (int t = 0; t < kTrials; ++t) {
+      hip_check(hipEventRecord(e0, s), "hipEventRecord(tune)");
+      for (int r = 0; r < kReps; ++r) launch();
+      hip_check(hipEventRecord(e1, s), "hipEventRecord(tune)");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize(tune)");
+      float ms = 0.0f;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime(tune)");
+      best = std::min(best, 1e3 * static_cast<double>(ms) / kReps);
+    }
+    return best;
+  };
+  const PrimalBufs pb{X_[cur_], AX_, X_[1 - cur_], XBAR_, DX_, AX_};
+  const DualBufs db{Y_[cur_], KX_[cur_], AY_, AKX_, Y_[1 - cur_], KX_[1 - cur_], DY_, KDX_, AY_, AKX_};
+  HipBackend::Tuning p{"batch_primal_step", kt_.rows, kt_.nnz(), {}, 4};
+  HipBackend::Tuning d{"batch_dual_step", k_.rows, k_.nnz(), {}, 4};
+  for (int w : {1, 2, 4, 8}) {
+    p.us.push_back({w, mean_us([&] {
+                      launch_batch_primal_step(s, dkt_, Y_[cur_], data_, d_params_, 1, span_, pb, false, w);
+                    })});
+    d.us.push_back({w, mean_us([&] {
+                      launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, 1, span_, db, false, w);
+                    })});
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  auto fastest = [](const HipBackend::Tuning& t) {
+    return std::min_element(t.us.begin(), t.us.end(),
+                            [](const auto& a, const auto& b) { return a.second < b.second; })->first;
+  };
+  p.chosen = pw_ = fastest(p);
+  d.chosen = dw_ = fastest(d);
+  be_.record_tuning(ktp, std::move(p));
+  be_.record_tuning(kp, std::move(d));
+}
+
 // Moves the live lanes into the lowest slots, in lane order, when that frees
 // a whole 16-slot (128-byte) line of every row: the step kernels then touch
 // ceil(live / 16) lines per row instead of the span of the scattered slots.
@@ -610,7 +689,8 @@ std::vector<LaneOutcome> Engine::run() {
   }
   upload_lanes();
   active_ = (count_ == 64) ? ~0ull : ((1ull << count_) - 1ull);
-  launch_batch_spmm(be_.stream(), dk_, X_[0], KX_[0], nullptr, nullptr, nullptr, active_);
+  tune();
+  launch_batch_spmm(be_.stream(), dk_, X_[0], KX_[0], nullptr, nullptr, nullptr, active_, dw_);
   cur_ = 0;
   auto lap = [&](double Phases::*field) {
     const Clock::time_point now = Clock::now();

@@ -56,12 +56,12 @@ __device__ __forceinline__ LaneScalars lane_scalars(const LaneParams& p, int ste
 // wave waits for one round of memory latency rather than three. A retired
 // lane issues no vector load and stores nothing: with the live lanes kept in
 // the lowest slots, a row touches only the cache lines that hold them.
-template <bool kDiag>
-__global__ void __launch_bounds__(256)
+template <bool kDiag, int kWaves>
+__global__ void __launch_bounds__(64 * kWaves)
     bprimal_kernel(DevCsr kt, const double* __restrict__ y, BatchData d, PrimalBufs b,
                    const LaneParams* __restrict__ params, int step, Int span) {
   const int lane = static_cast<int>(threadIdx.x & 63);
-  const Int j = wave_row();
+  const Int j = lanes::wave_row_in<kWaves>();
   if (j >= kt.rows || lane >= span) return;
   const size_t o = at(j, lane);
   const LaneParams P = params[lane];
@@ -82,12 +82,12 @@ __global__ void __launch_bounds__(256)
   if constexpr (kDiag) b.dx[o] = dd;
 }
 
-template <bool kDiag>
-__global__ void __launch_bounds__(256)
+template <bool kDiag, int kWaves>
+__global__ void __launch_bounds__(64 * kWaves)
     bdual_kernel(DevCsr k, const double* __restrict__ x_bar, BatchData d, DualBufs b,
                  const LaneParams* __restrict__ params, int step, Int span) {
   const int lane = static_cast<int>(threadIdx.x & 63);
-  const Int i = wave_row();
+  const Int i = lanes::wave_row_in<kWaves>();
   if (i >= k.rows || lane >= span) return;
   const size_t o = at(i, lane);
   const LaneParams P = params[lane];
@@ -126,12 +126,13 @@ __device__ __forceinline__ bool lane_on(const LaneParams* params, unsigned long 
   return params != nullptr ? params[lane].active != 0 : ((mask >> lane) & 1ull) != 0;
 }
 
-__global__ void __launch_bounds__(256)
+template <int kWaves>
+__global__ void __launch_bounds__(64 * kWaves)
     bspmm_kernel(DevCsr a, const double* __restrict__ v1, double* __restrict__ out1,
                  const double* __restrict__ v2, double* __restrict__ out2,
                  const LaneParams* __restrict__ params, unsigned long long mask) {
   const int lane = static_cast<int>(threadIdx.x & 63);
-  const Int r = wave_row();
+  const Int r = lanes::wave_row_in<kWaves>();
   if (r >= a.rows) return;  // uniform across the wave
   // The row sums are lane-serial (no cross-lane operation), so lanes beyond
   // the span sit out and the rest store only if live.
@@ -392,52 +393,81 @@ void check_launch(const char* what) {
     throw std::runtime_error(std::string("launch ") + what + ": " + hipGetErrorString(e));
 }
 
-inline dim3 row_grid(Int rows) {
-  return dim3(static_cast<unsigned>((rows + kWavesPerBlock - 1) / kWavesPerBlock));
-}
-
+// One wave per row, `waves` rows per workgroup.
 template <class K, class... Args>
-void launch_rows(K kernel, Int rows, hipStream_t s, const LaunchEvents* ev, const char* what,
-                 Args... args) {
+void launch_rows(K kernel, int waves, Int rows, hipStream_t s, const LaunchEvents* ev,
+                 const char* what, Args... args) {
   if (rows <= 0) return;
+  const dim3 grid(static_cast<unsigned>((rows + waves - 1) / waves));
+  const dim3 block(static_cast<unsigned>(64 * waves));
   if (ev != nullptr && ev->start != nullptr) {
-    hipExtLaunchKernelGGL(kernel, row_grid(rows), dim3(256), 0, s, ev->start, ev->stop, 0, args...);
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev->start, ev->stop, 0, args...);
   } else {
-    hipLaunchKernelGGL(kernel, row_grid(rows), dim3(256), 0, s, args...);
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
   }
   check_launch(what);
 }
+
+// The kernel instance for a shape (row-waves per workgroup).
+template <template <int> class Pick>
+auto by_waves(int waves) {
+  switch (waves) {
+    case 1: return Pick<1>::get();
+    case 2: return Pick<2>::get();
+    case 8: return Pick<8>::get();
+    default: return Pick<4>::get();
+  }
+}
+template <bool kDiag>
+struct PrimalPick {
+  template <int W>
+  struct At {
+    static auto get() { return &bprimal_kernel<kDiag, W>; }
+  };
+};
+template <bool kDiag>
+struct DualPick {
+  template <int W>
+  struct At {
+    static auto get() { return &bdual_kernel<kDiag, W>; }
+  };
+};
+template <int W>
+struct SpmmAt {
+  static auto get() { return &bspmm_kernel<W>; }
+};
 
 }  // namespace
 
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                               const BatchData& d, const LaneParams* params, int step, Int span,
-                              const PrimalBufs& b, bool diag, const LaunchEvents* ev) {
+                              const PrimalBufs& b, bool diag, int waves, const LaunchEvents* ev) {
   if (diag) {
-    launch_rows(bprimal_kernel<true>, kt.rows, s, ev, "batch_primal_step", kt, y, d, b, params, step,
-                span);
+    launch_rows(by_waves<PrimalPick<true>::At>(waves), waves, kt.rows, s, ev, "batch_primal_step",
+                kt, y, d, b, params, step, span);
   } else {
-    launch_rows(bprimal_kernel<false>, kt.rows, s, ev, "batch_primal_step", kt, y, d, b, params, step,
-                span);
+    launch_rows(by_waves<PrimalPick<false>::At>(waves), waves, kt.rows, s, ev, "batch_primal_step",
+                kt, y, d, b, params, step, span);
   }
 }
 
 void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                             const BatchData& d, const LaneParams* params, int step, Int span,
-                            const DualBufs& b, bool diag, const LaunchEvents* ev) {
+                            const DualBufs& b, bool diag, int waves, const LaunchEvents* ev) {
   if (diag) {
-    launch_rows(bdual_kernel<true>, k.rows, s, ev, "batch_dual_step", k, x_bar, d, b, params, step,
-                span);
+    launch_rows(by_waves<DualPick<true>::At>(waves), waves, k.rows, s, ev, "batch_dual_step", k,
+                x_bar, d, b, params, step, span);
   } else {
-    launch_rows(bdual_kernel<false>, k.rows, s, ev, "batch_dual_step", k, x_bar, d, b, params, step,
-                span);
+    launch_rows(by_waves<DualPick<false>::At>(waves), waves, k.rows, s, ev, "batch_dual_step", k,
+                x_bar, d, b, params, step, span);
   }
 }
 
 void launch_batch_spmm(hipStream_t s, const DevCsr& a, const double* v1, double* out1,
                        const double* v2, double* out2, const LaneParams* params,
-                       unsigned long long mask) {
-  launch_rows(bspmm_kernel, a.rows, s, nullptr, "batch_spmm", a, v1, out1, v2, out2, params, mask);
+                       unsigned long long mask, int waves) {
+  launch_rows(by_waves<SpmmAt>(waves), waves, a.rows, s, nullptr, "batch_spmm", a, v1, out1, v2,
+              out2, params, mask);
 }
 
 namespace {

@@ -61,19 +61,23 @@ inline constexpr int kLaneCheckStride = 16;
 
 // `span`: the lanes below it load and compute (the live ones among them
 // store); a kernel argument, so the bound costs no dependent load.
+// `waves`: rows (one per wave) per workgroup, 1, 2, 4 or 8; every shape
+// computes the same bits.
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                               const BatchData& d, const LaneParams* params, int step, Int span,
-                              const PrimalBufs& b, bool diag, const LaunchEvents* ev = nullptr);
+                              const PrimalBufs& b, bool diag, int waves = 4,
+                              const LaunchEvents* ev = nullptr);
 void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                             const BatchData& d, const LaneParams* params, int step, Int span,
-                            const DualBufs& b, bool diag, const LaunchEvents* ev = nullptr);
+                            const DualBufs& b, bool diag, int waves = 4,
+                            const LaunchEvents* ev = nullptr);
 // out1 = A v1 (and out2 = A v2 when v2 is non-null) for the active lanes:
 // those whose LaneParams say so when `params` is given (read at run time, so
 // a captured graph follows retirements), else those set in `mask`. Other
 // lanes are left alone.
 void launch_batch_spmm(hipStream_t s, const DevCsr& a, const double* v1, double* out1,
                        const double* v2, double* out2, const LaneParams* params,
-                       unsigned long long mask = ~0ull);
+                       unsigned long long mask = ~0ull, int waves = 4);
 // Per-lane check reductions into out[lane * kLaneCheckStride + slot] (slots
 // as CheckSlot) for the active lanes: movement sums, convergence terms of the
 // unscaled problem, and both halves of the Farkas test. Deterministic: a

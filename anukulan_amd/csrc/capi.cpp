@@ -353,6 +353,19 @@ int anuk_profile_report(anuk_ctx* ctx, char* buf, size_t cap, size_t* needed) {
   });
 }
 
+int anuk_tuning_report(anuk_ctx* ctx, char* buf, size_t cap, size_t* needed) {
+  if (ctx == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    const std::string r = ctx->be->tuning_report();
+    if (needed) *needed = r.size() + 1;
+    if (buf != nullptr && cap > 0) {
+      const size_t k = std::min(cap - 1, r.size());
+      std::memcpy(buf, r.data(), k);
+      buf[k] = '\0';
+    }
+  });
+}
+
 // ---- host layout ------------------------------------------------------------
 
 int anuk_lp_from_model(const anuk_model* model, anuk_lp** out) {
@@ -504,6 +517,10 @@ anuk_pdhg_options checked_options(const anuk_pdhg_options* options) {
   need(o.max_iterations >= 0, "max_iterations must be non-negative");
   need(o.polish_first_iteration > 0, "polish_first_iteration must be positive");
   need(o.product_order >= 0 && o.product_order <= 2, "product_order must be 0, 1 or 2");
+  need(o.autotune == 0 || o.autotune == 1, "autotune must be 0 or 1");
+  need(o.batch_waves == 0 || o.batch_waves == 1 || o.batch_waves == 2 || o.batch_waves == 4 ||
+           o.batch_waves == 8,
+       "batch_waves must be 0, 1, 2, 4 or 8");
   return o;
 }
 

@@ -21,11 +21,15 @@ __device__ __forceinline__ int xcd_remap(int i, int g) {
   return x * per + (x < rem ? x : rem) + local;
 }
 
-// The row this wave owns, made wave-uniform so the matrix loads are scalar.
-__device__ __forceinline__ Int wave_row() {
+// The row this wave owns in a launch of kWaves row-waves per workgroup,
+// made wave-uniform so the matrix loads are scalar.
+template <int kWaves>
+__device__ __forceinline__ Int wave_row_in() {
   const int blk = xcd_remap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
-  return __builtin_amdgcn_readfirstlane(blk * kWavesPerBlock + static_cast<int>(threadIdx.x >> 6));
+  return __builtin_amdgcn_readfirstlane(blk * kWaves + static_cast<int>(threadIdx.x >> 6));
 }
+
+__device__ __forceinline__ Int wave_row() { return wave_row_in<kWavesPerBlock>(); }
 
 __device__ __forceinline__ size_t at(Int row, int lane) {
   return static_cast<size_t>(row) * kLaneCount + static_cast<size_t>(lane);

@@ -1031,6 +1031,9 @@ void default_options(anuk_pdhg_options* o) {
   o->log_frequency = 5000;
   o->stop = nullptr;
   o->use_graphs = 1;
+  o->product_order = 0;
+  o->autotune = 1;
+  o->batch_waves = 0;
 }
 
 }  // namespace anuk

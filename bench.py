@@ -104,6 +104,25 @@ def cpu_baseline_parallel(base, cases, tol, threads):
                       f"thread on {threads} threads: {iters} iterations in {wall:.1f} s wall"}
 
 
+def parse_tuning(text):
+    """Lines of Context.tuning_report(): kernel, rows, nnz, the mean launch
+    time of each shape (row-waves per workgroup) and the shape chosen."""
+    out = []
+    for line in text.splitlines():
+        parts = line.split()
+        if not parts:
+            continue
+        rec = {"kernel": parts[0], "us_by_waves": {}}
+        for p in parts[1:]:
+            k, v = p.split("=")
+            if k.startswith("us@"):
+                rec["us_by_waves"][k[3:]] = float(v)
+            else:
+                rec[k] = int(v)
+        out.append(rec)
+    return out
+
+
 def simplex_optima(periods, count):
     """The reference simplex's FP64 optimum of each case of the stack, as
     committed in tests/golden/pdhg_reference.json (stack24), or None."""
@@ -312,6 +331,7 @@ def main():
     value = len(lps) * args.steps / elapsed
     solved = sum(1 for r in results if r.status == "optimal")
     lane_iters = sum(r.iterations for r in results)
+    tuning = ctx.tuning_report()
     optima = simplex_optima(args.periods, len(lps))
 
     plain = None
@@ -369,6 +389,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_par,
         "plain": plain, "c1": c1, "c3": c3,
         "batch_phases_s": {k: round(v, 4) for k, v in results[0].phases.items()},
+        "autotune": parse_tuning(tuning),
         "kernels": {k: {"mean_us": v["mean_us"], "calls": v["calls"], "GB/s": v["gbs"]}
                     for k, v in kernels.items()},
     }

@@ -315,6 +315,14 @@ typedef struct anuk_pdhg_options {
                                           (long rows summed by a wave in a fixed tree,
                                           large K x split into per-XCD column panels);
                                           deterministic either way */
+  int32_t autotune;                    /* 1: the batched solver times its launch shapes
+                                          (row-waves per workgroup: 1, 2, 4, 8) on the
+                                          stack's K and K^T once per matrix and keeps the
+                                          fastest; every shape computes the same bits.
+                                          0: four waves */
+  int32_t batch_waves;                 /* 0: as autotune decides; 1, 2, 4 or 8: the batched
+                                          step kernels use this many row-waves per
+                                          workgroup */
 } anuk_pdhg_options;
 
 void anuk_pdhg_default_options(anuk_pdhg_options* options);
@@ -448,6 +456,12 @@ int anuk_pdhg_adjoint(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,
                       const double* x, const double* y, const double* w_x, const double* w_y,
                       double* grad_c, double* grad_q, double* grad_lower, double* grad_upper,
                       anuk_adjoint_info* info);
+
+/* The launch shapes the batched solver's tuner measured and chose on this
+ * context, one line per tuned kernel and matrix:
+ *   <kernel> rows=<r> nnz=<z> us@1=<t> us@2=<t> us@4=<t> us@8=<t> chosen=<w>
+ * Same buffer protocol as anuk_profile_report. No reference counterpart. */
+int anuk_tuning_report(anuk_ctx* ctx, char* buffer, size_t capacity, size_t* needed);
 
 /* estimate_matrix_norm (pdhg.hpp:580-581, pdhg.cpp:223-282) of lp's K as
  * given (no scaling), by power iteration on the device. */

@@ -164,7 +164,10 @@ void ref_pdhg_default_options(anuk_pdhg_options* o) {
   o->initial_primal_weight = p.initial_primal_weight;
   o->verbose = p.verbose;
   o->log_frequency = p.log_frequency;
-  o->use_graphs = 1;
+  o->use_graphs = 1;     // the fields below have no reference counterpart:
+  o->product_order = 0;  // the backend's own defaults, so the structs compare
+  o->autotune = 1;
+  o->batch_waves = 0;
 }
 
 // ---- LP construction (standard_form.cpp:72-165, mps_reader.cpp) ----------

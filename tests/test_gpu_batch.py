@@ -90,6 +90,23 @@ def test_full_stack_against_the_simplex_optima(ctx):
     assert max(errs) <= 1e-6
 
 
+def test_every_launch_shape_gives_the_same_bits(ctx):
+    # The tuner picks among row-waves per workgroup (1, 2, 4, 8); a lane's
+    # arithmetic must not depend on the shape, so every shape and the tuned
+    # choice give the same iterates to the bit.
+    base, cases, models = stack(4, 8)
+    lps = [m.to_lp() for m in models]
+    want = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=4)
+    for w in (1, 2, 8, 0):
+        got = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=w)
+        for g, h in zip(got, want):
+            assert g.iterations == h.iterations, w
+            assert np.array_equal(g.x, h.x) and np.array_equal(g.y, h.y), w
+    report = ctx.tuning_report()
+    print("\n" + report)
+    assert "batch_primal_step" in report and "batch_dual_step" in report and "chosen=" in report
+
+
 def test_more_than_one_wave_of_cases(ctx):
     # 70 LPs run as two groups of lanes; duplicates must give identical answers.
     base, cases, models = stack(4, 7)

@@ -528,26 +528,8 @@ void Parser::finish() {
     warn("negative UP bounds seen on " + std::to_string(negative_up_) + " column(s); reader is using the " +
          std::string(opt_.negative_up_minus_infinity ? "CPLEX" : "HiGHS") + " convention");
 
-  // The model's own consistency checks (model.cpp:15-62), in the same order.
-  auto invalid = [&](const std::string& why) { fail("model failed validation: " + why); };
-  for (std::size_t j = 0; j < uz(n); ++j) {
-    if (std::isnan(m_.col_lower[j]) || std::isnan(m_.col_upper[j]))
-      invalid("NaN column bound at column " + std::to_string(j));
-    if (m_.col_lower[j] > m_.col_upper[j]) invalid("empty column bound interval at column " + std::to_string(j));
-    if (std::isnan(m_.objective[j])) invalid("NaN objective coefficient at column " + std::to_string(j));
-  }
-  for (std::size_t i = 0; i < uz(rows); ++i) {
-    if (std::isnan(m_.row_lower[i]) || std::isnan(m_.row_upper[i]))
-      invalid("NaN row bound at row " + std::to_string(i));
-    if (m_.row_lower[i] > m_.row_upper[i]) invalid("empty row bound interval at row " + std::to_string(i));
-  }
-  if (m_.hessian.nnz() > 0) {
-    const Csr qt = m_.hessian.transpose();
-    if (qt.start != m_.hessian.start || qt.index != m_.hessian.index)
-      invalid("hessian sparsity pattern is not symmetric");
-    for (std::size_t k = 0; k < qt.value.size(); ++k)
-      if (qt.value[k] != m_.hessian.value[k]) invalid("hessian is not symmetric");
-  }
+  std::string why;
+  if (!validate_model(m_, &why)) fail("model failed validation: " + why);
   m_.fill_coo();
 }
 
@@ -633,6 +615,46 @@ std::string unique_name(const std::unordered_set<std::string>& taken, const std:
 }
 
 }  // namespace
+
+// The model's own consistency checks (model.cpp:15-62), in the same order.
+bool validate_model(const MpsModel& m, std::string* error) {
+  auto bad = [&](const std::string& why) {
+    if (error) *error = why;
+    return false;
+  };
+  std::string csr_error;
+  if (!m.a.validate(&csr_error)) return bad(csr_error);
+  const std::size_t n = uz(m.a.cols), rows = uz(m.a.rows);
+  if (m.objective.size() != n) return bad("objective length does not match columns");
+  if (m.col_lower.size() != n || m.col_upper.size() != n)
+    return bad("column bound length does not match columns");
+  if (m.col_type.size() != n) return bad("column type length does not match columns");
+  if (m.row_lower.size() != rows || m.row_upper.size() != rows)
+    return bad("row bound length does not match rows");
+  if (!m.row_names.empty() && m.row_names.size() != rows) return bad("row name length does not match rows");
+  if (!m.col_names.empty() && m.col_names.size() != n) return bad("column name length does not match columns");
+  for (std::size_t j = 0; j < n; ++j) {
+    if (std::isnan(m.col_lower[j]) || std::isnan(m.col_upper[j]))
+      return bad("NaN column bound at column " + std::to_string(j));
+    if (m.col_lower[j] > m.col_upper[j]) return bad("empty column bound interval at column " + std::to_string(j));
+    if (std::isnan(m.objective[j])) return bad("NaN objective coefficient at column " + std::to_string(j));
+  }
+  for (std::size_t i = 0; i < rows; ++i) {
+    if (std::isnan(m.row_lower[i]) || std::isnan(m.row_upper[i]))
+      return bad("NaN row bound at row " + std::to_string(i));
+    if (m.row_lower[i] > m.row_upper[i]) return bad("empty row bound interval at row " + std::to_string(i));
+  }
+  if (m.hessian.nnz() > 0) {
+    if (m.hessian.rows != m.a.cols || m.hessian.cols != m.a.cols)
+      return bad("hessian is not square in the number of columns");
+    const Csr qt = m.hessian.transpose();
+    if (qt.start != m.hessian.start || qt.index != m.hessian.index)
+      return bad("hessian sparsity pattern is not symmetric");
+    for (std::size_t k = 0; k < qt.value.size(); ++k)
+      if (qt.value[k] != m.hessian.value[k]) return bad("hessian is not symmetric");
+  }
+  return true;
+}
 
 void MpsModel::fill_coo() {
   coo_row.resize(a.value.size());

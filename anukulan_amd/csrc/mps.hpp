@@ -79,6 +79,11 @@ struct FileError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// The model's consistency checks (model.cpp:15-62): false, with the first
+// problem in `error`, for mismatched lengths, NaN data, an empty bound
+// interval or an asymmetric Hessian.
+bool validate_model(const MpsModel& m, std::string* error);
+
 MpsModel read_mps_text(std::string_view text, const std::string& source, const MpsOptions& opt);
 MpsModel read_mps_file(const std::string& path, const MpsOptions& opt);
 

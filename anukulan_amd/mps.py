@@ -98,7 +98,16 @@ def _copy(ptr, n, dtype):
     return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
 
 
-def _take(h) -> MpsModel:
+def _names_of(get, h, count):
+    """The names, or [] for a model that has none (one built in memory)."""
+    if count == 0 or get(h, 0) is None:
+        return []
+    return [get(h, k).decode() for k in range(count)]
+
+
+def _take(h, owned=True) -> MpsModel:
+    """Copies the model out of an anuk_mps handle, freeing the handle when
+    this side owns it."""
     try:
         info = _lib.MpsInfo()
         lib.anuk_mps_info_get(h, byref(info))
@@ -123,15 +132,16 @@ def _take(h) -> MpsModel:
             col_lower=_copy(m.col_lower, cols, np.float64),
             col_upper=_copy(m.col_upper, cols, np.float64),
             col_type=_copy(lib.anuk_mps_col_types(h), cols, np.int32),
-            row_names=[lib.anuk_mps_row_name(h, r).decode() for r in range(rows)],
-            col_names=[lib.anuk_mps_col_name(h, c).decode() for c in range(cols)],
+            row_names=_names_of(lib.anuk_mps_row_name, h, rows),
+            col_names=_names_of(lib.anuk_mps_col_name, h, cols),
             maximize=bool(info.maximize), objective_offset=info.objective_offset, hessian=hess,
             warnings=[lib.anuk_mps_warning(h, k).decode() for k in range(info.warnings)],
             free_rows_dropped=info.free_rows_dropped,
             objective_nonzeros=int(info.objective_nonzeros),
             fixed_format=bool(info.fixed_format))
     finally:
-        lib.anuk_mps_free(h)
+        if owned:
+            lib.anuk_mps_free(h)
 
 
 def read_mps(path: str, **opts) -> MpsModel:
@@ -168,8 +178,8 @@ def write_mps(path: str, model, name: Optional[str] = None, row_names=None, col_
     generated names and continuous columns."""
     if isinstance(model, MpsModel):
         ri, ci, v = model.row_index(), model.index, model.value
-        row_names = model.row_names if row_names is None else row_names
-        col_names = model.col_names if col_names is None else col_names
+        row_names = (model.row_names or None) if row_names is None else row_names
+        col_names = (model.col_names or None) if col_names is None else col_names
         col_type = model.col_type if col_type is None else col_type
     else:
         ri, ci, v = model.row_index, model.col_index, model.value

@@ -15,6 +15,7 @@
 #include "anukulan/backend.hpp"
 #include "anukulan/mps_reader.hpp"
 #include "anukulan/pdhg.hpp"
+#include "anukulan/presolve.hpp"
 #include "anukulan/scaling.hpp"
 #include "anukulan/simplex.hpp"
 #include "anukulan/standard_form.hpp"
@@ -323,6 +324,107 @@ const char* ref_mps_col_name(const void* h, int32_t j) {
 }
 const char* ref_mps_warning(const void* h, int32_t k) {
   return static_cast<const RefMps*>(h)->r.warnings[static_cast<std::size_t>(k)].c_str();
+}
+
+// ---- Presolve (presolve.cpp:1215-1317) ------------------------------------
+
+struct RefPresolve {
+  PresolveResult r;
+  RefMps reduced;  // the reduced model, through the ref_mps_* accessors
+};
+
+// flags: the fourteen PresolveOptions switches in header order, then
+// max_rounds; tols: feasibility, infeasibility, bound relaxation, max new
+// finite bound. NULL for either: the defaults.
+void* ref_presolve_mps(const char* text, int64_t length, const int32_t* flags, const double* tols) {
+  try {
+    std::istringstream in(std::string(text, static_cast<std::size_t>(length)));
+    const MpsReadResult read = read_mps_stream(in, "<text>");
+    if (!read.ok) {
+      g_error = read.error;
+      return nullptr;
+    }
+    PresolveOptions o;
+    if (flags != nullptr) {
+      bool* b[] = {&o.empty_rows, &o.singleton_rows, &o.redundant_rows, &o.forcing_rows,
+                   &o.duplicate_rows, &o.fixed_columns, &o.empty_columns,
+                   &o.free_column_singletons, &o.bound_tightening, &o.slack_column_singletons,
+                   &o.inequality_column_singletons, &o.dual_fixing, &o.doubleton_equations,
+                   &o.coefficient_tightening};
+      for (int k = 0; k < 14; ++k) *b[k] = flags[k] != 0;
+      o.max_rounds = flags[14];
+    }
+    if (tols != nullptr) {
+      o.feasibility_tolerance = tols[0];
+      o.infeasibility_tolerance = tols[1];
+      o.bound_relaxation = tols[2];
+      o.max_new_finite_bound = tols[3];
+    }
+    auto* h = new RefPresolve;
+    h->r = presolve(read.model, o);
+    h->reduced.r.ok = h->r.status == PresolveStatus::kReduced;
+    h->reduced.r.model = h->r.reduced;
+    for (VarType t : h->r.reduced.col_type) h->reduced.col_type.push_back(static_cast<int32_t>(t));
+    return h;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+
+void ref_presolve_free(void* h) { delete static_cast<RefPresolve*>(h); }
+int ref_presolve_status(const void* h) { return static_cast<int>(static_cast<const RefPresolve*>(h)->r.status); }
+const char* ref_presolve_message(const void* h) { return static_cast<const RefPresolve*>(h)->r.message.c_str(); }
+int ref_presolve_dual_exact(const void* h) {
+  return static_cast<const RefPresolve*>(h)->r.postsolve.dual_is_exact() ? 1 : 0;
+}
+const void* ref_presolve_reduced(const void* h) { return &static_cast<const RefPresolve*>(h)->reduced; }
+
+// counts[0..21]: rows, cols, nonzeros removed; empty, singleton, redundant,
+// forcing, duplicate rows; fixed, empty columns; free, slack, inequality
+// singletons, declined; doubletons; dual fixed; coefficients tightened;
+// bounds tightened, made finite; rounds. counts_f[0]: largest new bound.
+void ref_presolve_counts(const void* h, int64_t* counts, double* largest) {
+  const PresolveCounts& c = static_cast<const RefPresolve*>(h)->r.counts;
+  const Int v[] = {c.rows_removed, c.cols_removed, c.nonzeros_removed, c.empty_rows,
+                   c.singleton_rows, c.redundant_rows, c.forcing_rows, c.duplicate_rows,
+                   c.fixed_columns, c.empty_columns, c.free_column_singletons,
+                   c.slack_column_singletons, c.inequality_column_singletons,
+                   c.slack_singletons_declined, c.doubleton_equations, c.dual_fixed_columns,
+                   c.coefficients_tightened, c.bounds_tightened, c.bounds_made_finite, c.rounds};
+  for (int k = 0; k < 20; ++k) counts[k] = v[k];
+  *largest = c.largest_new_finite_bound;
+}
+
+void ref_postsolve_primal(const void* h, const double* rx, int64_t n, double* x) {
+  const RefPresolve* p = static_cast<const RefPresolve*>(h);
+  const std::vector<double> full = p->r.postsolve.apply(std::vector<double>(rx, rx + n));
+  std::memcpy(x, full.data(), full.size() * sizeof(double));
+}
+
+void ref_postsolve_dual(const void* h, const double* ry, int64_t m, const double* rc, int64_t n,
+                        const double* oc, int64_t on, double* y) {
+  const RefPresolve* p = static_cast<const RefPresolve*>(h);
+  const std::vector<double> full = p->r.postsolve.apply_dual(
+      std::vector<double>(ry, ry + m), std::vector<double>(rc, rc + n), std::vector<double>(oc, oc + on));
+  std::memcpy(y, full.data(), full.size() * sizeof(double));
+}
+
+// The reduced model in standard form (to_standard_form), as a RefLp.
+void* ref_presolve_lp(const void* h) {
+  try {
+    StandardFormResult sf = to_standard_form(static_cast<const RefPresolve*>(h)->r.reduced);
+    if (!sf.ok) {
+      g_error = sf.error;
+      return nullptr;
+    }
+    auto* out = new RefLp;
+    out->lp = std::move(sf.lp);
+    return out;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
 }
 
 void* ref_lp_from_standard(int32_t rows, int32_t cols, int32_t neq, const int32_t* start,

@@ -295,6 +295,21 @@ def _load_ref():
     for f in ("ref_mps_row_name", "ref_mps_col_name", "ref_mps_warning"):
         getattr(L, f).restype = c_char_p
         getattr(L, f).argtypes = [c_void_p, c_int32]
+    L.ref_presolve_mps.restype = c_void_p
+    L.ref_presolve_mps.argtypes = [c_char_p, ctypes.c_int64, IP, DP]
+    L.ref_presolve_free.argtypes = [c_void_p]
+    L.ref_presolve_status.argtypes = [c_void_p]
+    L.ref_presolve_message.restype = c_char_p
+    L.ref_presolve_message.argtypes = [c_void_p]
+    L.ref_presolve_dual_exact.argtypes = [c_void_p]
+    L.ref_presolve_reduced.restype = c_void_p
+    L.ref_presolve_reduced.argtypes = [c_void_p]
+    L.ref_presolve_counts.argtypes = [c_void_p, POINTER(ctypes.c_int64), DP]
+    L.ref_postsolve_primal.argtypes = [c_void_p, DP, ctypes.c_int64, DP]
+    L.ref_postsolve_dual.argtypes = [c_void_p, DP, ctypes.c_int64, DP, ctypes.c_int64, DP,
+                                     ctypes.c_int64, DP]
+    L.ref_presolve_lp.restype = c_void_p
+    L.ref_presolve_lp.argtypes = [c_void_p]
     return L
 
 
@@ -309,35 +324,101 @@ def ref_read_mps(text, format=0, negative_up=0, infinity_threshold=1e30) -> dict
     if not h:
         raise RuntimeError(ref.ref_last_error().decode())
     try:
-        out = {"ok": bool(ref.ref_mps_ok(h)), "error": ref.ref_mps_error(h).decode()}
-        sz = (ctypes.c_int64 * 9)()
-        ref.ref_mps_sizes(h, sz)
-        rows, cols, nnz, qnnz = sz[0], sz[1], sz[2], sz[3]
-        out["warnings"] = [ref.ref_mps_warning(h, k).decode() for k in range(sz[8])]
-        if not out["ok"]:
-            return out
-        a = {"start": np.zeros(rows + 1, np.int32), "index": np.zeros(nnz, np.int32),
-             "value": np.zeros(nnz), "objective": np.zeros(cols), "row_lower": np.zeros(rows),
-             "row_upper": np.zeros(rows), "col_lower": np.zeros(cols), "col_upper": np.zeros(cols),
-             "col_type": np.zeros(cols, np.int32)}
-        q = (np.zeros(cols + 1, np.int32), np.zeros(qnnz, np.int32), np.zeros(qnnz)) if qnnz else None
-
-        def p(x, t):
-            return x.ctypes.data_as(t) if x is not None and x.size else None
-        ref.ref_mps_arrays(h, p(a["start"], IP), p(a["index"], IP), p(a["value"], DP),
-                           p(q[0], IP) if q else None, p(q[1], IP) if q else None,
-                           p(q[2], DP) if q else None, p(a["objective"], DP),
-                           p(a["row_lower"], DP), p(a["row_upper"], DP), p(a["col_lower"], DP),
-                           p(a["col_upper"], DP), p(a["col_type"], IP))
-        out.update(a)
-        out.update(rows=rows, cols=cols, maximize=bool(sz[4]), free_rows_dropped=sz[5],
-                   objective_nonzeros=sz[6], fixed_format=bool(sz[7]), hessian=q,
-                   objective_offset=ref.ref_mps_offset(h), name=ref.ref_mps_name(h).decode(),
-                   row_names=[ref.ref_mps_row_name(h, i).decode() for i in range(rows)],
-                   col_names=[ref.ref_mps_col_name(h, j).decode() for j in range(cols)])
-        return out
+        return _ref_mps_export(h)
     finally:
         ref.ref_mps_free(h)
+
+
+PRESOLVE_COUNTS = ("rows_removed", "cols_removed", "nonzeros_removed", "empty_rows",
+                   "singleton_rows", "redundant_rows", "forcing_rows", "duplicate_rows",
+                   "fixed_columns", "empty_columns", "free_column_singletons",
+                   "slack_column_singletons", "inequality_column_singletons",
+                   "slack_singletons_declined", "doubleton_equations", "dual_fixed_columns",
+                   "coefficients_tightened", "bounds_tightened", "bounds_made_finite", "rounds")
+PRESOLVE_FLAGS = ("empty_rows", "singleton_rows", "redundant_rows", "forcing_rows",
+                  "duplicate_rows", "fixed_columns", "empty_columns", "free_column_singletons",
+                  "bound_tightening", "slack_column_singletons", "inequality_column_singletons",
+                  "dual_fixing", "doubleton_equations", "coefficient_tightening")
+
+
+class RefPresolve:
+    """The reference's presolve (presolve.cpp:1215-1317) of an MPS text, read
+    by the reference's reader; keyword options as PresolveOptions."""
+
+    def __init__(self, text, **opts):
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        defaults = dict.fromkeys(PRESOLVE_FLAGS, 1)
+        defaults["coefficient_tightening"] = 0
+        flags = (c_int32 * 15)(*[int(opts.get(f, defaults[f])) for f in PRESOLVE_FLAGS],
+                               int(opts.get("max_rounds", 30)))
+        tols = (c_double * 4)(opts.get("feasibility_tolerance", 1e-9),
+                              opts.get("infeasibility_tolerance", 1e-7),
+                              opts.get("bound_relaxation", 0.0),
+                              opts.get("max_new_finite_bound", 1e7))
+        self.h = ref.ref_presolve_mps(data, len(data), flags, tols)
+        if not self.h:
+            raise RuntimeError(ref.ref_last_error().decode())
+        self.status = {0: "reduced", 1: "infeasible", 2: "unbounded"}[ref.ref_presolve_status(self.h)]
+        self.message = ref.ref_presolve_message(self.h).decode()
+        self.dual_exact = bool(ref.ref_presolve_dual_exact(self.h))
+        counts = (ctypes.c_int64 * 20)()
+        largest = c_double()
+        ref.ref_presolve_counts(self.h, counts, byref(largest))
+        self.counts = dict(zip(PRESOLVE_COUNTS, counts))
+        self.counts["largest_new_finite_bound"] = largest.value
+        self.reduced = (_ref_mps_export(ref.ref_presolve_reduced(self.h))
+                        if self.status == "reduced" else None)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref.ref_presolve_free(self.h)
+            self.h = None
+
+    def postsolve_primal(self, rx):
+        rx = f64(rx)
+        x = np.zeros(max(1, len(self.reduced["objective"]) + self.counts["cols_removed"]))
+        ref.ref_postsolve_primal(self.h, _dp(rx), rx.size, _dp(x))
+        return x[:len(self.reduced["objective"]) + self.counts["cols_removed"]]
+
+    def postsolve_dual(self, ry, rc, oc, rows):
+        ry, rc, oc = f64(ry), f64(rc), f64(oc)
+        y = np.zeros(max(1, rows))
+        ref.ref_postsolve_dual(self.h, _dp(ry), ry.size, _dp(rc), rc.size, _dp(oc), oc.size, _dp(y))
+        return y[:rows]
+
+    def lp(self):
+        return RefLp(ref.ref_presolve_lp(self.h))
+
+
+def _ref_mps_export(h) -> dict:
+    """Every field of the model behind a reference MpsReadResult handle."""
+    out = {"ok": bool(ref.ref_mps_ok(h)), "error": ref.ref_mps_error(h).decode()}
+    sz = (ctypes.c_int64 * 9)()
+    ref.ref_mps_sizes(h, sz)
+    rows, cols, nnz, qnnz = sz[0], sz[1], sz[2], sz[3]
+    out["warnings"] = [ref.ref_mps_warning(h, k).decode() for k in range(sz[8])]
+    if not out["ok"]:
+        return out
+    a = {"start": np.zeros(rows + 1, np.int32), "index": np.zeros(nnz, np.int32),
+         "value": np.zeros(nnz), "objective": np.zeros(cols), "row_lower": np.zeros(rows),
+         "row_upper": np.zeros(rows), "col_lower": np.zeros(cols), "col_upper": np.zeros(cols),
+         "col_type": np.zeros(cols, np.int32)}
+    q = (np.zeros(cols + 1, np.int32), np.zeros(qnnz, np.int32), np.zeros(qnnz)) if qnnz else None
+
+    def p(x, t):
+        return x.ctypes.data_as(t) if x is not None and x.size else None
+    ref.ref_mps_arrays(h, p(a["start"], IP), p(a["index"], IP), p(a["value"], DP),
+                       p(q[0], IP) if q else None, p(q[1], IP) if q else None,
+                       p(q[2], DP) if q else None, p(a["objective"], DP),
+                       p(a["row_lower"], DP), p(a["row_upper"], DP), p(a["col_lower"], DP),
+                       p(a["col_upper"], DP), p(a["col_type"], IP))
+    out.update(a)
+    out.update(rows=rows, cols=cols, maximize=bool(sz[4]), free_rows_dropped=sz[5],
+               objective_nonzeros=sz[6], fixed_format=bool(sz[7]), hessian=q,
+               objective_offset=ref.ref_mps_offset(h), name=ref.ref_mps_name(h).decode(),
+               row_names=[ref.ref_mps_row_name(h, i).decode() for i in range(rows)],
+               col_names=[ref.ref_mps_col_name(h, j).decode() for j in range(cols)])
+    return out
 
 
 class RefLp:

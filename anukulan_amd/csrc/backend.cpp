@@ -1,6 +1,7 @@
 #include "backend.hpp"
 
 #include <algorithm>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -311,21 +312,41 @@ const DevCsr& HipBackend::prepare_panels(const CsrView& a) {
     bound[uz(p)] = std::min(std::max(bound[uz(p)], bound[uz(p) - 1]), a.cols);
   }
   // Row r of panel p: the part of row r with columns in [bound[p], bound[p+1]).
+  // The per-row cuts, then each panel's row starts and entries, are
+  // independent, so they are built on kPanels threads.
   const Int rows = a.rows;
   std::vector<Int> start(uz(kPanels) * uz(rows) + 1, 0);
   std::vector<Int> cut(uz(rows) * (kPanels + 1));
-  for (Int r = 0; r < rows; ++r) {
-    const Int* b = a.index + a.start[r];
-    const Int* e = a.index + a.start[r + 1];
-    for (int p = 0; p <= kPanels; ++p)
-      cut[uz(r) * (kPanels + 1) + uz(p)] =
-          static_cast<Int>(std::lower_bound(b, e, bound[uz(p)]) - a.index);
-    cut[uz(r) * (kPanels + 1) + kPanels] = a.start[r + 1];
+  auto on_threads = [](int count, const auto& body) {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < count; ++t) pool.emplace_back(body, t);
+    body(0);
+    for (std::thread& th : pool) th.join();
+  };
+  on_threads(kPanels, [&](int t) {
+    const Int r0 = static_cast<Int>(static_cast<long>(rows) * t / kPanels);
+    const Int r1 = static_cast<Int>(static_cast<long>(rows) * (t + 1) / kPanels);
+    for (Int r = r0; r < r1; ++r) {
+      const Int* b = a.index + a.start[r];
+      const Int* e = a.index + a.start[r + 1];
+      for (int p = 0; p <= kPanels; ++p)
+        cut[uz(r) * (kPanels + 1) + uz(p)] =
+            static_cast<Int>(std::lower_bound(b, e, bound[uz(p)]) - a.index);
+      cut[uz(r) * (kPanels + 1) + kPanels] = a.start[r + 1];
+    }
+  });
+  // Where each panel's entries begin: the sum of the earlier panels' sizes.
+  std::vector<long> panel_at(kPanels + 1, 0);
+  for (int p = 0; p < kPanels; ++p) {
+    long n = 0;
+    for (Int r = 0; r < rows; ++r)
+      n += cut[uz(r) * (kPanels + 1) + uz(p) + 1] - cut[uz(r) * (kPanels + 1) + uz(p)];
+    panel_at[uz(p) + 1] = panel_at[uz(p)] + n;
   }
-  size_t at = 0;
   std::vector<Int> index(uz(nnz));
   std::vector<double> value(uz(nnz));
-  for (int p = 0; p < kPanels; ++p) {
+  on_threads(kPanels, [&](int p) {
+    size_t at = static_cast<size_t>(panel_at[uz(p)]);
     for (Int r = 0; r < rows; ++r) {
       const Int k0 = cut[uz(r) * (kPanels + 1) + uz(p)], k1 = cut[uz(r) * (kPanels + 1) + uz(p) + 1];
       for (Int k = k0; k < k1; ++k) {
@@ -335,7 +356,7 @@ const DevCsr& HipBackend::prepare_panels(const CsrView& a) {
       }
       start[uz(p) * uz(rows) + uz(r) + 1] = static_cast<Int>(at);
     }
-  }
+  });
   const CsrView stacked{key, kPanels * rows, a.cols, start.data(), index.data(), value.data()};
   prepare(stacked);
   // The panel stack only ever runs in the fast order: give it that plan.

@@ -4,6 +4,7 @@
 #include <atomic>
 #include <cmath>
 #include <stdexcept>
+#include <thread>
 
 namespace anuk {
 
@@ -97,12 +98,36 @@ Csr Csr::transpose() const {
   return t;
 }
 
+// Each row's sum is serial in storage order whichever thread computes it, so
+// splitting the rows over threads leaves every bit as it is.
 void Csr::multiply(const double* x, double* y) const {
-  for (Int r = 0; r < rows; ++r) {
-    double s = 0.0;
-    for (Int k = start[uz(r)]; k < start[uz(r) + 1]; ++k) s += value[uz(k)] * x[index[uz(k)]];
-    y[r] = s;
+  auto rows_in = [&](Int r0, Int r1) {
+    for (Int r = r0; r < r1; ++r) {
+      double s = 0.0;
+      for (Int k = start[uz(r)]; k < start[uz(r) + 1]; ++k) s += value[uz(k)] * x[index[uz(k)]];
+      y[r] = s;
+    }
+  };
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const Int threads = static_cast<Int>(std::min(16u, hw));
+  if (nnz() < (Int{1} << 20) || threads < 2 || rows < 2 * threads) {
+    rows_in(0, rows);
+    return;
   }
+  // Row ranges of about equal nonzero counts.
+  std::vector<Int> cut(uz(threads) + 1, rows);
+  cut[0] = 0;
+  const double per = static_cast<double>(nnz()) / threads;
+  Int r = 0;
+  for (Int t = 1; t < threads; ++t) {
+    const double target = per * t;
+    while (r < rows && static_cast<double>(start[uz(r)]) < target) ++r;
+    cut[uz(t)] = r;
+  }
+  std::vector<std::thread> pool;
+  for (Int t = 1; t < threads; ++t) pool.emplace_back(rows_in, cut[uz(t)], cut[uz(t) + 1]);
+  rows_in(cut[0], cut[1]);
+  for (std::thread& th : pool) th.join();
 }
 
 bool Csr::validate(std::string* error) const {

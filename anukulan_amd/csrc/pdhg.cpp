@@ -122,7 +122,10 @@ class Solver {
   const StandardLp& orig_;
   const anuk_pdhg_options& opt_;
   SolveResult res_;
-  StandardLp lp_;
+  // The LP iterated on: the caller's own when no sweep scales it, else the
+  // scaled copy.
+  StandardLp scaled_;
+  const StandardLp* lp_ = nullptr;
   Scaling scaling_;
   Clock::time_point start_, phase_mark_;
   Int n_ = 0, m_ = 0;
@@ -193,9 +196,9 @@ Solver::~Solver() {
   if (h_pinned_ != nullptr) (void)hipHostFree(h_pinned_);
   delete scales_;
   --g_depth;
-  if (outermost_) {
-    be_.release_one(lp_.k.id);
-    be_.release_one(lp_.kt.id);
+  if (outermost_ && lp_ != nullptr) {
+    be_.release_one(lp_->k.id);
+    be_.release_one(lp_->kt.id);
   }
 }
 
@@ -260,7 +263,7 @@ void Solver::allocate_device() {
   panel_sums_ = take();
   prob_.n = n_;
   prob_.m = m_;
-  prob_.num_equalities = lp_.num_equalities;
+  prob_.num_equalities = lp_->num_equalities;
   prob_.c = c; prob_.lower = lo; prob_.upper = hi; prob_.q = q;
   prob_.orig_c = oc; prob_.orig_lower = olo; prob_.orig_upper = ohi; prob_.orig_q = oq;
   prob_.row_scale = rs; prob_.col_scale = cs;
@@ -271,10 +274,10 @@ void Solver::upload_problem(const std::vector<double>& x0, const std::vector<dou
   auto up = [&](const double* dst, const std::vector<double>& v) {
     be_.upload(v.data(), const_cast<double*>(dst), static_cast<Int>(v.size()));
   };
-  up(prob_.c, lp_.c);
-  up(prob_.lower, lp_.lower);
-  up(prob_.upper, lp_.upper);
-  up(prob_.q, lp_.q);
+  up(prob_.c, lp_->c);
+  up(prob_.lower, lp_->lower);
+  up(prob_.upper, lp_->upper);
+  up(prob_.q, lp_->q);
   up(prob_.orig_c, orig_.c);
   up(prob_.orig_lower, orig_.lower);
   up(prob_.orig_upper, orig_.upper);
@@ -301,13 +304,15 @@ SolveResult Solver::run() {
   ++g_depth;
   n_ = orig_.cols();
   m_ = orig_.rows();
-  lp_ = orig_;
   const bool scales = opt_.ruiz_iterations > 0 || opt_.pock_chambolle != 0;
   if (scales) {
-    scaling_ = lp_.k.nnz() >= kDeviceScalingNnz
-                   ? scale_lp_device(be_, &lp_, opt_.ruiz_iterations, opt_.pock_chambolle != 0)
-                   : scale_lp(&lp_, opt_.ruiz_iterations, opt_.pock_chambolle != 0);
+    scaled_ = orig_;
+    lp_ = &scaled_;
+    scaling_ = scaled_.k.nnz() >= kDeviceScalingNnz
+                   ? scale_lp_device(be_, &scaled_, opt_.ruiz_iterations, opt_.pock_chambolle != 0)
+                   : scale_lp(&scaled_, opt_.ruiz_iterations, opt_.pock_chambolle != 0);
   } else {
+    lp_ = &orig_;
     // No sweep changes nothing: keep K, K^T and their identities, so a
     // polishing sub-solve reuses the enclosing solve's device copies.
     scaling_.row_scale.assign(uz(m_), 1.0);
@@ -315,17 +320,17 @@ SolveResult Solver::run() {
   }
   res_.phases.scaling = phase_end();
 
-  dk_ = be_.prepare(view_of(lp_.k));
-  dkt_ = be_.prepare(view_of(lp_.kt));
+  dk_ = be_.prepare(view_of(lp_->k));
+  dkt_ = be_.prepare(view_of(lp_->kt));
   // The fast product order by default only where it pays: 4M nonzeros and up.
-  fast_ = opt_.product_order == 2 || (opt_.product_order == 0 && lp_.k.nnz() >= (Int{1} << 22));
+  fast_ = opt_.product_order == 2 || (opt_.product_order == 0 && lp_->k.nnz() >= (Int{1} << 22));
   panels_ = fast_ && static_cast<double>(n_) * sizeof(double) > kPanelVectorBytes && m_ > 0;
-  if (panels_) dpk_ = be_.prepare_panels(view_of(lp_.k));
-  if (fast_) dkt_fast_ = be_.prepare_fast(view_of(lp_.kt));
-  if (fast_ && !panels_) dk_fast_ = be_.prepare_fast(view_of(lp_.k));
+  if (panels_) dpk_ = be_.prepare_panels(view_of(lp_->k));
+  if (fast_) dkt_fast_ = be_.prepare_fast(view_of(lp_->kt));
+  if (fast_ && !panels_) dk_fast_ = be_.prepare_fast(view_of(lp_->k));
 
   matrix_norm_ = opt_.known_matrix_norm > 0.0 ? opt_.known_matrix_norm
-                                             : estimate_matrix_norm(be_, lp_, 200, 1e-8);
+                                             : estimate_matrix_norm(be_, *lp_, 200, 1e-8);
   res_.phases.matrix_norm = phase_end();
   res_.matrix_norm_estimate = matrix_norm_;
   if (matrix_norm_ <= 0.0) {
@@ -354,7 +359,7 @@ SolveResult Solver::run() {
   if (opt_.constant_step_size) eta_ = opt_.constant_step_scale / matrix_norm_;
   omega_ = 1.0;
   if (opt_.primal_weight_updates) {
-    const double cn = host_two_norm(lp_.c), qn = host_two_norm(lp_.q);
+    const double cn = host_two_norm(lp_->c), qn = host_two_norm(lp_->q);
     if (cn > 1e-12 && qn > 1e-12) omega_ = cn / qn;
   }
   if (opt_.initial_step_size > 0.0) eta_ = opt_.initial_step_size;
@@ -367,11 +372,11 @@ SolveResult Solver::run() {
   if (warm) {
     x.assign(opt_.warm_x, opt_.warm_x + n_);
     y.assign(opt_.warm_y, opt_.warm_y + m_);
-    for (Int i = lp_.num_equalities; i < m_; ++i)
+    for (Int i = lp_->num_equalities; i < m_; ++i)
       if (!(y[uz(i)] > 0.0)) y[uz(i)] = 0.0;
   }
   for (Int j = 0; j < n_; ++j) {
-    const double lo = lp_.lower[uz(j)], hi = lp_.upper[uz(j)];
+    const double lo = lp_->lower[uz(j)], hi = lp_->upper[uz(j)];
     double v = warm ? x[uz(j)] : 0.0;
     if (!std::isfinite(v)) v = 0.0;
     if (v < lo) v = std::isfinite(lo) ? lo : 0.0;
@@ -397,7 +402,7 @@ SolveResult Solver::run() {
   halpern_ = opt_.halpern && opt_.restarts;
   reflection_ = halpern_ ? opt_.reflection : 0.0;
   if (!opt_.restart_on_fixed_point && opt_.restarts)
-    kkt_at_restart_ = weighted_kkt_error(lp_, x, y, omega_);
+    kkt_at_restart_ = weighted_kkt_error(*lp_, x, y, omega_);
   next_polish_ = opt_.polish_first_iteration;
   res_.phases.setup = phase_end();
 
@@ -431,7 +436,7 @@ void Solver::enqueue_steps(int steps, bool has_check, int parity) {
   // Algorithmic bytes per launch: the matrix once (8 B value + 4 B index per
   // nonzero, 4 B per row start), the gathered vector once, and the seven
   // vectors the epilogue reads and writes (eight or nine on the check step).
-  const double nnz = static_cast<double>(lp_.k.nnz()), n = n_, m = m_;
+  const double nnz = static_cast<double>(lp_->k.nnz()), n = n_, m = m_;
   for (int step = 0; step < steps; ++step) {
     const bool diag = has_check && step == steps - 1;
     const unsigned flags = diag ? kStepDiag : 0u;
@@ -501,7 +506,7 @@ void Solver::run_period(int steps, bool has_check) {
   P.halpern = halpern_ ? 1 : 0;
   P.reset_at_start = anchor_needs_reset_ ? 1 : 0;
   P.k_at_start = halpern_k_;
-  P.num_equalities = lp_.num_equalities;
+  P.num_equalities = lp_->num_equalities;
   std::memcpy(h_pinned_ + 32, &P, sizeof(P));
   hip_check(hipMemcpyAsync(d_params_, h_pinned_ + 32, sizeof(P), hipMemcpyHostToDevice, be_.stream()),
             "upload step scalars");
@@ -565,7 +570,7 @@ void Solver::loop_fused() {
     const bool need_kkt = (opt_.restarts && !opt_.restart_on_fixed_point) || opt_.verbose;
     if (need_kkt) {
       pull_iterate();
-      kkt = weighted_kkt_error(lp_, host_x_, host_y_, omega_);
+      kkt = weighted_kkt_error(*lp_, host_x_, host_y_, omega_);
     }
     const bool stop = after_check(i, r, farkas, kkt, false);
     res_.phases.checks += phase_end();
@@ -590,7 +595,7 @@ void Solver::loop_general() {
       be_.primal_update(n_, tau, reflection_, w, xb_[p], prob_.c, kt_y_, prob_.lower, prob_.upper,
                         blend ? anchor_x_ : anchor_null, xb_[1 - p], dx_, x_bar_);
       launch_spmv(be_.stream(), dk_, x_bar_, k_x_bar_);
-      be_.dual_update(m_, lp_.num_equalities, sigma, reflection_, w, yb_[p], prob_.q, k_x_bar_,
+      be_.dual_update(m_, lp_->num_equalities, sigma, reflection_, w, yb_[p], prob_.q, k_x_bar_,
                       kxb_[p], blend ? anchor_y_ : anchor_null, blend ? anchor_kx_ : anchor_null,
                       yb_[1 - p], dy_, kxb_[1 - p], k_dx_);
       if (!adaptive_) {
@@ -648,13 +653,13 @@ void Solver::loop_general() {
     if (opt_.detect_infeasibility) {
       host_dy.resize(uz(m_));
       be_.download(dy_, host_dy.data(), m_);
-      farkas = host_farkas(lp_, host_dy, opt_.infeasibility_tolerance);
+      farkas = host_farkas(*lp_, host_dy, opt_.infeasibility_tolerance);
     }
     const bool need_kkt = (opt_.restarts && !halpern_) ||
                           (opt_.restarts && !opt_.restart_on_fixed_point) || opt_.verbose;
     const bool need_average = opt_.restarts && !halpern_ && sum_weight_ > 0.0;
     if (need_kkt || need_average) pull_iterate();
-    const double current_kkt = need_kkt ? weighted_kkt_error(lp_, host_x_, host_y_, omega_) : 0.0;
+    const double current_kkt = need_kkt ? weighted_kkt_error(*lp_, host_x_, host_y_, omega_) : 0.0;
     bool use_average = false;
     double candidate_kkt = current_kkt;
     if (need_average) {
@@ -664,7 +669,7 @@ void Solver::loop_general() {
       host_avg_y_.resize(uz(m_));
       be_.download(avg_x_, host_avg_x_.data(), n_);
       be_.download(avg_y_, host_avg_y_.data(), m_);
-      const double avg_kkt = weighted_kkt_error(lp_, host_avg_x_, host_avg_y_, omega_);
+      const double avg_kkt = weighted_kkt_error(*lp_, host_avg_x_, host_avg_y_, omega_);
       if (avg_kkt < current_kkt) {
         use_average = true;
         candidate_kkt = avg_kkt;
@@ -830,7 +835,7 @@ void Solver::restart(Int) {
   last_candidate_kkt_ = kInf;
   last_residual_ = kInf;
   residual_at_restart_ = std::sqrt(std::fmax(movement_, 0.0));
-  if (!opt_.restart_on_fixed_point) kkt_at_restart_ = weighted_kkt_error(lp_, host_x_, host_y_, omega_);
+  if (!opt_.restart_on_fixed_point) kkt_at_restart_ = weighted_kkt_error(*lp_, host_x_, host_y_, omega_);
   ++res_.restarts;
 }
 
@@ -839,21 +844,54 @@ void Solver::restart(Int) {
 void Solver::run_polish(const std::vector<double>& sx, const std::vector<double>& sy, Int budget,
                         std::vector<double>* ox, std::vector<double>* oy, Residual* orr,
                         Int* spent) {
+  // The two feasibility problems share this solve's K and K^T: when the
+  // matrices are this solve's own scaled copy they are lent for each
+  // sub-solve's duration (moved in and back, identities with them, so the
+  // device copies are reused); when they are the caller's, each problem holds
+  // a copy. Primal drops the objective, dual the right-hand side and the
+  // finite bound values.
+  const bool lend = lp_ == &scaled_;
   if (!feas_built_) {
-    // Same K (and identity) as this solve; primal drops the objective, dual
-    // drops the right-hand side and the finite bound values.
-    primal_feas_ = lp_;
+    for (StandardLp* f : {&primal_feas_, &dual_feas_}) {
+      if (!lend) {
+        f->k = lp_->k;
+        f->kt = lp_->kt;
+      }
+      f->c = lp_->c;
+      f->q = lp_->q;
+      f->lower = lp_->lower;
+      f->upper = lp_->upper;
+      f->num_equalities = lp_->num_equalities;
+      f->objective_scale = lp_->objective_scale;
+      f->objective_offset = 0.0;
+      f->row_model = lp_->row_model;
+      f->row_sign = lp_->row_sign;
+    }
     std::fill(primal_feas_.c.begin(), primal_feas_.c.end(), 0.0);
-    primal_feas_.objective_offset = 0.0;
-    dual_feas_ = lp_;
     std::fill(dual_feas_.q.begin(), dual_feas_.q.end(), 0.0);
     for (std::size_t j = 0; j < dual_feas_.lower.size(); ++j) {
       if (dual_feas_.lower[j] > -kInf) dual_feas_.lower[j] = 0.0;
       if (dual_feas_.upper[j] < kInf) dual_feas_.upper[j] = 0.0;
     }
-    dual_feas_.objective_offset = 0.0;
     feas_built_ = true;
   }
+  struct Lent {  // moves K and K^T into a feasibility problem and back
+    StandardLp& from;
+    StandardLp& to;
+    bool on;
+    Lent(StandardLp& a, StandardLp& b, bool lend) : from(a), to(b), on(lend) {
+      if (on) {
+        std::swap(from.k, to.k);
+        std::swap(from.kt, to.kt);
+      }
+    }
+    ~Lent() {
+      if (on) {
+        std::swap(from.k, to.k);
+        std::swap(from.kt, to.kt);
+      }
+    }
+  };
   anuk_pdhg_options sub = opt_;
   sub.polish_feasibility = 0;
   sub.polish_on_exit = 0;
@@ -873,12 +911,20 @@ void Solver::run_polish(const std::vector<double>& sx, const std::vector<double>
   sub.termination = ANUK_TERM_PRIMAL_FEASIBILITY;
   sub.warm_x = sx.data();
   sub.warm_y = zy.data();
-  const SolveResult pp = Solver(be_, primal_feas_, sub).run();
+  SolveResult pp;
+  {
+    const Lent lent(scaled_, primal_feas_, lend);
+    pp = Solver(be_, primal_feas_, sub).run();
+  }
 
   sub.termination = ANUK_TERM_DUAL_FEASIBILITY;
   sub.warm_x = zx.data();
   sub.warm_y = sy.data();
-  const SolveResult dp = Solver(be_, dual_feas_, sub).run();
+  SolveResult dp;
+  {
+    const Lent lent(scaled_, dual_feas_, lend);
+    dp = Solver(be_, dual_feas_, sub).run();
+  }
 
   *spent = pp.iterations + dp.iterations;
   scaling_.unscale_primal(pp.x, ox);
@@ -896,7 +942,10 @@ void Solver::finish() {
     pull_iterate();
     scaling_.unscale_primal(host_x_, &res_.x);
     scaling_.unscale_dual(host_y_, &res_.y);
-    res_.residual = evaluate_residual(orig_, res_.x, res_.y);
+    // A polishing sub-solve's caller reads only its point and iteration
+    // count (and a sub-solve never polishes on exit), so the host residual is
+    // computed for the outermost solve only.
+    if (outermost_) res_.residual = evaluate_residual(orig_, res_.x, res_.y);
     const bool worth = opt_.polish_feasibility && opt_.polish_on_exit &&
                        opt_.termination == ANUK_TERM_FULL && status_ != ANUK_PDHG_OPTIMAL &&
                        status_ != ANUK_PDHG_PRIMAL_INFEASIBLE && status_ != ANUK_PDHG_NUMERICAL_ERROR &&

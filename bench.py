@@ -211,6 +211,8 @@ def c1_section(ctx, args):
                         f"seed 20261204), plain PDHG to rel KKT 1e-4",
             "build_s": round(build_s, 2), "spmv": spmv, "fused_steps": steps,
             "solve": {"status": r.status, "iterations": r.iterations,
+                      "polish_attempts": r.polish_attempts,
+                      "polish_iterations": r.polish_iterations, "restarts": r.restarts,
                       "objective": r.objective, "seconds": round(solve_s, 3),
                       "iterations_per_s": round(r.iterations / it_s, 1) if it_s > 0 else None,
                       "phases_s": {k2: round(v, 4) for k2, v in r.phases.items()}}}

@@ -128,6 +128,29 @@ class PdhgResult(ctypes.Structure):
     ]
 
 
+class NodeResult(ctypes.Structure):
+    _fields_ = [("optimal", c_int32), ("proved_infeasible", c_int32), ("bound", c_double),
+                ("relaxation", PdhgResult)]
+
+
+class FinishOptions(ctypes.Structure):
+    _fields_ = [("primal_tolerance", c_double), ("dual_tolerance", c_double),
+                ("pivot_tolerance", c_double), ("max_iterations", c_int32),
+                ("refactor_interval", c_int32), ("crossover", c_int32),
+                ("interior_tolerance", c_double), ("crossover_pivot_tolerance", c_double),
+                ("max_candidates", c_int32), ("crossover_refactor_interval", c_int32),
+                ("dual_weight", c_double), ("ruiz_iterations", c_int32), ("threads", c_int32)]
+
+
+class FinishResult(ctypes.Structure):
+    _fields_ = [(n, c_int32) for n in (
+        "status", "iterations", "phase_one_iterations", "refactorizations", "bland_iterations",
+        "started_warm", "candidates", "pushed", "rejected_small_pivot", "logicals_remaining",
+        "stopped_on_refactorization")] + [
+        ("objective", c_double), ("primal_violation", c_double), ("dual_violation", c_double),
+        ("seconds", c_double), ("message", c_char * 64)]
+
+
 class AdjointOptions(ctypes.Structure):
     _fields_ = [("method", c_int32), ("max_iterations", c_int32), ("tolerance", c_double),
                 ("active_tolerance", c_double), ("primal_weight", c_double)]
@@ -185,6 +208,11 @@ SIGNATURES = {
                                       POINTER(PdhgResult), DP, DP]),
     "anuk_solve_pdhg_multi": (c_int, [c_void_p, POINTER(c_void_p), c_int32, POINTER(PdhgOptions),
                                       POINTER(PdhgResult), DP, DP]),
+    "anuk_solve_node_relaxations": (c_int, [c_void_p, c_void_p, c_int32, DP, DP, POINTER(PdhgOptions),
+                                            POINTER(NodeResult), DP, DP]),
+    "anuk_finish_default_options": (None, [POINTER(FinishOptions)]),
+    "anuk_exact_finish": (c_int, [POINTER(c_void_p), c_int32, DP, DP, POINTER(FinishOptions),
+                                  POINTER(FinishResult), DP, DP, IP]),
     "anuk_adjoint_default_options": (None, [POINTER(AdjointOptions)]),
     "anuk_pdhg_adjoint": (c_int, [c_void_p, POINTER(c_void_p), c_int32, POINTER(PdhgOptions),
                                   POINTER(AdjointOptions), DP, DP, DP, DP, DP, DP, DP, DP,

@@ -17,6 +17,8 @@
 #include "adjoint.hpp"
 #include "batch.hpp"
 #include "multi.hpp"
+#include "nodes.hpp"
+#include "finish.hpp"
 #include "scaling.hpp"
 #include "pdhg.hpp"
 
@@ -613,6 +615,126 @@ int anuk_solve_pdhg_multi(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t coun
       if (y_out && rs[i].y.size() == rows) std::memcpy(y_out + yo, rs[i].y.data(), rows * sizeof(double));
       xo += cols;
       yo += rows;
+    }
+  });
+}
+
+int anuk_solve_node_relaxations(anuk_ctx* ctx, const anuk_lp* lp, int32_t count,
+                                const double* lower, const double* upper,
+                                const anuk_pdhg_options* options, anuk_node_result* results,
+                                double* x_out, double* y_out) {
+  if (ctx == nullptr || lp == nullptr || count < 0 || (count > 0 && (results == nullptr ||
+      lower == nullptr || upper == nullptr)))
+    return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    const anuk_pdhg_options o = checked_options(options);
+    const std::vector<anuk::NodeRelaxation> rs =
+        anuk::solve_node_relaxations(*ctx->be, lp->lp, lower, upper, count, o);
+    const size_t cols = static_cast<size_t>(lp->lp.cols()), rows = static_cast<size_t>(lp->lp.rows());
+    for (size_t i = 0; i < rs.size(); ++i) {
+      anuk_node_result& r = results[i];
+      std::memset(&r, 0, sizeof(r));
+      r.optimal = rs[i].optimal ? 1 : 0;
+      r.proved_infeasible = rs[i].proved_infeasible ? 1 : 0;
+      r.bound = rs[i].bound;
+      fill_result(rs[i].solve, &r.relaxation);
+      if (x_out && rs[i].solve.x.size() == cols)
+        std::memcpy(x_out + i * cols, rs[i].solve.x.data(), cols * sizeof(double));
+      if (y_out && rs[i].solve.y.size() == rows)
+        std::memcpy(y_out + i * rows, rs[i].solve.y.data(), rows * sizeof(double));
+    }
+  });
+}
+
+static anuk::FinishOptions finish_options_of(const anuk_finish_options& o) {
+  anuk::FinishOptions f;
+  f.primal_tolerance = o.primal_tolerance;
+  f.dual_tolerance = o.dual_tolerance;
+  f.pivot_tolerance = o.pivot_tolerance;
+  f.max_iterations = o.max_iterations;
+  f.refactor_interval = o.refactor_interval;
+  f.crossover = o.crossover;
+  f.interior_tolerance = o.interior_tolerance;
+  f.crossover_pivot_tolerance = o.crossover_pivot_tolerance;
+  f.max_candidates = o.max_candidates;
+  f.crossover_refactor_interval = o.crossover_refactor_interval;
+  f.dual_weight = o.dual_weight;
+  f.ruiz_iterations = o.ruiz_iterations;
+  f.threads = o.threads;
+  return f;
+}
+
+void anuk_finish_default_options(anuk_finish_options* options) {
+  if (options == nullptr) return;
+  const anuk::FinishOptions f;
+  options->primal_tolerance = f.primal_tolerance;
+  options->dual_tolerance = f.dual_tolerance;
+  options->pivot_tolerance = f.pivot_tolerance;
+  options->max_iterations = f.max_iterations;
+  options->refactor_interval = f.refactor_interval;
+  options->crossover = f.crossover;
+  options->interior_tolerance = f.interior_tolerance;
+  options->crossover_pivot_tolerance = f.crossover_pivot_tolerance;
+  options->max_candidates = f.max_candidates;
+  options->crossover_refactor_interval = f.crossover_refactor_interval;
+  options->dual_weight = f.dual_weight;
+  options->ruiz_iterations = f.ruiz_iterations;
+  options->threads = f.threads;
+}
+
+int anuk_exact_finish(const anuk_lp* const* lps, int32_t count, const double* x, const double* y,
+                      const anuk_finish_options* options, anuk_finish_result* results,
+                      double* x_out, double* y_out, int32_t* basic_out) {
+  if (lps == nullptr || count < 0 || (count > 0 && results == nullptr))
+    return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  return guarded(nullptr, [&] {
+    anuk_finish_options o;
+    anuk_finish_default_options(&o);
+    if (options != nullptr) o = *options;
+    need(o.primal_tolerance > 0.0 && o.dual_tolerance > 0.0 && o.pivot_tolerance > 0.0,
+         "finish tolerances must be positive");
+    need(o.max_iterations >= 0 && o.refactor_interval > 0 && o.ruiz_iterations >= 0 && o.threads >= 0,
+         "finish limits must be non-negative (refactor_interval positive)");
+    std::vector<const anuk::StandardLp*> v;
+    std::vector<const double*> xs, ys;
+    size_t co = 0, ro = 0;
+    for (int32_t i = 0; i < count; ++i) {
+      need(lps[i] != nullptr, "null LP among those to finish");
+      v.push_back(&lps[i]->lp);
+      xs.push_back(x ? x + co : nullptr);
+      ys.push_back(y ? y + ro : nullptr);
+      co += static_cast<size_t>(lps[i]->lp.cols());
+      ro += static_cast<size_t>(lps[i]->lp.rows());
+    }
+    const std::vector<anuk::FinishResult> rs = anuk::exact_finish_many(v, xs, ys, finish_options_of(o));
+    co = ro = 0;
+    for (size_t i = 0; i < rs.size(); ++i) {
+      const anuk::FinishResult& r = rs[i];
+      anuk_finish_result& o2 = results[i];
+      std::memset(&o2, 0, sizeof(o2));
+      o2.status = r.status;
+      o2.iterations = r.iterations;
+      o2.phase_one_iterations = r.phase_one_iterations;
+      o2.refactorizations = r.refactorizations;
+      o2.bland_iterations = r.bland_iterations;
+      o2.started_warm = r.started_warm ? 1 : 0;
+      o2.candidates = r.crossover.candidates;
+      o2.pushed = r.crossover.pushed;
+      o2.rejected_small_pivot = r.crossover.rejected_small_pivot;
+      o2.logicals_remaining = r.crossover.logicals_remaining;
+      o2.stopped_on_refactorization = r.crossover.stopped_on_refactorization ? 1 : 0;
+      o2.objective = r.objective;
+      o2.primal_violation = r.primal_violation;
+      o2.dual_violation = r.dual_violation;
+      o2.seconds = r.seconds;
+      std::strncpy(o2.message, r.message.c_str(), sizeof(o2.message) - 1);
+      const size_t cols = static_cast<size_t>(v[i]->cols()), rows = static_cast<size_t>(v[i]->rows());
+      if (x_out && r.x.size() == cols) std::memcpy(x_out + co, r.x.data(), cols * sizeof(double));
+      if (y_out && r.y.size() == rows) std::memcpy(y_out + ro, r.y.data(), rows * sizeof(double));
+      if (basic_out && r.basic.size() == rows)
+        for (size_t k = 0; k < rows; ++k) basic_out[ro + k] = r.basic[k];
+      co += cols;
+      ro += rows;
     }
   });
 }

@@ -11,7 +11,10 @@ events the dispatch records (hipExtLaunchKernel) and divides each launch's
 algorithmic bytes by its time (`roofline`, dominant kernel), the
 1M-column random LP's SpMV bandwidth and plain solve (`c1`), the uneven
 512-LP stack solved in one set of launches (`c3`: LPs/s, p50/p99
-time-to-solve), and the
+time-to-solve), a 64-node branch-and-bound frontier's relaxations solved
+in one batch (`nodes`: nodes/s against plain and reference solves), the
+stack finished exactly (`finish`: batched PDHG to 1e-8, then crossover and
+simplex per case on the host cores, against the reference simplex), and the
 reference's own solve_pdhg (oracle/_ref, one thread) on a bounded sample of
 the cases (`cpu_baseline`), and the same on every case, one case per thread
 over the host's cores (`cpu_baseline_all_cores`).
@@ -259,6 +262,136 @@ def c3_section(ctx, args):
                              "max_rel_objective_diff": worst}}
 
 
+def nodes_section(ctx, args):
+    """Branch-and-bound node relaxations: the 64 nodes of the depth-6
+    frontier of the 12-period refinery plan (crude purchases and product
+    sales integer; branched on the six most fractional of them at the root
+    optimum, the recipe in tests/golden/nodes_reference.json), each solved
+    as the reference tree's first-order node solver solves it (default
+    options),
+    all 64 in one batch; beside it the plain GPU solver one node at a time
+    and the reference's solve_pdhg (oracle/_ref, one thread), each on a
+    bounded sample of the nodes."""
+    import numpy as np
+    from anukulan_amd import LP, generators as G, solve_pdhg
+    from anukulan_amd.nodes import branch_frontier, solve_node_relaxations
+
+    with open(os.path.join(ROOT, "tests", "golden", "nodes_reference.json")) as f:
+        fx = json.load(f)["frontiers"]["refinery12_depth6"]
+    lp = G.refinery_model(fx["periods"]).to_lp()
+    e = lp.export()
+    lo, hi = branch_frontier(e["lower"], e["upper"], fx["columns"],
+                             [float.fromhex(v) for v in fx["values"]])
+    solve_node_relaxations(ctx, lp, lo[:2], hi[:2])  # warm-up (tuner, graphs)
+    t0 = time.perf_counter()
+    got = solve_node_relaxations(ctx, lp, lo, hi)
+    batched_s = time.perf_counter() - t0
+    want = fx["nodes"]
+    verdicts = sum((g.optimal, g.proved_infeasible) == (w["optimal"], w["proved_infeasible"])
+                   for g, w in zip(got, want))
+    worst = max(abs(g.bound - float.fromhex(w["bound"])) / max(1.0, abs(float.fromhex(w["bound"])))
+                for g, w in zip(got, want) if w["optimal"])
+
+    def node_lp(i):
+        return LP.from_standard(lp.rows, lp.cols, e["num_equalities"], e["start"], e["index"],
+                                e["value"], e["c"], e["q"], lo[i], hi[i], e["objective_scale"],
+                                e["objective_offset"])
+
+    plain_n, plain_s = 0, 0.0
+    while plain_n < len(want) and plain_s < args.nodes_plain_seconds:
+        nl = node_lp(plain_n)
+        t1 = time.perf_counter()
+        solve_pdhg(ctx, nl)
+        plain_s += time.perf_counter() - t1
+        plain_n += 1
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        try:
+            import _oracle
+            ok = _oracle.ref is not None
+        except Exception:
+            ok = False
+        if ok:
+            cpu_n, cpu_s = 0, 0.0
+            while cpu_n < len(want) and cpu_s < args.nodes_cpu_seconds:
+                R = _oracle.RefLp.from_export(dict(e, lower=lo[cpu_n], upper=hi[cpu_n]))
+                t1 = time.perf_counter()
+                R.solve(threads=1)
+                cpu_s += time.perf_counter() - t1
+                cpu_n += 1
+            cpu = {"value": round(cpu_n / cpu_s, 3), "unit": "nodes/s", "cores": 1,
+                   "kind": "reference", "sample": f"first {cpu_n} nodes of the frontier"}
+    value = len(want) / batched_s
+    return {"workload": f"depth-{fx['depth']} branch-and-bound frontier of the "
+                        f"{fx['periods']}-period refinery plan ({len(want)} node relaxations "
+                        f"sharing K, default options: rel KKT 1e-6)",
+            "rows": lp.rows, "cols": lp.cols, "nnz": lp.nnz,
+            "nodes_per_s": round(value, 2), "seconds": round(batched_s, 3),
+            "iterations": int(sum(g.relaxation.iterations for g in got)),
+            "verdicts_as_reference": f"{verdicts}/{len(want)}",
+            "max_rel_bound_diff_vs_reference": worst,
+            "plain_sample": {"nodes": plain_n, "seconds": round(plain_s, 3),
+                             "nodes_per_s": round(plain_n / plain_s, 2) if plain_s > 0 else None},
+            "cpu_baseline": cpu,
+            "batched_vs_plain": round(value / (plain_n / plain_s), 2) if plain_s > 0 else None,
+            "batched_vs_cpu": round(value / cpu["value"], 2) if cpu else None}
+
+
+def finish_section(ctx, args, base, cases, lps, optima):
+    """Exact finish of the case stack: every case solved by the batched
+    device solver to rel KKT 1e-8, then finished to an optimal basis on the
+    host cores (crossover of the first-order point, then the bounded primal
+    simplex from it), one case per thread; beside it the same finish from
+    all logicals (cold) and the reference's simplex (oracle/_ref, one
+    thread) on a bounded sample. Error is against the reference simplex's
+    FP64 optima."""
+    from anukulan_amd import solve_pdhg_batch
+    from anukulan_amd.finish import exact_finish
+
+    t0 = time.perf_counter()
+    pts = solve_pdhg_batch(ctx, lps, tolerance=1e-8)
+    pdhg_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    warm = exact_finish(lps, [p.x for p in pts], [p.y for p in pts])
+    warm_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cold = exact_finish(lps)
+    cold_s = time.perf_counter() - t0
+    err = [abs(w.objective - o) / max(1.0, abs(o)) for w, o in zip(warm, optima)] if optima else []
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        try:
+            import _oracle
+            ok = _oracle.ref is not None
+        except Exception:
+            ok = False
+        if ok:
+            from anukulan_amd import generators as G
+            n_done, spent = 0, 0.0
+            while n_done < len(cases) and spent < args.finish_cpu_seconds:
+                R = _oracle.RefLp.from_natural(G.case_model(base, cases[n_done]))
+                t1 = time.perf_counter()
+                R.simplex()
+                spent += time.perf_counter() - t1
+                n_done += 1
+            cpu = {"value": round(n_done / spent, 3), "unit": "cases/s", "cores": 1,
+                   "kind": "reference", "sample": f"reference simplex on the first {n_done} cases"}
+    total = pdhg_s + warm_s
+    threads = min(16, os.cpu_count() or 1)
+    return {"workload": f"{len(lps)}-case stack: batched PDHG to rel KKT 1e-8, then crossover "
+                        f"and primal simplex per case on up to {threads} host threads",
+            "cases_per_s": round(len(lps) / total, 3), "seconds": round(total, 3),
+            "pdhg_s": round(pdhg_s, 3), "finish_s": round(warm_s, 3),
+            "cases_optimal": sum(w.status == "optimal" for w in warm),
+            "pivots_warm": sum(w.iterations for w in warm),
+            "pivots_cold": sum(c.iterations for c in cold),
+            "cold_finish_s": round(cold_s, 3),
+            "max_rel_error_vs_simplex": max(err) if err else None,
+            "cpu_baseline": cpu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +415,12 @@ def main():
     ap.add_argument("--c3-count", type=int, default=512)
     ap.add_argument("--c3-plain-seconds", type=float, default=10.0,
                     help="time spent on plain one-at-a-time solves of the uneven stack's LPs")
+    ap.add_argument("--no-finish", action="store_true", help="skip the exact finish of the stack")
+    ap.add_argument("--finish-cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--no-nodes", action="store_true",
+                    help="skip the branch-and-bound node frontier")
+    ap.add_argument("--nodes-plain-seconds", type=float, default=8.0)
+    ap.add_argument("--nodes-cpu-seconds", type=float, default=8.0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB and exit: the "
                          "known-size reference for HBM byte counters")
@@ -368,6 +507,8 @@ def main():
 
     c1 = None if args.no_c1 else c1_section(ctx, args)
     c3 = None if args.no_c3 else c3_section(ctx, args)
+    nodes = None if args.no_nodes else nodes_section(ctx, args)
+    finish = None if args.no_finish else finish_section(ctx, args, base, cases, lps, optima)
 
     cpu = cpu_par = None
     if not args.no_cpu_baseline:
@@ -389,7 +530,7 @@ def main():
                    "parallelism": "single GPU"},
         "error_vs_simplex": error_vs_optimum(results, optima),
         "roofline": roofline, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_par,
-        "plain": plain, "c1": c1, "c3": c3,
+        "plain": plain, "c1": c1, "c3": c3, "nodes": nodes, "finish": finish,
         "batch_phases_s": {k: round(v, 4) for k, v in results[0].phases.items()},
         "autotune": parse_tuning(tuning),
         "kernels": {k: {"mean_us": v["mean_us"], "calls": v["calls"], "GB/s": v["gbs"]}

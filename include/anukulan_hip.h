@@ -406,6 +406,108 @@ int anuk_solve_pdhg_multi(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t coun
                           double* x_out, double* y_out);
 
 /* ------------------------------------------------------------------------ */
+/* Branch-and-bound node relaxations                                          */
+/* ------------------------------------------------------------------------ */
+
+/* NodeSolution (branch_and_bound.cpp:44-57) of a first-order node solve, the
+ * vectors aside: `optimal` when the relaxation converged with its dual
+ * feasible to 1e-7 * max(1, |dual objective|), and then `bound` is the dual
+ * objective (standard form, a lower bound on the node's subtree); otherwise
+ * `bound` is -infinity. `proved_infeasible` when the solve ended with a
+ * Farkas certificate. `relaxation` is the node's own solve. */
+typedef struct anuk_node_result {
+  int32_t optimal;
+  int32_t proved_infeasible;
+  double bound;
+  anuk_pdhg_result relaxation;
+} anuk_node_result;
+
+/* Replaces the first-order branch of the reference tree's solve_node
+ * (branch_and_bound.cpp:528-566, one solve_pdhg per node) for `count` nodes
+ * at once. Node i is `lp` with column bounds lower + i * cols and
+ * upper + i * cols (standard-form columns, as Node::lower/upper,
+ * branch_and_bound.cpp:17-19); the nodes share lp's matrix, so options the
+ * batched loop runs solve them together (anuk_solve_pdhg_batch), and other
+ * options solve them one after another with the plain device solver. Every
+ * node starts cold, as the reference's does (it passes warm_x without
+ * warm_y, which pdhg.cpp:481-484 treats as no warm start): options->warm_x /
+ * warm_y must be NULL. x_out (count * cols) and y_out (count * rows)
+ * receive each node's unscaled primal and dual; either may be NULL. */
+int anuk_solve_node_relaxations(anuk_ctx* ctx, const anuk_lp* lp, int32_t count,
+                                const double* lower, const double* upper,
+                                const anuk_pdhg_options* options, anuk_node_result* results,
+                                double* x_out, double* y_out);
+
+/* ------------------------------------------------------------------------ */
+/* Exact finish: crossover and a warm-started simplex on the host            */
+/* ------------------------------------------------------------------------ */
+
+/* CrossoverOptions (crossover.hpp:37-83) and the simplex's own settings.
+ * Tolerances are in the units of the equilibrated LP the simplex runs on. */
+typedef struct anuk_finish_options {
+  double primal_tolerance;             /* 1e-9 */
+  double dual_tolerance;               /* 1e-9 */
+  double pivot_tolerance;              /* 1e-9: smallest |alpha| the ratio test divides by */
+  int32_t max_iterations;              /* 200000 */
+  int32_t refactor_interval;           /* 100 */
+  int32_t crossover;                   /* 1: start from the point's crossover basis; 0: from
+                                          all logicals (a cold start) */
+  double interior_tolerance;           /* 1e-6 (CrossoverOptions::interior_tolerance) */
+  double crossover_pivot_tolerance;    /* 1e-7 (CrossoverOptions::pivot_tolerance) */
+  int32_t max_candidates;              /* -1: no cap */
+  int32_t crossover_refactor_interval; /* 50 */
+  double dual_weight;                  /* 1.0 */
+  int32_t ruiz_iterations;             /* 10: equilibration sweeps before the simplex */
+  int32_t threads;                     /* 0: one case per host core, at most 16 */
+} anuk_finish_options;
+
+void anuk_finish_default_options(anuk_finish_options* options);
+
+enum anuk_finish_status {
+  ANUK_FINISH_OPTIMAL = 0,
+  ANUK_FINISH_INFEASIBLE = 1,
+  ANUK_FINISH_UNBOUNDED = 2,
+  ANUK_FINISH_ITERATION_LIMIT = 3,
+  ANUK_FINISH_NUMERICAL_ERROR = 4
+};
+
+/* SimplexResult (simplex.hpp:385-431) and CrossoverResult
+ * (crossover.hpp:86-111), the numbers only. */
+typedef struct anuk_finish_result {
+  int32_t status;                      /* anuk_finish_status */
+  int32_t iterations;                  /* simplex pivots and bound flips */
+  int32_t phase_one_iterations;
+  int32_t refactorizations;
+  int32_t bland_iterations;            /* pivots taken under Bland's rule (stalling) */
+  int32_t started_warm;                /* the crossover pushed at least one column */
+  int32_t candidates;                  /* crossover: columns the point has interior */
+  int32_t pushed;                      /* crossover: structurals basic in the handed-over basis */
+  int32_t rejected_small_pivot;
+  int32_t logicals_remaining;
+  int32_t stopped_on_refactorization;
+  double objective;                    /* model sense */
+  double primal_violation;             /* on the LP as given: max row / bound violation */
+  double dual_violation;               /* max reduced-cost sign violation at the basis */
+  double seconds;
+  char message[64];
+} anuk_finish_result;
+
+/* Finishes `count` LPs exactly: for each, the crossover of its point
+ * (crossover_basis, crossover.cpp:35-229, the same candidate, merit and
+ * bound rules) gives a starting basis, and a bounded primal simplex of this
+ * library takes it to an optimal vertex, as the reference does after its
+ * first-order seed (app/anukulan_cli.cpp:592-631). x holds the points end to
+ * end (sum of cols; NULL: every LP starts cold from all logicals), y the
+ * duals end to end (sum of rows; NULL: nonbasic bounds by proximity).
+ * x_out / y_out receive the vertices and row duals end to end, basic_out
+ * (sum of rows) each basis position's column (j < cols: structural j;
+ * cols + i: row i's logical); any may be NULL. Runs on the host, one LP per
+ * thread; it needs no device. */
+int anuk_exact_finish(const anuk_lp* const* lps, int32_t count, const double* x, const double* y,
+                      const anuk_finish_options* options, anuk_finish_result* results,
+                      double* x_out, double* y_out, int32_t* basic_out);
+
+/* ------------------------------------------------------------------------ */
 /* Sensitivities of a solution (no reference counterpart)                    */
 /* ------------------------------------------------------------------------ */
 

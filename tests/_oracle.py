@@ -517,3 +517,16 @@ class RefLp:
         ref.ref_simplex(self.h, _dp(x), _dp(y), byref(obj), byref(st), _ip(basic), byref(cnt))
         return {"status": st.value, "objective": obj.value, "x": x, "y": y,
                 "basic": basic[:cnt.value].copy()}
+
+
+def node_verdict(result) -> dict:
+    """What the reference tree reads off a first-order node solve
+    (branch_and_bound.cpp:533-564): proved infeasible on a Farkas
+    certificate; `optimal`, with the dual objective as the subtree's bound,
+    only when the solve converged and its dual is feasible to
+    1e-7 * max(1, |dual objective|); otherwise the bound stays -inf."""
+    dual_obj = result.residual["dual_objective"]
+    enough = 1e-7 * max(1.0, abs(dual_obj))
+    optimal = result.status == "optimal" and result.residual["dual_residual_inf"] <= enough
+    return {"optimal": bool(optimal), "proved_infeasible": result.status == "primal infeasible",
+            "bound": dual_obj if optimal else -float("inf")}

@@ -604,6 +604,22 @@ __global__ void __launch_bounds__(kThreads)
   d[i] = v > 0.0 ? 1.0 / sqrt(v) : 1.0;
 }
 
+// a[i] += add[l] for the segment l with bounds[l] <= i < bounds[l + 1]
+// (bounds: segments + 1 entries, bounds[0] = 0, bounds[segments] = count).
+__global__ void __launch_bounds__(kThreads)
+    add_segment_offsets_kernel(Int* __restrict__ a, Int count, const Int* __restrict__ bounds,
+                               const Int* __restrict__ add, Int segments) {
+  const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (i >= count) return;
+  Int lo = 0, hi = segments;
+  while (hi - lo > 1) {
+    const Int mid = lo + (hi - lo) / 2;
+    if (bounds[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  a[i] += add[lo];
+}
+
 __global__ void __launch_bounds__(kThreads)
     scale_values_kernel(const Int* __restrict__ start, const Int* __restrict__ index, Int rows,
                         double* __restrict__ value, const double* __restrict__ first,
@@ -1073,6 +1089,14 @@ void launch_inv_sqrt(hipStream_t s, const double* norm, Int n, double* d) {
   if (n <= 0) return;
   hipLaunchKernelGGL(inv_sqrt_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, norm, n, d);
   check_launch("inverse square roots");
+}
+
+void launch_add_segment_offsets(hipStream_t s, Int* a, Int count, const Int* bounds, const Int* add,
+                                Int segments) {
+  if (count <= 0 || segments <= 0) return;
+  hipLaunchKernelGGL(add_segment_offsets_kernel, dim3(grid_for(count)), dim3(kThreads), 0, s, a, count,
+                     bounds, add, segments);
+  check_launch("segment offsets");
 }
 
 void launch_scale_values(hipStream_t s, const Int* start, const Int* index, Int rows,

@@ -226,6 +226,10 @@ void launch_row_norms(hipStream_t s, const Int* start, Int rows, const double* v
                       double* out);
 // d = 1 / sqrt(norm) where norm > 0, else 1.
 void launch_inv_sqrt(hipStream_t s, const double* norm, Int n, double* d);
+// a[i] += add[l] where bounds[l] <= i < bounds[l + 1]: shifts each LP's
+// slice of a concatenated index or row-start array by that LP's offset.
+void launch_add_segment_offsets(hipStream_t s, Int* a, Int count, const Int* bounds, const Int* add,
+                                Int segments);
 // Every entry a[r][k] becomes (a * first) * second, rounding after each
 // product; `first_by_row` says whether `first` is indexed by the row (and
 // `second` by the column) or the other way round.

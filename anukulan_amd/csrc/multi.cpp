@@ -32,7 +32,11 @@ struct Stack {
   Int lps = 0;
   Int n = 0, m = 0;
   std::vector<Int> col_off, row_off;  // lps + 1
-  Csr k, kt;                          // block diagonal, as given (scaled on the device only)
+  std::vector<Int> nz_off;            // lps + 1: where each LP's entries start in K (and K^T)
+  Int nnz = 0;
+  // The LPs themselves: K and K^T are concatenated on the device only
+  // (upload_stack), unscaled, and scaled there.
+  std::vector<const StandardLp*> lp;
   std::vector<Int> eq_end;            // global end row of each LP's equalities
   std::vector<double> row_scale, col_scale;
   std::vector<double> c, q, lower, upper;  // scaled
@@ -71,60 +75,40 @@ void parallel_for(size_t count, F f) {
     if (e) std::rethrow_exception(e);
 }
 
-// Concatenates the LPs: K and K^T block-diagonally, the vectors end to end
-// (each LP's slice written by its own host thread). The matrices stay
-// unscaled here; the scaling runs on the device (scale_stack).
+// Lays the LPs end to end: offsets of each LP's columns, rows and entries,
+// and its vectors copied into the stack's (each LP's slice by its own host
+// thread). K and K^T are not concatenated here: each LP's arrays go straight
+// to their place in the device stack (upload_stack).
 Stack concat_stack(const std::vector<const StandardLp*>& lps) {
   Stack st;
   st.lps = static_cast<Int>(lps.size());
+  st.lp = lps;
   st.scaling.resize(lps.size());
   st.col_off.assign(lps.size() + 1, 0);
   st.row_off.assign(lps.size() + 1, 0);
-  std::vector<size_t> nz_off(lps.size() + 1, 0);
+  st.nz_off.assign(lps.size() + 1, 0);
+  long nz = 0;
   for (size_t i = 0; i < lps.size(); ++i) {
     const long cols = static_cast<long>(st.col_off[i]) + lps[i]->cols();
     const long rows = static_cast<long>(st.row_off[i]) + lps[i]->rows();
-    if (cols > (1L << 30) || rows > (1L << 30)) throw std::invalid_argument("multi: stack too large");
+    nz += lps[i]->k.nnz();
+    if (cols > (1L << 30) || rows > (1L << 30) || nz > (1L << 30))
+      throw std::invalid_argument("multi: stack too large");
     st.col_off[i + 1] = static_cast<Int>(cols);
     st.row_off[i + 1] = static_cast<Int>(rows);
-    nz_off[i + 1] = nz_off[i] + static_cast<size_t>(lps[i]->k.nnz());
+    st.nz_off[i + 1] = static_cast<Int>(nz);
   }
-  const size_t nnz = nz_off.back();
-  if (nnz > static_cast<size_t>(1) << 30) throw std::invalid_argument("multi: stack too large");
   st.n = st.col_off.back();
   st.m = st.row_off.back();
-  for (Csr* out : {&st.k, &st.kt}) {
-    const bool transpose = out == &st.kt;
-    out->rows = transpose ? st.n : st.m;
-    out->cols = transpose ? st.m : st.n;
-    out->start.assign(uz(out->rows) + 1, 0);
-    out->index.resize(nnz);
-    out->value.resize(nnz);
-  }
+  st.nnz = st.nz_off.back();
   for (std::vector<double>* v : {&st.oc, &st.olo, &st.ohi}) v->resize(uz(st.n));
   st.oq.resize(uz(st.m));
   parallel_for(lps.size(), [&](size_t i) {
-    for (int side = 0; side < 2; ++side) {
-      const bool transpose = side == 1;
-      const Csr& a = transpose ? lps[i]->kt : lps[i]->k;
-      Csr& out = transpose ? st.kt : st.k;
-      const Int r0 = transpose ? st.col_off[i] : st.row_off[i];
-      const Int c0 = transpose ? st.row_off[i] : st.col_off[i];
-      const size_t at = nz_off[i];
-      for (Int r = 0; r < a.rows; ++r)
-        out.start[uz(r0 + r) + 1] = static_cast<Int>(at) + a.start[uz(r) + 1];
-      for (Int e = 0; e < a.nnz(); ++e) {
-        out.index[at + uz(e)] = a.index[uz(e)] + c0;
-        out.value[at + uz(e)] = a.value[uz(e)];
-      }
-    }
     std::copy(lps[i]->c.begin(), lps[i]->c.end(), st.oc.begin() + st.col_off[i]);
     std::copy(lps[i]->lower.begin(), lps[i]->lower.end(), st.olo.begin() + st.col_off[i]);
     std::copy(lps[i]->upper.begin(), lps[i]->upper.end(), st.ohi.begin() + st.col_off[i]);
     std::copy(lps[i]->q.begin(), lps[i]->q.end(), st.oq.begin() + st.row_off[i]);
   });
-  st.k.renew_id();
-  st.kt.renew_id();
   for (size_t i = 0; i < lps.size(); ++i) st.eq_end.push_back(st.row_off[i] + lps[i]->num_equalities);
   return st;
 }
@@ -156,6 +140,14 @@ struct DeviceStack {
   DeviceStack& operator=(const DeviceStack&) = delete;
 
   template <class T>
+  T* alloc(size_t n) {
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc(stack)");
+    owned.push_back(p);
+    return static_cast<T*>(p);
+  }
+
+  template <class T>
   const T* put(const std::vector<T>& h) {
     void* p = nullptr;
     hip_check(hipMalloc(&p, std::max<size_t>(h.size(), 1) * sizeof(T)), "hipMalloc(stack)");
@@ -173,16 +165,18 @@ struct Plan {
   std::vector<Int> block_row, block_lp, lp_blocks;
 };
 
-Plan plan_stack(const Csr& a, const std::vector<Int>& off) {
+Plan plan_stack(const Stack& st, bool transpose) {
+  const std::vector<Int>& off = transpose ? st.col_off : st.row_off;
   Plan p;
   p.block_row.push_back(0);
   p.lp_blocks.push_back(0);
   for (size_t l = 0; l + 1 < off.size(); ++l) {
     const Int rows = off[l + 1] - off[l];
     if (rows > 0) {
+      const Csr& a = transpose ? st.lp[l]->kt : st.lp[l]->k;
       std::int32_t* br = nullptr;
       Int blocks = 0;
-      plan_blocks(rows, a.start.data() + off[l], kTileNnz, kTileRows, &br, &blocks);
+      plan_blocks(rows, a.start.data(), kTileNnz, kTileRows, &br, &blocks);
       for (Int b = 1; b <= blocks; ++b) {
         p.block_row.push_back(off[l] + br[b]);
         p.block_lp.push_back(static_cast<Int>(l));
@@ -213,22 +207,49 @@ Chunks chunk_stack(const std::vector<Int>& off) {
 }
 
 void upload_stack(const Stack& st, DeviceStack* ds) {
-  auto dev_csr = [&](const Csr& a, const Plan& p) {
+  hipStream_t s = ds->be.stream();
+  const Int* d_col_off = ds->put(st.col_off);
+  const Int* d_row_off = ds->put(st.row_off);
+  const Int* d_nz_off = ds->put(st.nz_off);
+  // Each LP's CSR arrays are copied to their place in the stacked CSR as
+  // they are, then shifted on the device: row starts by the LP's first
+  // entry, column indices by its first column (row, for K^T).
+  auto dev_csr = [&](bool transpose, const Plan& p) {
     DevCsr d;
-    d.rows = a.rows;
-    d.cols = a.cols;
-    d.nnz = a.nnz();
+    d.rows = transpose ? st.n : st.m;
+    d.cols = transpose ? st.m : st.n;
+    d.nnz = st.nnz;
     d.blocks = static_cast<Int>(p.block_lp.size());
-    d.start = ds->put(a.start);
-    d.index = ds->put(a.index);
-    d.value = ds->put(a.value);
+    const std::vector<Int>& roff = transpose ? st.col_off : st.row_off;
+    Int* start = ds->alloc<Int>(uz(d.rows) + 1);
+    Int* index = ds->alloc<Int>(uz(d.nnz));
+    double* value = ds->alloc<double>(uz(d.nnz));
+    hip_check(hipMemsetAsync(start, 0, sizeof(Int), s), "stack row starts");
+    for (Int l = 0; l < st.lps; ++l) {
+      const Csr& a = transpose ? st.lp[uz(l)]->kt : st.lp[uz(l)]->k;
+      const size_t rows = uz(a.rows), nnz = uz(a.nnz()), at = uz(st.nz_off[uz(l)]);
+      if (rows > 0)
+        hip_check(hipMemcpyAsync(start + roff[uz(l)] + 1, a.start.data() + 1, rows * sizeof(Int),
+                                 hipMemcpyHostToDevice, s), "upload stack");
+      if (nnz > 0) {
+        hip_check(hipMemcpyAsync(index + at, a.index.data(), nnz * sizeof(Int), hipMemcpyHostToDevice, s),
+                  "upload stack");
+        hip_check(hipMemcpyAsync(value + at, a.value.data(), nnz * sizeof(double), hipMemcpyHostToDevice, s),
+                  "upload stack");
+      }
+    }
+    launch_add_segment_offsets(s, start + 1, d.rows, transpose ? d_col_off : d_row_off, d_nz_off, st.lps);
+    launch_add_segment_offsets(s, index, d.nnz, d_nz_off, transpose ? d_row_off : d_col_off, st.lps);
+    d.start = start;
+    d.index = index;
+    d.value = value;
     d.block_row = ds->put(p.block_row);
     return d;
   };
-  const Plan pk = plan_stack(st.k, st.row_off);
-  const Plan pkt = plan_stack(st.kt, st.col_off);
-  ds->k = dev_csr(st.k, pk);
-  ds->kt = dev_csr(st.kt, pkt);
+  const Plan pk = plan_stack(st, false);
+  const Plan pkt = plan_stack(st, true);
+  ds->k = dev_csr(false, pk);
+  ds->kt = dev_csr(true, pkt);
   ds->k_value = const_cast<double*>(ds->k.value);
   ds->kt_value = const_cast<double*>(ds->kt.value);
   ds->k_block_lp = ds->put(pk.block_lp);
@@ -266,7 +287,9 @@ void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, const 
     int slot = kMultiScratchSlot + 80;
     auto put = [&](const std::vector<double>& h) {
       double* d = be.scratch(slot++, std::max<size_t>(h.size(), 1));
-      be.upload_staged(h.data(), d, h.size());
+      if (!h.empty())
+        hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s),
+                  "stack scaling");
       return d;
     };
     double* q = put(st->q);
@@ -340,7 +363,7 @@ std::vector<double> stack_norms(HipBackend& be, const Stack& st, const DeviceSta
   std::vector<double> v(uz(st.n), 0.0);
   for (Int l = 0; l < L; ++l) {
     const Int n = st.cols(l), m = st.rows(l);
-    const Int nnz = st.k.start[uz(st.row_off[uz(l) + 1])] - st.k.start[uz(st.row_off[uz(l)])];
+    const Int nnz = st.nz_off[uz(l) + 1] - st.nz_off[uz(l)];
     if (n == 0 || m == 0 || nnz == 0) continue;
     std::vector<double> s(uz(n));
     for (Int j = 0; j < n; ++j) s[uz(j)] = 1.0 + 0.1 * static_cast<double>(j % 7);

@@ -109,10 +109,23 @@ __global__ void __launch_bounds__(kThreads)
         hi_r[u] = a.start[r + 1] - k0;
       }
     }
+    if (fast) {
+      // Large K: the index and value streams are read once per product, so
+      // they are loaded non-temporally and leave the L2 to the gathered
+      // vector's lines.
 #pragma unroll
-    for (int u = 0; u < kTile / kThreads; ++u) {
-      const Int k = static_cast<Int>(threadIdx.x) + u * kThreads;
-      if (k < len) tile[k] = a.value[k0 + k] * x[a.index[k0 + k]];
+      for (int u = 0; u < kTile / kThreads; ++u) {
+        const Int k = static_cast<Int>(threadIdx.x) + u * kThreads;
+        if (k < len)
+          tile[k] = __builtin_nontemporal_load(a.value + k0 + k) *
+                    x[__builtin_nontemporal_load(a.index + k0 + k)];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kTile / kThreads; ++u) {
+        const Int k = static_cast<Int>(threadIdx.x) + u * kThreads;
+        if (k < len) tile[k] = a.value[k0 + k] * x[a.index[k0 + k]];
+      }
     }
     __syncthreads();
     // Rows in rounds of kThreads, one row per thread per round.

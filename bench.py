@@ -13,8 +13,9 @@ algorithmic bytes by its time (`roofline`, dominant kernel), the
 512-LP stack solved in one set of launches (`c3`: LPs/s, p50/p99
 time-to-solve), a 64-node branch-and-bound frontier's relaxations solved
 in one batch (`nodes`: nodes/s against plain and reference solves), the
-stack finished exactly (`finish`: batched PDHG to 1e-8, then crossover and
-simplex per case on the host cores, against the reference simplex), and the
+stack finished exactly (`finish`: batched PDHG to a seed tolerance, then
+crossover and simplex per case on the host cores, against the reference
+simplex), and the
 reference's own solve_pdhg (oracle/_ref, one thread) on a bounded sample of
 the cases (`cpu_baseline`), and the same on every case, one case per thread
 over the host's cores (`cpu_baseline_all_cores`).
@@ -340,25 +341,35 @@ def nodes_section(ctx, args):
 
 def finish_section(ctx, args, base, cases, lps, optima):
     """Exact finish of the case stack: every case solved by the batched
-    device solver to rel KKT 1e-8, then finished to an optimal basis on the
-    host cores (crossover of the first-order point, then the bounded primal
-    simplex from it), one case per thread; beside it the same finish from
-    all logicals (cold) and the reference's simplex (oracle/_ref, one
-    thread) on a bounded sample. Error is against the reference simplex's
-    FP64 optima."""
+    device solver to a seed tolerance (1e-4, 1e-6 or 1e-8), then finished to
+    an optimal basis on the host cores (crossover of the first-order point,
+    then the bounded primal simplex from it), one case per thread; beside it
+    the same finish from all logicals (cold) and the reference's simplex
+    (oracle/_ref, one thread) on a bounded sample. Errors are against the
+    reference simplex's FP64 optima."""
     from anukulan_amd import solve_pdhg_batch
     from anukulan_amd.finish import exact_finish
 
-    t0 = time.perf_counter()
-    pts = solve_pdhg_batch(ctx, lps, tolerance=1e-8)
-    pdhg_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    warm = exact_finish(lps, [p.x for p in pts], [p.y for p in pts])
-    warm_s = time.perf_counter() - t0
+    seeds = {}
+    for tol in (1e-4, 1e-6, 1e-8):
+        t0 = time.perf_counter()
+        pts = solve_pdhg_batch(ctx, lps, tolerance=tol)
+        pdhg_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        warm = exact_finish(lps, [p.x for p in pts], [p.y for p in pts])
+        fin_s = time.perf_counter() - t0
+        err = ([abs(w.objective - o) / max(1.0, abs(o)) for w, o in zip(warm, optima)]
+               if optima else [])
+        seeds[f"{tol:g}"] = {"pdhg_s": round(pdhg_s, 3), "finish_s": round(fin_s, 3),
+                             "cases_per_s": round(len(lps) / (pdhg_s + fin_s), 3),
+                             "pivots": sum(w.iterations for w in warm),
+                             "cases_optimal": sum(w.status == "optimal" for w in warm),
+                             "max_rel_error_vs_simplex": max(err) if err else None}
     t0 = time.perf_counter()
     cold = exact_finish(lps)
     cold_s = time.perf_counter() - t0
-    err = [abs(w.objective - o) / max(1.0, abs(o)) for w, o in zip(warm, optima)] if optima else []
+    cold_err = ([abs(c.objective - o) / max(1.0, abs(o)) for c, o in zip(cold, optima)]
+                if optima else [])
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -378,17 +389,15 @@ def finish_section(ctx, args, base, cases, lps, optima):
                 n_done += 1
             cpu = {"value": round(n_done / spent, 3), "unit": "cases/s", "cores": 1,
                    "kind": "reference", "sample": f"reference simplex on the first {n_done} cases"}
-    total = pdhg_s + warm_s
-    threads = min(16, os.cpu_count() or 1)
-    return {"workload": f"{len(lps)}-case stack: batched PDHG to rel KKT 1e-8, then crossover "
-                        f"and primal simplex per case on up to {threads} host threads",
-            "cases_per_s": round(len(lps) / total, 3), "seconds": round(total, 3),
-            "pdhg_s": round(pdhg_s, 3), "finish_s": round(warm_s, 3),
-            "cases_optimal": sum(w.status == "optimal" for w in warm),
-            "pivots_warm": sum(w.iterations for w in warm),
-            "pivots_cold": sum(c.iterations for c in cold),
-            "cold_finish_s": round(cold_s, 3),
-            "max_rel_error_vs_simplex": max(err) if err else None,
+    best = max(seeds, key=lambda k: seeds[k]["cases_per_s"])
+    threads = min(16, len(os.sched_getaffinity(0)))
+    return {"workload": f"{len(lps)}-case stack: batched PDHG to a seed tolerance, then crossover "
+                        f"and primal simplex per case on {threads} host threads",
+            "cases_per_s": seeds[best]["cases_per_s"], "best_seed_tolerance": best,
+            "by_seed_tolerance": seeds,
+            "cold": {"seconds": round(cold_s, 3), "cases_per_s": round(len(lps) / cold_s, 3),
+                     "pivots": sum(c.iterations for c in cold),
+                     "max_rel_error_vs_simplex": max(cold_err) if cold_err else None},
             "cpu_baseline": cpu}
 
 

@@ -237,8 +237,13 @@ std::vector<AdjointOutcome> pdhg_adjoint(HipBackend& be, const std::vector<const
     HipBackend& b;
     std::uint64_t a, c;
     ~Release() {
-      b.release_one(a);
-      b.release_one(c);
+      // A teardown error is dropped: the error that unwinds past here is
+      // the one the caller gets, and a destructor must not throw.
+      try {
+        b.release_one(a);
+        b.release_one(c);
+      } catch (...) {
+      }
     }
   } release{be, k.id, kt.id};
 

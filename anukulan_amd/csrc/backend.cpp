@@ -113,7 +113,15 @@ HipBackend::Timed::Timed(HipBackend& b, const char* name, double bytes) : b_(b),
 
 HipBackend::Timed::~Timed() {
   if (!active_) return;
-  hipEvent_t stop = b_.take_event();
+  // A destructor must not throw: without a stop event the interval is
+  // dropped and its start event returns to the pool.
+  hipEvent_t stop = nullptr;
+  try {
+    stop = b_.take_event();
+  } catch (...) {
+    b_.event_pool_.push_back(start_);
+    return;
+  }
   (void)hipEventRecord(stop, b_.stream_);
   b_.profile_[name_].pairs.emplace_back(start_, stop);
 }

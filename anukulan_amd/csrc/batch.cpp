@@ -897,8 +897,13 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
     HipBackend& b;
     std::uint64_t a, c;
     ~Release() {
-      b.release_one(a);
-      b.release_one(c);
+      // A teardown error is dropped: the error that unwinds past here is
+      // the one the caller gets, and a destructor must not throw.
+      try {
+        b.release_one(a);
+        b.release_one(c);
+      } catch (...) {
+      }
     }
   } release{be, k.id, kt.id};
   const double norm = opt.known_matrix_norm > 0.0 ? opt.known_matrix_norm

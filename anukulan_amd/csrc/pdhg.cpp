@@ -1020,7 +1020,13 @@ double estimate_matrix_norm(HipBackend& be, const StandardLp& lp, int iterations
     HipBackend& b;
     double* p[3];
     ~Free() {
-      for (double* q : p) b.deallocate(q);
+      // A teardown error is dropped: a destructor must not throw.
+      for (double* q : p) {
+        try {
+          b.deallocate(q);
+        } catch (...) {
+        }
+      }
     }
   } guard{be, {d_v, d_kv, d_ktkv}};
   be.upload(v.data(), d_v, n);

@@ -92,7 +92,8 @@ class BasisLu {
   // scratch
   std::vector<double> x_;
   std::vector<Int> mark_, stack_, child_, topo_, touched_;
-  std::vector<double> yk_;
+  // Pivot-order work vector of ftran/btran (a solve changes no state).
+  mutable std::vector<double> yk_;
 };
 
 bool BasisLu::factor(const Form& f, const std::vector<Int>& head, std::vector<Int>* dependent,
@@ -231,7 +232,7 @@ bool BasisLu::factor(const Form& f, const std::vector<Int>& head, std::vector<In
 
 void BasisLu::ftran(std::vector<double>& b, std::vector<double>& z) const {
   const Int m = m_;
-  std::vector<double>& y = const_cast<std::vector<double>&>(yk_);
+  std::vector<double>& y = yk_;
   for (Int k = 0; k < m; ++k) {
     const double v = b[uz(p_[uz(k)])];
     y[uz(k)] = v;
@@ -262,7 +263,7 @@ void BasisLu::btran(std::vector<double>& c, std::vector<double>& y) const {
     for (Int e = eta_start_[t]; e < eta_start_[t + 1]; ++e) s -= eta_val_[uz(e)] * c[uz(eta_pos_[uz(e)])];
     c[uz(r)] = s / eta_pivot_[t];
   }
-  std::vector<double>& w = const_cast<std::vector<double>&>(yk_);
+  std::vector<double>& w = yk_;
   for (Int k = 0; k < m; ++k) {
     double s = c[uz(q_[uz(k)])];
     for (Int e = ustart_[uz(k)]; e < ustart_[uz(k) + 1]; ++e) s -= uval_[uz(e)] * w[uz(ustep_[uz(e)])];

@@ -8,7 +8,8 @@ near the optimum; by the reference simplex itself, live, on seeded random
 LPs of the shape its crossover tests use (oracle/_ref); and by the
 properties those tests hold (test_crossover.cpp:72-170): every crossover
 basis is one column per row, all distinct, and starting from it never
-changes the optimum.
+changes the optimum. End to end, the reference's refinery.mps presolved,
+finished and postsolved gives its published optimum.
 """
 import json
 import os
@@ -138,6 +139,31 @@ def test_matches_the_reference_simplex_live():
             e = lps[seed - 1].export()
             ref_obj = float(e["c"] @ w["x"]) * e["objective_scale"] + e["objective_offset"]
             assert abs(r.objective - ref_obj) <= 1e-9 * max(1.0, abs(ref_obj)), seed
+
+
+def test_refinery_mps_presolved_finished_and_postsolved():
+    # The reference's data file end to end on the host: presolve (the
+    # reference's reductions), the exact finish of the reduced LP from all
+    # logicals, postsolve. The answer is the file's published optimum
+    # 2.426062266887e10 (bench/results/ipm_refinery.txt:19), and it is
+    # feasible for the model as read.
+    import gzip
+    from anukulan_amd.mps import read_mps_text
+    from anukulan_amd.presolve import presolve_mps_text
+    raw = gzip.decompress(open(os.path.join(HERE, "golden", "refinery.mps.gz"), "rb").read())
+    P = presolve_mps_text(raw)
+    r = exact_finish([P.reduced.to_lp()])[0]
+    assert r.status == "optimal", r.message
+    x = P.postsolve_primal(r.x)
+    m = read_mps_text(raw)
+    act = np.zeros(m.rows)
+    np.add.at(act, m.row_index(), m.value * x[m.index])
+    scale = 1.0 + np.max(np.abs(np.where(np.isfinite(m.row_upper), m.row_upper, 0.0)))
+    assert np.max(np.maximum(m.row_lower - act, act - m.row_upper)) <= 1e-12 * scale
+    assert np.all(x >= m.col_lower - 1e-9) and np.all(x <= m.col_upper + 1e-9)
+    obj = float(m.objective @ x) + m.objective_offset
+    assert abs(obj - 2.426062266887e10) <= 1e-12 * 2.426062266887e10
+    assert abs(r.objective - obj) <= 1e-12 * abs(obj)
 
 
 def test_bad_arguments():

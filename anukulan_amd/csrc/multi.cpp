@@ -639,19 +639,16 @@ void Engine::upload() {
   put_range(*data_.oq, prob_.orig_q, st_.row_off);
   put_range(*data_.row_scale, prob_.row_scale, st_.row_off);
   // Starting points (pdhg.cpp:479-503): a warm start put in the dual cone and
-  // the primal box, a cold one the projection of zero.
-  std::vector<double> x(uz(st_.n), 0.0), y(uz(st_.m), 0.0);
-  for (Int l = 0; l < L_; ++l) {
+  // the primal box, a cold one the projection of zero. Built per LP, so a
+  // sub-engine polishing a few LPs touches only their ranges on the host.
+  auto start_point = [&](Int l, double* x, double* y) {
     const LpRun& r = runs_[uz(l)];
-    if (!r.participates) continue;
     const Int c0 = st_.col_off[uz(l)], r0 = st_.row_off[uz(l)];
     const bool warm = !r.warm_x.empty() && !r.warm_y.empty();
-    if (warm) {
-      for (Int i = 0; i < st_.rows(l); ++i) {
-        double v = r.warm_y[uz(i)];
-        if (r0 + i >= st_.eq_end[uz(l)] && !(v > 0.0)) v = 0.0;
-        y[uz(r0 + i)] = v;
-      }
+    for (Int i = 0; i < st_.rows(l); ++i) {
+      double v = warm ? r.warm_y[uz(i)] : 0.0;
+      if (warm && r0 + i >= st_.eq_end[uz(l)] && !(v > 0.0)) v = 0.0;
+      y[uz(i)] = v;
     }
     for (Int j = 0; j < st_.cols(l); ++j) {
       const double lo = (*data_.lower)[uz(c0 + j)], hi = (*data_.upper)[uz(c0 + j)];
@@ -659,11 +656,26 @@ void Engine::upload() {
       if (!std::isfinite(v)) v = 0.0;
       if (v < lo) v = std::isfinite(lo) ? lo : 0.0;
       if (v > hi) v = std::isfinite(hi) ? hi : 0.0;
-      x[uz(c0 + j)] = v;
+      x[uz(j)] = v;
+    }
+  };
+  if (whole) {
+    std::vector<double> x(uz(st_.n), 0.0), y(uz(st_.m), 0.0);
+    for (Int l = 0; l < L_; ++l)
+      start_point(l, x.data() + st_.col_off[uz(l)], y.data() + st_.row_off[uz(l)]);
+    put_range(x, X_[0], st_.col_off);
+    put_range(y, Y_[0], st_.row_off);
+  } else {
+    std::vector<double> x, y;
+    for (Int l = 0; l < L_; ++l) {
+      if (!runs_[uz(l)].participates) continue;
+      x.assign(uz(st_.cols(l)), 0.0);
+      y.assign(uz(st_.rows(l)), 0.0);
+      start_point(l, x.data(), y.data());
+      if (!x.empty()) be_.upload_staged(x.data(), X_[0] + st_.col_off[uz(l)], x.size());
+      if (!y.empty()) be_.upload_staged(y.data(), Y_[0] + st_.row_off[uz(l)], y.size());
     }
   }
-  put_range(x, X_[0], st_.col_off);
-  put_range(y, Y_[0], st_.row_off);
   be_.copy(X_[0], RX_, st_.n);
   be_.copy(X_[0], PRX_, st_.n);
   be_.copy(Y_[0], RY_, st_.m);

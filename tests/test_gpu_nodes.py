@@ -134,6 +134,29 @@ def test_node_arguments_are_checked(ctx):
         solve_node_relaxations(ctx, lp, np.zeros((1, 2)), np.ones((1, 2)), max_iterations=-5)
 
 
+def test_frontier_larger_than_one_call_of_the_batch(ctx):
+    # 300 nodes go to the batch in chunks of 256 (and groups of 64 lanes):
+    # every node's result must equal its own solve's, wherever it lands.
+    lp = G.refinery_model(2).to_lp()
+    e = lp.export()
+    cols = [j for j in range(lp.cols) if np.isfinite(e["upper"][j]) and e["upper"][j] >= 4.0]
+    lo = np.tile(e["lower"], (300, 1))
+    hi = np.tile(e["upper"], (300, 1))
+    for i in range(300):
+        j = cols[i % len(cols)]
+        hi[i, j] = np.floor(e["upper"][j] / (2.0 + i // len(cols)))
+    got = solve_node_relaxations(ctx, lp, lo, hi)
+    assert len(got) == 300
+    for i in (0, 63, 64, 255, 256, 299):
+        node = LP.from_standard(lp.rows, lp.cols, e["num_equalities"], e["start"], e["index"],
+                                e["value"], e["c"], e["q"], lo[i], hi[i], e["objective_scale"],
+                                e["objective_offset"])
+        p = solve_pdhg(ctx, node)
+        assert got[i].relaxation.status == p.status, i
+        assert abs(got[i].relaxation.iterations - p.iterations) <= 0.02 * p.iterations, i
+        assert rel(got[i].relaxation.objective, p.objective) <= 1e-8, i
+
+
 # ---- a best-first tree over batched relaxations ---------------------------
 
 def enumerate_optimum(e, box):

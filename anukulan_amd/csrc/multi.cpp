@@ -949,7 +949,10 @@ std::vector<Outcome> Engine::run() {
     const int steps = period_end - iteration;
     const bool has_check = (period_end % freq == 0);
     upload_params();
-    const bool graph = opt_.use_graphs != 0 && has_check && steps == freq && !be_.profiling();
+    // Sub-engines (polishing) run a few short periods: capturing and
+    // instantiating their graphs costs more than launching directly.
+    const bool graph = opt_.use_graphs != 0 && depth_ == 0 && has_check && steps == freq &&
+                       !be_.profiling();
     if (graph) {
       hip_check(hipGraphLaunch(graph_for(cur_), be_.stream()), "hipGraphLaunch(multi)");
     } else {

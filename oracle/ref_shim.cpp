@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "anukulan/backend.hpp"
+#include "anukulan/crossover.hpp"
 #include "anukulan/mps_reader.hpp"
 #include "anukulan/pdhg.hpp"
 #include "anukulan/presolve.hpp"
@@ -584,6 +585,30 @@ int ref_simplex(const void* h, double* x, double* y, double* objective, int32_t*
     if (y) std::memcpy(y, r.y.data(), r.y.size() * sizeof(double));
     if (basic_count) *basic_count = static_cast<int32_t>(r.final_basic.size());
     if (basic) std::memcpy(basic, r.final_basic.data(), r.final_basic.size() * sizeof(Int));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// The reference crossover (crossover.hpp:114-117) of a point x (cols) and
+// dual y (rows, or null), default options: its counts in info[0..5]
+// (ok, candidates, pushed, rejected_small_pivot, logicals_remaining,
+// stopped_on_refactorization) and its basis (rows entries) in basic.
+int ref_crossover(const void* h, const double* x, const double* y, int32_t* info, int32_t* basic) {
+  try {
+    const StandardLp& lp = static_cast<const RefLp*>(h)->lp;
+    std::vector<double> xv(x, x + lp.num_cols());
+    std::vector<double> yv;
+    if (y) yv.assign(y, y + lp.num_rows());
+    const CrossoverResult r = crossover_basis(lp, xv, yv);
+    info[0] = r.ok ? 1 : 0;
+    info[1] = static_cast<int32_t>(r.candidates);
+    info[2] = static_cast<int32_t>(r.pushed);
+    info[3] = static_cast<int32_t>(r.rejected_small_pivot);
+    info[4] = static_cast<int32_t>(r.logicals_remaining);
+    info[5] = r.stopped_on_refactorization ? 1 : 0;
+    if (basic && r.ok) std::memcpy(basic, r.basic.data(), r.basic.size() * sizeof(Int));
     return 0;
   } catch (const std::exception& e) {
     return fail(e);

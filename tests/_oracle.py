@@ -263,6 +263,7 @@ def _load_ref():
     L.ref_estimate_matrix_norm.argtypes = [c_void_p, c_int32, c_double, c_int32]
     L.ref_evaluate_residual.argtypes = [c_void_p, DP, DP, POINTER(P.PdhgResidual)]
     L.ref_simplex.argtypes = [c_void_p, DP, DP, DP, IP, IP, IP]
+    L.ref_crossover.argtypes = [c_void_p, DP, DP, IP, IP]
     L.ref_op_multiply.argtypes = [c_int32, c_int32, c_int32, IP, IP, DP, DP, DP]
     L.ref_op_transpose.argtypes = [c_int32, c_int32, IP, IP, DP, IP, IP, DP]
     for f in ("ref_op_dot",):
@@ -508,6 +509,20 @@ class RefLp:
         r = P.PdhgResidual()
         ref.ref_evaluate_residual(self.h, _dp(f64(x)), _dp(f64(y)), byref(r))
         return {k: getattr(r, k) for k, _ in P.PdhgResidual._fields_}
+
+    def crossover(self, x, y=None):
+        """The reference's crossover_basis of the point (x, y): its counts and
+        the basis it hands over."""
+        info = np.zeros(6, np.int32)
+        basic = np.zeros(self.rows, np.int32)
+        xv = f64(x)
+        yv = f64(y) if y is not None else None
+        rc = ref.ref_crossover(self.h, _dp(xv), _dp(yv), _ip(info), _ip(basic))
+        if rc != 0:
+            raise RuntimeError(ref.ref_last_error().decode())
+        return {"ok": bool(info[0]), "candidates": int(info[1]), "pushed": int(info[2]),
+                "rejected_small_pivot": int(info[3]), "logicals_remaining": int(info[4]),
+                "stopped_on_refactorization": bool(info[5]), "basic": basic}
 
     def simplex(self):
         x, y = np.zeros(self.cols), np.zeros(self.rows)

@@ -141,6 +141,27 @@ def test_matches_the_reference_simplex_live():
             assert abs(r.objective - ref_obj) <= 1e-9 * max(1.0, abs(ref_obj)), seed
 
 
+@pytest.mark.skipif(_oracle.ref is None, reason="oracle/_ref not built")
+def test_crossover_counts_are_the_references():
+    # On the LP as given (no equilibration), the crossover applies the
+    # reference's rules to the same point from the same all-logical basis:
+    # the same candidates, and the same columns pushed into the basis.
+    lps = stack_lps(3)
+    cold = exact_finish(lps)
+    g = np.random.default_rng(11)
+    for lp, c in zip(lps, cold):
+        e = lp.export()
+        x = np.clip(c.x + 1e-6 * (1.0 + np.abs(c.x)) * g.standard_normal(lp.cols), e["lower"], e["upper"])
+        y = c.y + 1e-6 * (1.0 + np.abs(c.y)) * g.standard_normal(lp.rows)
+        mine = exact_finish([lp], [x], [y], ruiz_iterations=0)[0]
+        ref = _oracle.RefLp.from_export(e).crossover(x, y)
+        assert ref["ok"]
+        assert mine.info["candidates"] == ref["candidates"]
+        assert mine.info["pushed"] == ref["pushed"]
+        assert mine.info["rejected_small_pivot"] == ref["rejected_small_pivot"]
+        assert mine.status == "optimal" and rel(mine.objective, c.objective) <= 1e-9
+
+
 def test_refinery_mps_presolved_finished_and_postsolved():
     # The reference's data file end to end on the host: presolve (the
     # reference's reductions), the exact finish of the reduced LP from all

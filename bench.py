@@ -194,6 +194,27 @@ def c1_section(ctx, args):
         ctx.set_profiling(False)
         spmv[name] = {"mean_us": row["mean_us"], "achieved_GBs": round(row["gbs"], 1),
                       "frac": round(row["gbs"] / HBM_PEAK_GBS, 4), "launches": row["timed"]}
+    # Control: the same rows (same lengths, same values) with each row's
+    # columns contiguous near its diagonal, so x's lines are reused from
+    # cache. The kernel and the byte count are the same; only the gather
+    # pattern differs, which separates what the kernel streams from what
+    # the random pattern costs.
+    lens = np.diff(e["start"]).astype(np.int64)
+    base = np.minimum((np.arange(lp.rows, dtype=np.int64) * lp.cols) // lp.rows,
+                      lp.cols - np.maximum(lens, 1))
+    within = np.arange(int(lens.sum()), dtype=np.int64) - np.repeat(e["start"][:-1].astype(np.int64), lens)
+    band = Matrix(lp.rows, lp.cols, e["start"], (np.repeat(base, lens) + within).astype(np.int32),
+                  e["value"])
+    ctx.prepare(band)
+    ctx.multiply(band, x, y)
+    ctx.set_profiling(True)
+    for _ in range(args.c1_reps):
+        ctx.multiply(band, x, y)
+    row = parse_profile(ctx.profile_report())["spmv"]
+    ctx.set_profiling(False)
+    spmv["K x, banded control"] = {"mean_us": row["mean_us"], "achieved_GBs": round(row["gbs"], 1),
+                                    "frac": round(row["gbs"] / HBM_PEAK_GBS, 4),
+                                    "launches": row["timed"]}
     for v in (x, y, yv, xv):
         v.free()
     ctx.release()

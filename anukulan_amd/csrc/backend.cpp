@@ -304,9 +304,17 @@ const DevCsr& HipBackend::prepare_fast(const CsrView& a) {
 }
 
 const DevCsr& HipBackend::prepare_panels(const CsrView& a) {
-  const std::uint64_t key = a.id ^ kPanelKey;
-  auto it = matrices_.find(key);
+  auto it = matrices_.find(a.id ^ kPanelKey);
   if (it != matrices_.end()) return it->second.view;
+  return install_panels(build_panels(a));
+}
+
+bool HipBackend::has_panels(const CsrView& a) const {
+  return matrices_.count(a.id ^ kPanelKey) != 0;
+}
+
+HipBackend::PanelStack HipBackend::build_panels(const CsrView& a) {
+  const std::uint64_t key = a.id ^ kPanelKey;
   const Int nnz = a.nnz();
   // Column boundaries that split the nonzeros into kPanels near-equal parts.
   std::vector<long> col_nnz(uz(a.cols) + 1, 0);
@@ -365,10 +373,24 @@ const DevCsr& HipBackend::prepare_panels(const CsrView& a) {
       start[uz(p) * uz(rows) + uz(r) + 1] = static_cast<Int>(at);
     }
   });
-  const CsrView stacked{key, kPanels * rows, a.cols, start.data(), index.data(), value.data()};
+  PanelStack out;
+  out.key = key;
+  out.rows = kPanels * rows;
+  out.cols = a.cols;
+  out.start = std::move(start);
+  out.index = std::move(index);
+  out.value = std::move(value);
+  return out;
+}
+
+const DevCsr& HipBackend::install_panels(PanelStack stack) {
+  auto it = matrices_.find(stack.key);
+  if (it != matrices_.end()) return it->second.view;
+  const CsrView stacked{stack.key, stack.rows, stack.cols, stack.start.data(), stack.index.data(),
+                        stack.value.data()};
   prepare(stacked);
   // The panel stack only ever runs in the fast order: give it that plan.
-  DeviceMatrix& dm = matrices_.at(key);
+  DeviceMatrix& dm = matrices_.at(stack.key);
   replan_fast(stacked, &dm);
   return dm.view;
 }

@@ -67,6 +67,17 @@ class HipBackend {
   // kernels.hpp), built on the host, planned for the fast order and cached
   // with `a`.
   const DevCsr& prepare_panels(const CsrView& a);
+  // The same in two halves: the host build (no device call, safe on another
+  // thread while this backend runs) and its upload and plan.
+  struct PanelStack {
+    std::uint64_t key = 0;
+    Int rows = 0, cols = 0;
+    std::vector<Int> start, index;
+    std::vector<double> value;
+  };
+  bool has_panels(const CsrView& a) const;
+  static PanelStack build_panels(const CsrView& a);
+  const DevCsr& install_panels(PanelStack stack);
   void release();
   void release_one(std::uint64_t id);
   void multiply(const CsrView& a, const double* x, double* y);

@@ -3,6 +3,7 @@
 #include "scaling.hpp"
 
 #include <algorithm>
+#include <future>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -325,12 +326,20 @@ SolveResult Solver::run() {
   // The fast product order by default only where it pays: 4M nonzeros and up.
   fast_ = opt_.product_order == 2 || (opt_.product_order == 0 && lp_->k.nnz() >= (Int{1} << 22));
   panels_ = fast_ && static_cast<double>(n_) * sizeof(double) > kPanelVectorBytes && m_ > 0;
-  if (panels_) dpk_ = be_.prepare_panels(view_of(lp_->k));
+  // The panel stack is built on the host while the device runs the power
+  // iteration (which needs only K and K^T), then uploaded.
+  std::future<HipBackend::PanelStack> panel_build;
+  if (panels_ && !be_.has_panels(view_of(lp_->k))) {
+    const CsrView kv = view_of(lp_->k);
+    panel_build = std::async(std::launch::async, [kv] { return HipBackend::build_panels(kv); });
+  }
   if (fast_) dkt_fast_ = be_.prepare_fast(view_of(lp_->kt));
   if (fast_ && !panels_) dk_fast_ = be_.prepare_fast(view_of(lp_->k));
 
   matrix_norm_ = opt_.known_matrix_norm > 0.0 ? opt_.known_matrix_norm
                                              : estimate_matrix_norm(be_, *lp_, 200, 1e-8);
+  if (panels_) dpk_ = panel_build.valid() ? be_.install_panels(panel_build.get())
+                                          : be_.prepare_panels(view_of(lp_->k));
   res_.phases.matrix_norm = phase_end();
   res_.matrix_norm_estimate = matrix_norm_;
   if (matrix_norm_ <= 0.0) {

@@ -599,6 +599,21 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kThreads)
     row_norms_kernel(const Int* __restrict__ start, Int rows, const double* __restrict__ value,
                      int one_norm, double* __restrict__ out) {
+  if (!one_norm) {
+    // The largest |a| of a row: kRowLanes lanes take a stride each and the
+    // maxima meet by shuffles (a maximum has no order to keep).
+    const long t = static_cast<long>(blockIdx.x) * kThreads + static_cast<long>(threadIdx.x);
+    const Int r = static_cast<Int>(t / kRowLanes);
+    const int sub = static_cast<int>(t % kRowLanes);
+    double acc = 0.0;
+    if (r < rows)
+      for (Int k = start[r] + sub; k < start[r + 1]; k += kRowLanes) acc = fmax(acc, fabs(value[k]));
+#pragma unroll
+    for (int w = kRowLanes / 2; w > 0; w >>= 1) acc = fmax(acc, __shfl_xor(acc, w, kRowLanes));
+    if (r < rows && sub == 0) out[r] = acc;
+    return;
+  }
+  // The one-norm keeps the serial sum in storage order (the CPU's bits).
   const Int r = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
   if (r >= rows) return;
   double acc = 0.0;
@@ -637,9 +652,13 @@ __global__ void __launch_bounds__(kThreads)
     scale_values_kernel(const Int* __restrict__ start, const Int* __restrict__ index, Int rows,
                         double* __restrict__ value, const double* __restrict__ first,
                         const double* __restrict__ second, int first_by_row) {
-  const Int r = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  // kRowLanes lanes per row, striding it: each entry is scaled on its own,
+  // so the split changes no bits, and a row's entries load together.
+  const long t = static_cast<long>(blockIdx.x) * kThreads + static_cast<long>(threadIdx.x);
+  const Int r = static_cast<Int>(t / kRowLanes);
+  const Int sub = static_cast<Int>(t % kRowLanes);
   if (r >= rows) return;
-  for (Int k = start[r]; k < start[r + 1]; ++k) {
+  for (Int k = start[r] + sub; k < start[r + 1]; k += kRowLanes) {
     double v = value[k];
     if (first_by_row) {
       v = v * first[r];
@@ -744,6 +763,10 @@ __global__ void dual_update_kernel(Int m, Int neq, double sigma, double gamma, d
 }
 
 inline unsigned grid_for(Int n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+// Blocks for kRowLanes lanes per row.
+inline unsigned lanes_grid(Int rows) {
+  return static_cast<unsigned>((static_cast<long>(rows) * kRowLanes + kThreads - 1) / kThreads);
+}
 
 void check_launch(const char* what) {
   const hipError_t e = hipGetLastError();
@@ -1093,7 +1116,8 @@ void launch_multi_divide(hipStream_t s, const ChunkPlan& plan, const double* v,
 void launch_row_norms(hipStream_t s, const Int* start, Int rows, const double* value, bool one_norm,
                       double* out) {
   if (rows <= 0) return;
-  hipLaunchKernelGGL(row_norms_kernel, dim3(grid_for(rows)), dim3(kThreads), 0, s, start, rows, value,
+  const unsigned grid = one_norm ? grid_for(rows) : lanes_grid(rows);
+  hipLaunchKernelGGL(row_norms_kernel, dim3(grid), dim3(kThreads), 0, s, start, rows, value,
                      one_norm ? 1 : 0, out);
   check_launch("row norms");
 }
@@ -1116,7 +1140,7 @@ void launch_scale_values(hipStream_t s, const Int* start, const Int* index, Int 
                          double* value, const double* first, const double* second,
                          bool first_by_row) {
   if (rows <= 0) return;
-  hipLaunchKernelGGL(scale_values_kernel, dim3(grid_for(rows)), dim3(kThreads), 0, s, start, index,
+  hipLaunchKernelGGL(scale_values_kernel, dim3(lanes_grid(rows)), dim3(kThreads), 0, s, start, index,
                      rows, value, first, second, first_by_row ? 1 : 0);
   check_launch("scale values");
 }

@@ -18,6 +18,7 @@ namespace anuk {
 using Int = std::int32_t;
 
 inline constexpr int kThreads = 256;      // one workgroup = 4 wave64s
+inline constexpr int kRowLanes = 8;      // lanes per row in the scaling sweeps
 inline constexpr int kTileNnz = 1024;     // products staged in LDS per row block
 inline constexpr int kTileRows = 256;     // most rows per row block
 // Plans for the fast product order. Measured on the 1M-column random LP,

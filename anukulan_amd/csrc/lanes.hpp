@@ -61,7 +61,23 @@ __device__ __forceinline__ double row_sum(const DevCsr& a, Int row, const double
     chunk_sum<4>(a, k, v, lane, s);
     k += 4;
   }
-  for (; k < hi; ++k) s += a.value[k] * v[at(a.index[k], lane)];
+  // The last one to three terms: their gathers go out together too (a row
+  // of K^T is often this short), and the adds stay in order. (Masking every
+  // round to eight instead was slower on the dual side.)
+  const Int rest = hi - k;
+  if (rest > 0) {
+    double w[3], x[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      if (u < rest) {
+        w[u] = a.value[k + u];
+        x[u] = v[at(a.index[k + u], lane)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+      if (u < rest) s += w[u] * x[u];
+  }
   return s;
 }
 

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <iterator>
 #include <map>
+#include <memory>
 #include <stdexcept>
 
 #include "batch_kernels.hpp"
@@ -197,12 +198,18 @@ void Engine::allocate() {
 // Row-interleaves one vector per lane (rows entries each) and uploads it.
 void upload_interleaved(HipBackend& be, double* dst, Int rows, int count,
                         const std::vector<const std::vector<double>*>& cols) {
-  std::vector<double> buf(static_cast<size_t>(rows) * kLanes, 0.0);
-  for (int l = 0; l < count; ++l) {
-    const std::vector<double>& v = *cols[static_cast<size_t>(l)];
-    for (Int r = 0; r < rows; ++r) buf[static_cast<size_t>(r) * kLanes + l] = v[uz(r)];
+  // Filled row by row (one 512-byte line at a time, the padding lanes
+  // zeroed), so the writes stream instead of striding across the buffer.
+  const size_t size = static_cast<size_t>(rows) * kLanes;
+  std::unique_ptr<double[]> buf(new double[std::max<size_t>(size, 1)]);
+  std::vector<const double*> src(static_cast<size_t>(count));
+  for (int l = 0; l < count; ++l) src[static_cast<size_t>(l)] = cols[static_cast<size_t>(l)]->data();
+  for (Int r = 0; r < rows; ++r) {
+    double* line = buf.get() + static_cast<size_t>(r) * kLanes;
+    for (int l = 0; l < count; ++l) line[l] = src[static_cast<size_t>(l)][r];
+    for (int l = count; l < kLanes; ++l) line[l] = 0.0;
   }
-  be.upload_staged(buf.data(), dst, buf.size());
+  be.upload_staged(buf.get(), dst, size);
 }
 
 void Engine::upload_lanes() {

@@ -384,9 +384,11 @@ int anuk_solve_pdhg(anuk_ctx* ctx, const anuk_lp* lp, const anuk_pdhg_options* o
  * rows, columns, equality count and CSR arrays (a base plan and its what-if
  * cases), differing in c, q and bounds. No reference counterpart: the
  * reference solves such a stack one LP at a time (solve_pdhg per case). Each
- * LP gets exactly the result its own anuk_solve_pdhg gives: results[i], its
- * primal in x_out + i * cols and its dual in y_out + i * rows (either may be
- * NULL). Runs the options the fused loop runs (constant step size, Halpern
+ * LP gets the result its own anuk_solve_pdhg gives: the same steps, bit for
+ * bit, and the same decisions and iteration count wherever the check's
+ * reductions (folded in a different fixed tree, rounding-level apart) do
+ * not tip one; results[i], its primal in x_out + i * cols and its dual in
+ * y_out + i * rows (either may be NULL). Runs the options the fused loop runs (constant step size, Halpern
  * or no restarts, fixed-point restart test); others fail with
  * ANUK_ERR_UNSUPPORTED. options->warm_x / warm_y must be NULL. */
 int anuk_solve_pdhg_batch(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,

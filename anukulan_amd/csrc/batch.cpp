@@ -196,43 +196,21 @@ void Engine::allocate() {
 }
 
 // Row-interleaves one vector per lane (rows entries each) and uploads it.
-void upload_interleaved(HipBackend& be, double* dst, Int rows, int count,
-                        const std::vector<const std::vector<double>*>& cols) {
-  // Filled row by row (one 512-byte line at a time, the padding lanes
-  // zeroed), so the writes stream instead of striding across the buffer.
-  const size_t size = static_cast<size_t>(rows) * kLanes;
-  std::unique_ptr<double[]> buf(new double[std::max<size_t>(size, 1)]);
-  std::vector<const double*> src(static_cast<size_t>(count));
-  for (int l = 0; l < count; ++l) src[static_cast<size_t>(l)] = cols[static_cast<size_t>(l)]->data();
+// Row-interleaves one vector per lane (rows entries each) into `line`:
+// filled row by row (one 512-byte line at a time, the padding lanes
+// zeroed), so the writes stream instead of striding across the buffer.
+void interleave(double* out, Int rows, int count, const std::vector<const double*>& src) {
   for (Int r = 0; r < rows; ++r) {
-    double* line = buf.get() + static_cast<size_t>(r) * kLanes;
+    double* line = out + static_cast<size_t>(r) * kLanes;
     for (int l = 0; l < count; ++l) line[l] = src[static_cast<size_t>(l)][r];
     for (int l = count; l < kLanes; ++l) line[l] = 0.0;
   }
-  be.upload_staged(buf.get(), dst, size);
 }
 
 void Engine::upload_lanes() {
-  auto gather = [&](std::vector<double> Lane::*f) {
-    std::vector<const std::vector<double>*> v;
-    for (Lane& l : lanes_) v.push_back(&(l.*f));
-    return v;
-  };
-  upload_interleaved(be_, const_cast<double*>(data_.c), n_, count_, gather(&Lane::c));
-  upload_interleaved(be_, const_cast<double*>(data_.lower), n_, count_, gather(&Lane::lower));
-  upload_interleaved(be_, const_cast<double*>(data_.upper), n_, count_, gather(&Lane::upper));
-  upload_interleaved(be_, const_cast<double*>(data_.orig_c), n_, count_, gather(&Lane::oc));
-  upload_interleaved(be_, const_cast<double*>(data_.orig_lower), n_, count_, gather(&Lane::olo));
-  upload_interleaved(be_, const_cast<double*>(data_.orig_upper), n_, count_, gather(&Lane::ohi));
-  upload_interleaved(be_, const_cast<double*>(data_.q), m_, count_, gather(&Lane::q));
-  upload_interleaved(be_, const_cast<double*>(data_.orig_q), m_, count_, gather(&Lane::oq));
-  be_.upload(col_scale_.data(), const_cast<double*>(data_.col_scale), n_);
-  be_.upload(row_scale_.data(), const_cast<double*>(data_.row_scale), m_);
-
   // Starting points: a warm start is put in the dual cone and the primal box,
   // a cold one is the projection of zero (pdhg.cpp:479-503).
   std::vector<std::vector<double>> xs(static_cast<size_t>(count_)), ys(static_cast<size_t>(count_));
-  std::vector<const std::vector<double>*> px, py;
   for (int l = 0; l < count_; ++l) {
     Lane& ln = lanes_[static_cast<size_t>(l)];
     const bool warm = !ln.warm_x.empty() && !ln.warm_y.empty();
@@ -254,11 +232,56 @@ void Engine::upload_lanes() {
       if (v > hi) v = std::isfinite(hi) ? hi : 0.0;
       x[uz(j)] = v;
     }
-    px.push_back(&x);
-    py.push_back(&y);
   }
-  upload_interleaved(be_, X_[0], n_, count_, px);
-  upload_interleaved(be_, Y_[0], m_, count_, py);
+  // Every lane vector is interleaved straight into one pinned staging area
+  // and copied asynchronously; one wait covers them all.
+  struct Item {
+    double* dst;
+    Int rows;
+    std::vector<const double*> src;
+  };
+  auto field = [&](std::vector<double> Lane::*f) {
+    std::vector<const double*> v;
+    for (Lane& l : lanes_) v.push_back((l.*f).data());
+    return v;
+  };
+  auto own = [&](std::vector<std::vector<double>>& vs) {
+    std::vector<const double*> v;
+    for (auto& x : vs) v.push_back(x.data());
+    return v;
+  };
+  const Item items[] = {
+      {const_cast<double*>(data_.c), n_, field(&Lane::c)},
+      {const_cast<double*>(data_.lower), n_, field(&Lane::lower)},
+      {const_cast<double*>(data_.upper), n_, field(&Lane::upper)},
+      {const_cast<double*>(data_.orig_c), n_, field(&Lane::oc)},
+      {const_cast<double*>(data_.orig_lower), n_, field(&Lane::olo)},
+      {const_cast<double*>(data_.orig_upper), n_, field(&Lane::ohi)},
+      {const_cast<double*>(data_.q), m_, field(&Lane::q)},
+      {const_cast<double*>(data_.orig_q), m_, field(&Lane::oq)},
+      {X_[0], n_, own(xs)},
+      {Y_[0], m_, own(ys)},
+  };
+  size_t total = static_cast<size_t>(n_) + static_cast<size_t>(m_);  // the shared diagonals
+  for (const Item& it : items) total += static_cast<size_t>(it.rows) * kLanes;
+  double* stage = be_.pinned_scratch(total);
+  hipStream_t s = be_.stream();
+  size_t off = 0;
+  auto send = [&](double* dst, size_t count) {
+    if (count > 0)
+      hip_check(hipMemcpyAsync(dst, stage + off, count * sizeof(double), hipMemcpyHostToDevice, s),
+                "upload lanes");
+    off += count;
+  };
+  for (const Item& it : items) {
+    interleave(stage + off, it.rows, count_, it.src);
+    send(it.dst, static_cast<size_t>(it.rows) * kLanes);
+  }
+  std::copy(col_scale_.begin(), col_scale_.end(), stage + off);
+  send(const_cast<double*>(data_.col_scale), uz(n_));
+  std::copy(row_scale_.begin(), row_scale_.end(), stage + off);
+  send(const_cast<double*>(data_.row_scale), uz(m_));
+  be_.synchronize();
   // The first restart point (and the one before it) is the starting point.
   be_.copy(X_[0], RX_, n_ * kLanes);
   be_.copy(X_[0], PRX_, n_ * kLanes);

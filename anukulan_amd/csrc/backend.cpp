@@ -45,6 +45,7 @@ HipBackend::~HipBackend() {
   if (result_dev_) (void)hipFree(result_dev_);
   if (result_host_) (void)hipHostFree(result_host_);
   if (pinned_) (void)hipHostFree(pinned_);
+  for (auto& kv : pinned_slots_) (void)hipHostFree(kv.second.first);
   if (stop_event_) (void)hipEventDestroy(stop_event_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -200,6 +201,21 @@ double* HipBackend::pinned_scratch(std::size_t doubles) {
   hip_check(hipHostMalloc(&pinned_, pinned_size_ * sizeof(double), hipHostMallocDefault),
             "hipHostMalloc(staging)");
   return pinned_;
+}
+
+double* HipBackend::pinned_slot(int slot, std::size_t doubles) {
+  auto it = pinned_slots_.find(slot);
+  if (it != pinned_slots_.end() && it->second.second >= doubles) return it->second.first;
+  if (it != pinned_slots_.end()) {
+    synchronize();
+    (void)hipHostFree(it->second.first);
+    pinned_slots_.erase(it);
+  }
+  double* p = nullptr;
+  const std::size_t size = std::max<std::size_t>(doubles, 512);
+  hip_check(hipHostMalloc(&p, size * sizeof(double), hipHostMallocDefault), "hipHostMalloc(slot)");
+  pinned_slots_[slot] = {p, size};
+  return p;
 }
 
 void HipBackend::upload_staged(const double* host, double* target, std::size_t n) {

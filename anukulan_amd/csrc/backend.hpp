@@ -112,6 +112,10 @@ class HipBackend {
   double* scratch(int slot, std::size_t doubles);
   // A pinned host staging buffer of at least `doubles`, grown on demand.
   double* pinned_scratch(std::size_t doubles);
+  // Page-locked host memory kept per slot for the context's life (engine
+  // parameter and readback staging), like scratch(): allocating and freeing
+  // it per solve costs more than the solve's own small transfers.
+  double* pinned_slot(int slot, std::size_t doubles);
   // Copies n doubles host -> device through the pinned staging buffer and
   // waits (fast DMA for large uploads from pageable memory).
   void upload_staged(const double* host, double* target, std::size_t n);
@@ -174,6 +178,7 @@ class HipBackend {
   hipStream_t stream_ = nullptr;
   std::unordered_map<std::uint64_t, DeviceMatrix> matrices_;
   std::map<int, std::pair<double*, std::size_t>> scratch_;
+  std::map<int, std::pair<double*, std::size_t>> pinned_slots_;
   double* pinned_ = nullptr;
   std::size_t pinned_size_ = 0;
   double* result_dev_ = nullptr;

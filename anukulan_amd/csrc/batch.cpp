@@ -143,10 +143,9 @@ Engine::~Engine() {
     (void)hipGraphExecDestroy(g.second.second);
     (void)hipGraphDestroy(g.second.first);
   }
-  // The slab is the backend's scratch for this nesting depth, kept for the
-  // next batch of the same depth.
+  // The slab and the pinned staging are the backend's, per nesting depth,
+  // kept for the next batch of the same depth.
   if (slab_) (void)hipStreamSynchronize(be_.stream());
-  if (pinned_) (void)hipHostFree(pinned_);
 }
 
 constexpr size_t kPinnedDoubles = 64 * 8 + 64 * kLaneCheckStride + 128;
@@ -191,8 +190,7 @@ void Engine::allocate() {
   data_.c = c; data_.lower = lo; data_.upper = hi; data_.q = q;
   data_.orig_c = oc; data_.orig_lower = olo; data_.orig_upper = ohi; data_.orig_q = oq;
   data_.row_scale = rs; data_.col_scale = cs;
-  hip_check(hipHostMalloc(&pinned_, kPinnedDoubles * sizeof(double), hipHostMallocDefault),
-            "hipHostMalloc(batch)");
+  pinned_ = be_.pinned_slot(kBatchScratchSlot + depth_, kPinnedDoubles);
 }
 
 // Row-interleaves one vector per lane (rows entries each) and uploads it.

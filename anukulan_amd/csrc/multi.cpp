@@ -482,10 +482,8 @@ class Engine {
       (void)hipGraphExecDestroy(g.second.second);
       (void)hipGraphDestroy(g.second.first);
     }
-    if (pinned_) {
-      (void)hipStreamSynchronize(be_.stream());
-      (void)hipHostFree(pinned_);
-    }
+    // The pinned staging is the backend's, kept for the next engine.
+    if (pinned_) (void)hipStreamSynchronize(be_.stream());
   }
   Engine(const Engine&) = delete;
   Engine& operator=(const Engine&) = delete;
@@ -608,8 +606,7 @@ void Engine::allocate() {
   prob_.orig_c = oc; prob_.orig_lower = olo; prob_.orig_upper = ohi; prob_.orig_q = oq;
   prob_.row_scale = rs; prob_.col_scale = cs;
   pinned_doubles_ = std::max(L * kParamDoubles, L * kCheckSlots) + L * 2 + 64;
-  hip_check(hipHostMalloc(&pinned_, pinned_doubles_ * sizeof(double), hipHostMallocDefault),
-            "hipHostMalloc(multi)");
+  pinned_ = be_.pinned_slot(kMultiScratchSlot + depth_, pinned_doubles_);
 }
 
 void Engine::upload() {

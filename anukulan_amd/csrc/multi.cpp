@@ -619,11 +619,25 @@ void Engine::upload() {
       be_.upload_staged(h.data(), const_cast<double*>(d), h.size());
       return;
     }
+    // The participating LPs' ranges go through one pinned staging area,
+    // each copied asynchronously, with one wait per vector.
+    size_t need = 0;
+    for (Int l = 0; l < L_; ++l)
+      if (runs_[uz(l)].participates) need += uz(off[uz(l) + 1] - off[uz(l)]);
+    if (need == 0) return;
+    double* stage = be_.pinned_scratch(need);
+    size_t at = 0;
     for (Int l = 0; l < L_; ++l) {
       if (!runs_[uz(l)].participates) continue;
       const Int a = off[uz(l)], b = off[uz(l) + 1];
-      if (b > a) be_.upload_staged(h.data() + a, const_cast<double*>(d) + a, uz(b - a));
+      if (b <= a) continue;
+      std::copy(h.begin() + a, h.begin() + b, stage + at);
+      hip_check(hipMemcpyAsync(const_cast<double*>(d) + a, stage + at, uz(b - a) * sizeof(double),
+                               hipMemcpyHostToDevice, be_.stream()),
+                "upload stack ranges");
+      at += uz(b - a);
     }
+    be_.synchronize();
   };
   put_range(*data_.c, prob_.c, st_.col_off);
   put_range(*data_.lower, prob_.lower, st_.col_off);
@@ -663,15 +677,28 @@ void Engine::upload() {
     put_range(x, X_[0], st_.col_off);
     put_range(y, Y_[0], st_.row_off);
   } else {
-    std::vector<double> x, y;
+    // Each participating LP's start point is written straight into pinned
+    // staging and copied asynchronously; one wait covers them all.
+    size_t need = 0;
+    for (Int l = 0; l < L_; ++l)
+      if (runs_[uz(l)].participates) need += uz(st_.cols(l)) + uz(st_.rows(l));
+    double* stage = be_.pinned_scratch(std::max<size_t>(need, 1));
+    size_t at = 0;
     for (Int l = 0; l < L_; ++l) {
       if (!runs_[uz(l)].participates) continue;
-      x.assign(uz(st_.cols(l)), 0.0);
-      y.assign(uz(st_.rows(l)), 0.0);
-      start_point(l, x.data(), y.data());
-      if (!x.empty()) be_.upload_staged(x.data(), X_[0] + st_.col_off[uz(l)], x.size());
-      if (!y.empty()) be_.upload_staged(y.data(), Y_[0] + st_.row_off[uz(l)], y.size());
+      const size_t nc = uz(st_.cols(l)), nr = uz(st_.rows(l));
+      start_point(l, stage + at, stage + at + nc);
+      if (nc > 0)
+        hip_check(hipMemcpyAsync(X_[0] + st_.col_off[uz(l)], stage + at, nc * sizeof(double),
+                                 hipMemcpyHostToDevice, be_.stream()),
+                  "upload start points");
+      if (nr > 0)
+        hip_check(hipMemcpyAsync(Y_[0] + st_.row_off[uz(l)], stage + at + nc, nr * sizeof(double),
+                                 hipMemcpyHostToDevice, be_.stream()),
+                  "upload start points");
+      at += nc + nr;
     }
+    be_.synchronize();
   }
   be_.copy(X_[0], RX_, st_.n);
   be_.copy(X_[0], PRX_, st_.n);

@@ -695,6 +695,32 @@ __global__ void fill_kernel(double* p, Int n, double v) {
   if (i < n) p[i] = v;
 }
 
+// Bounds of the dual feasibility problem: a finite bound becomes 0, an
+// infinite one stays.
+__global__ void dual_bounds_kernel(Int n, const double* __restrict__ lo, const double* __restrict__ hi,
+                                   double* __restrict__ dlo, double* __restrict__ dhi) {
+  const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (i < n) {
+    const double l = lo[i], h = hi[i];
+    dlo[i] = l > -__builtin_inf() ? 0.0 : l;
+    dhi[i] = h < __builtin_inf() ? 0.0 : h;
+  }
+}
+
+// The cold primal start (pdhg.cpp:479-503 with no warm start): zero moved
+// into the box, onto a finite bound it violates, else left at zero.
+__global__ void cold_start_kernel(Int n, const double* __restrict__ lo, const double* __restrict__ hi,
+                                  double* __restrict__ x) {
+  const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (i < n) {
+    const double l = lo[i], h = hi[i];
+    double v = 0.0;
+    if (v < l) v = isfinite(l) ? l : 0.0;
+    if (v > h) v = isfinite(h) ? h : 0.0;
+    x[i] = v;
+  }
+}
+
 __global__ void accumulate_kernel(Int n, double w, const double* v, double* sum) {
   const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
   if (i < n) sum[i] += w * v[i];
@@ -906,6 +932,19 @@ void launch_fill(hipStream_t s, double* p, Int n, double v) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
   check_launch("fill");
+}
+
+void launch_dual_bounds(hipStream_t s, Int n, const double* lo, const double* hi, double* dlo,
+                        double* dhi) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(dual_bounds_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, n, lo, hi, dlo, dhi);
+  check_launch("dual bounds");
+}
+
+void launch_cold_start(hipStream_t s, Int n, const double* lo, const double* hi, double* x) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(cold_start_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, n, lo, hi, x);
+  check_launch("cold start");
 }
 
 void launch_accumulate(hipStream_t s, Int n, double w, const double* v, double* sum) {

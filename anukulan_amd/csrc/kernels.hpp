@@ -155,6 +155,11 @@ void launch_spmv(hipStream_t s, const DevCsr& a, const double* x, double* y,
                  const LaunchEvents* ev = nullptr);
 
 void launch_fill(hipStream_t s, double* p, Int n, double v);
+// dlo/dhi = lo/hi with every finite bound replaced by 0.
+void launch_dual_bounds(hipStream_t s, Int n, const double* lo, const double* hi, double* dlo,
+                        double* dhi);
+// x = the cold start in the box [lo, hi]: 0, or the finite bound it violates.
+void launch_cold_start(hipStream_t s, Int n, const double* lo, const double* hi, double* x);
 void launch_accumulate(hipStream_t s, Int n, double w, const double* v, double* sum);
 void launch_scale_into(hipStream_t s, Int n, double w, const double* sum, double* out);
 void launch_primal_update(hipStream_t s, Int n, double tau, double gamma, double h,

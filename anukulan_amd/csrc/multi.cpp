@@ -42,10 +42,9 @@ struct Stack {
   std::vector<double> c, q, lower, upper;  // scaled
   std::vector<double> oc, oq, olo, ohi;    // as given
   std::vector<Scaling> scaling;            // per LP
-  // The polishing sub-problems' data, stack-wide (built once per solve):
-  // zeros and ones of both sizes, and the dual problem's bounds (the finite
-  // ones zeroed).
-  std::vector<double> zero_n, zero_m, ones_n, ones_m, dual_lower, dual_upper;
+  // The dual feasibility problem's bounds (the finite ones zeroed), read on
+  // the host for the dual polishing start points.
+  std::vector<double> dual_lower, dual_upper;
   Int cols(Int l) const { return col_off[uz(l) + 1] - col_off[uz(l)]; }
   Int rows(Int l) const { return row_off[uz(l) + 1] - row_off[uz(l)]; }
 };
@@ -129,6 +128,13 @@ struct DeviceStack {
   const Int* col_lp_chunks = nullptr;
   const Int* row_lp_chunks = nullptr;
   Int col_chunks = 0, row_chunks = 0;
+  // The stack's vectors on the device, in the stack's layout (set by
+  // scale_stack): scaled, as given, the scales, and the dual problem's
+  // bounds. Engines copy their ranges from these on the device.
+  const double *c = nullptr, *q = nullptr, *lower = nullptr, *upper = nullptr;
+  const double *oc = nullptr, *oq = nullptr, *olo = nullptr, *ohi = nullptr;
+  const double *row_scale = nullptr, *col_scale = nullptr;
+  const double *dual_lower = nullptr, *dual_upper = nullptr;
   std::vector<void*> owned;
 
   explicit DeviceStack(HipBackend& b) : be(b) {}
@@ -273,7 +279,7 @@ void upload_stack(const Stack& st, DeviceStack* ds) {
 // the stack's sweeps are each LP's sweeps, with the same operations in the
 // same order: every LP gets exactly its own scale_lp's bits. The scaled
 // vectors and the diagonals come back to the host.
-void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, const DeviceStack& ds) {
+void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, DeviceStack* ds) {
   const Int m = st->m, n = st->n;
   st->c = st->oc;
   st->q = st->oq;
@@ -281,38 +287,56 @@ void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, const 
   st->upper = st->ohi;
   st->row_scale.assign(uz(m), 1.0);
   st->col_scale.assign(uz(n), 1.0);
-  const bool scales = opt.ruiz_iterations > 0 || opt.pock_chambolle != 0;
-  if (scales) {
-    hipStream_t s = be.stream();
-    int slot = kMultiScratchSlot + 80;
-    auto put = [&](const std::vector<double>& h) {
-      double* d = be.scratch(slot++, std::max<size_t>(h.size(), 1));
-      if (!h.empty())
-        hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s),
-                  "stack scaling");
-      return d;
-    };
-    double* q = put(st->q);
-    double* c = put(st->c);
-    double* lo = put(st->lower);
-    double* hi = put(st->upper);
-    double* rs = put(st->row_scale);
-    double* cs = put(st->col_scale);
-    double* rn = be.scratch(slot++, uz(m) + 1);
-    double* cn = be.scratch(slot++, uz(n) + 1);
-    double* d1 = be.scratch(slot++, uz(m) + 1);
-    double* d2 = be.scratch(slot++, uz(n) + 1);
+  hipStream_t s = be.stream();
+  int slot = kMultiScratchSlot + 100;
+  auto dev = [&](Int len) { return be.scratch(slot++, uz(std::max<Int>(len, 1))); };
+  auto put = [&](const std::vector<double>& h) {
+    double* d = dev(static_cast<Int>(h.size()));
+    if (!h.empty())
+      hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s),
+                "stack vectors");
+    return d;
+  };
+  auto clone = [&](const double* from, Int len) {
+    double* d = dev(len);
+    if (len > 0)
+      hip_check(hipMemcpyAsync(d, from, uz(len) * sizeof(double), hipMemcpyDeviceToDevice, s),
+                "stack vectors");
+    return d;
+  };
+  // The vectors as given cross to the device once; the scaled ones start as
+  // copies of them there, and the scales as ones.
+  double* oq = put(st->oq);
+  double* oc = put(st->oc);
+  double* olo = put(st->olo);
+  double* ohi = put(st->ohi);
+  double* q = clone(oq, m);
+  double* c = clone(oc, n);
+  double* lo = clone(olo, n);
+  double* hi = clone(ohi, n);
+  double* rs = dev(m);
+  double* cs = dev(n);
+  launch_fill(s, rs, m, 1.0);
+  launch_fill(s, cs, n, 1.0);
+  if (opt.ruiz_iterations > 0 || opt.pock_chambolle != 0) {
+    int tmp = kMultiScratchSlot + 80;
+    double* rn = be.scratch(tmp++, uz(m) + 1);
+    double* cn = be.scratch(tmp++, uz(n) + 1);
+    double* d1 = be.scratch(tmp++, uz(m) + 1);
+    double* d2 = be.scratch(tmp++, uz(n) + 1);
     auto sweep = [&](bool one_norm) {
-      launch_row_norms(s, ds.k.start, m, ds.k_value, one_norm, rn);
-      launch_row_norms(s, ds.kt.start, n, ds.kt_value, one_norm, cn);
+      launch_row_norms(s, ds->k.start, m, ds->k_value, one_norm, rn);
+      launch_row_norms(s, ds->kt.start, n, ds->kt_value, one_norm, cn);
       launch_inv_sqrt(s, rn, m, d1);
       launch_inv_sqrt(s, cn, n, d2);
-      launch_scale_values(s, ds.k.start, ds.k.index, m, ds.k_value, d1, d2, true);
-      launch_scale_values(s, ds.kt.start, ds.kt.index, n, ds.kt_value, d1, d2, false);
+      launch_scale_values(s, ds->k.start, ds->k.index, m, ds->k_value, d1, d2, true);
+      launch_scale_values(s, ds->kt.start, ds->kt.index, n, ds->kt_value, d1, d2, false);
       launch_scale_vectors(s, m, d1, q, rs, n, d2, c, lo, hi, cs);
     };
     for (int i = 0; i < opt.ruiz_iterations; ++i) sweep(false);
     if (opt.pock_chambolle) sweep(true);
+    // The host keeps the scaled vectors for the per-LP norms, start points
+    // and unscaling.
     auto get = [&](const double* d, std::vector<double>* h) {
       if (!h->empty())
         hip_check(hipMemcpyAsync(h->data(), d, h->size() * sizeof(double), hipMemcpyDeviceToHost, s),
@@ -324,17 +348,29 @@ void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, const 
     get(hi, &st->upper);
     get(rs, &st->row_scale);
     get(cs, &st->col_scale);
-    be.synchronize();
   }
-  st->zero_n.assign(uz(n), 0.0);
-  st->zero_m.assign(uz(m), 0.0);
-  st->ones_n.assign(uz(n), 1.0);
-  st->ones_m.assign(uz(m), 1.0);
-  st->dual_lower = st->lower;
-  st->dual_upper = st->upper;
+  double* dlo = dev(n);
+  double* dhi = dev(n);
+  launch_dual_bounds(s, n, lo, hi, dlo, dhi);
+  ds->c = c;
+  ds->q = q;
+  ds->lower = lo;
+  ds->upper = hi;
+  ds->oc = oc;
+  ds->oq = oq;
+  ds->olo = olo;
+  ds->ohi = ohi;
+  ds->row_scale = rs;
+  ds->col_scale = cs;
+  ds->dual_lower = dlo;
+  ds->dual_upper = dhi;
+  be.synchronize();
+  st->dual_lower.resize(uz(n));
+  st->dual_upper.resize(uz(n));
   for (Int j = 0; j < n; ++j) {
-    if (st->dual_lower[uz(j)] > -kInf) st->dual_lower[uz(j)] = 0.0;
-    if (st->dual_upper[uz(j)] < kInf) st->dual_upper[uz(j)] = 0.0;
+    const double l = st->lower[uz(j)], h = st->upper[uz(j)];
+    st->dual_lower[uz(j)] = l > -kInf ? 0.0 : l;
+    st->dual_upper[uz(j)] = h < kInf ? 0.0 : h;
   }
   for (Int l = 0; l < st->lps; ++l) {
     Scaling& sc = st->scaling[uz(l)];
@@ -447,17 +483,19 @@ struct Outcome {
 };
 
 // Iteration and convergence data of an engine, concatenated over the stack.
+// One of an engine's problem vectors: where its device copy comes from (a
+// device array in the stack's layout, copied range by range, or, when
+// `dev` is null, the constant `fill`), and its host copy, read for the
+// per-LP norms (objective, right-hand side) and start points (bounds); a
+// null host copy reads as zeros.
+struct Vec {
+  const double* dev = nullptr;
+  const std::vector<double>* host = nullptr;
+  double fill = 0.0;
+};
+
 struct EngineData {
-  const std::vector<double>* c;
-  const std::vector<double>* q;
-  const std::vector<double>* lower;
-  const std::vector<double>* upper;
-  const std::vector<double>* oc;
-  const std::vector<double>* oq;
-  const std::vector<double>* olo;
-  const std::vector<double>* ohi;
-  const std::vector<double>* row_scale;
-  const std::vector<double>* col_scale;
+  Vec c, q, lower, upper, oc, oq, olo, ohi, row_scale, col_scale;
 };
 
 struct PolishContext {
@@ -610,45 +648,41 @@ void Engine::allocate() {
 }
 
 void Engine::upload() {
-  // A sub-engine (polishing a few LPs) uploads only its LPs' ranges.
+  // A sub-engine (polishing a few LPs) fills only its LPs' ranges.
   Int taking = 0;
   for (const LpRun& r : runs_) taking += r.participates ? 1 : 0;
   const bool whole = taking == L_;
-  auto put_range = [&](const std::vector<double>& h, const double* d, const std::vector<Int>& off) {
+  hipStream_t s = be_.stream();
+  // Copied on the device from the stack's arrays, or filled with a
+  // constant; the slab is zeroed when allocated, so zeros need no work.
+  auto put_range = [&](const Vec& v, const double* to, const std::vector<Int>& off) {
+    if (v.dev == nullptr && v.fill == 0.0) return;
+    double* d = const_cast<double*>(to);  // the engine's own slab
+    auto one = [&](Int a, Int b) {
+      if (b <= a) return;
+      if (v.dev != nullptr)
+        hip_check(hipMemcpyAsync(d + a, v.dev + a, uz(b - a) * sizeof(double), hipMemcpyDeviceToDevice, s),
+                  "engine vectors");
+      else
+        launch_fill(s, d + a, b - a, v.fill);
+    };
     if (whole) {
-      be_.upload_staged(h.data(), const_cast<double*>(d), h.size());
+      one(0, off.back());
       return;
     }
-    // The participating LPs' ranges go through one pinned staging area,
-    // each copied asynchronously, with one wait per vector.
-    size_t need = 0;
     for (Int l = 0; l < L_; ++l)
-      if (runs_[uz(l)].participates) need += uz(off[uz(l) + 1] - off[uz(l)]);
-    if (need == 0) return;
-    double* stage = be_.pinned_scratch(need);
-    size_t at = 0;
-    for (Int l = 0; l < L_; ++l) {
-      if (!runs_[uz(l)].participates) continue;
-      const Int a = off[uz(l)], b = off[uz(l) + 1];
-      if (b <= a) continue;
-      std::copy(h.begin() + a, h.begin() + b, stage + at);
-      hip_check(hipMemcpyAsync(const_cast<double*>(d) + a, stage + at, uz(b - a) * sizeof(double),
-                               hipMemcpyHostToDevice, be_.stream()),
-                "upload stack ranges");
-      at += uz(b - a);
-    }
-    be_.synchronize();
+      if (runs_[uz(l)].participates) one(off[uz(l)], off[uz(l) + 1]);
   };
-  put_range(*data_.c, prob_.c, st_.col_off);
-  put_range(*data_.lower, prob_.lower, st_.col_off);
-  put_range(*data_.upper, prob_.upper, st_.col_off);
-  put_range(*data_.oc, prob_.orig_c, st_.col_off);
-  put_range(*data_.olo, prob_.orig_lower, st_.col_off);
-  put_range(*data_.ohi, prob_.orig_upper, st_.col_off);
-  put_range(*data_.col_scale, prob_.col_scale, st_.col_off);
-  put_range(*data_.q, prob_.q, st_.row_off);
-  put_range(*data_.oq, prob_.orig_q, st_.row_off);
-  put_range(*data_.row_scale, prob_.row_scale, st_.row_off);
+  put_range(data_.c, prob_.c, st_.col_off);
+  put_range(data_.lower, prob_.lower, st_.col_off);
+  put_range(data_.upper, prob_.upper, st_.col_off);
+  put_range(data_.oc, prob_.orig_c, st_.col_off);
+  put_range(data_.olo, prob_.orig_lower, st_.col_off);
+  put_range(data_.ohi, prob_.orig_upper, st_.col_off);
+  put_range(data_.col_scale, prob_.col_scale, st_.col_off);
+  put_range(data_.q, prob_.q, st_.row_off);
+  put_range(data_.oq, prob_.orig_q, st_.row_off);
+  put_range(data_.row_scale, prob_.row_scale, st_.row_off);
   // Starting points (pdhg.cpp:479-503): a warm start put in the dual cone and
   // the primal box, a cold one the projection of zero. Built per LP, so a
   // sub-engine polishing a few LPs touches only their ranges on the host.
@@ -662,7 +696,7 @@ void Engine::upload() {
       y[uz(i)] = v;
     }
     for (Int j = 0; j < st_.cols(l); ++j) {
-      const double lo = (*data_.lower)[uz(c0 + j)], hi = (*data_.upper)[uz(c0 + j)];
+      const double lo = (*data_.lower.host)[uz(c0 + j)], hi = (*data_.upper.host)[uz(c0 + j)];
       double v = warm ? r.warm_x[uz(j)] : 0.0;
       if (!std::isfinite(v)) v = 0.0;
       if (v < lo) v = std::isfinite(lo) ? lo : 0.0;
@@ -670,12 +704,18 @@ void Engine::upload() {
       x[uz(j)] = v;
     }
   };
-  if (whole) {
+  const bool warm = std::any_of(runs_.begin(), runs_.end(), [](const LpRun& r) {
+    return r.participates && !r.warm_x.empty() && !r.warm_y.empty();
+  });
+  if (whole && !warm) {
+    // Every LP cold: y stays zero (the slab), x is zero moved into the box.
+    launch_cold_start(s, st_.n, prob_.lower, prob_.upper, X_[0]);
+  } else if (whole) {
     std::vector<double> x(uz(st_.n), 0.0), y(uz(st_.m), 0.0);
     for (Int l = 0; l < L_; ++l)
       start_point(l, x.data() + st_.col_off[uz(l)], y.data() + st_.row_off[uz(l)]);
-    put_range(x, X_[0], st_.col_off);
-    put_range(y, Y_[0], st_.row_off);
+    be_.upload_staged(x.data(), X_[0], x.size());
+    be_.upload_staged(y.data(), Y_[0], y.size());
   } else {
     // Each participating LP's start point is written straight into pinned
     // staging and copied asynchronously; one wait covers them all.
@@ -841,12 +881,6 @@ std::vector<Polished> polish_stack(HipBackend& be, const Stack& st, const Device
   sub.time_limit_seconds = std::fmax(0.0, opt.time_limit_seconds - seconds_since(start));
   sub.warm_x = nullptr;
   sub.warm_y = nullptr;
-  const std::vector<double>& ones_n = st.ones_n;
-  const std::vector<double>& ones_m = st.ones_m;
-  const std::vector<double>& zero_n = st.zero_n;
-  const std::vector<double>& zero_m = st.zero_m;
-  const std::vector<double>& dlo = st.dual_lower;
-  const std::vector<double>& dhi = st.dual_upper;
   auto runs_for = [&](bool primal) {
     std::vector<LpRun> runs(uz(st.lps));
     for (LpRun& r : runs) r.participates = false;
@@ -868,11 +902,16 @@ std::vector<Polished> polish_stack(HipBackend& be, const Stack& st, const Device
   };
   std::vector<LpRun> pr = runs_for(true), dr = runs_for(false);
   sub.termination = ANUK_TERM_PRIMAL_FEASIBILITY;
-  const EngineData pdata{&zero_n, &st.q, &st.lower, &st.upper, &zero_n, &st.q, &st.lower, &st.upper,
-                         &ones_m, &ones_n};
+  // The feasibility problems in the scaled space (scales of one): the
+  // primal one drops the objective, the dual one the right-hand side and
+  // puts the dual problem's bounds on the columns.
+  const Vec zero{}, one{nullptr, nullptr, 1.0};
+  const Vec q{ds.q, &st.q}, lo{ds.lower, &st.lower}, hi{ds.upper, &st.upper};
+  const Vec c{ds.c, &st.c}, dlo{ds.dual_lower, &st.dual_lower}, dhi{ds.dual_upper, &st.dual_upper};
+  const EngineData pdata{zero, q, lo, hi, zero, q, lo, hi, one, one};
   const std::vector<Outcome> po = run_engine(be, st, ds, norms, sub, pdata, pr, start, nullptr, nullptr, 1);
   sub.termination = ANUK_TERM_DUAL_FEASIBILITY;
-  const EngineData ddata{&st.c, &zero_m, &dlo, &dhi, &st.c, &zero_m, &dlo, &dhi, &ones_m, &ones_n};
+  const EngineData ddata{c, zero, dlo, dhi, c, zero, dlo, dhi, one, one};
   const std::vector<Outcome> dq = run_engine(be, st, ds, norms, sub, ddata, dr, start, nullptr, nullptr, 1);
   std::vector<Polished> out(who.size());
   for (size_t s = 0; s < who.size(); ++s) {
@@ -918,15 +957,18 @@ void Engine::run_polish(const std::vector<Int>& who, Int budget, std::vector<Out
   }
 }
 
-double range_two_norm(const std::vector<double>& v, Int a, Int b) {
+// Norms of a range of a host vector; a null vector is all zeros.
+double range_two_norm(const std::vector<double>* v, Int a, Int b) {
   double s = 0.0;
-  for (Int i = a; i < b; ++i) s += v[uz(i)] * v[uz(i)];
+  if (v != nullptr)
+    for (Int i = a; i < b; ++i) s += (*v)[uz(i)] * (*v)[uz(i)];
   return std::sqrt(s);
 }
 
-double range_inf_norm(const std::vector<double>& v, Int a, Int b) {
+double range_inf_norm(const std::vector<double>* v, Int a, Int b) {
   double s = 0.0;
-  for (Int i = a; i < b; ++i) s = std::fmax(s, std::fabs(v[uz(i)]));
+  if (v != nullptr)
+    for (Int i = a; i < b; ++i) s = std::fmax(s, std::fabs((*v)[uz(i)]));
   return s;
 }
 
@@ -941,12 +983,12 @@ std::vector<Outcome> Engine::run() {
     if (!r.participates) continue;
     const Int c0 = st_.col_off[uz(l)], c1 = st_.col_off[uz(l) + 1];
     const Int r0 = st_.row_off[uz(l)], r1 = st_.row_off[uz(l) + 1];
-    r.q_two = range_two_norm(*data_.oq, r0, r1);
-    r.c_two = range_two_norm(*data_.oc, c0, c1);
-    r.q_inf = range_inf_norm(*data_.oq, r0, r1);
-    r.c_inf = range_inf_norm(*data_.oc, c0, c1);
+    r.q_two = range_two_norm(data_.oq.host, r0, r1);
+    r.c_two = range_two_norm(data_.oc.host, c0, c1);
+    r.q_inf = range_inf_norm(data_.oq.host, r0, r1);
+    r.c_inf = range_inf_norm(data_.oc.host, c0, c1);
     Control& c = ctl_[uz(l)];
-    c.init(opt_, norms_[uz(l)], range_two_norm(*data_.c, c0, c1), range_two_norm(*data_.q, r0, r1));
+    c.init(opt_, norms_[uz(l)], range_two_norm(data_.c.host, c0, c1), range_two_norm(data_.q.host, r0, r1));
     if (r.initial_eta > 0.0) c.eta = r.initial_eta;
     if (r.initial_omega > 0.0) c.omega = r.initial_omega;
     active_[uz(l)] = norms_[uz(l)] > 0.0 ? 1 : 0;
@@ -1123,15 +1165,16 @@ std::vector<SolveResult> solve_pdhg_multi(HipBackend& be, const std::vector<cons
   Stack st = concat_stack(lps);
   DeviceStack ds(be);
   upload_stack(st, &ds);
-  scale_stack(be, opt, &st, ds);
+  scale_stack(be, opt, &st, &ds);
   lap(&Phases::scaling);
   std::vector<double> norms(uz(st.lps), opt.known_matrix_norm);
   if (!(opt.known_matrix_norm > 0.0)) norms = stack_norms(be, st, ds, 200, 1e-8);
   lap(&Phases::matrix_norm);
 
   std::vector<LpRun> runs(uz(st.lps));
-  const EngineData data{&st.c, &st.q, &st.lower, &st.upper, &st.oc, &st.oq, &st.olo, &st.ohi,
-                        &st.row_scale, &st.col_scale};
+  const EngineData data{{ds.c, &st.c},   {ds.q, &st.q},     {ds.lower, &st.lower}, {ds.upper, &st.upper},
+                        {ds.oc, &st.oc}, {ds.oq, &st.oq},   {ds.olo, &st.olo},     {ds.ohi, &st.ohi},
+                        {ds.row_scale},  {ds.col_scale}};
   PolishContext pc{&lps};
   std::vector<Outcome> outcomes = run_engine(be, st, ds, norms, opt, data, runs, start,
                                              opt.polish_feasibility ? &pc : nullptr, &phases, 0);

@@ -423,11 +423,39 @@ std::vector<double> stack_norms(HipBackend& be, const Stack& st, const DeviceSta
               "norm flags");
   };
   put_flags();
+  // The products run over the blocks of the LPs still iterating (a block
+  // map); the map is rebuilt once a quarter of its blocks belong to LPs
+  // that have finished.
+  DevCsr k_run = ds.k, kt_run = ds.kt;
+  std::vector<Int> km, ktm;
+  auto compact = [&] {
+    // Queued products may still read the previous maps.
+    be.synchronize();
+    km.clear();
+    ktm.clear();
+    for (Int l = 0; l < L; ++l) {
+      if (!on[uz(l)]) continue;
+      for (Int b = ds.k_lp_blocks_host[uz(l)]; b < ds.k_lp_blocks_host[uz(l) + 1]; ++b) km.push_back(b);
+      for (Int b = ds.kt_lp_blocks_host[uz(l)]; b < ds.kt_lp_blocks_host[uz(l) + 1]; ++b) ktm.push_back(b);
+    }
+    auto put = [&](int slot, const std::vector<Int>& h) {
+      Int* d = reinterpret_cast<Int*>(be.scratch(slot, h.size() / 2 + 1));
+      if (!h.empty())
+        hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(Int), hipMemcpyHostToDevice, be.stream()),
+                  "norm block map");
+      return d;
+    };
+    k_run.block_map = put(kMultiScratchSlot + 97, km);
+    kt_run.block_map = put(kMultiScratchSlot + 98, ktm);
+    k_run.blocks = static_cast<Int>(km.size());
+    kt_run.blocks = static_cast<Int>(ktm.size());
+  };
+  compact();
   std::vector<double> out(uz(L)), div(uz(L), 1.0);
   hipStream_t s = be.stream();
   for (int it = 0; it < iterations && std::any_of(on.begin(), on.end(), [](char c) { return c; }); ++it) {
-    launch_multi_product(s, ds.k, ds.k_block_lp, d_v, d_kv, d_params);
-    launch_multi_product(s, ds.kt, ds.kt_block_lp, d_kv, d_u, d_params);
+    launch_multi_product(s, k_run, ds.k_block_lp, d_v, d_kv, d_params);
+    launch_multi_product(s, kt_run, ds.kt_block_lp, d_kv, d_u, d_params);
     launch_multi_sumsq(s, ds.cols, d_u, nullptr, nullptr, nullptr, false, d_params, d_part,
                        ds.col_lp_chunks, L, d_out, 1, 0);
     hip_check(hipMemcpyAsync(out.data(), d_out, uz(L) * sizeof(double), hipMemcpyDeviceToHost, s), "norms");
@@ -454,7 +482,13 @@ std::vector<double> stack_norms(HipBackend& be, const Stack& st, const DeviceSta
       norm[uz(l)] = estimate;
     }
     hip_check(hipMemcpyAsync(d_div, div.data(), uz(L) * sizeof(double), hipMemcpyHostToDevice, s), "divisors");
-    if (changed) put_flags();
+    if (changed) {
+      put_flags();
+      Int live = 0;
+      for (Int l = 0; l < L; ++l)
+        if (on[uz(l)]) live += ds.k_lp_blocks_host[uz(l) + 1] - ds.k_lp_blocks_host[uz(l)];
+      if (4 * live < 3 * k_run.blocks) compact();
+    }
     launch_multi_divide(s, ds.cols, d_u, d_div, d_params, d_v);
   }
   be.synchronize();
@@ -938,10 +972,15 @@ void Engine::run_polish(const std::vector<Int>& who, Int budget, std::vector<Out
   be_.synchronize();
   std::vector<Polished> pol = polish_stack(be_, st_, ds_, norms_, opt_, who, sx, sy, omega, eta,
                                            budget, start_);
+  // Each polished point is checked on the original LP, one LP per host thread.
+  std::vector<Residual> checked(who.size());
+  parallel_for(who.size(), [&](size_t s) {
+    checked[s] = evaluate_residual(*(*polish_->originals)[uz(who[s])], pol[s].x, pol[s].y);
+  });
   for (size_t s = 0; s < who.size(); ++s) {
     const Int l = who[s];
     Outcome& o = out[uz(l)];
-    const Residual tr = evaluate_residual(*(*polish_->originals)[uz(l)], pol[s].x, pol[s].y);
+    const Residual& tr = checked[s];
     ++o.polish_attempts;
     o.polish_iterations += pol[s].spent;
     Control& c = ctl_[uz(l)];

@@ -682,30 +682,25 @@ void Engine::allocate() {
 }
 
 void Engine::upload() {
-  // A sub-engine (polishing a few LPs) fills only its LPs' ranges.
+  // A sub-engine (polishing a few LPs) builds start points for its LPs only.
   Int taking = 0;
   for (const LpRun& r : runs_) taking += r.participates ? 1 : 0;
   const bool whole = taking == L_;
   hipStream_t s = be_.stream();
   // Copied on the device from the stack's arrays, or filled with a
-  // constant; the slab is zeroed when allocated, so zeros need no work.
+  // constant; the slab is zeroed when allocated, so zeros need no work. A
+  // sub-engine takes whole vectors too: one copy costs less than one per
+  // LP, and every kernel reads only the ranges of LPs it runs (blocks and
+  // chunks are per LP, gated by each LP's flag).
   auto put_range = [&](const Vec& v, const double* to, const std::vector<Int>& off) {
-    if (v.dev == nullptr && v.fill == 0.0) return;
+    const Int len = off.back();
+    if ((v.dev == nullptr && v.fill == 0.0) || len <= 0) return;
     double* d = const_cast<double*>(to);  // the engine's own slab
-    auto one = [&](Int a, Int b) {
-      if (b <= a) return;
-      if (v.dev != nullptr)
-        hip_check(hipMemcpyAsync(d + a, v.dev + a, uz(b - a) * sizeof(double), hipMemcpyDeviceToDevice, s),
-                  "engine vectors");
-      else
-        launch_fill(s, d + a, b - a, v.fill);
-    };
-    if (whole) {
-      one(0, off.back());
-      return;
-    }
-    for (Int l = 0; l < L_; ++l)
-      if (runs_[uz(l)].participates) one(off[uz(l)], off[uz(l) + 1]);
+    if (v.dev != nullptr)
+      hip_check(hipMemcpyAsync(d, v.dev, uz(len) * sizeof(double), hipMemcpyDeviceToDevice, s),
+                "engine vectors");
+    else
+      launch_fill(s, d, len, v.fill);
   };
   put_range(data_.c, prob_.c, st_.col_off);
   put_range(data_.lower, prob_.lower, st_.col_off);

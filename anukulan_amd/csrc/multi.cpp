@@ -42,9 +42,6 @@ struct Stack {
   std::vector<double> c, q, lower, upper;  // scaled
   std::vector<double> oc, oq, olo, ohi;    // as given
   std::vector<Scaling> scaling;            // per LP
-  // The dual feasibility problem's bounds (the finite ones zeroed), read on
-  // the host for the dual polishing start points.
-  std::vector<double> dual_lower, dual_upper;
   Int cols(Int l) const { return col_off[uz(l) + 1] - col_off[uz(l)]; }
   Int rows(Int l) const { return row_off[uz(l) + 1] - row_off[uz(l)]; }
 };
@@ -281,12 +278,19 @@ void upload_stack(const Stack& st, DeviceStack* ds) {
 // vectors and the diagonals come back to the host.
 void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, DeviceStack* ds) {
   const Int m = st->m, n = st->n;
-  st->c = st->oc;
-  st->q = st->oq;
-  st->lower = st->olo;
-  st->upper = st->ohi;
-  st->row_scale.assign(uz(m), 1.0);
-  st->col_scale.assign(uz(n), 1.0);
+  const bool sweeps = opt.ruiz_iterations > 0 || opt.pock_chambolle != 0;
+  if (sweeps) {
+    // Read back from the device below.
+    for (std::vector<double>* v : {&st->c, &st->lower, &st->upper, &st->col_scale}) v->resize(uz(n));
+    for (std::vector<double>* v : {&st->q, &st->row_scale}) v->resize(uz(m));
+  } else {
+    st->c = st->oc;
+    st->q = st->oq;
+    st->lower = st->olo;
+    st->upper = st->ohi;
+    st->row_scale.assign(uz(m), 1.0);
+    st->col_scale.assign(uz(n), 1.0);
+  }
   hipStream_t s = be.stream();
   int slot = kMultiScratchSlot + 100;
   auto dev = [&](Int len) { return be.scratch(slot++, uz(std::max<Int>(len, 1))); };
@@ -318,7 +322,7 @@ void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, Device
   double* cs = dev(n);
   launch_fill(s, rs, m, 1.0);
   launch_fill(s, cs, n, 1.0);
-  if (opt.ruiz_iterations > 0 || opt.pock_chambolle != 0) {
+  if (sweeps) {
     int tmp = kMultiScratchSlot + 80;
     double* rn = be.scratch(tmp++, uz(m) + 1);
     double* cn = be.scratch(tmp++, uz(n) + 1);
@@ -365,18 +369,11 @@ void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, Device
   ds->dual_lower = dlo;
   ds->dual_upper = dhi;
   be.synchronize();
-  st->dual_lower.resize(uz(n));
-  st->dual_upper.resize(uz(n));
-  for (Int j = 0; j < n; ++j) {
-    const double l = st->lower[uz(j)], h = st->upper[uz(j)];
-    st->dual_lower[uz(j)] = l > -kInf ? 0.0 : l;
-    st->dual_upper[uz(j)] = h < kInf ? 0.0 : h;
-  }
-  for (Int l = 0; l < st->lps; ++l) {
-    Scaling& sc = st->scaling[uz(l)];
-    sc.row_scale.assign(st->row_scale.begin() + st->row_off[uz(l)], st->row_scale.begin() + st->row_off[uz(l) + 1]);
-    sc.col_scale.assign(st->col_scale.begin() + st->col_off[uz(l)], st->col_scale.begin() + st->col_off[uz(l) + 1]);
-  }
+  parallel_for(uz(st->lps), [&](size_t l) {
+    Scaling& sc = st->scaling[l];
+    sc.row_scale.assign(st->row_scale.begin() + st->row_off[l], st->row_scale.begin() + st->row_off[l + 1]);
+    sc.col_scale.assign(st->col_scale.begin() + st->col_off[l], st->col_scale.begin() + st->col_off[l + 1]);
+  });
 }
 
 // Device-resident per-LP flags as IterParams (only `active` is read).
@@ -725,7 +722,8 @@ void Engine::upload() {
       y[uz(i)] = v;
     }
     for (Int j = 0; j < st_.cols(l); ++j) {
-      const double lo = (*data_.lower.host)[uz(c0 + j)], hi = (*data_.upper.host)[uz(c0 + j)];
+      const double lo = data_.lower.host != nullptr ? (*data_.lower.host)[uz(c0 + j)] : 0.0;
+      const double hi = data_.upper.host != nullptr ? (*data_.upper.host)[uz(c0 + j)] : 0.0;
       double v = warm ? r.warm_x[uz(j)] : 0.0;
       if (!std::isfinite(v)) v = 0.0;
       if (v < lo) v = std::isfinite(lo) ? lo : 0.0;
@@ -936,7 +934,9 @@ std::vector<Polished> polish_stack(HipBackend& be, const Stack& st, const Device
   // puts the dual problem's bounds on the columns.
   const Vec zero{}, one{nullptr, nullptr, 1.0};
   const Vec q{ds.q, &st.q}, lo{ds.lower, &st.lower}, hi{ds.upper, &st.upper};
-  const Vec c{ds.c, &st.c}, dlo{ds.dual_lower, &st.dual_lower}, dhi{ds.dual_upper, &st.dual_upper};
+  // The dual problem starts every LP at x = 0, inside its bounds (0 or
+  // infinite), so its bounds need no host copy for the start points.
+  const Vec c{ds.c, &st.c}, dlo{ds.dual_lower}, dhi{ds.dual_upper};
   const EngineData pdata{zero, q, lo, hi, zero, q, lo, hi, one, one};
   const std::vector<Outcome> po = run_engine(be, st, ds, norms, sub, pdata, pr, start, nullptr, nullptr, 1);
   sub.termination = ANUK_TERM_DUAL_FEASIBILITY;

@@ -6,8 +6,8 @@ the reference's own solve_pdhg reaches on LP i, and what the plain GPU solve
 of LP i reaches. Each lane's row sums and elementwise steps are the
 reference's operations in the reference's order (bit-identical); only the
 check reductions (a fixed tree instead of a serial loop) and ||K|| differ in
-the last bits, and they only steer decisions. The bar is the north star's:
-same status, objective within 1e-8 relative, iterations within 2%.
+the last bits, and they only steer decisions. The bar: the same status,
+the objective within 1e-8 relative, the iteration count within 2%.
 """
 import numpy as np
 import pytest

@@ -1,6 +1,6 @@
 """End-to-end PDHG parity on the GPU.
 
-The bar (BASELINE north star): on the same seeded LP and tolerance, the final
+The bar: on the same seeded LP and tolerance, the final
 objective within 1e-8 relative of the reference CPU PDHG's and the iteration
 count within 2%. The reference numbers come from the committed fixture
 tests/golden/pdhg_reference.json (written by the reference's own solve_pdhg

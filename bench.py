@@ -478,8 +478,8 @@ def main():
                     help="time limit of the random LP's plain solve")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the case stack would shard by case across GPUs, which is not built here)",
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a case stack would shard by case across GPUs, one process each; not implemented)",
               file=sys.stderr)
         sys.exit(2)
 

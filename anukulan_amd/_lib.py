@@ -53,32 +53,6 @@ class MpsInfo(ctypes.Structure):
                 ("objective_nonzeros", c_int64)]
 
 
-class PresolveOptions(ctypes.Structure):
-    _fields_ = [(n, c_int32) for n in (
-        "empty_rows", "singleton_rows", "redundant_rows", "forcing_rows", "duplicate_rows",
-        "fixed_columns", "empty_columns", "free_column_singletons", "bound_tightening",
-        "slack_column_singletons", "inequality_column_singletons", "dual_fixing",
-        "doubleton_equations", "coefficient_tightening", "max_rounds")] + [
-        ("feasibility_tolerance", c_double), ("infeasibility_tolerance", c_double),
-        ("bound_relaxation", c_double), ("max_new_finite_bound", c_double)]
-
-
-PRESOLVE_COUNTS = ("rows_removed", "cols_removed", "nonzeros_removed", "empty_rows",
-                   "singleton_rows", "redundant_rows", "forcing_rows", "duplicate_rows",
-                   "fixed_columns", "empty_columns", "free_column_singletons",
-                   "slack_column_singletons", "inequality_column_singletons",
-                   "slack_singletons_declined", "doubleton_equations", "dual_fixed_columns",
-                   "coefficients_tightened", "bounds_tightened", "bounds_made_finite", "rounds")
-
-
-class PresolveInfo(ctypes.Structure):
-    _fields_ = ([("status", c_int32), ("dual_exact", c_int32), ("original_rows", c_int32),
-                 ("original_cols", c_int32), ("original_nnz", c_int64), ("rows", c_int32),
-                 ("cols", c_int32), ("nnz", c_int64)] +
-                [(n, c_int32) for n in PRESOLVE_COUNTS] +
-                [("largest_new_finite_bound", c_double)])
-
-
 class LpInfo(ctypes.Structure):
     _fields_ = [("rows", c_int32), ("cols", c_int32), ("nnz", c_int32),
                 ("num_equalities", c_int32), ("objective_scale", c_double),
@@ -235,16 +209,6 @@ SIGNATURES = {
     "anuk_mps_warning": (c_char_p, [c_void_p, c_int32]),
     "anuk_mps_write": (c_int, [c_char_p, POINTER(Model), c_char_p, POINTER(c_char_p),
                                POINTER(c_char_p), IP]),
-    "anuk_presolve_default_options": (None, [POINTER(PresolveOptions)]),
-    "anuk_presolve_mps": (c_int, [c_void_p, POINTER(PresolveOptions), POINTER(c_void_p)]),
-    "anuk_presolve_model": (c_int, [POINTER(Model), IP, POINTER(PresolveOptions),
-                                    POINTER(c_void_p)]),
-    "anuk_presolve_free": (None, [c_void_p]),
-    "anuk_presolve_info_get": (c_int, [c_void_p, POINTER(PresolveInfo)]),
-    "anuk_presolve_message": (c_char_p, [c_void_p]),
-    "anuk_presolve_reduced": (c_void_p, [c_void_p]),
-    "anuk_postsolve_primal": (c_int, [c_void_p, DP, DP]),
-    "anuk_postsolve_dual": (c_int, [c_void_p, DP, DP, DP, DP]),
 }
 
 

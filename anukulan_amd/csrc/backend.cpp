@@ -65,15 +65,24 @@ hipEvent_t HipBackend::take_event() {
 
 // Folds the elapsed times of an entry's recorded pairs into its totals and
 // returns the events to the pool. Waits for the stream.
+// A pair whose time cannot be read (its launch was skipped, so the events
+// were never recorded) is counted as dropped, not timed, and the error the
+// query left is cleared so the next launch check does not report it.
 void HipBackend::harvest(ProfileEntry& e) const {
   (void)hipStreamSynchronize(stream_);
   for (auto& p : e.pairs) {
     float t = 0.0f;
-    if (hipEventElapsedTime(&t, p.first, p.second) == hipSuccess) e.ms += static_cast<double>(t);
-    event_pool_.push_back(p.first);
-    event_pool_.push_back(p.second);
+    if (hipEventElapsedTime(&t, p.start, p.stop) == hipSuccess) {
+      e.ms += static_cast<double>(t);
+      e.bytes += p.bytes;
+      ++e.timed;
+    } else {
+      ++e.dropped;
+      (void)hipGetLastError();
+    }
+    event_pool_.push_back(p.start);
+    event_pool_.push_back(p.stop);
   }
-  e.timed += static_cast<long>(e.pairs.size());
   e.pairs.clear();
 }
 
@@ -81,8 +90,8 @@ void HipBackend::clear_profile() {
   (void)hipStreamSynchronize(stream_);
   for (auto& kv : profile_) {
     for (auto& p : kv.second.pairs) {
-      event_pool_.push_back(p.first);
-      event_pool_.push_back(p.second);
+      event_pool_.push_back(p.start);
+      event_pool_.push_back(p.stop);
     }
   }
   profile_.clear();
@@ -96,16 +105,15 @@ LaunchEvents HipBackend::launch_events(const char* name, double bytes) {
   if (static_cast<int>(e.pairs.size()) >= kMaxTimed) harvest(e);
   ev.start = take_event();
   ev.stop = take_event();
-  e.pairs.emplace_back(ev.start, ev.stop);
-  e.bytes += bytes;
+  e.pairs.push_back({ev.start, ev.stop, bytes});
   return ev;
 }
 
-HipBackend::Timed::Timed(HipBackend& b, const char* name, double bytes) : b_(b), name_(name) {
+HipBackend::Timed::Timed(HipBackend& b, const char* name, double bytes)
+    : b_(b), name_(name), bytes_(bytes) {
   if (!b_.profiling_) return;
   auto& e = b_.profile_[name_];
   ++e.calls;
-  e.bytes += bytes;
   if (static_cast<int>(e.pairs.size()) >= kMaxTimed) b_.harvest(e);
   start_ = b_.take_event();
   hip_check(hipEventRecord(start_, b_.stream_), "hipEventRecord");
@@ -124,7 +132,7 @@ HipBackend::Timed::~Timed() {
     return;
   }
   (void)hipEventRecord(stop, b_.stream_);
-  b_.profile_[name_].pairs.emplace_back(start_, stop);
+  b_.profile_[name_].pairs.push_back({start_, stop, bytes_});
 }
 
 // ---- memory ----------------------------------------------------------------
@@ -548,8 +556,9 @@ void HipBackend::set_profiling(bool on) {
   profiling_ = on;
 }
 
-// One line per launch group: calls, timed calls (all of them), mean
-// microseconds, and the total (mean x calls) with its share.
+// One line per launch group: calls, timed calls, mean microseconds, the
+// total (mean x calls) with its share, GB/s of the timed launches'
+// algorithmic bytes, and the pairs whose time could not be read.
 std::string HipBackend::profile_report() const {
   if (profile_.empty()) return {};
   (void)hipStreamSynchronize(stream_);
@@ -557,6 +566,7 @@ std::string HipBackend::profile_report() const {
     std::string name;
     long calls;
     long timed;
+    long dropped;
     double mean_us;
     double total_ms;
     double gbs;  // algorithmic bytes of the timed launches over their time
@@ -570,17 +580,18 @@ std::string HipBackend::profile_report() const {
     const double mean_us = timed > 0 ? 1e3 * ms / static_cast<double>(timed) : 0.0;
     const double est = 1e-3 * mean_us * static_cast<double>(kv.second.calls);
     const double gbs = ms > 0.0 ? kv.second.bytes / (1e-3 * ms) / 1e9 : 0.0;
-    rows.push_back({kv.first, kv.second.calls, timed, mean_us, est, gbs});
+    rows.push_back({kv.first, kv.second.calls, timed, kv.second.dropped, mean_us, est, gbs});
     total += est;
   }
   std::sort(rows.begin(), rows.end(), [](const Row& a, const Row& b) { return a.total_ms > b.total_ms; });
   std::string out =
-      "  kernel                    calls    timed      mean us     total ms   share      GB/s\n";
+      "  kernel                    calls    timed      mean us     total ms   share      GB/s"
+      "  dropped\n";
   char line[220];
   for (const auto& r : rows) {
-    std::snprintf(line, sizeof(line), "  %-22s %8ld %8ld %12.3f %12.3f %6.1f%% %9.1f\n",
+    std::snprintf(line, sizeof(line), "  %-22s %8ld %8ld %12.3f %12.3f %6.1f%% %9.1f %8ld\n",
                   r.name.c_str(), r.calls, r.timed, r.mean_us, r.total_ms,
-                  total > 0 ? 100.0 * r.total_ms / total : 0.0, r.gbs);
+                  total > 0 ? 100.0 * r.total_ms / total : 0.0, r.gbs, r.dropped);
     out += line;
   }
   return out;

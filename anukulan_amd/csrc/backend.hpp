@@ -160,6 +160,7 @@ class HipBackend {
    private:
     HipBackend& b_;
     const char* name_;
+    double bytes_;
     hipEvent_t start_ = nullptr;
     bool active_ = false;
   };
@@ -188,12 +189,17 @@ class HipBackend {
   // name holds kMaxTimed outstanding pairs their times are read (one stream
   // wait) and the events reused, which bounds the event count on long solves.
   static constexpr int kMaxTimed = 4096;
+  struct PendingPair {
+    hipEvent_t start, stop;
+    double bytes;  // algorithmic bytes of the launch this pair times
+  };
   struct ProfileEntry {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
+    std::vector<PendingPair> pairs;
     long calls = 0;
     long timed = 0;      // launches whose time is in `ms`
+    long dropped = 0;    // pairs whose time could not be read
     double ms = 0.0;
-    double bytes = 0.0;  // algorithmic bytes of all timed launches
+    double bytes = 0.0;  // algorithmic bytes of the timed launches
   };
   void harvest(ProfileEntry& e) const;
   mutable std::map<std::string, ProfileEntry> profile_;

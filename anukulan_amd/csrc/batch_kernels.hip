@@ -400,6 +400,7 @@ void launch_rows(K kernel, int waves, Int rows, hipStream_t s, const LaunchEvent
   if (rows <= 0) return;
   const dim3 grid(static_cast<unsigned>((rows + waves - 1) / waves));
   const dim3 block(static_cast<unsigned>(64 * waves));
+  validate_launch(what, grid, block, ev);
   if (ev != nullptr && ev->start != nullptr) {
     hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev->start, ev->stop, 0, args...);
   } else {

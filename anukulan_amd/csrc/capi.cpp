@@ -13,7 +13,6 @@
 #include "backend.hpp"
 #include "lp.hpp"
 #include "mps.hpp"
-#include "presolve.hpp"
 #include "adjoint.hpp"
 #include "batch.hpp"
 #include "multi.hpp"
@@ -33,11 +32,6 @@ struct anuk_lp {
 
 struct anuk_mps {
   anuk::MpsModel m;
-};
-
-struct anuk_presolve {
-  anuk::PresolveOutcome r;
-  anuk_mps reduced;  // the reduced model, as an MPS model handle
 };
 
 namespace {
@@ -970,191 +964,5 @@ int anuk_mps_write(const char* path, const anuk_model* model, const char* name,
   });
 }
 
-// ---- presolve -------------------------------------------------------------
-
-void anuk_presolve_default_options(anuk_presolve_options* o) {
-  if (o == nullptr) return;
-  const anuk::PresolveOptions d;
-  o->empty_rows = d.empty_rows;
-  o->singleton_rows = d.singleton_rows;
-  o->redundant_rows = d.redundant_rows;
-  o->forcing_rows = d.forcing_rows;
-  o->duplicate_rows = d.duplicate_rows;
-  o->fixed_columns = d.fixed_columns;
-  o->empty_columns = d.empty_columns;
-  o->free_column_singletons = d.free_column_singletons;
-  o->bound_tightening = d.bound_tightening;
-  o->slack_column_singletons = d.slack_column_singletons;
-  o->inequality_column_singletons = d.inequality_column_singletons;
-  o->dual_fixing = d.dual_fixing;
-  o->doubleton_equations = d.doubleton_equations;
-  o->coefficient_tightening = d.coefficient_tightening;
-  o->max_rounds = d.max_rounds;
-  o->feasibility_tolerance = d.feasibility_tolerance;
-  o->infeasibility_tolerance = d.infeasibility_tolerance;
-  o->bound_relaxation = d.bound_relaxation;
-  o->max_new_finite_bound = d.max_new_finite_bound;
-}
-
-static anuk::PresolveOptions presolve_options_of(const anuk_presolve_options* o) {
-  anuk::PresolveOptions p;
-  if (o == nullptr) return p;
-  need(o->max_rounds >= 0, "max_rounds must be non-negative");
-  p.empty_rows = o->empty_rows != 0;
-  p.singleton_rows = o->singleton_rows != 0;
-  p.redundant_rows = o->redundant_rows != 0;
-  p.forcing_rows = o->forcing_rows != 0;
-  p.duplicate_rows = o->duplicate_rows != 0;
-  p.fixed_columns = o->fixed_columns != 0;
-  p.empty_columns = o->empty_columns != 0;
-  p.free_column_singletons = o->free_column_singletons != 0;
-  p.bound_tightening = o->bound_tightening != 0;
-  p.slack_column_singletons = o->slack_column_singletons != 0;
-  p.inequality_column_singletons = o->inequality_column_singletons != 0;
-  p.dual_fixing = o->dual_fixing != 0;
-  p.doubleton_equations = o->doubleton_equations != 0;
-  p.coefficient_tightening = o->coefficient_tightening != 0;
-  p.max_rounds = o->max_rounds;
-  p.feasibility_tolerance = o->feasibility_tolerance;
-  p.infeasibility_tolerance = o->infeasibility_tolerance;
-  p.bound_relaxation = o->bound_relaxation;
-  p.max_new_finite_bound = o->max_new_finite_bound;
-  return p;
-}
-
-static void finish_presolve(anuk_presolve* h) {
-  if (h->r.status == anuk::PresolveStatus::kReduced) h->reduced.m = std::move(h->r.reduced);
-}
-
-int anuk_presolve_mps(const anuk_mps* model, const anuk_presolve_options* options,
-                      anuk_presolve** out) {
-  if (model == nullptr || out == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
-  *out = nullptr;
-  return guarded(nullptr, [&] {
-    auto h = std::make_unique<anuk_presolve>();
-    h->r = anuk::presolve(model->m, presolve_options_of(options));
-    finish_presolve(h.get());
-    *out = h.release();
-  });
-}
-
-int anuk_presolve_model(const anuk_model* model, const int32_t* col_types,
-                        const anuk_presolve_options* options, anuk_presolve** out) {
-  if (model == nullptr || out == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
-  *out = nullptr;
-  return guarded(nullptr, [&] {
-    need(model->rows >= 0 && model->cols >= 0 && model->nnz >= 0, "negative model size");
-    need(model->cols == 0 || (model->objective && model->col_lower && model->col_upper), "null column data");
-    need(model->rows == 0 || (model->row_lower && model->row_upper), "null row data");
-    need(model->nnz == 0 || (model->row_index && model->col_index && model->value), "null triplets");
-    anuk::MpsModel m;
-    std::vector<anuk::Entry> e(static_cast<size_t>(model->nnz));
-    for (int64_t k = 0; k < model->nnz; ++k) e[static_cast<size_t>(k)] = {model->row_index[k], model->col_index[k], model->value[k]};
-    m.a = anuk::Csr::from_triplets(model->rows, model->cols, std::move(e));
-    m.hessian = anuk::Csr::from_triplets(0, 0, {});
-    const size_t n = static_cast<size_t>(model->cols), r = static_cast<size_t>(model->rows);
-    if (n > 0) {
-      m.objective.assign(model->objective, model->objective + n);
-      m.col_lower.assign(model->col_lower, model->col_lower + n);
-      m.col_upper.assign(model->col_upper, model->col_upper + n);
-    }
-    if (r > 0) {
-      m.row_lower.assign(model->row_lower, model->row_lower + r);
-      m.row_upper.assign(model->row_upper, model->row_upper + r);
-    }
-    m.col_type.assign(n, anuk::kContinuous);
-    if (col_types)
-      for (size_t j = 0; j < n; ++j) {
-        need(col_types[j] >= 0 && col_types[j] <= 3, "column type out of range");
-        m.col_type[j] = col_types[j];
-      }
-    m.maximize = model->maximize != 0;
-    m.objective_offset = model->objective_offset;
-    auto h = std::make_unique<anuk_presolve>();
-    h->r = anuk::presolve(m, presolve_options_of(options));
-    finish_presolve(h.get());
-    *out = h.release();
-  });
-}
-
-void anuk_presolve_free(anuk_presolve* p) { delete p; }
-
-int anuk_presolve_info_get(const anuk_presolve* p, anuk_presolve_info* out) {
-  if (p == nullptr || out == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
-  const anuk::PresolveOutcome& r = p->r;
-  const anuk::PresolveCounts& c = r.counts;
-  std::memset(out, 0, sizeof(*out));
-  out->status = static_cast<int32_t>(r.status);
-  out->dual_exact = r.post.dual_exact() ? 1 : 0;
-  out->original_rows = r.original_rows;
-  out->original_cols = r.original_cols;
-  out->original_nnz = r.original_nnz;
-  if (r.status == anuk::PresolveStatus::kReduced) {
-    out->rows = p->reduced.m.a.rows;
-    out->cols = p->reduced.m.a.cols;
-    out->nnz = p->reduced.m.a.nnz();
-  }
-  out->rows_removed = c.rows_removed;
-  out->cols_removed = c.cols_removed;
-  out->nonzeros_removed = c.nonzeros_removed;
-  out->empty_rows = c.empty_rows;
-  out->singleton_rows = c.singleton_rows;
-  out->redundant_rows = c.redundant_rows;
-  out->forcing_rows = c.forcing_rows;
-  out->duplicate_rows = c.duplicate_rows;
-  out->fixed_columns = c.fixed_columns;
-  out->empty_columns = c.empty_columns;
-  out->free_column_singletons = c.free_column_singletons;
-  out->slack_column_singletons = c.slack_column_singletons;
-  out->inequality_column_singletons = c.inequality_column_singletons;
-  out->slack_singletons_declined = c.slack_singletons_declined;
-  out->doubleton_equations = c.doubleton_equations;
-  out->dual_fixed_columns = c.dual_fixed_columns;
-  out->coefficients_tightened = c.coefficients_tightened;
-  out->bounds_tightened = c.bounds_tightened;
-  out->bounds_made_finite = c.bounds_made_finite;
-  out->rounds = c.rounds;
-  out->largest_new_finite_bound = c.largest_new_finite_bound;
-  return ANUK_OK;
-}
-
-const char* anuk_presolve_message(const anuk_presolve* p) {
-  return p == nullptr ? nullptr : p->r.message.c_str();
-}
-
-const anuk_mps* anuk_presolve_reduced(const anuk_presolve* p) {
-  return (p == nullptr || p->r.status != anuk::PresolveStatus::kReduced) ? nullptr : &p->reduced;
-}
-
-int anuk_postsolve_primal(const anuk_presolve* p, const double* reduced_x, double* x) {
-  if (p == nullptr || x == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
-  return guarded(nullptr, [&] {
-    need(p->r.status == anuk::PresolveStatus::kReduced, "presolve did not reduce the model");
-    const size_t n = p->r.post.col_map().size();
-    need(n == 0 || reduced_x != nullptr, "null reduced x");
-    const std::vector<double> rx(reduced_x, reduced_x + n);
-    const std::vector<double> full = p->r.post.primal(rx);
-    std::memcpy(x, full.data(), full.size() * sizeof(double));
-  });
-}
-
-int anuk_postsolve_dual(const anuk_presolve* p, const double* reduced_y,
-                        const double* reduced_costs, const double* original_cost, double* y) {
-  if (p == nullptr || y == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
-  return guarded(nullptr, [&] {
-    need(p->r.status == anuk::PresolveStatus::kReduced, "presolve did not reduce the model");
-    const anuk::Postsolve& post = p->r.post;
-    const size_t m = post.row_map().size(), n = post.col_map().size();
-    const size_t on = static_cast<size_t>(post.original_cols());
-    need(m == 0 || reduced_y != nullptr, "null reduced y");
-    const std::vector<double> ry(reduced_y, reduced_y + m);
-    const std::vector<double> rc = reduced_costs ? std::vector<double>(reduced_costs, reduced_costs + n)
-                                                 : std::vector<double>{};
-    const std::vector<double> oc = original_cost ? std::vector<double>(original_cost, original_cost + on)
-                                                 : std::vector<double>{};
-    const std::vector<double> full = post.dual(ry, rc, oc);
-    std::memcpy(y, full.data(), full.size() * sizeof(double));
-  });
-}
 
 }  // extern "C"

@@ -818,8 +818,9 @@ void spmv_with(hipStream_t s, const DevCsr& a, const double* x, const Epi& e, do
   const dim3 grid(static_cast<unsigned>(a.blocks));
   // The fast order runs on plans with the large tile (prepare_fast), the
   // storage order on plans with the small one.
+  validate_launch(what, grid, dim3(kThreads), ev);
   auto launch = [&](auto kernel) {
-    if (ev != nullptr && (ev->start != nullptr || ev->stop != nullptr)) {
+    if (ev != nullptr && ev->start != nullptr) {
       hipExtLaunchKernelGGL(kernel, grid, dim3(kThreads), 0, s, ev->start, ev->stop, 0, a, x, e,
                             partials, fast ? 1 : 0);
     } else {
@@ -857,16 +858,19 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// The panel products and the combine are two dispatches, each timed by its
+// own event pair (ev: the products, ev_combine: the combine).
 template <class Epi>
 void panel_step_with(hipStream_t s, const DevCsr& panels, double* sums, Int rows,
-                     const double* x, const Epi& e, double* partials, const LaunchEvents* ev) {
+                     const double* x, const Epi& e, double* partials, const LaunchEvents* ev,
+                     const LaunchEvents* ev_combine) {
   if (rows <= 0) return;
-  const LaunchEvents first{ev != nullptr ? ev->start : nullptr, nullptr};
-  spmv_with(s, panels, x, PlainEpi{sums}, nullptr, "panel products", &first, true);
+  spmv_with(s, panels, x, PlainEpi{sums}, nullptr, "panel products", ev, true);
   const dim3 grid(static_cast<unsigned>((rows + kThreads - 1) / kThreads));
-  if (ev != nullptr && ev->stop != nullptr) {
-    hipExtLaunchKernelGGL(panel_combine_kernel<Epi>, grid, dim3(kThreads), 0, s, nullptr, ev->stop, 0,
-                          rows, sums, e, partials);
+  validate_launch("panel combine", grid, dim3(kThreads), ev_combine);
+  if (ev_combine != nullptr && ev_combine->start != nullptr) {
+    hipExtLaunchKernelGGL(panel_combine_kernel<Epi>, grid, dim3(kThreads), 0, s, ev_combine->start,
+                          ev_combine->stop, 0, rows, sums, e, partials);
   } else {
     hipLaunchKernelGGL(panel_combine_kernel<Epi>, grid, dim3(kThreads), 0, s, rows, sums, e, partials);
   }
@@ -1049,15 +1053,17 @@ Int panel_combine_blocks(Int rows) { return (rows + kThreads - 1) / kThreads; }
 void launch_dual_step_panels(hipStream_t s, const DevCsr& panels, double* panel_sums,
                              const double* x_bar, const DevProblem& p, const IterParams* params,
                              int step, const DualBufs& b, unsigned flags, double* partials,
-                             const LaunchEvents* ev) {
+                             const LaunchEvents* ev, const LaunchEvents* ev_combine) {
   const Int m = p.m;
   if (panels.rows != kPanels * m) throw std::invalid_argument("dual step: panel stack does not fit K");
   if (flags & kStepDiag) {
     panel_step_with(s, panels, panel_sums, m, x_bar,
-                    DualStepEpi<true>{p, b, OneLp{params}, step, {}, false}, partials, ev);
+                    DualStepEpi<true>{p, b, OneLp{params}, step, {}, false}, partials, ev,
+                    ev_combine);
   } else {
     panel_step_with(s, panels, panel_sums, m, x_bar,
-                    DualStepEpi<false>{p, b, OneLp{params}, step, {}, false}, partials, ev);
+                    DualStepEpi<false>{p, b, OneLp{params}, step, {}, false}, partials, ev,
+                    ev_combine);
   }
 }
 

@@ -12,6 +12,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "launch_check.hpp"
 
 namespace anuk {
 
@@ -134,6 +138,21 @@ struct LaunchEvents {
   hipEvent_t stop = nullptr;
 };
 
+// Throws std::invalid_argument unless launch_shape_error accepts the launch
+// (grid and workgroup shape, and the event pair of `ev`, which may be null).
+inline void validate_launch(const char* what, dim3 grid, dim3 block, const LaunchEvents* ev) {
+  LaunchShape s;
+  s.gx = grid.x;
+  s.gy = grid.y;
+  s.gz = grid.z;
+  s.bx = block.x;
+  s.by = block.y;
+  s.bz = block.z;
+  const char* why = launch_shape_error(s, ev != nullptr ? ev->start : nullptr,
+                                       ev != nullptr ? ev->stop : nullptr);
+  if (why != nullptr) throw std::invalid_argument(std::string("launch ") + what + ": " + why);
+}
+
 // Rows longer than this are summed by a whole wave in the fast product
 // order; at most this many are summed serially by one thread.
 inline constexpr Int kSerialRowMax = 32;
@@ -187,6 +206,7 @@ void launch_convergence_terms(hipStream_t s, Int n, Int m, const DevProblem& p,
                               const double* kt_y, double* partials_a, double* partials_b,
                               double* out7);
 
+
 // The fused PDHG step. `flags` selects the diagnostic variant; `step` is the
 // position of this step in its period (for the Halpern weight).
 // `fast` selects the fast product order (see anuk_pdhg_options).
@@ -205,7 +225,8 @@ void launch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
 void launch_dual_step_panels(hipStream_t s, const DevCsr& panels, double* panel_sums,
                              const double* x_bar, const DevProblem& p, const IterParams* params,
                              int step, const DualBufs& b, unsigned flags, double* partials,
-                             const LaunchEvents* ev = nullptr);
+                             const LaunchEvents* ev = nullptr,
+                             const LaunchEvents* ev_combine = nullptr);
 Int panel_combine_blocks(Int rows);
 // Check products: dual-side convergence terms from K^T y, and the Farkas
 // support from K^T dy. The Farkas product reads the row slots of `check`

@@ -461,8 +461,12 @@ void Solver::enqueue_steps(int steps, bool has_check, int parity) {
                                               12.0 * nnz + 4.0 * (m + 1) + 8.0 * n +
                                                   (diag ? 72.0 : 56.0) * m);
     if (panels_) {
+      // Two dispatches: the panel products carry the step's bytes, the
+      // combine is timed as its own entry (its partial sums are re-read).
+      const LaunchEvents ce =
+          be_.launch_events(diag ? "dual_step_combine_check" : "dual_step_combine", 0.0);
       launch_dual_step_panels(s, dpk_, panel_sums_, x_bar_, prob_, d_params_, step, db, flags,
-                              part_b_, &de);
+                              part_b_, &de, &ce);
     } else {
       launch_dual_step(s, fast_ ? dk_fast_ : dk_, x_bar_, prob_, d_params_, step, db, flags, part_b_,
                        &de, fast_);

@@ -652,72 +652,6 @@ int anuk_mps_write(const char* path, const anuk_model* model, const char* name,
                    const char* const* row_names, const char* const* col_names,
                    const int32_t* col_types);
 
-/* ------------------------------------------------------------------------ */
-/* Presolve and postsolve: presolve / PostsolveStack                         */
-/* (presolve.hpp:54-489, presolve.cpp:1215-1317)                             */
-/* ------------------------------------------------------------------------ */
-
-typedef struct anuk_presolve anuk_presolve;
-
-/* PresolveOptions (presolve.hpp:54-330), flags as 0/1. */
-typedef struct anuk_presolve_options {
-  int32_t empty_rows, singleton_rows, redundant_rows, forcing_rows, duplicate_rows;
-  int32_t fixed_columns, empty_columns, free_column_singletons, bound_tightening;
-  int32_t slack_column_singletons, inequality_column_singletons, dual_fixing;
-  int32_t doubleton_equations, coefficient_tightening;
-  int32_t max_rounds;
-  double feasibility_tolerance;
-  double infeasibility_tolerance;
-  double bound_relaxation;
-  double max_new_finite_bound;
-} anuk_presolve_options;
-void anuk_presolve_default_options(anuk_presolve_options* out);
-
-/* Presolves the model of an MPS file (types and names included) into a
- * handle holding the reduced model and the postsolve record. The status
- * (PresolveStatus: 0 reduced, 1 infeasible, 2 unbounded) is in the info; a
- * proof of infeasibility or unboundedness is not an error. options may be
- * NULL for the defaults. */
-int anuk_presolve_mps(const anuk_mps* model, const anuk_presolve_options* options,
-                      anuk_presolve** out);
-/* The same for a model given as triplets; col_types may be NULL (all
- * continuous). */
-int anuk_presolve_model(const anuk_model* model, const int32_t* col_types,
-                        const anuk_presolve_options* options, anuk_presolve** out);
-void anuk_presolve_free(anuk_presolve* p);
-
-/* PresolveResult's status, sizes and PresolveCounts (presolve.hpp:418-471). */
-typedef struct anuk_presolve_info {
-  int32_t status;
-  int32_t dual_exact;           /* PostsolveStack::dual_is_exact */
-  int32_t original_rows, original_cols;
-  int64_t original_nnz;
-  int32_t rows, cols;           /* the reduced model (status 0) */
-  int64_t nnz;
-  int32_t rows_removed, cols_removed, nonzeros_removed;
-  int32_t empty_rows, singleton_rows, redundant_rows, forcing_rows, duplicate_rows;
-  int32_t fixed_columns, empty_columns, free_column_singletons;
-  int32_t slack_column_singletons, inequality_column_singletons, slack_singletons_declined;
-  int32_t doubleton_equations, dual_fixed_columns, coefficients_tightened;
-  int32_t bounds_tightened, bounds_made_finite, rounds;
-  double largest_new_finite_bound;
-} anuk_presolve_info;
-int anuk_presolve_info_get(const anuk_presolve* p, anuk_presolve_info* out);
-/* PresolveResult::message ("" when reduced). */
-const char* anuk_presolve_message(const anuk_presolve* p);
-/* The reduced model, owned by p (NULL unless status is 0): every anuk_mps_*
- * accessor applies, and anuk_mps_model + anuk_lp_from_model give the LP to
- * solve. */
-const anuk_mps* anuk_presolve_reduced(const anuk_presolve* p);
-
-/* PostsolveStack::apply: x (original cols) from the reduced model's x. */
-int anuk_postsolve_primal(const anuk_presolve* p, const double* reduced_x, double* x);
-/* PostsolveStack::apply_dual: the original rows' duals from the reduced
- * model's row duals (reduced rows), its reduced costs (reduced cols, or NULL:
- * zero) and the original objective (original cols, or NULL: zero). */
-int anuk_postsolve_dual(const anuk_presolve* p, const double* reduced_y,
-                        const double* reduced_costs, const double* original_cost, double* y);
-
 #ifdef __cplusplus
 }
 #endif

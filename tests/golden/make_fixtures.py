@@ -6,7 +6,9 @@ the seeded refinery plans, and records what parity is judged against:
 status, iterations, restarts, polish counts, objective, ||K|| and the
 simplex optimum. Run in the container that has the reference sources:
 
-    make -C oracle && python tests/golden/make_fixtures.py [--only random|stack24]
+    make -C oracle && python tests/golden/make_fixtures.py [--only SECTION]
+
+SECTION is one of random, stack24, stack24_cases, random_1e-8.
 """
 import json
 import os
@@ -60,7 +62,34 @@ def stack_section(periods=24, threads=8):
             "simplex_objective": [r["objective"] for r in sx]}
 
 
-SECTIONS = {"random": random_section, "stack24": stack_section}
+def stack_cases_section(periods=24, threads=8):
+    """The reference's solve_pdhg on every case of the case stack at 1e-4 and
+    1e-8 (status, iterations, objective, ...): the per-case parity bar of the
+    batched and plain GPU solves."""
+    from concurrent.futures import ThreadPoolExecutor
+    base, cases = G.case_stack(periods)
+    lps = [_oracle.RefLp.from_natural(G.case_model(base, c)) for c in cases]
+    out = {"periods": periods, "cases": len(cases)}
+    for tol in ("1e-4", "1e-8"):
+        with ThreadPoolExecutor(max_workers=threads) as pool:
+            out[tol] = list(pool.map(lambda lp: record(lp, float(tol)), lps))
+    return out
+
+
+def random_tight_section():
+    """The 1M-column random LP at 1e-8 (threaded CPU backend, serial bits),
+    and a 40k x 100k LP of the same generator at 1e-8."""
+    out = {}
+    for m, n, threads in ((40000, 100000, 8), (400000, 1000000, 8)):
+        rlp = _oracle.RefLp.from_natural(G.random_lp(m=m, n=n))
+        out[f"{m}x{n}"] = {"seed": 20261204, "rows": rlp.rows, "cols": rlp.cols, "nnz": rlp.nnz,
+                           "1e-8": record(rlp, 1e-8, threads=threads)}
+        print(m, n, out[f"{m}x{n}"], flush=True)
+    return out
+
+
+SECTIONS = {"random": random_section, "stack24": stack_section,
+            "stack24_cases": stack_cases_section, "random_1e-8": random_tight_section}
 
 
 def main():
@@ -74,7 +103,8 @@ def main():
             json.dump(out, f, indent=1, sort_keys=True)
         print("updated", path)
         return
-    out = {"known": {}, "refinery": {}, "random": random_section(), "stack24": stack_section()}
+    out = {"known": {}, "refinery": {}, "random": random_section(), "stack24": stack_section(),
+           "stack24_cases": stack_cases_section(), "random_1e-8": random_tight_section()}
     for name in KNOWN:
         rlp = _oracle.RefLp.from_natural(natural(name))
         out["known"][name] = {"1e-8": record(rlp, 1e-8), "1e-4": record(rlp, 1e-4),

@@ -8,8 +8,9 @@ near the optimum; by the reference simplex itself, live, on seeded random
 LPs of the shape its crossover tests use (oracle/_ref); and by the
 properties those tests hold (test_crossover.cpp:72-170): every crossover
 basis is one column per row, all distinct, and starting from it never
-changes the optimum. End to end, the reference's refinery.mps presolved,
-finished and postsolved gives its published optimum.
+changes the optimum. End to end, the reference's refinery.mps presolved by
+the reference's presolve, finished, and postsolved gives its published
+optimum.
 """
 import json
 import os
@@ -162,18 +163,24 @@ def test_crossover_counts_are_the_references():
         assert mine.status == "optimal" and rel(mine.objective, c.objective) <= 1e-9
 
 
-def test_refinery_mps_presolved_finished_and_postsolved():
-    # The reference's data file end to end on the host: presolve (the
-    # reference's reductions), the exact finish of the reduced LP from all
-    # logicals, postsolve. The answer is the file's published optimum
-    # 2.426062266887e10 (bench/results/ipm_refinery.txt:19), and it is
-    # feasible for the model as read.
+def test_refinery_mps_presolved_finished_and_postsolved(ref):
+    # The reference's data file end to end on the host: the reference's own
+    # presolve (oracle/_ref, test infrastructure), the exact finish of the
+    # reduced LP from all logicals, the reference's postsolve. The answer is
+    # the file's published optimum 2.426062266887e10
+    # (bench/results/ipm_refinery.txt:19), and it is feasible for the model
+    # as read.
     import gzip
     from anukulan_amd.mps import read_mps_text
-    from anukulan_amd.presolve import presolve_mps_text
     raw = gzip.decompress(open(os.path.join(HERE, "golden", "refinery.mps.gz"), "rb").read())
-    P = presolve_mps_text(raw)
-    r = exact_finish([P.reduced.to_lp()])[0]
+    P = ref.RefPresolve(raw)
+    assert P.status == "reduced"
+    d = P.reduced
+    rows_of = np.repeat(np.arange(d["rows"], dtype=np.int32), np.diff(d["start"]))
+    reduced = G.NaturalModel(d["rows"], d["cols"], rows_of, d["index"], d["value"], d["objective"],
+                             d["row_lower"], d["row_upper"], d["col_lower"], d["col_upper"],
+                             d["maximize"], d["objective_offset"])
+    r = exact_finish([reduced.to_lp()])[0]
     assert r.status == "optimal", r.message
     x = P.postsolve_primal(r.x)
     m = read_mps_text(raw)

@@ -90,6 +90,53 @@ def test_full_stack_against_the_simplex_optima(ctx):
     assert max(errs) <= 1e-6
 
 
+def stack_fixture():
+    import json
+    import os
+    return json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "pdhg_reference.json")))["stack24_cases"]
+
+
+def check_against_reference(got, want, label):
+    """Per case: the reference's status and polish outcome, the objective
+    within 1e-8 relative, the iteration count within 2%."""
+    same = 0
+    for i, (g, w) in enumerate(zip(got, want)):
+        assert g.status == w["status"], (label, i, g.status, w["status"])
+        assert g.polished == w["polished"], (label, i)
+        assert relerr(g.objective, w["objective"]) <= 1e-8, (label, i, g.objective, w["objective"])
+        assert abs(g.iterations - w["iterations"]) <= 0.02 * w["iterations"], (
+            label, i, g.iterations, w["iterations"])
+        same += g.iterations == w["iterations"]
+    print(f"\n{label}: {same} of {len(want)} cases took exactly the reference's iteration count")
+    return same
+
+
+@pytest.mark.parametrize("tol", ["1e-4", "1e-8"])
+def test_full_stack_per_case_against_the_reference(ctx, tol):
+    # All 61 cases of the 24-period stack, batched, against the reference's
+    # solve_pdhg on each case (tests/golden/pdhg_reference.json
+    # stack24_cases, make_fixtures.py). At 1e-8 five cases stop at the
+    # 200,000-iteration limit in the reference; they must stop there here too.
+    fx = stack_fixture()
+    base, cases, models = stack(24)
+    assert len(models) == fx["cases"] == 61
+    got = solve_pdhg_batch(ctx, [m.to_lp() for m in models], tolerance=float(tol))
+    check_against_reference(got, fx[tol], f"batched {tol}")
+
+
+def test_full_stack_plain_per_case_against_the_reference(ctx):
+    # The plain GPU solver on every case at 1e-4, and at 1e-8 on a sample
+    # that holds two of the cases the reference stops at the iteration limit.
+    fx = stack_fixture()
+    base, cases, models = stack(24)
+    got = [solve_pdhg(ctx, m.to_lp(), tolerance=1e-4) for m in models]
+    check_against_reference(got, fx["1e-4"], "plain 1e-4")
+    sample = [0, 1, 4, 7, 30, 60]
+    got = [solve_pdhg(ctx, models[i].to_lp(), tolerance=1e-8) for i in sample]
+    check_against_reference(got, [fx["1e-8"][i] for i in sample], "plain 1e-8 sample")
+
+
 def test_every_launch_shape_gives_the_same_bits(ctx):
     # The tuner picks among row-waves per workgroup (1, 2, 4, 8); a lane's
     # arithmetic must not depend on the shape, so every shape and the tuned

@@ -47,6 +47,87 @@ def parse_profile(text):
     return rows
 
 
+def host_info():
+    """What the CPU figures ran on: cores the OS reports, cores this process
+    may use, the CPU model, and the pool size used (the usable cores, capped
+    at 16: a one-GPU box's share of its host)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0))
+    return {"nproc": os.cpu_count(), "usable_cores": usable, "cpu_model": model,
+            "pool_threads": min(16, usable),
+            "pool_cap_reason": "a one-GPU box's share of the host is 16 cores"}
+
+
+def ref_oracle():
+    """The reference build (test infrastructure) for the CPU-baseline legs,
+    or None."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        import _oracle
+    except Exception:
+        return None
+    return _oracle if _oracle.ref is not None else None
+
+
+def c0_section(ctx, args):
+    """The 12-period refinery plan, plain FP64 PDHG at rel KKT 1e-4 and 1e-8:
+    iterations, time to solve and iterations/s on the GPU (median of
+    args.c0_reps solves after one warm-up), the objective's error against
+    the reference simplex's FP64 optimum, and beside it the reference's
+    solve_pdhg on the host at t=1, t=default_thread_count() (threaded
+    backend) and t=the pool size."""
+    from anukulan_amd import generators as G, solve_pdhg
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "pdhg_reference.json")))
+    fx = fx["refinery"]["12"]
+    model = G.refinery_model(12)
+    lp = model.to_lp()
+    opt = fx["simplex_objective"]
+    out = {"workload": f"12-period refinery plan ({lp.rows} x {lp.cols}, {lp.nnz} nnz), plain "
+                       f"FP64 PDHG", "simplex_optimum": opt, "by_tolerance": {}}
+    ref = None if args.no_cpu_baseline else ref_oracle()
+    hi = host_info()
+    for tol in ("1e-4", "1e-8"):
+        solve_pdhg(ctx, lp, tolerance=float(tol))
+        runs = []
+        for _ in range(args.c0_reps):
+            t0 = time.perf_counter()
+            r = solve_pdhg(ctx, lp, tolerance=float(tol))
+            runs.append((time.perf_counter() - t0, r))
+        runs.sort(key=lambda t: t[0])
+        sec, r = runs[len(runs) // 2]
+        rec = {"status": r.status, "iterations": r.iterations,
+               "reference_iterations": fx[tol]["iterations"],
+               "seconds": round(sec, 4), "lps_per_s": round(1.0 / sec, 2),
+               "iterations_per_s": round(r.iterations / sec, 1),
+               "objective": r.objective,
+               "rel_error_vs_simplex": abs(r.objective - opt) / max(1.0, abs(opt)),
+               "rel_diff_vs_reference_objective":
+                   abs(r.objective - fx[tol]["objective"]) / max(1.0, abs(fx[tol]["objective"]))}
+        if ref is not None:
+            R = ref.RefLp.from_natural(model)
+            host = {}
+            for label, t in (("t1", 1), ("t_default", min(16, ref.ref_default_thread_count())),
+                             ("t_pool", hi["pool_threads"])):
+                t0 = time.perf_counter()
+                w = R.solve(threads=t, tolerance=float(tol))
+                hs = time.perf_counter() - t0
+                host[label] = {"threads": t, "seconds": round(hs, 4),
+                               "iterations": w.iterations,
+                               "iterations_per_s": round(w.iterations / hs, 1)}
+            rec["cpu_reference"] = host
+            rec["gpu_vs_cpu_t1"] = round(host["t1"]["seconds"] / sec, 2)
+        out["by_tolerance"][tol] = rec
+    return out
+
+
 def build_stack(periods, count):
     from anukulan_amd import generators as G
     base, cases = G.case_stack(periods)
@@ -106,6 +187,30 @@ def cpu_baseline_parallel(base, cases, tol, threads):
             "kind": "reference",
             "sample": f"all {len(lps)} cases solved by the reference's solve_pdhg, one case per "
                       f"thread on {threads} threads: {iters} iterations in {wall:.1f} s wall"}
+
+
+def cpu_baseline_threaded(base, cases, tol, budget_s):
+    """The reference build's solve_pdhg on its threaded CPU backend at
+    t = default_thread_count() (hw/2, threading.cpp:162; capped at the pool
+    size), one case at a time in stack order until `budget_s` of solve time
+    is spent."""
+    ref = ref_oracle()
+    if ref is None:
+        return None
+    from anukulan_amd import generators as G
+    t = min(host_info()["pool_threads"], ref.ref_default_thread_count())
+    done, iters, spent = 0, 0, 0.0
+    while done < len(cases) and spent < budget_s:
+        rlp = ref.RefLp.from_natural(G.case_model(base, cases[done]))
+        t0 = time.perf_counter()
+        r = rlp.solve(threads=t, tolerance=tol)
+        spent += time.perf_counter() - t0
+        done += 1
+        iters += r.iterations
+    return {"value": round(done / spent, 4), "unit": "cases/s", "cores": t, "kind": "reference",
+            "sample": f"first {done} of {len(cases)} cases solved one at a time by the reference's "
+                      f"solve_pdhg on its threaded backend with {t} threads: {iters} iterations "
+                      f"in {spent:.1f} s"}
 
 
 def parse_tuning(text):
@@ -215,6 +320,31 @@ def c1_section(ctx, args):
     spmv["K x, banded control"] = {"mean_us": row["mean_us"], "achieved_GBs": round(row["gbs"], 1),
                                     "frac": round(row["gbs"] / HBM_PEAK_GBS, 4),
                                     "launches": row["timed"]}
+    host = None
+    ref = None if args.no_cpu_baseline else ref_oracle()
+    if ref is not None:
+        # The reference backend's multiply on the host (serial, and its
+        # threaded backend over the pool), and a bounded run of its
+        # solve_pdhg on the threaded backend for iterations/s.
+        hi = host_info()
+        xh = rng.uniform(-1, 1, lp.cols)
+        bytes_kx = 12.0 * lp.nnz + 4.0 * (lp.rows + 1) + 8.0 * lp.cols + 8.0 * lp.rows
+        host = {"spmv_K_x": {}}
+        for label, t in (("t1", 1), ("t_pool", hi["pool_threads"])):
+            sec = ref.ref_time_multiply(e["start"], e["index"], e["value"], xh, threads=t,
+                                        reps=args.c1_host_reps)
+            host["spmv_K_x"][label] = {"threads": t, "mean_us": round(1e6 * sec, 1),
+                                       "achieved_GBs": round(bytes_kx / sec / 1e9, 2)}
+        R = ref.RefLp.from_export(e)
+        t0 = time.perf_counter()
+        w = R.solve(threads=hi["pool_threads"], tolerance=1e-4,
+                    max_iterations=args.c1_host_iterations)
+        hs = time.perf_counter() - t0
+        host["solve_sample"] = {"threads": hi["pool_threads"], "iterations": w.iterations,
+                                "seconds": round(hs, 2),
+                                "iterations_per_s": round(w.iterations / hs, 1),
+                                "note": "reference solve_pdhg capped at "
+                                        f"{args.c1_host_iterations} iterations, setup included"}
     for v in (x, y, yv, xv):
         v.free()
     ctx.release()
@@ -248,6 +378,7 @@ def c1_section(ctx, args):
     return {"workload": f"random LP m={lp.rows} n={lp.cols} nnz={lp.nnz} (Zipf column counts, "
                         f"seed 20261204), plain PDHG to rel KKT 1e-4",
             "build_s": round(build_s, 2), "spmv": spmv, "fused_steps": steps,
+            "cpu_reference": host,
             "solve": {"status": r.status, "iterations": r.iterations,
                       "polish_attempts": r.polish_attempts,
                       "polish_iterations": r.polish_iterations, "restarts": r.restarts,
@@ -439,6 +570,36 @@ def finish_section(ctx, args, base, cases, lps, optima):
                 n_done += 1
             cpu = {"value": round(n_done / spent, 3), "unit": "cases/s", "cores": 1,
                    "kind": "reference", "sample": f"reference simplex on the first {n_done} cases"}
+            # The reference's own case-stack workflow (app/anukulan_cli.cpp:1065-1230,
+            # `family --from-first`): the base plan solved cold, then each
+            # case re-solved by the dual simplex warm-started from the base's
+            # final basis. Its dual simplex stalls at the pivot limit on this
+            # base plan from all logicals, so the base is solved by its
+            # primal (the reference's default algorithm).
+            R0 = _oracle.RefLp.from_natural(G.case_model(base, cases[0]))
+            t1 = time.perf_counter()
+            b0 = R0.simplex_warm(dual=False)
+            base_s = time.perf_counter() - t1
+            warm_n, warm_s, warm_piv, warm_ok = 0, 0.0, 0, 0
+            while 1 + warm_n < len(cases) and warm_s < args.finish_cpu_seconds:
+                R = _oracle.RefLp.from_natural(G.case_model(base, cases[1 + warm_n]))
+                t1 = time.perf_counter()
+                w = R.simplex_warm(dual=True, start=(b0["basic"], b0["var_status"]))
+                warm_s += time.perf_counter() - t1
+                warm_piv += w["pivots"]
+                warm_ok += w["status"] == 0
+                warm_n += 1
+            per_case = warm_s / max(warm_n, 1)
+            cpu["warm_stack_workflow"] = {
+                "kind": "reference", "cores": 1, "base_cold_s": round(base_s, 3),
+                "base_status": int(b0["status"]), "warm_cases": warm_n,
+                "warm_cases_optimal": warm_ok, "warm_s": round(warm_s, 3),
+                "warm_pivots": warm_piv,
+                "warm_cases_per_s": round(warm_n / warm_s, 3) if warm_s > 0 else None,
+                "stack_cases_per_s_projected": round(
+                    len(cases) / (base_s + per_case * (len(cases) - 1)), 3),
+                "sample": f"base plan cold (primal), then the first {warm_n} what-if cases "
+                          "warm-started from its basis (dual simplex), one thread"}
     best = max(seeds, key=lambda k: seeds[k]["cases_per_s"])
     threads = min(16, len(os.sched_getaffinity(0)))
     return {"workload": f"{len(lps)}-case stack: batched PDHG to a seed tolerance, then crossover "
@@ -476,7 +637,8 @@ def main():
     ap.add_argument("--c3-plain-seconds", type=float, default=10.0,
                     help="time spent on plain one-at-a-time solves of the uneven stack's LPs")
     ap.add_argument("--no-finish", action="store_true", help="skip the exact finish of the stack")
-    ap.add_argument("--finish-cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--finish-cpu-seconds", type=float, default=8.0,
+                    help="time for each reference simplex sample of the exact finish")
     ap.add_argument("--no-nodes", action="store_true",
                     help="skip the branch-and-bound node frontier")
     ap.add_argument("--nodes-plain-seconds", type=float, default=8.0)
@@ -484,6 +646,12 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB and exit: the "
                          "known-size reference for HBM byte counters")
+    ap.add_argument("--no-c0", action="store_true", help="skip the 12-period plain solve")
+    ap.add_argument("--c0-reps", type=int, default=5)
+    ap.add_argument("--c1-host-reps", type=int, default=5,
+                    help="reference host SpMV repetitions on the 1M-column LP")
+    ap.add_argument("--c1-host-iterations", type=int, default=160,
+                    help="iteration cap of the reference's host solve sample on the 1M-column LP")
     ap.add_argument("--c1-rows", type=int, default=400_000)
     ap.add_argument("--c1-cols", type=int, default=1_000_000)
     ap.add_argument("--c1-reps", type=int, default=50)
@@ -565,16 +733,18 @@ def main():
                         "traffic_source": "profiles/pmc_traffic.txt (rocprofv3 --pmc, calibrated)",
                         "mean_launch_us": k["mean_us"], "launches_timed": k["timed"]}
 
+    c0 = None if args.no_c0 else c0_section(ctx, args)
     c1 = None if args.no_c1 else c1_section(ctx, args)
     c3 = None if args.no_c3 else c3_section(ctx, args)
     nodes = None if args.no_nodes else nodes_section(ctx, args)
     finish = None if args.no_finish else finish_section(ctx, args, base, cases, lps, optima)
 
-    cpu = cpu_par = None
+    cpu = cpu_par = cpu_thr = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(base, cases, args.tol, args.cpu_seconds)
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        threads = args.cpu_threads or host_info()["pool_threads"]
         cpu_par = cpu_baseline_parallel(base, cases, args.tol, threads)
+        cpu_thr = cpu_baseline_threaded(base, cases, args.tol, args.cpu_seconds / 3)
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "cases/s", "n_gpus": 1,
@@ -590,7 +760,8 @@ def main():
                    "parallelism": "single GPU"},
         "error_vs_simplex": error_vs_optimum(results, optima),
         "roofline": roofline, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_par,
-        "plain": plain, "c1": c1, "c3": c3, "nodes": nodes, "finish": finish,
+        "cpu_baseline_threaded_backend": cpu_thr, "host": host_info(),
+        "plain": plain, "c0": c0, "c1": c1, "c3": c3, "nodes": nodes, "finish": finish,
         "batch_phases_s": {k: round(v, 4) for k, v in results[0].phases.items()},
         "autotune": parse_tuning(tuning),
         "kernels": {k: {"mean_us": v["mean_us"], "calls": v["calls"], "GB/s": v["gbs"]}

@@ -6,6 +6,7 @@
 // Every function here is a thin marshalling wrapper around one reference
 // function; the arithmetic is the reference's.
 #include <cmath>
+#include <chrono>
 #include <cstring>
 #include <exception>
 #include <sstream>
@@ -25,6 +26,12 @@
 #include "anukulan_hip.h"
 
 using namespace anukulan;
+
+#ifdef ANUKULAN_REF_HIP
+namespace anukulan {
+const LinAlgBackend& hip_backend();  // hip_adapter.cpp
+}
+#endif
 
 namespace {
 
@@ -526,6 +533,11 @@ int ref_solve_pdhg(const void* h, const anuk_pdhg_options* o, int32_t threads,
       p.warm_y = &wy;
     }
     p.backend = threads > 1 ? &threaded_cpu_backend(threads) : &cpu_backend();
+#ifdef ANUKULAN_REF_HIP
+    // threads < 0: the HIP library through the LinAlgBackend adapter
+    // (hip_adapter.cpp), driven by the reference's own solve_pdhg.
+    if (threads < 0) p.backend = &hip_backend();
+#endif
     const PdhgResult r = solve_pdhg(lp, p);
     std::memset(res, 0, sizeof(*res));
     res->status = static_cast<int32_t>(r.status);
@@ -591,6 +603,49 @@ int ref_simplex(const void* h, double* x, double* y, double* objective, int32_t*
   }
 }
 
+// The reference simplex with a chosen algorithm (dual: 1) and, when
+// start_basic and start_status are given (start_count basic entries,
+// start_status one code per variable of the logical form: 0 basic, 1 at
+// lower, 2 at upper, 3 free), warm-started from that basis
+// (SimplexOptions::start_basic/start_status, simplex.hpp:295-301). Returns
+// the final basis and statuses in the same codes, the pivots and the
+// seconds the simplex reports.
+int ref_simplex_warm(const void* h, int32_t dual, const int32_t* start_basic,
+                     int32_t start_count, const int32_t* start_status, int32_t status_count,
+                     double* x, double* y, double* objective, int32_t* status, int32_t* basic,
+                     int32_t* basic_count, int32_t* var_status, int32_t* var_count,
+                     int64_t* pivots, double* seconds) {
+  try {
+    const StandardLp& lp = static_cast<const RefLp*>(h)->lp;
+    SimplexOptions so;
+    so.algorithm = dual ? SimplexOptions::Algorithm::kDual : SimplexOptions::Algorithm::kPrimal;
+    std::vector<Int> sb;
+    std::vector<VarStatus> ss;
+    if (start_basic != nullptr && start_status != nullptr) {
+      sb.assign(start_basic, start_basic + start_count);
+      for (int32_t k = 0; k < status_count; ++k) ss.push_back(static_cast<VarStatus>(start_status[k]));
+      so.start_basic = &sb;
+      so.start_status = &ss;
+    }
+    const SimplexResult r = solve_lp(lp, so);
+    *status = static_cast<int32_t>(r.status);
+    *objective = r.objective;
+    if (x) std::memcpy(x, r.x.data(), r.x.size() * sizeof(double));
+    if (y) std::memcpy(y, r.y.data(), r.y.size() * sizeof(double));
+    *basic_count = static_cast<int32_t>(r.final_basic.size());
+    if (basic) std::memcpy(basic, r.final_basic.data(), r.final_basic.size() * sizeof(Int));
+    *var_count = static_cast<int32_t>(r.final_status.size());
+    if (var_status)
+      for (size_t k = 0; k < r.final_status.size(); ++k)
+        var_status[k] = static_cast<int32_t>(r.final_status[k]);
+    *pivots = static_cast<int64_t>(r.iterations);
+    *seconds = r.solve_seconds;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
 // The reference crossover (crossover.hpp:114-117) of a point x (cols) and
 // dual y (rows, or null), default options: its counts in info[0..5]
 // (ok, candidates, pushed, rejected_small_pivot, logicals_remaining,
@@ -626,6 +681,21 @@ void ref_op_multiply(int32_t threads, int32_t rows, int32_t cols, const int32_t*
                      double* y) {
   const SparseMatrix a = csr_to_matrix(rows, cols, start, index, value);
   pick(threads).multiply(a, x, y);
+}
+
+// Seconds per call of the reference backend's multiply (threads as in
+// ref_op_multiply) over `reps` calls, the matrix built once beforehand: the
+// host SpMV time the CPU baseline reports.
+double ref_time_multiply(int32_t threads, int32_t rows, int32_t cols, const int32_t* start,
+                         const int32_t* index, const double* value, const double* x, double* y,
+                         int32_t reps) {
+  const SparseMatrix a = csr_to_matrix(rows, cols, start, index, value);
+  const LinAlgBackend& be = pick(threads);
+  be.multiply(a, x, y);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int32_t r = 0; r < reps; ++r) be.multiply(a, x, y);
+  const auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count() / std::max<int32_t>(reps, 1);
 }
 
 void ref_op_transpose(int32_t rows, int32_t cols, const int32_t* start,

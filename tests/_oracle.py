@@ -17,6 +17,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libanukulan_ref.so")
+# The reference's CPU path plus the LinAlgBackend adapter over the HIP
+# library: RefLp handles made by it solve with threads=-1 on the device.
+REF_HIP_PATH = os.path.join(ROOT, "oracle", "_ref", "libanukulan_ref_hip.so")
+CONTRACT_PATH = os.path.join(ROOT, "oracle", "_ref", "test_backend_hip")
 
 DP = POINTER(c_double)
 IP = POINTER(c_int32)
@@ -237,11 +241,11 @@ def to_standard_form(rows, cols, ri, ci, v, objective, row_lower, row_upper, col
 
 # ---- the reference build ---------------------------------------------------------
 
-def _load_ref():
-    if not os.path.exists(REF_PATH):
+def _load_ref(path=REF_PATH):
+    if not os.path.exists(path):
         return None
     from anukulan_amd import _lib as P  # struct layouts only: the shared header's types
-    L = ctypes.CDLL(REF_PATH)
+    L = ctypes.CDLL(path)
     L.ref_last_error.restype = c_char_p
     L.ref_default_thread_count.restype = c_int
     L.ref_pdhg_default_options.argtypes = [POINTER(P.PdhgOptions)]
@@ -264,8 +268,14 @@ def _load_ref():
     L.ref_evaluate_residual.argtypes = [c_void_p, DP, DP, POINTER(P.PdhgResidual)]
     L.ref_simplex.argtypes = [c_void_p, DP, DP, DP, IP, IP, IP]
     L.ref_crossover.argtypes = [c_void_p, DP, DP, IP, IP]
+    if hasattr(L, "ref_simplex_warm"):
+        L.ref_simplex_warm.argtypes = [c_void_p, c_int32, IP, c_int32, IP, c_int32, DP, DP, DP,
+                                       IP, IP, IP, IP, IP, POINTER(c_int64), DP]
     L.ref_op_multiply.argtypes = [c_int32, c_int32, c_int32, IP, IP, DP, DP, DP]
     L.ref_op_transpose.argtypes = [c_int32, c_int32, IP, IP, DP, IP, IP, DP]
+    if hasattr(L, "ref_time_multiply"):
+        L.ref_time_multiply.restype = c_double
+        L.ref_time_multiply.argtypes = [c_int32, c_int32, c_int32, IP, IP, DP, DP, DP, c_int32]
     for f in ("ref_op_dot",):
         getattr(L, f).restype = c_double
         getattr(L, f).argtypes = [DP, DP, c_int32]
@@ -315,6 +325,26 @@ def _load_ref():
 
 
 ref = _load_ref()
+
+
+def load_ref_hip():
+    """The reference build with the HIP adapter (loads libanukulan_hip.so;
+    solving with it needs a GPU), or None when it was not built."""
+    return _load_ref(REF_HIP_PATH)
+
+
+def ref_time_multiply(start, index, value, x, threads=1, reps=10):
+    """Seconds per y = A x of the reference backend (threads > 1: its
+    threaded backend), the matrix built once before the clock starts."""
+    rows = len(start) - 1
+    y = np.zeros(rows)
+    return ref.ref_time_multiply(threads, rows, len(x), _ip(i32(start)), _ip(i32(index)),
+                                 _dp(f64(value)), _dp(f64(x)), _dp(y), reps)
+
+
+def ref_default_thread_count() -> int:
+    """The reference's default_thread_count() (threading.cpp:162)."""
+    return int(ref.ref_default_thread_count())
 
 
 def ref_read_mps(text, format=0, negative_up=0, infinity_threshold=1e30) -> dict:
@@ -423,33 +453,36 @@ def _ref_mps_export(h) -> dict:
 
 
 class RefLp:
-    """A StandardLp held by the reference build."""
+    """A StandardLp held by the reference build (`lib`: the library that
+    made the handle, libanukulan_ref.so unless given)."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, lib=None):
+        self.L = lib if lib is not None else ref
         if not handle:
-            raise RuntimeError("reference build: " + ref.ref_last_error().decode())
+            raise RuntimeError("reference build: " + self.L.ref_last_error().decode())
         from anukulan_amd import _lib as P
         self.h = handle
         info = P.LpInfo()
-        ref.ref_lp_info(self.h, byref(info))
+        self.L.ref_lp_info(self.h, byref(info))
         self.rows, self.cols, self.nnz = info.rows, info.cols, info.nnz
         self.num_equalities = info.num_equalities
         self.objective_scale, self.objective_offset = info.objective_scale, info.objective_offset
 
     def __del__(self):
         if getattr(self, "h", None):
-            ref.ref_lp_free(self.h)
+            self.L.ref_lp_free(self.h)
             self.h = None
 
     @staticmethod
-    def from_natural(nm):
+    def from_natural(nm, lib=None):
         from anukulan_amd import _lib as P
+        L = lib if lib is not None else ref
         ri, ci, v = i32(nm.row_index), i32(nm.col_index), f64(nm.value)
         arrs = [f64(a) for a in (nm.objective, nm.row_lower, nm.row_upper, nm.col_lower,
                                  nm.col_upper)]
         m = P.Model(nm.rows, nm.cols, v.size, _ip(ri), _ip(ci), _dp(v), *[_dp(a) for a in arrs],
                     1 if nm.maximize else 0, nm.objective_offset)
-        return RefLp(ref.ref_lp_from_model(byref(m)))
+        return RefLp(L.ref_lp_from_model(byref(m)), L)
 
     @staticmethod
     def from_mps(text: str):
@@ -468,7 +501,7 @@ class RefLp:
              "value": np.zeros(max(self.nnz, 1)), "c": np.zeros(self.cols), "q": np.zeros(self.rows),
              "lower": np.zeros(self.cols), "upper": np.zeros(self.cols),
              "row_model": np.zeros(self.rows, np.int32), "row_sign": np.zeros(self.rows)}
-        ref.ref_lp_export(self.h, _ip(o["start"]), _ip(o["index"]), _dp(o["value"]), _dp(o["c"]),
+        self.L.ref_lp_export(self.h, _ip(o["start"]), _ip(o["index"]), _dp(o["value"]), _dp(o["c"]),
                           _dp(o["q"]), _dp(o["lower"]), _dp(o["upper"]), _ip(o["row_model"]),
                           _dp(o["row_sign"]))
         o["index"], o["value"] = o["index"][:self.nnz], o["value"][:self.nnz]
@@ -479,8 +512,8 @@ class RefLp:
 
     def scaled(self, ruiz=10, pc=True):
         rs, cs = np.zeros(self.rows), np.zeros(self.cols)
-        h = ref.ref_lp_scale(self.h, ruiz, 1 if pc else 0, _dp(rs), _dp(cs))
-        return RefLp(h), rs, cs
+        h = self.L.ref_lp_scale(self.h, ruiz, 1 if pc else 0, _dp(rs), _dp(cs))
+        return RefLp(h, self.L), rs, cs
 
     def solve(self, threads=1, warm_x=None, warm_y=None, **overrides):
         from anukulan_amd import _lib as P
@@ -496,18 +529,18 @@ class RefLp:
             o.warm_x, o.warm_y = _dp(wx), _dp(wy)
         x, y = np.zeros(self.cols), np.zeros(self.rows)
         r = P.PdhgResult()
-        rc = ref.ref_solve_pdhg(self.h, byref(o), threads, byref(r), _dp(x), _dp(y))
+        rc = self.L.ref_solve_pdhg(self.h, byref(o), threads, byref(r), _dp(x), _dp(y))
         if rc != 0:
-            raise RuntimeError(ref.ref_last_error().decode())
+            raise RuntimeError(self.L.ref_last_error().decode())
         return result_from(r, x, y)
 
     def matrix_norm(self, iterations=200, tol=1e-8, use_backend=True):
-        return ref.ref_estimate_matrix_norm(self.h, iterations, tol, 1 if use_backend else 0)
+        return self.L.ref_estimate_matrix_norm(self.h, iterations, tol, 1 if use_backend else 0)
 
     def evaluate_residual(self, x, y):
         from anukulan_amd import _lib as P
         r = P.PdhgResidual()
-        ref.ref_evaluate_residual(self.h, _dp(f64(x)), _dp(f64(y)), byref(r))
+        self.L.ref_evaluate_residual(self.h, _dp(f64(x)), _dp(f64(y)), byref(r))
         return {k: getattr(r, k) for k, _ in P.PdhgResidual._fields_}
 
     def crossover(self, x, y=None):
@@ -517,19 +550,43 @@ class RefLp:
         basic = np.zeros(self.rows, np.int32)
         xv = f64(x)
         yv = f64(y) if y is not None else None
-        rc = ref.ref_crossover(self.h, _dp(xv), _dp(yv), _ip(info), _ip(basic))
+        rc = self.L.ref_crossover(self.h, _dp(xv), _dp(yv), _ip(info), _ip(basic))
         if rc != 0:
-            raise RuntimeError(ref.ref_last_error().decode())
+            raise RuntimeError(self.L.ref_last_error().decode())
         return {"ok": bool(info[0]), "candidates": int(info[1]), "pushed": int(info[2]),
                 "rejected_small_pivot": int(info[3]), "logicals_remaining": int(info[4]),
                 "stopped_on_refactorization": bool(info[5]), "basic": basic}
+
+    def simplex_warm(self, dual=True, start=None):
+        """The reference simplex (dual by default, falling back to the
+        primal as solve_lp does), warm-started from `start` = (basic,
+        statuses) of an earlier result when given: what the reference's
+        case-stack workflow does for each case (app/anukulan_cli.cpp:1065-1230,
+        `family --from-first`)."""
+        x, y = np.zeros(self.cols), np.zeros(self.rows)
+        obj, st, bc, vc = c_double(), c_int32(), c_int32(), c_int32()
+        piv, sec = c_int64(), c_double()
+        basic = np.zeros(self.rows + self.cols, np.int32)
+        vstat = np.zeros(self.rows + self.cols, np.int32)
+        sb = ss = None
+        if start is not None:
+            sb, ss = i32(start[0]), i32(start[1])
+        rc = self.L.ref_simplex_warm(self.h, 1 if dual else 0, _ip(sb), 0 if sb is None else sb.size,
+                                  _ip(ss), 0 if ss is None else ss.size, _dp(x), _dp(y),
+                                  byref(obj), byref(st), _ip(basic), byref(bc), _ip(vstat),
+                                  byref(vc), byref(piv), byref(sec))
+        if rc != 0:
+            raise RuntimeError(self.L.ref_last_error().decode())
+        return {"status": st.value, "objective": obj.value, "x": x, "y": y,
+                "basic": basic[:bc.value].copy(), "var_status": vstat[:vc.value].copy(),
+                "pivots": piv.value, "seconds": sec.value}
 
     def simplex(self):
         x, y = np.zeros(self.cols), np.zeros(self.rows)
         obj, st = c_double(), c_int32()
         basic = np.zeros(self.rows + self.cols, np.int32)
         cnt = c_int32()
-        ref.ref_simplex(self.h, _dp(x), _dp(y), byref(obj), byref(st), _ip(basic), byref(cnt))
+        self.L.ref_simplex(self.h, _dp(x), _dp(y), byref(obj), byref(st), _ip(basic), byref(cnt))
         return {"status": st.value, "objective": obj.value, "x": x, "y": y,
                 "basic": basic[:cnt.value].copy()}
 

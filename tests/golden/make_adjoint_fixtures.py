@@ -8,15 +8,23 @@ finite differences through reference simplex re-solves, for
   stack4   the first 6 cases of the 4-period refinery stack; L = w'x with
            w ~ N(0,1) on the product-sale and crude-purchase columns; full
            gradients and finite differences at 200 coordinates per case.
-  stack24  all 61 cases of the 24-period stack, the same loss; the basis
-           sensitivities at 200 coordinates per case and the degeneracy of
-           each case's vertex (basic columns at a bound, nonbasic columns
-           with zero reduced cost).
+  stack24  all 61 cases of the 24-period stack (the 61-LP case stack), the
+           same loss; the basis sensitivities at 200 coordinates per case,
+           the degeneracy of each case's vertex (basic columns at a bound,
+           nonbasic columns with zero reduced cost), and one-sided finite
+           differences at those coordinates: (L(theta + h) - L(theta)) / h
+           and (L(theta) - L(theta - h)) / h, h = 1e-6 max(1, |theta|), each
+           L through a re-solve by the reference's simplex warm-started from
+           the case's own optimal basis (dual simplex, falling back to the
+           primal as solve_lp does), the way the reference's case-stack
+           workflow re-solves a case (app/anukulan_cli.cpp:1065-1230).
+           Where the two sides agree the derivative along that coordinate
+           is the same from either side.
 
 Everything is in standard-form coordinates (c, q, lower, upper). Run in the
 container that has the reference sources (several minutes):
 
-    make -C oracle && python tests/golden/make_adjoint_fixtures.py
+    make -C oracle && python tests/golden/make_adjoint_fixtures.py [--only stack24]
 """
 import os
 import sys
@@ -89,9 +97,70 @@ def solve_case(model, w_x, w_y, rng, fd):
     return out
 
 
+def one_sided(rlp_e, basis, field, index, loss, l0, rel_step=1e-6):
+    """(L(theta + h) - L(theta)) / h and (L(theta) - L(theta - h)) / h
+    through warm re-solves from `basis` (nan when a re-solve fails)."""
+    h = rel_step * max(1.0, abs(float(rlp_e[field][index])))
+    out = []
+    for s in (1.0, -1.0):
+        sx = _oracle.RefLp.from_export(S.perturbed(rlp_e, field, index, s * h)).simplex_warm(
+            dual=True, start=basis)
+        if sx["status"] != 0:
+            out.append(np.nan)
+            continue
+        out.append(s * (loss(sx["x"]) - l0) / h)
+    return out
+
+
+def stack24_case(i, base, case, w, coord_seed):
+    model = G.case_model(base, case)
+    rlp = _oracle.RefLp.from_natural(model)
+    e = rlp.export()
+    sx = rlp.simplex_warm(dual=False)  # the reference's default solve (primal, cold)
+    assert sx["status"] == 0, model.name
+    grads = S.basis_sensitivity(e, sx["basic"], sx["x"], w, None)
+    coords = pick_coordinates(np.random.default_rng(coord_seed), grads, COORDS)
+    basis = (sx["basic"], sx["var_status"])
+    l0 = float(w @ sx["x"])
+    fd = np.array([one_sided(e, basis, FIELDS[f], int(j), lambda x: float(w @ x), l0)
+                   for f, j in coords])
+    return {"coords": coords, "basis": np.array([grads[f][j] for f, j in coords]),
+            "fd_plus": fd[:, 0], "fd_minus": fd[:, 1], "degeneracy": degeneracy(e, sx),
+            "objective": sx["objective"]}
+
+
+def stack24_section(data, threads=8):
+    """The 61-case stack: weights from a seeded stream, one coordinate seed
+    per case, cases solved in parallel (the ctypes calls release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    base, cases = G.case_stack(24)
+    rng = np.random.default_rng(20261207)
+    ws = [seeds_on(base, rng) for _ in cases]
+    coord_seeds = rng.integers(0, 2**32, size=len(cases))
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        rs = list(pool.map(lambda k: stack24_case(k, base, cases[k], ws[k], int(coord_seeds[k])),
+                           range(len(cases))))
+    for i, r in enumerate(rs):
+        data[f"stack24_{i}_w_x"] = ws[i]
+        data[f"stack24_{i}_coords"] = r["coords"]
+        data[f"stack24_{i}_basis"] = r["basis"]
+        data[f"stack24_{i}_fd_plus"] = r["fd_plus"]
+        data[f"stack24_{i}_fd_minus"] = r["fd_minus"]
+        data[f"stack24_{i}_degeneracy"] = np.array(r["degeneracy"])
+        data[f"stack24_{i}_objective"] = np.array(r["objective"])
+        print("stack24", i, r["degeneracy"], flush=True)
+
+
 def main():
     if _oracle.ref is None:
         sys.exit("oracle/_ref/libanukulan_ref.so is not built")
+    path = os.path.join(HERE, "adjoint_reference.npz")
+    if sys.argv[1:] == ["--only", "stack24"]:
+        data = {k: v for k, v in np.load(path).items() if not k.startswith("stack24_")}
+        stack24_section(data)
+        np.savez_compressed(path, **data)
+        print("updated", path)
+        return
     data = {}
     rng = np.random.default_rng(20261205)
     model = G.random_lp(**RANDOM_SIZE)
@@ -118,17 +187,7 @@ def main():
             data[f"stack4_{i}_grad_{name}"] = g
         print("stack4", i, r["degeneracy"])
 
-    base, cases = G.case_stack(24)
-    for i, case in enumerate(cases):
-        model = G.case_model(base, case)
-        w = seeds_on(base, rng)
-        r = solve_case(model, w, None, rng, fd=False)
-        data[f"stack24_{i}_w_x"] = w
-        data[f"stack24_{i}_coords"] = r["coords"]
-        data[f"stack24_{i}_basis"] = r["basis_values"]
-        data[f"stack24_{i}_degeneracy"] = np.array(r["degeneracy"])
-        print("stack24", i, r["degeneracy"], flush=True)
-    path = os.path.join(HERE, "adjoint_reference.npz")
+    stack24_section(data)
     np.savez_compressed(path, **data)
     print("wrote", path)
 

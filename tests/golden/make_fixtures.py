@@ -77,8 +77,9 @@ def stack_cases_section(periods=24, threads=8):
 
 
 def random_tight_section():
-    """The 1M-column random LP at 1e-8 (threaded CPU backend, serial bits),
-    and a 40k x 100k LP of the same generator at 1e-8."""
+    """The random LP generator at 1e-8 (threaded CPU backend, serial bits):
+    40k x 100k, which the reference solves, and the 1M-column LP, where the
+    reference stops at its default time limit (the record says where)."""
     out = {}
     for m, n, threads in ((40000, 100000, 8), (400000, 1000000, 8)):
         rlp = _oracle.RefLp.from_natural(G.random_lp(m=m, n=n))

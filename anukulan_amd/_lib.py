@@ -165,6 +165,8 @@ SIGNATURES = {
                                      c_double, DP, DP]),
     "anuk_convergence_terms": (c_int, [c_void_p, c_int32, c_int32, POINTER(ConvProblem), c_void_p,
                                        c_void_p, c_void_p, c_void_p, POINTER(ConvTerms)]),
+    "anuk_convergence_terms_noalloc": (c_int, [c_void_p, c_int32, c_int32, POINTER(ConvProblem), c_void_p,
+                                       c_void_p, c_void_p, c_void_p, POINTER(ConvTerms)]),
     "anuk_set_profiling": (c_int, [c_void_p, c_int]),
     "anuk_profile_report": (c_int, [c_void_p, c_char_p, c_size_t, POINTER(c_size_t)]),
     "anuk_tuning_report": (c_int, [c_void_p, c_char_p, c_size_t, POINTER(c_size_t)]),

@@ -224,6 +224,18 @@ class Context:
                    kt_y.ptr, byref(t))
         return {f: getattr(t, f) for f, _ in _lib.ConvTerms._fields_}
 
+    def convergence_terms_noalloc(self, n, m, c, lower, upper, q, row_scale, col_scale, neq, x,
+                                  k_x, y, kt_y) -> dict:
+        """convergence_terms through the allocation-free entry point (the
+        library's device scratch array; calls on one device must not
+        overlap)."""
+        dev = [ctypes.cast(v.ptr, DP) for v in (c, lower, upper, q, row_scale, col_scale)]
+        p = _lib.ConvProblem(*dev, int(neq))
+        t = _lib.ConvTerms()
+        self._call(lib.anuk_convergence_terms_noalloc, int(n), int(m), byref(p), x.ptr, k_x.ptr,
+                   y.ptr, kt_y.ptr, byref(t))
+        return {f: getattr(t, f) for f, _ in _lib.ConvTerms._fields_}
+
     def set_profiling(self, on: bool):
         self._call(lib.anuk_set_profiling, 1 if on else 0)
 

@@ -32,6 +32,7 @@ HipBackend::HipBackend(int device) : device_(device) {
   hip_check(hipHostMalloc(&result_host_, kResultDoubles * sizeof(double), hipHostMallocDefault),
             "hipHostMalloc(result)");
   hip_check(hipEventCreate(&stop_event_), "hipEventCreate");
+  noalloc_scratch_init(device);
 }
 
 HipBackend::~HipBackend() {
@@ -510,6 +511,31 @@ void HipBackend::step_size_terms(Int n, Int m, const double* dx, const double* d
   // The absolute value belongs to the backend (backend.cpp:135-139).
   *interaction = std::fabs(r[0]);
   *movement = omega * r[2] + r[1] / omega;
+}
+
+Terms HipBackend::convergence_terms_noalloc(Int n, Int m, const DevProblem& p, const double* x,
+                                            const double* k_x, const double* y,
+                                            const double* kt_y) {
+  if (static_cast<size_t>(reduce_grid(m)) * kMaxRed > kNoAllocPartials ||
+      static_cast<size_t>(reduce_grid(n)) * kMaxRed > kNoAllocPartials)
+    throw std::invalid_argument("convergence terms: partials exceed the scratch array");
+  double* sc = noalloc_scratch(device_);
+  launch_convergence_terms(stream_, n, m, p, x, k_x, y, kt_y, sc, sc + kNoAllocPartials,
+                           sc + kNoAllocResultOffset);
+  hip_check(hipMemcpyAsync(result_host_, sc + kNoAllocResultOffset, 7 * sizeof(double),
+                           hipMemcpyDeviceToHost, stream_),
+            "convergence terms readback");
+  synchronize();
+  const double* r = result_host_;
+  Terms out;
+  out.primal_sq = r[0];
+  out.primal_inf = r[1];
+  out.q_dot_y = r[2];
+  out.dual_sq = r[3];
+  out.dual_inf = r[4];
+  out.bound_term = r[5];
+  out.c_dot_x = r[6];
+  return out;
 }
 
 Terms HipBackend::convergence_terms(Int n, Int m, const DevProblem& p, const double* x,

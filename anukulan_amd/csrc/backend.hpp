@@ -101,6 +101,12 @@ class HipBackend {
                        double omega, double* interaction, double* movement);
   Terms convergence_terms(Int n, Int m, const DevProblem& p, const double* x,
                           const double* k_x, const double* y, const double* kt_y);
+  // The same terms with every scratch buffer taken from the library's
+  // device array (noalloc_scratch) and the result read back through the
+  // pinned buffer made at construction: no device or host allocation, no
+  // profiling events. Calls that overlap on one device race.
+  Terms convergence_terms_noalloc(Int n, Int m, const DevProblem& p, const double* x,
+                                  const double* k_x, const double* y, const double* kt_y);
 
   // profiling (backend.hpp:203-214)
   void set_profiling(bool on);

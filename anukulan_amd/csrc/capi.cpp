@@ -337,6 +337,28 @@ int anuk_convergence_terms(anuk_ctx* ctx, int32_t n, int32_t m, const anuk_conv_
   });
 }
 
+int anuk_convergence_terms_noalloc(anuk_ctx* ctx, int32_t n, int32_t m,
+                                   const anuk_conv_problem* lp, const double* x,
+                                   const double* k_x, const double* y, const double* kt_y,
+                                   anuk_conv_terms* out) {
+  if (ctx == nullptr || out == nullptr) return ANUK_ERR_INVALID;
+  return guarded(ctx, [&] {
+    need(n >= 0 && m >= 0, "negative size");
+    need(n == 0 || (x && kt_y && lp && lp->c && lp->lower && lp->upper && lp->col_scale),
+         "null column vector");
+    need(m == 0 || (k_x && y && lp && lp->q && lp->row_scale), "null row vector");
+    const anuk::Terms t =
+        ctx->be->convergence_terms_noalloc(n, m, problem_of(lp, n, m), x, k_x, y, kt_y);
+    out->primal_sq = t.primal_sq;
+    out->primal_inf = t.primal_inf;
+    out->q_dot_y = t.q_dot_y;
+    out->dual_sq = t.dual_sq;
+    out->dual_inf = t.dual_inf;
+    out->bound_term = t.bound_term;
+    out->c_dot_x = t.c_dot_x;
+  });
+}
+
 int anuk_set_profiling(anuk_ctx* ctx, int on) {
   if (ctx == nullptr) return ANUK_ERR_INVALID;
   return guarded(ctx, [&] { ctx->be->set_profiling(on != 0); });

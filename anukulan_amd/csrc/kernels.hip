@@ -6,7 +6,10 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace anuk {
@@ -1022,6 +1025,37 @@ void launch_convergence_terms(hipStream_t s, Int n, Int m, const DevProblem& p,
   fold(s, {FoldSeg{partials_a, reduce_grid(m), 3, 0, ConvRowOp::kMaxMask},
            FoldSeg{partials_b, reduce_grid(n), 4, 3, ConvColOp::kMaxMask}},
        out7);
+}
+
+// The allocation-free entry points' scratch: a file-scope device array. In
+// host code its name is only a stand-in; the kernels get the address that
+// hipGetSymbolAddress returns on each device.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;  // device -> address of `scratch` there
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr.count(device) != 0) return;
+  int current = -1;
+  if (hipGetDevice(&current) != hipSuccess || current != device)
+    throw std::runtime_error("noalloc scratch: device " + std::to_string(device) + " is not current");
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr)
+    throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<double*>(ptr);
+}
+
+double* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it == g_scratch_ptr.end())
+    throw std::runtime_error("noalloc scratch: not set up on device " + std::to_string(device));
+  return it->second;
 }
 
 void launch_primal_step(hipStream_t s, const DevCsr& kt, const double* y, const DevProblem& p,

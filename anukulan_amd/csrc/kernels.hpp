@@ -206,6 +206,22 @@ void launch_convergence_terms(hipStream_t s, Int n, Int m, const DevProblem& p,
                               const double* kt_y, double* partials_a, double* partials_b,
                               double* out7);
 
+// The library-wide device scratch array of the allocation-free entry
+// points: one copy per device, held for the life of the process, shared by
+// every call on that device (calls that overlap on one device race).
+// Layout for the convergence terms: the row-side partials at [0,
+// 1024 * kMaxRed), the column-side partials after them, the 7 results at
+// kNoAllocResultOffset. Every value a call reads is written earlier in the
+// same call, so nothing needs clearing between calls.
+inline constexpr size_t kNoAllocPartials = static_cast<size_t>(1024) * kMaxRed;
+inline constexpr size_t kNoAllocResultOffset = 2 * kNoAllocPartials;
+inline constexpr size_t kNoAllocScratchDoubles = kNoAllocResultOffset + 64;
+// Looks up the array's device address on `device` (which must be current)
+// once and keeps it; throws if the lookup fails. Called from context set-up.
+void noalloc_scratch_init(int device);
+// The address noalloc_scratch_init found for `device`; throws if it has not
+// run on that device.
+double* noalloc_scratch(int device);
 
 // The fused PDHG step. `flags` selects the diagnostic variant; `step` is the
 // position of this step in its period (for the Halpern weight).

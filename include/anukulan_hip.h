@@ -176,6 +176,23 @@ int anuk_convergence_terms(anuk_ctx* ctx, int32_t n, int32_t m,
                            const double* k_x, const double* y, const double* kt_y,
                            anuk_conv_terms* out);
 
+/* anuk_convergence_terms without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device or host memory during a call
+ * (code around HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, same kernels and the same bits as
+ * anuk_convergence_terms; the per-block partials and the 7 results live in
+ * one library-wide device array of 2 x 1024 x 8 + 64 doubles (131,584
+ * bytes) per device, set up when the context is created and held until the
+ * process ends, and the result comes back through the context's pinned
+ * buffer. The partial count is at most 1024 blocks per side whatever n and
+ * m are; sizes whose partials would not fit are rejected with
+ * ANUK_ERR_INVALID. Calls that overlap on one device (several contexts or
+ * threads) race on the array: serialise them. */
+int anuk_convergence_terms_noalloc(anuk_ctx* ctx, int32_t n, int32_t m,
+                                   const anuk_conv_problem* lp, const double* x,
+                                   const double* k_x, const double* y, const double* kt_y,
+                                   anuk_conv_terms* out);
+
 /* LinAlgBackend::set_profiling / profile_report (backend.hpp:213-214). The
  * report is written NUL-terminated into buf (truncated to cap); *needed gets
  * the full length including the NUL. */

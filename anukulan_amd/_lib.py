@@ -79,7 +79,7 @@ class PdhgOptions(ctypes.Structure):
         ("known_matrix_norm", c_double), ("initial_step_size", c_double),
         ("initial_primal_weight", c_double), ("verbose", c_int32), ("log_frequency", c_int32),
         ("stop", IP), ("use_graphs", c_int32), ("product_order", c_int32), ("autotune", c_int32),
-        ("batch_waves", c_int32),
+        ("batch_waves", c_int32), ("batch_rows", c_int32),
     ]
 
 
@@ -127,13 +127,15 @@ class FinishResult(ctypes.Structure):
 
 class AdjointOptions(ctypes.Structure):
     _fields_ = [("method", c_int32), ("max_iterations", c_int32), ("tolerance", c_double),
-                ("active_tolerance", c_double), ("primal_weight", c_double)]
+                ("active_tolerance", c_double), ("primal_weight", c_double),
+                ("active_set", c_int32)]
 
 
 class AdjointInfo(ctypes.Structure):
     _fields_ = [("iterations", c_int32), ("converged", c_int32), ("residual_primal", c_double),
                 ("residual_dual", c_double), ("free_columns", c_int32), ("binding_rows", c_int32),
-                ("lower_columns", c_int32), ("upper_columns", c_int32)]
+                ("lower_columns", c_int32), ("upper_columns", c_int32),
+                ("basis_completed", c_int32)]
 
 
 # name -> (restype, argtypes); every function the header declares.

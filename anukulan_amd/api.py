@@ -463,6 +463,7 @@ def estimate_matrix_norm(ctx: Context, lp: LP, iterations=200, tolerance=1e-8) -
 # ---- sensitivities -----------------------------------------------------------
 
 ADJOINT_METHODS = {"cg": 0, "fixed_point": 1}
+ACTIVE_SETS = {"auto": 0, "point": 1, "basis": 2}
 
 
 @dataclass
@@ -481,6 +482,7 @@ class AdjointResult:
     binding_rows: int
     lower_columns: int
     upper_columns: int
+    basis_completed: bool = False
 
     def model_gradients(self, lp: LP, model_rows: int) -> dict:
         """The same gradients with respect to the natural model's data:
@@ -509,6 +511,8 @@ def adjoint_options(**overrides) -> _lib.AdjointOptions:
     for k, v in overrides.items():
         if k == "method" and isinstance(v, str):
             v = ADJOINT_METHODS[v]
+        if k == "active_set" and isinstance(v, str):
+            v = ACTIVE_SETS[v]
         if not hasattr(o, k):
             raise KeyError(f"unknown adjoint option {k}")
         setattr(o, k, v)
@@ -544,5 +548,6 @@ def pdhg_adjoint(ctx: Context, lps, xs, ys, w_x=None, w_y=None,
     out = [AdjointResult(gc[i].copy(), gq[i].copy(), gl[i].copy(), gu[i].copy(),
                          info[i].iterations, bool(info[i].converged), info[i].residual_primal,
                          info[i].residual_dual, info[i].free_columns, info[i].binding_rows,
-                         info[i].lower_columns, info[i].upper_columns) for i in range(count)]
+                         info[i].lower_columns, info[i].upper_columns,
+                         bool(info[i].basis_completed)) for i in range(count)]
     return out[0] if single else out

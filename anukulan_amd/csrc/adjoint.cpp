@@ -10,6 +10,7 @@
 
 #include "adjoint_kernels.hpp"
 #include "batch_kernels.hpp"
+#include "finish.hpp"
 #include "pdhg.hpp"
 
 namespace anuk {
@@ -58,6 +59,32 @@ LaneSets classify(const StandardLp& lp, const double* x, double tol) {
         (i < lp.num_equalities || kx[uz(i)] - q <= tol * (1.0 + std::fabs(q))) ? 1 : 0;
   }
   return s;
+}
+
+// The active set of an optimal basis (the exact finish's): the basic
+// columns are free, the nonbasic columns sit at the bound their state names,
+// and a row binds unless its logical is basic.
+LaneSets classify_basis(const StandardLp& lp, const FinishResult& r) {
+  const Int n = lp.cols(), m = lp.rows();
+  LaneSets s;
+  s.free_col.assign(uz(n), 0);
+  s.at_lower.assign(uz(n), 0);
+  s.at_upper.assign(uz(n), 0);
+  s.binding.assign(uz(m), 0);
+  for (Int j = 0; j < n; ++j) {
+    const VarState v = r.state[uz(j)];
+    s.free_col[uz(j)] = v == VarState::kBasic;
+    s.at_lower[uz(j)] = v == VarState::kLower;
+    s.at_upper[uz(j)] = v == VarState::kUpper;
+  }
+  for (Int i = 0; i < m; ++i) s.binding[uz(i)] = r.state[uz(n + i)] != VarState::kBasic;
+  return s;
+}
+
+Int count_set(const std::vector<char>& v) {
+  Int c = 0;
+  for (char e : v) c += e != 0;
+  return c;
 }
 
 // One lane of an interleaved vector, times an optional 0/1 mask lane.
@@ -194,6 +221,7 @@ void default_adjoint_options(anuk_adjoint_options* o) {
   o->tolerance = 1e-11;
   o->active_tolerance = 1e-6;
   o->primal_weight = 1.0;
+  o->active_set = ANUK_ACTIVE_AUTO;
 }
 
 std::vector<AdjointOutcome> pdhg_adjoint(HipBackend& be, const std::vector<const StandardLp*>& lps,
@@ -206,6 +234,9 @@ std::vector<AdjointOutcome> pdhg_adjoint(HipBackend& be, const std::vector<const
   if (lps.empty()) return {};
   if (opt.method != ANUK_ADJOINT_CG && opt.method != ANUK_ADJOINT_FIXED_POINT)
     throw std::invalid_argument("adjoint: unknown method");
+  if (opt.active_set != ANUK_ACTIVE_AUTO && opt.active_set != ANUK_ACTIVE_POINT &&
+      opt.active_set != ANUK_ACTIVE_BASIS)
+    throw std::invalid_argument("adjoint: unknown active-set rule");
   if (opt.max_iterations < 0 || !(opt.tolerance >= 0.0) || !(opt.active_tolerance >= 0.0) ||
       !(opt.primal_weight > 0.0))
     throw std::invalid_argument("adjoint: invalid options");
@@ -256,6 +287,36 @@ std::vector<AdjointOutcome> pdhg_adjoint(HipBackend& be, const std::vector<const
     eta = norm > 0.0 ? 0.9 / norm : 0.0;
   }
 
+  // The active set of every LP: read off its pair, and completed to an
+  // optimal basis by the exact finish (all such LPs at once, one per host
+  // thread) where the rule asks for it or the point's system is not square.
+  std::vector<LaneSets> all_sets(lps.size());
+  std::vector<size_t> complete;
+  for (size_t i = 0; i < lps.size(); ++i) {
+    all_sets[i] = classify(*lps[i], xs[i], opt.active_tolerance);
+    const bool square = count_set(all_sets[i].free_col) == count_set(all_sets[i].binding);
+    if (opt.active_set == ANUK_ACTIVE_BASIS || (opt.active_set == ANUK_ACTIVE_AUTO && !square))
+      complete.push_back(i);
+  }
+  std::vector<bool> completed(lps.size(), false);
+  if (!complete.empty()) {
+    std::vector<const StandardLp*> fl;
+    std::vector<const double*> fx, fy;
+    for (size_t i : complete) {
+      fl.push_back(lps[i]);
+      fx.push_back(xs[i]);
+      fy.push_back(ys[i]);
+    }
+    const std::vector<FinishResult> fin = exact_finish_many(fl, fx, fy, FinishOptions{});
+    for (size_t k = 0; k < complete.size(); ++k) {
+      // A case the finish could not bring to an optimal basis keeps the
+      // point's active set (and says so through basis_completed).
+      if (fin[k].status != kFinishOptimal) continue;
+      all_sets[complete[k]] = classify_basis(*lps[complete[k]], fin[k]);
+      completed[complete[k]] = true;
+    }
+  }
+
   AdjointEngine engine(be, dk, dkt, n, m);
   std::vector<AdjointOutcome> out(lps.size());
   for (size_t g0 = 0; g0 < lps.size(); g0 += kLanes) {
@@ -266,8 +327,9 @@ std::vector<AdjointOutcome> pdhg_adjoint(HipBackend& be, const std::vector<const
     for (size_t i = g0; i < g1; ++i) {
       const int lane = static_cast<int>(i - g0);
       LaneSets& ls = sets[uz(lane)];
-      ls = classify(*lps[i], xs[i], opt.active_tolerance);
+      ls = all_sets[i];
       AdjointOutcome& o = out[i];
+      o.basis_completed = completed[i];
       for (Int j = 0; j < n; ++j) {
         P(j, lane) = ls.free_col[uz(j)] ? 1.0 : 0.0;
         const double seed = wxs[i] != nullptr ? wxs[i][j] : 0.0;

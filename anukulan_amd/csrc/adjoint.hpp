@@ -15,9 +15,16 @@
 //   dL/dbound_j = (D2 w_x - K~' mu)_j / D2_j   for the columns at that bound
 //
 // with B~ = K~[A, F]. At a nondegenerate vertex B~ is square and these are
-// the basis sensitivities B^-1; where it is not square (a degenerate vertex)
-// the least-squares, minimum-norm solutions are returned, which is what the
-// fixed point of the transposed map reaches from zero.
+// the basis sensitivities B^-1. At a degenerate vertex (columns at a bound
+// that a basis keeps, rows whose slack is zero but basic) B~ read off the
+// point is not square, and its least-squares, minimum-norm solution pins
+// those columns to their bounds, which is not a derivative of the solution.
+// There the active set is completed to an optimal basis (active_set AUTO or
+// BASIS): the exact finish (finish.hpp) crosses over from the pair and runs
+// the simplex, and F becomes the basic columns, A the rows whose logical is
+// nonbasic, and the bound sets the nonbasic columns at each bound. B~ is then
+// square and nonsingular, and the same solve returns that basis's
+// sensitivities.
 #pragma once
 
 #include <vector>
@@ -35,6 +42,7 @@ struct AdjointOutcome {
   double residual_primal = 0.0;  // relative normal-equation residual of B' mu = g
   double residual_dual = 0.0;    // relative normal-equation residual of B nu = h
   Int free_columns = 0, binding_rows = 0, lower_columns = 0, upper_columns = 0;
+  bool basis_completed = false;  // the active set came from an optimal basis
 };
 
 // LPs must share K (as for the batched solve). x[i], y[i] are LP i's

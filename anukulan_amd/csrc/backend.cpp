@@ -565,11 +565,12 @@ std::string HipBackend::tuning_report() const {
     out += t.kernel;
     std::snprintf(buf, sizeof(buf), " rows=%d nnz=%d", t.rows, t.nnz);
     out += buf;
+    // A shape is waves per workgroup * 10 + rows per wave.
     for (const auto& c : t.us) {
-      std::snprintf(buf, sizeof(buf), " us@%d=%.3f", c.first, c.second);
+      std::snprintf(buf, sizeof(buf), " us@w%dr%d=%.3f", c.first / 10, c.first % 10, c.second);
       out += buf;
     }
-    std::snprintf(buf, sizeof(buf), " chosen=%d\n", t.chosen);
+    std::snprintf(buf, sizeof(buf), " chosen=w%dr%d\n", t.chosen / 10, t.chosen % 10);
     out += buf;
   }
   return out;

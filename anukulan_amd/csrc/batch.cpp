@@ -124,8 +124,10 @@ class Engine {
   // One past the highest live slot, rounded up to a whole 16-slot line: the
   // step kernels' lane bound, so at most four graph variants per parity.
   Int span_ = kLanes;
-  // Rows per workgroup of the primal (K^T) and dual (K) launches.
-  int pw_ = 4, dw_ = 4;
+  // Waves per workgroup and rows per wave of the primal (K^T) and dual (K)
+  // launches.
+  int pw_ = 2, dw_ = 2;
+  int pr_ = 1, dr_ = 1;
 
   double* slab_ = nullptr;
   double *X_[2], *Y_[2], *KX_[2];
@@ -327,12 +329,13 @@ void Engine::enqueue_period(int steps, bool has_check, int parity) {
     const LaunchEvents pe =
         be_.launch_events(diag ? "batch_primal_step_check" : "batch_primal_step",
                           12.0 * nnz + 4.0 * (n + 1) + 8.0 * lanes * (m + (diag ? 8.0 : 7.0) * n));
-    launch_batch_primal_step(s, dkt_, Y_[p], data_, d_params_, step, span_, pb, diag, pw_, &pe);
+    launch_batch_primal_step(s, dkt_, Y_[p], data_, d_params_, step, span_, pb, diag, pw_, pr_,
+                             &pe);
     DualBufs db{Y_[p], KX_[p], AY_, AKX_, Y_[1 - p], KX_[1 - p], DY_, KDX_, AY_, AKX_};
     const LaunchEvents de =
         be_.launch_events(diag ? "batch_dual_step_check" : "batch_dual_step",
                           12.0 * nnz + 4.0 * (m + 1) + 8.0 * lanes * (n + (diag ? 9.0 : 7.0) * m));
-    launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, step, span_, db, diag, dw_, &de);
+    launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, step, span_, db, diag, dw_, dr_, &de);
     p = 1 - p;
   }
   if (has_check) {
@@ -588,24 +591,39 @@ std::uint64_t pattern_key(const Csr& a) {
   return h;
 }
 
-// The launch-shape tuner: times the primal and dual step kernels with 1, 2,
-// 4 and 8 row-waves per workgroup on this stack's K^T and K (the mean of 20
-// back-to-back launches, best of three trials) and keeps the fastest, per
-// sparsity pattern for the context's life. Every shape computes the same
-// bits, so the choice changes only the time. The trial launches read the
-// current iterate and write only buffers the first real step overwrites (the
-// next parity, x-bar), with a step index that opens no Halpern epoch.
+// The launch-shape tuner: times the primal and dual step kernels in every
+// shape (1, 2, 4 or 8 waves per workgroup; 1, 2 or 4 consecutive rows per
+// wave) on this stack's K^T and K and keeps, per sparsity pattern for the
+// context's life, the fastest, if it beats the default shape (two waves,
+// one row: near the fastest on every pattern measured) by at least
+// kTuneMargin; otherwise the default stays, so timing noise between
+// near-equal shapes does not flip the choice. A shape's
+// time is the best of kTrials means of kReps back-to-back launches. Every
+// shape computes the same bits, so the choice changes only the time. The
+// trial launches read the current iterate and write only buffers the first
+// real step overwrites (the next parity, x-bar), with a step index that
+// opens no Halpern epoch.
+namespace {
+constexpr double kTuneMargin = 0.03;
+constexpr int kDefaultShape = 21;  // waves * 10 + rows per wave
+int shape_waves(int shape) { return shape / 10; }
+int shape_rows(int shape) { return shape % 10; }
+}  // namespace
+
 void Engine::tune() {
-  if (opt_.batch_waves != 0) {
-    pw_ = dw_ = opt_.batch_waves;
+  if (opt_.batch_waves != 0 || opt_.batch_rows != 0) {
+    pw_ = dw_ = opt_.batch_waves != 0 ? opt_.batch_waves : 2;
+    pr_ = dr_ = opt_.batch_rows != 0 ? opt_.batch_rows : 1;
     return;
   }
   const std::uint64_t kp = pattern_key(k_), ktp = pattern_key(kt_);
   const HipBackend::Tuning* tp = be_.tuned("batch_primal_step", ktp);
   const HipBackend::Tuning* td = be_.tuned("batch_dual_step", kp);
   if (tp != nullptr && td != nullptr) {
-    pw_ = tp->chosen;
-    dw_ = td->chosen;
+    pw_ = shape_waves(tp->chosen);
+    pr_ = shape_rows(tp->chosen);
+    dw_ = shape_waves(td->chosen);
+    dr_ = shape_rows(td->chosen);
     return;
   }
   if (opt_.autotune == 0 || depth_ != 0 || be_.profiling()) return;
@@ -614,7 +632,7 @@ void Engine::tune() {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hip_check(hipEventCreate(&e0), "hipEventCreate(tune)");
   hip_check(hipEventCreate(&e1), "hipEventCreate(tune)");
-  constexpr int kReps = 20, kTrials = 3;
+  constexpr int kReps = 20, kTrials = 5;
   auto mean_us = [&](const auto& launch) {
     double best = 1e300;
     for (int t = 0; t < kTrials; ++t) {
@@ -630,24 +648,37 @@ void Engine::tune() {
   };
   const PrimalBufs pb{X_[cur_], AX_, X_[1 - cur_], XBAR_, DX_, AX_};
   const DualBufs db{Y_[cur_], KX_[cur_], AY_, AKX_, Y_[1 - cur_], KX_[1 - cur_], DY_, KDX_, AY_, AKX_};
-  HipBackend::Tuning p{"batch_primal_step", kt_.rows, kt_.nnz(), {}, 4};
-  HipBackend::Tuning d{"batch_dual_step", k_.rows, k_.nnz(), {}, 4};
+  HipBackend::Tuning p{"batch_primal_step", kt_.rows, kt_.nnz(), {}, kDefaultShape};
+  HipBackend::Tuning d{"batch_dual_step", k_.rows, k_.nnz(), {}, kDefaultShape};
   for (int w : {1, 2, 4, 8}) {
-    p.us.push_back({w, mean_us([&] {
-                      launch_batch_primal_step(s, dkt_, Y_[cur_], data_, d_params_, 1, span_, pb, false, w);
-                    })});
-    d.us.push_back({w, mean_us([&] {
-                      launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, 1, span_, db, false, w);
-                    })});
+    for (int r : {1, 2, 4}) {
+      p.us.push_back({w * 10 + r, mean_us([&] {
+                        launch_batch_primal_step(s, dkt_, Y_[cur_], data_, d_params_, 1, span_, pb,
+                                                 false, w, r);
+                      })});
+      d.us.push_back({w * 10 + r, mean_us([&] {
+                        launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, 1, span_, db,
+                                               false, w, r);
+                      })});
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  auto fastest = [](const HipBackend::Tuning& t) {
-    return std::min_element(t.us.begin(), t.us.end(),
-                            [](const auto& a, const auto& b) { return a.second < b.second; })->first;
+  auto pick = [](const HipBackend::Tuning& t) {
+    const auto best = std::min_element(t.us.begin(), t.us.end(), [](const auto& a, const auto& b) {
+      return a.second < b.second;
+    });
+    double base = best->second;
+    for (const auto& c : t.us)
+      if (c.first == kDefaultShape) base = c.second;
+    return best->second < (1.0 - kTuneMargin) * base ? best->first : kDefaultShape;
   };
-  p.chosen = pw_ = fastest(p);
-  d.chosen = dw_ = fastest(d);
+  p.chosen = pick(p);
+  d.chosen = pick(d);
+  pw_ = shape_waves(p.chosen);
+  pr_ = shape_rows(p.chosen);
+  dw_ = shape_waves(d.chosen);
+  dr_ = shape_rows(d.chosen);
   be_.record_tuning(ktp, std::move(p));
   be_.record_tuning(kp, std::move(d));
 }

@@ -121,6 +121,110 @@ __global__ void __launch_bounds__(64 * kWaves)
   }
 }
 
+// The same steps with kRows consecutive rows per wave, their nonzeros taken
+// as one segment (lanes::segment_sums): fewer waves, and every load of the
+// wave's rows in flight at once. Each lane's arithmetic is the one-row
+// kernel's, term for term, so every shape gives the same bits.
+constexpr int kSegment = 16;
+
+template <bool kDiag, int kWaves, int kRows>
+__global__ void __launch_bounds__(64 * kWaves)
+    bprimal_rows_kernel(DevCsr kt, const double* __restrict__ y, BatchData d, PrimalBufs b,
+                        const LaneParams* __restrict__ params, int step, Int span) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const Int j0 = lanes::wave_row_in<kWaves>() * kRows;
+  if (j0 >= kt.rows || lane >= span) return;
+  const LaneParams P = params[lane];
+  double xj[kRows], cj[kRows], lo[kRows], hi[kRows], anc[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    if (j0 + r < kt.rows) {
+      const size_t o = at(j0 + r, lane);
+      xj[r] = b.x[o];
+      cj[r] = d.c[o];
+      lo[r] = d.lower[o];
+      hi[r] = d.upper[o];
+      anc[r] = b.anchor[o];
+    }
+  }
+  double kty[kRows];
+  lanes::segment_sums<kRows, kSegment>(kt, j0, y, lane, kty);
+  if (!P.active) return;
+  const LaneScalars sc = lane_scalars(P, step);
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    if (j0 + r >= kt.rows) break;
+    const size_t o = at(j0 + r, lane);
+    double v = xj[r] - sc.tau * (cj[r] - kty[r]);
+    if (v < lo[r]) v = lo[r];
+    if (v > hi[r]) v = hi[r];
+    const double dd = v - xj[r];
+    b.x_bar[o] = v + dd;
+    double next = v;
+    if (sc.gamma != 0.0) next += sc.gamma * dd;
+    if (sc.blend) next = sc.w * next + (1.0 - sc.w) * anc[r];
+    b.x_out[o] = next;
+    if (sc.open_epoch) b.anchor_out[o] = next;
+    if constexpr (kDiag) b.dx[o] = dd;
+  }
+}
+
+template <bool kDiag, int kWaves, int kRows>
+__global__ void __launch_bounds__(64 * kWaves)
+    bdual_rows_kernel(DevCsr k, const double* __restrict__ x_bar, BatchData d, DualBufs b,
+                      const LaneParams* __restrict__ params, int step, Int span) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const Int i0 = lanes::wave_row_in<kWaves>() * kRows;
+  if (i0 >= k.rows || lane >= span) return;
+  const LaneParams P = params[lane];
+  double yi[kRows], kx[kRows], qi[kRows], ay[kRows], ak[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    if (i0 + r < k.rows) {
+      const size_t o = at(i0 + r, lane);
+      yi[r] = b.y[o];
+      kx[r] = b.k_x[o];
+      qi[r] = d.q[o];
+      ay[r] = b.anchor_y[o];
+      ak[r] = b.anchor_kx[o];
+    }
+  }
+  double kxb[kRows];
+  lanes::segment_sums<kRows, kSegment>(k, i0, x_bar, lane, kxb);
+  if (!P.active) return;
+  const LaneScalars sc = lane_scalars(P, step);
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const Int i = i0 + r;
+    if (i >= k.rows) break;
+    const size_t o = at(i, lane);
+    double v = yi[r] + sc.sigma * (qi[r] - kxb[r]);
+    if (i >= d.num_equalities && v < 0.0) v = 0.0;
+    const double dd = v - yi[r];
+    const double kd = 0.5 * (kxb[r] - kx[r]);
+    double ny = v;
+    double nk = 0.5 * (kxb[r] + kx[r]);
+    if (sc.gamma != 0.0) {
+      ny += sc.gamma * dd;
+      nk += sc.gamma * kd;
+    }
+    if (sc.blend) {
+      ny = sc.w * ny + (1.0 - sc.w) * ay[r];
+      nk = sc.w * nk + (1.0 - sc.w) * ak[r];
+    }
+    b.y_out[o] = ny;
+    b.k_out[o] = nk;
+    if (sc.open_epoch) {
+      b.anchor_y_out[o] = ny;
+      b.anchor_kx_out[o] = nk;
+    }
+    if constexpr (kDiag) {
+      b.dy[o] = dd;
+      b.k_dx[o] = kd;
+    }
+  }
+}
+
 __device__ __forceinline__ bool lane_on(const LaneParams* params, unsigned long long mask,
                                         int lane) {
   return params != nullptr ? params[lane].active != 0 : ((mask >> lane) & 1ull) != 0;
@@ -393,12 +497,13 @@ void check_launch(const char* what) {
     throw std::runtime_error(std::string("launch ") + what + ": " + hipGetErrorString(e));
 }
 
-// One wave per row, `waves` rows per workgroup.
+// `per_wave` consecutive rows per wave, `waves` waves per workgroup.
 template <class K, class... Args>
-void launch_rows(K kernel, int waves, Int rows, hipStream_t s, const LaunchEvents* ev,
-                 const char* what, Args... args) {
+void launch_rows_per(K kernel, int waves, int per_wave, Int rows, hipStream_t s,
+                     const LaunchEvents* ev, const char* what, Args... args) {
   if (rows <= 0) return;
-  const dim3 grid(static_cast<unsigned>((rows + waves - 1) / waves));
+  const Int units = (rows + per_wave - 1) / per_wave;
+  const dim3 grid(static_cast<unsigned>((units + waves - 1) / waves));
   const dim3 block(static_cast<unsigned>(64 * waves));
   validate_launch(what, grid, block, ev);
   if (ev != nullptr && ev->start != nullptr) {
@@ -407,6 +512,13 @@ void launch_rows(K kernel, int waves, Int rows, hipStream_t s, const LaunchEvent
     hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
   }
   check_launch(what);
+}
+
+// One wave per row, `waves` rows per workgroup.
+template <class K, class... Args>
+void launch_rows(K kernel, int waves, Int rows, hipStream_t s, const LaunchEvents* ev,
+                 const char* what, Args... args) {
+  launch_rows_per(kernel, waves, 1, rows, s, ev, what, args...);
 }
 
 // The kernel instance for a shape (row-waves per workgroup).
@@ -440,27 +552,81 @@ struct SpmmAt {
 
 }  // namespace
 
+namespace {
+// The multi-row instance for (waves, rows), rows 2 or 4.
+template <bool kDiag, int R>
+auto primal_rows_by_waves(int waves) {
+  switch (waves) {
+    case 1: return &bprimal_rows_kernel<kDiag, 1, R>;
+    case 2: return &bprimal_rows_kernel<kDiag, 2, R>;
+    case 8: return &bprimal_rows_kernel<kDiag, 8, R>;
+    default: return &bprimal_rows_kernel<kDiag, 4, R>;
+  }
+}
+template <bool kDiag, int R>
+auto dual_rows_by_waves(int waves) {
+  switch (waves) {
+    case 1: return &bdual_rows_kernel<kDiag, 1, R>;
+    case 2: return &bdual_rows_kernel<kDiag, 2, R>;
+    case 8: return &bdual_rows_kernel<kDiag, 8, R>;
+    default: return &bdual_rows_kernel<kDiag, 4, R>;
+  }
+}
+
+template <bool kDiag>
+void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const BatchData& d,
+                       const LaneParams* params, int step, Int span, const PrimalBufs& b,
+                       int waves, int rows, const LaunchEvents* ev) {
+  const char* what = "batch_primal_step";
+  if (rows == 4) {
+    launch_rows_per(primal_rows_by_waves<kDiag, 4>(waves), waves, 4, kt.rows, s, ev, what, kt, y,
+                    d, b, params, step, span);
+  } else if (rows == 2) {
+    launch_rows_per(primal_rows_by_waves<kDiag, 2>(waves), waves, 2, kt.rows, s, ev, what, kt, y,
+                    d, b, params, step, span);
+  } else {
+    launch_rows(by_waves<PrimalPick<kDiag>::template At>(waves), waves, kt.rows, s, ev, what, kt,
+                y, d, b, params, step, span);
+  }
+}
+
+template <bool kDiag>
+void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const BatchData& d,
+                     const LaneParams* params, int step, Int span, const DualBufs& b, int waves,
+                     int rows, const LaunchEvents* ev) {
+  const char* what = "batch_dual_step";
+  if (rows == 4) {
+    launch_rows_per(dual_rows_by_waves<kDiag, 4>(waves), waves, 4, k.rows, s, ev, what, k, x_bar,
+                    d, b, params, step, span);
+  } else if (rows == 2) {
+    launch_rows_per(dual_rows_by_waves<kDiag, 2>(waves), waves, 2, k.rows, s, ev, what, k, x_bar,
+                    d, b, params, step, span);
+  } else {
+    launch_rows(by_waves<DualPick<kDiag>::template At>(waves), waves, k.rows, s, ev, what, k,
+                x_bar, d, b, params, step, span);
+  }
+}
+}  // namespace
+
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                               const BatchData& d, const LaneParams* params, int step, Int span,
-                              const PrimalBufs& b, bool diag, int waves, const LaunchEvents* ev) {
+                              const PrimalBufs& b, bool diag, int waves, int rows,
+                              const LaunchEvents* ev) {
   if (diag) {
-    launch_rows(by_waves<PrimalPick<true>::At>(waves), waves, kt.rows, s, ev, "batch_primal_step",
-                kt, y, d, b, params, step, span);
+    primal_step_shape<true>(s, kt, y, d, params, step, span, b, waves, rows, ev);
   } else {
-    launch_rows(by_waves<PrimalPick<false>::At>(waves), waves, kt.rows, s, ev, "batch_primal_step",
-                kt, y, d, b, params, step, span);
+    primal_step_shape<false>(s, kt, y, d, params, step, span, b, waves, rows, ev);
   }
 }
 
 void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                             const BatchData& d, const LaneParams* params, int step, Int span,
-                            const DualBufs& b, bool diag, int waves, const LaunchEvents* ev) {
+                            const DualBufs& b, bool diag, int waves, int rows,
+                            const LaunchEvents* ev) {
   if (diag) {
-    launch_rows(by_waves<DualPick<true>::At>(waves), waves, k.rows, s, ev, "batch_dual_step", k,
-                x_bar, d, b, params, step, span);
+    dual_step_shape<true>(s, k, x_bar, d, params, step, span, b, waves, rows, ev);
   } else {
-    launch_rows(by_waves<DualPick<false>::At>(waves), waves, k.rows, s, ev, "batch_dual_step", k,
-                x_bar, d, b, params, step, span);
+    dual_step_shape<false>(s, k, x_bar, d, params, step, span, b, waves, rows, ev);
   }
 }
 

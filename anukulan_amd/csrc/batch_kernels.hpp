@@ -61,15 +61,15 @@ inline constexpr int kLaneCheckStride = 16;
 
 // `span`: the lanes below it load and compute (the live ones among them
 // store); a kernel argument, so the bound costs no dependent load.
-// `waves`: rows (one per wave) per workgroup, 1, 2, 4 or 8; every shape
-// computes the same bits.
+// `waves`: waves per workgroup, 1, 2, 4 or 8; `rows`: consecutive rows per
+// wave, 1, 2 or 4. Every shape computes the same bits.
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                               const BatchData& d, const LaneParams* params, int step, Int span,
-                              const PrimalBufs& b, bool diag, int waves = 4,
+                              const PrimalBufs& b, bool diag, int waves = 4, int rows = 1,
                               const LaunchEvents* ev = nullptr);
 void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                             const BatchData& d, const LaneParams* params, int step, Int span,
-                            const DualBufs& b, bool diag, int waves = 4,
+                            const DualBufs& b, bool diag, int waves = 4, int rows = 1,
                             const LaunchEvents* ev = nullptr);
 // out1 = A v1 (and out2 = A v2 when v2 is non-null) for the active lanes:
 // those whose LaneParams say so when `params` is given (read at run time, so

@@ -545,6 +545,8 @@ anuk_pdhg_options checked_options(const anuk_pdhg_options* options) {
   need(o.batch_waves == 0 || o.batch_waves == 1 || o.batch_waves == 2 || o.batch_waves == 4 ||
            o.batch_waves == 8,
        "batch_waves must be 0, 1, 2, 4 or 8");
+  need(o.batch_rows == 0 || o.batch_rows == 1 || o.batch_rows == 2 || o.batch_rows == 4,
+       "batch_rows must be 0, 1, 2 or 4");
   return o;
 }
 
@@ -804,6 +806,7 @@ int anuk_pdhg_adjoint(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,
         f.binding_rows = a.binding_rows;
         f.lower_columns = a.lower_columns;
         f.upper_columns = a.upper_columns;
+        f.basis_completed = a.basis_completed ? 1 : 0;
       }
       col_off += n;
       row_off += m;

@@ -81,6 +81,48 @@ __device__ __forceinline__ double row_sum(const DevCsr& a, Int row, const double
   return s;
 }
 
+// Row sums of the kRows consecutive rows r0, r0 + 1, ... of A times
+// v[:, lane] (rows at or past a.rows are empty). Consecutive rows are one
+// contiguous run of the CSR, [start[r0], start[r0 + kRows]), so the wave
+// takes it in rounds of kSeg nonzeros: a round issues all its index and
+// value loads (wave-uniform) and all its gathers together, then adds each
+// term to its row's sum in storage order. A term outside a row adds +0.0 to
+// that row's sum, which leaves it unchanged (a sum that starts at +0.0
+// never becomes -0.0), so every row sum has the bits of the serial
+// storage-order loop.
+template <int kRows, int kSeg>
+__device__ __forceinline__ void segment_sums(const DevCsr& a, Int r0, const double* __restrict__ v,
+                                             int lane, double (&s)[kRows]) {
+  Int st[kRows + 1];
+#pragma unroll
+  for (int r = 0; r <= kRows; ++r) st[r] = a.start[min(r0 + r, a.rows)];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) s[r] = 0.0;
+  for (Int p0 = st[0]; p0 < st[kRows]; p0 += kSeg) {
+    double w[kSeg], x[kSeg];
+#pragma unroll
+    for (int u = 0; u < kSeg; ++u) {
+      const Int p = p0 + u;
+      if (p < st[kRows]) {
+        w[u] = a.value[p];
+        x[u] = v[at(a.index[p], lane)];
+      } else {
+        w[u] = 0.0;
+        x[u] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+#pragma unroll
+      for (int u = 0; u < kSeg; ++u) {
+        const Int p = p0 + u;
+        const double t = (p >= st[r] && p < st[r + 1]) ? w[u] * x[u] : 0.0;
+        s[r] += t;
+      }
+    }
+  }
+}
+
 // Blocks needed for one wave per row.
 inline unsigned row_blocks(Int rows) {
   return static_cast<unsigned>((rows + kWavesPerBlock - 1) / kWavesPerBlock);

@@ -1102,6 +1102,7 @@ void default_options(anuk_pdhg_options* o) {
   o->product_order = 0;
   o->autotune = 1;
   o->batch_waves = 0;
+  o->batch_rows = 0;
 }
 
 }  // namespace anuk

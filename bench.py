@@ -215,17 +215,20 @@ def cpu_baseline_threaded(base, cases, tol, budget_s):
 
 def parse_tuning(text):
     """Lines of Context.tuning_report(): kernel, rows, nnz, the mean launch
-    time of each shape (row-waves per workgroup) and the shape chosen."""
+    time of each shape (w<waves per workgroup>r<rows per wave>) and the shape
+    chosen."""
     out = []
     for line in text.splitlines():
         parts = line.split()
         if not parts:
             continue
-        rec = {"kernel": parts[0], "us_by_waves": {}}
+        rec = {"kernel": parts[0], "us_by_shape": {}}
         for p in parts[1:]:
             k, v = p.split("=")
             if k.startswith("us@"):
-                rec["us_by_waves"][k[3:]] = float(v)
+                rec["us_by_shape"][k[3:]] = float(v)
+            elif k == "chosen":
+                rec[k] = v
             else:
                 rec[k] = int(v)
         out.append(rec)

@@ -333,13 +333,15 @@ typedef struct anuk_pdhg_options {
                                           large K x split into per-XCD column panels);
                                           deterministic either way */
   int32_t autotune;                    /* 1: the batched solver times its launch shapes
-                                          (row-waves per workgroup: 1, 2, 4, 8) on the
-                                          stack's K and K^T once per matrix and keeps the
-                                          fastest; every shape computes the same bits.
-                                          0: four waves */
+                                          (waves per workgroup: 1, 2, 4, 8; rows per
+                                          wave: 1, 2, 4) on the stack's K and K^T once per
+                                          matrix and keeps the fastest when it beats two
+                                          waves of one row by 3%; every shape computes
+                                          the same bits. 0: two waves, one row */
   int32_t batch_waves;                 /* 0: as autotune decides; 1, 2, 4 or 8: the batched
-                                          step kernels use this many row-waves per
-                                          workgroup */
+                                          step kernels use this many waves per workgroup */
+  int32_t batch_rows;                  /* 0: as autotune decides; 1, 2 or 4: consecutive
+                                          rows per wave of the batched step kernels */
 } anuk_pdhg_options;
 
 void anuk_pdhg_default_options(anuk_pdhg_options* options);
@@ -535,6 +537,17 @@ enum anuk_adjoint_method {
   ANUK_ADJOINT_FIXED_POINT = 1  /* the transposed PDHG map iterated as is */
 };
 
+/* Where the final active set comes from. */
+enum anuk_adjoint_active_set {
+  /* Read off the point; where that leaves the system non-square (a
+   * degenerate vertex: columns at a bound that the basis keeps), complete it
+   * to an optimal basis by the exact finish from the point (anuk_exact_finish:
+   * crossover, then the simplex) and take the active set from that basis. */
+  ANUK_ACTIVE_AUTO = 0,
+  ANUK_ACTIVE_POINT = 1,  /* read off the point only (active_tolerance) */
+  ANUK_ACTIVE_BASIS = 2   /* always completed to an optimal basis */
+};
+
 typedef struct anuk_adjoint_options {
   int32_t method;           /* ANUK_ADJOINT_CG */
   int32_t max_iterations;   /* 20000 */
@@ -547,6 +560,7 @@ typedef struct anuk_adjoint_options {
                                of its rhs binds */
   double primal_weight;     /* 1.0: omega of the fixed-point method's steps,
                                tau = 0.9 / (||K~|| omega), sigma = 0.9 omega / ||K~|| */
+  int32_t active_set;       /* ANUK_ACTIVE_AUTO */
 } anuk_adjoint_options;
 
 typedef struct anuk_adjoint_info {
@@ -558,6 +572,7 @@ typedef struct anuk_adjoint_info {
   int32_t binding_rows;
   int32_t lower_columns;
   int32_t upper_columns;
+  int32_t basis_completed;  /* 1: the active set came from an optimal basis */
 } anuk_adjoint_info;
 
 void anuk_adjoint_default_options(anuk_adjoint_options* options);
@@ -579,8 +594,9 @@ int anuk_pdhg_adjoint(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,
                       anuk_adjoint_info* info);
 
 /* The launch shapes the batched solver's tuner measured and chose on this
- * context, one line per tuned kernel and matrix:
- *   <kernel> rows=<r> nnz=<z> us@1=<t> us@2=<t> us@4=<t> us@8=<t> chosen=<w>
+ * context, one line per tuned kernel and matrix, a shape written w<waves per
+ * workgroup>r<rows per wave>:
+ *   <kernel> rows=<r> nnz=<z> us@w1r1=<t> ... us@w8r4=<t> chosen=w<w>r<r>
  * Same buffer protocol as anuk_profile_report. No reference counterpart. */
 int anuk_tuning_report(anuk_ctx* ctx, char* buffer, size_t capacity, size_t* needed);
 

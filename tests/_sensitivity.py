@@ -80,3 +80,44 @@ def central_difference(ref, e, field, index, loss, rel_step=1e-6):
             return None
         vals.append(loss(sx["x"], sx["y"]))
     return (vals[0] - vals[1]) / (2.0 * h)
+
+
+def basis_directions(e, basic, x, field, index, tol=1e-9):
+    """Whether the basis `basic` at the vertex x stays primal feasible when
+    coordinate (field, index) moves up and when it moves down: (up, down).
+    Only then is its sensitivity a one-sided derivative in that direction.
+    A move of q_i shifts x_B by B^-1 e_i; a move of the bound a nonbasic
+    column sits at shifts x_B by -B^-1 K[:, j]; c moves no primal value.
+    A basic variable at a bound (a degenerate one, the row logicals
+    included: logical i is K x - q, in [0, inf) on inequality rows and 0 on
+    equality rows) must not be pushed past it."""
+    if field == "c":
+        return True, True
+    m, n = len(e["start"]) - 1, len(e["c"])
+    basic = np.asarray(basic, dtype=np.int64)
+    B, k = basis_matrix(e, basic)
+    if field == "q":
+        rhs = np.zeros(m)
+        rhs[index] = 1.0
+    else:
+        rhs = -k[:, int(index)].toarray().ravel()
+    d = splu(B).solve(rhs)
+    kx = k @ x
+    neq = e["num_equalities"]
+    up_ok = down_ok = True
+    for pos, j in enumerate(basic):
+        if abs(d[pos]) <= 1e-12:
+            continue
+        if j < n:
+            v, lo, hi = x[j], e["lower"][j], e["upper"][j]
+        else:
+            r = int(j) - n
+            v, lo, hi = kx[r] - e["q"][r], 0.0, (0.0 if r < neq else np.inf)
+        at_lo = np.isfinite(lo) and abs(v - lo) <= tol * (1 + abs(lo))
+        at_hi = np.isfinite(hi) and abs(v - hi) <= tol * (1 + abs(hi))
+        # moving up by t > 0 changes v by t d; down by -t d
+        if (at_lo and d[pos] < 0) or (at_hi and d[pos] > 0):
+            up_ok = False
+        if (at_lo and d[pos] > 0) or (at_hi and d[pos] < 0):
+            down_ok = False
+    return up_ok, down_ok

@@ -137,21 +137,24 @@ def test_full_stack_plain_per_case_against_the_reference(ctx):
     check_against_reference(got, [fx["1e-8"][i] for i in sample], "plain 1e-8 sample")
 
 
-def test_every_launch_shape_gives_the_same_bits(ctx):
-    # The tuner picks among row-waves per workgroup (1, 2, 4, 8); a lane's
-    # arithmetic must not depend on the shape, so every shape and the tuned
-    # choice give the same iterates to the bit.
-    base, cases, models = stack(4, 8)
+@pytest.mark.parametrize("periods", [4, 24])
+def test_every_launch_shape_gives_the_same_bits(ctx, periods):
+    # The tuner picks among waves per workgroup (1, 2, 4, 8) and consecutive
+    # rows per wave (1, 2, 4); a lane's arithmetic must not depend on the
+    # shape, so every shape and the tuned choice give the same iterates to
+    # the bit.
+    base, cases, models = stack(periods, 8)
     lps = [m.to_lp() for m in models]
-    want = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=4)
-    for w in (1, 2, 8, 0):
-        got = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=w)
+    want = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=4, batch_rows=1)
+    for w, r in ((1, 1), (2, 1), (8, 1), (1, 2), (4, 2), (8, 2), (1, 4), (2, 4), (4, 4), (8, 4),
+                 (0, 0)):
+        got = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=w, batch_rows=r)
         for g, h in zip(got, want):
-            assert g.iterations == h.iterations, w
-            assert np.array_equal(g.x, h.x) and np.array_equal(g.y, h.y), w
+            assert g.iterations == h.iterations, (w, r)
+            assert np.array_equal(g.x, h.x) and np.array_equal(g.y, h.y), (w, r)
     report = ctx.tuning_report()
     print("\n" + report)
-    assert "batch_primal_step" in report and "batch_dual_step" in report and "chosen=" in report
+    assert "batch_primal_step" in report and "batch_dual_step" in report and "chosen=w" in report
 
 
 def test_more_than_one_wave_of_cases(ctx):

@@ -52,3 +52,18 @@ def test_basis_sensitivities_equal_finite_differences(ref):
     loss = lambda x, y: float(fx["random_w_x"] @ x + fx["random_w_y"] @ y)  # noqa: E731
     d = S.central_difference(ref, e, "q", 0, loss)
     assert abs(d - grads[1][0]) <= 1e-4 * max(1.0, abs(d))
+
+
+def test_basis_directions_at_a_nondegenerate_vertex(ref):
+    # At a nondegenerate vertex no basic variable sits at a bound, so the
+    # basis stays feasible both ways along every coordinate (the checker the
+    # adjoint test uses to recognise coordinates a degenerate basis cannot
+    # differentiate).
+    import _sensitivity as S
+    from anukulan_amd import generators as G
+    rlp = ref.RefLp.from_natural(G.random_lp(m=60, n=150, seed=9))
+    e = rlp.export()
+    sx = rlp.simplex()
+    assert sx["status"] == 0
+    for i in range(0, len(e["q"]), 7):
+        assert S.basis_directions(e, sx["basic"], sx["x"], "q", i) == (True, True)

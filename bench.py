@@ -293,8 +293,9 @@ def c1_section(ctx, args):
     ctx.multiply_transpose(kt, yv, xv)
     ctx.synchronize()
     spmv = {}
-    for name, fn in (("K x", lambda: ctx.multiply(k, x, y)),
-                     ("K'y", lambda: ctx.multiply_transpose(kt, yv, xv))):
+    timed_products = () if args.no_profile else (
+        ("K x", lambda: ctx.multiply(k, x, y)), ("K'y", lambda: ctx.multiply_transpose(kt, yv, xv)))
+    for name, fn in timed_products:
         ctx.set_profiling(True)
         for _ in range(args.c1_reps):
             fn()
@@ -315,14 +316,23 @@ def c1_section(ctx, args):
                   e["value"])
     ctx.prepare(band)
     ctx.multiply(band, x, y)
-    ctx.set_profiling(True)
-    for _ in range(args.c1_reps):
-        ctx.multiply(band, x, y)
-    row = parse_profile(ctx.profile_report())["spmv"]
-    ctx.set_profiling(False)
-    spmv["K x, banded control"] = {"mean_us": row["mean_us"], "achieved_GBs": round(row["gbs"], 1),
-                                    "frac": round(row["gbs"] / HBM_PEAK_GBS, 4),
-                                    "launches": row["timed"]}
+    if not args.no_profile:
+        ctx.set_profiling(True)
+        for _ in range(args.c1_reps):
+            ctx.multiply(band, x, y)
+        row = parse_profile(ctx.profile_report())["spmv"]
+        ctx.set_profiling(False)
+        spmv["K x, banded control"] = {"mean_us": row["mean_us"],
+                                        "achieved_GBs": round(row["gbs"], 1),
+                                        "frac": round(row["gbs"] / HBM_PEAK_GBS, 4),
+                                        "launches": row["timed"]}
+    else:
+        # Launched plainly, so an external profiler sees the same products.
+        for _ in range(args.c1_reps):
+            ctx.multiply(k, x, y)
+            ctx.multiply_transpose(kt, yv, xv)
+            ctx.multiply(band, x, y)
+        ctx.synchronize()
     host = None
     ref = None if args.no_cpu_baseline else ref_oracle()
     if ref is not None:
@@ -356,11 +366,13 @@ def c1_section(ctx, args):
     r = solve_pdhg(ctx, lp, tolerance=1e-4, time_limit_seconds=args.c1_seconds,
                    use_graphs=0 if args.no_graphs else 1)
     solve_s = time.perf_counter() - t0
-    ctx.set_profiling(True)
-    solve_pdhg(ctx, lp, tolerance=1e-4, max_iterations=min(max(r.iterations, 80), 2000),
-               use_graphs=0)
-    kern = parse_profile(ctx.profile_report())
-    ctx.set_profiling(False)
+    kern = {}
+    if not args.no_profile:
+        ctx.set_profiling(True)
+        solve_pdhg(ctx, lp, tolerance=1e-4, max_iterations=min(max(r.iterations, 80), 2000),
+                   use_graphs=0)
+        kern = parse_profile(ctx.profile_report())
+        ctx.set_profiling(False)
     steps = {}
     for name in ("primal_step", "dual_step"):
         if name not in kern:
@@ -630,10 +642,15 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the all-cores reference baseline (default: the usable "
                          "cores, at most 16)")
-    ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="no HIP-event timing pass (under an external profiler)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly (for external profilers)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the batched kernels' default launch shape instead of timing "
+                         "the shapes (under an external profiler, whose overhead would skew "
+                         "the timing)")
     ap.add_argument("--no-c1", action="store_true", help="skip the 1M-column random LP")
     ap.add_argument("--no-c3", action="store_true", help="skip the uneven 512-LP stack")
     ap.add_argument("--c3-count", type=int, default=512)
@@ -688,8 +705,10 @@ def main():
     graphs = 0 if args.no_graphs else 1
     ctx = Context(0)
 
+    tune = 0 if args.no_autotune else 1
+
     def batched():
-        return solve_pdhg_batch(ctx, lps, tolerance=args.tol, use_graphs=graphs)
+        return solve_pdhg_batch(ctx, lps, tolerance=args.tol, use_graphs=graphs, autotune=tune)
 
     for _ in range(args.warmup):
         batched()

@@ -8,7 +8,7 @@ simplex optimum. Run in the container that has the reference sources:
 
     make -C oracle && python tests/golden/make_fixtures.py [--only SECTION]
 
-SECTION is one of random, stack24, stack24_cases, random_1e-8.
+SECTION is one of random, stack24, stack24_cases, random_1e-8, uneven512.
 """
 import json
 import os
@@ -89,7 +89,23 @@ def random_tight_section():
     return out
 
 
-SECTIONS = {"random": random_section, "stack24": stack_section,
+def uneven_sample_section(threads=8):
+    """The reference's solve_pdhg at 1e-4 on every 32nd LP of the 512-LP
+    uneven stack (generators.uneven_stack): 16 LPs, refinery plans and
+    random LPs, to hold the full-size stack's LPs to."""
+    from concurrent.futures import ThreadPoolExecutor
+    models = G.uneven_stack(512)
+    picks = list(range(0, 512, 32))
+
+    def one(i):
+        rlp = _oracle.RefLp.from_natural(models[i])
+        return {"index": i, "name": models[i].name, "rows": rlp.rows, "cols": rlp.cols,
+                "nnz": rlp.nnz, "1e-4": record(rlp, 1e-4)}
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        return {"count": 512, "lps": list(pool.map(one, picks))}
+
+
+SECTIONS = {"random": random_section, "stack24": stack_section, "uneven512": uneven_sample_section,
             "stack24_cases": stack_cases_section, "random_1e-8": random_tight_section}
 
 
@@ -105,7 +121,8 @@ def main():
         print("updated", path)
         return
     out = {"known": {}, "refinery": {}, "random": random_section(), "stack24": stack_section(),
-           "stack24_cases": stack_cases_section(), "random_1e-8": random_tight_section()}
+           "stack24_cases": stack_cases_section(), "random_1e-8": random_tight_section(),
+           "uneven512": uneven_sample_section()}
     for name in KNOWN:
         rlp = _oracle.RefLp.from_natural(natural(name))
         out["known"][name] = {"1e-8": record(rlp, 1e-8), "1e-4": record(rlp, 1e-4),

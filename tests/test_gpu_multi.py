@@ -103,3 +103,37 @@ def test_small_uneven_stack_solves(ctx):
         r = lp.evaluate_residual(g.x, g.y)
         assert r["relative_primal"] <= 1e-4 and r["relative_dual"] <= 1e-4
         assert r["relative_gap"] <= 1e-4
+
+
+def test_full_512_lp_uneven_stack(ctx):
+    # The full uneven stack (512 LPs, 3.5M rows, 8.9M columns, 63M nonzeros)
+    # in one set of launches: every LP optimal at 1e-4 with its residual on
+    # the LP as given within the tolerance (relative primal, dual and gap,
+    # and the infinity-norm forms), and every 32nd LP held to the reference's
+    # solve_pdhg on it (tests/golden/pdhg_reference.json uneven512, written by
+    # make_fixtures.py): same status, objective within 1e-8 relative,
+    # iterations within 2%.
+    fx = FIXTURE["uneven512"]
+    models = G.uneven_stack(fx["count"])
+    lps = [m.to_lp() for m in models]
+    got = solve_pdhg_multi(ctx, lps, tolerance=1e-4)
+    assert all(g.status == "optimal" for g in got), [i for i, g in enumerate(got)
+                                                      if g.status != "optimal"]
+    worst = 0.0
+    for lp, g in zip(lps, got):
+        r = lp.evaluate_residual(g.x, g.y)
+        for k in ("relative_primal", "relative_dual", "relative_gap", "relative_primal_inf",
+                  "relative_dual_inf"):
+            assert r[k] <= 1e-4, k
+            worst = max(worst, r[k])
+    same = 0
+    for rec in fx["lps"]:
+        g, lp, want = got[rec["index"]], lps[rec["index"]], rec["1e-4"]
+        assert (lp.rows, lp.cols, lp.nnz) == (rec["rows"], rec["cols"], rec["nnz"])
+        assert g.status == want["status"]
+        assert relerr(g.objective, want["objective"]) <= 1e-8, (rec["index"], g.objective,
+                                                                 want["objective"])
+        assert abs(g.iterations - want["iterations"]) <= 0.02 * want["iterations"]
+        same += g.iterations == want["iterations"]
+    print(f"\n512 LPs optimal, worst relative residual {worst:.2e}; {same} of {len(fx['lps'])} "
+          f"sampled LPs took exactly the reference's iteration count")

@@ -56,17 +56,26 @@ __device__ __forceinline__ LaneScalars lane_scalars(const LaneParams& p, int ste
 // wave waits for one round of memory latency rather than three. A retired
 // lane issues no vector load and stores nothing: with the live lanes kept in
 // the lowest slots, a row touches only the cache lines that hold them.
+// The leading arguments (the CSR's arrays, the gathered vector, the row
+// count, the slot span, the grid size, the step and the lane table) are
+// plain scalars so they can be preloaded into SGPRs at wave launch
+// (compiled with kernel-argument preloading): the row's start is then the
+// first load the wave waits on. The epilogue's buffers come as structs,
+// loaded while the row sum is in flight.
 template <bool kDiag, int kWaves>
 __global__ void __launch_bounds__(64 * kWaves)
-    bprimal_kernel(DevCsr kt, const double* __restrict__ y, BatchData d, PrimalBufs b,
-                   const LaneParams* __restrict__ params, int step, Int span) {
+    bprimal_kernel(const Int* __restrict__ kt_start, const Int* __restrict__ kt_index,
+                   const double* __restrict__ kt_value, const double* __restrict__ y, Int rows,
+                   Int span, int blocks, int step, const LaneParams* __restrict__ params,
+                   BatchData d, PrimalBufs b) {
   const int lane = static_cast<int>(threadIdx.x & 63);
-  const Int j = lanes::wave_row_in<kWaves>();
-  if (j >= kt.rows || lane >= span) return;
+  const Int j = lanes::wave_row_of<kWaves>(blocks);
+  if (j >= rows || lane >= span) return;
+  const lanes::CsrArrays kt{kt_start, kt_index, kt_value};
   const size_t o = at(j, lane);
   const LaneParams P = params[lane];
   const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
-  const double kty = row_sum(kt, j, y, lane);
+  const double kty = lanes::row_sum(kt, j, y, lane);
   if (!P.active) return;
   const LaneScalars sc = lane_scalars(P, step);
   double v = xj - sc.tau * (cj - kty);
@@ -84,15 +93,18 @@ __global__ void __launch_bounds__(64 * kWaves)
 
 template <bool kDiag, int kWaves>
 __global__ void __launch_bounds__(64 * kWaves)
-    bdual_kernel(DevCsr k, const double* __restrict__ x_bar, BatchData d, DualBufs b,
-                 const LaneParams* __restrict__ params, int step, Int span) {
+    bdual_kernel(const Int* __restrict__ k_start, const Int* __restrict__ k_index,
+                 const double* __restrict__ k_value, const double* __restrict__ x_bar, Int rows,
+                 Int span, int blocks, int step, const LaneParams* __restrict__ params,
+                 BatchData d, DualBufs b) {
   const int lane = static_cast<int>(threadIdx.x & 63);
-  const Int i = lanes::wave_row_in<kWaves>();
-  if (i >= k.rows || lane >= span) return;
+  const Int i = lanes::wave_row_of<kWaves>(blocks);
+  if (i >= rows || lane >= span) return;
+  const lanes::CsrArrays k{k_start, k_index, k_value};
   const size_t o = at(i, lane);
   const LaneParams P = params[lane];
   const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
-  const double kxb = row_sum(k, i, x_bar, lane);
+  const double kxb = lanes::row_sum(k, i, x_bar, lane);
   if (!P.active) return;
   const LaneScalars sc = lane_scalars(P, step);
   double v = yi + sc.sigma * (qi - kxb);
@@ -585,8 +597,9 @@ void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const B
     launch_rows_per(primal_rows_by_waves<kDiag, 2>(waves), waves, 2, kt.rows, s, ev, what, kt, y,
                     d, b, params, step, span);
   } else {
-    launch_rows(by_waves<PrimalPick<kDiag>::template At>(waves), waves, kt.rows, s, ev, what, kt,
-                y, d, b, params, step, span);
+    const int blocks = static_cast<int>((kt.rows + waves - 1) / waves);
+    launch_rows(by_waves<PrimalPick<kDiag>::template At>(waves), waves, kt.rows, s, ev, what,
+                kt.start, kt.index, kt.value, y, kt.rows, span, blocks, step, params, d, b);
   }
 }
 
@@ -602,8 +615,9 @@ void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const 
     launch_rows_per(dual_rows_by_waves<kDiag, 2>(waves), waves, 2, k.rows, s, ev, what, k, x_bar,
                     d, b, params, step, span);
   } else {
-    launch_rows(by_waves<DualPick<kDiag>::template At>(waves), waves, k.rows, s, ev, what, k,
-                x_bar, d, b, params, step, span);
+    const int blocks = static_cast<int>((k.rows + waves - 1) / waves);
+    launch_rows(by_waves<DualPick<kDiag>::template At>(waves), waves, k.rows, s, ev, what,
+                k.start, k.index, k.value, x_bar, k.rows, span, blocks, step, params, d, b);
   }
 }
 }  // namespace

@@ -31,6 +31,14 @@ __device__ __forceinline__ Int wave_row_in() {
 
 __device__ __forceinline__ Int wave_row() { return wave_row_in<kWavesPerBlock>(); }
 
+// The same with the grid size passed in (a preloaded kernel argument)
+// instead of read from the dispatch's argument block.
+template <int kWaves>
+__device__ __forceinline__ Int wave_row_of(int blocks) {
+  const int blk = xcd_remap(static_cast<int>(blockIdx.x), blocks);
+  return __builtin_amdgcn_readfirstlane(blk * kWaves + static_cast<int>(threadIdx.x >> 6));
+}
+
 __device__ __forceinline__ size_t at(Int row, int lane) {
   return static_cast<size_t>(row) * kLaneCount + static_cast<size_t>(lane);
 }
@@ -49,6 +57,56 @@ __device__ __forceinline__ void chunk_sum(const DevCsr& a, Int k, const double* 
   }
 #pragma unroll
   for (int u = 0; u < W; ++u) s += w[u] * x[u];
+}
+
+// A CSR's three arrays, passed as leading scalar kernel arguments so the
+// hardware can preload them into SGPRs at wave launch: the first load of a
+// row sum (its start) then waits on no argument load.
+struct CsrArrays {
+  const Int* __restrict__ start;
+  const Int* __restrict__ index;
+  const double* __restrict__ value;
+};
+
+template <int W>
+__device__ __forceinline__ void chunk_sum(const CsrArrays& a, Int k, const double* __restrict__ v,
+                                          int lane, double& s) {
+  double w[W], x[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    w[u] = a.value[k + u];
+    x[u] = v[at(a.index[k + u], lane)];
+  }
+#pragma unroll
+  for (int u = 0; u < W; ++u) s += w[u] * x[u];
+}
+
+// row_sum below over the arrays alone (the same terms in the same order).
+__device__ __forceinline__ double row_sum(const CsrArrays& a, Int row,
+                                          const double* __restrict__ v, int lane) {
+  const Int lo = a.start[row], hi = a.start[row + 1];
+  double s = 0.0;
+  Int k = lo;
+  for (; k + 8 <= hi; k += 8) chunk_sum<8>(a, k, v, lane, s);
+  if (k + 4 <= hi) {
+    chunk_sum<4>(a, k, v, lane, s);
+    k += 4;
+  }
+  const Int rest = hi - k;
+  if (rest > 0) {
+    double w[3], x[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      if (u < rest) {
+        w[u] = a.value[k + u];
+        x[u] = v[at(a.index[k + u], lane)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+      if (u < rest) s += w[u] * x[u];
+  }
+  return s;
 }
 
 __device__ __forceinline__ double row_sum(const DevCsr& a, Int row, const double* __restrict__ v,

@@ -80,18 +80,32 @@ __device__ double block_sum(double v) {
 // row and may add to up to kR reduction values, folded per block into
 // partials[block * kR ...].
 // ---------------------------------------------------------------------------
+//
+// The leading arguments (the CSR's arrays, the block plan, the gathered
+// vector, the grid size) are plain scalars so they can be preloaded into
+// SGPRs at wave launch (the file is built with kernel-argument preloading),
+// and the block's row bounds and nonzero range are loaded before the
+// epilogue's skip test, so the block waits on the plan and the row starts
+// while its step scalars arrive.
 template <class Epi, int kTile>
 __global__ void __launch_bounds__(kThreads)
-    spmv_kernel(DevCsr a, const double* __restrict__ x, Epi epi, double* __restrict__ partials,
-                int fast) {
+    spmv_kernel(const Int* __restrict__ a_start, const Int* __restrict__ a_index,
+                const double* __restrict__ a_value, const Int* __restrict__ block_row,
+                const Int* __restrict__ block_map, const double* __restrict__ x, int blocks,
+                int fast, Epi epi, double* __restrict__ partials) {
   __shared__ double tile[kTile];
   constexpr int R = Epi::kR > 0 ? Epi::kR : 1;
-  int b = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
-  if (a.block_map != nullptr) b = a.block_map[b];
+  struct {
+    const Int* __restrict__ start;
+    const Int* __restrict__ index;
+    const double* __restrict__ value;
+  } a{a_start, a_index, a_value};
+  int b = xcd_block(static_cast<int>(blockIdx.x), blocks);
+  if (block_map != nullptr) b = block_map[b];
+  const Int r0 = block_row[b], r1 = block_row[b + 1];
+  const Int k0 = a.start[r0], k1 = a.start[r1];
   const Epi e = epi.resolve(b);
   if (e.skip()) return;  // uniform across the block
-  const Int r0 = a.block_row[b], r1 = a.block_row[b + 1];
-  const Int k0 = a.start[r0], k1 = a.start[r1];
   const Int len = k1 - k0;
   double red[R];
 #pragma unroll
@@ -822,12 +836,16 @@ void spmv_with(hipStream_t s, const DevCsr& a, const double* x, const Epi& e, do
   // The fast order runs on plans with the large tile (prepare_fast), the
   // storage order on plans with the small one.
   validate_launch(what, grid, dim3(kThreads), ev);
+  const int blocks = static_cast<int>(a.blocks);
+  const int f = fast ? 1 : 0;
   auto launch = [&](auto kernel) {
     if (ev != nullptr && ev->start != nullptr) {
-      hipExtLaunchKernelGGL(kernel, grid, dim3(kThreads), 0, s, ev->start, ev->stop, 0, a, x, e,
-                            partials, fast ? 1 : 0);
+      hipExtLaunchKernelGGL(kernel, grid, dim3(kThreads), 0, s, ev->start, ev->stop, 0, a.start,
+                            a.index, a.value, a.block_row, a.block_map, x, blocks, f, e,
+                            partials);
     } else {
-      hipLaunchKernelGGL(kernel, grid, dim3(kThreads), 0, s, a, x, e, partials, fast ? 1 : 0);
+      hipLaunchKernelGGL(kernel, grid, dim3(kThreads), 0, s, a.start, a.index, a.value,
+                         a.block_row, a.block_map, x, blocks, f, e, partials);
     }
   };
   if (fast) {

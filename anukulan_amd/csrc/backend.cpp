@@ -295,6 +295,7 @@ namespace {
 // never reach them).
 constexpr std::uint64_t kPanelKey = 1ull << 63;
 constexpr std::uint64_t kFastKey = 1ull << 62;
+constexpr std::uint64_t kSlotKey = 1ull << 61;
 }  // namespace
 
 void HipBackend::replan_fast(const CsrView& a, DeviceMatrix* dm) {
@@ -324,6 +325,57 @@ const DevCsr& HipBackend::prepare_fast(const CsrView& a) {
   DeviceMatrix dm;
   dm.view = prepare(a);  // the arrays stay owned by the matrix's own entry
   replan_fast(a, &dm);
+  auto ins = matrices_.emplace(key, std::move(dm));
+  return ins.first->second.view;
+}
+
+const DevCsr& HipBackend::prepare_slots(const CsrView& a) {
+  const std::uint64_t key = a.id ^ kSlotKey;
+  auto it = matrices_.find(key);
+  if (it != matrices_.end()) return it->second.view;
+  DeviceMatrix dm;
+  dm.view = prepare(a);  // the CSR arrays stay owned by the matrix's own entry
+  // The kernels gather through a buffer resource with 31-bit offsets: a
+  // gathered vector of 64 lanes per row must stay below 2^31 bytes.
+  if (a.cols >= kSlotMaxCols) {
+    auto ins = matrices_.emplace(key, std::move(dm));
+    return ins.first->second.view;
+  }
+  Int longest = 0;
+  for (Int r = 0; r < a.rows; ++r) longest = std::max(longest, a.start[r + 1] - a.start[r]);
+  Int width = kSlotWidths[0];
+  for (const int w : kSlotWidths) {
+    width = w;
+    if (longest <= w) break;
+  }
+  const size_t rows = uz(a.rows), cells = rows * uz(width);
+  std::vector<Int> row(2 * rows), index(cells, 0);
+  std::vector<double> value(cells, 0.0);
+  for (size_t r = 0; r < rows; ++r) {
+    const Int lo = a.start[r], len = a.start[r + 1] - lo;
+    row[2 * r] = lo;
+    row[2 * r + 1] = len;
+    for (Int u = 0; u < std::min(len, width); ++u) {
+      index[r * uz(width) + uz(u)] = a.index[lo + u];
+      value[r * uz(width) + uz(u)] = a.value[lo + u];
+    }
+    // Padding repeats the row's last index, so its discarded gathers read a
+    // line the row reads anyway rather than all waves reading one line.
+    for (Int u = std::min(len, width); u < width && len > 0; ++u)
+      index[r * uz(width) + uz(u)] = a.index[lo + std::min(len, width) - 1];
+  }
+  auto put = [&](const void* src, size_t bytes) -> void* {
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, std::max<size_t>(bytes, 8)), "hipMalloc(slots)");
+    dm.owned.push_back(p);
+    if (bytes > 0) hip_check(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, stream_), "upload slots");
+    return p;
+  };
+  dm.view.slot_width = width;
+  dm.view.slot_row = static_cast<const Int*>(put(row.data(), row.size() * sizeof(Int)));
+  dm.view.slot_index = static_cast<const Int*>(put(index.data(), index.size() * sizeof(Int)));
+  dm.view.slot_value = static_cast<const double*>(put(value.data(), value.size() * sizeof(double)));
+  synchronize();
   auto ins = matrices_.emplace(key, std::move(dm));
   return ins.first->second.view;
 }
@@ -427,8 +479,8 @@ void HipBackend::release() {
 }
 
 void HipBackend::release_one(std::uint64_t id) {
-  // The fast plan borrows the matrix's arrays; it goes first.
-  for (const std::uint64_t key : {id ^ kFastKey, id ^ kPanelKey, id}) {
+  // The fast plan and the slot copy borrow the matrix's arrays; they go first.
+  for (const std::uint64_t key : {id ^ kFastKey, id ^ kSlotKey, id ^ kPanelKey, id}) {
     auto it = matrices_.find(key);
     if (it == matrices_.end()) continue;
     synchronize();
@@ -565,12 +617,15 @@ std::string HipBackend::tuning_report() const {
     out += t.kernel;
     std::snprintf(buf, sizeof(buf), " rows=%d nnz=%d", t.rows, t.nnz);
     out += buf;
-    // A shape is waves per workgroup * 10 + rows per wave.
+    // A shape is 100 * layout + 10 * waves per workgroup + rows per wave;
+    // layout 1, the fixed-width row copy, is marked "s".
     for (const auto& c : t.us) {
-      std::snprintf(buf, sizeof(buf), " us@w%dr%d=%.3f", c.first / 10, c.first % 10, c.second);
+      std::snprintf(buf, sizeof(buf), " us@w%dr%d%s=%.3f", (c.first % 100) / 10, c.first % 10,
+                    c.first >= 100 ? "s" : "", c.second);
       out += buf;
     }
-    std::snprintf(buf, sizeof(buf), " chosen=w%dr%d\n", t.chosen / 10, t.chosen % 10);
+    std::snprintf(buf, sizeof(buf), " chosen=w%dr%d%s\n", (t.chosen % 100) / 10, t.chosen % 10,
+                  t.chosen >= 100 ? "s" : "");
     out += buf;
   }
   return out;

@@ -63,6 +63,9 @@ class HipBackend {
   // `a` with the fast order's row-block plan (large tiles), sharing the
   // arrays of prepare(a); cached with `a`.
   const DevCsr& prepare_fast(const CsrView& a);
+  // `a` with its fixed-width copy (DevCsr::slot_*) for the one-wave-per-row
+  // batch kernels, sharing the arrays of prepare(a); cached with `a`.
+  const DevCsr& prepare_slots(const CsrView& a);
   // The column-panel stack of `a` (kPanels panels balanced by nonzeros, see
   // kernels.hpp), built on the host, planned for the fast order and cached
   // with `a`.

@@ -111,7 +111,8 @@ class Engine {
 
   Int n_ = 0, m_ = 0;
   int count_ = 0;
-  DevCsr dk_, dkt_;
+  DevCsr dk_, dkt_;    // the CSR
+  DevCsr dks_, dkts_;  // the same with their fixed-width row copies
   std::vector<Control> ctl_;
   std::vector<anuk_pdhg_options> lane_opt_;  // each lane's options (its termination test)
   Mask active_ = 0;  // by lane
@@ -124,10 +125,11 @@ class Engine {
   // One past the highest live slot, rounded up to a whole 16-slot line: the
   // step kernels' lane bound, so at most four graph variants per parity.
   Int span_ = kLanes;
-  // Waves per workgroup and rows per wave of the primal (K^T) and dual (K)
-  // launches.
+  // Waves per workgroup, rows per wave and row layout (the fixed-width copy
+  // or the CSR) of the primal (K^T) and dual (K) launches.
   int pw_ = 2, dw_ = 2;
   int pr_ = 1, dr_ = 1;
+  bool ps_ = false, ds_ = false;
 
   double* slab_ = nullptr;
   double *X_[2], *Y_[2], *KX_[2];
@@ -195,15 +197,15 @@ void Engine::allocate() {
   pinned_ = be_.pinned_slot(kBatchScratchSlot + depth_, kPinnedDoubles);
 }
 
-// Row-interleaves one vector per lane (rows entries each) and uploads it.
-// Row-interleaves one vector per lane (rows entries each) into `line`:
-// filled row by row (one 512-byte line at a time, the padding lanes
-// zeroed), so the writes stream instead of striding across the buffer.
-void interleave(double* out, Int rows, int count, const std::vector<const double*>& src) {
+// Row-interleaves one vector per lane (rows entries each) into `out`, whose
+// rows are `width` slots apart: filled row by row (the padding slots up to
+// `width` zeroed), so the writes stream instead of striding across the
+// buffer.
+void interleave(double* out, Int rows, int count, int width, const std::vector<const double*>& src) {
   for (Int r = 0; r < rows; ++r) {
-    double* line = out + static_cast<size_t>(r) * kLanes;
+    double* line = out + static_cast<size_t>(r) * static_cast<size_t>(width);
     for (int l = 0; l < count; ++l) line[l] = src[static_cast<size_t>(l)][r];
-    for (int l = count; l < kLanes; ++l) line[l] = 0.0;
+    for (int l = count; l < width; ++l) line[l] = 0.0;
   }
 }
 
@@ -234,7 +236,11 @@ void Engine::upload_lanes() {
     }
   }
   // Every lane vector is interleaved straight into one pinned staging area
-  // and copied asynchronously; one wait covers them all.
+  // and copied asynchronously; one wait covers them all. Only the slots up to
+  // the lane count rounded to 16 are staged and sent, as a strided copy into
+  // the rows' 512-byte lines: the slab is zeroed, so the slots above stay
+  // the zero padding (a small batch, such as a polishing sub-batch, then
+  // moves a fraction of the bytes).
   struct Item {
     double* dst;
     Int rows;
@@ -262,8 +268,9 @@ void Engine::upload_lanes() {
       {X_[0], n_, own(xs)},
       {Y_[0], m_, own(ys)},
   };
+  const int width = std::min(kLanes, (count_ + 15) / 16 * 16);
   size_t total = static_cast<size_t>(n_) + static_cast<size_t>(m_);  // the shared diagonals
-  for (const Item& it : items) total += static_cast<size_t>(it.rows) * kLanes;
+  for (const Item& it : items) total += static_cast<size_t>(it.rows) * static_cast<size_t>(width);
   double* stage = be_.pinned_scratch(total);
   hipStream_t s = be_.stream();
   size_t off = 0;
@@ -274,8 +281,16 @@ void Engine::upload_lanes() {
     off += count;
   };
   for (const Item& it : items) {
-    interleave(stage + off, it.rows, count_, it.src);
-    send(it.dst, static_cast<size_t>(it.rows) * kLanes);
+    interleave(stage + off, it.rows, count_, width, it.src);
+    if (width == kLanes) {
+      send(it.dst, static_cast<size_t>(it.rows) * kLanes);
+    } else if (it.rows > 0) {
+      const size_t line = static_cast<size_t>(width) * sizeof(double);
+      hip_check(hipMemcpy2DAsync(it.dst, kLanes * sizeof(double), stage + off, line, line,
+                                 static_cast<size_t>(it.rows), hipMemcpyHostToDevice, s),
+                "upload lanes");
+      off += static_cast<size_t>(it.rows) * static_cast<size_t>(width);
+    }
   }
   std::copy(col_scale_.begin(), col_scale_.end(), stage + off);
   send(const_cast<double*>(data_.col_scale), uz(n_));
@@ -329,13 +344,14 @@ void Engine::enqueue_period(int steps, bool has_check, int parity) {
     const LaunchEvents pe =
         be_.launch_events(diag ? "batch_primal_step_check" : "batch_primal_step",
                           12.0 * nnz + 4.0 * (n + 1) + 8.0 * lanes * (m + (diag ? 8.0 : 7.0) * n));
-    launch_batch_primal_step(s, dkt_, Y_[p], data_, d_params_, step, span_, pb, diag, pw_, pr_,
-                             &pe);
+    launch_batch_primal_step(s, ps_ ? dkts_ : dkt_, Y_[p], data_, d_params_, step, span_, pb, diag,
+                             pw_, pr_, &pe);
     DualBufs db{Y_[p], KX_[p], AY_, AKX_, Y_[1 - p], KX_[1 - p], DY_, KDX_, AY_, AKX_};
     const LaunchEvents de =
         be_.launch_events(diag ? "batch_dual_step_check" : "batch_dual_step",
                           12.0 * nnz + 4.0 * (m + 1) + 8.0 * lanes * (n + (diag ? 9.0 : 7.0) * m));
-    launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, step, span_, db, diag, dw_, dr_, &de);
+    launch_batch_dual_step(s, ds_ ? dks_ : dk_, XBAR_, data_, d_params_, step, span_, db, diag, dw_,
+                           dr_, &de);
     p = 1 - p;
   }
   if (has_check) {
@@ -593,11 +609,11 @@ std::uint64_t pattern_key(const Csr& a) {
 
 // The launch-shape tuner: times the primal and dual step kernels in every
 // shape (1, 2, 4 or 8 waves per workgroup; 1, 2 or 4 consecutive rows per
-// wave) on this stack's K^T and K and keeps, per sparsity pattern for the
-// context's life, the fastest, if it beats the default shape (two waves,
-// one row: near the fastest on every pattern measured) by at least
-// kTuneMargin; otherwise the default stays, so timing noise between
-// near-equal shapes does not flip the choice. A shape's
+// wave from the CSR, or one row per wave from the fixed-width row copy) on
+// this stack's K^T and K and keeps, per sparsity pattern for the context's
+// life, the fastest, if it beats the default shape by at least kTuneMargin;
+// otherwise the default stays, so timing noise between near-equal shapes
+// does not flip the choice. A shape's
 // time is the best of kTrials means of kReps back-to-back launches. Every
 // shape computes the same bits, so the choice changes only the time. The
 // trial launches read the current iterate and write only buffers the first
@@ -605,25 +621,34 @@ std::uint64_t pattern_key(const Csr& a) {
 // opens no Halpern epoch.
 namespace {
 constexpr double kTuneMargin = 0.03;
-constexpr int kDefaultShape = 21;  // waves * 10 + rows per wave
-int shape_waves(int shape) { return shape / 10; }
+// A shape is 100 * layout + 10 * waves + rows per wave, layout 1 for the
+// fixed-width row copy and 0 for the CSR.
+constexpr int kDefaultShape = 21;
+int shape_waves(int shape) { return (shape % 100) / 10; }
 int shape_rows(int shape) { return shape % 10; }
+bool shape_slots(int shape) { return shape >= 100; }
 }  // namespace
 
 void Engine::tune() {
-  if (opt_.batch_waves != 0 || opt_.batch_rows != 0) {
-    pw_ = dw_ = opt_.batch_waves != 0 ? opt_.batch_waves : 2;
-    pr_ = dr_ = opt_.batch_rows != 0 ? opt_.batch_rows : 1;
+  if (opt_.batch_waves != 0 || opt_.batch_rows != 0 || opt_.batch_layout != 0) {
+    pw_ = dw_ = opt_.batch_waves != 0 ? opt_.batch_waves : shape_waves(kDefaultShape);
+    pr_ = dr_ = opt_.batch_rows != 0 ? opt_.batch_rows : shape_rows(kDefaultShape);
+    ps_ = ds_ = opt_.batch_layout != 0 ? opt_.batch_layout == 2 : shape_slots(kDefaultShape);
     return;
   }
+  pw_ = dw_ = shape_waves(kDefaultShape);
+  pr_ = dr_ = shape_rows(kDefaultShape);
+  ps_ = ds_ = shape_slots(kDefaultShape);
   const std::uint64_t kp = pattern_key(k_), ktp = pattern_key(kt_);
   const HipBackend::Tuning* tp = be_.tuned("batch_primal_step", ktp);
   const HipBackend::Tuning* td = be_.tuned("batch_dual_step", kp);
   if (tp != nullptr && td != nullptr) {
     pw_ = shape_waves(tp->chosen);
     pr_ = shape_rows(tp->chosen);
+    ps_ = shape_slots(tp->chosen);
     dw_ = shape_waves(td->chosen);
     dr_ = shape_rows(td->chosen);
+    ds_ = shape_slots(td->chosen);
     return;
   }
   if (opt_.autotune == 0 || depth_ != 0 || be_.profiling()) return;
@@ -661,6 +686,14 @@ void Engine::tune() {
                                                false, w, r);
                       })});
     }
+    p.us.push_back({100 + w * 10 + 1, mean_us([&] {
+                      launch_batch_primal_step(s, dkts_, Y_[cur_], data_, d_params_, 1, span_, pb,
+                                               false, w, 1);
+                    })});
+    d.us.push_back({100 + w * 10 + 1, mean_us([&] {
+                      launch_batch_dual_step(s, dks_, XBAR_, data_, d_params_, 1, span_, db, false,
+                                             w, 1);
+                    })});
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -677,8 +710,10 @@ void Engine::tune() {
   d.chosen = pick(d);
   pw_ = shape_waves(p.chosen);
   pr_ = shape_rows(p.chosen);
+  ps_ = shape_slots(p.chosen);
   dw_ = shape_waves(d.chosen);
   dr_ = shape_rows(d.chosen);
+  ds_ = shape_slots(d.chosen);
   be_.record_tuning(ktp, std::move(p));
   be_.record_tuning(kp, std::move(d));
 }
@@ -730,6 +765,8 @@ std::vector<LaneOutcome> Engine::run() {
   m_ = k_.rows;
   dk_ = be_.prepare(view_of(k_));
   dkt_ = be_.prepare(view_of(kt_));
+  dks_ = be_.prepare_slots(view_of(k_));
+  dkts_ = be_.prepare_slots(view_of(kt_));
   allocate();
   for (Lane& l : lanes_) l.set_scales();
   ctl_.resize(static_cast<size_t>(count_));

@@ -56,55 +56,106 @@ __device__ __forceinline__ LaneScalars lane_scalars(const LaneParams& p, int ste
 // wave waits for one round of memory latency rather than three. A retired
 // lane issues no vector load and stores nothing: with the live lanes kept in
 // the lowest slots, a row touches only the cache lines that hold them.
-// The leading arguments (the CSR's arrays, the gathered vector, the row
+// The leading arguments (the row structure, the gathered vector, the row
 // count, the slot span, the grid size, the step and the lane table) are
 // plain scalars so they can be preloaded into SGPRs at wave launch
-// (compiled with kernel-argument preloading): the row's start is then the
-// first load the wave waits on. The epilogue's buffers come as structs,
-// loaded while the row sum is in flight.
-template <bool kDiag, int kWaves>
+// (compiled with kernel-argument preloading). The epilogue's buffers come as
+// structs, loaded while the row sum is in flight.
+//
+// The row structure is the CSR (start, index, value) when kW is 0, and the
+// fixed-width copy (row, index, value of DevCsr::slot_*) of width kW
+// otherwise: then the row's first loads depend on the row number alone, and
+// a row longer than kW continues in the CSR's index and value arrays (the
+// trailing arguments).
+// The row-sum halves for either structure: with the CSR, the gather half
+// does nothing and the finish half takes the whole row sum.
+template <int kW>
+struct StepRow {
+  lanes::SlotRow<kW> r;
+};
+template <>
+struct StepRow<0> {};
+
+template <int kW>
+__device__ __forceinline__ void step_gather(const Int* __restrict__ s0, const Int* __restrict__ s1,
+                                            const double* __restrict__ s2, Int row,
+                                            const double* __restrict__ v, Int v_rows, int lane,
+                                            StepRow<kW>& g) {
+  if constexpr (kW != 0)
+    lanes::slot_gather<kW>(lanes::SlotArrays{s0, s1, s2}, row, v, v_rows, lane, g.r);
+}
+
+template <int kW>
+__device__ __forceinline__ double step_finish(const Int* __restrict__ s0,
+                                              const Int* __restrict__ s1,
+                                              const double* __restrict__ s2,
+                                              const Int* __restrict__ index,
+                                              const double* __restrict__ value, Int row,
+                                              const double* __restrict__ v, int lane,
+                                              const StepRow<kW>& g) {
+  if constexpr (kW == 0) {
+    return lanes::row_sum(lanes::CsrArrays{s0, s1, s2}, row, v, lane);
+  } else {
+    return lanes::slot_finish<kW>(lanes::SlotArrays{s0, s1, s2},
+                                  lanes::CsrArrays{nullptr, index, value}, row, g.r, v, lane);
+  }
+}
+
+// The register budget of a step kernel: a wide fixed-width row keeps all its
+// gathers and the epilogue's operands in flight, which needs more than the
+// 64 VGPRs of eight waves per SIMD; four leave 128.
+template <int kW>
+inline constexpr int kStepMaxWaves = kW >= 32 ? 4 : 8;
+
+template <bool kDiag, int kWaves, int kW>
 __global__ void __launch_bounds__(64 * kWaves)
-    bprimal_kernel(const Int* __restrict__ kt_start, const Int* __restrict__ kt_index,
-                   const double* __restrict__ kt_value, const double* __restrict__ y, Int rows,
-                   Int span, int blocks, int step, const LaneParams* __restrict__ params,
-                   BatchData d, PrimalBufs b) {
+    __attribute__((amdgpu_waves_per_eu(1, kStepMaxWaves<kW>)))
+    bprimal_kernel(const Int* __restrict__ s0, const Int* __restrict__ s1,
+                   const double* __restrict__ s2, const double* __restrict__ y, Int rows,
+                   Int span, int blocks, int step, Int y_rows, const LaneParams* __restrict__ params,
+                   BatchData d, PrimalBufs b, const Int* __restrict__ kt_index,
+                   const double* __restrict__ kt_value) {
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int j = lanes::wave_row_of<kWaves>(blocks);
   if (j >= rows || lane >= span) return;
-  const lanes::CsrArrays kt{kt_start, kt_index, kt_value};
   const size_t o = at(j, lane);
+  StepRow<kW> g;
+  step_gather<kW>(s0, s1, s2, j, y, y_rows, lane, g);
   const LaneParams P = params[lane];
   const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
-  const double kty = lanes::row_sum(kt, j, y, lane);
+  const double kty = step_finish<kW>(s0, s1, s2, kt_index, kt_value, j, y, lane, g);
   if (!P.active) return;
   const LaneScalars sc = lane_scalars(P, step);
   double v = xj - sc.tau * (cj - kty);
   if (v < lo) v = lo;
   if (v > hi) v = hi;
   const double dd = v - xj;
-  b.x_bar[o] = v + dd;
+  stream_out(b.x_bar + o, v + dd);
   double next = v;
   if (sc.gamma != 0.0) next += sc.gamma * dd;
   if (sc.blend) next = sc.w * next + (1.0 - sc.w) * anc;
-  b.x_out[o] = next;
-  if (sc.open_epoch) b.anchor_out[o] = next;
-  if constexpr (kDiag) b.dx[o] = dd;
+  stream_out(b.x_out + o, next);
+  if (sc.open_epoch) stream_out(b.anchor_out + o, next);
+  if constexpr (kDiag) stream_out(b.dx + o, dd);
 }
 
-template <bool kDiag, int kWaves>
+template <bool kDiag, int kWaves, int kW>
 __global__ void __launch_bounds__(64 * kWaves)
-    bdual_kernel(const Int* __restrict__ k_start, const Int* __restrict__ k_index,
-                 const double* __restrict__ k_value, const double* __restrict__ x_bar, Int rows,
-                 Int span, int blocks, int step, const LaneParams* __restrict__ params,
-                 BatchData d, DualBufs b) {
+    __attribute__((amdgpu_waves_per_eu(1, kStepMaxWaves<kW>)))
+    bdual_kernel(const Int* __restrict__ s0, const Int* __restrict__ s1,
+                 const double* __restrict__ s2, const double* __restrict__ x_bar, Int rows,
+                 Int span, int blocks, int step, Int x_rows, const LaneParams* __restrict__ params,
+                 BatchData d, DualBufs b, const Int* __restrict__ k_index,
+                 const double* __restrict__ k_value) {
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int i = lanes::wave_row_of<kWaves>(blocks);
   if (i >= rows || lane >= span) return;
-  const lanes::CsrArrays k{k_start, k_index, k_value};
   const size_t o = at(i, lane);
+  StepRow<kW> g;
+  step_gather<kW>(s0, s1, s2, i, x_bar, x_rows, lane, g);
   const LaneParams P = params[lane];
   const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
-  const double kxb = lanes::row_sum(k, i, x_bar, lane);
+  const double kxb = step_finish<kW>(s0, s1, s2, k_index, k_value, i, x_bar, lane, g);
   if (!P.active) return;
   const LaneScalars sc = lane_scalars(P, step);
   double v = yi + sc.sigma * (qi - kxb);
@@ -121,15 +172,15 @@ __global__ void __launch_bounds__(64 * kWaves)
     ny = sc.w * ny + (1.0 - sc.w) * ay;
     nk = sc.w * nk + (1.0 - sc.w) * ak;
   }
-  b.y_out[o] = ny;
-  b.k_out[o] = nk;
+  stream_out(b.y_out + o, ny);
+  stream_out(b.k_out + o, nk);
   if (sc.open_epoch) {
-    b.anchor_y_out[o] = ny;
-    b.anchor_kx_out[o] = nk;
+    stream_out(b.anchor_y_out + o, ny);
+    stream_out(b.anchor_kx_out + o, nk);
   }
   if constexpr (kDiag) {
-    b.dy[o] = dd;
-    b.k_dx[o] = kd;
+    stream_out(b.dy + o, dd);
+    stream_out(b.k_dx + o, kd);
   }
 }
 
@@ -171,13 +222,13 @@ __global__ void __launch_bounds__(64 * kWaves)
     if (v < lo[r]) v = lo[r];
     if (v > hi[r]) v = hi[r];
     const double dd = v - xj[r];
-    b.x_bar[o] = v + dd;
+    stream_out(b.x_bar + o, v + dd);
     double next = v;
     if (sc.gamma != 0.0) next += sc.gamma * dd;
     if (sc.blend) next = sc.w * next + (1.0 - sc.w) * anc[r];
-    b.x_out[o] = next;
-    if (sc.open_epoch) b.anchor_out[o] = next;
-    if constexpr (kDiag) b.dx[o] = dd;
+    stream_out(b.x_out + o, next);
+    if (sc.open_epoch) stream_out(b.anchor_out + o, next);
+    if constexpr (kDiag) stream_out(b.dx + o, dd);
   }
 }
 
@@ -224,15 +275,15 @@ __global__ void __launch_bounds__(64 * kWaves)
       ny = sc.w * ny + (1.0 - sc.w) * ay[r];
       nk = sc.w * nk + (1.0 - sc.w) * ak[r];
     }
-    b.y_out[o] = ny;
-    b.k_out[o] = nk;
+    stream_out(b.y_out + o, ny);
+    stream_out(b.k_out + o, nk);
     if (sc.open_epoch) {
-      b.anchor_y_out[o] = ny;
-      b.anchor_kx_out[o] = nk;
+      stream_out(b.anchor_y_out + o, ny);
+      stream_out(b.anchor_kx_out + o, nk);
     }
     if constexpr (kDiag) {
-      b.dy[o] = dd;
-      b.k_dx[o] = kd;
+      stream_out(b.dy + o, dd);
+      stream_out(b.k_dx + o, kd);
     }
   }
 }
@@ -543,20 +594,54 @@ auto by_waves(int waves) {
     default: return Pick<4>::get();
   }
 }
+// The one-row step kernel for (waves, row layout): slot width 0 is the CSR.
+template <bool kDiag, int kW>
+auto primal_by_waves(int waves) {
+  switch (waves) {
+    case 1: return &bprimal_kernel<kDiag, 1, kW>;
+    case 2: return &bprimal_kernel<kDiag, 2, kW>;
+    case 8: return &bprimal_kernel<kDiag, 8, kW>;
+    default: return &bprimal_kernel<kDiag, 4, kW>;
+  }
+}
 template <bool kDiag>
-struct PrimalPick {
-  template <int W>
-  struct At {
-    static auto get() { return &bprimal_kernel<kDiag, W>; }
-  };
-};
+auto primal_one_row(int waves, int width) {
+  switch (width) {
+    case 8: return primal_by_waves<kDiag, 8>(waves);
+    case 16: return primal_by_waves<kDiag, 16>(waves);
+    case 32: return primal_by_waves<kDiag, 32>(waves);
+    default: return primal_by_waves<kDiag, 0>(waves);
+  }
+}
+template <bool kDiag, int kW>
+auto dual_by_waves(int waves) {
+  switch (waves) {
+    case 1: return &bdual_kernel<kDiag, 1, kW>;
+    case 2: return &bdual_kernel<kDiag, 2, kW>;
+    case 8: return &bdual_kernel<kDiag, 8, kW>;
+    default: return &bdual_kernel<kDiag, 4, kW>;
+  }
+}
 template <bool kDiag>
-struct DualPick {
-  template <int W>
-  struct At {
-    static auto get() { return &bdual_kernel<kDiag, W>; }
-  };
+auto dual_one_row(int waves, int width) {
+  switch (width) {
+    case 8: return dual_by_waves<kDiag, 8>(waves);
+    case 16: return dual_by_waves<kDiag, 16>(waves);
+    case 32: return dual_by_waves<kDiag, 32>(waves);
+    default: return dual_by_waves<kDiag, 0>(waves);
+  }
+}
+// The three leading row-structure arguments of a one-row step kernel.
+struct RowStructure {
+  const Int* s0;
+  const Int* s1;
+  const double* s2;
+  int width;
 };
+RowStructure row_structure(const DevCsr& a) {
+  if (a.slot_width > 0) return {a.slot_row, a.slot_index, a.slot_value, a.slot_width};
+  return {a.start, a.index, a.value, 0};
+}
 template <int W>
 struct SpmmAt {
   static auto get() { return &bspmm_kernel<W>; }
@@ -598,8 +683,9 @@ void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const B
                     d, b, params, step, span);
   } else {
     const int blocks = static_cast<int>((kt.rows + waves - 1) / waves);
-    launch_rows(by_waves<PrimalPick<kDiag>::template At>(waves), waves, kt.rows, s, ev, what,
-                kt.start, kt.index, kt.value, y, kt.rows, span, blocks, step, params, d, b);
+    const RowStructure t = row_structure(kt);
+    launch_rows(primal_one_row<kDiag>(waves, t.width), waves, kt.rows, s, ev, what, t.s0, t.s1,
+                t.s2, y, kt.rows, span, blocks, step, kt.cols, params, d, b, kt.index, kt.value);
   }
 }
 
@@ -616,8 +702,9 @@ void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const 
                     d, b, params, step, span);
   } else {
     const int blocks = static_cast<int>((k.rows + waves - 1) / waves);
-    launch_rows(by_waves<DualPick<kDiag>::template At>(waves), waves, k.rows, s, ev, what,
-                k.start, k.index, k.value, x_bar, k.rows, span, blocks, step, params, d, b);
+    const RowStructure t = row_structure(k);
+    launch_rows(dual_one_row<kDiag>(waves, t.width), waves, k.rows, s, ev, what, t.s0, t.s1, t.s2,
+                x_bar, k.rows, span, blocks, step, k.cols, params, d, b, k.index, k.value);
   }
 }
 }  // namespace

@@ -62,7 +62,9 @@ inline constexpr int kLaneCheckStride = 16;
 // `span`: the lanes below it load and compute (the live ones among them
 // store); a kernel argument, so the bound costs no dependent load.
 // `waves`: waves per workgroup, 1, 2, 4 or 8; `rows`: consecutive rows per
-// wave, 1, 2 or 4. Every shape computes the same bits.
+// wave, 1, 2 or 4. With one row per wave the kernels read the matrix's
+// fixed-width row copy when it carries one (DevCsr::slot_width > 0), else
+// its CSR. Every shape computes the same bits.
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                               const BatchData& d, const LaneParams* params, int step, Int span,
                               const PrimalBufs& b, bool diag, int waves = 4, int rows = 1,

@@ -547,6 +547,10 @@ anuk_pdhg_options checked_options(const anuk_pdhg_options* options) {
        "batch_waves must be 0, 1, 2, 4 or 8");
   need(o.batch_rows == 0 || o.batch_rows == 1 || o.batch_rows == 2 || o.batch_rows == 4,
        "batch_rows must be 0, 1, 2 or 4");
+  need(o.batch_layout == 0 || o.batch_layout == 1 || o.batch_layout == 2,
+       "batch_layout must be 0, 1 or 2");
+  need(o.batch_layout != 2 || o.batch_rows == 0 || o.batch_rows == 1,
+       "batch_layout 2 (the fixed-width row copy) runs one row per wave");
   return o;
 }
 

@@ -81,11 +81,10 @@ __device__ __forceinline__ void chunk_sum(const CsrArrays& a, Int k, const doubl
   for (int u = 0; u < W; ++u) s += w[u] * x[u];
 }
 
-// row_sum below over the arrays alone (the same terms in the same order).
-__device__ __forceinline__ double row_sum(const CsrArrays& a, Int row,
-                                          const double* __restrict__ v, int lane) {
-  const Int lo = a.start[row], hi = a.start[row + 1];
-  double s = 0.0;
+// Adds the terms [lo, hi) of a's storage to s in storage order (row_sum
+// below, over a range).
+__device__ __forceinline__ double range_sum(const CsrArrays& a, Int lo, Int hi,
+                                            const double* __restrict__ v, int lane, double s) {
   Int k = lo;
   for (; k + 8 <= hi; k += 8) chunk_sum<8>(a, k, v, lane, s);
   if (k + 4 <= hi) {
@@ -106,6 +105,85 @@ __device__ __forceinline__ double row_sum(const CsrArrays& a, Int row,
     for (int u = 0; u < 3; ++u)
       if (u < rest) s += w[u] * x[u];
   }
+  return s;
+}
+
+// row_sum below over the arrays alone (the same terms in the same order).
+__device__ __forceinline__ double row_sum(const CsrArrays& a, Int row,
+                                          const double* __restrict__ v, int lane) {
+  return range_sum(a, a.start[row], a.start[row + 1], v, lane, 0.0);
+}
+
+// The fixed-width copy of a CSR (DevCsr::slot_*), as leading scalar kernel
+// arguments.
+struct SlotArrays {
+  const Int* __restrict__ row;      // [rows][2]: start, length
+  const Int* __restrict__ index;    // [rows][W]
+  const double* __restrict__ value; // [rows][W]
+};
+
+// A raw buffer resource over `bytes` bytes at p (stride 0, the data-format
+// word of the gfx9 family): a load through it at an offset of `bytes` or
+// more returns 0 and reads no memory, which turns a gather on or off with
+// no branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(const void* p, Int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), static_cast<short>(0), bytes,
+                                           0x00020000);
+}
+// A row sum over the fixed-width copy, in two halves so a kernel can put its
+// own loads between them. slot_gather reads the row's start and length and
+// its W indices: the addresses depend on the row alone, so the scalar loads
+// go out with the wave's first loads, and the row's gathers are then all in
+// flight at once. The gathers go through a buffer resource over v
+// (`v_rows` rows of 64 lanes); a slot past the row's length gets an offset
+// outside it, reads nothing and yields 0. slot_finish adds the row's terms
+// in storage order: a slot past the length adds +0.0, which leaves the sum
+// unchanged (a sum that starts at +0.0 is never -0.0), so the result has
+// the bits of the serial loop. A row longer than W continues from the CSR
+// `a`. v_rows * 512 must stay below 2^31 (DevCsr::slot_width is 0 for
+// wider matrices).
+template <int W>
+struct SlotRow {
+  Int lo, len;
+  double x[W];
+};
+
+template <int W>
+__device__ __forceinline__ void slot_gather(const SlotArrays& t, Int row,
+                                            const double* __restrict__ v, Int v_rows, int lane,
+                                            SlotRow<W>& r) {
+  r.lo = t.row[2 * row];
+  r.len = t.row[2 * row + 1];
+  const size_t base = static_cast<size_t>(row) * W;
+  const __amdgpu_buffer_rsrc_t rs = buffer_over(v, v_rows * (kLaneCount * 8));
+  const unsigned mine = static_cast<unsigned>(lane) * 8u;
+  Int ix[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) ix[u] = t.index[base + u];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    // Arithmetic rather than a select, so every index load stays
+    // unconditional: the top bit is set exactly when u >= len.
+    const unsigned past = static_cast<unsigned>(r.len - 1 - u) >> 31;
+    const unsigned at_row =
+        static_cast<unsigned>(ix[u]) * static_cast<unsigned>(kLaneCount * 8) + (past << 31);
+    r.x[u] = __builtin_bit_cast(
+        double, __builtin_amdgcn_raw_buffer_load_b64(rs, static_cast<int>(at_row + mine), 0, 0));
+  }
+}
+
+template <int W>
+__device__ __forceinline__ double slot_finish(const SlotArrays& t, const CsrArrays& a, Int row,
+                                              const SlotRow<W>& r, const double* __restrict__ v,
+                                              int lane) {
+  const size_t base = static_cast<size_t>(row) * W;
+  double s = 0.0;
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    const double p = t.value[base + u] * r.x[u];
+    s += u < r.len ? p : 0.0;
+  }
+  if (r.len > W) s = range_sum(a, r.lo + W, r.lo + r.len, v, lane, s);
   return s;
 }
 

@@ -1103,6 +1103,7 @@ void default_options(anuk_pdhg_options* o) {
   o->autotune = 1;
   o->batch_waves = 0;
   o->batch_rows = 0;
+  o->batch_layout = 0;
 }
 
 }  // namespace anuk

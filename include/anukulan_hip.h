@@ -334,14 +334,20 @@ typedef struct anuk_pdhg_options {
                                           deterministic either way */
   int32_t autotune;                    /* 1: the batched solver times its launch shapes
                                           (waves per workgroup: 1, 2, 4, 8; rows per
-                                          wave: 1, 2, 4) on the stack's K and K^T once per
-                                          matrix and keeps the fastest when it beats two
-                                          waves of one row by 3%; every shape computes
-                                          the same bits. 0: two waves, one row */
+                                          wave: 1, 2, 4; the matrix read from its CSR or
+                                          from a fixed-width row copy) on the stack's K
+                                          and K^T once per matrix and keeps the fastest
+                                          when it beats the default shape by 3%; every
+                                          shape computes the same bits. 0: the default
+                                          shape */
   int32_t batch_waves;                 /* 0: as autotune decides; 1, 2, 4 or 8: the batched
                                           step kernels use this many waves per workgroup */
   int32_t batch_rows;                  /* 0: as autotune decides; 1, 2 or 4: consecutive
                                           rows per wave of the batched step kernels */
+  int32_t batch_layout;                /* 0: as autotune decides; 1: the batched step
+                                          kernels read K and K^T from their CSR; 2: from a
+                                          fixed-width copy of each row's first entries
+                                          (one row per wave only) */
 } anuk_pdhg_options;
 
 void anuk_pdhg_default_options(anuk_pdhg_options* options);

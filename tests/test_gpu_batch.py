@@ -139,22 +139,43 @@ def test_full_stack_plain_per_case_against_the_reference(ctx):
 
 @pytest.mark.parametrize("periods", [4, 24])
 def test_every_launch_shape_gives_the_same_bits(ctx, periods):
-    # The tuner picks among waves per workgroup (1, 2, 4, 8) and consecutive
-    # rows per wave (1, 2, 4); a lane's arithmetic must not depend on the
-    # shape, so every shape and the tuned choice give the same iterates to
-    # the bit.
+    # The tuner picks among waves per workgroup (1, 2, 4, 8), consecutive
+    # rows per wave (1, 2, 4) and the matrix's layout (its CSR, or the
+    # fixed-width copy of each row's first entries: 8 wide for K^T here, 32
+    # for K); a lane's arithmetic must not depend on the shape, so every
+    # shape and the tuned choice give the same iterates to the bit.
     base, cases, models = stack(periods, 8)
     lps = [m.to_lp() for m in models]
-    want = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=4, batch_rows=1)
-    for w, r in ((1, 1), (2, 1), (8, 1), (1, 2), (4, 2), (8, 2), (1, 4), (2, 4), (4, 4), (8, 4),
-                 (0, 0)):
-        got = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=w, batch_rows=r)
+    want = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=4, batch_rows=1,
+                            batch_layout=1)
+    for w, r, s in ((1, 1, 1), (2, 1, 1), (8, 1, 1), (1, 2, 1), (4, 2, 1), (8, 2, 1), (1, 4, 1),
+                    (2, 4, 1), (4, 4, 1), (8, 4, 1), (1, 1, 2), (2, 1, 2), (4, 1, 2), (8, 1, 2),
+                    (0, 0, 0)):
+        got = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=w, batch_rows=r,
+                               batch_layout=s)
         for g, h in zip(got, want):
-            assert g.iterations == h.iterations, (w, r)
-            assert np.array_equal(g.x, h.x) and np.array_equal(g.y, h.y), (w, r)
+            assert g.iterations == h.iterations, (w, r, s)
+            assert np.array_equal(g.x, h.x) and np.array_equal(g.y, h.y), (w, r, s)
     report = ctx.tuning_report()
     print("\n" + report)
     assert "batch_primal_step" in report and "batch_dual_step" in report and "chosen=w" in report
+
+
+def test_fixed_width_layout_with_rows_past_its_width(ctx):
+    # A random LP whose K has rows of up to 41 entries and whose K^T has rows
+    # of up to 97: both fixed-width copies are 32 wide, and the longer rows
+    # continue in the CSR. Six cases (scaled objectives) batched with either
+    # layout must give the same iterates to the bit.
+    base = G.random_lp(150, 700, seed=7)
+    models = [G.NaturalModel(**{**base.__dict__, "objective": base.objective * (1.0 + 0.01 * k)})
+              for k in range(6)]
+    lps = [m.to_lp() for m in models]
+    want = solve_pdhg_batch(ctx, lps, tolerance=1e-6, batch_waves=2, batch_rows=1, batch_layout=1)
+    got = solve_pdhg_batch(ctx, lps, tolerance=1e-6, batch_waves=2, batch_rows=1, batch_layout=2)
+    assert all(h.status == "optimal" for h in want)
+    for g, h in zip(got, want):
+        assert g.status == h.status and g.iterations == h.iterations
+        assert np.array_equal(g.x, h.x) and np.array_equal(g.y, h.y)
 
 
 def test_more_than_one_wave_of_cases(ctx):

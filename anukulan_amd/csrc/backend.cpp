@@ -609,6 +609,24 @@ Terms HipBackend::convergence_terms(Int n, Int m, const DevProblem& p, const dou
 
 // ---- tuning ------------------------------------------------------------------
 
+namespace {
+// A batched-step shape, 100 * layout + 10 * waves per workgroup + rows per
+// wave, as "w<waves>r<rows>" with an "s" for layout 1 (the fixed-width row
+// copy); a stack launch choice, 200 + 10 * binned + non-temporal, as "plan"
+// or "binned" with "+nt".
+std::string shape_label(int shape) {
+  char buf[32];
+  if (shape >= 200) {
+    std::snprintf(buf, sizeof(buf), "%s%s", (shape / 10) % 10 != 0 ? "binned" : "plan",
+                  shape % 10 != 0 ? "+nt" : "");
+  } else {
+    std::snprintf(buf, sizeof(buf), "w%dr%d%s", (shape % 100) / 10, shape % 10,
+                  shape >= 100 ? "s" : "");
+  }
+  return buf;
+}
+}  // namespace
+
 std::string HipBackend::tuning_report() const {
   std::string out;
   char buf[96];
@@ -617,16 +635,11 @@ std::string HipBackend::tuning_report() const {
     out += t.kernel;
     std::snprintf(buf, sizeof(buf), " rows=%d nnz=%d", t.rows, t.nnz);
     out += buf;
-    // A shape is 100 * layout + 10 * waves per workgroup + rows per wave;
-    // layout 1, the fixed-width row copy, is marked "s".
     for (const auto& c : t.us) {
-      std::snprintf(buf, sizeof(buf), " us@w%dr%d%s=%.3f", (c.first % 100) / 10, c.first % 10,
-                    c.first >= 100 ? "s" : "", c.second);
+      std::snprintf(buf, sizeof(buf), " us@%s=%.3f", shape_label(c.first).c_str(), c.second);
       out += buf;
     }
-    std::snprintf(buf, sizeof(buf), " chosen=w%dr%d%s\n", (t.chosen % 100) / 10, t.chosen % 10,
-                  t.chosen >= 100 ? "s" : "");
-    out += buf;
+    out += " chosen=" + shape_label(t.chosen) + "\n";
   }
   return out;
 }

@@ -87,12 +87,19 @@ __device__ double block_sum(double v) {
 // and the block's row bounds and nonzero range are loaded before the
 // epilogue's skip test, so the block waits on the plan and the row starts
 // while its step scalars arrive.
+// `mode` holds kModeWide (rows longer than kSerialRowMax summed across a
+// wave in a fixed butterfly: the fast order, not the CPU's bits) and
+// kModeStream (the index and value streams read non-temporally: the same
+// bits).
+constexpr int kModeWide = 1;
+constexpr int kModeStream = 2;
+
 template <class Epi, int kTile>
 __global__ void __launch_bounds__(kThreads)
     spmv_kernel(const Int* __restrict__ a_start, const Int* __restrict__ a_index,
                 const double* __restrict__ a_value, const Int* __restrict__ block_row,
                 const Int* __restrict__ block_map, const double* __restrict__ x, int blocks,
-                int fast, Epi epi, double* __restrict__ partials) {
+                int mode, Epi epi, double* __restrict__ partials) {
   __shared__ double tile[kTile];
   constexpr int R = Epi::kR > 0 ? Epi::kR : 1;
   struct {
@@ -126,10 +133,10 @@ __global__ void __launch_bounds__(kThreads)
         hi_r[u] = a.start[r + 1] - k0;
       }
     }
-    if (fast) {
-      // Large K: the index and value streams are read once per product, so
-      // they are loaded non-temporally and leave the L2 to the gathered
-      // vector's lines.
+    if (mode & kModeStream) {
+      // The index and value streams are read once per product, so they are
+      // loaded non-temporally and leave the L2 to the gathered vector's
+      // lines.
 #pragma unroll
       for (int u = 0; u < kTile / kThreads; ++u) {
         const Int k = static_cast<Int>(threadIdx.x) + u * kThreads;
@@ -154,11 +161,11 @@ __global__ void __launch_bounds__(kThreads)
       double sum = 0.0;
       bool wide = false;
       if (r < r1) {
-        wide = fast && hi_r[u] - lo_r[u] > kSerialRowMax;
+        wide = (mode & kModeWide) != 0 && hi_r[u] - lo_r[u] > kSerialRowMax;
         if (!wide)
           for (Int k = lo_r[u]; k < hi_r[u]; ++k) sum += tile[k];
       }
-      if (fast) {
+      if (mode & kModeWide) {
         // The wave's long rows, one at a time: lanes stride the row's staged
         // products, a butterfly adds them (a fixed tree), and the row's own
         // thread keeps the total for its epilogue.
@@ -837,7 +844,7 @@ void spmv_with(hipStream_t s, const DevCsr& a, const double* x, const Epi& e, do
   // storage order on plans with the small one.
   validate_launch(what, grid, dim3(kThreads), ev);
   const int blocks = static_cast<int>(a.blocks);
-  const int f = fast ? 1 : 0;
+  const int f = (fast ? kModeWide | kModeStream : 0) | (a.stream_loads ? kModeStream : 0);
   auto launch = [&](auto kernel) {
     if (ev != nullptr && ev->start != nullptr) {
       hipExtLaunchKernelGGL(kernel, grid, dim3(kThreads), 0, s, ev->start, ev->stop, 0, a.start,

@@ -59,6 +59,9 @@ struct DevCsr {
   // Optional compaction: launched block i runs planned block block_map[i]
   // (`blocks` then counts the launched ones). Null: every planned block.
   const Int* block_map = nullptr;
+  // Read the index and value streams with non-temporal loads, which leaves
+  // the L2 to the gathered vector's lines (the same bits either way).
+  bool stream_loads = false;
   // Optional fixed-width copy for kernels that give each row one wave
   // (slot_width 0: not built). Row r's first slot_width entries sit at
   // [r * slot_width, (r + 1) * slot_width) of slot_index / slot_value,

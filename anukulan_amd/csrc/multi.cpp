@@ -121,6 +121,13 @@ struct DeviceStack {
   const Int* k_lp_blocks = nullptr;
   const Int* kt_lp_blocks = nullptr;
   std::vector<Int> k_lp_blocks_host, kt_lp_blocks_host;  // the same, on the host
+  // Each planned block's place when the blocks are ordered by work
+  // (nonzeros plus rows, heaviest first), and whether the launches bin the
+  // blocks by it (launch_blocks) or take the plan's order; chosen by
+  // tune_stack, with the index and value streams' load kind (DevCsr::
+  // stream_loads). Every choice gives the same bits.
+  std::vector<Int> k_rank, kt_rank;
+  bool k_binned = false, kt_binned = false;
   ChunkPlan cols, rows;
   const Int* col_lp_chunks = nullptr;
   const Int* row_lp_chunks = nullptr;
@@ -166,11 +173,13 @@ struct DeviceStack {
 // and each LP's block range.
 struct Plan {
   std::vector<Int> block_row, block_lp, lp_blocks;
+  std::vector<Int> rank;  // each block's place by work, heaviest first
 };
 
 Plan plan_stack(const Stack& st, bool transpose) {
   const std::vector<Int>& off = transpose ? st.col_off : st.row_off;
   Plan p;
+  std::vector<long> work;
   p.block_row.push_back(0);
   p.lp_blocks.push_back(0);
   for (size_t l = 0; l + 1 < off.size(); ++l) {
@@ -183,12 +192,47 @@ Plan plan_stack(const Stack& st, bool transpose) {
       for (Int b = 1; b <= blocks; ++b) {
         p.block_row.push_back(off[l] + br[b]);
         p.block_lp.push_back(static_cast<Int>(l));
+        work.push_back(static_cast<long>(a.start[uz(br[b])] - a.start[uz(br[b - 1])]) +
+                       (br[b] - br[b - 1]));
       }
       std::free(br);
     }
     p.lp_blocks.push_back(static_cast<Int>(p.block_lp.size()));
   }
+  std::vector<Int> by_work(work.size());
+  for (size_t b = 0; b < by_work.size(); ++b) by_work[b] = static_cast<Int>(b);
+  std::stable_sort(by_work.begin(), by_work.end(),
+                   [&](Int a, Int b) { return work[uz(a)] > work[uz(b)]; });
+  p.rank.assign(work.size(), 0);
+  for (size_t i = 0; i < by_work.size(); ++i) p.rank[uz(by_work[i])] = static_cast<Int>(i);
   return p;
+}
+
+// The blocks of one side to launch for the LPs marked in `on`, in the plan's
+// order, or binned: the kernels give XCD x the x-th of eight contiguous
+// ranges of the launch (xcd_block), so each XCD keeps the same LPs (and
+// their gathered vectors in its own L2), but within its range the blocks
+// run heaviest first, the long single-row blocks ahead of the tiles, so no
+// long block starts last. (Dealing the heaviest blocks over all eight
+// ranges instead was measured twice as slow on the 512-LP stack: an LP's
+// vector then has to be fetched into every XCD's L2.)
+std::vector<Int> launch_blocks(const std::vector<Int>& lp_blocks, const std::vector<char>& on,
+                               bool binned, const std::vector<Int>& rank) {
+  std::vector<Int> bl;
+  for (size_t l = 0; l + 1 < lp_blocks.size(); ++l) {
+    if (!on[l]) continue;
+    for (Int b = lp_blocks[l]; b < lp_blocks[l + 1]; ++b) bl.push_back(b);
+  }
+  if (!binned) return bl;
+  const size_t g = bl.size(), per = g / 8, rem = g % 8;
+  size_t lo = 0;
+  for (size_t x = 0; x < 8; ++x) {
+    const size_t hi = lo + per + (x < rem ? 1 : 0);
+    std::sort(bl.begin() + static_cast<long>(lo), bl.begin() + static_cast<long>(hi),
+              [&](Int a, Int b) { return rank[uz(a)] < rank[uz(b)]; });
+    lo = hi;
+  }
+  return bl;
 }
 
 struct Chunks {
@@ -261,6 +305,8 @@ void upload_stack(const Stack& st, DeviceStack* ds) {
   ds->kt_lp_blocks = ds->put(pkt.lp_blocks);
   ds->k_lp_blocks_host = pk.lp_blocks;
   ds->kt_lp_blocks_host = pkt.lp_blocks;
+  ds->k_rank = pk.rank;
+  ds->kt_rank = pkt.rank;
   const Chunks cc = chunk_stack(st.col_off), rc = chunk_stack(st.row_off);
   ds->col_chunks = static_cast<Int>(cc.lp.size());
   ds->row_chunks = static_cast<Int>(rc.lp.size());
@@ -386,6 +432,121 @@ std::vector<IterParams> flags(const std::vector<char>& on) {
   return p;
 }
 
+// The stack's launch tuner. Per side (K and K^T) it times the plain product
+// over all LPs with the blocks launched in the plan's order or binned
+// (launch_blocks), each with the index and value streams read plainly or
+// non-temporally, and keeps, per stack pattern for the context's life, the
+// fastest, if it beats the plan's order with plain loads by kStackMargin.
+// Every choice gives the same bits: a block's row sums and epilogue do not
+// depend on when it runs, and each LP's reductions fold its partials in
+// plan order. A choice is 200 + 10 * binned + non-temporal.
+namespace {
+constexpr double kStackMargin = 0.03;
+constexpr int kStackDefault = 200;
+bool stack_binned(int choice) { return (choice / 10) % 10 != 0; }
+bool stack_stream(int choice) { return choice % 10 != 0; }
+
+std::uint64_t stack_key(const Stack& st) {
+  std::uint64_t h = 1469598103934665603ull;
+  auto mix = [&](std::uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  for (const std::vector<Int>* v : {&st.col_off, &st.row_off, &st.nz_off})
+    for (Int x : *v) mix(static_cast<std::uint64_t>(static_cast<std::uint32_t>(x)));
+  return h;
+}
+}  // namespace
+
+void tune_stack(HipBackend& be, const Stack& st, DeviceStack* ds, const anuk_pdhg_options& opt) {
+  auto apply = [&](bool k_side, int choice) {
+    (k_side ? ds->k_binned : ds->kt_binned) = stack_binned(choice);
+    (k_side ? ds->k : ds->kt).stream_loads = stack_stream(choice);
+  };
+  if (opt.multi_order != 0 || opt.multi_stream_loads != 0) {
+    const int choice = kStackDefault +
+                       10 * (opt.multi_order != 0 ? (opt.multi_order == 2 ? 1 : 0)
+                                                  : (stack_binned(kStackDefault) ? 1 : 0)) +
+                       (opt.multi_stream_loads != 0 ? (opt.multi_stream_loads == 2 ? 1 : 0)
+                                                    : (stack_stream(kStackDefault) ? 1 : 0));
+    apply(true, choice);
+    apply(false, choice);
+    return;
+  }
+  const std::uint64_t key = stack_key(st);
+  const HipBackend::Tuning* tk = be.tuned("multi_k_product", key);
+  const HipBackend::Tuning* tkt = be.tuned("multi_kt_product", key);
+  if (tk != nullptr && tkt != nullptr) {
+    apply(true, tk->chosen);
+    apply(false, tkt->chosen);
+    return;
+  }
+  if (opt.autotune == 0 || be.profiling() || ds->k.blocks <= 0 || ds->kt.blocks <= 0) return;
+  hipStream_t s = be.stream();
+  double* x = be.scratch(kMultiScratchSlot + 80, uz(st.n));
+  double* y = be.scratch(kMultiScratchSlot + 81, uz(st.m));
+  be.fill(x, st.n, 1.0);
+  be.fill(y, st.m, 1.0);
+  auto* params = reinterpret_cast<IterParams*>(
+      be.scratch(kMultiScratchSlot + 82, uz(st.lps) * kParamDoubles + 1));
+  const std::vector<char> all(uz(st.lps), 1);
+  const std::vector<IterParams> on = flags(all);
+  hip_check(hipMemcpyAsync(params, on.data(), on.size() * sizeof(IterParams), hipMemcpyHostToDevice, s),
+            "tune flags");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hip_check(hipEventCreate(&e0), "hipEventCreate(tune stack)");
+  hip_check(hipEventCreate(&e1), "hipEventCreate(tune stack)");
+  constexpr int kReps = 5, kTrials = 3;
+  auto side = [&](bool k_side) {
+    const DevCsr& a = k_side ? ds->k : ds->kt;
+    const Int* block_lp = k_side ? ds->k_block_lp : ds->kt_block_lp;
+    const std::vector<Int>& lp_blocks = k_side ? ds->k_lp_blocks_host : ds->kt_lp_blocks_host;
+    const std::vector<Int>& rank = k_side ? ds->k_rank : ds->kt_rank;
+    HipBackend::Tuning t{k_side ? "multi_k_product" : "multi_kt_product", a.rows, a.nnz, {},
+                         kStackDefault};
+    for (int binned : {0, 1}) {
+      const std::vector<Int> order = launch_blocks(lp_blocks, all, binned != 0, rank);
+      Int* map = reinterpret_cast<Int*>(be.scratch(kMultiScratchSlot + 83 + binned, order.size() / 2 + 1));
+      hip_check(hipMemcpyAsync(map, order.data(), order.size() * sizeof(Int), hipMemcpyHostToDevice, s),
+                "tune block map");
+      be.synchronize();  // the host order goes out of scope
+      for (int stream : {0, 1}) {
+        DevCsr run = a;
+        run.block_map = map;
+        run.blocks = static_cast<Int>(order.size());
+        run.stream_loads = stream != 0;
+        const double* in = k_side ? x : y;
+        double* out = k_side ? y : x;
+        launch_multi_product(s, run, block_lp, in, out, params);  // warm
+        double best = 1e300;
+        for (int trial = 0; trial < kTrials; ++trial) {
+          hip_check(hipEventRecord(e0, s), "hipEventRecord(tune stack)");
+          for (int r = 0; r < kReps; ++r) launch_multi_product(s, run, block_lp, in, out, params);
+          hip_check(hipEventRecord(e1, s), "hipEventRecord(tune stack)");
+          hip_check(hipEventSynchronize(e1), "hipEventSynchronize(tune stack)");
+          float ms = 0.0f;
+          hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime(tune stack)");
+          best = std::min(best, 1e3 * static_cast<double>(ms) / kReps);
+        }
+        t.us.push_back({kStackDefault + 10 * binned + stream, best});
+      }
+    }
+    double base = 0.0, fastest = 1e300;
+    int pick = kStackDefault;
+    for (const auto& c : t.us) {
+      if (c.first == kStackDefault) base = c.second;
+      if (c.second < fastest) {
+        fastest = c.second;
+        pick = c.first;
+      }
+    }
+    t.chosen = fastest < (1.0 - kStackMargin) * base ? pick : kStackDefault;
+    apply(k_side, t.chosen);
+    be.record_tuning(key, std::move(t));
+  };
+  side(true);
+  side(false);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
 // ||K_l|| for every LP by the power iteration on K_l^T K_l (pdhg.cpp:223-263),
 // all LPs in the same launches, each stopping on its own test.
 std::vector<double> stack_norms(HipBackend& be, const Stack& st, const DeviceStack& ds,
@@ -428,13 +589,8 @@ std::vector<double> stack_norms(HipBackend& be, const Stack& st, const DeviceSta
   auto compact = [&] {
     // Queued products may still read the previous maps.
     be.synchronize();
-    km.clear();
-    ktm.clear();
-    for (Int l = 0; l < L; ++l) {
-      if (!on[uz(l)]) continue;
-      for (Int b = ds.k_lp_blocks_host[uz(l)]; b < ds.k_lp_blocks_host[uz(l) + 1]; ++b) km.push_back(b);
-      for (Int b = ds.kt_lp_blocks_host[uz(l)]; b < ds.kt_lp_blocks_host[uz(l) + 1]; ++b) ktm.push_back(b);
-    }
+    km = launch_blocks(ds.k_lp_blocks_host, on, ds.k_binned, ds.k_rank);
+    ktm = launch_blocks(ds.kt_lp_blocks_host, on, ds.kt_binned, ds.kt_rank);
     auto put = [&](int slot, const std::vector<Int>& h) {
       Int* d = reinterpret_cast<Int*>(be.scratch(slot, h.size() / 2 + 1));
       if (!h.empty())
@@ -603,12 +759,9 @@ class Engine {
 };
 
 void Engine::compact() {
-  std::vector<Int> km, ktm;
-  for (Int l = 0; l < L_; ++l) {
-    if (!active_[uz(l)]) continue;
-    for (Int b = ds_.k_lp_blocks_host[uz(l)]; b < ds_.k_lp_blocks_host[uz(l) + 1]; ++b) km.push_back(b);
-    for (Int b = ds_.kt_lp_blocks_host[uz(l)]; b < ds_.kt_lp_blocks_host[uz(l) + 1]; ++b) ktm.push_back(b);
-  }
+  const std::vector<Int> km = launch_blocks(ds_.k_lp_blocks_host, active_, ds_.k_binned, ds_.k_rank);
+  const std::vector<Int> ktm =
+      launch_blocks(ds_.kt_lp_blocks_host, active_, ds_.kt_binned, ds_.kt_rank);
   auto put = [&](int slot, const std::vector<Int>& h) {
     Int* d = reinterpret_cast<Int*>(be_.scratch(slot, h.size() / 2 + 1));
     if (!h.empty())
@@ -1200,6 +1353,7 @@ std::vector<SolveResult> solve_pdhg_multi(HipBackend& be, const std::vector<cons
   DeviceStack ds(be);
   upload_stack(st, &ds);
   scale_stack(be, opt, &st, &ds);
+  tune_stack(be, st, &ds, opt);
   lap(&Phases::scaling);
   std::vector<double> norms(uz(st.lps), opt.known_matrix_norm);
   if (!(opt.known_matrix_norm > 0.0)) norms = stack_norms(be, st, ds, 200, 1e-8);

@@ -215,7 +215,9 @@ def cpu_baseline_threaded(base, cases, tol, budget_s):
 
 def parse_tuning(text):
     """Lines of Context.tuning_report(): kernel, rows, nnz, the mean launch
-    time of each shape (w<waves per workgroup>r<rows per wave>) and the shape
+    time of each shape (batched steps: w<waves per workgroup>r<rows per
+    wave>, "s" for the fixed-width row copy; stack products: plan or
+    balanced block order, "+nt" for non-temporal loads) and the shape
     chosen."""
     out = []
     for line in text.splitlines():
@@ -438,6 +440,8 @@ def c3_section(ctx, args):
             "iterations": {"p50": int(np.percentile([r.iterations for r in rs], 50)),
                            "max": int(max(r.iterations for r in rs))},
             "phases_s": {k: round(v, 4) for k, v in rs[0].phases.items()},
+            "autotune": [t for t in parse_tuning(ctx.tuning_report())
+                         if t["kernel"].startswith("multi_")],
             "plain_sample": {"lps": plain_n, "seconds": round(plain_s, 3),
                              "lps_per_s": round(plain_n / plain_s, 2) if plain_s > 0 else None,
                              "max_rel_objective_diff": worst}}

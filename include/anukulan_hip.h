@@ -348,6 +348,18 @@ typedef struct anuk_pdhg_options {
                                           kernels read K and K^T from their CSR; 2: from a
                                           fixed-width copy of each row's first entries
                                           (one row per wave only) */
+  int32_t multi_order;                 /* anuk_solve_pdhg_multi: 0: as autotune decides;
+                                          1: row blocks launched in the stack's order; 2:
+                                          binned (each XCD's share of the blocks run
+                                          heaviest first) */
+  int32_t multi_stream_loads;          /* anuk_solve_pdhg_multi: 0: as autotune decides;
+                                          1: plain loads of the matrices' index and value
+                                          streams; 2: non-temporal loads. With autotune
+                                          the solver times the four choices on the
+                                          stack's K and K^T once per stack pattern and
+                                          keeps the fastest when it beats the stack's
+                                          order with plain loads by 3%; every choice
+                                          computes the same bits */
 } anuk_pdhg_options;
 
 void anuk_pdhg_default_options(anuk_pdhg_options* options);

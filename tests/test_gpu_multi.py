@@ -59,6 +59,25 @@ def test_stack_reaches_the_reference_fixtures(ctx):
         assert abs(g.iterations - want["iterations"]) <= 0.02 * want["iterations"]
 
 
+def test_every_launch_choice_gives_the_same_bits(ctx):
+    # The stack tuner picks the row blocks' launch order (the stack's, or
+    # binned: each XCD's share run heaviest first) and the load kind of the
+    # index and value streams; a block's sums and each LP's reductions do
+    # not depend on either, so every choice and the tuned one give the same
+    # iterates to the bit.
+    lps = [m.to_lp() for m in mixed_models()]
+    want = solve_pdhg_multi(ctx, lps, tolerance=1e-6, multi_order=1, multi_stream_loads=1)
+    for order, stream in ((1, 2), (2, 1), (2, 2), (0, 0)):
+        got = solve_pdhg_multi(ctx, lps, tolerance=1e-6, multi_order=order,
+                               multi_stream_loads=stream)
+        for g, h in zip(got, want):
+            assert g.iterations == h.iterations, (order, stream)
+            assert np.array_equal(g.x, h.x) and np.array_equal(g.y, h.y), (order, stream)
+    report = ctx.tuning_report()
+    print("\n" + report)
+    assert "multi_k_product" in report and "multi_kt_product" in report
+
+
 def test_one_lp_stack_is_the_plain_solve(ctx):
     lp = G.refinery_model(4).to_lp()
     g = solve_pdhg_multi(ctx, [lp], tolerance=1e-8)[0]

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <iterator>
 #include <map>
 #include <memory>
@@ -11,6 +12,7 @@
 
 #include "batch_kernels.hpp"
 #include "control.hpp"
+#include "host_pool.hpp"
 
 namespace anuk {
 
@@ -202,18 +204,24 @@ void Engine::allocate() {
 // `width` zeroed), so the writes stream instead of striding across the
 // buffer.
 void interleave(double* out, Int rows, int count, int width, const std::vector<const double*>& src) {
-  for (Int r = 0; r < rows; ++r) {
-    double* line = out + static_cast<size_t>(r) * static_cast<size_t>(width);
-    for (int l = 0; l < count; ++l) line[l] = src[static_cast<size_t>(l)][r];
-    for (int l = count; l < width; ++l) line[l] = 0.0;
-  }
+  constexpr Int kRowsPerTask = 1024;
+  const Int tasks = (rows + kRowsPerTask - 1) / kRowsPerTask;
+  HostPool::shared().run(uz(tasks), [&](size_t t) {
+    const Int r1 = std::min<Int>(rows, static_cast<Int>(t + 1) * kRowsPerTask);
+    for (Int r = static_cast<Int>(t) * kRowsPerTask; r < r1; ++r) {
+      double* line = out + static_cast<size_t>(r) * static_cast<size_t>(width);
+      for (int l = 0; l < count; ++l) line[l] = src[static_cast<size_t>(l)][r];
+      for (int l = count; l < width; ++l) line[l] = 0.0;
+    }
+  });
 }
 
 void Engine::upload_lanes() {
   // Starting points: a warm start is put in the dual cone and the primal box,
   // a cold one is the projection of zero (pdhg.cpp:479-503).
   std::vector<std::vector<double>> xs(static_cast<size_t>(count_)), ys(static_cast<size_t>(count_));
-  for (int l = 0; l < count_; ++l) {
+  HostPool::shared().run(static_cast<size_t>(count_), [&](size_t li) {
+    const int l = static_cast<int>(li);
     Lane& ln = lanes_[static_cast<size_t>(l)];
     const bool warm = !ln.warm_x.empty() && !ln.warm_y.empty();
     std::vector<double>& x = xs[static_cast<size_t>(l)];
@@ -234,7 +242,7 @@ void Engine::upload_lanes() {
       if (v > hi) v = std::isfinite(hi) ? hi : 0.0;
       x[uz(j)] = v;
     }
-  }
+  });
   // Every lane vector is interleaved straight into one pinned staging area
   // and copied asynchronously; one wait covers them all. Only the slots up to
   // the lane count rounded to 16 are staged and sent, as a strided copy into
@@ -481,7 +489,7 @@ std::vector<Polished> polish_lanes(HipBackend& be, const Csr& k, const Csr& kt, 
   sub.warm_y = nullptr;
   const size_t cnt = src.size();
   std::vector<Lane> primal(cnt), dual(cnt);
-  for (size_t s = 0; s < cnt; ++s) {
+  HostPool::shared().run(cnt, [&](size_t s) {
     Lane& p = primal[s];
     p.c.assign(uz(n), 0.0);
     p.q = src[s]->q;
@@ -508,7 +516,7 @@ std::vector<Polished> polish_lanes(HipBackend& be, const Csr& k, const Csr& kt, 
       t->olo = t->lower;
       t->ohi = t->upper;
     }
-  }
+  });
   const std::vector<double> ones_m(uz(m), 1.0), ones_n(uz(n), 1.0);
   std::vector<LaneOutcome> pr, dr;
   if (2 * cnt <= static_cast<size_t>(kLanes)) {
@@ -534,11 +542,11 @@ std::vector<Polished> polish_lanes(HipBackend& be, const Csr& k, const Csr& kt, 
     dr = run_engine(be, k, kt, neq, ones_m, ones_n, norm, sub, dual, start, nullptr);
   }
   std::vector<Polished> out(cnt);
-  for (size_t s = 0; s < cnt; ++s) {
+  HostPool::shared().run(cnt, [&](size_t s) {
     scaling.unscale_primal(pr[s].x, &out[s].x);
     scaling.unscale_dual(dr[s].y, &out[s].y);
     out[s].spent = pr[s].iterations + dr[s].iterations;
-  }
+  });
   return out;
 }
 
@@ -573,12 +581,17 @@ void Engine::run_polish(const std::vector<int>& who, Int budget, std::vector<Lan
   std::vector<Polished> pol = polish_lanes(be_, k_, kt_, neq_, norm_, opt_, src, sx, sy, omega,
                                            ctl_[static_cast<size_t>(who[0])].eta, budget,
                                            *polish_->scaling, start_);
+  std::vector<Residual> trs(static_cast<size_t>(cnt));
+  HostPool::shared().run(static_cast<size_t>(cnt), [&](size_t s) {
+    const int l = order[s];
+    trs[s] = evaluate_residual(*(*polish_->originals)[static_cast<size_t>(l)], pol[s].x, pol[s].y);
+  });
   for (int s = 0; s < cnt; ++s) {
     const int l = order[static_cast<size_t>(s)];
     LaneOutcome& o = out[static_cast<size_t>(l)];
     std::vector<double>& tx = pol[static_cast<size_t>(s)].x;
     std::vector<double>& ty = pol[static_cast<size_t>(s)].y;
-    const Residual tr = evaluate_residual(*(*polish_->originals)[static_cast<size_t>(l)], tx, ty);
+    const Residual& tr = trs[static_cast<size_t>(s)];
     ++o.polish_attempts;
     o.polish_iterations += pol[static_cast<size_t>(s)].spent;
     Control& c = ctl_[static_cast<size_t>(l)];
@@ -657,44 +670,54 @@ void Engine::tune() {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hip_check(hipEventCreate(&e0), "hipEventCreate(tune)");
   hip_check(hipEventCreate(&e1), "hipEventCreate(tune)");
+  // Each trial times every candidate once (kReps back-to-back launches), so
+  // the trials of one shape are spread over the whole tuning run; a shape's
+  // time is its best trial. (A shape timed in one burst had read 30% slow in
+  // some runs and fast in others.)
   constexpr int kReps = 20, kTrials = 5;
-  auto mean_us = [&](const auto& launch) {
-    double best = 1e300;
-    for (int t = 0; t < kTrials; ++t) {
-      hip_check(hipEventRecord(e0, s), "hipEventRecord(tune)");
-      for (int r = 0; r < kReps; ++r) launch();
-      hip_check(hipEventRecord(e1, s), "hipEventRecord(tune)");
-      hip_check(hipEventSynchronize(e1), "hipEventSynchronize(tune)");
-      float ms = 0.0f;
-      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime(tune)");
-      best = std::min(best, 1e3 * static_cast<double>(ms) / kReps);
-    }
-    return best;
-  };
   const PrimalBufs pb{X_[cur_], AX_, X_[1 - cur_], XBAR_, DX_, AX_};
   const DualBufs db{Y_[cur_], KX_[cur_], AY_, AKX_, Y_[1 - cur_], KX_[1 - cur_], DY_, KDX_, AY_, AKX_};
   HipBackend::Tuning p{"batch_primal_step", kt_.rows, kt_.nnz(), {}, kDefaultShape};
   HipBackend::Tuning d{"batch_dual_step", k_.rows, k_.nnz(), {}, kDefaultShape};
+  struct Candidate {
+    HipBackend::Tuning* side;
+    int shape;
+    std::function<void()> launch;
+  };
+  std::vector<Candidate> cands;
   for (int w : {1, 2, 4, 8}) {
     for (int r : {1, 2, 4}) {
-      p.us.push_back({w * 10 + r, mean_us([&] {
-                        launch_batch_primal_step(s, dkt_, Y_[cur_], data_, d_params_, 1, span_, pb,
-                                                 false, w, r);
-                      })});
-      d.us.push_back({w * 10 + r, mean_us([&] {
-                        launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, 1, span_, db,
-                                               false, w, r);
-                      })});
+      cands.push_back({&p, w * 10 + r, [&, w, r] {
+                         launch_batch_primal_step(s, dkt_, Y_[cur_], data_, d_params_, 1, span_,
+                                                  pb, false, w, r);
+                       }});
+      cands.push_back({&d, w * 10 + r, [&, w, r] {
+                         launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, 1, span_, db,
+                                                false, w, r);
+                       }});
     }
-    p.us.push_back({100 + w * 10 + 1, mean_us([&] {
-                      launch_batch_primal_step(s, dkts_, Y_[cur_], data_, d_params_, 1, span_, pb,
-                                               false, w, 1);
-                    })});
-    d.us.push_back({100 + w * 10 + 1, mean_us([&] {
-                      launch_batch_dual_step(s, dks_, XBAR_, data_, d_params_, 1, span_, db, false,
-                                             w, 1);
-                    })});
+    cands.push_back({&p, 100 + w * 10 + 1, [&, w] {
+                       launch_batch_primal_step(s, dkts_, Y_[cur_], data_, d_params_, 1, span_, pb,
+                                                false, w, 1);
+                     }});
+    cands.push_back({&d, 100 + w * 10 + 1, [&, w] {
+                       launch_batch_dual_step(s, dks_, XBAR_, data_, d_params_, 1, span_, db,
+                                              false, w, 1);
+                     }});
   }
+  std::vector<double> best(cands.size(), 1e300);
+  for (int t = 0; t < kTrials; ++t) {
+    for (size_t c = 0; c < cands.size(); ++c) {
+      hip_check(hipEventRecord(e0, s), "hipEventRecord(tune)");
+      for (int r = 0; r < kReps; ++r) cands[c].launch();
+      hip_check(hipEventRecord(e1, s), "hipEventRecord(tune)");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize(tune)");
+      float ms = 0.0f;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime(tune)");
+      best[c] = std::min(best[c], 1e3 * static_cast<double>(ms) / kReps);
+    }
+  }
+  for (size_t c = 0; c < cands.size(); ++c) cands[c].side->us.push_back({cands[c].shape, best[c]});
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   auto pick = [](const HipBackend::Tuning& t) {
@@ -969,7 +992,7 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
   }
   Scaling scaling;
   std::vector<Lane> lanes(lps.size());
-  for (size_t i = 0; i < lps.size(); ++i) {
+  HostPool::shared().run(lps.size(), [&](size_t i) {
     Lane& ln = lanes[i];
     ln.c = lps[i]->c;
     ln.q = lps[i]->q;
@@ -982,7 +1005,7 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
     ln.oq = lps[i]->q;
     ln.olo = lps[i]->lower;
     ln.ohi = lps[i]->upper;
-  }
+  });
   lap(&Phases::scaling);
   StandardLp shape;
   shape.k = k;
@@ -1030,6 +1053,15 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
   // Final answers, and one more polish for the LPs that ran out of
   // iterations (pdhg.cpp:1205-1269), batched by budget.
   std::map<Int, std::vector<size_t>> exit_polish;
+  // Each unpolished result unscaled and measured on its own LP, in parallel.
+  HostPool::shared().run(lps.size(), [&](size_t i) {
+    LaneOutcome& o = outcomes[i];
+    if (o.polished) return;
+    SolveResult& r = results[i];
+    scaling.unscale_primal(o.x, &r.x);
+    scaling.unscale_dual(o.y, &r.y);
+    r.residual = evaluate_residual(*lps[i], r.x, r.y);
+  });
   for (size_t i = 0; i < lps.size(); ++i) {
     LaneOutcome& o = outcomes[i];
     SolveResult& r = results[i];
@@ -1049,9 +1081,6 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
       r.residual = o.polished_r;
       r.polished = true;
     } else {
-      scaling.unscale_primal(o.x, &r.x);
-      scaling.unscale_dual(o.y, &r.y);
-      r.residual = evaluate_residual(*lps[i], r.x, r.y);
       const bool worth = opt.polish_feasibility && opt.polish_on_exit &&
                          opt.termination == ANUK_TERM_FULL && r.status != ANUK_PDHG_OPTIMAL &&
                          r.status != ANUK_PDHG_PRIMAL_INFEASIBLE &&

@@ -755,6 +755,14 @@ __global__ void scale_into_kernel(Int n, double w, const double* sum, double* ou
   if (i < n) out[i] = sum[i] / w;
 }
 
+// out = sum / sqrt(*sq), the divisor read on the device (the device's fp64
+// sqrt and division are correctly rounded, so the bits are those of
+// scale_into with the host's sqrt of the same value).
+__global__ void scale_by_root_kernel(Int n, const double* sq, const double* sum, double* out) {
+  const Int i = static_cast<Int>(blockIdx.x) * kThreads + static_cast<Int>(threadIdx.x);
+  if (i < n) out[i] = sum[i] / sqrt(*sq);
+}
+
 // The standalone fused updates (LinAlgBackend::primal_update/dual_update).
 // Every input is read into registers before any output is written, so x_out
 // may alias x and y_out / k_out may alias y / k_x.
@@ -989,6 +997,12 @@ void launch_scale_into(hipStream_t s, Int n, double w, const double* sum, double
   if (n <= 0) return;
   hipLaunchKernelGGL(scale_into_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, n, w, sum, out);
   check_launch("scale_into");
+}
+
+void launch_scale_by_root(hipStream_t s, Int n, const double* sq, const double* sum, double* out) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(scale_by_root_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, n, sq, sum, out);
+  check_launch("scale_by_root");
 }
 
 void launch_primal_update(hipStream_t s, Int n, double tau, double gamma, double h,

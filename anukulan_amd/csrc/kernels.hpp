@@ -209,6 +209,8 @@ void launch_dual_bounds(hipStream_t s, Int n, const double* lo, const double* hi
 void launch_cold_start(hipStream_t s, Int n, const double* lo, const double* hi, double* x);
 void launch_accumulate(hipStream_t s, Int n, double w, const double* v, double* sum);
 void launch_scale_into(hipStream_t s, Int n, double w, const double* sum, double* out);
+// out = sum / sqrt(*sq) with *sq read on the device.
+void launch_scale_by_root(hipStream_t s, Int n, const double* sq, const double* sum, double* out);
 void launch_primal_update(hipStream_t s, Int n, double tau, double gamma, double h,
                           const double* x, const double* c, const double* kt_y,
                           const double* lower, const double* upper, const double* anchor,

@@ -501,6 +501,10 @@ void tune_stack(HipBackend& be, const Stack& st, DeviceStack* ds, const anuk_pdh
     const std::vector<Int>& rank = k_side ? ds->k_rank : ds->kt_rank;
     HipBackend::Tuning t{k_side ? "multi_k_product" : "multi_kt_product", a.rows, a.nnz, {},
                          kStackDefault};
+    // The four candidates, each trial timing all of them in turn (a
+    // candidate's time is its best trial).
+    std::vector<DevCsr> runs;
+    std::vector<int> choices;
     for (int binned : {0, 1}) {
       const std::vector<Int> order = launch_blocks(lp_blocks, all, binned != 0, rank);
       Int* map = reinterpret_cast<Int*>(be.scratch(kMultiScratchSlot + 83 + binned, order.size() / 2 + 1));
@@ -512,22 +516,26 @@ void tune_stack(HipBackend& be, const Stack& st, DeviceStack* ds, const anuk_pdh
         run.block_map = map;
         run.blocks = static_cast<Int>(order.size());
         run.stream_loads = stream != 0;
-        const double* in = k_side ? x : y;
-        double* out = k_side ? y : x;
-        launch_multi_product(s, run, block_lp, in, out, params);  // warm
-        double best = 1e300;
-        for (int trial = 0; trial < kTrials; ++trial) {
-          hip_check(hipEventRecord(e0, s), "hipEventRecord(tune stack)");
-          for (int r = 0; r < kReps; ++r) launch_multi_product(s, run, block_lp, in, out, params);
-          hip_check(hipEventRecord(e1, s), "hipEventRecord(tune stack)");
-          hip_check(hipEventSynchronize(e1), "hipEventSynchronize(tune stack)");
-          float ms = 0.0f;
-          hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime(tune stack)");
-          best = std::min(best, 1e3 * static_cast<double>(ms) / kReps);
-        }
-        t.us.push_back({kStackDefault + 10 * binned + stream, best});
+        runs.push_back(run);
+        choices.push_back(kStackDefault + 10 * binned + stream);
       }
     }
+    const double* in = k_side ? x : y;
+    double* out = k_side ? y : x;
+    std::vector<double> best(runs.size(), 1e300);
+    for (const DevCsr& run : runs) launch_multi_product(s, run, block_lp, in, out, params);  // warm
+    for (int trial = 0; trial < kTrials; ++trial) {
+      for (size_t c = 0; c < runs.size(); ++c) {
+        hip_check(hipEventRecord(e0, s), "hipEventRecord(tune stack)");
+        for (int r = 0; r < kReps; ++r) launch_multi_product(s, runs[c], block_lp, in, out, params);
+        hip_check(hipEventRecord(e1, s), "hipEventRecord(tune stack)");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize(tune stack)");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime(tune stack)");
+        best[c] = std::min(best[c], 1e3 * static_cast<double>(ms) / kReps);
+      }
+    }
+    for (size_t c = 0; c < runs.size(); ++c) t.us.push_back({choices[c], best[c]});
     double base = 0.0, fastest = 1e300;
     int pick = kStackDefault;
     for (const auto& c : t.us) {

@@ -1026,12 +1026,21 @@ double estimate_matrix_norm(HipBackend& be, const StandardLp& lp, int iterations
   for (double& e : v) e /= norm;
   const DevCsr& k = be.prepare(view_of(lp.k));
   const DevCsr& kt = be.prepare(view_of(lp.kt));
+  // The iterations run in rounds of kRound without waiting on the host: each
+  // leaves its ||K^T K v||^2 in sq[it] (the reduction two_norm uses) and
+  // divides by its square root on the device. The host then replays the
+  // test of each iteration of the round in order, so the estimate and the
+  // iteration it stops at are those of the loop that waits every time; the
+  // iterations a round runs past the stop change nothing that is returned.
+  constexpr int kRound = 16;
   double* d_v = be.allocate(n);
   double* d_kv = be.allocate(m);
   double* d_ktkv = be.allocate(n);
+  double* d_sq = be.allocate(std::max(iterations, 1));
+  double* d_part = be.allocate(1024 * kMaxRed);
   struct Free {
     HipBackend& b;
-    double* p[3];
+    double* p[5];
     ~Free() {
       // A teardown error is dropped: a destructor must not throw.
       for (double* q : p) {
@@ -1041,18 +1050,27 @@ double estimate_matrix_norm(HipBackend& be, const StandardLp& lp, int iterations
         }
       }
     }
-  } guard{be, {d_v, d_kv, d_ktkv}};
+  } guard{be, {d_v, d_kv, d_ktkv, d_sq, d_part}};
   be.upload(v.data(), d_v, n);
+  hipStream_t s = be.stream();
+  std::vector<double> sq(uz(std::max(iterations, 1)));
   double sigma = 0.0;
-  for (int it = 0; it < iterations; ++it) {
-    launch_spmv(be.stream(), k, d_v, d_kv);
-    launch_spmv(be.stream(), kt, d_kv, d_ktkv);
-    const double next = be.two_norm(d_ktkv, n);
-    if (next == 0.0) return 0.0;
-    be.scale_into(n, next, d_ktkv, d_v);
-    const double estimate = std::sqrt(next);
-    if (it > 0 && std::fabs(estimate - sigma) <= tolerance * estimate) return estimate;
-    sigma = estimate;
+  for (int it0 = 0; it0 < iterations; it0 += kRound) {
+    const int it1 = std::min(iterations, it0 + kRound);
+    for (int it = it0; it < it1; ++it) {
+      launch_spmv(s, k, d_v, d_kv);
+      launch_spmv(s, kt, d_kv, d_ktkv);
+      launch_dot(s, d_ktkv, d_ktkv, n, d_part, d_sq + it);
+      launch_scale_by_root(s, n, d_sq + it, d_ktkv, d_v);
+    }
+    be.download(d_sq + it0, sq.data() + it0, it1 - it0);
+    for (int it = it0; it < it1; ++it) {
+      const double next = std::sqrt(sq[uz(it)]);
+      if (next == 0.0) return 0.0;
+      const double estimate = std::sqrt(next);
+      if (it > 0 && std::fabs(estimate - sigma) <= tolerance * estimate) return estimate;
+      sigma = estimate;
+    }
   }
   return sigma;
 }

@@ -80,7 +80,7 @@ class PdhgOptions(ctypes.Structure):
         ("initial_primal_weight", c_double), ("verbose", c_int32), ("log_frequency", c_int32),
         ("stop", IP), ("use_graphs", c_int32), ("product_order", c_int32), ("autotune", c_int32),
         ("batch_waves", c_int32), ("batch_rows", c_int32), ("batch_layout", c_int32),
-        ("multi_order", c_int32), ("multi_stream_loads", c_int32),
+        ("batch_pack", c_int32), ("multi_order", c_int32), ("multi_stream_loads", c_int32),
     ]
 
 

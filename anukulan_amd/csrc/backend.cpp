@@ -278,6 +278,7 @@ const DevCsr& HipBackend::prepare(const CsrView& a) {
   dm.view.cols = a.cols;
   dm.view.nnz = nnz;
   dm.view.blocks = blocks;
+  for (Int r = 0; r < a.rows; ++r) dm.view.longest = std::max(dm.view.longest, a.start[r + 1] - a.start[r]);
   dm.view.start = static_cast<const Int*>(put(a.start, (static_cast<size_t>(a.rows) + 1) * sizeof(Int)));
   dm.view.index = static_cast<const Int*>(put(a.index, static_cast<size_t>(nnz) * sizeof(Int)));
   dm.view.value = static_cast<const double*>(put(a.value, static_cast<size_t>(nnz) * sizeof(double)));

@@ -132,6 +132,8 @@ class Engine {
   int pw_ = 2, dw_ = 2;
   int pr_ = 1, dr_ = 1;
   bool ps_ = false, ds_ = false;
+  // Narrow spans (32 live slots or fewer) run the packed step kernels.
+  bool pack_ = true;
 
   double* slab_ = nullptr;
   double *X_[2], *Y_[2], *KX_[2];
@@ -353,13 +355,13 @@ void Engine::enqueue_period(int steps, bool has_check, int parity) {
         be_.launch_events(diag ? "batch_primal_step_check" : "batch_primal_step",
                           12.0 * nnz + 4.0 * (n + 1) + 8.0 * lanes * (m + (diag ? 8.0 : 7.0) * n));
     launch_batch_primal_step(s, ps_ ? dkts_ : dkt_, Y_[p], data_, d_params_, step, span_, pb, diag,
-                             pw_, pr_, &pe);
+                             pw_, pr_, pack_, &pe);
     DualBufs db{Y_[p], KX_[p], AY_, AKX_, Y_[1 - p], KX_[1 - p], DY_, KDX_, AY_, AKX_};
     const LaunchEvents de =
         be_.launch_events(diag ? "batch_dual_step_check" : "batch_dual_step",
                           12.0 * nnz + 4.0 * (m + 1) + 8.0 * lanes * (n + (diag ? 9.0 : 7.0) * m));
     launch_batch_dual_step(s, ds_ ? dks_ : dk_, XBAR_, data_, d_params_, step, span_, db, diag, dw_,
-                           dr_, &de);
+                           dr_, pack_, &de);
     p = 1 - p;
   }
   if (has_check) {
@@ -808,6 +810,7 @@ std::vector<LaneOutcome> Engine::run() {
   }
   upload_lanes();
   active_ = (count_ == 64) ? ~0ull : ((1ull << count_) - 1ull);
+  pack_ = opt_.batch_pack == 0;
   tune();
   launch_batch_spmm(be_.stream(), dk_, X_[0], KX_[0], nullptr, nullptr, nullptr, active_, dw_);
   cur_ = 0;

@@ -101,6 +101,59 @@ __device__ __forceinline__ double step_finish(const Int* __restrict__ s0,
   }
 }
 
+// The primal step's epilogue for one (row, lane): the projected step,
+// reflection and Halpern blend of backend.cpp:60-85, in the reference's
+// operation order.
+template <bool kDiag>
+__device__ __forceinline__ void primal_epilogue(const LaneParams& P, int step, size_t o, double xj,
+                                                double cj, double lo, double hi, double anc,
+                                                double kty, const PrimalBufs& b) {
+  const LaneScalars sc = lane_scalars(P, step);
+  double v = xj - sc.tau * (cj - kty);
+  if (v < lo) v = lo;
+  if (v > hi) v = hi;
+  const double dd = v - xj;
+  stream_out(b.x_bar + o, v + dd);
+  double next = v;
+  if (sc.gamma != 0.0) next += sc.gamma * dd;
+  if (sc.blend) next = sc.w * next + (1.0 - sc.w) * anc;
+  stream_out(b.x_out + o, next);
+  if (sc.open_epoch) stream_out(b.anchor_out + o, next);
+  if constexpr (kDiag) stream_out(b.dx + o, dd);
+}
+
+// The dual step's epilogue for one (row, lane) (backend.cpp:87-111).
+template <bool kDiag>
+__device__ __forceinline__ void dual_epilogue(const LaneParams& P, int step, Int i, Int neq,
+                                              size_t o, double yi, double kx, double qi, double ay,
+                                              double ak, double kxb, const DualBufs& b) {
+  const LaneScalars sc = lane_scalars(P, step);
+  double v = yi + sc.sigma * (qi - kxb);
+  if (i >= neq && v < 0.0) v = 0.0;
+  const double dd = v - yi;
+  const double kd = 0.5 * (kxb - kx);
+  double ny = v;
+  double nk = 0.5 * (kxb + kx);
+  if (sc.gamma != 0.0) {
+    ny += sc.gamma * dd;
+    nk += sc.gamma * kd;
+  }
+  if (sc.blend) {
+    ny = sc.w * ny + (1.0 - sc.w) * ay;
+    nk = sc.w * nk + (1.0 - sc.w) * ak;
+  }
+  stream_out(b.y_out + o, ny);
+  stream_out(b.k_out + o, nk);
+  if (sc.open_epoch) {
+    stream_out(b.anchor_y_out + o, ny);
+    stream_out(b.anchor_kx_out + o, nk);
+  }
+  if constexpr (kDiag) {
+    stream_out(b.dy + o, dd);
+    stream_out(b.k_dx + o, kd);
+  }
+}
+
 // The register budget of a step kernel: a wide fixed-width row keeps all its
 // gathers and the epilogue's operands in flight, which needs more than the
 // 64 VGPRs of eight waves per SIMD; four leave 128.
@@ -125,18 +178,7 @@ __global__ void __launch_bounds__(64 * kWaves)
   const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
   const double kty = step_finish<kW>(s0, s1, s2, kt_index, kt_value, j, y, lane, g);
   if (!P.active) return;
-  const LaneScalars sc = lane_scalars(P, step);
-  double v = xj - sc.tau * (cj - kty);
-  if (v < lo) v = lo;
-  if (v > hi) v = hi;
-  const double dd = v - xj;
-  stream_out(b.x_bar + o, v + dd);
-  double next = v;
-  if (sc.gamma != 0.0) next += sc.gamma * dd;
-  if (sc.blend) next = sc.w * next + (1.0 - sc.w) * anc;
-  stream_out(b.x_out + o, next);
-  if (sc.open_epoch) stream_out(b.anchor_out + o, next);
-  if constexpr (kDiag) stream_out(b.dx + o, dd);
+  primal_epilogue<kDiag>(P, step, o, xj, cj, lo, hi, anc, kty, b);
 }
 
 template <bool kDiag, int kWaves, int kW>
@@ -157,31 +199,87 @@ __global__ void __launch_bounds__(64 * kWaves)
   const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
   const double kxb = step_finish<kW>(s0, s1, s2, k_index, k_value, i, x_bar, lane, g);
   if (!P.active) return;
-  const LaneScalars sc = lane_scalars(P, step);
-  double v = yi + sc.sigma * (qi - kxb);
-  if (i >= d.num_equalities && v < 0.0) v = 0.0;
-  const double dd = v - yi;
-  const double kd = 0.5 * (kxb - kx);
-  double ny = v;
-  double nk = 0.5 * (kxb + kx);
-  if (sc.gamma != 0.0) {
-    ny += sc.gamma * dd;
-    nk += sc.gamma * kd;
+  dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
+}
+
+// Packed steps for a narrow span: with every live lane in the lowest
+// kSpan = 64 / kG slots (compaction keeps them there), one wave takes kG
+// consecutive rows, lane (r, s) working on row j0 + r for slot s. Each lane
+// sums its row's entries in storage order and runs the one-row kernel's
+// epilogue, so the bits are the same; the launch has kG times fewer waves,
+// and it is the per-wave cost that a step pays once few lanes are live. As
+// the rows of a wave differ, the row structure comes in vector loads. A
+// round takes eight entries per lane with no branch: a position past the
+// lane's row reads the row's first entry instead (position 0 for an empty
+// row; the matrix has entries), so all the index and value loads go out
+// together and then all the gathers; the extra term adds +0.0, which leaves
+// a sum that started at +0.0 unchanged, so the sum has the serial loop's
+// bits.
+__device__ __forceinline__ double packed_row_sum(const Int* __restrict__ start,
+                                                 const Int* __restrict__ index,
+                                                 const double* __restrict__ value, Int row,
+                                                 const double* __restrict__ v, int slot) {
+  const Int lo = start[row], len = start[row + 1] - lo;
+  const Int safe = len > 0 ? lo : 0;
+  double s = 0.0;
+  for (Int u0 = 0; __any(u0 < len); u0 += 8) {
+    Int p[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) p[u] = u0 + u < len ? lo + u0 + u : safe;
+    Int ix[8];
+    double w[8], x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      ix[u] = index[p[u]];
+      w[u] = value[p[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = v[at(ix[u], slot)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double t = w[u] * x[u];
+      s += u0 + u < len ? t : 0.0;
+    }
   }
-  if (sc.blend) {
-    ny = sc.w * ny + (1.0 - sc.w) * ay;
-    nk = sc.w * nk + (1.0 - sc.w) * ak;
-  }
-  stream_out(b.y_out + o, ny);
-  stream_out(b.k_out + o, nk);
-  if (sc.open_epoch) {
-    stream_out(b.anchor_y_out + o, ny);
-    stream_out(b.anchor_kx_out + o, nk);
-  }
-  if constexpr (kDiag) {
-    stream_out(b.dy + o, dd);
-    stream_out(b.k_dx + o, kd);
-  }
+  return s;
+}
+
+template <bool kDiag, int kWaves, int kG>
+__global__ void __launch_bounds__(64 * kWaves)
+    bprimal_packed_kernel(const Int* __restrict__ kt_start, const Int* __restrict__ kt_index,
+                          const double* __restrict__ kt_value, const double* __restrict__ y,
+                          Int rows, int blocks, int step, const LaneParams* __restrict__ params,
+                          BatchData d, PrimalBufs b) {
+  constexpr int kSpan = 64 / kG;
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int slot = lane % kSpan;
+  const Int j = lanes::wave_row_of<kWaves>(blocks) * kG + lane / kSpan;
+  if (j >= rows) return;
+  const size_t o = at(j, slot);
+  const LaneParams P = params[slot];
+  const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
+  const double kty = packed_row_sum(kt_start, kt_index, kt_value, j, y, slot);
+  if (!P.active) return;
+  primal_epilogue<kDiag>(P, step, o, xj, cj, lo, hi, anc, kty, b);
+}
+
+template <bool kDiag, int kWaves, int kG>
+__global__ void __launch_bounds__(64 * kWaves)
+    bdual_packed_kernel(const Int* __restrict__ k_start, const Int* __restrict__ k_index,
+                        const double* __restrict__ k_value, const double* __restrict__ x_bar,
+                        Int rows, int blocks, int step, const LaneParams* __restrict__ params,
+                        BatchData d, DualBufs b) {
+  constexpr int kSpan = 64 / kG;
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int slot = lane % kSpan;
+  const Int i = lanes::wave_row_of<kWaves>(blocks) * kG + lane / kSpan;
+  if (i >= rows) return;
+  const size_t o = at(i, slot);
+  const LaneParams P = params[slot];
+  const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
+  const double kxb = packed_row_sum(k_start, k_index, k_value, i, x_bar, slot);
+  if (!P.active) return;
+  dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
 }
 
 // The same steps with kRows consecutive rows per wave, their nonzeros taken
@@ -670,12 +768,48 @@ auto dual_rows_by_waves(int waves) {
   }
 }
 
+// The packed instance for (waves, rows per wave kG).
+template <bool kDiag, int kG>
+auto primal_packed_by_waves(int waves) {
+  switch (waves) {
+    case 1: return &bprimal_packed_kernel<kDiag, 1, kG>;
+    case 2: return &bprimal_packed_kernel<kDiag, 2, kG>;
+    case 8: return &bprimal_packed_kernel<kDiag, 8, kG>;
+    default: return &bprimal_packed_kernel<kDiag, 4, kG>;
+  }
+}
+template <bool kDiag, int kG>
+auto dual_packed_by_waves(int waves) {
+  switch (waves) {
+    case 1: return &bdual_packed_kernel<kDiag, 1, kG>;
+    case 2: return &bdual_packed_kernel<kDiag, 2, kG>;
+    case 8: return &bdual_packed_kernel<kDiag, 8, kG>;
+    default: return &bdual_packed_kernel<kDiag, 4, kG>;
+  }
+}
+// Rows per wave of the packed steps for a span (0: the span is too wide, or
+// a row is longer than one round of the packed sum: a wave's rows wait for
+// its longest, which was measured to cost more than the waves saved on K of
+// the refinery stacks, whose rows run to 30 entries).
+int packed_rows(const DevCsr& a, Int span) {
+  if (a.longest > 8) return 0;
+  return span <= 16 ? 4 : span <= 32 ? 2 : 0;
+}
+
 template <bool kDiag>
 void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const BatchData& d,
                        const LaneParams* params, int step, Int span, const PrimalBufs& b,
-                       int waves, int rows, const LaunchEvents* ev) {
+                       int waves, int rows, bool pack, const LaunchEvents* ev) {
   const char* what = "batch_primal_step";
-  if (rows == 4) {
+  const int g = pack && rows == 1 && kt.slot_width == 0 ? packed_rows(kt, span) : 0;
+  if (g > 0) {
+    const Int units = (kt.rows + g - 1) / g;
+    const int blocks = static_cast<int>((units + waves - 1) / waves);
+    launch_rows_per(g == 4 ? primal_packed_by_waves<kDiag, 4>(waves)
+                           : primal_packed_by_waves<kDiag, 2>(waves),
+                    waves, g, kt.rows, s, ev, what, kt.start, kt.index, kt.value, y, kt.rows, blocks,
+                    step, params, d, b);
+  } else if (rows == 4) {
     launch_rows_per(primal_rows_by_waves<kDiag, 4>(waves), waves, 4, kt.rows, s, ev, what, kt, y,
                     d, b, params, step, span);
   } else if (rows == 2) {
@@ -692,9 +826,17 @@ void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const B
 template <bool kDiag>
 void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const BatchData& d,
                      const LaneParams* params, int step, Int span, const DualBufs& b, int waves,
-                     int rows, const LaunchEvents* ev) {
+                     int rows, bool pack, const LaunchEvents* ev) {
   const char* what = "batch_dual_step";
-  if (rows == 4) {
+  const int g = pack && rows == 1 && k.slot_width == 0 ? packed_rows(k, span) : 0;
+  if (g > 0) {
+    const Int units = (k.rows + g - 1) / g;
+    const int blocks = static_cast<int>((units + waves - 1) / waves);
+    launch_rows_per(g == 4 ? dual_packed_by_waves<kDiag, 4>(waves)
+                           : dual_packed_by_waves<kDiag, 2>(waves),
+                    waves, g, k.rows, s, ev, what, k.start, k.index, k.value, x_bar, k.rows, blocks,
+                    step, params, d, b);
+  } else if (rows == 4) {
     launch_rows_per(dual_rows_by_waves<kDiag, 4>(waves), waves, 4, k.rows, s, ev, what, k, x_bar,
                     d, b, params, step, span);
   } else if (rows == 2) {
@@ -711,23 +853,23 @@ void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const 
 
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                               const BatchData& d, const LaneParams* params, int step, Int span,
-                              const PrimalBufs& b, bool diag, int waves, int rows,
+                              const PrimalBufs& b, bool diag, int waves, int rows, bool pack,
                               const LaunchEvents* ev) {
   if (diag) {
-    primal_step_shape<true>(s, kt, y, d, params, step, span, b, waves, rows, ev);
+    primal_step_shape<true>(s, kt, y, d, params, step, span, b, waves, rows, pack, ev);
   } else {
-    primal_step_shape<false>(s, kt, y, d, params, step, span, b, waves, rows, ev);
+    primal_step_shape<false>(s, kt, y, d, params, step, span, b, waves, rows, pack, ev);
   }
 }
 
 void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                             const BatchData& d, const LaneParams* params, int step, Int span,
-                            const DualBufs& b, bool diag, int waves, int rows,
+                            const DualBufs& b, bool diag, int waves, int rows, bool pack,
                             const LaunchEvents* ev) {
   if (diag) {
-    dual_step_shape<true>(s, k, x_bar, d, params, step, span, b, waves, rows, ev);
+    dual_step_shape<true>(s, k, x_bar, d, params, step, span, b, waves, rows, pack, ev);
   } else {
-    dual_step_shape<false>(s, k, x_bar, d, params, step, span, b, waves, rows, ev);
+    dual_step_shape<false>(s, k, x_bar, d, params, step, span, b, waves, rows, pack, ev);
   }
 }
 

@@ -64,15 +64,18 @@ inline constexpr int kLaneCheckStride = 16;
 // `waves`: waves per workgroup, 1, 2, 4 or 8; `rows`: consecutive rows per
 // wave, 1, 2 or 4. With one row per wave the kernels read the matrix's
 // fixed-width row copy when it carries one (DevCsr::slot_width > 0), else
-// its CSR. Every shape computes the same bits.
+// its CSR; from the CSR, with `pack`, a span of 32 slots or fewer and no row
+// longer than 8 entries, one wave takes 64 / 32 = 2 or 64 / 16 = 4 rows, a
+// lane per (row, slot).
+// Every shape computes the same bits.
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                               const BatchData& d, const LaneParams* params, int step, Int span,
                               const PrimalBufs& b, bool diag, int waves = 4, int rows = 1,
-                              const LaunchEvents* ev = nullptr);
+                              bool pack = false, const LaunchEvents* ev = nullptr);
 void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                             const BatchData& d, const LaneParams* params, int step, Int span,
                             const DualBufs& b, bool diag, int waves = 4, int rows = 1,
-                            const LaunchEvents* ev = nullptr);
+                            bool pack = false, const LaunchEvents* ev = nullptr);
 // out1 = A v1 (and out2 = A v2 when v2 is non-null) for the active lanes:
 // those whose LaneParams say so when `params` is given (read at run time, so
 // a captured graph follows retirements), else those set in `mask`. Other

@@ -551,6 +551,7 @@ anuk_pdhg_options checked_options(const anuk_pdhg_options* options) {
        "batch_layout must be 0, 1 or 2");
   need(o.batch_layout != 2 || o.batch_rows == 0 || o.batch_rows == 1,
        "batch_layout 2 (the fixed-width row copy) runs one row per wave");
+  need(o.batch_pack == 0 || o.batch_pack == 1, "batch_pack must be 0 or 1");
   need(o.multi_order == 0 || o.multi_order == 1 || o.multi_order == 2,
        "multi_order must be 0, 1 or 2");
   need(o.multi_stream_loads == 0 || o.multi_stream_loads == 1 || o.multi_stream_loads == 2,

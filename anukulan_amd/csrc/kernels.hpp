@@ -56,6 +56,7 @@ struct DevCsr {
   const double* value = nullptr;
   const Int* block_row = nullptr;
   Int max_rows = kTileRows;  // most rows in one block of this plan
+  Int longest = 0;           // nonzeros in the longest row
   // Optional compaction: launched block i runs planned block block_map[i]
   // (`blocks` then counts the launched ones). Null: every planned block.
   const Int* block_map = nullptr;

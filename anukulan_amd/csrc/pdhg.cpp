@@ -1122,6 +1122,7 @@ void default_options(anuk_pdhg_options* o) {
   o->batch_waves = 0;
   o->batch_rows = 0;
   o->batch_layout = 0;
+  o->batch_pack = 0;
   o->multi_order = 0;
   o->multi_stream_loads = 0;
 }

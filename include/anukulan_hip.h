@@ -348,6 +348,11 @@ typedef struct anuk_pdhg_options {
                                           kernels read K and K^T from their CSR; 2: from a
                                           fixed-width copy of each row's first entries
                                           (one row per wave only) */
+  int32_t batch_pack;                  /* 0: once 32 slots or fewer are live, a batched
+                                          step kernel over a matrix whose rows hold 8
+                                          entries or fewer takes 2 or 4 rows per wave (a
+                                          lane per row and slot); 1: one row per wave
+                                          always. The bits are the same */
   int32_t multi_order;                 /* anuk_solve_pdhg_multi: 0: as autotune decides;
                                           1: row blocks launched in the stack's order; 2:
                                           binned (each XCD's share of the blocks run

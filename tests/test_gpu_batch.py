@@ -142,12 +142,15 @@ def test_every_launch_shape_gives_the_same_bits(ctx, periods):
     # The tuner picks among waves per workgroup (1, 2, 4, 8), consecutive
     # rows per wave (1, 2, 4) and the matrix's layout (its CSR, or the
     # fixed-width copy of each row's first entries: 8 wide for K^T here, 32
-    # for K); a lane's arithmetic must not depend on the shape, so every
-    # shape and the tuned choice give the same iterates to the bit.
+    # for K), and once the live lanes fit 32 or 16 slots the one-row CSR
+    # kernels pack 2 or 4 rows into a wave; a lane's arithmetic must not
+    # depend on the shape, so every shape, packed or not, and the tuned
+    # choice give the same iterates to the bit. (Eight cases of the stack
+    # retire at different iterations, so the runs pass through spans of 16.)
     base, cases, models = stack(periods, 8)
     lps = [m.to_lp() for m in models]
     want = solve_pdhg_batch(ctx, lps, tolerance=1e-8, batch_waves=4, batch_rows=1,
-                            batch_layout=1)
+                            batch_layout=1, batch_pack=1)
     for w, r, s in ((1, 1, 1), (2, 1, 1), (8, 1, 1), (1, 2, 1), (4, 2, 1), (8, 2, 1), (1, 4, 1),
                     (2, 4, 1), (4, 4, 1), (8, 4, 1), (1, 1, 2), (2, 1, 2), (4, 1, 2), (8, 1, 2),
                     (0, 0, 0)):
@@ -172,6 +175,23 @@ def test_fixed_width_layout_with_rows_past_its_width(ctx):
     lps = [m.to_lp() for m in models]
     want = solve_pdhg_batch(ctx, lps, tolerance=1e-6, batch_waves=2, batch_rows=1, batch_layout=1)
     got = solve_pdhg_batch(ctx, lps, tolerance=1e-6, batch_waves=2, batch_rows=1, batch_layout=2)
+    assert all(h.status == "optimal" for h in want)
+    for g, h in zip(got, want):
+        assert g.status == h.status and g.iterations == h.iterations
+        assert np.array_equal(g.x, h.x) and np.array_equal(g.y, h.y)
+
+
+def test_packed_dual_step_gives_the_same_bits(ctx):
+    # A random LP whose rows of K hold at most 6 entries (K^T up to 59): with
+    # six cases the dual step runs packed, four rows to a wave, and must
+    # give the iterates of one row per wave to the bit. (The refinery
+    # stacks pack the primal side: rows of K^T hold at most 8 entries.)
+    base = G.random_lp(2000, 800, mean_col_nnz=3.0, seed=5)
+    models = [G.NaturalModel(**{**base.__dict__, "objective": base.objective * (1.0 + 0.01 * k)})
+              for k in range(6)]
+    lps = [m.to_lp() for m in models]
+    want = solve_pdhg_batch(ctx, lps, tolerance=1e-6, batch_waves=2, batch_rows=1, batch_pack=1)
+    got = solve_pdhg_batch(ctx, lps, tolerance=1e-6, batch_waves=2, batch_rows=1, batch_pack=0)
     assert all(h.status == "optimal" for h in want)
     for g, h in zip(got, want):
         assert g.status == h.status and g.iterations == h.iterations

@@ -201,7 +201,7 @@ struct PlainEpi {
   double* y;
   __device__ PlainEpi resolve(int) const { return *this; }
   __device__ bool skip() const { return false; }
-  __device__ void operator()(Int r, double s, double*) const { stream_out(y + r, s); }
+  __device__ void operator()(Int r, double s, double*) const { y[r] = s; }
 };
 
 // Where a fused epilogue finds its step scalars: one IterParams for the
@@ -278,14 +278,14 @@ struct PrimalStepEpi {
     if (v < p.lower[j]) v = p.lower[j];
     if (v > p.upper[j]) v = p.upper[j];
     const double d = v - xj;
-    stream_out(b.x_bar + j, v + d);
+    b.x_bar[j] = v + d;
     double next = v;
     if (sc.gamma != 0.0) next += sc.gamma * d;
     if (sc.blend) next = sc.w * next + (1.0 - sc.w) * b.anchor[j];
-    stream_out(b.x_out + j, next);
-    if (sc.open_epoch) stream_out(b.anchor_out + j, next);
+    b.x_out[j] = next;
+    if (sc.open_epoch) b.anchor_out[j] = next;
     if constexpr (kDiag) {
-      stream_out(b.dx + j, d);
+      b.dx[j] = d;
       red[0] += d * d;
     }
   }
@@ -330,15 +330,15 @@ struct DualStepEpi {
       ny = sc.w * ny + (1.0 - sc.w) * b.anchor_y[i];
       nk = sc.w * nk + (1.0 - sc.w) * b.anchor_kx[i];
     }
-    stream_out(b.y_out + i, ny);
-    stream_out(b.k_out + i, nk);
+    b.y_out[i] = ny;
+    b.k_out[i] = nk;
     if (sc.open_epoch) {
-      stream_out(b.anchor_y_out + i, ny);
-      stream_out(b.anchor_kx_out + i, nk);
+      b.anchor_y_out[i] = ny;
+      b.anchor_kx_out[i] = nk;
     }
     if constexpr (kDiag) {
-      stream_out(b.dy + i, d);
-      stream_out(b.k_dx + i, kd);
+      b.dy[i] = d;
+      b.k_dx[i] = kd;
       red[0] += d * d;
       const double scale = p.row_scale[i];
       const double row = scale != 0.0 ? nk / scale : nk;
@@ -553,7 +553,7 @@ struct MultiPlainEpi {
     return e;
   }
   __device__ bool skip() const { return off; }
-  __device__ void operator()(Int r, double s, double*) const { stream_out(y + r, s); }
+  __device__ void operator()(Int r, double s, double*) const { y[r] = s; }
 };
 
 // One thread per (LP, value): folds the LP's block partials in block order.

@@ -756,7 +756,7 @@ def main():
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(k["gbs"] / HBM_PEAK_GBS, 4),
                         "traffic": measured_traffic(name),
-                        "traffic_source": "profiles/pmc_traffic.txt (rocprofv3 --pmc, calibrated)",
+                        "traffic_source": "profiles/pmc_traffic.txt (rocprofv3 --pmc: size-resolved L2-to-memory read requests plus WRITE_SIZE)",
                         "mean_launch_us": k["mean_us"], "launches_timed": k["timed"]}
 
     c0 = None if args.no_c0 else c0_section(ctx, args)

@@ -297,6 +297,7 @@ namespace {
 constexpr std::uint64_t kPanelKey = 1ull << 63;
 constexpr std::uint64_t kFastKey = 1ull << 62;
 constexpr std::uint64_t kSlotKey = 1ull << 61;
+constexpr std::uint64_t kBinKey = 1ull << 60;
 }  // namespace
 
 void HipBackend::replan_fast(const CsrView& a, DeviceMatrix* dm) {
@@ -376,6 +377,45 @@ const DevCsr& HipBackend::prepare_slots(const CsrView& a) {
   dm.view.slot_row = static_cast<const Int*>(put(row.data(), row.size() * sizeof(Int)));
   dm.view.slot_index = static_cast<const Int*>(put(index.data(), index.size() * sizeof(Int)));
   dm.view.slot_value = static_cast<const double*>(put(value.data(), value.size() * sizeof(double)));
+  synchronize();
+  auto ins = matrices_.emplace(key, std::move(dm));
+  return ins.first->second.view;
+}
+
+const DevCsr& HipBackend::prepare_bins(const CsrView& a) {
+  const std::uint64_t key = a.id ^ kBinKey;
+  auto it = matrices_.find(key);
+  if (it != matrices_.end()) return it->second.view;
+  DeviceMatrix dm;
+  dm.view = prepare(a);  // the CSR arrays stay owned by the matrix's own entry
+  std::vector<Int> short_rows, long_rows;
+  for (Int r = 0; r < a.rows; ++r)
+    (a.start[r + 1] - a.start[r] <= kBinShortMax ? short_rows : long_rows).push_back(r);
+  auto entry = [&](std::vector<Int>* plan, Int r) {
+    if (r < 0) {
+      plan->insert(plan->end(), {-1, 0, 0, 0});
+    } else {
+      plan->insert(plan->end(), {r, a.start[r], a.start[r + 1] - a.start[r], 0});
+    }
+  };
+  for (int p = 0; p < 2; ++p) {
+    const Int g = p == 0 ? 2 : 4;
+    const Int waves = (static_cast<Int>(short_rows.size()) + g - 1) / g;
+    std::vector<Int> plan;
+    plan.reserve(4 * (uz(waves * g) + long_rows.size()));
+    for (Int e = 0; e < waves * g; ++e)
+      entry(&plan, e < static_cast<Int>(short_rows.size()) ? short_rows[uz(e)] : -1);
+    for (Int r : long_rows) entry(&plan, r);
+    void* d = nullptr;
+    hip_check(hipMalloc(&d, std::max<size_t>(plan.size(), 4) * sizeof(Int)), "hipMalloc(bins)");
+    dm.owned.push_back(d);
+    if (!plan.empty())
+      hip_check(hipMemcpyAsync(d, plan.data(), plan.size() * sizeof(Int), hipMemcpyHostToDevice, stream_),
+                "upload bins");
+    dm.view.bin_plan[p] = static_cast<const Int*>(d);
+    dm.view.bin_short[p] = waves;
+  }
+  dm.view.bin_long = static_cast<Int>(long_rows.size());
   synchronize();
   auto ins = matrices_.emplace(key, std::move(dm));
   return ins.first->second.view;
@@ -481,7 +521,7 @@ void HipBackend::release() {
 
 void HipBackend::release_one(std::uint64_t id) {
   // The fast plan and the slot copy borrow the matrix's arrays; they go first.
-  for (const std::uint64_t key : {id ^ kFastKey, id ^ kSlotKey, id ^ kPanelKey, id}) {
+  for (const std::uint64_t key : {id ^ kFastKey, id ^ kSlotKey, id ^ kBinKey, id ^ kPanelKey, id}) {
     auto it = matrices_.find(key);
     if (it == matrices_.end()) continue;
     synchronize();

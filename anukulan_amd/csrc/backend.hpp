@@ -66,6 +66,9 @@ class HipBackend {
   // `a` with its fixed-width copy (DevCsr::slot_*) for the one-wave-per-row
   // batch kernels, sharing the arrays of prepare(a); cached with `a`.
   const DevCsr& prepare_slots(const CsrView& a);
+  // `a` with its row bins (DevCsr::bin_*) for the packed batched steps,
+  // sharing the arrays of prepare(a); cached with `a`.
+  const DevCsr& prepare_bins(const CsrView& a);
   // The column-panel stack of `a` (kPanels panels balanced by nonzeros, see
   // kernels.hpp), built on the host, planned for the fast order and cached
   // with `a`.

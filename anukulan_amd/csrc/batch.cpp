@@ -788,8 +788,8 @@ std::vector<LaneOutcome> Engine::run() {
   if (count_ <= 0 || count_ > kLanes) throw std::invalid_argument("a batch holds 1 to 64 LPs");
   n_ = k_.cols;
   m_ = k_.rows;
-  dk_ = be_.prepare(view_of(k_));
-  dkt_ = be_.prepare(view_of(kt_));
+  dk_ = be_.prepare_bins(view_of(k_));
+  dkt_ = be_.prepare_bins(view_of(kt_));
   dks_ = be_.prepare_slots(view_of(k_));
   dkts_ = be_.prepare_slots(view_of(kt_));
   allocate();

@@ -215,11 +215,10 @@ __global__ void __launch_bounds__(64 * kWaves)
 // together and then all the gathers; the extra term adds +0.0, which leaves
 // a sum that started at +0.0 unchanged, so the sum has the serial loop's
 // bits.
-__device__ __forceinline__ double packed_row_sum(const Int* __restrict__ start,
-                                                 const Int* __restrict__ index,
-                                                 const double* __restrict__ value, Int row,
-                                                 const double* __restrict__ v, int slot) {
-  const Int lo = start[row], len = start[row + 1] - lo;
+__device__ __forceinline__ double packed_range_sum(const Int* __restrict__ index,
+                                                   const double* __restrict__ value, Int lo,
+                                                   Int len, const double* __restrict__ v,
+                                                   int slot) {
   const Int safe = len > 0 ? lo : 0;
   double s = 0.0;
   for (Int u0 = 0; __any(u0 < len); u0 += 8) {
@@ -242,6 +241,14 @@ __device__ __forceinline__ double packed_row_sum(const Int* __restrict__ start,
     }
   }
   return s;
+}
+
+__device__ __forceinline__ double packed_row_sum(const Int* __restrict__ start,
+                                                 const Int* __restrict__ index,
+                                                 const double* __restrict__ value, Int row,
+                                                 const double* __restrict__ v, int slot) {
+  const Int lo = start[row];
+  return packed_range_sum(index, value, lo, start[row + 1] - lo, v, slot);
 }
 
 template <bool kDiag, int kWaves, int kG>
@@ -280,6 +287,82 @@ __global__ void __launch_bounds__(64 * kWaves)
   const double kxb = packed_row_sum(k_start, k_index, k_value, i, x_bar, slot);
   if (!P.active) return;
   dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
+}
+
+// Binned steps for a narrow span over a matrix with long rows: the rows are
+// binned by length (DevCsr::bin_plan). The first short_waves waves each take
+// kG rows of at most kBinShortMax entries, packed as above; every longer row
+// then gets a wave of its own, summed in pipelined rounds of eight with
+// scalar loads. A wave reads its rows' {row, start, length} from the plan,
+// which replaces the load of the row start. Same bits as the one-row kernel.
+template <bool kDiag, int kWaves, int kG>
+__global__ void __launch_bounds__(64 * kWaves)
+    bprimal_binned_kernel(const int4* __restrict__ plan, Int short_waves, Int units,
+                          const Int* __restrict__ kt_index, const double* __restrict__ kt_value,
+                          const double* __restrict__ y, int blocks, int step, Int span,
+                          const LaneParams* __restrict__ params, BatchData d, PrimalBufs b) {
+  constexpr int kSpan = 64 / kG;
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const Int w = lanes::wave_row_of<kWaves>(blocks);
+  if (w >= units) return;  // the last workgroup's spare waves
+  if (w < short_waves) {
+    const int slot = lane % kSpan;
+    const int4 e = plan[w * kG + lane / kSpan];
+    const Int j = e.x;
+    if (j < 0) return;
+    const size_t o = at(j, slot);
+    const LaneParams P = params[slot];
+    const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
+    const double kty = packed_range_sum(kt_index, kt_value, e.y, e.z, y, slot);
+    if (!P.active) return;
+    primal_epilogue<kDiag>(P, step, o, xj, cj, lo, hi, anc, kty, b);
+  } else {
+    if (lane >= span) return;
+    const int4 e = plan[short_waves * kG + (w - short_waves)];
+    const Int j = e.x;
+    const size_t o = at(j, lane);
+    const LaneParams P = params[lane];
+    const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
+    const double kty = lanes::range_sum(lanes::CsrArrays{nullptr, kt_index, kt_value}, e.y,
+                                        e.y + e.z, y, lane, 0.0);
+    if (!P.active) return;
+    primal_epilogue<kDiag>(P, step, o, xj, cj, lo, hi, anc, kty, b);
+  }
+}
+
+template <bool kDiag, int kWaves, int kG>
+__global__ void __launch_bounds__(64 * kWaves)
+    bdual_binned_kernel(const int4* __restrict__ plan, Int short_waves, Int units,
+                        const Int* __restrict__ k_index, const double* __restrict__ k_value,
+                        const double* __restrict__ x_bar, int blocks, int step, Int span,
+                        const LaneParams* __restrict__ params, BatchData d, DualBufs b) {
+  constexpr int kSpan = 64 / kG;
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const Int w = lanes::wave_row_of<kWaves>(blocks);
+  if (w >= units) return;  // the last workgroup's spare waves
+  if (w < short_waves) {
+    const int slot = lane % kSpan;
+    const int4 e = plan[w * kG + lane / kSpan];
+    const Int i = e.x;
+    if (i < 0) return;
+    const size_t o = at(i, slot);
+    const LaneParams P = params[slot];
+    const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
+    const double kxb = packed_range_sum(k_index, k_value, e.y, e.z, x_bar, slot);
+    if (!P.active) return;
+    dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
+  } else {
+    if (lane >= span) return;
+    const int4 e = plan[short_waves * kG + (w - short_waves)];
+    const Int i = e.x;
+    const size_t o = at(i, lane);
+    const LaneParams P = params[lane];
+    const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
+    const double kxb = lanes::range_sum(lanes::CsrArrays{nullptr, k_index, k_value}, e.y, e.y + e.z,
+                                        x_bar, lane, 0.0);
+    if (!P.active) return;
+    dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
+  }
 }
 
 // The same steps with kRows consecutive rows per wave, their nonzeros taken
@@ -787,13 +870,39 @@ auto dual_packed_by_waves(int waves) {
     default: return &bdual_packed_kernel<kDiag, 4, kG>;
   }
 }
-// Rows per wave of the packed steps for a span (0: the span is too wide, or
-// a row is longer than one round of the packed sum: a wave's rows wait for
-// its longest, which was measured to cost more than the waves saved on K of
-// the refinery stacks, whose rows run to 30 entries).
-int packed_rows(const DevCsr& a, Int span) {
-  if (a.longest > 8) return 0;
-  return span <= 16 ? 4 : span <= 32 ? 2 : 0;
+// Rows per wave of the packed steps for a span (0: the span is too wide).
+int packed_rows(Int span) { return span <= 16 ? 4 : span <= 32 ? 2 : 0; }
+
+// How a narrow span runs over `a`: every row in packed waves when no row is
+// longer than one round of the packed sum; else, with the row bins, the
+// short rows packed and the long ones a wave each (packing K of the
+// refinery stacks, whose rows run to 30 entries, whole was measured slower:
+// a packed wave waits for its longest row); else one row per wave.
+enum class Narrow { kNone, kPacked, kBinned };
+Narrow narrow_mode(const DevCsr& a, int g) {
+  if (g == 0) return Narrow::kNone;
+  if (a.longest <= kBinShortMax) return Narrow::kPacked;
+  if (a.bin_short[g == 4 ? 1 : 0] >= 0) return Narrow::kBinned;
+  return Narrow::kNone;
+}
+
+template <bool kDiag, int kG>
+auto primal_binned_by_waves(int waves) {
+  switch (waves) {
+    case 1: return &bprimal_binned_kernel<kDiag, 1, kG>;
+    case 2: return &bprimal_binned_kernel<kDiag, 2, kG>;
+    case 8: return &bprimal_binned_kernel<kDiag, 8, kG>;
+    default: return &bprimal_binned_kernel<kDiag, 4, kG>;
+  }
+}
+template <bool kDiag, int kG>
+auto dual_binned_by_waves(int waves) {
+  switch (waves) {
+    case 1: return &bdual_binned_kernel<kDiag, 1, kG>;
+    case 2: return &bdual_binned_kernel<kDiag, 2, kG>;
+    case 8: return &bdual_binned_kernel<kDiag, 8, kG>;
+    default: return &bdual_binned_kernel<kDiag, 4, kG>;
+  }
 }
 
 template <bool kDiag>
@@ -801,8 +910,17 @@ void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const B
                        const LaneParams* params, int step, Int span, const PrimalBufs& b,
                        int waves, int rows, bool pack, const LaunchEvents* ev) {
   const char* what = "batch_primal_step";
-  const int g = pack && rows == 1 && kt.slot_width == 0 ? packed_rows(kt, span) : 0;
-  if (g > 0) {
+  const int g = pack && rows == 1 && kt.slot_width == 0 ? packed_rows(span) : 0;
+  const Narrow mode = narrow_mode(kt, g);
+  if (mode == Narrow::kBinned) {
+    const int p = g == 4 ? 1 : 0;
+    const Int units = kt.bin_short[p] + kt.bin_long;
+    const int blocks = static_cast<int>((units + waves - 1) / waves);
+    launch_rows_per(g == 4 ? primal_binned_by_waves<kDiag, 4>(waves)
+                           : primal_binned_by_waves<kDiag, 2>(waves),
+                    waves, 1, units, s, ev, what, reinterpret_cast<const int4*>(kt.bin_plan[p]),
+                    kt.bin_short[p], units, kt.index, kt.value, y, blocks, step, span, params, d, b);
+  } else if (mode == Narrow::kPacked) {
     const Int units = (kt.rows + g - 1) / g;
     const int blocks = static_cast<int>((units + waves - 1) / waves);
     launch_rows_per(g == 4 ? primal_packed_by_waves<kDiag, 4>(waves)
@@ -828,8 +946,17 @@ void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const 
                      const LaneParams* params, int step, Int span, const DualBufs& b, int waves,
                      int rows, bool pack, const LaunchEvents* ev) {
   const char* what = "batch_dual_step";
-  const int g = pack && rows == 1 && k.slot_width == 0 ? packed_rows(k, span) : 0;
-  if (g > 0) {
+  const int g = pack && rows == 1 && k.slot_width == 0 ? packed_rows(span) : 0;
+  const Narrow mode = narrow_mode(k, g);
+  if (mode == Narrow::kBinned) {
+    const int p = g == 4 ? 1 : 0;
+    const Int units = k.bin_short[p] + k.bin_long;
+    const int blocks = static_cast<int>((units + waves - 1) / waves);
+    launch_rows_per(g == 4 ? dual_binned_by_waves<kDiag, 4>(waves)
+                           : dual_binned_by_waves<kDiag, 2>(waves),
+                    waves, 1, units, s, ev, what, reinterpret_cast<const int4*>(k.bin_plan[p]),
+                    k.bin_short[p], units, k.index, k.value, x_bar, blocks, step, span, params, d, b);
+  } else if (mode == Narrow::kPacked) {
     const Int units = (k.rows + g - 1) / g;
     const int blocks = static_cast<int>((units + waves - 1) / waves);
     launch_rows_per(g == 4 ? dual_packed_by_waves<kDiag, 4>(waves)

@@ -74,7 +74,19 @@ struct DevCsr {
   const Int* slot_row = nullptr;
   const Int* slot_index = nullptr;
   const double* slot_value = nullptr;
+  // Optional row bins for the packed batched steps (bin_short[p] < 0: not
+  // built). bin_plan[p] (p = 0 for two rows a wave, 1 for four) holds
+  // {row, start, length, 0} entries: bin_short[p] waves of 2 or 4 rows of
+  // at most kBinShortMax entries each (row -1 pads the last), then the
+  // bin_long longer rows, one wave each.
+  const Int* bin_plan[2] = {nullptr, nullptr};
+  Int bin_short[2] = {-1, -1};
+  Int bin_long = 0;
 };
+
+// The longest row the packed steps take several to a wave: one round of
+// their sum.
+inline constexpr Int kBinShortMax = 8;
 
 // Widths of the fixed-width copy: the smallest that holds every row, or the
 // widest with longer rows continuing in the CSR.

@@ -616,10 +616,14 @@ int anuk_pdhg_adjoint(anuk_ctx* ctx, const anuk_lp* const* lps, int32_t count,
                       double* grad_c, double* grad_q, double* grad_lower, double* grad_upper,
                       anuk_adjoint_info* info);
 
-/* The launch shapes the batched solver's tuner measured and chose on this
- * context, one line per tuned kernel and matrix, a shape written w<waves per
- * workgroup>r<rows per wave>:
- *   <kernel> rows=<r> nnz=<z> us@w1r1=<t> ... us@w8r4=<t> chosen=w<w>r<r>
+/* The launch shapes the tuners measured and chose on this context, one line
+ * per tuned kernel and matrix:
+ *   <kernel> rows=<r> nnz=<z> us@<shape>=<t> ... chosen=<shape>
+ * For the batched steps (batch_primal_step, batch_dual_step) a shape is
+ * w<waves per workgroup>r<rows per wave>, with a trailing "s" for the
+ * fixed-width row copy; for the uneven stack's products (multi_k_product,
+ * multi_kt_product) it is "plan" or "binned" (the row blocks' launch
+ * order), with "+nt" for non-temporal index and value loads.
  * Same buffer protocol as anuk_profile_report. No reference counterpart. */
 int anuk_tuning_report(anuk_ctx* ctx, char* buffer, size_t capacity, size_t* needed);
 

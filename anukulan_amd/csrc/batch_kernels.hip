@@ -292,15 +292,17 @@ __global__ void __launch_bounds__(64 * kWaves)
 // Binned steps for a narrow span over a matrix with long rows: the rows are
 // binned by length (DevCsr::bin_plan). The first short_waves waves each take
 // kG rows of at most kBinShortMax entries, packed as above; every longer row
-// then gets a wave of its own, summed in pipelined rounds of eight with
-// scalar loads. A wave reads its rows' {row, start, length} from the plan,
-// which replaces the load of the row start. Same bits as the one-row kernel.
+// then gets a wave of its own, summed in one round (lanes::round_row_sum:
+// the row's entries in one vector load, all its gathers at once). A wave
+// reads its rows' {row, start, length} from the plan, which replaces the
+// load of the row start. Same bits as the one-row kernel.
 template <bool kDiag, int kWaves, int kG>
 __global__ void __launch_bounds__(64 * kWaves)
     bprimal_binned_kernel(const int4* __restrict__ plan, Int short_waves, Int units,
                           const Int* __restrict__ kt_index, const double* __restrict__ kt_value,
                           const double* __restrict__ y, int blocks, int step, Int span,
-                          const LaneParams* __restrict__ params, BatchData d, PrimalBufs b) {
+                          Int y_rows, const LaneParams* __restrict__ params, BatchData d,
+                          PrimalBufs b) {
   constexpr int kSpan = 64 / kG;
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int w = lanes::wave_row_of<kWaves>(blocks);
@@ -317,15 +319,22 @@ __global__ void __launch_bounds__(64 * kWaves)
     if (!P.active) return;
     primal_epilogue<kDiag>(P, step, o, xj, cj, lo, hi, anc, kty, b);
   } else {
-    if (lane >= span) return;
+    // A long row, summed in one round: every lane takes part in the row sum.
     const int4 e = plan[short_waves * kG + (w - short_waves)];
     const Int j = e.x;
+    const bool live = lane < span;
     const size_t o = at(j, lane);
     const LaneParams P = params[lane];
-    const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
-    const double kty = lanes::range_sum(lanes::CsrArrays{nullptr, kt_index, kt_value}, e.y,
-                                        e.y + e.z, y, lane, 0.0);
-    if (!P.active) return;
+    double xj = 0.0, cj = 0.0, lo = 0.0, hi = 0.0, anc = 0.0;
+    if (live) {
+      xj = b.x[o];
+      cj = d.c[o];
+      lo = d.lower[o];
+      hi = d.upper[o];
+      anc = b.anchor[o];
+    }
+    const double kty = lanes::round_row_sum(kt_index, kt_value, e.y, e.z, y, y_rows, lane, live);
+    if (!live || !P.active) return;
     primal_epilogue<kDiag>(P, step, o, xj, cj, lo, hi, anc, kty, b);
   }
 }
@@ -335,7 +344,8 @@ __global__ void __launch_bounds__(64 * kWaves)
     bdual_binned_kernel(const int4* __restrict__ plan, Int short_waves, Int units,
                         const Int* __restrict__ k_index, const double* __restrict__ k_value,
                         const double* __restrict__ x_bar, int blocks, int step, Int span,
-                        const LaneParams* __restrict__ params, BatchData d, DualBufs b) {
+                        Int x_rows, const LaneParams* __restrict__ params, BatchData d,
+                        DualBufs b) {
   constexpr int kSpan = 64 / kG;
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int w = lanes::wave_row_of<kWaves>(blocks);
@@ -352,15 +362,22 @@ __global__ void __launch_bounds__(64 * kWaves)
     if (!P.active) return;
     dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
   } else {
-    if (lane >= span) return;
+    // A long row, summed in one round: every lane takes part in the row sum.
     const int4 e = plan[short_waves * kG + (w - short_waves)];
     const Int i = e.x;
+    const bool live = lane < span;
     const size_t o = at(i, lane);
     const LaneParams P = params[lane];
-    const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
-    const double kxb = lanes::range_sum(lanes::CsrArrays{nullptr, k_index, k_value}, e.y, e.y + e.z,
-                                        x_bar, lane, 0.0);
-    if (!P.active) return;
+    double yi = 0.0, kx = 0.0, qi = 0.0, ay = 0.0, ak = 0.0;
+    if (live) {
+      yi = b.y[o];
+      kx = b.k_x[o];
+      qi = d.q[o];
+      ay = b.anchor_y[o];
+      ak = b.anchor_kx[o];
+    }
+    const double kxb = lanes::round_row_sum(k_index, k_value, e.y, e.z, x_bar, x_rows, lane, live);
+    if (!live || !P.active) return;
     dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
   }
 }
@@ -919,7 +936,8 @@ void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const B
     launch_rows_per(g == 4 ? primal_binned_by_waves<kDiag, 4>(waves)
                            : primal_binned_by_waves<kDiag, 2>(waves),
                     waves, 1, units, s, ev, what, reinterpret_cast<const int4*>(kt.bin_plan[p]),
-                    kt.bin_short[p], units, kt.index, kt.value, y, blocks, step, span, params, d, b);
+                    kt.bin_short[p], units, kt.index, kt.value, y, blocks, step, span, kt.cols, params,
+                    d, b);
   } else if (mode == Narrow::kPacked) {
     const Int units = (kt.rows + g - 1) / g;
     const int blocks = static_cast<int>((units + waves - 1) / waves);
@@ -955,7 +973,8 @@ void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const 
     launch_rows_per(g == 4 ? dual_binned_by_waves<kDiag, 4>(waves)
                            : dual_binned_by_waves<kDiag, 2>(waves),
                     waves, 1, units, s, ev, what, reinterpret_cast<const int4*>(k.bin_plan[p]),
-                    k.bin_short[p], units, k.index, k.value, x_bar, blocks, step, span, params, d, b);
+                    k.bin_short[p], units, k.index, k.value, x_bar, blocks, step, span, k.cols, params, d,
+                    b);
   } else if (mode == Narrow::kPacked) {
     const Int units = (k.rows + g - 1) / g;
     const int blocks = static_cast<int>((units + waves - 1) / waves);

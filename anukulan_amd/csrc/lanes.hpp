@@ -209,6 +209,80 @@ __device__ __forceinline__ void slot_gather(const SlotArrays& t, Int row,
   }
 }
 
+// A row of len entries from lo, 1 <= len <= kMax, summed with one load of
+// its structure and one round of gathers. Lane u of the wave loads entry
+// min(u, len - 1) with a vector load; readlane then hands each entry's index
+// and value to every lane, and all kMax gathers go out together. So every
+// lane of the wave must execute this call (the broadcasts read the lanes
+// that loaded the entries); a lane with `gather` false, and every position
+// past the row's length, gathers through the buffer resource at an offset
+// outside it, which reads no memory and yields 0. Each lane adds its terms
+// in storage order, a position past the length adding +0.0 (which leaves a
+// sum that started at +0.0 unchanged), so the sum has the bits of the serial
+// loop. v_rows * 512 must stay below 2^31.
+template <int kMax>
+__device__ __forceinline__ double round_sum(const Int* __restrict__ index,
+                                           const double* __restrict__ value, Int lo, Int len,
+                                           const double* __restrict__ v, Int v_rows, int lane,
+                                           bool gather) {
+  static_assert(kMax >= 1 && kMax <= kLaneCount, "one entry per lane");
+  const Int mine = lo + (lane < len ? lane : len - 1);
+  const int my_ix = index[mine];
+  const unsigned long long my_w = __builtin_bit_cast(unsigned long long, value[mine]);
+  const int w_lo = static_cast<int>(static_cast<unsigned>(my_w));
+  const int w_hi = static_cast<int>(static_cast<unsigned>(my_w >> 32));
+  const __amdgpu_buffer_rsrc_t rs = buffer_over(v, v_rows * (kLaneCount * 8));
+  const unsigned off = static_cast<unsigned>(lane) * 8u;
+  const unsigned dead = gather ? 0u : 1u;
+  double x[kMax];
+#pragma unroll
+  for (int t = 0; t < kMax; ++t) {
+    const unsigned ix = static_cast<unsigned>(__builtin_amdgcn_readlane(my_ix, t));
+    const unsigned past = static_cast<unsigned>(len - 1 - t) >> 31;
+    const unsigned at_row =
+        (ix * static_cast<unsigned>(kLaneCount * 8) + off) | ((past | dead) << 31);
+    x[t] = __builtin_bit_cast(
+        double, __builtin_amdgcn_raw_buffer_load_b64(rs, static_cast<int>(at_row), 0, 0));
+  }
+  // Keep every gather ahead of the first add: left to itself, the scheduler
+  // interleaves the adds and issues the last gathers only after the first
+  // ones have returned.
+  __builtin_amdgcn_sched_barrier(0);
+  double s = 0.0;
+#pragma unroll
+  for (int t = 0; t < kMax; ++t) {
+    const unsigned lo32 = static_cast<unsigned>(__builtin_amdgcn_readlane(w_lo, t));
+    const unsigned hi32 = static_cast<unsigned>(__builtin_amdgcn_readlane(w_hi, t));
+    const double w =
+        __builtin_bit_cast(double, (static_cast<unsigned long long>(hi32) << 32) | lo32);
+    const double p = w * x[t];
+    s += t < len ? p : 0.0;
+  }
+  return s;
+}
+
+// The longest row round_row_sum takes in one round.
+inline constexpr Int kRoundMax = 32;
+
+// A whole row [lo, lo + len) by round_sum in the narrowest round that holds
+// it, and in range_sum's pipelined rounds past kRoundMax (or for a gathered
+// vector too large for 31-bit buffer offsets). Every lane of the wave must
+// execute this; a lane with `live` false gathers nothing and gets 0. Same
+// bits as row_sum.
+__device__ __forceinline__ double round_row_sum(const Int* __restrict__ index,
+                                               const double* __restrict__ value, Int lo, Int len,
+                                               const double* __restrict__ v, Int v_rows, int lane,
+                                               bool live) {
+  if (len <= 0) return 0.0;
+  if (len > kRoundMax || v_rows >= kSlotMaxCols) {
+    if (!live) return 0.0;
+    return range_sum(CsrArrays{nullptr, index, value}, lo, lo + len, v, lane, 0.0);
+  }
+  if (len <= 8) return round_sum<8>(index, value, lo, len, v, v_rows, lane, live);
+  if (len <= 16) return round_sum<16>(index, value, lo, len, v, v_rows, lane, live);
+  return round_sum<kRoundMax>(index, value, lo, len, v, v_rows, lane, live);
+}
+
 template <int W>
 __device__ __forceinline__ double slot_finish(const SlotArrays& t, const CsrArrays& a, Int row,
                                               const SlotRow<W>& r, const double* __restrict__ v,

@@ -181,6 +181,25 @@ def test_fixed_width_layout_with_rows_past_its_width(ctx):
         assert np.array_equal(g.x, h.x) and np.array_equal(g.y, h.y)
 
 
+@pytest.mark.parametrize("cases", [6, 20])
+def test_binned_long_rows_give_the_same_bits(ctx, cases):
+    # A random LP whose rows run from 2 to 25 entries in K and to 35 in K^T:
+    # with the span at 16 (6 cases) or 32 (20 cases) both steps run binned,
+    # short rows packed and every row longer than 8 a wave of its own,
+    # summed in one round of 16 or 32 gathers (K^T's rows past 32 in rounds
+    # of eight). The iterates must be those of one row per wave to the bit.
+    base = G.random_lp(300, 900, mean_col_nnz=4.0, seed=3)
+    models = [G.NaturalModel(**{**base.__dict__, "objective": base.objective * (1.0 + 0.01 * k)})
+              for k in range(cases)]
+    lps = [m.to_lp() for m in models]
+    want = solve_pdhg_batch(ctx, lps, tolerance=1e-6, batch_waves=2, batch_rows=1, batch_pack=1)
+    got = solve_pdhg_batch(ctx, lps, tolerance=1e-6, batch_waves=2, batch_rows=1, batch_pack=0)
+    assert all(h.status == "optimal" for h in want)
+    for g, h in zip(got, want):
+        assert g.status == h.status and g.iterations == h.iterations
+        assert np.array_equal(g.x, h.x) and np.array_equal(g.y, h.y)
+
+
 def test_packed_dual_step_gives_the_same_bits(ctx):
     # A random LP whose rows of K hold at most 6 entries (K^T up to 59): with
     # six cases the dual step runs packed, four rows to a wave, and must

@@ -222,7 +222,9 @@ double* HipBackend::pinned_slot(int slot, std::size_t doubles) {
   }
   double* p = nullptr;
   const std::size_t size = std::max<std::size_t>(doubles, 512);
-  hip_check(hipHostMalloc(&p, size * sizeof(double), hipHostMallocDefault), "hipHostMalloc(slot)");
+  // Coherent: a kernel may read the host's latest writes here, and the host
+  // may poll values a kernel stores.
+  hip_check(hipHostMalloc(&p, size * sizeof(double), hipHostMallocCoherent), "hipHostMalloc(slot)");
   pinned_slots_[slot] = {p, size};
   return p;
 }

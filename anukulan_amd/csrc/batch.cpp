@@ -80,10 +80,16 @@ class Engine {
  private:
   void allocate();
   void upload_lanes();
-  void upload_params();
-  void enqueue_period(int steps, bool has_check, int parity);
+  // Stages the lane table of the next period in page-locked memory and, unless
+  // a captured period copies it itself (`staged_only`), uploads it.
+  void upload_params(bool staged_only = false);
+  // `host_io`: the period copies the staged lane table in first and
+  // publishes its check to the host last (launch_batch_params/_publish).
+  void enqueue_period(int steps, bool has_check, int parity, bool host_io = false);
   hipGraphExec_t graph_for(int parity);
   void fetch(const double* dev, double* host, size_t doubles);
+  // Waits for the period numbered seq_ to publish its check.
+  void wait_published();
   void retire_extract(Mask mask, std::vector<LaneOutcome>& out);
   void run_restarts(Mask mask);
   void run_polish(const std::vector<int>& who, Int budget, std::vector<LaneOutcome>& out,
@@ -142,7 +148,9 @@ class Engine {
   double *EXT_, *CHECK_, *RESTART_, *CHECKP_;
   LaneParams* d_params_ = nullptr;
   BatchData data_;
-  double* pinned_ = nullptr;  // params + check + restart staging
+  double* pinned_ = nullptr;  // params + check + restart staging, then the publish flag
+  Int* flag_ = nullptr;       // the last period whose check reached pinned_
+  Int seq_ = 0;               // the last period staged
   std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
 };
 
@@ -156,7 +164,7 @@ Engine::~Engine() {
   if (slab_) (void)hipStreamSynchronize(be_.stream());
 }
 
-constexpr size_t kPinnedDoubles = 64 * 8 + 64 * kLaneCheckStride + 128;
+constexpr size_t kPinnedDoubles = 64 * 8 + 64 * kLaneCheckStride + 128 + 8;
 static_assert(sizeof(LaneTable) <= 64 * 8 * sizeof(double), "the lane table fits its slab entry");
 
 void Engine::allocate() {
@@ -199,6 +207,9 @@ void Engine::allocate() {
   data_.orig_c = oc; data_.orig_lower = olo; data_.orig_upper = ohi; data_.orig_q = oq;
   data_.row_scale = rs; data_.col_scale = cs;
   pinned_ = be_.pinned_slot(kBatchScratchSlot + depth_, kPinnedDoubles);
+  flag_ = reinterpret_cast<Int*>(pinned_ + kPinnedDoubles - 8);
+  __atomic_store_n(flag_, Int{-1}, __ATOMIC_RELEASE);
+  seq_ = 0;
 }
 
 // Row-interleaves one vector per lane (rows entries each) into `out`, whose
@@ -314,8 +325,9 @@ void Engine::upload_lanes() {
   be_.copy(Y_[0], PRY_, m_ * kLanes);
 }
 
-void Engine::upload_params() {
+void Engine::upload_params(bool staged_only) {
   auto* T = reinterpret_cast<LaneTable*>(pinned_);
+  T->seq = ++seq_;
   LaneParams* P = T->lane;
   T->span = 0;
   for (int slot = 0; slot < kLanes; ++slot) {
@@ -335,12 +347,14 @@ void Engine::upload_params() {
     P[slot] = lp;
   }
   span_ = std::max<Int>(16, (T->span + 15) / 16 * 16);
+  if (staged_only) return;
   hip_check(hipMemcpyAsync(d_params_, T, sizeof(LaneTable), hipMemcpyHostToDevice, be_.stream()),
             "upload lane params");
 }
 
-void Engine::enqueue_period(int steps, bool has_check, int parity) {
+void Engine::enqueue_period(int steps, bool has_check, int parity, bool host_io) {
   hipStream_t s = be_.stream();
+  if (host_io) launch_batch_params(s, reinterpret_cast<const LaneTable*>(pinned_), d_params_);
   int p = parity;
   // Algorithmic bytes per launch: the matrix once (8 B value + 4 B index per
   // nonzero, 4 B per row start), the gathered vector once per active LP, and
@@ -371,6 +385,7 @@ void Engine::enqueue_period(int steps, bool has_check, int parity) {
                       d_params_, ~0ull, pw_);
     launch_batch_check(s, data_, X_[p], Y_[p], KX_[p], DX_, DY_, KTY_, LAM_, farkas, d_params_,
                        CHECKP_, CHECK_);
+    if (host_io) launch_batch_publish(s, CHECK_, d_params_, pinned_ + 64 * 8, flag_);
   }
 }
 
@@ -384,7 +399,7 @@ hipGraphExec_t Engine::graph_for(int parity) {
   hipGraphExec_t e = nullptr;
   hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   try {
-    enqueue_period(opt_.termination_check_frequency, true, parity);
+    enqueue_period(opt_.termination_check_frequency, true, parity, true);
   } catch (...) {
     (void)hipStreamEndCapture(s, &g);
     if (g) (void)hipGraphDestroy(g);
@@ -394,6 +409,21 @@ hipGraphExec_t Engine::graph_for(int parity) {
   hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "hipGraphInstantiate");
   graphs_[key] = {g, e};
   return e;
+}
+
+void Engine::wait_published() {
+  // Polling the flag sees the check about as soon as it lands; a period that
+  // takes unusually long (or a fault, which leaves the flag unset) falls back
+  // to the stream's own wait, which reports a device error.
+  const Int want = seq_;
+  const Clock::time_point t0 = Clock::now();
+  for (unsigned spin = 1;; ++spin) {
+    if (__atomic_load_n(flag_, __ATOMIC_ACQUIRE) == want) return;
+    if ((spin & 1023u) == 0 && Clock::now() - t0 > std::chrono::milliseconds(50)) break;
+  }
+  be_.synchronize();
+  if (__atomic_load_n(flag_, __ATOMIC_ACQUIRE) != want)
+    throw std::runtime_error("batch: a period finished without publishing its check");
 }
 
 void Engine::fetch(const double* dev, double* host, size_t doubles) {
@@ -830,11 +860,11 @@ std::vector<LaneOutcome> Engine::run() {
     const Int period_end = std::min<Int>((iteration / freq + 1) * freq, opt_.max_iterations);
     const int steps = period_end - iteration;
     const bool has_check = (period_end % freq == 0);
-    upload_params();
     // Polishing sub-batches run a few periods each; capturing graphs for
     // them costs more than it saves.
     const bool graph =
         opt_.use_graphs != 0 && depth_ == 0 && has_check && steps == freq && !be_.profiling();
+    upload_params(graph);
     if (graph) {
       hip_check(hipGraphLaunch(graph_for(cur_), be_.stream()), "hipGraphLaunch(batch)");
     } else {
@@ -849,7 +879,11 @@ std::vector<LaneOutcome> Engine::run() {
       lap(&Phases::iterating);
       continue;
     }
-    fetch(CHECK_, h_check, static_cast<size_t>(64) * kLaneCheckStride);
+    if (graph) {
+      wait_published();
+    } else {
+      fetch(CHECK_, h_check, static_cast<size_t>(64) * kLaneCheckStride);
+    }
     lap(&Phases::iterating);
     const Int i = period_end - 1;
     const double elapsed = seconds_since(start_);

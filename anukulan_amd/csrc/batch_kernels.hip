@@ -752,6 +752,45 @@ __global__ void bextract_kernel(Int rows, const double* __restrict__ v, unsigned
     out[static_cast<size_t>(slot) * rows + r] = v[at(r, lane)];
 }
 
+// The lane table, word by word (a period's first launch); each thread's
+// loads from host memory are all in flight before its stores.
+constexpr int kTableWords = static_cast<int>(sizeof(LaneTable) / sizeof(Int));
+constexpr int kTableRounds = (kTableWords + 255) / 256;
+__global__ void __launch_bounds__(256)
+    bparams_kernel(const Int* __restrict__ staged, Int* __restrict__ table) {
+  Int w[kTableRounds];
+#pragma unroll
+  for (int r = 0; r < kTableRounds; ++r) {
+    const int i = static_cast<int>(threadIdx.x) + 256 * r;
+    w[r] = i < kTableWords ? staged[i] : 0;
+  }
+#pragma unroll
+  for (int r = 0; r < kTableRounds; ++r) {
+    const int i = static_cast<int>(threadIdx.x) + 256 * r;
+    if (i < kTableWords) table[i] = w[r];
+  }
+}
+
+// The check values out to the host, then the period's number into *flag
+// (a period's last launch): every thread's stores are made visible system
+// wide before the workgroup barrier, and the flag is stored after it with
+// release semantics at system scope.
+constexpr int kPublishRounds = kLanes * kLaneCheckStride / 256;
+static_assert(kLanes * kLaneCheckStride % 256 == 0, "the check values split evenly over 256 threads");
+__global__ void __launch_bounds__(256)
+    bpublish_kernel(const double* __restrict__ check, const LaneTable* __restrict__ table,
+                    double* __restrict__ host_check, Int* flag) {
+  double v[kPublishRounds];
+#pragma unroll
+  for (int r = 0; r < kPublishRounds; ++r) v[r] = check[threadIdx.x + 256 * r];
+#pragma unroll
+  for (int r = 0; r < kPublishRounds; ++r) host_check[threadIdx.x + 256 * r] = v[r];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(flag, table->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 void check_launch(const char* what) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess)
@@ -1017,6 +1056,20 @@ void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
   } else {
     dual_step_shape<false>(s, k, x_bar, d, params, step, span, b, waves, rows, pack, ev);
   }
+}
+
+void launch_batch_params(hipStream_t s, const LaneTable* staged, LaneParams* table) {
+  static_assert(sizeof(LaneTable) % sizeof(Int) == 0, "the table copies by words");
+  hipLaunchKernelGGL(bparams_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<const Int*>(staged),
+                     reinterpret_cast<Int*>(table));
+  check_launch("batch_params");
+}
+
+void launch_batch_publish(hipStream_t s, const double* check, const LaneParams* table,
+                          double* host_check, Int* flag) {
+  hipLaunchKernelGGL(bpublish_kernel, dim3(1), dim3(256), 0, s, check,
+                     reinterpret_cast<const LaneTable*>(table), host_check, flag);
+  check_launch("batch_publish");
 }
 
 void launch_batch_spmm(hipStream_t s, const DevCsr& a, const double* v1, double* out1,

@@ -39,6 +39,7 @@ struct LaneParams {
 struct LaneTable {
   LaneParams lane[64];
   Int span;
+  Int seq;  // the period's number, which launch_batch_publish hands back
 };
 
 // Problem data, per LP row-interleaved except the shared scaling diagonals.
@@ -77,6 +78,15 @@ void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                             const BatchData& d, const LaneParams* params, int step, Int span,
                             const DualBufs& b, bool diag, int waves = 4, int rows = 1,
                             bool pack = false, const LaunchEvents* ev = nullptr);
+// A period's host traffic without a host wait on the device side: its first
+// launch copies the lane table the host staged in page-locked (coherent)
+// memory into the device table the step kernels read, and its last copies
+// the per-lane check values out to page-locked memory and then sets *flag to
+// the table's period number, after the values and with system scope, so the
+// host can poll the flag instead of waiting on the stream.
+void launch_batch_params(hipStream_t s, const LaneTable* staged, LaneParams* table);
+void launch_batch_publish(hipStream_t s, const double* check, const LaneParams* table,
+                          double* host_check, Int* flag);
 // out1 = A v1 (and out2 = A v2 when v2 is non-null) for the active lanes:
 // those whose LaneParams say so when `params` is given (read at run time, so
 // a captured graph follows retirements), else those set in `mask`. Other

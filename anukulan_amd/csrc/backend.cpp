@@ -655,11 +655,14 @@ Terms HipBackend::convergence_terms(Int n, Int m, const DevProblem& p, const dou
 namespace {
 // A batched-step shape, 100 * layout + 10 * waves per workgroup + rows per
 // wave, as "w<waves>r<rows>" with an "s" for layout 1 (the fixed-width row
-// copy); a stack launch choice, 200 + 10 * binned + non-temporal, as "plan"
-// or "binned" with "+nt".
+// copy); a narrow-span shape, 300 + 10 * waves + 1, as "w<waves>r1n"; a
+// stack launch choice, 200 + 10 * binned + non-temporal, as "plan" or
+// "binned" with "+nt".
 std::string shape_label(int shape) {
   char buf[32];
-  if (shape >= 200) {
+  if (shape >= 300) {
+    std::snprintf(buf, sizeof(buf), "w%dr1n", (shape % 100) / 10);
+  } else if (shape >= 200) {
     std::snprintf(buf, sizeof(buf), "%s%s", (shape / 10) % 10 != 0 ? "binned" : "plan",
                   shape % 10 != 0 ? "+nt" : "");
   } else {

@@ -138,6 +138,8 @@ class Engine {
   int pw_ = 2, dw_ = 2;
   int pr_ = 1, dr_ = 1;
   bool ps_ = false, ds_ = false;
+  // Waves per workgroup of the packed and binned launches of narrow spans.
+  int pnw_ = 1, dnw_ = 1;
   // Narrow spans (32 live slots or fewer) run the packed step kernels.
   bool pack_ = true;
 
@@ -369,13 +371,13 @@ void Engine::enqueue_period(int steps, bool has_check, int parity, bool host_io)
         be_.launch_events(diag ? "batch_primal_step_check" : "batch_primal_step",
                           12.0 * nnz + 4.0 * (n + 1) + 8.0 * lanes * (m + (diag ? 8.0 : 7.0) * n));
     launch_batch_primal_step(s, ps_ ? dkts_ : dkt_, Y_[p], data_, d_params_, step, span_, pb, diag,
-                             pw_, pr_, pack_, &pe);
+                             pw_, pr_, pack_, &pe, pnw_);
     DualBufs db{Y_[p], KX_[p], AY_, AKX_, Y_[1 - p], KX_[1 - p], DY_, KDX_, AY_, AKX_};
     const LaunchEvents de =
         be_.launch_events(diag ? "batch_dual_step_check" : "batch_dual_step",
                           12.0 * nnz + 4.0 * (m + 1) + 8.0 * lanes * (n + (diag ? 9.0 : 7.0) * m));
     launch_batch_dual_step(s, ds_ ? dks_ : dk_, XBAR_, data_, d_params_, step, span_, db, diag, dw_,
-                           dr_, pack_, &de);
+                           dr_, pack_, &de, dnw_);
     p = 1 - p;
   }
   if (has_check) {
@@ -655,8 +657,10 @@ std::uint64_t pattern_key(const Csr& a) {
 // The launch-shape tuner: times the primal and dual step kernels in every
 // shape (1, 2, 4 or 8 waves per workgroup; 1, 2 or 4 consecutive rows per
 // wave from the CSR, or one row per wave from the fixed-width row copy) on
-// this stack's K^T and K and keeps, per sparsity pattern for the context's
-// life, the fastest, if it beats the default shape by at least kTuneMargin;
+// this stack's K^T and K, and the packed and binned launches of a narrow
+// span (16 slots) with 1, 2, 4 or 8 waves per workgroup, and keeps, per
+// sparsity pattern for the context's life, the fastest of each, if it beats
+// the default by at least kTuneMargin;
 // otherwise the default stays, so timing noise between near-equal shapes
 // does not flip the choice. A shape's
 // time is the best of kTrials means of kReps back-to-back launches. Every
@@ -667,14 +671,21 @@ std::uint64_t pattern_key(const Csr& a) {
 namespace {
 constexpr double kTuneMargin = 0.03;
 // A shape is 100 * layout + 10 * waves + rows per wave, layout 1 for the
-// fixed-width row copy and 0 for the CSR.
+// fixed-width row copy and 0 for the CSR; a narrow-span shape is 300 + 10 *
+// waves + 1. One wave per workgroup measured fastest for the narrow
+// launches (a 2% shorter solve of the 24-period stack than two waves; four
+// and eight were slower still).
 constexpr int kDefaultShape = 21;
+constexpr int kDefaultNarrow = 311;
+constexpr Int kNarrowSpan = 16;
 int shape_waves(int shape) { return (shape % 100) / 10; }
 int shape_rows(int shape) { return shape % 10; }
 bool shape_slots(int shape) { return shape >= 100; }
 }  // namespace
 
 void Engine::tune() {
+  pnw_ = dnw_ = shape_waves(kDefaultNarrow);
+  if (opt_.batch_waves != 0) pnw_ = dnw_ = opt_.batch_waves;
   if (opt_.batch_waves != 0 || opt_.batch_rows != 0 || opt_.batch_layout != 0) {
     pw_ = dw_ = opt_.batch_waves != 0 ? opt_.batch_waves : shape_waves(kDefaultShape);
     pr_ = dr_ = opt_.batch_rows != 0 ? opt_.batch_rows : shape_rows(kDefaultShape);
@@ -687,13 +698,17 @@ void Engine::tune() {
   const std::uint64_t kp = pattern_key(k_), ktp = pattern_key(kt_);
   const HipBackend::Tuning* tp = be_.tuned("batch_primal_step", ktp);
   const HipBackend::Tuning* td = be_.tuned("batch_dual_step", kp);
-  if (tp != nullptr && td != nullptr) {
+  const HipBackend::Tuning* tpn = be_.tuned("batch_primal_step_narrow", ktp);
+  const HipBackend::Tuning* tdn = be_.tuned("batch_dual_step_narrow", kp);
+  if (tp != nullptr && td != nullptr && tpn != nullptr && tdn != nullptr) {
     pw_ = shape_waves(tp->chosen);
     pr_ = shape_rows(tp->chosen);
     ps_ = shape_slots(tp->chosen);
     dw_ = shape_waves(td->chosen);
     dr_ = shape_rows(td->chosen);
     ds_ = shape_slots(td->chosen);
+    pnw_ = shape_waves(tpn->chosen);
+    dnw_ = shape_waves(tdn->chosen);
     return;
   }
   if (opt_.autotune == 0 || depth_ != 0 || be_.profiling()) return;
@@ -711,6 +726,8 @@ void Engine::tune() {
   const DualBufs db{Y_[cur_], KX_[cur_], AY_, AKX_, Y_[1 - cur_], KX_[1 - cur_], DY_, KDX_, AY_, AKX_};
   HipBackend::Tuning p{"batch_primal_step", kt_.rows, kt_.nnz(), {}, kDefaultShape};
   HipBackend::Tuning d{"batch_dual_step", k_.rows, k_.nnz(), {}, kDefaultShape};
+  HipBackend::Tuning pn{"batch_primal_step_narrow", kt_.rows, kt_.nnz(), {}, kDefaultNarrow};
+  HipBackend::Tuning dn{"batch_dual_step_narrow", k_.rows, k_.nnz(), {}, kDefaultNarrow};
   struct Candidate {
     HipBackend::Tuning* side;
     int shape;
@@ -736,6 +753,14 @@ void Engine::tune() {
                        launch_batch_dual_step(s, dks_, XBAR_, data_, d_params_, 1, span_, db,
                                               false, w, 1);
                      }});
+    cands.push_back({&pn, 300 + w * 10 + 1, [&, w] {
+                       launch_batch_primal_step(s, dkt_, Y_[cur_], data_, d_params_, 1, kNarrowSpan,
+                                                pb, false, w, 1, true, nullptr, w);
+                     }});
+    cands.push_back({&dn, 300 + w * 10 + 1, [&, w] {
+                       launch_batch_dual_step(s, dk_, XBAR_, data_, d_params_, 1, kNarrowSpan, db,
+                                              false, w, 1, true, nullptr, w);
+                     }});
   }
   std::vector<double> best(cands.size(), 1e300);
   for (int t = 0; t < kTrials; ++t) {
@@ -752,17 +777,21 @@ void Engine::tune() {
   for (size_t c = 0; c < cands.size(); ++c) cands[c].side->us.push_back({cands[c].shape, best[c]});
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  auto pick = [](const HipBackend::Tuning& t) {
+  auto pick = [](const HipBackend::Tuning& t, int fallback) {
     const auto best = std::min_element(t.us.begin(), t.us.end(), [](const auto& a, const auto& b) {
       return a.second < b.second;
     });
     double base = best->second;
     for (const auto& c : t.us)
-      if (c.first == kDefaultShape) base = c.second;
-    return best->second < (1.0 - kTuneMargin) * base ? best->first : kDefaultShape;
+      if (c.first == fallback) base = c.second;
+    return best->second < (1.0 - kTuneMargin) * base ? best->first : fallback;
   };
-  p.chosen = pick(p);
-  d.chosen = pick(d);
+  p.chosen = pick(p, kDefaultShape);
+  d.chosen = pick(d, kDefaultShape);
+  pn.chosen = pick(pn, kDefaultNarrow);
+  dn.chosen = pick(dn, kDefaultNarrow);
+  pnw_ = shape_waves(pn.chosen);
+  dnw_ = shape_waves(dn.chosen);
   pw_ = shape_waves(p.chosen);
   pr_ = shape_rows(p.chosen);
   ps_ = shape_slots(p.chosen);
@@ -771,6 +800,8 @@ void Engine::tune() {
   ds_ = shape_slots(d.chosen);
   be_.record_tuning(ktp, std::move(p));
   be_.record_tuning(kp, std::move(d));
+  be_.record_tuning(ktp, std::move(pn));
+  be_.record_tuning(kp, std::move(dn));
 }
 
 // Moves the live lanes into the lowest slots, in lane order, when that frees

@@ -964,10 +964,11 @@ auto dual_binned_by_waves(int waves) {
 template <bool kDiag>
 void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const BatchData& d,
                        const LaneParams* params, int step, Int span, const PrimalBufs& b,
-                       int waves, int rows, bool pack, const LaunchEvents* ev) {
+                       int waves, int rows, bool pack, const LaunchEvents* ev, int narrow_waves) {
   const char* what = "batch_primal_step";
   const int g = pack && rows == 1 && kt.slot_width == 0 ? packed_rows(span) : 0;
   const Narrow mode = narrow_mode(kt, g);
+  if (mode != Narrow::kNone && narrow_waves != 0) waves = narrow_waves;
   if (mode == Narrow::kBinned) {
     const int p = g == 4 ? 1 : 0;
     const Int units = kt.bin_short[p] + kt.bin_long;
@@ -1001,10 +1002,11 @@ void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const B
 template <bool kDiag>
 void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const BatchData& d,
                      const LaneParams* params, int step, Int span, const DualBufs& b, int waves,
-                     int rows, bool pack, const LaunchEvents* ev) {
+                     int rows, bool pack, const LaunchEvents* ev, int narrow_waves) {
   const char* what = "batch_dual_step";
   const int g = pack && rows == 1 && k.slot_width == 0 ? packed_rows(span) : 0;
   const Narrow mode = narrow_mode(k, g);
+  if (mode != Narrow::kNone && narrow_waves != 0) waves = narrow_waves;
   if (mode == Narrow::kBinned) {
     const int p = g == 4 ? 1 : 0;
     const Int units = k.bin_short[p] + k.bin_long;
@@ -1039,22 +1041,26 @@ void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const 
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                               const BatchData& d, const LaneParams* params, int step, Int span,
                               const PrimalBufs& b, bool diag, int waves, int rows, bool pack,
-                              const LaunchEvents* ev) {
+                              const LaunchEvents* ev, int narrow_waves) {
   if (diag) {
-    primal_step_shape<true>(s, kt, y, d, params, step, span, b, waves, rows, pack, ev);
+    primal_step_shape<true>(s, kt, y, d, params, step, span, b, waves, rows, pack, ev,
+                            narrow_waves);
   } else {
-    primal_step_shape<false>(s, kt, y, d, params, step, span, b, waves, rows, pack, ev);
+    primal_step_shape<false>(s, kt, y, d, params, step, span, b, waves, rows, pack, ev,
+                             narrow_waves);
   }
 }
 
 void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                             const BatchData& d, const LaneParams* params, int step, Int span,
                             const DualBufs& b, bool diag, int waves, int rows, bool pack,
-                            const LaunchEvents* ev) {
+                            const LaunchEvents* ev, int narrow_waves) {
   if (diag) {
-    dual_step_shape<true>(s, k, x_bar, d, params, step, span, b, waves, rows, pack, ev);
+    dual_step_shape<true>(s, k, x_bar, d, params, step, span, b, waves, rows, pack, ev,
+                          narrow_waves);
   } else {
-    dual_step_shape<false>(s, k, x_bar, d, params, step, span, b, waves, rows, pack, ev);
+    dual_step_shape<false>(s, k, x_bar, d, params, step, span, b, waves, rows, pack, ev,
+                           narrow_waves);
   }
 }
 

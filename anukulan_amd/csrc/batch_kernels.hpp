@@ -68,16 +68,19 @@ inline constexpr int kLaneCheckStride = 16;
 // its CSR; from the CSR, with `pack` and a span of 32 slots or fewer, one
 // wave takes 64 / 32 = 2 or 64 / 16 = 4 rows of at most kBinShortMax
 // entries, a lane per (row, slot), and a longer row (with the matrix's row
-// bins, DevCsr::bin_*) a wave of its own.
+// bins, DevCsr::bin_*) a wave of its own; those narrow launches run
+// `narrow_waves` waves per workgroup (0: `waves`).
 // Every shape computes the same bits.
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,
                               const BatchData& d, const LaneParams* params, int step, Int span,
                               const PrimalBufs& b, bool diag, int waves = 4, int rows = 1,
-                              bool pack = false, const LaunchEvents* ev = nullptr);
+                              bool pack = false, const LaunchEvents* ev = nullptr,
+                              int narrow_waves = 0);
 void launch_batch_dual_step(hipStream_t s, const DevCsr& k, const double* x_bar,
                             const BatchData& d, const LaneParams* params, int step, Int span,
                             const DualBufs& b, bool diag, int waves = 4, int rows = 1,
-                            bool pack = false, const LaunchEvents* ev = nullptr);
+                            bool pack = false, const LaunchEvents* ev = nullptr,
+                            int narrow_waves = 0);
 // A period's host traffic without a host wait on the device side: its first
 // launch copies the lane table the host staged in page-locked (coherent)
 // memory into the device table the step kernels read, and its last copies

@@ -335,7 +335,9 @@ typedef struct anuk_pdhg_options {
   int32_t autotune;                    /* 1: the batched solver times its launch shapes
                                           (waves per workgroup: 1, 2, 4, 8; rows per
                                           wave: 1, 2, 4; the matrix read from its CSR or
-                                          from a fixed-width row copy) on the stack's K
+                                          from a fixed-width row copy; and the waves per
+                                          workgroup of the packed launches of a narrow
+                                          span) on the stack's K
                                           and K^T once per matrix and keeps the fastest
                                           when it beats the default shape by 3%; every
                                           shape computes the same bits. 0: the default

@@ -418,6 +418,38 @@ const DevCsr& HipBackend::prepare_bins(const CsrView& a) {
     dm.view.bin_short[p] = waves;
   }
   dm.view.bin_long = static_cast<Int>(long_rows.size());
+  // The fixed-width copies: every short entry of either plan (the four-row
+  // plan pads furthest), then every long row's first kBinLongWidth entries.
+  auto fill = [&](const std::vector<Int>& rows, size_t count, Int width, std::vector<Int>* ix,
+                  std::vector<double>* val) {
+    ix->assign(count * uz(width), 0);
+    val->assign(count * uz(width), 0.0);
+    for (size_t e = 0; e < rows.size(); ++e) {
+      const Int lo = a.start[rows[e]], len = std::min(a.start[rows[e] + 1] - lo, width);
+      for (Int u = 0; u < width; ++u) {
+        const Int at = lo + std::min(u, len - 1);
+        (*ix)[e * uz(width) + uz(u)] = len > 0 ? a.index[at] : 0;
+        (*val)[e * uz(width) + uz(u)] = u < len ? a.value[at] : 0.0;
+      }
+    }
+  };
+  std::vector<Int> ix8, ix32;
+  std::vector<double> val8, val32;
+  fill(short_rows, (short_rows.size() + 3) / 4 * 4, kBinShortMax, &ix8, &val8);
+  fill(long_rows, long_rows.size(), kBinLongWidth, &ix32, &val32);
+  auto put = [&](const void* src, size_t bytes) -> void* {
+    void* d = nullptr;
+    hip_check(hipMalloc(&d, std::max<size_t>(bytes, 16)), "hipMalloc(bin copies)");
+    dm.owned.push_back(d);
+    if (bytes > 0)
+      hip_check(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, stream_), "upload bin copies");
+    return d;
+  };
+  dm.view.bin_index8 = static_cast<const Int*>(put(ix8.data(), ix8.size() * sizeof(Int)));
+  dm.view.bin_value8 = static_cast<const double*>(put(val8.data(), val8.size() * sizeof(double)));
+  dm.view.bin_index32 = static_cast<const Int*>(put(ix32.data(), ix32.size() * sizeof(Int)));
+  dm.view.bin_value32 =
+      static_cast<const double*>(put(val32.data(), val32.size() * sizeof(double)));
   synchronize();
   auto ins = matrices_.emplace(key, std::move(dm));
   return ins.first->second.view;

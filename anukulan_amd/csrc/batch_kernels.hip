@@ -289,38 +289,122 @@ __global__ void __launch_bounds__(64 * kWaves)
   dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
 }
 
-// Binned steps for a narrow span over a matrix with long rows: the rows are
-// binned by length (DevCsr::bin_plan). The first short_waves waves each take
-// kG rows of at most kBinShortMax entries, packed as above; every longer row
-// then gets a wave of its own, summed in one round (lanes::round_row_sum:
-// the row's entries in one vector load, all its gathers at once). A wave
-// reads its rows' {row, start, length} from the plan, which replaces the
-// load of the row start. Same bits as the one-row kernel.
+// Binned steps for a narrow span: the rows are binned by length
+// (DevCsr::bin_plan). The first short_waves waves each take kG rows of at
+// most kBinShortMax entries, a lane per (row, slot) as in the packed steps;
+// every longer row then gets a wave of its own, summed in one round
+// (lanes::round_sum_loaded: the row's entries in one vector load, all its
+// gathers at once). A lane reads its row's {row, start, length} from the
+// plan and, in parallel, the row's entries from their fixed-width copy
+// (DevCsr::bin_index8 / bin_index32), which is addressed by the plan entry
+// alone; a long row past kBinLongWidth entries is summed from the CSR in
+// pipelined rounds instead. Same bits as the one-row kernel.
+struct BinArgs {
+  const int4* __restrict__ plan;
+  const Int* __restrict__ index8;
+  const double* __restrict__ value8;
+  const Int* __restrict__ index32;
+  const double* __restrict__ value32;
+};
+BinArgs bin_args(const DevCsr& a, int p) {
+  return {reinterpret_cast<const int4*>(a.bin_plan[p]), a.bin_index8, a.bin_value8, a.bin_index32,
+          a.bin_value32};
+}
+
+// A short entry's row: its 8 indices and values in 16-byte loads from the
+// fixed-width copy (bin8_load, issued before the plan entry is back), then
+// the gathers together and the terms in storage order (bin8_sum: a position
+// past len adds +0.0; the copy's padding repeats the row's last index, a
+// line the row reads anyway).
+struct Bin8 {
+  int4 i0, i1;
+  double2 w0, w1, w2, w3;
+};
+__device__ __forceinline__ Bin8 bin8_load(const Int* __restrict__ index8,
+                                          const double* __restrict__ value8, Int e) {
+  const int4* ip = reinterpret_cast<const int4*>(index8 + static_cast<size_t>(e) * kBinShortMax);
+  const double2* wp =
+      reinterpret_cast<const double2*>(value8 + static_cast<size_t>(e) * kBinShortMax);
+  return {ip[0], ip[1], wp[0], wp[1], wp[2], wp[3]};
+}
+// Marks a short entry's loaded row as used here. The lanes of a padding
+// entry or of a retired LP store nothing, so without this the compiler moves
+// their test ahead of the row's loads, which then wait for the plan entry
+// and the lane's parameters instead of going out beside them.
+__device__ __forceinline__ void bin8_keep(const Bin8& r) {
+  asm volatile("" ::"v"(r.i0.x), "v"(r.i0.y), "v"(r.i0.z), "v"(r.i0.w), "v"(r.i1.x), "v"(r.i1.y),
+               "v"(r.i1.z), "v"(r.i1.w));
+  asm volatile("" ::"v"(r.w0.x), "v"(r.w0.y), "v"(r.w1.x), "v"(r.w1.y), "v"(r.w2.x), "v"(r.w2.y),
+               "v"(r.w3.x), "v"(r.w3.y));
+}
+__device__ __forceinline__ double bin8_sum(const Bin8& r, Int len, const double* __restrict__ v,
+                                           int slot) {
+  const Int ix[8] = {r.i0.x, r.i0.y, r.i0.z, r.i0.w, r.i1.x, r.i1.y, r.i1.z, r.i1.w};
+  const double w[8] = {r.w0.x, r.w0.y, r.w1.x, r.w1.y, r.w2.x, r.w2.y, r.w3.x, r.w3.y};
+  double x[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) x[u] = v[at(ix[u], slot)];
+  double s = 0.0;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const double t = w[u] * x[u];
+    s += u < len ? t : 0.0;
+  }
+  return s;
+}
+
+// A long row's sum (every lane of the wave calls this): its first
+// kBinLongWidth entries, one per lane, from the fixed-width copy (my_ix,
+// my_w: loaded by the caller ahead of the plan entry), summed in one round
+// of 16 or 32 gathers; a longer row (or a gathered vector too large for the
+// round's 31-bit offsets) in range_sum's rounds from the CSR.
+__device__ __forceinline__ double bin32_sum(Int my_ix, double my_w, const Int* __restrict__ index,
+                                            const double* __restrict__ value, Int lo, Int len,
+                                            const double* __restrict__ v, Int v_rows, int lane,
+                                            bool live) {
+  if (len > kBinLongWidth || v_rows >= kSlotMaxCols) {
+    if (!live) return 0.0;
+    return lanes::range_sum(lanes::CsrArrays{nullptr, index, value}, lo, lo + len, v, lane, 0.0);
+  }
+  if (len <= 16) return lanes::round_sum_loaded<16>(my_ix, my_w, len, v, v_rows, lane, live);
+  return lanes::round_sum_loaded<kBinLongWidth>(my_ix, my_w, len, v, v_rows, lane, live);
+}
+
 template <bool kDiag, int kWaves, int kG>
 __global__ void __launch_bounds__(64 * kWaves)
-    bprimal_binned_kernel(const int4* __restrict__ plan, Int short_waves, Int units,
+    bprimal_binned_kernel(const int4* __restrict__ plan, const Int* __restrict__ index8,
+                          const double* __restrict__ value8, const Int* __restrict__ index32,
+                          const double* __restrict__ value32, Int short_waves, Int units,
+                          int blocks, int step, Int span, Int y_rows,
                           const Int* __restrict__ kt_index, const double* __restrict__ kt_value,
-                          const double* __restrict__ y, int blocks, int step, Int span,
-                          Int y_rows, const LaneParams* __restrict__ params, BatchData d,
-                          PrimalBufs b) {
+                          const double* __restrict__ y, const LaneParams* __restrict__ params,
+                          BatchData d, PrimalBufs b) {
   constexpr int kSpan = 64 / kG;
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int w = lanes::wave_row_of<kWaves>(blocks);
   if (w >= units) return;  // the last workgroup's spare waves
   if (w < short_waves) {
     const int slot = lane % kSpan;
-    const int4 e = plan[w * kG + lane / kSpan];
-    const Int j = e.x;
-    if (j < 0) return;
-    const size_t o = at(j, slot);
+    const Int en = w * kG + lane / kSpan;
+    const Bin8 r = bin8_load(index8, value8, en);
+    const int4 e = plan[en];
     const LaneParams P = params[slot];
+    bin8_keep(r);
+    // A padding entry (row -1) computes on row 0 and stores nothing.
+    const bool pad = e.x < 0;
+    const Int j = pad ? 0 : e.x;
+    const size_t o = at(j, slot);
     const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
-    const double kty = packed_range_sum(kt_index, kt_value, e.y, e.z, y, slot);
-    if (!P.active) return;
+    const double kty = bin8_sum(r, e.z, y, slot);
+    if (pad || !P.active) return;
     primal_epilogue<kDiag>(P, step, o, xj, cj, lo, hi, anc, kty, b);
   } else {
-    // A long row, summed in one round: every lane takes part in the row sum.
-    const int4 e = plan[short_waves * kG + (w - short_waves)];
+    // A long row: every lane takes part in its row sum.
+    const Int q = w - short_waves;
+    const size_t at32 = static_cast<size_t>(q) * kBinLongWidth + static_cast<size_t>(lane & 31);
+    const Int my_ix = index32[at32];
+    const double my_w = value32[at32];
+    const int4 e = plan[short_waves * kG + q];
     const Int j = e.x;
     const bool live = lane < span;
     const size_t o = at(j, lane);
@@ -333,7 +417,7 @@ __global__ void __launch_bounds__(64 * kWaves)
       hi = d.upper[o];
       anc = b.anchor[o];
     }
-    const double kty = lanes::round_row_sum(kt_index, kt_value, e.y, e.z, y, y_rows, lane, live);
+    const double kty = bin32_sum(my_ix, my_w, kt_index, kt_value, e.y, e.z, y, y_rows, lane, live);
     if (!live || !P.active) return;
     primal_epilogue<kDiag>(P, step, o, xj, cj, lo, hi, anc, kty, b);
   }
@@ -341,29 +425,39 @@ __global__ void __launch_bounds__(64 * kWaves)
 
 template <bool kDiag, int kWaves, int kG>
 __global__ void __launch_bounds__(64 * kWaves)
-    bdual_binned_kernel(const int4* __restrict__ plan, Int short_waves, Int units,
+    bdual_binned_kernel(const int4* __restrict__ plan, const Int* __restrict__ index8,
+                        const double* __restrict__ value8, const Int* __restrict__ index32,
+                        const double* __restrict__ value32, Int short_waves, Int units,
+                        int blocks, int step, Int span, Int x_rows,
                         const Int* __restrict__ k_index, const double* __restrict__ k_value,
-                        const double* __restrict__ x_bar, int blocks, int step, Int span,
-                        Int x_rows, const LaneParams* __restrict__ params, BatchData d,
-                        DualBufs b) {
+                        const double* __restrict__ x_bar, const LaneParams* __restrict__ params,
+                        BatchData d, DualBufs b) {
   constexpr int kSpan = 64 / kG;
   const int lane = static_cast<int>(threadIdx.x & 63);
   const Int w = lanes::wave_row_of<kWaves>(blocks);
   if (w >= units) return;  // the last workgroup's spare waves
   if (w < short_waves) {
     const int slot = lane % kSpan;
-    const int4 e = plan[w * kG + lane / kSpan];
-    const Int i = e.x;
-    if (i < 0) return;
-    const size_t o = at(i, slot);
+    const Int en = w * kG + lane / kSpan;
+    const Bin8 r = bin8_load(index8, value8, en);
+    const int4 e = plan[en];
     const LaneParams P = params[slot];
+    bin8_keep(r);
+    // A padding entry (row -1) computes on row 0 and stores nothing.
+    const bool pad = e.x < 0;
+    const Int i = pad ? 0 : e.x;
+    const size_t o = at(i, slot);
     const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
-    const double kxb = packed_range_sum(k_index, k_value, e.y, e.z, x_bar, slot);
-    if (!P.active) return;
+    const double kxb = bin8_sum(r, e.z, x_bar, slot);
+    if (pad || !P.active) return;
     dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
   } else {
-    // A long row, summed in one round: every lane takes part in the row sum.
-    const int4 e = plan[short_waves * kG + (w - short_waves)];
+    // A long row: every lane takes part in its row sum.
+    const Int q = w - short_waves;
+    const size_t at32 = static_cast<size_t>(q) * kBinLongWidth + static_cast<size_t>(lane & 31);
+    const Int my_ix = index32[at32];
+    const double my_w = value32[at32];
+    const int4 e = plan[short_waves * kG + q];
     const Int i = e.x;
     const bool live = lane < span;
     const size_t o = at(i, lane);
@@ -376,7 +470,8 @@ __global__ void __launch_bounds__(64 * kWaves)
       ay = b.anchor_y[o];
       ak = b.anchor_kx[o];
     }
-    const double kxb = lanes::round_row_sum(k_index, k_value, e.y, e.z, x_bar, x_rows, lane, live);
+    const double kxb =
+        bin32_sum(my_ix, my_w, k_index, k_value, e.y, e.z, x_bar, x_rows, lane, live);
     if (!live || !P.active) return;
     dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
   }
@@ -929,16 +1024,17 @@ auto dual_packed_by_waves(int waves) {
 // Rows per wave of the packed steps for a span (0: the span is too wide).
 int packed_rows(Int span) { return span <= 16 ? 4 : span <= 32 ? 2 : 0; }
 
-// How a narrow span runs over `a`: every row in packed waves when no row is
-// longer than one round of the packed sum; else, with the row bins, the
-// short rows packed and the long ones a wave each (packing K of the
-// refinery stacks, whose rows run to 30 entries, whole was measured slower:
-// a packed wave waits for its longest row); else one row per wave.
+// How a narrow span runs over `a`: with the row bins, the short rows packed
+// from their fixed-width copy and the long ones a wave each (packing K of
+// the refinery stacks, whose rows run to 30 entries, whole was measured
+// slower: a packed wave waits for its longest row); without them, every row
+// in packed waves from the CSR when no row is longer than one round of the
+// packed sum, else one row per wave.
 enum class Narrow { kNone, kPacked, kBinned };
 Narrow narrow_mode(const DevCsr& a, int g) {
   if (g == 0) return Narrow::kNone;
-  if (a.longest <= kBinShortMax) return Narrow::kPacked;
   if (a.bin_short[g == 4 ? 1 : 0] >= 0) return Narrow::kBinned;
+  if (a.longest <= kBinShortMax) return Narrow::kPacked;
   return Narrow::kNone;
 }
 
@@ -973,11 +1069,12 @@ void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const B
     const int p = g == 4 ? 1 : 0;
     const Int units = kt.bin_short[p] + kt.bin_long;
     const int blocks = static_cast<int>((units + waves - 1) / waves);
+    const BinArgs a = bin_args(kt, p);
     launch_rows_per(g == 4 ? primal_binned_by_waves<kDiag, 4>(waves)
                            : primal_binned_by_waves<kDiag, 2>(waves),
-                    waves, 1, units, s, ev, what, reinterpret_cast<const int4*>(kt.bin_plan[p]),
-                    kt.bin_short[p], units, kt.index, kt.value, y, blocks, step, span, kt.cols, params,
-                    d, b);
+                    waves, 1, units, s, ev, what, a.plan, a.index8, a.value8, a.index32, a.value32,
+                    kt.bin_short[p], units, blocks, step, span, kt.cols, kt.index, kt.value, y,
+                    params, d, b);
   } else if (mode == Narrow::kPacked) {
     const Int units = (kt.rows + g - 1) / g;
     const int blocks = static_cast<int>((units + waves - 1) / waves);
@@ -1011,11 +1108,12 @@ void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const 
     const int p = g == 4 ? 1 : 0;
     const Int units = k.bin_short[p] + k.bin_long;
     const int blocks = static_cast<int>((units + waves - 1) / waves);
+    const BinArgs a = bin_args(k, p);
     launch_rows_per(g == 4 ? dual_binned_by_waves<kDiag, 4>(waves)
                            : dual_binned_by_waves<kDiag, 2>(waves),
-                    waves, 1, units, s, ev, what, reinterpret_cast<const int4*>(k.bin_plan[p]),
-                    k.bin_short[p], units, k.index, k.value, x_bar, blocks, step, span, k.cols, params, d,
-                    b);
+                    waves, 1, units, s, ev, what, a.plan, a.index8, a.value8, a.index32, a.value32,
+                    k.bin_short[p], units, blocks, step, span, k.cols, k.index, k.value, x_bar,
+                    params, d, b);
   } else if (mode == Narrow::kPacked) {
     const Int units = (k.rows + g - 1) / g;
     const int blocks = static_cast<int>((units + waves - 1) / waves);

@@ -82,11 +82,23 @@ struct DevCsr {
   const Int* bin_plan[2] = {nullptr, nullptr};
   Int bin_short[2] = {-1, -1};
   Int bin_long = 0;
+  // Fixed-width copies of the binned rows, addressed by plan entry so their
+  // loads need not wait for the plan's: bin_index8/bin_value8 hold short
+  // entry e's entries at [e * 8, e * 8 + 8) (the short entries are the same
+  // rows in the same order in both plans), bin_index32/bin_value32 long row
+  // q's first kBinLongWidth at [q * 32, q * 32 + 32). Padding repeats the
+  // row's last index with value 0 (index 0 for a padding entry).
+  const Int* bin_index8 = nullptr;
+  const double* bin_value8 = nullptr;
+  const Int* bin_index32 = nullptr;
+  const double* bin_value32 = nullptr;
 };
 
 // The longest row the packed steps take several to a wave: one round of
 // their sum.
 inline constexpr Int kBinShortMax = 8;
+// Entries of a long row in its fixed-width copy (DevCsr::bin_index32).
+inline constexpr Int kBinLongWidth = 32;
 
 // Widths of the fixed-width copy: the smallest that holds every row, or the
 // widest with longer rows continuing in the CSR.

@@ -219,16 +219,15 @@ __device__ __forceinline__ void slot_gather(const SlotArrays& t, Int row,
 // outside it, which reads no memory and yields 0. Each lane adds its terms
 // in storage order, a position past the length adding +0.0 (which leaves a
 // sum that started at +0.0 unchanged), so the sum has the bits of the serial
-// loop. v_rows * 512 must stay below 2^31.
+// loop. v_rows * 512 must stay below 2^31. round_sum_loaded is the part
+// after the load, for a caller that loads the entries itself: lane u holds
+// entry u (for u < len) in my_ix and my_wd.
 template <int kMax>
-__device__ __forceinline__ double round_sum(const Int* __restrict__ index,
-                                           const double* __restrict__ value, Int lo, Int len,
-                                           const double* __restrict__ v, Int v_rows, int lane,
-                                           bool gather) {
+__device__ __forceinline__ double round_sum_loaded(Int my_ix, double my_wd, Int len,
+                                                  const double* __restrict__ v, Int v_rows,
+                                                  int lane, bool gather) {
   static_assert(kMax >= 1 && kMax <= kLaneCount, "one entry per lane");
-  const Int mine = lo + (lane < len ? lane : len - 1);
-  const int my_ix = index[mine];
-  const unsigned long long my_w = __builtin_bit_cast(unsigned long long, value[mine]);
+  const unsigned long long my_w = __builtin_bit_cast(unsigned long long, my_wd);
   const int w_lo = static_cast<int>(static_cast<unsigned>(my_w));
   const int w_hi = static_cast<int>(static_cast<unsigned>(my_w >> 32));
   const __amdgpu_buffer_rsrc_t rs = buffer_over(v, v_rows * (kLaneCount * 8));
@@ -259,6 +258,15 @@ __device__ __forceinline__ double round_sum(const Int* __restrict__ index,
     s += t < len ? p : 0.0;
   }
   return s;
+}
+
+template <int kMax>
+__device__ __forceinline__ double round_sum(const Int* __restrict__ index,
+                                           const double* __restrict__ value, Int lo, Int len,
+                                           const double* __restrict__ v, Int v_rows, int lane,
+                                           bool gather) {
+  const Int mine = lo + (lane < len ? lane : len - 1);
+  return round_sum_loaded<kMax>(index[mine], value[mine], len, v, v_rows, lane, gather);
 }
 
 // The longest row round_row_sum takes in one round.

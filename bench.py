@@ -328,13 +328,34 @@ def c1_section(ctx, args):
                                         "achieved_GBs": round(row["gbs"], 1),
                                         "frac": round(row["gbs"] / HBM_PEAK_GBS, 4),
                                         "launches": row["timed"]}
+    # Control: K's own index stream with one entry per row, so the product
+    # makes exactly K x's gathers of x, in the same order, with no row sums
+    # (and writes one output per entry). Its time is what the gathers alone
+    # cost on this pattern.
+    one = Matrix(lp.nnz, lp.cols, np.arange(lp.nnz + 1, dtype=np.int32), e["index"], e["value"])
+    yg = ctx.allocate(lp.nnz)
+    ctx.prepare(one)
+    ctx.multiply(one, x, yg)
+    if not args.no_profile:
+        ctx.set_profiling(True)
+        for _ in range(args.c1_reps):
+            ctx.multiply(one, x, yg)
+        row = parse_profile(ctx.profile_report())["spmv"]
+        ctx.set_profiling(False)
+        spmv["K x, gather control"] = {
+            "mean_us": row["mean_us"], "launches": row["timed"],
+            "gathers_per_s": round(lp.nnz / (1e-6 * row["mean_us"]), 0),
+            "what": "K's index stream as one entry per row: the same gathers of x, no row sums, "
+                    "8 B written per entry"}
     else:
         # Launched plainly, so an external profiler sees the same products.
         for _ in range(args.c1_reps):
             ctx.multiply(k, x, y)
             ctx.multiply_transpose(kt, yv, xv)
             ctx.multiply(band, x, y)
+            ctx.multiply(one, x, yg)
         ctx.synchronize()
+    yg.free()
     host = None
     ref = None if args.no_cpu_baseline else ref_oracle()
     if ref is not None:

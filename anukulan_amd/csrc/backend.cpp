@@ -37,6 +37,7 @@ HipBackend::HipBackend(int device) : device_(device) {
 
 HipBackend::~HipBackend() {
   // Teardown: failures here have nowhere to go, so their statuses are dropped.
+  helper_.reset();
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   clear_profile();
@@ -52,6 +53,11 @@ HipBackend::~HipBackend() {
 }
 
 void HipBackend::synchronize() { hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+HipBackend& HipBackend::helper() {
+  if (!helper_) helper_ = std::make_unique<HipBackend>(device_);
+  return *helper_;
+}
 
 hipEvent_t HipBackend::take_event() {
   if (!event_pool_.empty()) {

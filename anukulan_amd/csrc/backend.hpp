@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -49,6 +50,11 @@ class HipBackend {
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
   void synchronize();
+  // A second backend on the same device, with its own stream, memory and
+  // matrix cache, made on first use and kept for this one's life: work that
+  // runs on another host thread beside this backend's (the batched solver's
+  // feasibility polishing) goes through it.
+  HipBackend& helper();
 
   // memory (backend.hpp:53-58)
   double* allocate(Int n);
@@ -189,6 +195,7 @@ class HipBackend {
 
   int device_ = 0;
   hipStream_t stream_ = nullptr;
+  std::unique_ptr<HipBackend> helper_;
   std::unordered_map<std::uint64_t, DeviceMatrix> matrices_;
   std::map<int, std::pair<double*, std::size_t>> scratch_;
   std::map<int, std::pair<double*, std::size_t>> pinned_slots_;

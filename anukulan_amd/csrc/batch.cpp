@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <future>
 #include <iterator>
 #include <map>
 #include <memory>
@@ -58,6 +59,12 @@ struct LaneOutcome {
   Int polish_attempts = 0, polish_iterations = 0;
 };
 
+// One polished pair per seed, unscaled, with the iterations it cost.
+struct Polished {
+  std::vector<double> x, y;
+  Int spent = 0;
+};
+
 // What a polishing attempt needs from the enclosing batch; absent in
 // sub-batches, which never polish.
 struct PolishContext {
@@ -92,8 +99,22 @@ class Engine {
   void wait_published();
   void retire_extract(Mask mask, std::vector<LaneOutcome>& out);
   void run_restarts(Mask mask);
-  void run_polish(const std::vector<int>& who, Int budget, std::vector<LaneOutcome>& out,
-                  std::vector<bool>* stopped, Int iteration);
+  // Feasibility polishing of the main batch runs beside its periods: an
+  // attempt takes its seeds at its check, runs its sub-batches through the
+  // backend's helper on another host thread, and is applied once it is back
+  // (settle_polish). A failed attempt changes nothing (the lane went on from
+  // the check, as it does after a failed attempt); a successful one retires
+  // the lane as of that check, and the periods it ran since are discarded.
+  struct PendingPolish {
+    std::vector<int> lanes;         // in the slot order of the seed extract
+    Int iteration = 0;              // index of the check that started it
+    std::vector<Control> at_check;  // each lane's control as that check left it
+    std::future<std::vector<Polished>> result;
+  };
+  void start_polish(const std::vector<int>& who, Int budget, Int iteration);
+  // Applies the attempts that are back (every one, waiting, with `wait`).
+  void settle_polish(std::vector<LaneOutcome>& out, bool wait);
+  void apply_polish(PendingPolish& p, std::vector<LaneOutcome>& out);
   void compact();
   void tune();
   // The device slots of a set of lanes.
@@ -154,6 +175,9 @@ class Engine {
   Int* flag_ = nullptr;       // the last period whose check reached pinned_
   Int seq_ = 0;               // the last period staged
   std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
+  std::vector<PendingPolish> pending_;
+  Mask polishing_ = 0;  // lanes with an attempt in flight
+  Mask settled_ = 0;    // lanes whose outcome a successful attempt fixed
 };
 
 Engine::~Engine() {
@@ -487,11 +511,6 @@ std::vector<LaneOutcome> run_engine(HipBackend& be, const Csr& k, const Csr& kt,
   return e.run();
 }
 
-// One polished pair per seed, unscaled, with the iterations it cost.
-struct Polished {
-  std::vector<double> x, y;
-  Int spent = 0;
-};
 
 // The two feasibility sub-problems of each seed, as one sub-batch each
 // (pdhg.cpp:680-743): primal with c = 0 started from (x, 0), dual with q = 0
@@ -587,9 +606,7 @@ std::vector<Polished> polish_lanes(HipBackend& be, const Csr& k, const Csr& kt, 
 // A polish attempt for the lanes `who` at a check: adopted, and the lane
 // stopped as optimal, when the polished pair passes the full test on the
 // lane's original LP.
-void Engine::run_polish(const std::vector<int>& who, Int budget, std::vector<LaneOutcome>& out,
-                        std::vector<bool>* stopped, Int iteration) {
-  (void)iteration;
+void Engine::start_polish(const std::vector<int>& who, Int budget, Int iteration) {
   // The extract comes out in slot order, so the lanes are taken in that order.
   std::vector<int> order(who);
   std::sort(order.begin(), order.end(), [&](int a, int b) { return slot_of_[a] < slot_of_[b]; });
@@ -605,39 +622,79 @@ void Engine::run_polish(const std::vector<int>& who, Int budget, std::vector<Lan
   std::vector<const Lane*> src;
   std::vector<std::vector<double>> sx, sy;
   std::vector<double> omega;
+  PendingPolish pp;
+  pp.lanes = order;
+  pp.iteration = iteration;
   for (int s = 0; s < cnt; ++s) {
     const int l = order[static_cast<size_t>(s)];
     src.push_back(&lanes_[static_cast<size_t>(l)]);
     sx.emplace_back(xs.begin() + static_cast<long>(s) * n_, xs.begin() + static_cast<long>(s + 1) * n_);
     sy.emplace_back(ys.begin() + static_cast<long>(s) * m_, ys.begin() + static_cast<long>(s + 1) * m_);
     omega.push_back(ctl_[static_cast<size_t>(l)].omega);
+    pp.at_check.push_back(ctl_[static_cast<size_t>(l)]);
+    polishing_ |= 1ull << l;
   }
-  std::vector<Polished> pol = polish_lanes(be_, k_, kt_, neq_, norm_, opt_, src, sx, sy, omega,
-                                           ctl_[static_cast<size_t>(who[0])].eta, budget,
-                                           *polish_->scaling, start_);
-  std::vector<Residual> trs(static_cast<size_t>(cnt));
-  HostPool::shared().run(static_cast<size_t>(cnt), [&](size_t s) {
-    const int l = order[s];
+  const double eta = ctl_[static_cast<size_t>(who[0])].eta;
+  HipBackend* helper = &be_.helper();
+  const int device = be_.device();
+  // Everything the task reads is the engine's and outlives it: the engine
+  // waits for every attempt before it returns (and its futures wait in its
+  // destructor).
+  pp.result = std::async(std::launch::async, [this, helper, device, eta, budget,
+                                              src = std::move(src), sx = std::move(sx),
+                                              sy = std::move(sy), omega = std::move(omega)]() {
+    hip_check(hipSetDevice(device), "hipSetDevice(polish)");
+    return polish_lanes(*helper, k_, kt_, neq_, norm_, opt_, src, sx, sy, omega, eta, budget,
+                        *polish_->scaling, start_);
+  });
+  pending_.push_back(std::move(pp));
+}
+
+void Engine::apply_polish(PendingPolish& pp, std::vector<LaneOutcome>& out) {
+  std::vector<Polished> pol = pp.result.get();
+  const size_t cnt = pp.lanes.size();
+  std::vector<Residual> trs(cnt);
+  HostPool::shared().run(cnt, [&](size_t s) {
+    const int l = pp.lanes[s];
     trs[s] = evaluate_residual(*(*polish_->originals)[static_cast<size_t>(l)], pol[s].x, pol[s].y);
   });
-  for (int s = 0; s < cnt; ++s) {
-    const int l = order[static_cast<size_t>(s)];
+  for (size_t s = 0; s < cnt; ++s) {
+    const int l = pp.lanes[s];
+    const Mask bit = 1ull << l;
     LaneOutcome& o = out[static_cast<size_t>(l)];
-    std::vector<double>& tx = pol[static_cast<size_t>(s)].x;
-    std::vector<double>& ty = pol[static_cast<size_t>(s)].y;
-    const Residual& tr = trs[static_cast<size_t>(s)];
     ++o.polish_attempts;
-    o.polish_iterations += pol[static_cast<size_t>(s)].spent;
-    Control& c = ctl_[static_cast<size_t>(l)];
-    if (tr.converged(c.tol, opt_.absolute_tolerance, opt_.require_inf_norm_termination != 0,
-                     c.gap_tol)) {
-      o.x = std::move(tx);
-      o.y = std::move(ty);
-      o.polished = true;
-      o.polished_r = tr;
-      c.status = ANUK_PDHG_OPTIMAL;
-      c.message = "converged after feasibility polishing";
-      (*stopped)[static_cast<size_t>(l)] = true;
+    o.polish_iterations += pol[s].spent;
+    polishing_ &= ~bit;
+    const Control& c = pp.at_check[s];
+    const Residual& tr = trs[s];
+    if (!tr.converged(c.tol, opt_.absolute_tolerance, opt_.require_inf_norm_termination != 0,
+                      c.gap_tol))
+      continue;
+    // Converged: the lane's outcome is the polished pair as of the check that
+    // started the attempt, whatever the lane did since.
+    o.x = std::move(pol[s].x);
+    o.y = std::move(pol[s].y);
+    o.polished = true;
+    o.polished_r = tr;
+    o.status = ANUK_PDHG_OPTIMAL;
+    o.message = "converged after feasibility polishing";
+    o.iterations = pp.iteration + 1;
+    o.seconds = seconds_since(start_);
+    o.restarts = c.restarts;
+    o.eta = c.eta;
+    o.omega = c.omega;
+    settled_ |= bit;
+    active_ &= ~bit;
+  }
+}
+
+void Engine::settle_polish(std::vector<LaneOutcome>& out, bool wait) {
+  for (auto it = pending_.begin(); it != pending_.end();) {
+    if (wait || it->result.wait_for(std::chrono::seconds(0)) == std::future_status::ready) {
+      apply_polish(*it, out);
+      it = pending_.erase(it);
+    } else {
+      ++it;
     }
   }
 }
@@ -916,6 +973,7 @@ std::vector<LaneOutcome> Engine::run() {
       fetch(CHECK_, h_check, static_cast<size_t>(64) * kLaneCheckStride);
     }
     lap(&Phases::iterating);
+    if (!pending_.empty()) settle_polish(out, false);
     const Int i = period_end - 1;
     const double elapsed = seconds_since(start_);
 
@@ -956,9 +1014,19 @@ std::vector<LaneOutcome> Engine::run() {
       }
     }
     lap(&Phases::checks);
-    for (const auto& kv : polish_by_budget) {
-      if (polish_ == nullptr) continue;  // sub-batches never polish
-      run_polish(kv.second, kv.first, out, &stopped, i);
+    if (polish_ != nullptr && !polish_by_budget.empty()) {  // sub-batches never polish
+      // A lane due another attempt while one is in flight has that one
+      // settled first (it may have succeeded, which retires the lane).
+      Mask due = 0;
+      for (const auto& kv : polish_by_budget)
+        for (int l : kv.second) due |= 1ull << l;
+      if ((due & polishing_) != 0) settle_polish(out, true);
+      for (const auto& kv : polish_by_budget) {
+        std::vector<int> who;
+        for (int l : kv.second)
+          if ((active_ >> l) & 1ull) who.push_back(l);
+        if (!who.empty()) start_polish(who, kv.first, i);
+      }
     }
     lap(&Phases::polishing);
     Mask restart = 0;
@@ -974,7 +1042,7 @@ std::vector<LaneOutcome> Engine::run() {
     }
     Mask retiring = 0;
     for (int l = 0; l < count_; ++l) {
-      if (!stopped[static_cast<size_t>(l)]) continue;
+      if (!stopped[static_cast<size_t>(l)] || ((settled_ >> l) & 1ull)) continue;
       retiring |= 1ull << l;
       LaneOutcome& o = out[static_cast<size_t>(l)];
       o.iterations = i + 1;
@@ -983,6 +1051,9 @@ std::vector<LaneOutcome> Engine::run() {
     retire_extract(retiring, out);
     active_ &= ~retiring;
     run_restarts(restart & active_);
+    // Every live lane waiting on an attempt: settle rather than run periods
+    // that a success would discard.
+    if (polishing_ != 0 && (active_ & ~polishing_) == 0) settle_polish(out, true);
     compact();
     lap(&Phases::checks);
   }
@@ -995,6 +1066,7 @@ std::vector<LaneOutcome> Engine::run() {
   }
   retire_extract(rest, out);
   for (int l = 0; l < count_; ++l) {
+    if ((settled_ >> l) & 1ull) continue;
     LaneOutcome& o = out[static_cast<size_t>(l)];
     const Control& c = ctl_[static_cast<size_t>(l)];
     o.status = c.status;
@@ -1003,6 +1075,9 @@ std::vector<LaneOutcome> Engine::run() {
     o.eta = c.eta;
     o.omega = c.omega;
   }
+  // Attempts still in flight decide last: a success stands for its lane
+  // whatever the lane did after the attempt's check.
+  settle_polish(out, true);
   return out;
 }
 

@@ -97,13 +97,19 @@ def stack_fixture():
                                        "pdhg_reference.json")))["stack24_cases"]
 
 
-def check_against_reference(got, want, label):
+def check_against_reference(got, want, label, counts=False):
     """Per case: the reference's status and polish outcome, the objective
-    within 1e-8 relative, the iteration count within 2%."""
+    within 1e-8 relative, the iteration count within 2%; with `counts`, also
+    exactly the reference's restarts, polish attempts and polish iterations
+    (the batched solver polishes beside its main periods, and must still take
+    each case's polishing decisions as the reference does)."""
     same = 0
     for i, (g, w) in enumerate(zip(got, want)):
         assert g.status == w["status"], (label, i, g.status, w["status"])
         assert g.polished == w["polished"], (label, i)
+        if counts:
+            assert (g.restarts, g.polish_attempts, g.polish_iterations) == (
+                w["restarts"], w["polish_attempts"], w["polish_iterations"]), (label, i)
         assert relerr(g.objective, w["objective"]) <= 1e-8, (label, i, g.objective, w["objective"])
         assert abs(g.iterations - w["iterations"]) <= 0.02 * w["iterations"], (
             label, i, g.iterations, w["iterations"])
@@ -122,7 +128,7 @@ def test_full_stack_per_case_against_the_reference(ctx, tol):
     base, cases, models = stack(24)
     assert len(models) == fx["cases"] == 61
     got = solve_pdhg_batch(ctx, [m.to_lp() for m in models], tolerance=float(tol))
-    check_against_reference(got, fx[tol], f"batched {tol}")
+    check_against_reference(got, fx[tol], f"batched {tol}", counts=True)
 
 
 def test_full_stack_plain_per_case_against_the_reference(ctx):

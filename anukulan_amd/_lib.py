@@ -81,6 +81,7 @@ class PdhgOptions(ctypes.Structure):
         ("stop", IP), ("use_graphs", c_int32), ("product_order", c_int32), ("autotune", c_int32),
         ("batch_waves", c_int32), ("batch_rows", c_int32), ("batch_layout", c_int32),
         ("batch_pack", c_int32), ("multi_order", c_int32), ("multi_stream_loads", c_int32),
+        ("small_lp_engine", c_int32),
     ]
 
 

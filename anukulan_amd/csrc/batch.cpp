@@ -1174,8 +1174,11 @@ std::vector<SolveResult> solve_pdhg_batch(HipBackend& be, const std::vector<cons
 
   std::vector<SolveResult> results(lps.size());
   if (norm <= 0.0) {
-    // No coupling: every LP gets its plain solve's componentwise answer.
-    for (size_t i = 0; i < lps.size(); ++i) results[i] = solve_pdhg(be, *lps[i], opt);
+    // No coupling: every LP gets its plain solve's componentwise answer
+    // (from the plain kernels, which a batch of one would route back here).
+    anuk_pdhg_options plain = opt;
+    plain.small_lp_engine = 1;
+    for (size_t i = 0; i < lps.size(); ++i) results[i] = solve_pdhg(be, *lps[i], plain);
     return results;
   }
 

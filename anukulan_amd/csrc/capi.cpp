@@ -556,6 +556,7 @@ anuk_pdhg_options checked_options(const anuk_pdhg_options* options) {
        "multi_order must be 0, 1 or 2");
   need(o.multi_stream_loads == 0 || o.multi_stream_loads == 1 || o.multi_stream_loads == 2,
        "multi_stream_loads must be 0, 1 or 2");
+  need(o.small_lp_engine == 0 || o.small_lp_engine == 1, "small_lp_engine must be 0 or 1");
   return o;
 }
 

@@ -367,6 +367,13 @@ typedef struct anuk_pdhg_options {
                                           keeps the fastest when it beats the stack's
                                           order with plain loads by 3%; every choice
                                           computes the same bits */
+  int32_t small_lp_engine;             /* anuk_solve_pdhg: 0: an LP of at most 30,000
+                                          nonzeros whose rows of K and K^T hold at most 32
+                                          entries, with options the batched solver runs,
+                                          is solved as a batch of one (its launches give
+                                          each row a wave: 1.15-1.4x faster on such LPs,
+                                          with the same decisions); 1: the plain fused
+                                          kernels always */
 } anuk_pdhg_options;
 
 void anuk_pdhg_default_options(anuk_pdhg_options* options);

@@ -51,7 +51,7 @@ def test_batch_matches_plain_gpu_solves(ctx, tol):
     lps = [m.to_lp() for m in models]
     got = solve_pdhg_batch(ctx, lps, tolerance=tol)
     for lp, g in zip(lps, got):
-        p = solve_pdhg(ctx, lp, tolerance=tol)
+        p = solve_pdhg(ctx, lp, tolerance=tol, small_lp_engine=1)  # the plain fused kernels
         assert g.status == p.status == "optimal"
         assert relerr(g.objective, p.objective) <= 1e-8
         assert abs(g.iterations - p.iterations) <= 0.02 * p.iterations

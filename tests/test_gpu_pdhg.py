@@ -30,12 +30,18 @@ def known_lp(name):
     return natural(name).to_lp()
 
 
+# small_lp_engine 0 solves these small LPs as batches of one; 1 keeps the
+# plain fused kernels. Both must meet the same bar.
+ENGINES = pytest.mark.parametrize("engine", [0, 1], ids=["small-lp", "plain-kernels"])
+
+
+@ENGINES
 @pytest.mark.parametrize("name", ["wyndor", "equalities", "boxed_free", "ranged", "degenerate",
                                   "offset", "empty_matrix", "norm_diag"])
-def test_known_optimum(ctx, name):
+def test_known_optimum(ctx, name, engine):
     from golden.known_lps import KNOWN
     d = KNOWN[name]
-    r = solve_pdhg(ctx, known_lp(name), tolerance=1e-8)
+    r = solve_pdhg(ctx, known_lp(name), tolerance=1e-8, small_lp_engine=engine)
     assert r.status == "optimal", r
     assert abs(r.objective - d["optimum"]) <= 1e-6 * max(1.0, abs(d["optimum"]))
     if d["x"] is not None:
@@ -57,13 +63,14 @@ def test_matrix_norm_of_diag_3_4(ctx):
     assert abs(estimate_matrix_norm(ctx, known_lp("norm_diag"), 500, 1e-12) - 4.0) <= 1e-6
 
 
+@ENGINES
 @pytest.mark.parametrize("periods", [4, 12])
 @pytest.mark.parametrize("tol", ["1e-4", "1e-8"])
-def test_refinery_matches_reference(ctx, periods, tol):
+def test_refinery_matches_reference(ctx, periods, tol, engine):
     fx = FIX["refinery"][str(periods)]
     lp = G.refinery_model(periods).to_lp()
     assert (lp.rows, lp.cols, lp.nnz) == (fx["rows"], fx["cols"], fx["nnz"])
-    r = solve_pdhg(ctx, lp, tolerance=float(tol))
+    r = solve_pdhg(ctx, lp, tolerance=float(tol), small_lp_engine=engine)
     ref = fx[tol]
     print(f"\nrefinery{periods} tol {tol}: gpu {r.iterations} it obj {r.objective:.12e} | "
           f"ref {ref['iterations']} it obj {ref['objective']:.12e} | simplex "
@@ -77,11 +84,12 @@ def test_refinery_matches_reference(ctx, periods, tol):
     assert err <= (1e-3 if tol == "1e-4" else 1e-7)
 
 
-def test_graph_replay_is_bit_identical_to_direct_launches(ctx):
+@ENGINES
+def test_graph_replay_is_bit_identical_to_direct_launches(ctx, engine):
     lp = G.refinery_model(4).to_lp()
-    a = solve_pdhg(ctx, lp, tolerance=1e-6, use_graphs=1)
-    b = solve_pdhg(ctx, lp, tolerance=1e-6, use_graphs=0)
-    c = solve_pdhg(ctx, lp, tolerance=1e-6, use_graphs=1)
+    a = solve_pdhg(ctx, lp, tolerance=1e-6, use_graphs=1, small_lp_engine=engine)
+    b = solve_pdhg(ctx, lp, tolerance=1e-6, use_graphs=0, small_lp_engine=engine)
+    c = solve_pdhg(ctx, lp, tolerance=1e-6, use_graphs=1, small_lp_engine=engine)
     assert a.iterations == b.iterations == c.iterations
     assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
     assert np.array_equal(a.x, c.x)

@@ -1051,9 +1051,8 @@ std::vector<LaneOutcome> Engine::run() {
     retire_extract(retiring, out);
     active_ &= ~retiring;
     run_restarts(restart & active_);
-    // Every live lane waiting on an attempt: settle rather than run periods
-    // that a success would discard.
-    if (polishing_ != 0 && (active_ & ~polishing_) == 0) settle_polish(out, true);
+    // Lanes waiting on an attempt keep iterating: that costs nothing while
+    // the attempt runs, and a failed attempt leaves them that much further.
     compact();
     lap(&Phases::checks);
   }

@@ -159,9 +159,9 @@ class Engine {
   int pw_ = 2, dw_ = 2;
   int pr_ = 1, dr_ = 1;
   bool ps_ = false, ds_ = false;
-  // Waves per workgroup of the packed and binned launches of narrow spans.
+  // Waves per workgroup of the binned launches of narrow spans.
   int pnw_ = 1, dnw_ = 1;
-  // Narrow spans (32 live slots or fewer) run the packed step kernels.
+  // Narrow spans (32 live slots or fewer) run the binned step kernels.
   bool pack_ = true;
 
   double* slab_ = nullptr;
@@ -714,7 +714,7 @@ std::uint64_t pattern_key(const Csr& a) {
 // The launch-shape tuner: times the primal and dual step kernels in every
 // shape (1, 2, 4 or 8 waves per workgroup; 1, 2 or 4 consecutive rows per
 // wave from the CSR, or one row per wave from the fixed-width row copy) on
-// this stack's K^T and K, and the packed and binned launches of a narrow
+// this stack's K^T and K, and the binned launches of a narrow
 // span (16 slots) with 1, 2, 4 or 8 waves per workgroup, and keeps, per
 // sparsity pattern for the context's life, the fastest of each, if it beats
 // the default by at least kTuneMargin;

@@ -202,97 +202,11 @@ __global__ void __launch_bounds__(64 * kWaves)
   dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
 }
 
-// Packed steps for a narrow span: with every live lane in the lowest
-// kSpan = 64 / kG slots (compaction keeps them there), one wave takes kG
-// consecutive rows, lane (r, s) working on row j0 + r for slot s. Each lane
-// sums its row's entries in storage order and runs the one-row kernel's
-// epilogue, so the bits are the same; the launch has kG times fewer waves,
-// and it is the per-wave cost that a step pays once few lanes are live. As
-// the rows of a wave differ, the row structure comes in vector loads. A
-// round takes eight entries per lane with no branch: a position past the
-// lane's row reads the row's first entry instead (position 0 for an empty
-// row; the matrix has entries), so all the index and value loads go out
-// together and then all the gathers; the extra term adds +0.0, which leaves
-// a sum that started at +0.0 unchanged, so the sum has the serial loop's
-// bits.
-__device__ __forceinline__ double packed_range_sum(const Int* __restrict__ index,
-                                                   const double* __restrict__ value, Int lo,
-                                                   Int len, const double* __restrict__ v,
-                                                   int slot) {
-  const Int safe = len > 0 ? lo : 0;
-  double s = 0.0;
-  for (Int u0 = 0; __any(u0 < len); u0 += 8) {
-    Int p[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) p[u] = u0 + u < len ? lo + u0 + u : safe;
-    Int ix[8];
-    double w[8], x[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      ix[u] = index[p[u]];
-      w[u] = value[p[u]];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) x[u] = v[at(ix[u], slot)];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const double t = w[u] * x[u];
-      s += u0 + u < len ? t : 0.0;
-    }
-  }
-  return s;
-}
-
-__device__ __forceinline__ double packed_row_sum(const Int* __restrict__ start,
-                                                 const Int* __restrict__ index,
-                                                 const double* __restrict__ value, Int row,
-                                                 const double* __restrict__ v, int slot) {
-  const Int lo = start[row];
-  return packed_range_sum(index, value, lo, start[row + 1] - lo, v, slot);
-}
-
-template <bool kDiag, int kWaves, int kG>
-__global__ void __launch_bounds__(64 * kWaves)
-    bprimal_packed_kernel(const Int* __restrict__ kt_start, const Int* __restrict__ kt_index,
-                          const double* __restrict__ kt_value, const double* __restrict__ y,
-                          Int rows, int blocks, int step, const LaneParams* __restrict__ params,
-                          BatchData d, PrimalBufs b) {
-  constexpr int kSpan = 64 / kG;
-  const int lane = static_cast<int>(threadIdx.x & 63);
-  const int slot = lane % kSpan;
-  const Int j = lanes::wave_row_of<kWaves>(blocks) * kG + lane / kSpan;
-  if (j >= rows) return;
-  const size_t o = at(j, slot);
-  const LaneParams P = params[slot];
-  const double xj = b.x[o], cj = d.c[o], lo = d.lower[o], hi = d.upper[o], anc = b.anchor[o];
-  const double kty = packed_row_sum(kt_start, kt_index, kt_value, j, y, slot);
-  if (!P.active) return;
-  primal_epilogue<kDiag>(P, step, o, xj, cj, lo, hi, anc, kty, b);
-}
-
-template <bool kDiag, int kWaves, int kG>
-__global__ void __launch_bounds__(64 * kWaves)
-    bdual_packed_kernel(const Int* __restrict__ k_start, const Int* __restrict__ k_index,
-                        const double* __restrict__ k_value, const double* __restrict__ x_bar,
-                        Int rows, int blocks, int step, const LaneParams* __restrict__ params,
-                        BatchData d, DualBufs b) {
-  constexpr int kSpan = 64 / kG;
-  const int lane = static_cast<int>(threadIdx.x & 63);
-  const int slot = lane % kSpan;
-  const Int i = lanes::wave_row_of<kWaves>(blocks) * kG + lane / kSpan;
-  if (i >= rows) return;
-  const size_t o = at(i, slot);
-  const LaneParams P = params[slot];
-  const double yi = b.y[o], kx = b.k_x[o], qi = d.q[o], ay = b.anchor_y[o], ak = b.anchor_kx[o];
-  const double kxb = packed_row_sum(k_start, k_index, k_value, i, x_bar, slot);
-  if (!P.active) return;
-  dual_epilogue<kDiag>(P, step, i, d.num_equalities, o, yi, kx, qi, ay, ak, kxb, b);
-}
-
 // Binned steps for a narrow span: the rows are binned by length
 // (DevCsr::bin_plan). The first short_waves waves each take kG rows of at
-// most kBinShortMax entries, a lane per (row, slot) as in the packed steps;
-// every longer row then gets a wave of its own, summed in one round
+// most kBinShortMax entries, a lane per (row, slot): slot s of the live
+// lanes, kept in the lowest 64 / kG slots by compaction, on row r; every
+// longer row then gets a wave of its own, summed in one round
 // (lanes::round_sum_loaded: the row's entries in one vector load, all its
 // gathers at once). A lane reads its row's {row, start, length} from the
 // plan and, in parallel, the row's entries from their fixed-width copy
@@ -1002,39 +916,19 @@ auto dual_rows_by_waves(int waves) {
   }
 }
 
-// The packed instance for (waves, rows per wave kG).
-template <bool kDiag, int kG>
-auto primal_packed_by_waves(int waves) {
-  switch (waves) {
-    case 1: return &bprimal_packed_kernel<kDiag, 1, kG>;
-    case 2: return &bprimal_packed_kernel<kDiag, 2, kG>;
-    case 8: return &bprimal_packed_kernel<kDiag, 8, kG>;
-    default: return &bprimal_packed_kernel<kDiag, 4, kG>;
-  }
-}
-template <bool kDiag, int kG>
-auto dual_packed_by_waves(int waves) {
-  switch (waves) {
-    case 1: return &bdual_packed_kernel<kDiag, 1, kG>;
-    case 2: return &bdual_packed_kernel<kDiag, 2, kG>;
-    case 8: return &bdual_packed_kernel<kDiag, 8, kG>;
-    default: return &bdual_packed_kernel<kDiag, 4, kG>;
-  }
-}
-// Rows per wave of the packed steps for a span (0: the span is too wide).
+// Rows per wave of the binned steps' short rows for a span (0: the span is
+// too wide).
 int packed_rows(Int span) { return span <= 16 ? 4 : span <= 32 ? 2 : 0; }
 
-// How a narrow span runs over `a`: with the row bins, the short rows packed
-// from their fixed-width copy and the long ones a wave each (packing K of
-// the refinery stacks, whose rows run to 30 entries, whole was measured
-// slower: a packed wave waits for its longest row); without them, every row
-// in packed waves from the CSR when no row is longer than one round of the
-// packed sum, else one row per wave.
-enum class Narrow { kNone, kPacked, kBinned };
+// How a narrow span runs over `a`: with the row bins, the short rows two or
+// four to a wave from their fixed-width copy and the long ones a wave each
+// (packing K of the refinery stacks, whose rows run to 30 entries, whole was
+// measured slower: a packed wave waits for its longest row); without them,
+// one row per wave.
+enum class Narrow { kNone, kBinned };
 Narrow narrow_mode(const DevCsr& a, int g) {
   if (g == 0) return Narrow::kNone;
   if (a.bin_short[g == 4 ? 1 : 0] >= 0) return Narrow::kBinned;
-  if (a.longest <= kBinShortMax) return Narrow::kPacked;
   return Narrow::kNone;
 }
 
@@ -1075,13 +969,6 @@ void primal_step_shape(hipStream_t s, const DevCsr& kt, const double* y, const B
                     waves, 1, units, s, ev, what, a.plan, a.index8, a.value8, a.index32, a.value32,
                     kt.bin_short[p], units, blocks, step, span, kt.cols, kt.index, kt.value, y,
                     params, d, b);
-  } else if (mode == Narrow::kPacked) {
-    const Int units = (kt.rows + g - 1) / g;
-    const int blocks = static_cast<int>((units + waves - 1) / waves);
-    launch_rows_per(g == 4 ? primal_packed_by_waves<kDiag, 4>(waves)
-                           : primal_packed_by_waves<kDiag, 2>(waves),
-                    waves, g, kt.rows, s, ev, what, kt.start, kt.index, kt.value, y, kt.rows, blocks,
-                    step, params, d, b);
   } else if (rows == 4) {
     launch_rows_per(primal_rows_by_waves<kDiag, 4>(waves), waves, 4, kt.rows, s, ev, what, kt, y,
                     d, b, params, step, span);
@@ -1114,13 +1001,6 @@ void dual_step_shape(hipStream_t s, const DevCsr& k, const double* x_bar, const 
                     waves, 1, units, s, ev, what, a.plan, a.index8, a.value8, a.index32, a.value32,
                     k.bin_short[p], units, blocks, step, span, k.cols, k.index, k.value, x_bar,
                     params, d, b);
-  } else if (mode == Narrow::kPacked) {
-    const Int units = (k.rows + g - 1) / g;
-    const int blocks = static_cast<int>((units + waves - 1) / waves);
-    launch_rows_per(g == 4 ? dual_packed_by_waves<kDiag, 4>(waves)
-                           : dual_packed_by_waves<kDiag, 2>(waves),
-                    waves, g, k.rows, s, ev, what, k.start, k.index, k.value, x_bar, k.rows, blocks,
-                    step, params, d, b);
   } else if (rows == 4) {
     launch_rows_per(dual_rows_by_waves<kDiag, 4>(waves), waves, 4, k.rows, s, ev, what, k, x_bar,
                     d, b, params, step, span);

@@ -65,10 +65,10 @@ inline constexpr int kLaneCheckStride = 16;
 // `waves`: waves per workgroup, 1, 2, 4 or 8; `rows`: consecutive rows per
 // wave, 1, 2 or 4. With one row per wave the kernels read the matrix's
 // fixed-width row copy when it carries one (DevCsr::slot_width > 0), else
-// its CSR; from the CSR, with `pack` and a span of 32 slots or fewer, one
-// wave takes 64 / 32 = 2 or 64 / 16 = 4 rows of at most kBinShortMax
-// entries, a lane per (row, slot), and a longer row (with the matrix's row
-// bins, DevCsr::bin_*) a wave of its own; those narrow launches run
+// its CSR; from the CSR, with `pack`, a span of 32 slots or fewer and the
+// matrix's row bins (DevCsr::bin_*), one wave takes 64 / 32 = 2 or
+// 64 / 16 = 4 rows of at most kBinShortMax entries, a lane per (row, slot),
+// and a longer row a wave of its own; those narrow launches run
 // `narrow_waves` waves per workgroup (0: `waves`).
 // Every shape computes the same bits.
 void launch_batch_primal_step(hipStream_t s, const DevCsr& kt, const double* y,

@@ -336,7 +336,7 @@ typedef struct anuk_pdhg_options {
                                           (waves per workgroup: 1, 2, 4, 8; rows per
                                           wave: 1, 2, 4; the matrix read from its CSR or
                                           from a fixed-width row copy; and the waves per
-                                          workgroup of the packed launches of a narrow
+                                          workgroup of the binned launches of a narrow
                                           span) on the stack's K
                                           and K^T once per matrix and keeps the fastest
                                           when it beats the default shape by 3%; every
@@ -351,10 +351,10 @@ typedef struct anuk_pdhg_options {
                                           fixed-width copy of each row's first entries
                                           (one row per wave only) */
   int32_t batch_pack;                  /* 0: once 32 slots or fewer are live, a batched
-                                          step kernel over a matrix whose rows hold 8
-                                          entries or fewer takes 2 or 4 rows per wave (a
-                                          lane per row and slot); 1: one row per wave
-                                          always. The bits are the same */
+                                          step takes its rows of 8 entries or fewer 2 or
+                                          4 to a wave (a lane per row and slot) and each
+                                          longer row a wave of its own; 1: one row per
+                                          wave always. The bits are the same */
   int32_t multi_order;                 /* anuk_solve_pdhg_multi: 0: as autotune decides;
                                           1: row blocks launched in the stack's order; 2:
                                           binned (each XCD's share of the blocks run

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <map>
 #include <stdexcept>
 #include <thread>
@@ -666,6 +667,12 @@ struct LpRun {
   double q_two = 0, c_two = 0, q_inf = 0, c_inf = 0;
 };
 
+// One polished pair per seed, unscaled, with the iterations it cost.
+struct Polished {
+  std::vector<double> x, y;
+  Int spent = 0;
+};
+
 struct Outcome {
   int status = ANUK_PDHG_ITERATION_LIMIT;
   Int iterations = 0, restarts = 0;
@@ -731,8 +738,20 @@ class Engine {
   void extract(Int l, Outcome* o);
   void compact();
   void run_restarts(const std::vector<Int>& who);
-  void run_polish(const std::vector<Int>& who, Int budget, std::vector<Outcome>& out,
-                  std::vector<char>* stopped);
+  // Feasibility polishing runs beside the main periods, as in the batched
+  // solver: an attempt takes its seeds at its check, runs its sub-engines
+  // through the backend's helper on another host thread, and is applied once
+  // it is back. A failed attempt changes nothing; a successful one retires
+  // its LP as of that check.
+  struct PendingPolish {
+    std::vector<Int> lps;
+    Int iteration = 0;              // index of the check that started it
+    std::vector<Control> at_check;  // each LP's control as that check left it
+    std::future<std::vector<Polished>> result;
+  };
+  void start_polish(const std::vector<Int>& who, Int budget, Int iteration);
+  void settle_polish(std::vector<Outcome>& out, bool wait);
+  void apply_polish(PendingPolish& p, std::vector<Outcome>& out);
 
   HipBackend& be_;
   const Stack& st_;
@@ -758,6 +777,9 @@ class Engine {
   double *XBAR_, *DX_, *AX_, *LAM_, *RX_, *PRX_;
   double *DY_, *KDX_, *AY_, *AKX_, *RY_, *PRY_;
   double *CHECK_, *RESTART_, *PA_, *PB_, *PC_, *PD_, *PCH_;
+  std::vector<PendingPolish> pending_;
+  std::vector<char> polishing_;  // per LP: an attempt in flight
+  std::vector<char> settled_;    // per LP: a successful attempt fixed the outcome
   IterParams* d_params_ = nullptr;
   IterParams* d_rparams_ = nullptr;
   DevProblem prob_;
@@ -1044,11 +1066,6 @@ void Engine::run_restarts(const std::vector<Int>& who) {
 // from (0, y), each LP with its step size and primal weight, no scaling (the
 // seeds are in the scaled space), as sub-engines over the same stack
 // (pdhg.cpp:680-743); then unscaled with each LP's diagonals.
-struct Polished {
-  std::vector<double> x, y;
-  Int spent = 0;
-};
-
 std::vector<Polished> polish_stack(HipBackend& be, const Stack& st, const DeviceStack& ds,
                                    const std::vector<double>& norms, const anuk_pdhg_options& opt,
                                    const std::vector<Int>& who,
@@ -1113,10 +1130,12 @@ std::vector<Polished> polish_stack(HipBackend& be, const Stack& st, const Device
   return out;
 }
 
-void Engine::run_polish(const std::vector<Int>& who, Int budget, std::vector<Outcome>& out,
-                        std::vector<char>* stopped) {
+void Engine::start_polish(const std::vector<Int>& who, Int budget, Int iteration) {
   std::vector<std::vector<double>> sx, sy;
   std::vector<double> omega, eta;
+  PendingPolish pp;
+  pp.lps = who;
+  pp.iteration = iteration;
   for (Int l : who) {
     Outcome tmp;
     extract(l, &tmp);
@@ -1124,30 +1143,67 @@ void Engine::run_polish(const std::vector<Int>& who, Int budget, std::vector<Out
     sy.push_back(std::move(tmp.y));
     omega.push_back(ctl_[uz(l)].omega);
     eta.push_back(ctl_[uz(l)].eta);
+    pp.at_check.push_back(ctl_[uz(l)]);
+    polishing_[uz(l)] = 1;
   }
   be_.synchronize();
-  std::vector<Polished> pol = polish_stack(be_, st_, ds_, norms_, opt_, who, sx, sy, omega, eta,
-                                           budget, start_);
-  // Each polished point is checked on the original LP, one LP per host thread.
-  std::vector<Residual> checked(who.size());
-  parallel_for(who.size(), [&](size_t s) {
-    checked[s] = evaluate_residual(*(*polish_->originals)[uz(who[s])], pol[s].x, pol[s].y);
+  HipBackend* helper = &be_.helper();
+  const int device = be_.device();
+  // The task reads the stack and its device copy (read-only, shared with
+  // the main periods) and writes only the helper's memory; the engine waits
+  // for every attempt before it returns.
+  pp.result = std::async(std::launch::async, [this, helper, device, budget, who,
+                                              sx = std::move(sx), sy = std::move(sy),
+                                              omega = std::move(omega), eta = std::move(eta)]() {
+    hip_check(hipSetDevice(device), "hipSetDevice(polish)");
+    return polish_stack(*helper, st_, ds_, norms_, opt_, who, sx, sy, omega, eta, budget, start_);
   });
-  for (size_t s = 0; s < who.size(); ++s) {
-    const Int l = who[s];
+  pending_.push_back(std::move(pp));
+}
+
+void Engine::apply_polish(PendingPolish& pp, std::vector<Outcome>& out) {
+  std::vector<Polished> pol = pp.result.get();
+  // Each polished point is checked on the original LP, one LP per host thread.
+  std::vector<Residual> checked(pp.lps.size());
+  parallel_for(pp.lps.size(), [&](size_t s) {
+    checked[s] = evaluate_residual(*(*polish_->originals)[uz(pp.lps[s])], pol[s].x, pol[s].y);
+  });
+  for (size_t s = 0; s < pp.lps.size(); ++s) {
+    const Int l = pp.lps[s];
     Outcome& o = out[uz(l)];
     const Residual& tr = checked[s];
     ++o.polish_attempts;
     o.polish_iterations += pol[s].spent;
-    Control& c = ctl_[uz(l)];
-    if (tr.converged(c.tol, opt_.absolute_tolerance, opt_.require_inf_norm_termination != 0, c.gap_tol)) {
-      o.x = std::move(pol[s].x);
-      o.y = std::move(pol[s].y);
-      o.polished = true;
-      o.polished_r = tr;
-      c.status = ANUK_PDHG_OPTIMAL;
-      c.message = "converged after feasibility polishing";
-      (*stopped)[uz(l)] = 1;
+    polishing_[uz(l)] = 0;
+    const Control& c = pp.at_check[s];
+    if (!tr.converged(c.tol, opt_.absolute_tolerance, opt_.require_inf_norm_termination != 0,
+                      c.gap_tol))
+      continue;
+    // Converged: the LP's outcome is the polished pair as of the check that
+    // started the attempt, whatever the LP did since.
+    o.x = std::move(pol[s].x);
+    o.y = std::move(pol[s].y);
+    o.polished = true;
+    o.polished_r = tr;
+    o.status = ANUK_PDHG_OPTIMAL;
+    o.message = "converged after feasibility polishing";
+    o.iterations = pp.iteration + 1;
+    o.seconds = seconds_since(start_);
+    o.restarts = c.restarts;
+    o.eta = c.eta;
+    o.omega = c.omega;
+    settled_[uz(l)] = 1;
+    active_[uz(l)] = 0;
+  }
+}
+
+void Engine::settle_polish(std::vector<Outcome>& out, bool wait) {
+  for (auto it = pending_.begin(); it != pending_.end();) {
+    if (wait || it->result.wait_for(std::chrono::seconds(0)) == std::future_status::ready) {
+      apply_polish(*it, out);
+      it = pending_.erase(it);
+    } else {
+      ++it;
     }
   }
 }
@@ -1173,6 +1229,8 @@ std::vector<Outcome> Engine::run() {
   allocate();
   ctl_.resize(uz(L_));
   active_.assign(uz(L_), 0);
+  polishing_.assign(uz(L_), 0);
+  settled_.assign(uz(L_), 0);
   for (Int l = 0; l < L_; ++l) {
     LpRun& r = runs_[uz(l)];
     if (!r.participates) continue;
@@ -1234,6 +1292,7 @@ std::vector<Outcome> Engine::run() {
               "multi check");
     be_.synchronize();
     lap(&Phases::iterating);
+    if (!pending_.empty()) settle_polish(out, false);
     const Int i = period_end - 1;
     const double elapsed = seconds_since(start_);
     std::vector<char> stopped(uz(L_), 0);
@@ -1273,9 +1332,19 @@ std::vector<Outcome> Engine::run() {
       }
     }
     lap(&Phases::checks);
-    for (const auto& kv : polish_by_budget) {
-      if (polish_ == nullptr) continue;
-      run_polish(kv.second, kv.first, out, &stopped);
+    if (polish_ != nullptr && !polish_by_budget.empty()) {  // sub-engines never polish
+      // An LP due another attempt while one is in flight has that one
+      // settled first (it may have succeeded, which retires the LP).
+      bool due_in_flight = false;
+      for (const auto& kv : polish_by_budget)
+        for (Int l : kv.second) due_in_flight = due_in_flight || polishing_[uz(l)] != 0;
+      if (due_in_flight) settle_polish(out, true);
+      for (const auto& kv : polish_by_budget) {
+        std::vector<Int> who;
+        for (Int l : kv.second)
+          if (active_[uz(l)]) who.push_back(l);
+        if (!who.empty()) start_polish(who, kv.first, i);
+      }
     }
     lap(&Phases::polishing);
     std::vector<Int> restart;
@@ -1289,7 +1358,7 @@ std::vector<Outcome> Engine::run() {
       }
     }
     for (Int l = 0; l < L_; ++l) {
-      if (!stopped[uz(l)]) continue;
+      if (!stopped[uz(l)] || settled_[uz(l)]) continue;
       Outcome& o = out[uz(l)];
       o.iterations = i + 1;
       o.seconds = seconds_since(start_);
@@ -1318,7 +1387,7 @@ std::vector<Outcome> Engine::run() {
   for (Int l = 0; l < L_; ++l) {
     Outcome& o = out[uz(l)];
     const Control& c = ctl_[uz(l)];
-    if (!runs_[uz(l)].participates) continue;
+    if (!runs_[uz(l)].participates || settled_[uz(l)]) continue;
     if (norms_[uz(l)] <= 0.0) {
       o.status = ANUK_PDHG_NUMERICAL_ERROR;
       o.message = "the constraint matrix is empty";
@@ -1330,6 +1399,9 @@ std::vector<Outcome> Engine::run() {
     o.eta = c.eta;
     o.omega = c.omega;
   }
+  // Attempts still in flight decide last: a success stands for its LP
+  // whatever the LP did after the attempt's check.
+  settle_polish(out, true);
   return out;
 }
 

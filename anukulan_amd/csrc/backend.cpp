@@ -1,5 +1,7 @@
 #include "backend.hpp"
 
+#include "host_pool.hpp"
+
 #include <algorithm>
 #include <thread>
 #include <cmath>
@@ -216,6 +218,58 @@ double* HipBackend::pinned_scratch(std::size_t doubles) {
   hip_check(hipHostMalloc(&pinned_, pinned_size_ * sizeof(double), hipHostMallocDefault),
             "hipHostMalloc(staging)");
   return pinned_;
+}
+
+void HipBackend::upload_concat(void* target,
+                               const std::vector<std::pair<const void*, std::size_t>>& parts) {
+  std::size_t total = 0;
+  for (const auto& p : parts) total += p.second;
+  if (total == 0) return;
+  constexpr std::size_t kChunk = std::size_t{64} << 20;  // bytes per staging chunk
+  constexpr std::size_t kTask = std::size_t{4} << 20;    // bytes per host copy task
+  constexpr int kStageSlot = 900;                        // pinned slots 900, 901
+  const std::size_t chunk = std::min(kChunk, total);
+  char* stage[2];
+  for (int b = 0; b < 2; ++b)
+    stage[b] = reinterpret_cast<char*>(pinned_slot(kStageSlot + b, (chunk + 7) / 8));
+  hipEvent_t done[2] = {take_event(), take_event()};
+  bool busy[2] = {false, false};
+  // Walks the parts as one byte stream: piece p from offset `at`.
+  std::size_t p = 0, at = 0, out = 0;
+  for (int b = 0; out < total; b ^= 1) {
+    if (busy[b]) hip_check(hipEventSynchronize(done[b]), "upload_concat");
+    const std::size_t len = std::min(chunk, total - out);
+    // The chunk's copy tasks: (destination offset in the chunk, source, bytes).
+    struct Task {
+      std::size_t off;
+      const char* src;
+      std::size_t bytes;
+    };
+    std::vector<Task> tasks;
+    for (std::size_t filled = 0; filled < len;) {
+      const std::size_t take = std::min({parts[p].second - at, len - filled, kTask});
+      if (take > 0)
+        tasks.push_back({filled, static_cast<const char*>(parts[p].first) + at, take});
+      filled += take;
+      at += take;
+      if (at == parts[p].second) {
+        ++p;
+        at = 0;
+      }
+    }
+    char* buf = stage[b];
+    HostPool::shared().run(tasks.size(), [&](std::size_t i) {
+      std::memcpy(buf + tasks[i].off, tasks[i].src, tasks[i].bytes);
+    });
+    hip_check(hipMemcpyAsync(static_cast<char*>(target) + out, buf, len, hipMemcpyHostToDevice, stream_),
+              "upload_concat");
+    hip_check(hipEventRecord(done[b], stream_), "upload_concat");
+    busy[b] = true;
+    out += len;
+  }
+  synchronize();
+  event_pool_.push_back(done[0]);
+  event_pool_.push_back(done[1]);
 }
 
 double* HipBackend::pinned_slot(int slot, std::size_t doubles) {

@@ -137,6 +137,11 @@ class HipBackend {
   // Copies n doubles host -> device through the pinned staging buffer and
   // waits (fast DMA for large uploads from pageable memory).
   void upload_staged(const double* host, double* target, std::size_t n);
+  // Copies the host ranges `parts` (pointer, bytes), concatenated in order,
+  // to `target` on the device, and waits: through two page-locked chunks,
+  // one filled by the host pool while the other's copy runs, so a large
+  // upload from many pageable pieces moves at DMA speed in few copies.
+  void upload_concat(void* target, const std::vector<std::pair<const void*, std::size_t>>& parts);
 
   // A small device result vector and its pinned host mirror.
   double* result_device() { return result_dev_; }

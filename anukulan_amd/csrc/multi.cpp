@@ -260,32 +260,29 @@ void upload_stack(const Stack& st, DeviceStack* ds) {
   const Int* d_row_off = ds->put(st.row_off);
   const Int* d_nz_off = ds->put(st.nz_off);
   // Each LP's CSR arrays are copied to their place in the stacked CSR as
-  // they are, then shifted on the device: row starts by the LP's first
-  // entry, column indices by its first column (row, for K^T).
+  // they are (the LPs' arrays end to end, through page-locked chunks), then
+  // shifted on the device: row starts by the LP's first entry, column
+  // indices by its first column (row, for K^T).
   auto dev_csr = [&](bool transpose, const Plan& p) {
     DevCsr d;
     d.rows = transpose ? st.n : st.m;
     d.cols = transpose ? st.m : st.n;
     d.nnz = st.nnz;
     d.blocks = static_cast<Int>(p.block_lp.size());
-    const std::vector<Int>& roff = transpose ? st.col_off : st.row_off;
     Int* start = ds->alloc<Int>(uz(d.rows) + 1);
     Int* index = ds->alloc<Int>(uz(d.nnz));
     double* value = ds->alloc<double>(uz(d.nnz));
     hip_check(hipMemsetAsync(start, 0, sizeof(Int), s), "stack row starts");
+    std::vector<std::pair<const void*, size_t>> starts, indices, values;
     for (Int l = 0; l < st.lps; ++l) {
       const Csr& a = transpose ? st.lp[uz(l)]->kt : st.lp[uz(l)]->k;
-      const size_t rows = uz(a.rows), nnz = uz(a.nnz()), at = uz(st.nz_off[uz(l)]);
-      if (rows > 0)
-        hip_check(hipMemcpyAsync(start + roff[uz(l)] + 1, a.start.data() + 1, rows * sizeof(Int),
-                                 hipMemcpyHostToDevice, s), "upload stack");
-      if (nnz > 0) {
-        hip_check(hipMemcpyAsync(index + at, a.index.data(), nnz * sizeof(Int), hipMemcpyHostToDevice, s),
-                  "upload stack");
-        hip_check(hipMemcpyAsync(value + at, a.value.data(), nnz * sizeof(double), hipMemcpyHostToDevice, s),
-                  "upload stack");
-      }
+      starts.push_back({a.start.data() + 1, uz(a.rows) * sizeof(Int)});
+      indices.push_back({a.index.data(), uz(a.nnz()) * sizeof(Int)});
+      values.push_back({a.value.data(), uz(a.nnz()) * sizeof(double)});
     }
+    ds->be.upload_concat(start + 1, starts);
+    ds->be.upload_concat(index, indices);
+    ds->be.upload_concat(value, values);
     launch_add_segment_offsets(s, start + 1, d.rows, transpose ? d_col_off : d_row_off, d_nz_off, st.lps);
     launch_add_segment_offsets(s, index, d.nnz, d_nz_off, transpose ? d_row_off : d_col_off, st.lps);
     d.start = start;

@@ -272,6 +272,43 @@ void HipBackend::upload_concat(void* target,
   event_pool_.push_back(done[1]);
 }
 
+void HipBackend::download_staged(const void* source, void* host, std::size_t bytes) {
+  if (bytes == 0) return;
+  constexpr std::size_t kChunk = std::size_t{64} << 20;
+  constexpr std::size_t kTask = std::size_t{4} << 20;
+  constexpr int kStageSlot = 900;  // the chunks upload_concat uses
+  const std::size_t chunk = std::min(kChunk, bytes);
+  char* stage[2];
+  for (int b = 0; b < 2; ++b)
+    stage[b] = reinterpret_cast<char*>(pinned_slot(kStageSlot + b, (chunk + 7) / 8));
+  hipEvent_t done[2] = {take_event(), take_event()};
+  const std::size_t count = (bytes + chunk - 1) / chunk;
+  auto issue = [&](std::size_t c) {
+    const int b = static_cast<int>(c & 1);
+    const std::size_t off = c * chunk, len = std::min(chunk, bytes - off);
+    hip_check(hipMemcpyAsync(stage[b], static_cast<const char*>(source) + off, len,
+                             hipMemcpyDeviceToHost, stream_),
+              "download_staged");
+    hip_check(hipEventRecord(done[b], stream_), "download_staged");
+  };
+  issue(0);
+  for (std::size_t c = 0; c < count; ++c) {
+    // Chunk c + 1 goes out before chunk c is copied out of its buffer.
+    if (c + 1 < count) issue(c + 1);
+    const int b = static_cast<int>(c & 1);
+    hip_check(hipEventSynchronize(done[b]), "download_staged");
+    const std::size_t off = c * chunk, len = std::min(chunk, bytes - off);
+    const char* buf = stage[b];
+    char* dst = static_cast<char*>(host) + off;
+    HostPool::shared().run((len + kTask - 1) / kTask, [&](std::size_t i) {
+      const std::size_t o = i * kTask;
+      std::memcpy(dst + o, buf + o, std::min(kTask, len - o));
+    });
+  }
+  event_pool_.push_back(done[0]);
+  event_pool_.push_back(done[1]);
+}
+
 double* HipBackend::pinned_slot(int slot, std::size_t doubles) {
   auto it = pinned_slots_.find(slot);
   if (it != pinned_slots_.end() && it->second.second >= doubles) return it->second.first;

@@ -142,6 +142,10 @@ class HipBackend {
   // one filled by the host pool while the other's copy runs, so a large
   // upload from many pageable pieces moves at DMA speed in few copies.
   void upload_concat(void* target, const std::vector<std::pair<const void*, std::size_t>>& parts);
+  // The reverse for one range: `bytes` from `source` on the device to
+  // `host`, through the same two page-locked chunks (one chunk's copy runs
+  // while the host pool empties the other), and waits.
+  void download_staged(const void* source, void* host, std::size_t bytes);
 
   // A small device result vector and its pinned host mirror.
   double* result_device() { return result_dev_; }

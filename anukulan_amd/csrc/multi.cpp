@@ -338,11 +338,10 @@ void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, Device
   hipStream_t s = be.stream();
   int slot = kMultiScratchSlot + 100;
   auto dev = [&](Int len) { return be.scratch(slot++, uz(std::max<Int>(len, 1))); };
+  // Whole-stack vectors cross PCIe through page-locked chunks.
   auto put = [&](const std::vector<double>& h) {
     double* d = dev(static_cast<Int>(h.size()));
-    if (!h.empty())
-      hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s),
-                "stack vectors");
+    be.upload_concat(d, {{h.data(), h.size() * sizeof(double)}});
     return d;
   };
   auto clone = [&](const double* from, Int len) {
@@ -386,9 +385,7 @@ void scale_stack(HipBackend& be, const anuk_pdhg_options& opt, Stack* st, Device
     // The host keeps the scaled vectors for the per-LP norms, start points
     // and unscaling.
     auto get = [&](const double* d, std::vector<double>* h) {
-      if (!h->empty())
-        hip_check(hipMemcpyAsync(h->data(), d, h->size() * sizeof(double), hipMemcpyDeviceToHost, s),
-                  "stack scaling");
+      be.download_staged(d, h->data(), h->size() * sizeof(double));
     };
     get(q, &st->q);
     get(c, &st->c);

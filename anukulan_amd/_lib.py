@@ -140,6 +140,23 @@ class AdjointInfo(ctypes.Structure):
                 ("basis_completed", c_int32)]
 
 
+class PresolveRules(ctypes.Structure):
+    _fields_ = [("empty_rows", c_int32), ("singleton_rows", c_int32), ("redundant_rows", c_int32),
+                ("fixed_columns", c_int32), ("empty_columns", c_int32),
+                ("slack_column_singletons", c_int32), ("max_rounds", c_int32),
+                ("tolerance", c_double), ("pivot_ratio", c_double)]
+
+
+class PresolveInfo(ctypes.Structure):
+    _fields_ = [("status", c_int32), ("failing_case", c_int32), ("rows", c_int32),
+                ("cols", c_int32), ("nnz", c_int64), ("original_rows", c_int32),
+                ("original_cols", c_int32), ("original_nnz", c_int64), ("empty_rows", c_int32),
+                ("free_rows", c_int32), ("singleton_rows", c_int32), ("redundant_rows", c_int32),
+                ("redundant_sides", c_int32), ("fixed_columns", c_int32),
+                ("empty_columns", c_int32), ("slack_column_singletons", c_int32),
+                ("rounds", c_int32)]
+
+
 # name -> (restype, argtypes); every function the header declares.
 SIGNATURES = {
     "anuk_create": (c_int, [c_int, POINTER(c_void_p)]),
@@ -215,6 +232,14 @@ SIGNATURES = {
     "anuk_mps_warning": (c_char_p, [c_void_p, c_int32]),
     "anuk_mps_write": (c_int, [c_char_p, POINTER(Model), c_char_p, POINTER(c_char_p),
                                POINTER(c_char_p), IP]),
+    "anuk_presolve_default_rules": (None, [POINTER(PresolveRules)]),
+    "anuk_presolve_stack": (c_int, [c_int32, c_int32, IP, IP, DP, c_int32, DP, DP, DP, DP, DP, DP,
+                                    POINTER(PresolveRules), POINTER(c_void_p)]),
+    "anuk_presolve_free": (None, [c_void_p]),
+    "anuk_presolve_get_info": (c_int, [c_void_p, POINTER(PresolveInfo)]),
+    "anuk_presolve_message": (c_char_p, [c_void_p]),
+    "anuk_presolve_reduced": (c_int, [c_void_p, IP, IP, DP, DP, DP, DP, DP, DP, DP, IP, IP]),
+    "anuk_postsolve": (c_int, [c_void_p, c_int32, DP, DP, DP, DP]),
 }
 
 

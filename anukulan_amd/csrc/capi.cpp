@@ -20,6 +20,7 @@
 #include "finish.hpp"
 #include "scaling.hpp"
 #include "pdhg.hpp"
+#include "stack_presolve.hpp"
 
 struct anuk_ctx {
   std::unique_ptr<anuk::HipBackend> be;
@@ -32,6 +33,10 @@ struct anuk_lp {
 
 struct anuk_mps {
   anuk::MpsModel m;
+};
+
+struct anuk_presolve {
+  anuk::StackPresolve p;
 };
 
 namespace {
@@ -999,5 +1004,135 @@ int anuk_mps_write(const char* path, const anuk_model* model, const char* name,
   });
 }
 
+
+void anuk_presolve_default_rules(anuk_presolve_rules* out) {
+  if (out == nullptr) return;
+  const anuk::PresolveRules r;
+  out->empty_rows = r.empty_rows;
+  out->singleton_rows = r.singleton_rows;
+  out->redundant_rows = r.redundant_rows;
+  out->fixed_columns = r.fixed_columns;
+  out->empty_columns = r.empty_columns;
+  out->slack_column_singletons = r.slack_column_singletons;
+  out->max_rounds = r.max_rounds;
+  out->tolerance = r.tolerance;
+  out->pivot_ratio = r.pivot_ratio;
+}
+
+int anuk_presolve_stack(int32_t rows, int32_t cols, const int32_t* start, const int32_t* index,
+                        const double* value, int32_t cases, const double* objective,
+                        const double* row_lower, const double* row_upper,
+                        const double* col_lower, const double* col_upper, const double* offset,
+                        const anuk_presolve_rules* rules, anuk_presolve** out) {
+  if (out == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded(nullptr, [&] {
+    need(rows >= 0 && cols >= 0 && cases >= 1, "presolve: negative sizes or no cases");
+    need(start != nullptr, "presolve: null row starts");
+    const size_t nc = static_cast<size_t>(cases) * static_cast<size_t>(cols);
+    const size_t nr = static_cast<size_t>(cases) * static_cast<size_t>(rows);
+    need(nc == 0 || (objective && col_lower && col_upper), "presolve: null column arrays");
+    need(nr == 0 || (row_lower && row_upper), "presolve: null row arrays");
+    anuk::PresolveRules r;
+    if (rules != nullptr) {
+      r.empty_rows = rules->empty_rows != 0;
+      r.singleton_rows = rules->singleton_rows != 0;
+      r.redundant_rows = rules->redundant_rows != 0;
+      r.fixed_columns = rules->fixed_columns != 0;
+      r.empty_columns = rules->empty_columns != 0;
+      r.slack_column_singletons = rules->slack_column_singletons != 0;
+      r.max_rounds = rules->max_rounds;
+      r.tolerance = rules->tolerance;
+      r.pivot_ratio = rules->pivot_ratio;
+      need(r.max_rounds >= 0 && r.tolerance >= 0.0 && r.pivot_ratio >= 0.0 && r.pivot_ratio <= 1.0,
+           "presolve: max_rounds, tolerance or pivot_ratio out of range");
+    }
+    need(start[0] == 0 && start[rows] >= 0, "presolve: row starts must run from 0 to nnz");
+    const size_t nnz = static_cast<size_t>(start[rows]);
+    need(nnz == 0 || (index && value), "presolve: null entries");
+    anuk::LpStack s;
+    s.rows = rows;
+    s.cols = cols;
+    s.cases = cases;
+    s.a = anuk::Csr::from_csr(rows, cols, start, index, value);
+    s.objective.assign(objective, objective + nc);
+    s.col_lower.assign(col_lower, col_lower + nc);
+    s.col_upper.assign(col_upper, col_upper + nc);
+    s.row_lower.assign(row_lower, row_lower + nr);
+    s.row_upper.assign(row_upper, row_upper + nr);
+    s.offset.assign(static_cast<size_t>(cases), 0.0);
+    if (offset) s.offset.assign(offset, offset + cases);
+    auto h = std::make_unique<anuk_presolve>();
+    h->p = anuk::presolve_stack(s, r);
+    *out = h.release();
+  });
+}
+
+void anuk_presolve_free(anuk_presolve* p) { delete p; }
+
+int anuk_presolve_get_info(const anuk_presolve* p, anuk_presolve_info* out) {
+  if (p == nullptr || out == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  const anuk::StackPresolve& s = p->p;
+  const anuk::PresolveCounts& c = s.counts;
+  out->status = static_cast<int32_t>(s.status);
+  out->failing_case = s.failing_case;
+  out->rows = s.reduced.rows;
+  out->cols = s.reduced.cols;
+  out->nnz = s.reduced.a.nnz();
+  out->original_rows = s.original_rows();
+  out->original_cols = s.original_cols();
+  out->original_nnz = static_cast<int64_t>(s.reduced.a.nnz()) + c.nonzeros_removed;
+  out->empty_rows = c.empty_rows;
+  out->free_rows = c.free_rows;
+  out->singleton_rows = c.singleton_rows;
+  out->redundant_rows = c.redundant_rows;
+  out->redundant_sides = c.redundant_sides;
+  out->fixed_columns = c.fixed_columns;
+  out->empty_columns = c.empty_columns;
+  out->slack_column_singletons = c.slack_column_singletons;
+  out->rounds = c.rounds;
+  return ANUK_OK;
+}
+
+const char* anuk_presolve_message(const anuk_presolve* p) {
+  return p == nullptr ? "" : p->p.message.c_str();
+}
+
+int anuk_presolve_reduced(const anuk_presolve* p, int32_t* start, int32_t* index, double* value,
+                          double* objective, double* row_lower, double* row_upper,
+                          double* col_lower, double* col_upper, double* offset,
+                          int32_t* row_origin, int32_t* col_origin) {
+  if (p == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  return guarded(nullptr, [&] {
+    const anuk::StackPresolve& s = p->p;
+    need(s.status == anuk::PresolveStatus::kReduced, "presolve: no reduced stack (see its status)");
+    const anuk::LpStack& r = s.reduced;
+    auto put = [](auto* dst, const auto& v) {
+      if (dst != nullptr && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    put(start, r.a.start);
+    put(index, r.a.index);
+    put(value, r.a.value);
+    put(objective, r.objective);
+    put(row_lower, r.row_lower);
+    put(row_upper, r.row_upper);
+    put(col_lower, r.col_lower);
+    put(col_upper, r.col_upper);
+    put(offset, r.offset);
+    put(row_origin, s.row_origin);
+    put(col_origin, s.col_origin);
+  });
+}
+
+int anuk_postsolve(const anuk_presolve* p, int32_t case_index, const double* reduced_x,
+                   const double* reduced_y, double* x, double* y) {
+  if (p == nullptr || x == nullptr || y == nullptr) return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  return guarded(nullptr, [&] {
+    const anuk::StackPresolve& s = p->p;
+    need((reduced_x != nullptr || s.reduced.cols == 0) && (reduced_y != nullptr || s.reduced.rows == 0),
+         "postsolve: null reduced solution");
+    s.postsolve(case_index, reduced_x, reduced_y, x, y);
+  });
+}
 
 }  // extern "C"

@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(kThreads)
 
 struct PlainEpi {
   static constexpr int kR = 0;
-  static constexpr unsigned kMaxMask = 0;
+  [[maybe_unused]] static constexpr unsigned kMaxMask = 0;
   double* y;
   __device__ PlainEpi resolve(int) const { return *this; }
   __device__ bool skip() const { return false; }
@@ -543,7 +543,7 @@ struct ConvColOp {
 // y = A x on the blocks of active LPs.
 struct MultiPlainEpi {
   static constexpr int kR = 0;
-  static constexpr unsigned kMaxMask = 0;
+  [[maybe_unused]] static constexpr unsigned kMaxMask = 0;
   double* y;
   ManyLps src;
   bool off;

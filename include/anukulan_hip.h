@@ -721,6 +721,93 @@ int anuk_mps_write(const char* path, const anuk_model* model, const char* name,
                    const char* const* row_names, const char* const* col_names,
                    const int32_t* col_types);
 
+/* ------------------------------------------------------------------------ */
+/* Presolve of a case stack in front of the batch: the reference's           */
+/* presolve(const Model&, const PresolveOptions&) (presolve.hpp:485) with    */
+/* PostsolveStack::apply / apply_dual (presolve.hpp:389-396), taken for a    */
+/* stack of cases that share A so that the reduced cases still share one.    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct anuk_presolve anuk_presolve;
+
+/* Which reductions may fire; the switches named as in PresolveOptions
+ * (presolve.hpp:59-70, 157) stand for the same reduction. All default to on
+ * except slack_column_singletons: each such substitution makes its equality
+ * row a ranged row, two opposed rows of the standard form, which on the
+ * refinery case stack costs the PDHG solves more iterations than the removed
+ * columns save (509,680 -> 683,360 lane iterations to 1e-4 over the 61
+ * cases of the 24-period stack). tolerance is the
+ * feasibility slack relative to max(1, |bound|); a column alone in an
+ * equality row is substituted out only when its coefficient is at least
+ * pivot_ratio times the largest magnitude in the row. */
+typedef struct anuk_presolve_rules {
+  int32_t empty_rows;
+  int32_t singleton_rows;
+  int32_t redundant_rows;
+  int32_t fixed_columns;
+  int32_t empty_columns;
+  int32_t slack_column_singletons;
+  int32_t max_rounds;
+  double tolerance;
+  double pivot_ratio;
+} anuk_presolve_rules;
+void anuk_presolve_default_rules(anuk_presolve_rules* out);
+
+enum anuk_presolve_status {
+  ANUK_PRESOLVE_REDUCED = 0,
+  ANUK_PRESOLVE_INFEASIBLE = 1, /* some case proved infeasible: failing_case names it */
+  ANUK_PRESOLVE_UNBOUNDED = 2   /* some case has a column in no row that its cost drives to infinity */
+};
+
+/* Sizes before and after, and how often each reduction fired
+ * (PresolveCounts, presolve.hpp:432-463, for the reductions made here). */
+typedef struct anuk_presolve_info {
+  int32_t status; /* anuk_presolve_status */
+  int32_t failing_case;
+  int32_t rows, cols;
+  int64_t nnz;
+  int32_t original_rows, original_cols;
+  int64_t original_nnz;
+  int32_t empty_rows, free_rows, singleton_rows, redundant_rows, redundant_sides;
+  int32_t fixed_columns, empty_columns, slack_column_singletons;
+  int32_t rounds;
+} anuk_presolve_info;
+
+/* Presolves `cases` LPs in minimisation form that share A (rows x cols, CSR
+ * with strictly increasing columns in each row and no explicit zeros). Case
+ * k's objective and column bounds start at k*cols, its row bounds at k*rows;
+ * offset holds one objective constant per case (NULL: zeros). Infinite
+ * bounds are +-HUGE_VAL. A structural reduction is taken only when every case
+ * allows it, so the reduced cases share one matrix again. *out owns the
+ * reduced stack and what postsolve needs; an infeasible or unbounded verdict
+ * is a status, not an error. rules may be NULL for the defaults. */
+int anuk_presolve_stack(int32_t rows, int32_t cols, const int32_t* start, const int32_t* index,
+                        const double* value, int32_t cases, const double* objective,
+                        const double* row_lower, const double* row_upper,
+                        const double* col_lower, const double* col_upper, const double* offset,
+                        const anuk_presolve_rules* rules, anuk_presolve** out);
+void anuk_presolve_free(anuk_presolve* p);
+int anuk_presolve_get_info(const anuk_presolve* p, anuk_presolve_info* out);
+/* Why the status is not REDUCED ("case k: ..."), empty otherwise. */
+const char* anuk_presolve_message(const anuk_presolve* p);
+
+/* The reduced stack, laid out like the input (start: rows+1, index and value:
+ * nnz, objective and column bounds: cases*cols, row bounds: cases*rows,
+ * offset: cases) and, for each reduced row and column, the original one it
+ * is. Any pointer may be NULL. */
+int anuk_presolve_reduced(const anuk_presolve* p, int32_t* start, int32_t* index, double* value,
+                          double* objective, double* row_lower, double* row_upper,
+                          double* col_lower, double* col_upper, double* offset,
+                          int32_t* row_origin, int32_t* col_origin);
+
+/* Case `case_index`'s solution of the reduced LP (reduced_x: reduced cols,
+ * reduced_y: one dual per reduced row) mapped back to the original LP
+ * (x: original cols, y: original rows). Row duals are positive on a row held
+ * at its lower side and negative at its upper side, reduced costs are
+ * c - A'y; every reduction made here inverts exactly on the dual side. */
+int anuk_postsolve(const anuk_presolve* p, int32_t case_index, const double* reduced_x,
+                   const double* reduced_y, double* x, double* y);
+
 #ifdef __cplusplus
 }
 #endif

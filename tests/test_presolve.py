@@ -137,6 +137,13 @@ def planted_lp(seed, m=24, n=30):
     i = new_row(-100.0, float(sum(x0[cols])) + 0.5)
     for j in cols:
         trip[(i, int(j))] = 1.0
+    # rows that are nearly redundant and bind at the optimum: activity
+    # ranges [0, 10] against a lower side of 0.2 (costs push the activity
+    # down) and an upper side of 9.8 (costs push it up)
+    for lo, hi, pair in ((0.2, INF, (1.0, 2.0)), (-INF, 9.8, (-1.0, -2.0))):
+        i = new_row(lo, hi)
+        for c in pair:
+            trip[(i, new_col(c, 0.0, 5.0, 0.1))] = 1.0
     # column singletons in equality rows: bounded slack, free, and a
     # negative coefficient
     for c, lo, hi, a in ((float(g.normal()), 0.0, 10.0, 1.0), (float(g.normal()), -INF, INF, 2.0),
@@ -355,3 +362,46 @@ def test_refinery_mps_keeps_its_optimum_beside_the_reference_presolve(ref):
           f"reference {len(theirs.reduced['row_lower'])}x{len(theirs.reduced['objective'])}; "
           f"optimum {want:.12e}, residual after postsolve {r}")
     assert ps.cols < nm.cols
+
+
+def test_every_readable_corpus_lp_keeps_its_verdict_and_optimum(ref):
+    """The MPS corpus (hand-written texts covering every section and bound
+    type, and seeded random models): every LP the reader accepts, presolved
+    with every rule on, keeps the reference simplex's verdict, and an
+    optimal one keeps its optimum and maps back to a KKT point."""
+    from golden.mps_corpus import all_cases
+    from anukulan_amd.mps import read_mps_text
+    OPTIMAL, UNBOUNDED, INFEASIBLE = 0, 1, 2
+    seen = {"optimal": 0, "infeasible": 0, "unbounded": 0, "reduced": 0}
+    for label, text, opts in all_cases():
+        kw = {}
+        if "negative_up" in opts:
+            kw["negative_up_bound"] = "cplex" if opts["negative_up"] else "highs"
+        if "format" in opts:
+            kw["format"] = {0: "auto", 1: "free", 2: "fixed"}[opts["format"]]
+        try:
+            m = read_mps_text(text, **kw)
+        except AnukError:
+            continue
+        if m.hessian is not None or m.cols == 0:
+            continue
+        nm = m.to_natural()
+        status = ref.RefLp.from_natural(nm).simplex()["status"]
+        if status not in (OPTIMAL, UNBOUNDED, INFEASIBLE):
+            continue
+        ps = presolve(nm, slack_column_singletons=1)
+        if status == OPTIMAL:
+            assert ps.status == "reduced", (label, ps.message)
+            check_round_trip(ref, nm, ps, tol=1e-8)
+            seen["optimal"] += 1
+        elif ps.status == "reduced":
+            # the reduced model must reach the same verdict
+            red = ps.model()
+            if red.cols:
+                assert ref.RefLp.from_natural(red).simplex()["status"] == status, label
+            seen["reduced"] += 1
+        else:
+            # a verdict without solving: never on an LP with an optimum
+            seen[ps.status] += 1
+    print("\ncorpus presolve verdicts:", seen)
+    assert seen["optimal"] >= 40

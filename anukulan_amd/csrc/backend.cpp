@@ -50,6 +50,7 @@ HipBackend::~HipBackend() {
   if (result_host_) (void)hipHostFree(result_host_);
   if (pinned_) (void)hipHostFree(pinned_);
   for (auto& kv : pinned_slots_) (void)hipHostFree(kv.second.first);
+  if (spare_slab_.first) (void)hipFree(spare_slab_.first);
   if (stop_event_) (void)hipEventDestroy(stop_event_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -222,8 +223,19 @@ double* HipBackend::pinned_scratch(std::size_t doubles) {
 
 void HipBackend::upload_concat(void* target,
                                const std::vector<std::pair<const void*, std::size_t>>& parts) {
+  std::vector<UploadPiece> pieces;
+  pieces.reserve(parts.size());
+  std::size_t at = 0;
+  for (const auto& p : parts) {
+    pieces.push_back({p.first, static_cast<char*>(target) + at, p.second});
+    at += p.second;
+  }
+  upload_scatter(pieces);
+}
+
+void HipBackend::upload_scatter(const std::vector<UploadPiece>& pieces) {
   std::size_t total = 0;
-  for (const auto& p : parts) total += p.second;
+  for (const auto& p : pieces) total += p.bytes;
   if (total == 0) return;
   constexpr std::size_t kChunk = std::size_t{64} << 20;  // bytes per staging chunk
   constexpr std::size_t kTask = std::size_t{4} << 20;    // bytes per host copy task
@@ -234,25 +246,37 @@ void HipBackend::upload_concat(void* target,
     stage[b] = reinterpret_cast<char*>(pinned_slot(kStageSlot + b, (chunk + 7) / 8));
   hipEvent_t done[2] = {take_event(), take_event()};
   bool busy[2] = {false, false};
-  // Walks the parts as one byte stream: piece p from offset `at`.
+  // Walks the pieces as one byte stream: piece p from offset `at`.
   std::size_t p = 0, at = 0, out = 0;
   for (int b = 0; out < total; b ^= 1) {
-    if (busy[b]) hip_check(hipEventSynchronize(done[b]), "upload_concat");
+    if (busy[b]) hip_check(hipEventSynchronize(done[b]), "upload_scatter");
     const std::size_t len = std::min(chunk, total - out);
-    // The chunk's copy tasks: (destination offset in the chunk, source, bytes).
+    // The chunk's host copy tasks (offset in the chunk, source, bytes) and
+    // its device copies (offset in the chunk, destination, bytes): one per
+    // piece fragment that the chunk holds.
     struct Task {
       std::size_t off;
       const char* src;
       std::size_t bytes;
     };
+    struct Send {
+      std::size_t off;
+      char* dst;
+      std::size_t bytes;
+    };
     std::vector<Task> tasks;
+    std::vector<Send> sends;
     for (std::size_t filled = 0; filled < len;) {
-      const std::size_t take = std::min({parts[p].second - at, len - filled, kTask});
-      if (take > 0)
-        tasks.push_back({filled, static_cast<const char*>(parts[p].first) + at, take});
-      filled += take;
-      at += take;
-      if (at == parts[p].second) {
+      const std::size_t left = std::min(pieces[p].bytes - at, len - filled);
+      if (left > 0) {
+        sends.push_back({filled, static_cast<char*>(pieces[p].device) + at, left});
+        for (std::size_t t = 0; t < left; t += kTask)
+          tasks.push_back({filled + t, static_cast<const char*>(pieces[p].host) + at + t,
+                           std::min(kTask, left - t)});
+      }
+      filled += left;
+      at += left;
+      if (at == pieces[p].bytes) {
         ++p;
         at = 0;
       }
@@ -261,9 +285,10 @@ void HipBackend::upload_concat(void* target,
     HostPool::shared().run(tasks.size(), [&](std::size_t i) {
       std::memcpy(buf + tasks[i].off, tasks[i].src, tasks[i].bytes);
     });
-    hip_check(hipMemcpyAsync(static_cast<char*>(target) + out, buf, len, hipMemcpyHostToDevice, stream_),
-              "upload_concat");
-    hip_check(hipEventRecord(done[b], stream_), "upload_concat");
+    for (const Send& sd : sends)
+      hip_check(hipMemcpyAsync(sd.dst, buf + sd.off, sd.bytes, hipMemcpyHostToDevice, stream_),
+                "upload_scatter");
+    hip_check(hipEventRecord(done[b], stream_), "upload_scatter");
     busy[b] = true;
     out += len;
   }
@@ -648,6 +673,36 @@ void HipBackend::release() {
   synchronize();
   for (auto& kv : matrices_) free_matrix(&kv.second);
   matrices_.clear();
+  if (spare_slab_.first) hip_check(hipFree(spare_slab_.first), "hipFree(spare slab)");
+  spare_slab_ = {nullptr, 0};
+}
+
+double* HipBackend::take_slab(std::size_t doubles) {
+  if (spare_slab_.first != nullptr && spare_slab_.second >= doubles) {
+    double* p = spare_slab_.first;
+    spare_slab_ = {nullptr, 0};
+    return p;
+  }
+  double* p = nullptr;
+  hip_check(hipMalloc(&p, std::max<std::size_t>(doubles, 1) * sizeof(double)), "hipMalloc(solve state)");
+  return p;
+}
+
+void HipBackend::give_slab(double* slab, std::size_t doubles) {
+  if (slab == nullptr) return;
+  // The slab may still be in use by queued work; the stream orders its next
+  // user after that work, and a hipFree waits for the device.
+  if (spare_slab_.first == nullptr) {
+    spare_slab_ = {slab, doubles};
+    return;
+  }
+  double* drop = slab;
+  if (doubles > spare_slab_.second) {
+    drop = spare_slab_.first;
+    spare_slab_ = {slab, doubles};
+  }
+  (void)hipStreamSynchronize(stream_);
+  (void)hipFree(drop);
 }
 
 void HipBackend::release_one(std::uint64_t id) {

@@ -142,10 +142,27 @@ class HipBackend {
   // one filled by the host pool while the other's copy runs, so a large
   // upload from many pageable pieces moves at DMA speed in few copies.
   void upload_concat(void* target, const std::vector<std::pair<const void*, std::size_t>>& parts);
+  // The same for host ranges that go to different places: each piece's
+  // bytes to its own device address, staged through the same two chunks.
+  struct UploadPiece {
+    const void* host;
+    void* device;
+    std::size_t bytes;
+  };
+  void upload_scatter(const std::vector<UploadPiece>& pieces);
   // The reverse for one range: `bytes` from `source` on the device to
   // `host`, through the same two page-locked chunks (one chunk's copy runs
   // while the host pool empties the other), and waits.
   void download_staged(const void* source, void* host, std::size_t bytes);
+
+  // A solver's device state slab of at least `doubles`: the spare one when
+  // it is large enough, else a new allocation. give_slab hands a slab back;
+  // the larger of it and the spare is kept, the other freed. Solves and
+  // their polishing sub-solves come and go in quick succession, and a
+  // hipMalloc / hipFree pair of a few hundred MB costs milliseconds and a
+  // device-wide wait each time.
+  double* take_slab(std::size_t doubles);
+  void give_slab(double* slab, std::size_t doubles);
 
   // A small device result vector and its pinned host mirror.
   double* result_device() { return result_dev_; }
@@ -208,6 +225,7 @@ class HipBackend {
   std::unordered_map<std::uint64_t, DeviceMatrix> matrices_;
   std::map<int, std::pair<double*, std::size_t>> scratch_;
   std::map<int, std::pair<double*, std::size_t>> pinned_slots_;
+  std::pair<double*, std::size_t> spare_slab_{nullptr, 0};
   double* pinned_ = nullptr;
   std::size_t pinned_size_ = 0;
   double* result_dev_ = nullptr;

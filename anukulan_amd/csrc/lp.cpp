@@ -1,5 +1,7 @@
 #include "lp.hpp"
 
+#include "host_pool.hpp"
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -108,26 +110,24 @@ void Csr::multiply(const double* x, double* y) const {
       y[r] = s;
     }
   };
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const Int threads = static_cast<Int>(std::min(16u, hw));
-  if (nnz() < (Int{1} << 20) || threads < 2 || rows < 2 * threads) {
+  HostPool& pool = HostPool::shared();
+  const Int tasks = static_cast<Int>(4 * pool.threads());
+  if (nnz() < (Int{1} << 20) || pool.threads() < 2 || rows < 2 * tasks) {
     rows_in(0, rows);
     return;
   }
-  // Row ranges of about equal nonzero counts.
-  std::vector<Int> cut(uz(threads) + 1, rows);
+  // Row ranges of about equal nonzero counts on the host pool; every row's
+  // sum is still serial in storage order.
+  std::vector<Int> cut(uz(tasks) + 1, rows);
   cut[0] = 0;
-  const double per = static_cast<double>(nnz()) / threads;
+  const double per = static_cast<double>(nnz()) / tasks;
   Int r = 0;
-  for (Int t = 1; t < threads; ++t) {
+  for (Int t = 1; t < tasks; ++t) {
     const double target = per * t;
     while (r < rows && static_cast<double>(start[uz(r)]) < target) ++r;
     cut[uz(t)] = r;
   }
-  std::vector<std::thread> pool;
-  for (Int t = 1; t < threads; ++t) pool.emplace_back(rows_in, cut[uz(t)], cut[uz(t) + 1]);
-  rows_in(cut[0], cut[1]);
-  for (std::thread& th : pool) th.join();
+  pool.run(uz(tasks), [&](std::size_t t) { rows_in(cut[t], cut[t + 1]); });
 }
 
 bool Csr::validate(std::string* error) const {

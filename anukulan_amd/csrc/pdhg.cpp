@@ -1,6 +1,7 @@
 #include "pdhg.hpp"
 
 #include "batch.hpp"
+#include "host_pool.hpp"
 #include "scaling.hpp"
 
 #include <algorithm>
@@ -9,6 +10,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <iterator>
 #include <map>
 #include <stdexcept>
 #include <utility>
@@ -82,6 +85,9 @@ bool host_farkas(const StandardLp& lp, const std::vector<double>& dy, double tol
 // Nesting depth of solves on this thread: polishing sub-solves share the
 // enclosing solve's device matrices, so only the outermost frees them.
 thread_local int g_depth = 0;
+// The backend's pinned slots for a solve's parameter and check staging, one
+// per nesting depth from this base.
+constexpr int kPinnedSlotBase = 800;
 
 class Solver {
  public:
@@ -135,6 +141,7 @@ class Solver {
 
   // device state, carved from one slab
   double* slab_ = nullptr;
+  std::size_t slab_doubles_ = 0;
   double* xb_[2] = {nullptr, nullptr};
   double* yb_[2] = {nullptr, nullptr};
   double* kxb_[2] = {nullptr, nullptr};
@@ -191,11 +198,7 @@ Solver::~Solver() {
     (void)hipGraphExecDestroy(kv.second.second);
     (void)hipGraphDestroy(kv.second.first);
   }
-  if (slab_ != nullptr) {
-    (void)hipStreamSynchronize(be_.stream());
-    (void)hipFree(slab_);
-  }
-  if (h_pinned_ != nullptr) (void)hipHostFree(h_pinned_);
+  if (slab_ != nullptr) be_.give_slab(slab_, slab_doubles_);
   delete scales_;
   --g_depth;
   if (outermost_ && lp_ != nullptr) {
@@ -242,7 +245,8 @@ void Solver::allocate_device() {
   };
   size_t total = 0;
   for (size_t d : layout) total += round(std::max<size_t>(d, 1));
-  hip_check(hipMalloc(&slab_, total * sizeof(double)), "hipMalloc(solve state)");
+  slab_ = be_.take_slab(total);
+  slab_doubles_ = total;
   hip_check(hipMemsetAsync(slab_, 0, total * sizeof(double), be_.stream()), "hipMemsetAsync(state)");
   double* p = slab_;
   size_t li = 0;
@@ -269,12 +273,17 @@ void Solver::allocate_device() {
   prob_.c = c; prob_.lower = lo; prob_.upper = hi; prob_.q = q;
   prob_.orig_c = oc; prob_.orig_lower = olo; prob_.orig_upper = ohi; prob_.orig_q = oq;
   prob_.row_scale = rs; prob_.col_scale = cs;
-  hip_check(hipHostMalloc(&h_pinned_, 64 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+  // Page-locked staging kept by the backend, one slot per nesting depth (a
+  // polishing sub-solve runs inside its parent's solve).
+  h_pinned_ = be_.pinned_slot(kPinnedSlotBase + g_depth, 64);
 }
 
 void Solver::upload_problem(const std::vector<double>& x0, const std::vector<double>& y0) {
+  // One staged transfer for all twelve vectors (two page-locked chunks in
+  // turn) rather than twelve synchronous copies from pageable memory.
+  std::vector<HipBackend::UploadPiece> pieces;
   auto up = [&](const double* dst, const std::vector<double>& v) {
-    be_.upload(v.data(), const_cast<double*>(dst), static_cast<Int>(v.size()));
+    pieces.push_back({v.data(), const_cast<double*>(dst), v.size() * sizeof(double)});
   };
   up(prob_.c, lp_->c);
   up(prob_.lower, lp_->lower);
@@ -288,6 +297,7 @@ void Solver::upload_problem(const std::vector<double>& x0, const std::vector<dou
   up(prob_.col_scale, scaling_.col_scale);
   up(xb_[0], x0);
   up(yb_[0], y0);
+  be_.upload_scatter(pieces);
 }
 
 void Solver::pull_iterate() {
@@ -866,26 +876,32 @@ void Solver::run_polish(const std::vector<double>& sx, const std::vector<double>
   // finite bound values.
   const bool lend = lp_ == &scaled_;
   if (!feas_built_) {
+    // Each vector of both problems is its own host task: at a million
+    // columns, first-touch page faults dominate a serial build.
+    const StandardLp& s = *lp_;
+    auto finite_to_zero = [](std::vector<double>* v, const std::vector<double>& from, double inf) {
+      v->resize(from.size());
+      for (std::size_t j = 0; j < from.size(); ++j) (*v)[j] = from[j] == inf ? inf : 0.0;
+    };
+    const std::function<void()> build[] = {
+        [&] { primal_feas_.c.assign(s.c.size(), 0.0); },
+        [&] { primal_feas_.q = s.q; },
+        [&] { primal_feas_.lower = s.lower; },
+        [&] { primal_feas_.upper = s.upper; },
+        [&] { primal_feas_.row_model = s.row_model; primal_feas_.row_sign = s.row_sign; },
+        [&] { dual_feas_.c = s.c; },
+        [&] { dual_feas_.q.assign(s.q.size(), 0.0); },
+        [&] { finite_to_zero(&dual_feas_.lower, s.lower, -kInf); },
+        [&] { finite_to_zero(&dual_feas_.upper, s.upper, kInf); },
+        [&] { dual_feas_.row_model = s.row_model; dual_feas_.row_sign = s.row_sign; },
+        [&] { if (!lend) { primal_feas_.k = s.k; primal_feas_.kt = s.kt; } },
+        [&] { if (!lend) { dual_feas_.k = s.k; dual_feas_.kt = s.kt; } },
+    };
+    HostPool::shared().run(std::size(build), [&](std::size_t t) { build[t](); });
     for (StandardLp* f : {&primal_feas_, &dual_feas_}) {
-      if (!lend) {
-        f->k = lp_->k;
-        f->kt = lp_->kt;
-      }
-      f->c = lp_->c;
-      f->q = lp_->q;
-      f->lower = lp_->lower;
-      f->upper = lp_->upper;
-      f->num_equalities = lp_->num_equalities;
-      f->objective_scale = lp_->objective_scale;
+      f->num_equalities = s.num_equalities;
+      f->objective_scale = s.objective_scale;
       f->objective_offset = 0.0;
-      f->row_model = lp_->row_model;
-      f->row_sign = lp_->row_sign;
-    }
-    std::fill(primal_feas_.c.begin(), primal_feas_.c.end(), 0.0);
-    std::fill(dual_feas_.q.begin(), dual_feas_.q.end(), 0.0);
-    for (std::size_t j = 0; j < dual_feas_.lower.size(); ++j) {
-      if (dual_feas_.lower[j] > -kInf) dual_feas_.lower[j] = 0.0;
-      if (dual_feas_.upper[j] < kInf) dual_feas_.upper[j] = 0.0;
     }
     feas_built_ = true;
   }

@@ -468,6 +468,68 @@ def c3_section(ctx, args):
                              "max_rel_objective_diff": worst}}
 
 
+def c4_section(ctx, args, base, cases):
+    """Sensitivities on the case stack: the 61 cases batched to rel KKT
+    1e-8, then the adjoint of the PDHG fixed point for L = w'x (w on the
+    purchase and sale columns) of every case in one launch set, timed; the
+    gradients against the committed one-sided finite differences and the
+    reference simplex's basis sensitivities B^-1 at 200 coordinates per case
+    (tests/golden/adjoint_reference.npz), on the coordinates where the two
+    one-sided differences agree (elsewhere the derivative is set-valued).
+    The reference has no adjoint, so no CPU figure stands beside it."""
+    import numpy as np
+    from anukulan_amd import generators as G, pdhg_adjoint, solve_pdhg_batch
+
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "adjoint_reference.npz"))
+    lps = [G.case_model(base, c).to_lp() for c in cases]
+    t0 = time.perf_counter()
+    rs = solve_pdhg_batch(ctx, lps, tolerance=1e-8, max_iterations=2_000_000,
+                          time_limit_seconds=1e9, use_graphs=0 if args.no_graphs else 1)
+    solve_s = time.perf_counter() - t0
+    ws = [fx[f"stack24_{i}_w_x"] for i in range(len(lps))]
+    t0 = time.perf_counter()
+    adj = pdhg_adjoint(ctx, lps, [r.x for r in rs], [r.y for r in rs], w_x=ws)
+    adjoint_s = time.perf_counter() - t0
+    unique = agree = 0
+    worst_fd = worst_b = 0.0
+    over = 0
+    for i, a in enumerate(adj):
+        g = (a.grad_c, a.grad_q, a.grad_lower, a.grad_upper)
+        got = np.array([g[f][j] for f, j in fx[f"stack24_{i}_coords"]])
+        p, m = fx[f"stack24_{i}_fd_plus"], fx[f"stack24_{i}_fd_minus"]
+        scale = np.maximum(1.0, np.maximum(np.abs(p), np.abs(m)))
+        u = np.isfinite(p) & np.isfinite(m) & (np.abs(p - m) <= 1e-6 * scale)
+        fd = 0.5 * (p + m)
+        basis = fx[f"stack24_{i}_basis"]
+        e_fd = np.abs(got - fd) / np.maximum(1.0, np.abs(fd))
+        v = u & (np.abs(basis - fd) <= 1e-6 * np.maximum(1.0, np.abs(fd)))
+        e_b = np.abs(got - basis) / np.maximum(1.0, np.abs(basis))
+        unique += int(u.sum())
+        agree += int(v.sum())
+        over += int((u & (e_fd > 1e-5)).sum())
+        if u.any():
+            worst_fd = max(worst_fd, float(e_fd[u].max()))
+        if v.any():
+            worst_b = max(worst_b, float(e_b[v].max()))
+    total = 200 * len(adj)
+    return {"workload": f"{len(lps)}-case {args.periods}-period stack: batched PDHG to rel KKT "
+                        f"1e-8, then the adjoint of L = w'x on the purchase and sale columns",
+            "cases": len(lps), "cases_optimal": int(sum(r.status == "optimal" for r in rs)),
+            "solve_s": round(solve_s, 3), "adjoint_s": round(adjoint_s, 3),
+            "adjoint_cases_per_s": round(len(lps) / adjoint_s, 2),
+            "adjoint_converged": int(sum(a.converged for a in adj)),
+            "adjoint_iterations_max": int(max(a.iterations for a in adj)),
+            "coordinates": total, "coordinates_with_a_derivative": unique,
+            "set_valued_share": round(1 - unique / total, 4),
+            "max_rel_error_vs_differences": worst_fd,
+            "coordinates_over_1e-5": over,
+            "over_1e-5_note": "tests/test_gpu_adjoint.py shows each such coordinate to be one "
+                              "along which the basis the adjoint completed is infeasible both "
+                              "ways, and allows at most one in a thousand",
+            "coordinates_where_basis_agrees": agree,
+            "max_rel_error_vs_basis": worst_b}
+
+
 def nodes_section(ctx, args):
     """Branch-and-bound node relaxations: the 64 nodes of the depth-6
     frontier of the 12-period refinery plan (crude purchases and product
@@ -678,6 +740,8 @@ def main():
                          "the timing)")
     ap.add_argument("--no-c1", action="store_true", help="skip the 1M-column random LP")
     ap.add_argument("--no-c3", action="store_true", help="skip the uneven 512-LP stack")
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the sensitivities of the 61-case 24-period stack")
     ap.add_argument("--c3-count", type=int, default=512)
     ap.add_argument("--c3-plain-seconds", type=float, default=10.0,
                     help="time spent on plain one-at-a-time solves of the uneven stack's LPs")
@@ -783,6 +847,7 @@ def main():
     c0 = None if args.no_c0 else c0_section(ctx, args)
     c1 = None if args.no_c1 else c1_section(ctx, args)
     c3 = None if args.no_c3 else c3_section(ctx, args)
+    c4 = None if args.no_c4 or args.periods != 24 or args.cases != 61 else c4_section(ctx, args, base, cases)
     nodes = None if args.no_nodes else nodes_section(ctx, args)
     finish = None if args.no_finish else finish_section(ctx, args, base, cases, lps, optima)
 
@@ -808,7 +873,7 @@ def main():
         "error_vs_simplex": error_vs_optimum(results, optima),
         "roofline": roofline, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_par,
         "cpu_baseline_threaded_backend": cpu_thr, "host": host_info(),
-        "plain": plain, "c0": c0, "c1": c1, "c3": c3, "nodes": nodes, "finish": finish,
+        "plain": plain, "c0": c0, "c1": c1, "c3": c3, "c4": c4, "nodes": nodes, "finish": finish,
         "batch_phases_s": {k: round(v, 4) for k, v in results[0].phases.items()},
         "autotune": parse_tuning(tuning),
         "kernels": {k: {"mean_us": v["mean_us"], "calls": v["calls"], "GB/s": v["gbs"]}

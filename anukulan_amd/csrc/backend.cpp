@@ -677,29 +677,32 @@ void HipBackend::release() {
   spare_slab_ = {nullptr, 0};
 }
 
-double* HipBackend::take_slab(std::size_t doubles) {
+double* HipBackend::take_slab(std::size_t doubles, std::size_t* capacity) {
   if (spare_slab_.first != nullptr && spare_slab_.second >= doubles) {
     double* p = spare_slab_.first;
+    *capacity = spare_slab_.second;
     spare_slab_ = {nullptr, 0};
     return p;
   }
   double* p = nullptr;
-  hip_check(hipMalloc(&p, std::max<std::size_t>(doubles, 1) * sizeof(double)), "hipMalloc(solve state)");
+  const std::size_t size = std::max<std::size_t>(doubles, 1);
+  hip_check(hipMalloc(&p, size * sizeof(double)), "hipMalloc(solve state)");
+  *capacity = size;
   return p;
 }
 
-void HipBackend::give_slab(double* slab, std::size_t doubles) {
+void HipBackend::give_slab(double* slab, std::size_t capacity) {
   if (slab == nullptr) return;
   // The slab may still be in use by queued work; the stream orders its next
   // user after that work, and a hipFree waits for the device.
   if (spare_slab_.first == nullptr) {
-    spare_slab_ = {slab, doubles};
+    spare_slab_ = {slab, capacity};
     return;
   }
   double* drop = slab;
-  if (doubles > spare_slab_.second) {
+  if (capacity > spare_slab_.second) {
     drop = spare_slab_.first;
-    spare_slab_ = {slab, doubles};
+    spare_slab_ = {slab, capacity};
   }
   (void)hipStreamSynchronize(stream_);
   (void)hipFree(drop);

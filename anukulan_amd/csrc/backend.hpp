@@ -161,8 +161,9 @@ class HipBackend {
   // their polishing sub-solves come and go in quick succession, and a
   // hipMalloc / hipFree pair of a few hundred MB costs milliseconds and a
   // device-wide wait each time.
-  double* take_slab(std::size_t doubles);
-  void give_slab(double* slab, std::size_t doubles);
+  // *capacity receives the slab's true size, which give_slab takes back.
+  double* take_slab(std::size_t doubles, std::size_t* capacity);
+  void give_slab(double* slab, std::size_t capacity);
 
   // A small device result vector and its pinned host mirror.
   double* result_device() { return result_dev_; }

@@ -141,7 +141,7 @@ class Solver {
 
   // device state, carved from one slab
   double* slab_ = nullptr;
-  std::size_t slab_doubles_ = 0;
+  std::size_t slab_doubles_ = 0;  // the slab's capacity
   double* xb_[2] = {nullptr, nullptr};
   double* yb_[2] = {nullptr, nullptr};
   double* kxb_[2] = {nullptr, nullptr};
@@ -245,8 +245,7 @@ void Solver::allocate_device() {
   };
   size_t total = 0;
   for (size_t d : layout) total += round(std::max<size_t>(d, 1));
-  slab_ = be_.take_slab(total);
-  slab_doubles_ = total;
+  slab_ = be_.take_slab(total, &slab_doubles_);
   hip_check(hipMemsetAsync(slab_, 0, total * sizeof(double), be_.stream()), "hipMemsetAsync(state)");
   double* p = slab_;
   size_t li = 0;

@@ -210,6 +210,9 @@ SIGNATURES = {
     "anuk_finish_default_options": (None, [POINTER(FinishOptions)]),
     "anuk_exact_finish": (c_int, [POINTER(c_void_p), c_int32, DP, DP, POINTER(FinishOptions),
                                   POINTER(FinishResult), DP, DP, IP]),
+    "anuk_exact_finish_basis": (c_int, [POINTER(c_void_p), c_int32, DP, DP, IP, IP,
+                                        POINTER(FinishOptions), POINTER(FinishResult), DP, DP, IP,
+                                        IP]),
     "anuk_adjoint_default_options": (None, [POINTER(AdjointOptions)]),
     "anuk_pdhg_adjoint": (c_int, [c_void_p, POINTER(c_void_p), c_int32, POINTER(PdhgOptions),
                                   POINTER(AdjointOptions), DP, DP, DP, DP, DP, DP, DP, DP,

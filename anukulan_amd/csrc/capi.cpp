@@ -715,11 +715,16 @@ void anuk_finish_default_options(anuk_finish_options* options) {
   options->threads = f.threads;
 }
 
-int anuk_exact_finish(const anuk_lp* const* lps, int32_t count, const double* x, const double* y,
-                      const anuk_finish_options* options, anuk_finish_result* results,
-                      double* x_out, double* y_out, int32_t* basic_out) {
+namespace {
+
+int finish_impl(const anuk_lp* const* lps, int32_t count, const double* x, const double* y,
+                const int32_t* start_basic, const int32_t* start_state,
+                const anuk_finish_options* options, anuk_finish_result* results, double* x_out,
+                double* y_out, int32_t* basic_out, int32_t* state_out) {
   if (lps == nullptr || count < 0 || (count > 0 && results == nullptr))
     return status_of(nullptr, ANUK_ERR_INVALID, "null argument");
+  if ((start_basic == nullptr) != (start_state == nullptr))
+    return status_of(nullptr, ANUK_ERR_INVALID, "a start basis needs both its columns and its states");
   return guarded(nullptr, [&] {
     anuk_finish_options o;
     anuk_finish_default_options(&o);
@@ -739,7 +744,22 @@ int anuk_exact_finish(const anuk_lp* const* lps, int32_t count, const double* x,
       co += static_cast<size_t>(lps[i]->lp.cols());
       ro += static_cast<size_t>(lps[i]->lp.rows());
     }
-    const std::vector<anuk::FinishResult> rs = anuk::exact_finish_many(v, xs, ys, finish_options_of(o));
+    anuk::FinishBasis start;
+    if (start_basic != nullptr && count > 0) {
+      const anuk::StandardLp& f = *v[0];
+      for (const anuk::StandardLp* lp : v)
+        need(lp->rows() == f.rows() && lp->cols() == f.cols(),
+             "a shared start basis needs every LP to have the same shape");
+      const size_t rows = static_cast<size_t>(f.rows()), cols = static_cast<size_t>(f.cols());
+      start.basic.assign(start_basic, start_basic + rows);
+      start.state.resize(rows + cols);
+      for (size_t j = 0; j < rows + cols; ++j) {
+        need(start_state[j] >= 0 && start_state[j] <= 3, "a start state is not 0 to 3");
+        start.state[j] = static_cast<anuk::VarState>(start_state[j]);
+      }
+    }
+    const std::vector<anuk::FinishResult> rs = anuk::exact_finish_many(
+        v, xs, ys, finish_options_of(o), start_basic != nullptr ? &start : nullptr);
     co = ro = 0;
     for (size_t i = 0; i < rs.size(); ++i) {
       const anuk::FinishResult& r = rs[i];
@@ -766,10 +786,31 @@ int anuk_exact_finish(const anuk_lp* const* lps, int32_t count, const double* x,
       if (y_out && r.y.size() == rows) std::memcpy(y_out + ro, r.y.data(), rows * sizeof(double));
       if (basic_out && r.basic.size() == rows)
         for (size_t k = 0; k < rows; ++k) basic_out[ro + k] = r.basic[k];
+      if (state_out && r.state.size() == rows + cols)
+        for (size_t k = 0; k < rows + cols; ++k)
+          state_out[co + ro + k] = static_cast<int32_t>(r.state[k]);
       co += cols;
       ro += rows;
     }
   });
+}
+
+}  // namespace
+
+int anuk_exact_finish(const anuk_lp* const* lps, int32_t count, const double* x, const double* y,
+                      const anuk_finish_options* options, anuk_finish_result* results,
+                      double* x_out, double* y_out, int32_t* basic_out) {
+  return finish_impl(lps, count, x, y, nullptr, nullptr, options, results, x_out, y_out, basic_out,
+                     nullptr);
+}
+
+int anuk_exact_finish_basis(const anuk_lp* const* lps, int32_t count, const double* x,
+                            const double* y, const int32_t* start_basic,
+                            const int32_t* start_state, const anuk_finish_options* options,
+                            anuk_finish_result* results, double* x_out, double* y_out,
+                            int32_t* basic_out, int32_t* state_out) {
+  return finish_impl(lps, count, x, y, start_basic, start_state, options, results, x_out, y_out,
+                     basic_out, state_out);
 }
 
 void anuk_adjoint_default_options(anuk_adjoint_options* options) {

@@ -708,7 +708,7 @@ double seconds_since(std::chrono::steady_clock::time_point t) {
 }  // namespace
 
 FinishResult exact_finish(const StandardLp& lp, const double* x, const double* y,
-                          const FinishOptions& opt) {
+                          const FinishOptions& opt, const FinishBasis* warm) {
   const auto start = std::chrono::steady_clock::now();
   FinishResult res;
   const Int n = lp.cols(), m = lp.rows();
@@ -727,10 +727,39 @@ FinishResult exact_finish(const StandardLp& lp, const double* x, const double* y
     head[uz(i)] = n + i;
     state[uz(n + i)] = VarState::kBasic;
   }
+  const bool warm_basis = warm != nullptr && warm->basic.size() == uz(m) &&
+                         warm->state.size() == uz(n + m);
+  if (warm != nullptr && !warm_basis)
+    throw std::invalid_argument("finish: a start basis needs one column per row and one state per column");
+  if (warm_basis) {
+    head = warm->basic;
+    state = warm->state;
+    std::vector<char> in_head(uz(n + m), 0);
+    for (const Int j : head) {
+      if (j < 0 || j >= n + m || in_head[uz(j)])
+        throw std::invalid_argument("finish: a start basis names a column twice or out of range");
+      in_head[uz(j)] = 1;
+    }
+    // States follow the head, and a nonbasic state sits on a bound this
+    // case has.
+    for (Int j = 0; j < n + m; ++j) {
+      VarState& v = state[uz(j)];
+      if (in_head[uz(j)]) {
+        v = VarState::kBasic;
+        continue;
+      }
+      const bool lo = f.lower[uz(j)] > -kInf, hi = f.upper[uz(j)] < kInf;
+      if (v == VarState::kBasic || (v == VarState::kLower && !lo) || (v == VarState::kUpper && !hi) ||
+          (v == VarState::kFree && (lo || hi)))
+        v = lo ? VarState::kLower : (hi ? VarState::kUpper : VarState::kFree);
+    }
+  }
   Simplex sx(f, opt);
   try {
     sx.install(head, state);
-    if (opt.crossover && x != nullptr) {
+    if (warm_basis) {
+      res.started_warm = true;
+    } else if (opt.crossover && x != nullptr) {
       std::vector<double> xs(uz(n)), rc, slack(uz(m), 0.0);
       for (Int j = 0; j < n; ++j) xs[uz(j)] = x[uz(j)] / sc.col_scale[uz(j)];
       std::vector<double> kx(uz(m), 0.0);
@@ -794,7 +823,7 @@ FinishResult exact_finish(const StandardLp& lp, const double* x, const double* y
 std::vector<FinishResult> exact_finish_many(const std::vector<const StandardLp*>& lps,
                                             const std::vector<const double*>& xs,
                                             const std::vector<const double*>& ys,
-                                            const FinishOptions& opt) {
+                                            const FinishOptions& opt, const FinishBasis* start) {
   const size_t count = lps.size();
   if (xs.size() != count || ys.size() != count) throw std::invalid_argument("finish: one point per LP");
   std::vector<FinishResult> out(count);
@@ -806,7 +835,8 @@ std::vector<FinishResult> exact_finish_many(const std::vector<const StandardLp*>
   for (size_t t = 0; t < threads; ++t) {
     pool.emplace_back([&, t] {
       try {
-        for (size_t i = t; i < count; i += threads) out[i] = exact_finish(*lps[i], xs[i], ys[i], opt);
+        for (size_t i = t; i < count; i += threads)
+          out[i] = exact_finish(*lps[i], xs[i], ys[i], opt, start);
       } catch (...) {
         errors[t] = std::current_exception();
       }

@@ -72,16 +72,31 @@ struct FinishResult {
   std::string message;
 };
 
+// A basis to start from: the column of each basis position (n + i: row i's
+// logical) and the state of every column of [K | -I], as FinishResult holds
+// them.
+struct FinishBasis {
+  std::vector<Int> basic;
+  std::vector<VarState> state;
+};
+
 // x (cols) and y (rows) are an unscaled primal-dual point for lp, or null
 // for a cold start (y alone may be null: bounds are then chosen by
-// proximity, as crossover_basis does without a dual).
+// proximity, as crossover_basis does without a dual). With `start`, the
+// simplex starts from that basis instead (the point is not used): the
+// reference's case-stack workflow, where the base plan's final basis
+// warm-starts each what-if case (app/anukulan_cli.cpp:1065-1230). A
+// nonbasic state the case's bounds cannot hold moves to a bound they can,
+// and a singular start is repaired with logicals.
 FinishResult exact_finish(const StandardLp& lp, const double* x, const double* y,
-                          const FinishOptions& opt);
+                          const FinishOptions& opt, const FinishBasis* start = nullptr);
 
-// Many cases at once, one per host thread.
+// Many cases at once, one per host thread; `start`, when given, warm-starts
+// every one of them.
 std::vector<FinishResult> exact_finish_many(const std::vector<const StandardLp*>& lps,
                                             const std::vector<const double*>& xs,
                                             const std::vector<const double*>& ys,
-                                            const FinishOptions& opt);
+                                            const FinishOptions& opt,
+                                            const FinishBasis* start = nullptr);
 
 }  // namespace anuk

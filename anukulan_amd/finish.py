@@ -11,6 +11,13 @@ something are finished here:
     res = solve_pdhg_batch(ctx, lps, tolerance=1e-6)
     fin = exact_finish(lps, [r.x for r in res], [r.y for r in res])
     fin[i].objective, fin[i].x, fin[i].basic
+
+The reference's case-stack workflow (app/anukulan_cli.cpp:1065-1230) starts
+every what-if case from the base plan's final basis instead
+(anuk_exact_finish_basis); a FinishResult is such a start:
+
+    base = exact_finish(lps[:1], [res[0].x], [res[0].y])[0]
+    rest = exact_finish(lps[1:], start=base)
 """
 from __future__ import annotations
 
@@ -53,16 +60,20 @@ class FinishResult:
     x: np.ndarray = field(repr=False)
     y: np.ndarray = field(repr=False)
     basic: np.ndarray = field(repr=False)
+    # each column of [K | -I]: 0 basic, 1 at lower, 2 at upper, 3 free
+    state: np.ndarray = field(repr=False, default=None)
 
     @property
     def iterations(self) -> int:
         return self.info["iterations"]
 
 
-def exact_finish(lps, xs=None, ys=None, **overrides) -> list:
+def exact_finish(lps, xs=None, ys=None, start=None, **overrides) -> list:
     """Finishes each LP of `lps` exactly. xs[i] (cols) and ys[i] (rows) are
     its first-order point; xs=None starts every LP cold from all logicals,
-    ys=None places nonbasic columns by proximity alone."""
+    ys=None places nonbasic columns by proximity alone. start (a
+    FinishResult of an LP of the same shape, or a (basic, state) pair)
+    starts every LP from that basis instead of its point."""
     lps = list(lps)
     if not lps:
         return []
@@ -80,14 +91,24 @@ def exact_finish(lps, xs=None, ys=None, **overrides) -> list:
                                     if count else np.zeros(0))
     x = stacked(xs, cols, "primal")
     y = stacked(ys, rows, "dual")
+    sb = ss = None
+    if start is not None:
+        basic, state = (start.basic, start.state) if isinstance(start, FinishResult) else start
+        sb = np.ascontiguousarray(basic, np.int32)
+        ss = np.ascontiguousarray(state, np.int32)
+        if sb.shape != (rows[0],) or ss.shape != (rows[0] + cols[0],):
+            raise ValueError("a start basis holds one column per row and one state per column")
     o = options(**overrides)
     handles = (c_void_p * count)(*[lp.h for lp in lps])
     results = (_lib.FinishResult * count)()
     xo, yo = np.empty(sum(cols)), np.empty(sum(rows))
     bo = np.empty(sum(rows), np.int32)
-    rc = lib.anuk_exact_finish(handles, count, _dp(x) if x is not None else None,
-                               _dp(y) if y is not None else None, byref(o), results, _dp(xo),
-                               _dp(yo), _ip(bo))
+    so = np.empty(sum(rows) + sum(cols), np.int32)
+    rc = lib.anuk_exact_finish_basis(handles, count, _dp(x) if x is not None else None,
+                                     _dp(y) if y is not None else None,
+                                     _ip(sb) if sb is not None else None,
+                                     _ip(ss) if ss is not None else None, byref(o), results,
+                                     _dp(xo), _dp(yo), _ip(bo), _ip(so))
     if rc != 0:
         raise AnukError(rc, lib.anuk_last_error(None).decode())
     out, co, ro = [], 0, 0
@@ -100,7 +121,8 @@ def exact_finish(lps, xs=None, ys=None, **overrides) -> list:
             seconds=r.seconds, message=r.message.decode(errors="replace"),
             info={k: int(getattr(r, k)) for k in INFO},
             x=xo[co:co + cols[i]].copy(), y=yo[ro:ro + rows[i]].copy(),
-            basic=bo[ro:ro + rows[i]].copy()))
+            basic=bo[ro:ro + rows[i]].copy(),
+            state=so[co + ro:co + ro + cols[i] + rows[i]].copy()))
         co += cols[i]
         ro += rows[i]
     return out

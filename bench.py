@@ -648,6 +648,27 @@ def finish_section(ctx, args, base, cases, lps, optima):
                                                                 key=lambda i: optima[i])
         deciding["max_rel_error_vs_simplex"] = max(
             abs(f.objective - optima[i]) / max(1.0, abs(optima[i])) for f, i in zip(fin, top))
+    # The reference's case-stack workflow with this library's pieces: the
+    # base plan's first-order point (device, 1e-4) crossed over and finished,
+    # then every what-if case started from the base's final basis on the
+    # host cores (anuk_exact_finish_basis), not from its own point.
+    from anukulan_amd import solve_pdhg
+    t0 = time.perf_counter()
+    p0 = solve_pdhg(ctx, lps[0], tolerance=1e-4)
+    f0 = exact_finish(lps[:1], [p0.x], [p0.y])[0]
+    base_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rest = exact_finish(lps[1:], start=f0)
+    rest_s = time.perf_counter() - t0
+    wf = [f0] + rest
+    wf_err = ([abs(w.objective - o) / max(1.0, abs(o)) for w, o in zip(wf, optima)]
+              if optima else [])
+    from_base = {"base_s": round(base_s, 3), "warm_s": round(rest_s, 3),
+                 "cases_per_s": round(len(lps) / (base_s + rest_s), 3),
+                 "base_pivots": f0.iterations, "warm_pivots": sum(w.iterations for w in rest),
+                 "warm_phase_one_pivots": sum(w.info["phase_one_iterations"] for w in rest),
+                 "cases_optimal": sum(w.status == "optimal" for w in wf),
+                 "max_rel_error_vs_simplex": max(wf_err) if wf_err else None}
     t0 = time.perf_counter()
     cold = exact_finish(lps)
     cold_s = time.perf_counter() - t0
@@ -704,10 +725,16 @@ def finish_section(ctx, args, base, cases, lps, optima):
                           "warm-started from its basis (dual simplex), one thread"}
     best = max(seeds, key=lambda k: seeds[k]["cases_per_s"])
     threads = min(16, len(os.sched_getaffinity(0)))
-    return {"workload": f"{len(lps)}-case stack: batched PDHG to a seed tolerance, then crossover "
-                        f"and primal simplex per case on {threads} host threads",
-            "cases_per_s": seeds[best]["cases_per_s"], "best_seed_tolerance": best,
+    top_rate = max(seeds[best]["cases_per_s"], from_base["cases_per_s"])
+    return {"workload": f"{len(lps)}-case stack finished to optimal bases on {threads} host "
+                        f"threads: each case from its own batched PDHG point (crossover, then "
+                        f"primal simplex), or every case from the base plan's final basis",
+            "cases_per_s": top_rate,
+            "best_method": ("from the base plan's basis" if top_rate == from_base["cases_per_s"]
+                            else f"own PDHG point at {best}"),
+            "best_seed_tolerance": best,
             "by_seed_tolerance": seeds,
+            "from_base_basis": from_base,
             "deciding_cases": deciding,
             "cold": {"seconds": round(cold_s, 3), "cases_per_s": round(len(lps) / cold_s, 3),
                      "pivots": sum(c.iterations for c in cold),

@@ -560,6 +560,26 @@ int anuk_exact_finish(const anuk_lp* const* lps, int32_t count, const double* x,
                       const anuk_finish_options* options, anuk_finish_result* results,
                       double* x_out, double* y_out, int32_t* basic_out);
 
+/* The same, with the case-stack workflow of the reference
+ * (app/anukulan_cli.cpp:1065-1230, `family --from-first`): given a start
+ * basis (start_basic: rows entries as basic_out gives them; start_state:
+ * cols + rows entries, 0 basic, 1 at lower, 2 at upper, 3 free, as
+ * SimplexOptions::start_basic / start_status, simplex.hpp:295-301), every LP
+ * starts its simplex from that basis instead of a crossover of its point,
+ * and so all LPs must have the same shape. The reference runs its dual
+ * simplex from the base plan's basis; here it is this library's primal
+ * simplex, whose composite phase one takes a basis that a case's new bounds
+ * or right-hand side made infeasible. A nonbasic state a case's bounds
+ * cannot hold moves to a bound they can. start_basic and start_state are
+ * both NULL or both given; with NULL this is anuk_exact_finish. state_out
+ * (sum of cols + rows, NULL allowed) receives each LP's final states in the
+ * same codes, so one LP's result can warm-start the others. */
+int anuk_exact_finish_basis(const anuk_lp* const* lps, int32_t count, const double* x,
+                            const double* y, const int32_t* start_basic,
+                            const int32_t* start_state, const anuk_finish_options* options,
+                            anuk_finish_result* results, double* x_out, double* y_out,
+                            int32_t* basic_out, int32_t* state_out);
+
 /* ------------------------------------------------------------------------ */
 /* Sensitivities of a solution (no reference counterpart)                    */
 /* ------------------------------------------------------------------------ */

@@ -194,6 +194,65 @@ def test_refinery_mps_presolved_finished_and_postsolved(ref):
     assert abs(r.objective - obj) <= 1e-12 * abs(obj)
 
 
+def test_stack_cases_from_the_base_basis_reach_the_optima():
+    # The reference's case-stack workflow (app/anukulan_cli.cpp:1065-1230):
+    # the base plan's final basis starts each what-if case. Price moves
+    # leave that basis primal feasible; supply, demand and capacity moves can
+    # make it infeasible, which the composite phase one repairs.
+    base, cases = G.case_stack(STACK["periods"])
+    picks = [0, 2, 19, 26, 29, 34, 41, 48, 53, 60]   # base; price; supply; demand; capacity
+    lps = [G.case_model(base, cases[i]).to_lp() for i in picks]
+    f0 = exact_finish(lps[:1])[0]
+    assert f0.status == "optimal" and f0.state.shape == (lps[0].rows + lps[0].cols,)
+    assert np.array_equal(np.flatnonzero(f0.state == 0), np.sort(f0.basic))
+    warm = exact_finish(lps[1:], start=f0)
+    cold = exact_finish(lps[1:])
+    for k, (w, c) in enumerate(zip(warm, cold)):
+        i = picks[k + 1]
+        assert w.status == "optimal" and w.started_warm, (i, w.message)
+        assert rel(w.objective, STACK["simplex_objective"][i]) <= 1e-9, (i, w.objective)
+        assert w.primal_violation <= 1e-6 and w.dual_violation <= 1e-6, i
+    assert sum(w.iterations for w in warm) * 3 < sum(c.iterations for c in cold)
+    # a (basic, state) pair works as well as a FinishResult
+    again = exact_finish(lps[1:2], start=(f0.basic, f0.state))[0]
+    assert again.objective == warm[0].objective and again.iterations == warm[0].iterations
+
+
+def test_a_start_state_the_bounds_cannot_hold_is_moved():
+    lp = natural("wyndor").to_lp()
+    f = exact_finish([lp])[0]
+    state = f.state.copy()
+    # every nonbasic column claims its upper bound, which wyndor's columns
+    # and logicals do not have
+    state[state != 0] = 2
+    r = exact_finish([lp], start=(f.basic, state))[0]
+    assert r.status == "optimal" and abs(r.objective - 36.0) <= 1e-9
+    # an all-logical start is a valid start too
+    logical = np.arange(lp.cols, lp.cols + lp.rows, dtype=np.int32)
+    st = np.ones(lp.cols + lp.rows, np.int32)
+    st[logical] = 0
+    r = exact_finish([lp], start=(logical, st))[0]
+    assert r.status == "optimal" and abs(r.objective - 36.0) <= 1e-9
+
+
+def test_bad_start_bases():
+    lp = natural("wyndor").to_lp()
+    f = exact_finish([lp])[0]
+    with pytest.raises(ValueError):
+        exact_finish([lp], start=(f.basic[:-1], f.state))
+    bad = f.state.copy()
+    bad[0] = 7
+    with pytest.raises(AnukError):
+        exact_finish([lp], start=(f.basic, bad))
+    twice = f.basic.copy()
+    twice[-1] = twice[0]
+    with pytest.raises(AnukError):
+        exact_finish([lp], start=(twice, f.state))
+    other = natural("equalities").to_lp()
+    with pytest.raises(AnukError):
+        exact_finish([lp, other], start=f)
+
+
 def test_bad_arguments():
     lp = natural("wyndor").to_lp()
     with pytest.raises(AnukError):

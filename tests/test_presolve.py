@@ -353,6 +353,10 @@ def test_refinery_mps_keeps_its_optimum_beside_the_reference_presolve(ref):
         os.unlink(f.name)
     ps = presolve(nm, slack_column_singletons=1)
     r = check_round_trip(ref, nm, ps, tol=1e-8)
+    # the file's published optimum (bench/results/ipm_refinery.txt:19),
+    # reached from the reduced model
+    _, _, reduced_opt = exact(ref, ps.model())
+    assert abs(reduced_opt - 2.426062266887e10) <= 1e-12 * 2.426062266887e10
     theirs = ref.RefPresolve(text)
     assert theirs.status == "reduced"
     # both reductions reach the original optimum; sizes are reported

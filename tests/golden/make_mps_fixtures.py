@@ -7,7 +7,8 @@
   tests/golden/mps_corpus.py and of the two files, as digests; the standard
   form the reference builds from refinery.mps (to_standard_form); and the
   reference's solve_pdhg on it (CPU backend, 8 threads, which gives the
-  serial backend's bits) at 1e-4 and 1e-6.
+  serial backend's bits) at 1e-4 and 1e-6, and on the LP the reference's
+  own presolve makes of it.
 
     make -C oracle && python tests/golden/make_mps_fixtures.py
 """
@@ -51,6 +52,20 @@ def main():
     # The optimum by the reference's interior point method with presolve
     # (bench/results/ipm_refinery.txt:19).
     out["pdhg"]["refinery"]["optimum"] = 2.426062266887e+10
+    # The reference's solve_pdhg on the LP its own presolve (default
+    # PresolveOptions) makes of refinery.mps: the configuration the
+    # reference's refinery figure is quoted with (docs/RESULTS.md:1834-1838).
+    pre = _oracle.RefPresolve(raw)
+    plp = pre.lp()
+    out["pdhg"]["refinery_presolved"] = {"shape": [plp.rows, plp.cols, plp.nnz],
+                                         "rows_removed": pre.counts["rows_removed"],
+                                         "cols_removed": pre.counts["cols_removed"]}
+    for tol in ("1e-4", "1e-6"):
+        s = plp.solve(threads=8, tolerance=float(tol), max_iterations=2_000_000,
+                      time_limit_seconds=1e9)
+        out["pdhg"]["refinery_presolved"][tol] = {
+            "status": s.status, "iterations": s.iterations, "objective": s.objective,
+            "restarts": s.restarts, "polish_attempts": s.polish_attempts}
     path = os.path.join(HERE, "mps_reference.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)

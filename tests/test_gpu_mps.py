@@ -47,3 +47,58 @@ def test_refinery_mps_batched_copies_agree(ctx, refinery_lp):
     assert all(r.status == "optimal" for r in rs)
     assert rs[0].iterations == rs[1].iterations == rs[2].iterations
     assert abs(rs[0].iterations - FIX["1e-4"]["iterations"]) <= 0.02 * FIX["1e-4"]["iterations"]
+
+
+PRESOLVED = json.load(open(os.path.join(GOLD, "mps_reference.json")))["pdhg"]["refinery_presolved"]
+
+
+@pytest.mark.parametrize("tol", ["1e-4", "1e-6"])
+def test_reference_presolved_refinery_matches_reference_pdhg(ctx, ref, tol):
+    # The reference's refinery figure is quoted with its presolve on
+    # (docs/RESULTS.md:1834-1838). Its presolve (oracle/_ref, test
+    # infrastructure) makes the reduced LP; the GPU solve of that LP must
+    # match the reference's solve_pdhg on it, and postsolved by the
+    # reference's own postsolve it is a point of the model as read.
+    from anukulan_amd import LP
+    raw = gzip.decompress(open(os.path.join(GOLD, "refinery.mps.gz"), "rb").read())
+    P = ref.RefPresolve(raw)
+    assert P.status == "reduced"
+    e = P.lp().export()
+    rows, cols = len(e["q"]), len(e["c"])
+    assert [rows, cols, len(e["value"])] == PRESOLVED["shape"]
+    lp = LP.from_standard(rows, cols, e["num_equalities"], e["start"], e["index"], e["value"],
+                          e["c"], e["q"], e["lower"], e["upper"], e["objective_scale"],
+                          e["objective_offset"])
+    r = solve_pdhg(ctx, lp, tolerance=float(tol), max_iterations=2_000_000, time_limit_seconds=1e9)
+    want = PRESOLVED[tol]
+    print(f"\npresolved refinery.mps at {tol}: {r.iterations} iterations (reference "
+          f"{want['iterations']}), objective {r.objective!r} (reference {want['objective']!r})")
+    assert r.status == want["status"] == "optimal"
+    assert abs(r.objective - want["objective"]) <= 1e-8 * abs(want["objective"])
+    assert abs(r.iterations - want["iterations"]) <= 0.02 * want["iterations"]
+    m = read_mps_text(raw, "refinery.mps")
+    x = P.postsolve_primal(r.x)
+    obj = float(m.objective @ x) + m.objective_offset
+    assert abs(obj - r.objective) <= 1e-9 * abs(obj)
+    assert abs(obj - FIX["optimum"]) <= (1e-3 if tol == "1e-4" else 1e-6) * abs(FIX["optimum"])
+
+
+def test_own_presolve_of_refinery_solves_to_the_optimum(ctx):
+    # The library's own presolve (columns alone in equality rows substituted
+    # out, as the reference's presolve does on this file), solved on the
+    # GPU and postsolved: the published optimum to the solve's accuracy.
+    from anukulan_amd.presolve import presolve
+    raw = gzip.decompress(open(os.path.join(GOLD, "refinery.mps.gz"), "rb").read())
+    nm = read_mps_text(raw, "refinery.mps").to_natural()
+    ps = presolve(nm, slack_column_singletons=1)
+    assert ps.cols < nm.cols
+    lp = ps.model().to_lp()
+    r = solve_pdhg(ctx, lp, tolerance=1e-6, max_iterations=2_000_000, time_limit_seconds=1e9)
+    assert r.status == "optimal"
+    x, _ = ps.postsolve_result(0, lp, r)
+    obj = float(nm.objective @ x) + nm.objective_offset
+    print(f"\nown presolve of refinery.mps: {ps.summary()}; {r.iterations} iterations at 1e-6 "
+          f"(reference presolve {PRESOLVED['1e-6']['iterations']}, none "
+          f"{FIX['1e-6']['iterations']}); objective {obj!r}")
+    assert abs(obj - r.objective) <= 1e-9 * abs(obj)
+    assert abs(obj - FIX["optimum"]) <= 1e-6 * abs(FIX["optimum"])

@@ -285,6 +285,42 @@ def test_a_reduction_one_case_forbids_is_not_taken(ref):
         check_round_trip(ref, G.case_model(nm, case), both, k)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_a_stack_whose_cases_move_different_numbers(ref, seed):
+    # Every case allows the same reductions, but each moves its own numbers:
+    # fixed columns at a different value per case (shifting that case's row
+    # bounds and offset), singleton rows with per-case bounds, per-case
+    # costs (moved by the column substitutions) and right-hand sides.
+    nm = planted_lp(200 + seed)
+    g = np.random.default_rng(seed)
+    fixed = np.flatnonzero(nm.col_lower == nm.col_upper)
+    has_entries = np.bincount(nm.row_index, minlength=nm.rows) > 0
+    cases = []
+    for k in range(3):
+        c = Case(f"c{k}", nm.objective * g.uniform(0.8, 1.2, nm.cols), nm.row_lower.copy(),
+                 nm.row_upper.copy(), nm.col_lower.copy(), nm.col_upper.copy())
+        shift = g.uniform(-0.3, 0.3, fixed.size)
+        c.col_lower[fixed] += shift
+        c.col_upper[fixed] += shift
+        # widen every finite row side a little, differently per case
+        c.row_lower[np.isfinite(c.row_lower)] -= g.uniform(0.0, 0.5, np.isfinite(c.row_lower).sum())
+        c.row_upper[np.isfinite(c.row_upper)] += g.uniform(0.0, 0.5, np.isfinite(c.row_upper).sum())
+        eq = (nm.row_lower == nm.row_upper) & has_entries   # an empty row keeps 0 = 0
+        c.row_lower[eq] = c.row_upper[eq] = nm.row_lower[eq] + g.uniform(-0.2, 0.2, eq.sum())
+        cases.append(c)
+    ps = presolve_stack(nm, cases, slack_column_singletons=1)
+    assert ps.status == "reduced"
+    assert ps.counts["fixed_columns"] >= 2 and ps.counts["slack_column_singletons"] >= 1
+    solved = 0
+    for k, c in enumerate(cases):
+        cm = G.case_model(nm, c)
+        if ref.RefLp.from_natural(cm).simplex()["status"] != 0:
+            continue   # a perturbation may leave a case without an optimum
+        check_round_trip(ref, cm, ps, k)
+        solved += 1
+    assert solved >= 1
+
+
 def test_infeasible_and_unbounded_name_the_case():
     from golden.known_lps import natural
     nm = natural("wyndor")

@@ -388,11 +388,13 @@ const DevCsr& HipBackend::prepare(const CsrView& a) {
     throw std::invalid_argument("prepare: malformed matrix");
   DeviceMatrix dm;
   const Int nnz = a.nnz();
+  // The arrays cross together, staged through the page-locked chunks.
+  std::vector<UploadPiece> pieces;
   auto put = [&](const void* src, size_t bytes) -> void* {
     void* p = nullptr;
     hip_check(hipMalloc(&p, std::max<size_t>(bytes, 8)), "hipMalloc(matrix)");
     dm.owned.push_back(p);
-    if (bytes > 0) hip_check(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, stream_), "upload matrix");
+    if (bytes > 0) pieces.push_back({src, p, bytes});
     return p;
   };
   Int* block_rows = nullptr;
@@ -408,7 +410,13 @@ const DevCsr& HipBackend::prepare(const CsrView& a) {
   dm.view.value = static_cast<const double*>(put(a.value, static_cast<size_t>(nnz) * sizeof(double)));
   dm.view.block_row =
       static_cast<const Int*>(put(block_rows, (static_cast<size_t>(blocks) + 1) * sizeof(Int)));
-  synchronize();
+  try {
+    upload_scatter(pieces);
+  } catch (...) {
+    std::free(block_rows);
+    free_matrix(&dm);
+    throw;
+  }
   std::free(block_rows);
   auto ins = matrices_.emplace(a.id, std::move(dm));
   return ins.first->second.view;

@@ -9,30 +9,32 @@ namespace anuk {
 
 namespace {
 
-// Device buffers for one scaling, freed on every path out.
+// Device buffers for one scaling, freed on every path out. Their host
+// contents go up together (send) through the page-locked chunks.
 struct Buffers {
   std::vector<void*> owned;
+  std::vector<HipBackend::UploadPiece> pieces;
   ~Buffers() {
     for (void* p : owned) (void)hipFree(p);
   }
   template <class T>
-  T* make(HipBackend& be, size_t count, const T* host) {
+  T* make(HipBackend&, size_t count, const T* host) {
     void* p = nullptr;
     hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc(scaling)");
     owned.push_back(p);
-    if (host != nullptr && count > 0)
-      hip_check(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, be.stream()),
-                "upload for scaling");
+    if (host != nullptr && count > 0) pieces.push_back({host, p, count * sizeof(T)});
     return static_cast<T*>(p);
+  }
+  void send(HipBackend& be) {
+    be.upload_scatter(pieces);
+    pieces.clear();
   }
 };
 
 template <class T>
 void fetch(HipBackend& be, const T* dev, std::vector<T>* host) {
   if (host->empty()) return;
-  hip_check(hipMemcpyAsync(host->data(), dev, host->size() * sizeof(T), hipMemcpyDeviceToHost,
-                           be.stream()),
-            "download after scaling");
+  be.download_staged(dev, host->data(), host->size() * sizeof(T));
 }
 
 }  // namespace
@@ -58,6 +60,7 @@ Scaling scale_lp_device(HipBackend& be, StandardLp* lp, int ruiz_iterations, boo
   double* cn = b.make<double>(be, uz(n), nullptr);
   double* d1 = b.make<double>(be, uz(m), nullptr);
   double* d2 = b.make<double>(be, uz(n), nullptr);
+  b.send(be);
   auto sweep = [&](bool one_norm) {
     // Both norms of the unswept matrix first (scaling.cpp:34-37): row norms
     // over K's rows, column norms over K^T's rows, which hold each column's

@@ -8,7 +8,8 @@ simplex optimum. Run in the container that has the reference sources:
 
     make -C oracle && python tests/golden/make_fixtures.py [--only SECTION]
 
-SECTION is one of random, stack24, stack24_cases, random_1e-8, uneven512.
+SECTION is one of random, stack24, stack24_cases, random_1e-8, uneven512,
+stack24_presolved.
 """
 import json
 import os
@@ -76,6 +77,22 @@ def stack_cases_section(periods=24, threads=8):
     return out
 
 
+def stack_presolved_section(periods=24, threads=8):
+    """The reference's solve_pdhg at 1e-4 on every case of the case stack
+    as this library's stack presolve reduces it (with the column
+    substitutions, anukulan_amd.presolve): the per-case parity bar of the
+    batched GPU solve of the reduced stack."""
+    from concurrent.futures import ThreadPoolExecutor
+    from anukulan_amd.presolve import presolve_stack
+    base, cases = G.case_stack(periods)
+    ps = presolve_stack(base, cases, slack_column_singletons=1)
+    lps = [_oracle.RefLp.from_natural(m) for m in ps.models()]
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        runs = list(pool.map(lambda lp: record(lp, 1e-4), lps))
+    return {"periods": periods, "cases": len(cases), "reduced": [ps.rows, ps.cols, int(ps.nnz)],
+            "1e-4": runs}
+
+
 def random_tight_section():
     """The random LP generator at 1e-8 (threaded CPU backend, serial bits):
     40k x 100k, which the reference solves, and the 1M-column LP, where the
@@ -106,7 +123,8 @@ def uneven_sample_section(threads=8):
 
 
 SECTIONS = {"random": random_section, "stack24": stack_section, "uneven512": uneven_sample_section,
-            "stack24_cases": stack_cases_section, "random_1e-8": random_tight_section}
+            "stack24_cases": stack_cases_section, "random_1e-8": random_tight_section,
+            "stack24_presolved": stack_presolved_section}
 
 
 def main():
@@ -122,7 +140,7 @@ def main():
         return
     out = {"known": {}, "refinery": {}, "random": random_section(), "stack24": stack_section(),
            "stack24_cases": stack_cases_section(), "random_1e-8": random_tight_section(),
-           "uneven512": uneven_sample_section()}
+           "uneven512": uneven_sample_section(), "stack24_presolved": stack_presolved_section()}
     for name in KNOWN:
         rlp = _oracle.RefLp.from_natural(natural(name))
         out["known"][name] = {"1e-8": record(rlp, 1e-8), "1e-4": record(rlp, 1e-4),

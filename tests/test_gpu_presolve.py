@@ -57,3 +57,24 @@ def test_presolved_plan_plain_solve(ctx):
     x, y = ps.postsolve_result(0, lp, r)
     obj = float(np.dot(nm.objective, x)) + nm.objective_offset
     assert relerr(obj, FIX["refinery"]["12"]["simplex_objective"]) <= 1e-7
+
+
+def test_presolved_stack_cases_match_the_reference_pdhg(ctx):
+    # Each reduced case (ranged rows where the substituted columns were)
+    # against the reference's solve_pdhg on the same reduced case
+    # (tests/golden/pdhg_reference.json stack24_presolved, make_fixtures.py):
+    # status, objective within 1e-8, iterations within 2%, and the same
+    # restart and polish counts.
+    fx = FIX["stack24_presolved"]
+    base, cases = G.case_stack(24)
+    ps = presolve_stack(base, cases, slack_column_singletons=1)
+    assert [ps.rows, ps.cols, ps.nnz] == fx["reduced"]
+    lps = [m.to_lp() for m in ps.models()]
+    rs = solve_pdhg_batch(ctx, lps, tolerance=1e-4)
+    for k, (r, want) in enumerate(zip(rs, fx["1e-4"])):
+        assert r.status == want["status"] == "optimal", k
+        assert relerr(r.objective, want["objective"]) <= 1e-8, (k, r.objective, want["objective"])
+        assert abs(r.iterations - want["iterations"]) <= 0.02 * want["iterations"], (k, r.iterations)
+        assert (r.restarts, r.polish_attempts) == (want["restarts"], want["polish_attempts"]), k
+    print(f"\npresolved stack: {sum(r.iterations for r in rs)} lane iterations "
+          f"(reference {sum(w['iterations'] for w in fx['1e-4'])})")
